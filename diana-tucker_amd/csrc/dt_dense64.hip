@@ -1,0 +1,562 @@
+// FP64 small dense linear algebra on the GPU (no vendor library).
+//
+// Reference counterparts (src/operator/operator_double.cpp):
+//   matmulNN/NT/TN :157-216  (cblas_dgemm)     -> gemm_f64_kernel (+ split-K)
+//   inverse        :84-103   (dgetrf+dgetri)   -> gj_inverse_kernel (Gauss-Jordan,
+//                                                 partial pivoting, one workgroup)
+//   QR             :131-155  (dgeqrf+dorgqr)   -> Cholesky-QR2 with a shifted
+//                                                 first pass when the Gram is not
+//                                                 numerically positive definite
+//                                                 (Fukaya et al., SISC 2020)
+//   orth1/orth2    :218-277  (MGS)             -> the same QR on the (transposed)
+//                                                 matrix: equal span, positive R diag
+// plus a cyclic-Jacobi symmetric eigensolver (Golub & Van Loan, 8.5) for
+// Rayleigh-Ritz blocks and an oversampled subspace iteration for the leading
+// eigenvectors of larger Gram matrices (HOSVD start factors).
+#include "dt_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+static inline dim3 gs_grid(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    if (g > 262144) g = 262144;
+    if (g < 1) g = 1;
+    return dim3((unsigned)g);
+}
+
+// ---------------------------------------------------------------------------
+// GEMM: C = alpha op(A) op(B) + beta C (column-major).  32x32 output tile per
+// workgroup, each thread 2x2 outputs, k staged in chunks of 16 through LDS.
+// With part != nullptr the workgroup (blockIdx.z) writes its unscaled partial
+// sum over its k range into part[z][i + m*j].
+__global__ __launch_bounds__(256) void gemm_f64_kernel(int tA, int tB, int64_t m, int64_t n,
+                                                       int64_t k, double alpha,
+                                                       const double *__restrict__ A, int64_t lda,
+                                                       const double *__restrict__ B, int64_t ldb,
+                                                       double beta, double *__restrict__ C,
+                                                       int64_t ldc, int64_t kchunk,
+                                                       double *__restrict__ part) {
+    __shared__ double As[32][17];
+    __shared__ double Bs[16][33];
+    const int64_t i0 = (int64_t)blockIdx.x * 32;
+    const int64_t j0 = (int64_t)blockIdx.y * 32;
+    const int64_t ks = (int64_t)blockIdx.z * kchunk;
+    int64_t ke = ks + kchunk;
+    if (ke > k) ke = k;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int64_t k0 = ks; k0 < ke; k0 += 16) {
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int e = threadIdx.x + 256 * q;
+            int i, kk;
+            if (tA) {
+                kk = e & 15;
+                i = e >> 4;
+            } else {
+                i = e & 31;
+                kk = e >> 5;
+            }
+            const int64_t gi = i0 + i, gk = k0 + kk;
+            double v = 0.0;
+            if (gi < m && gk < ke) v = tA ? A[gk + lda * gi] : A[gi + lda * gk];
+            As[i][kk] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int e = threadIdx.x + 256 * q;
+            int j, kk;
+            if (tB) {
+                j = e & 31;
+                kk = e >> 5;
+            } else {
+                kk = e & 15;
+                j = e >> 4;
+            }
+            const int64_t gj = j0 + j, gk = k0 + kk;
+            double v = 0.0;
+            if (gj < n && gk < ke) v = tB ? B[gj + ldb * gk] : B[gk + ldb * gj];
+            Bs[kk][j] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) {
+            const double a0 = As[tx][kk], a1 = As[tx + 16][kk];
+            const double b0 = Bs[kk][ty], b1 = Bs[kk][ty + 16];
+            acc[0][0] = fma(a0, b0, acc[0][0]);
+            acc[0][1] = fma(a0, b1, acc[0][1]);
+            acc[1][0] = fma(a1, b0, acc[1][0]);
+            acc[1][1] = fma(a1, b1, acc[1][1]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int64_t i = i0 + tx + 16 * a, j = j0 + ty + 16 * b;
+            if (i >= m || j >= n) continue;
+            if (part) {
+                part[(int64_t)blockIdx.z * m * n + i + m * j] = acc[a][b];
+            } else {
+                const double old = beta == 0.0 ? 0.0 : beta * C[i + ldc * j];
+                C[i + ldc * j] = alpha * acc[a][b] + old;
+            }
+        }
+}
+
+__global__ void gemm_reduce(const double *__restrict__ part, int nsplit, int64_t m, int64_t n,
+                            double alpha, double beta, double *__restrict__ C, int64_t ldc) {
+    const int64_t total = m * n;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        double s = 0.0;
+        for (int q = 0; q < nsplit; q++) s += part[(int64_t)q * total + idx];
+        const int64_t i = idx % m, j = idx / m;
+        const double old = beta == 0.0 ? 0.0 : beta * C[i + ldc * j];
+        C[i + ldc * j] = alpha * s + old;
+    }
+}
+
+int dt_gemm_f64_internal(dt_ctx *ctx, int tA, int tB, size_t m, size_t n, size_t k, double alpha,
+                         const double *A, size_t lda, const double *B, size_t ldb, double beta,
+                         double *C, size_t ldc) {
+    if (m == 0 || n == 0) return DT_OK;
+    const int64_t tiles = ceil_div((int64_t)m, 32) * ceil_div((int64_t)n, 32);
+    int64_t nsplit = 1;
+    if (k > 2048 && tiles < 512) {
+        nsplit = std::min<int64_t>(ceil_div((int64_t)k, 1024), std::max<int64_t>(1, 2048 / tiles));
+    }
+    int64_t kchunk = ceil_div(ceil_div((int64_t)k, nsplit), 16) * 16;
+    if (kchunk == 0) kchunk = 16;
+    nsplit = std::max<int64_t>(1, ceil_div((int64_t)k, kchunk));
+    dim3 grid((unsigned)ceil_div((int64_t)m, 32), (unsigned)ceil_div((int64_t)n, 32),
+              (unsigned)nsplit);
+    if (nsplit == 1) {
+        DT_LAUNCH("gemm_f64", gemm_f64_kernel, grid, dim3(256), 0, tA, tB, (int64_t)m, (int64_t)n,
+                  (int64_t)k, alpha, A, (int64_t)lda, B, (int64_t)ldb, beta, C, (int64_t)ldc, kchunk,
+                  (double *)nullptr);
+        return DT_OK;
+    }
+    double *part = (double *)dt_ws(ctx, WS_GEMM_PART, (size_t)(nsplit * m * n) * sizeof(double));
+    if (!part) return DT_ENOMEM;
+    DT_LAUNCH("gemm_f64", gemm_f64_kernel, grid, dim3(256), 0, tA, tB, (int64_t)m, (int64_t)n,
+              (int64_t)k, alpha, A, (int64_t)lda, B, (int64_t)ldb, beta, C, (int64_t)ldc, kchunk,
+              part);
+    DT_LAUNCH("gemm_reduce", gemm_reduce, gs_grid((int64_t)(m * n)), dim3(256), 0, part, (int)nsplit,
+              (int64_t)m, (int64_t)n, alpha, beta, C, (int64_t)ldc);
+    return DT_OK;
+}
+
+extern "C" int dt_gemm_f64(dt_ctx *ctx, int transA, int transB, size_t m, size_t n, size_t k,
+                           double alpha, const double *A, size_t lda, const double *B, size_t ldb,
+                           double beta, double *C, size_t ldc) {
+    DT_CHECK_ARG(ctx && A && B && C, "dt_gemm_f64: null argument");
+    DT_CHECK_ARG(ldc >= m && lda >= (transA ? k : m) && ldb >= (transB ? n : k),
+                 "dt_gemm_f64: leading dimension too small");
+    return dt_gemm_f64_internal(ctx, transA, transB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+// ---------------------------------------------------------------------------
+// Cholesky G = R^T R (R upper) and R^{-1}; one workgroup, global memory.
+// shift is added to the diagonal first.  info[0] = 0 or the failing pivot+1.
+__global__ __launch_bounds__(256) void chol_upper_kernel(const double *__restrict__ G, int n,
+                                                         double shift, double *__restrict__ R,
+                                                         double *__restrict__ Rinv,
+                                                         int *__restrict__ info) {
+    __shared__ int bad;
+    __shared__ double rjj_s;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    // W = upper(G) + shift I, stored in R.
+    for (int idx = tid; idx < n * n; idx += nt) {
+        const int i = idx % n, j = idx / n;
+        R[idx] = (i <= j) ? G[idx] + (i == j ? shift : 0.0) : 0.0;
+    }
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    for (int j = 0; j < n; j++) {
+        if (tid == 0) {
+            const double d = R[j + n * j];
+            if (!(d > 0.0) || !isfinite(d)) {
+                bad = j + 1;
+                rjj_s = 1.0;
+            } else {
+                rjj_s = sqrt(d);
+                R[j + n * j] = rjj_s;
+            }
+        }
+        __syncthreads();
+        if (bad) break;
+        const double rjj = rjj_s;
+        for (int k = j + 1 + tid; k < n; k += nt) R[j + n * k] /= rjj;
+        __syncthreads();
+        // trailing update of the upper triangle: W[i][k] -= R[j][i] R[j][k], j < i <= k
+        const int m = n - j - 1;
+        for (int idx = tid; idx < m * m; idx += nt) {
+            const int i = j + 1 + idx % m, k = j + 1 + idx / m;
+            if (i <= k) R[i + n * k] -= R[j + n * i] * R[j + n * k];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) info[0] = bad;
+    if (bad) return;
+    // Rinv: solve R X = I column by column (upper triangular back substitution).
+    for (int c = tid; c < n; c += nt) {
+        for (int i = 0; i < n; i++) Rinv[i + n * c] = 0.0;
+        for (int i = c; i >= 0; i--) {
+            double s = (i == c) ? 1.0 : 0.0;
+            for (int k = i + 1; k <= c; k++) s -= R[i + n * k] * Rinv[k + n * c];
+            Rinv[i + n * c] = s / R[i + n * i];
+        }
+    }
+}
+
+int dt_chol_upper(dt_ctx *ctx, const double *G, size_t n, double *R, double *Rinv, int *info_host) {
+    int *dinfo = (int *)dt_ws(ctx, WS_STATUS, 64);
+    if (!dinfo) return DT_ENOMEM;
+    DT_LAUNCH("chol", chol_upper_kernel, dim3(1), dim3(256), 0, G, (int)n, 0.0, R, Rinv, dinfo);
+    int *pin = (int *)ctx->pinned;
+    DT_HIP(hipMemcpyAsync(pin, dinfo, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    DT_HIP(hipStreamSynchronize(ctx->stream));
+    *info_host = pin[0];
+    return DT_OK;
+}
+
+// trace(G) -> out[0] (single thread; G is small)
+__global__ void trace_kernel(const double *__restrict__ G, int n, double *__restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < n; i++) s += fabs(G[i + n * i]);
+        out[0] = s;
+    }
+}
+
+// C = A B for small square upper-triangular n x n (R accumulation), general.
+__global__ void small_matmul(const double *__restrict__ A, const double *__restrict__ B,
+                             double *__restrict__ C, int n) {
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n * n; idx += gridDim.x * blockDim.x) {
+        const int i = idx % n, j = idx / n;
+        double s = 0.0;
+        for (int k = 0; k < n; k++) s += A[i + n * k] * B[k + n * j];
+        C[idx] = s;
+    }
+}
+
+__global__ void set_identity(double *__restrict__ A, int n) {
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n * n; idx += gridDim.x * blockDim.x)
+        A[idx] = (idx % n == idx / n) ? 1.0 : 0.0;
+}
+
+// Reduced QR through Cholesky-QR2 (a shifted Cholesky-QR first pass when the
+// plain one breaks down).  Q may alias A.
+extern "C" int dt_qr_f64(dt_ctx *ctx, double *Q, double *R, const double *A, size_t m, size_t n) {
+    DT_CHECK_ARG(ctx && Q && A, "dt_qr_f64: null argument");
+    DT_CHECK_ARG(m >= n && n >= 1, "dt_qr_f64: need m >= n >= 1 (got %zu x %zu)", m, n);
+    const size_t nn = n * n;
+    double *G = (double *)dt_ws(ctx, WS_SMALL0, nn * sizeof(double));
+    double *Rk = (double *)dt_ws(ctx, WS_SMALL1, nn * sizeof(double));
+    double *Rki = (double *)dt_ws(ctx, WS_SMALL2, nn * sizeof(double));
+    double *Racc = (double *)dt_ws(ctx, WS_QR_T1, 2 * nn * sizeof(double) + m * n * sizeof(double));
+    if (!G || !Rk || !Rki || !Racc) return DT_ENOMEM;
+    double *Rtmp = Racc + nn;
+    double *T = Rtmp + nn;  // m x n temporary
+    if (Q != A) DT_HIP(hipMemcpyAsync(Q, A, m * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    if (R) DT_LAUNCH("qr_small", set_identity, dim3(1), dim3(256), 0, Racc, (int)n);
+    int passes = 2;
+    bool shifted = false;
+    for (int pass = 0; pass < passes; pass++) {
+        DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, n, n, m, 1.0, Q, m, Q, m, 0.0, G, n));
+        int info = 0;
+        DT_TRY(dt_chol_upper(ctx, G, n, Rk, Rki, &info));
+        if (info != 0) {
+            if (shifted) return dt_fail(ctx, DT_ENUMERIC, "dt_qr_f64: Cholesky breakdown after shift");
+            // shift = 11 (m n + n (n+1)) u ||A||_2^2, bounded above by trace(G)
+            double *dtr = (double *)dt_ws(ctx, WS_STATUS, 64);
+            DT_LAUNCH("qr_small", trace_kernel, dim3(1), dim3(64), 0, G, (int)n, dtr);
+            DT_HIP(hipMemcpyAsync(ctx->pinned, dtr, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            DT_HIP(hipStreamSynchronize(ctx->stream));
+            const double tr = ctx->pinned[0];
+            if (!(tr > 0.0) || !std::isfinite(tr))
+                return dt_fail(ctx, DT_ENUMERIC, "dt_qr_f64: zero or non-finite matrix");
+            const double shift = 11.0 * ((double)m * n + (double)n * (n + 1)) * 1.1102230246251565e-16 * tr;
+            int *dinfo = (int *)dt_ws(ctx, WS_STATUS, 64);
+            DT_LAUNCH("chol", chol_upper_kernel, dim3(1), dim3(256), 0, G, (int)n, shift, Rk, Rki, dinfo);
+            int *pin = (int *)ctx->pinned;
+            DT_HIP(hipMemcpyAsync(pin, dinfo, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            DT_HIP(hipStreamSynchronize(ctx->stream));
+            if (pin[0] != 0)
+                return dt_fail(ctx, DT_ENUMERIC, "dt_qr_f64: rank-deficient input (pivot %d)", pin[0]);
+            shifted = true;
+            passes = 3;
+        }
+        DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, m, n, n, 1.0, Q, m, Rki, n, 0.0, T, m));
+        DT_HIP(hipMemcpyAsync(Q, T, m * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        if (R) {
+            DT_LAUNCH("qr_small", small_matmul, dim3(16), dim3(256), 0, Rk, Racc, Rtmp, (int)n);
+            DT_HIP(hipMemcpyAsync(Racc, Rtmp, nn * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        }
+    }
+    if (R) DT_HIP(hipMemcpyAsync(R, Racc, nn * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    return DT_OK;
+}
+
+extern "C" int dt_orth_rows_f64(dt_ctx *ctx, double *A, size_t len1, size_t len2) {
+    DT_CHECK_ARG(ctx && A, "dt_orth_rows_f64: null argument");
+    DT_CHECK_ARG(len2 >= len1, "dt_orth_rows_f64: need len2 >= len1");
+    double *T = (double *)dt_ws(ctx, WS_QR_T0, len1 * len2 * sizeof(double));
+    if (!T) return DT_ENOMEM;
+    DT_TRY(dt_transpose_f64(ctx, T, A, len1, len2));  // T: len2 x len1
+    DT_TRY(dt_qr_f64(ctx, T, nullptr, T, len2, len1));
+    return dt_transpose_f64(ctx, A, T, len2, len1);
+}
+
+// ---------------------------------------------------------------------------
+// Gauss-Jordan inverse with partial pivoting, one workgroup.  W: n x 2n scratch.
+__global__ __launch_bounds__(256) void gj_inverse_kernel(const double *__restrict__ A, int n,
+                                                         double *__restrict__ W,
+                                                         double *__restrict__ C,
+                                                         int *__restrict__ info) {
+    __shared__ int piv;
+    __shared__ int bad;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int ld = n;
+    for (int idx = tid; idx < 2 * n * n; idx += nt) {
+        const int i = idx % n, j = idx / n;
+        W[i + ld * j] = (j < n) ? A[i + n * j] : ((j - n) == i ? 1.0 : 0.0);
+    }
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    for (int col = 0; col < n; col++) {
+        if (tid == 0) {
+            int p = col;
+            double best = fabs(W[col + ld * col]);
+            for (int i = col + 1; i < n; i++) {
+                const double v = fabs(W[i + ld * col]);
+                if (v > best) {
+                    best = v;
+                    p = i;
+                }
+            }
+            piv = p;
+            if (!(best > 0.0) || !isfinite(best)) bad = col + 1;
+        }
+        __syncthreads();
+        if (bad) break;
+        const int p = piv;
+        if (p != col) {
+            for (int j = tid; j < 2 * n; j += nt) {
+                const double t = W[col + ld * j];
+                W[col + ld * j] = W[p + ld * j];
+                W[p + ld * j] = t;
+            }
+        }
+        __syncthreads();
+        const double d = W[col + ld * col];
+        __syncthreads();
+        for (int j = tid; j < 2 * n; j += nt) W[col + ld * j] /= d;
+        __syncthreads();
+        // eliminate column col from every other row
+        for (int idx = tid; idx < n * 2 * n; idx += nt) {
+            const int i = idx % n, j = idx / n;
+            if (i == col || j == col) continue;
+            W[i + ld * j] -= W[i + ld * col] * W[col + ld * j];
+        }
+        __syncthreads();
+        for (int i = tid; i < n; i += nt)
+            if (i != col) W[i + ld * col] = 0.0;
+        __syncthreads();
+    }
+    if (tid == 0) info[0] = bad;
+    if (bad) return;
+    for (int idx = tid; idx < n * n; idx += nt) {
+        const int i = idx % n, j = idx / n;
+        C[idx] = W[i + ld * (n + j)];
+    }
+}
+
+extern "C" int dt_inverse_f64(dt_ctx *ctx, double *C, const double *A, size_t n) {
+    DT_CHECK_ARG(ctx && C && A && n >= 1, "dt_inverse_f64: bad argument");
+    double *W = (double *)dt_ws(ctx, WS_SMALL3, 2 * n * n * sizeof(double));
+    int *dinfo = (int *)dt_ws(ctx, WS_STATUS, 64);
+    if (!W || !dinfo) return DT_ENOMEM;
+    DT_LAUNCH("inverse", gj_inverse_kernel, dim3(1), dim3(256), 0, A, (int)n, W, C, dinfo);
+    int *pin = (int *)ctx->pinned;
+    DT_HIP(hipMemcpyAsync(pin, dinfo, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    DT_HIP(hipStreamSynchronize(ctx->stream));
+    if (pin[0] != 0) return dt_fail(ctx, DT_ENUMERIC, "dt_inverse_f64: singular matrix (column %d)", pin[0]);
+    return DT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Cyclic Jacobi eigensolver, one workgroup, round-robin parallel ordering.
+// H (n x n symmetric) is overwritten; V receives the eigenvectors (columns),
+// w the eigenvalues, both sorted in descending order.  Work buffer: perm.
+__global__ __launch_bounds__(256) void jacobi_kernel(double *__restrict__ H, int n,
+                                                     double *__restrict__ V,
+                                                     double *__restrict__ w,
+                                                     double *__restrict__ rot,
+                                                     int *__restrict__ order) {
+    __shared__ double off_s, tot_s;
+    __shared__ int stop;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int np = (n + 1) & ~1;  // even number of players (n == np - 1: one bye)
+    for (int idx = tid; idx < n * n; idx += nt) V[idx] = (idx % n == idx / n) ? 1.0 : 0.0;
+    if (tid == 0) stop = 0;
+    __syncthreads();
+    // rot layout per pair q: [p, qq, c, s]
+    for (int sweep = 0; sweep < 40; sweep++) {
+        if (tid == 0) {
+            double off = 0.0, tot = 0.0;
+            for (int j = 0; j < n; j++)
+                for (int i = 0; i < n; i++) {
+                    const double h = H[i + n * j];
+                    tot += h * h;
+                    if (i != j) off += h * h;
+                }
+            off_s = off;
+            tot_s = tot;
+            if (!(off > 1e-30 * tot) || tot == 0.0) stop = 1;
+        }
+        __syncthreads();
+        if (stop) break;
+        for (int round = 0; round < np - 1; round++) {
+            // pairs of the round-robin schedule: player 0 fixed, others rotate
+            for (int q = tid; q < np / 2; q += nt) {
+                int a, b;
+                if (q == 0) {
+                    a = 0;
+                    b = 1 + (round % (np - 1));
+                } else {
+                    a = 1 + ((round + q) % (np - 1));
+                    b = 1 + ((round + np - 1 - q) % (np - 1));
+                }
+                int p = a < b ? a : b, qq = a < b ? b : a;
+                double c = 1.0, s = 0.0;
+                if (qq < n) {
+                    const double apq = H[p + n * qq];
+                    if (fabs(apq) > 1e-300) {
+                        const double app = H[p + n * p], aqq = H[qq + n * qq];
+                        const double tau = (aqq - app) / (2.0 * apq);
+                        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        s = t * c;
+                    }
+                }
+                rot[4 * q + 0] = (double)p;
+                rot[4 * q + 1] = (double)qq;
+                rot[4 * q + 2] = c;
+                rot[4 * q + 3] = s;
+            }
+            __syncthreads();
+            // H <- H J (columns p, q), V <- V J
+            for (int idx = tid; idx < (np / 2) * n; idx += nt) {
+                const int q = idx / n, r = idx % n;
+                const int p = (int)rot[4 * q], qq = (int)rot[4 * q + 1];
+                if (qq >= n) continue;
+                const double c = rot[4 * q + 2], s = rot[4 * q + 3];
+                const double hp = H[r + n * p], hq = H[r + n * qq];
+                H[r + n * p] = c * hp - s * hq;
+                H[r + n * qq] = s * hp + c * hq;
+                const double vp = V[r + n * p], vq = V[r + n * qq];
+                V[r + n * p] = c * vp - s * vq;
+                V[r + n * qq] = s * vp + c * vq;
+            }
+            __syncthreads();
+            // H <- J^T H (rows p, q)
+            for (int idx = tid; idx < (np / 2) * n; idx += nt) {
+                const int q = idx / n, r = idx % n;
+                const int p = (int)rot[4 * q], qq = (int)rot[4 * q + 1];
+                if (qq >= n) continue;
+                const double c = rot[4 * q + 2], s = rot[4 * q + 3];
+                const double hp = H[p + n * r], hq = H[qq + n * r];
+                H[p + n * r] = c * hp - s * hq;
+                H[qq + n * r] = s * hp + c * hq;
+            }
+            __syncthreads();
+        }
+    }
+    // sort descending (selection by one thread; n is small)
+    if (tid == 0) {
+        for (int i = 0; i < n; i++) order[i] = i;
+        for (int i = 0; i < n; i++) {
+            int best = i;
+            for (int j = i + 1; j < n; j++)
+                if (H[order[j] + n * order[j]] > H[order[best] + n * order[best]]) best = j;
+            const int t = order[i];
+            order[i] = order[best];
+            order[best] = t;
+        }
+        for (int i = 0; i < n; i++) w[i] = H[order[i] + n * order[i]];
+    }
+    __syncthreads();
+    // permute V columns into H (as scratch) then copy back
+    for (int idx = tid; idx < n * n; idx += nt) {
+        const int r = idx % n, j = idx / n;
+        H[idx] = V[r + n * order[j]];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < n * n; idx += nt) V[idx] = H[idx];
+}
+
+int dt_jacobi_eig(dt_ctx *ctx, double *H, size_t n, double *V, double *w) {
+    const size_t np = (n + 1) & ~(size_t)1;
+    double *rot = (double *)dt_ws(ctx, WS_EIG_H, (2 * np + 8) * sizeof(double) + n * sizeof(int) + 256);
+    if (!rot) return DT_ENOMEM;
+    int *order = (int *)(rot + 2 * np + 8);
+    DT_LAUNCH("jacobi", jacobi_kernel, dim3(1), dim3(256), 0, H, (int)n, V, w, rot, order);
+    return DT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Leading r eigenpairs of a symmetric n x n FP64 matrix.
+extern "C" int dt_syev_top_f64(dt_ctx *ctx, const double *S, size_t n, size_t r, double *V,
+                               double *w) {
+    DT_CHECK_ARG(ctx && S && V && r >= 1 && r <= n, "dt_syev_top_f64: bad argument");
+    if (n <= 160) {
+        double *H = (double *)dt_ws(ctx, WS_EIG_X, 2 * n * n * sizeof(double) + n * sizeof(double));
+        if (!H) return DT_ENOMEM;
+        double *Vall = H + n * n;
+        double *wall = Vall + n * n;
+        DT_HIP(hipMemcpyAsync(H, S, n * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        DT_TRY(dt_jacobi_eig(ctx, H, n, Vall, wall));
+        DT_HIP(hipMemcpyAsync(V, Vall, n * r * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        if (w) DT_HIP(hipMemcpyAsync(w, wall, r * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        return DT_OK;
+    }
+    // Oversampled subspace iteration with Rayleigh-Ritz every few steps.
+    const size_t p = std::min(n, std::max(2 * r, r + 16));
+    double *X = (double *)dt_ws(ctx, WS_EIG_X, (2 * n * p + 2 * p * p + 2 * p) * sizeof(double));
+    if (!X) return DT_ENOMEM;
+    double *Y = X + n * p;
+    double *H = Y + n * p;
+    double *W = H + p * p;
+    double *theta = W + p * p;
+    DT_TRY(dt_fill_normal_f64(ctx, X, n * p, 0x5eed5eedULL, 77));
+    DT_TRY(dt_qr_f64(ctx, X, nullptr, X, n, p));
+    std::vector<double> prev(r, 0.0), cur(p);
+    const int max_iter = 600;
+    for (int it = 1; it <= max_iter; it++) {
+        DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, n, p, n, 1.0, S, n, X, n, 0.0, Y, n));
+        if (it % 4 == 0 || it == max_iter) {
+            // Rayleigh-Ritz on span(X): H = X^T S X
+            DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, p, p, n, 1.0, X, n, Y, n, 0.0, H, p));
+            DT_TRY(dt_jacobi_eig(ctx, H, p, W, theta));
+            DT_HIP(hipMemcpyAsync(cur.data(), theta, p * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            DT_HIP(hipStreamSynchronize(ctx->stream));
+            double diff = 0.0;
+            for (size_t i = 0; i < r; i++) diff = std::max(diff, std::fabs(cur[i] - prev[i]));
+            for (size_t i = 0; i < r; i++) prev[i] = cur[i];
+            const double scale = std::max(std::fabs(cur[0]), 1e-300);
+            if (diff <= 1e-14 * scale || it == max_iter) {
+                // V = X W[:, :r]  (Ritz vectors)
+                DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, n, r, p, 1.0, X, n, W, p, 0.0, V, n));
+                if (w) DT_HIP(hipMemcpyAsync(w, theta, r * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+                return DT_OK;
+            }
+        }
+        DT_TRY(dt_qr_f64(ctx, X, nullptr, Y, n, p));
+    }
+    return DT_OK;
+}

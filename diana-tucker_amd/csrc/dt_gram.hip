@@ -1,0 +1,340 @@
+// Mode-n Gram S = A_(n) A_(n)^T, cross-Gram A_(n) B_(n)^T and FP64 sums of squares.
+//
+// Reference: Function::gram(A, n) (template/function/tensor.tpp:18-120) is,
+// at one rank, Operator::tenmat (operator_cpu.tpp:56-72) followed by one
+// cblas_dgemm NT (operator_double.cpp:176-194); Function::ttt_except
+// (tensor.tpp:132-259) is the same with two tensors; Operator<double>::fnorm
+// (operator_double.cpp:74-82) is an FP64 reduction.
+//
+// gram_mfma: symmetric rank-k update on v_mfma_f32_16x16x4_f32.  Workgroup =
+// one 64x64 tile of the lower triangle of S and one contiguous range of the
+// columns (f, b) of A_(n) (split-K).  Each 32-column chunk of the two row
+// panels is staged in LDS (row stride 34 floats: conflict-free MFMA operand
+// reads), the four waves each own a 32x32 sub-tile.  Partial tiles of all
+// splits are summed in FP64 by gram_reduce, which also mirrors the upper half.
+#include "dt_internal.hpp"
+
+template <typename T>
+__device__ __forceinline__ float ldg_f(const T *p);
+template <>
+__device__ __forceinline__ float ldg_f<float>(const float *p) { return *p; }
+template <>
+__device__ __forceinline__ float ldg_f<uint16_t>(const uint16_t *p) { return bf16_to_f32(*p); }
+
+__device__ __forceinline__ void tri_decode(int t, int &ti, int &tj) {
+    int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (r * (r + 1) / 2 > t) r--;
+    while ((r + 1) * (r + 2) / 2 <= t) r++;
+    ti = r;
+    tj = t - r * (r + 1) / 2;
+}
+
+constexpr int GS = 34;  // LDS row stride (floats)
+
+template <typename TA, bool F1>
+__global__ __launch_bounds__(256) void gram_mfma(const TA *__restrict__ A, float *__restrict__ Sp,
+                                                 int64_t F, int64_t I, int64_t Ctot,
+                                                 int64_t cols_per_split) {
+    __shared__ float Xa[64 * GS];
+    __shared__ float Xb[64 * GS];
+    int ti, tj;
+    tri_decode((int)blockIdx.x, ti, tj);
+    const bool diag = ti == tj;
+    const int64_t c_begin = (int64_t)blockIdx.y * cols_per_split;
+    int64_t c_end = c_begin + cols_per_split;
+    if (c_end > Ctot) c_end = Ctot;
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int c = lane & 15;
+    const int g = lane >> 4;
+    const int wi = (wave >> 1) * 32;
+    const int wj = (wave & 1) * 32;
+
+    floatx4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    for (int64_t c0 = c_begin; c0 < c_end; c0 += 32) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int e = threadIdx.x + 256 * q;
+            int r, kc;
+            if (F1) {
+                r = e & 63;
+                kc = e >> 6;
+            } else {
+                kc = e & 31;
+                r = e >> 5;
+            }
+            const int64_t kk = c0 + kc;
+            float va = 0.f, vb = 0.f;
+            if (kk < c_end) {
+                int64_t base;
+                if (F1) {
+                    base = I * kk;
+                } else {
+                    const int64_t b = kk / F;
+                    base = (kk - b * F) + F * I * b;
+                }
+                const int64_t ia = (int64_t)ti * 64 + r;
+                const int64_t ib = (int64_t)tj * 64 + r;
+                if (ia < I) va = ldg_f<TA>(A + base + F * ia);
+                if (!diag && ib < I) vb = ldg_f<TA>(A + base + F * ib);
+            }
+            Xa[r * GS + kc] = va;
+            if (!diag) Xb[r * GS + kc] = vb;
+        }
+        __syncthreads();
+        const float *XB = diag ? Xa : Xb;
+#pragma unroll
+        for (int ks = 0; ks < 32; ks += 4) {
+            const float a0 = Xa[(wi + c) * GS + ks + g];
+            const float a1 = Xa[(wi + 16 + c) * GS + ks + g];
+            const float b0 = XB[(wj + c) * GS + ks + g];
+            const float b1 = XB[(wj + 16 + c) * GS + ks + g];
+            acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    float *S = Sp + (int64_t)blockIdx.y * I * I;
+#pragma unroll
+    for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+        for (int nj = 0; nj < 2; nj++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int64_t i = (int64_t)ti * 64 + wi + 16 * mi + 4 * g + v;
+                const int64_t j = (int64_t)tj * 64 + wj + 16 * nj + c;
+                if (i < I && j < I) S[i + I * j] = acc[mi][nj][v];
+            }
+}
+
+// S[i, j] = sum over splits of the partial stored in the lower tile triangle.
+__global__ void gram_reduce(const float *__restrict__ Sp, double *__restrict__ S, int64_t I,
+                            int nsplit) {
+    const int64_t total = I * I;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        const int64_t i = idx % I, j = idx / I;
+        const int64_t src = (i / 64 >= j / 64) ? idx : (j + I * i);
+        double s = 0.0;
+        for (int q = 0; q < nsplit; q++) s += (double)Sp[(int64_t)q * total + src];
+        S[idx] = s;
+    }
+}
+
+// FP64 (and generic) Gram / cross-Gram: one thread per output entry.
+//   C[i + IA*j] = sum_{f,b} A[f + F*(i + IA*b)] * Bt[f + F*(j + JB*b)]
+template <typename T>
+__global__ void cross_gram_generic(const T *__restrict__ A, const T *__restrict__ Bt,
+                                   double *__restrict__ C, int64_t F, int64_t IA, int64_t JB,
+                                   int64_t Bk) {
+    const int64_t total = IA * JB;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        const int64_t i = idx % IA, j = idx / IA;
+        double s = 0.0;
+        for (int64_t b = 0; b < Bk; b++) {
+            const T *pa = A + F * (i + IA * b);
+            const T *pb = Bt + F * (j + JB * b);
+            for (int64_t f = 0; f < F; f++) s += (double)pa[f] * (double)pb[f];
+        }
+        C[idx] = s;
+    }
+}
+
+// ---- sums of squares ----------------------------------------------------------
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double red[16];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; i++) s += red[i];
+    }
+    __syncthreads();
+    return s;
+}
+
+template <typename T>
+__device__ __forceinline__ double ld_d(const T *p, int64_t i);
+template <>
+__device__ __forceinline__ double ld_d<double>(const double *p, int64_t i) { return p[i]; }
+template <>
+__device__ __forceinline__ double ld_d<float>(const float *p, int64_t i) { return (double)p[i]; }
+template <>
+__device__ __forceinline__ double ld_d<uint16_t>(const uint16_t *p, int64_t i) {
+    return (double)bf16_to_f32(p[i]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void sumsq_kernel(const T *__restrict__ A, int64_t n,
+                                                    double *__restrict__ part) {
+    double s = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double v = ld_d<T>(A, i);
+        s += v * v;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// float4-vectorised variant for FP32 data (n % 4 == 0, 16-byte aligned).
+__global__ __launch_bounds__(256) void sumsq_f32x4(const float4 *__restrict__ A, int64_t n4,
+                                                   double *__restrict__ part) {
+    double s = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        const float4 v = A[i];
+        s += (double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z + (double)v.w * v.w;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void sum_partials(const double *__restrict__ part, int n, double *__restrict__ out) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+static inline dim3 gs_grid(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    if (g > 262144) g = 262144;
+    if (g < 1) g = 1;
+    return dim3((unsigned)g);
+}
+
+// Sum of squares, result left on the device in out_dev[0].
+static int sumsq_to_dev(dt_ctx *ctx, const void *A, int dt, size_t n, double *out_dev) {
+    const int nb = 2048;
+    double *part = (double *)dt_ws(ctx, WS_SUMSQ, nb * sizeof(double));
+    if (!part) return DT_ENOMEM;
+    if (dt == DT_F32 && (n % 4) == 0 && (((uintptr_t)A) & 15) == 0) {
+        DT_LAUNCH("sumsq", sumsq_f32x4, dim3(nb), dim3(256), 0, (const float4 *)A, (int64_t)(n / 4),
+                  part);
+    } else if (dt == DT_F32) {
+        DT_LAUNCH("sumsq", sumsq_kernel<float>, dim3(nb), dim3(256), 0, (const float *)A, (int64_t)n,
+                  part);
+    } else if (dt == DT_F64) {
+        DT_LAUNCH("sumsq", sumsq_kernel<double>, dim3(nb), dim3(256), 0, (const double *)A,
+                  (int64_t)n, part);
+    } else if (dt == DT_BF16) {
+        DT_LAUNCH("sumsq", sumsq_kernel<uint16_t>, dim3(nb), dim3(256), 0, (const uint16_t *)A,
+                  (int64_t)n, part);
+    } else {
+        return dt_fail(ctx, DT_EUNSUPPORTED, "sumsq: unsupported type %d", dt);
+    }
+    DT_LAUNCH("sumsq", sum_partials, dim3(1), dim3(256), 0, part, nb, out_dev);
+    return DT_OK;
+}
+
+int dt_sumsq_dev(dt_ctx *ctx, const void *A, int dt, size_t n, double *out_host) {
+    if (n == 0) {
+        *out_host = 0.0;
+        return DT_OK;
+    }
+    double *dres = (double *)dt_ws(ctx, WS_STATUS, 64);
+    if (!dres) return DT_ENOMEM;
+    DT_TRY(sumsq_to_dev(ctx, A, dt, n, dres));
+    DT_HIP(hipMemcpyAsync(ctx->pinned, dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    DT_HIP(hipStreamSynchronize(ctx->stream));
+    *out_host = ctx->pinned[0];
+    return DT_OK;
+}
+
+extern "C" int dt_sumsq(dt_ctx *ctx, const void *A, int dt, size_t n, double *out_host) {
+    DT_CHECK_ARG(ctx && out_host && (A || n == 0), "dt_sumsq: null argument");
+    return dt_sumsq_dev(ctx, A, dt, n, out_host);
+}
+
+// S (I x I, FP64, device) = A_(n) A_(n)^T for A viewed as [F, I, B].
+int dt_gram_internal(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t I, int64_t B,
+                     double *S) {
+    if (I == 0) return DT_OK;
+    const int64_t Ctot = F * B;
+    if (dtA == DT_F64) {
+        DT_LAUNCH("gram_generic_f64", cross_gram_generic<double>, gs_grid(I * I), dim3(256), 0,
+                  (const double *)A, (const double *)A, S, F, I, I, B);
+        return DT_OK;
+    }
+    DT_CHECK_ARG(dtA == DT_F32 || dtA == DT_BF16, "gram: unsupported type %d", dtA);
+    const int nt = (int)ceil_div(I, 64);
+    const int ntiles = nt * (nt + 1) / 2;
+    int64_t nsplit = 2048 / ntiles;
+    if (nsplit < 1) nsplit = 1;
+    int64_t maxsplit = ceil_div(Ctot, 32);
+    if (nsplit > maxsplit) nsplit = maxsplit;
+    if (nsplit > 256) nsplit = 256;
+    int64_t cps = ceil_div(ceil_div(Ctot, nsplit), 32) * 32;
+    nsplit = ceil_div(Ctot, cps);
+    float *Sp = (float *)dt_ws(ctx, WS_GRAM_PART, (size_t)(nsplit * I * I) * sizeof(float));
+    if (!Sp) return DT_ENOMEM;
+    dim3 grid((unsigned)ntiles, (unsigned)nsplit);
+    if (dtA == DT_F32) {
+        if (F == 1) {
+            DT_LAUNCH("gram_mfma", (gram_mfma<float, true>), grid, dim3(256), 0, (const float *)A, Sp,
+                      F, I, Ctot, cps);
+        } else {
+            DT_LAUNCH("gram_mfma", (gram_mfma<float, false>), grid, dim3(256), 0, (const float *)A,
+                      Sp, F, I, Ctot, cps);
+        }
+    } else {
+        if (F == 1) {
+            DT_LAUNCH("gram_mfma", (gram_mfma<uint16_t, true>), grid, dim3(256), 0,
+                      (const uint16_t *)A, Sp, F, I, Ctot, cps);
+        } else {
+            DT_LAUNCH("gram_mfma", (gram_mfma<uint16_t, false>), grid, dim3(256), 0,
+                      (const uint16_t *)A, Sp, F, I, Ctot, cps);
+        }
+    }
+    DT_LAUNCH("gram_reduce", gram_reduce, gs_grid(I * I), dim3(256), 0, Sp, S, I, (int)nsplit);
+    return DT_OK;
+}
+
+extern "C" int dt_gram(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n,
+                       void *S, int dtS) {
+    DT_CHECK_ARG(ctx && A && S && shape, "dt_gram: null argument");
+    DT_CHECK_ARG(ndim >= 1 && ndim <= DT_MAX_NDIM && n >= 0 && n < ndim, "dt_gram: bad mode");
+    DT_CHECK_ARG(dtS == DT_F64 || dtS == DT_F32, "dt_gram: output must be F64 or F32");
+    int64_t F = 1, B = 1;
+    for (int i = 0; i < n; i++) F *= (int64_t)shape[i];
+    for (int i = n + 1; i < ndim; i++) B *= (int64_t)shape[i];
+    const int64_t I = (int64_t)shape[n];
+    if (dtS == DT_F64) return dt_gram_internal(ctx, A, dtA, F, I, B, (double *)S);
+    double *tmp = (double *)dt_ws(ctx, WS_SMALL3, (size_t)(I * I) * sizeof(double));
+    if (!tmp) return DT_ENOMEM;
+    DT_TRY(dt_gram_internal(ctx, A, dtA, F, I, B, tmp));
+    return dt_convert(ctx, S, DT_F32, tmp, DT_F64, (size_t)(I * I));
+}
+
+extern "C" int dt_ttt_except(dt_ctx *ctx, const void *A, const void *B, int dt, int ndim,
+                             const size_t *shapeA, size_t JB, int n, double *C) {
+    DT_CHECK_ARG(ctx && A && B && C && shapeA, "dt_ttt_except: null argument");
+    DT_CHECK_ARG(ndim >= 1 && ndim <= DT_MAX_NDIM && n >= 0 && n < ndim, "dt_ttt_except: bad mode");
+    int64_t F = 1, Bk = 1;
+    for (int i = 0; i < n; i++) F *= (int64_t)shapeA[i];
+    for (int i = n + 1; i < ndim; i++) Bk *= (int64_t)shapeA[i];
+    const int64_t IA = (int64_t)shapeA[n];
+    if (dt == DT_F64) {
+        DT_LAUNCH("ttt_except", cross_gram_generic<double>, gs_grid(IA * (int64_t)JB), dim3(256), 0,
+                  (const double *)A, (const double *)B, C, F, IA, (int64_t)JB, Bk);
+    } else if (dt == DT_F32) {
+        DT_LAUNCH("ttt_except", cross_gram_generic<float>, gs_grid(IA * (int64_t)JB), dim3(256), 0,
+                  (const float *)A, (const float *)B, C, F, IA, (int64_t)JB, Bk);
+    } else {
+        return dt_fail(ctx, DT_EUNSUPPORTED, "dt_ttt_except: type %d", dt);
+    }
+    return DT_OK;
+}

@@ -1,0 +1,176 @@
+// Internal declarations of libdiana_hip.so (gfx950).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstdarg>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "diana_hip.h"
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+struct dt_kstat {
+    int64_t count = 0;
+    double ms = 0.0;
+};
+
+struct dt_pending {
+    const char *name;
+    hipEvent_t a, b;
+};
+
+struct dt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool timing = false;
+    std::vector<dt_pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, dt_kstat> stats;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    // Workspace slots: each slot is a device buffer that only grows.
+    static constexpr int kSlots = 32;
+    void *ws[kSlots] = {};
+    size_t ws_size[kSlots] = {};
+    // Host-pinned scalar staging area for small device->host results.
+    double *pinned = nullptr;
+};
+
+// Workspace slot ids (one owner each, so nested helpers never alias).
+enum dt_ws_slot {
+    WS_TTM_M = 0,      // converted/transposed factor for dt_ttm
+    WS_GRAM_PART,      // gram split-K partials
+    WS_SUMSQ,          // sumsq block partials
+    WS_GEMM_PART,      // gemm split-K partials
+    WS_SMALL0,         // small FP64 scratch (R x R)
+    WS_SMALL1,
+    WS_SMALL2,
+    WS_SMALL3,
+    WS_QR_T0,          // QR working matrix
+    WS_QR_T1,
+    WS_EIG_X,          // subspace iteration
+    WS_EIG_Y,
+    WS_EIG_H,
+    WS_STATUS,         // device int status word(s)
+    WS_TTT,            // ttt_except staging
+    WS_GEN,            // generator scratch
+    WS_GEN2,
+    WS_SPARSE0,
+    WS_SPARSE1,
+    WS_SPARSE2,
+    WS_SPARSE3,
+    WS_MLP0,
+    WS_MLP1,
+    WS_MLP2,
+    WS_COUNT_
+};
+static_assert(WS_COUNT_ <= dt_ctx::kSlots, "too many workspace slots");
+
+int dt_fail(dt_ctx *ctx, int code, const char *fmt, ...);
+void *dt_ws(dt_ctx *ctx, int slot, size_t bytes);  // nullptr on failure (error set)
+hipEvent_t dt_get_event(dt_ctx *ctx);
+void dt_collect_timing(dt_ctx *ctx);
+
+#define DT_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return dt_fail(ctx, DT_EHIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,       \
+                           hipGetErrorString(e_));                                        \
+    } while (0)
+
+#define DT_TRY(call)                   \
+    do {                               \
+        int s_ = (call);               \
+        if (s_ != DT_OK) return s_;    \
+    } while (0)
+
+#define DT_CHECK_ARG(cond, ...)                                  \
+    do {                                                         \
+        if (!(cond)) return dt_fail(ctx, DT_EINVAL, __VA_ARGS__); \
+    } while (0)
+
+// Brackets one kernel launch with events when timing is on.
+struct dt_ktimer {
+    dt_ctx *c;
+    const char *name;
+    hipEvent_t a = nullptr;
+    dt_ktimer(dt_ctx *c_, const char *n) : c(c_), name(n) {
+        if (c->timing) {
+            a = dt_get_event(c);
+            hipEventRecord(a, c->stream);
+        }
+    }
+    ~dt_ktimer() {
+        if (c->timing) {
+            hipEvent_t b = dt_get_event(c);
+            hipEventRecord(b, c->stream);
+            c->pending.push_back({name, a, b});
+        }
+    }
+};
+
+#define DT_LAUNCH(name, kernel, grid, block, shmem, ...)                              \
+    do {                                                                              \
+        dt_ktimer kt_(ctx, name);                                                     \
+        hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream, __VA_ARGS__);    \
+    } while (0);                                                                      \
+    DT_HIP(hipGetLastError())
+
+// ---- bf16 helpers (storage = uint16_t, round to nearest even) ----------------
+__host__ __device__ inline float bf16_to_f32(uint16_t h) {
+    return __builtin_bit_cast(float, ((uint32_t)h) << 16);
+}
+__host__ __device__ inline uint16_t f32_to_bf16(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);  // NaN stays NaN
+    uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return (uint16_t)(u >> 16);
+}
+
+inline size_t dt_elem_size(int dt) {
+    switch (dt) {
+        case DT_F64: return 8;
+        case DT_F32: return 4;
+        case DT_BF16: return 2;
+        case DT_I32: return 4;
+        default: return 0;
+    }
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---- internal entry points shared between translation units -----------------
+// Generic mode-n product with a factor given in both layouts:
+//   Ukj : K x J column-major (k fastest)   -- "U layout"
+//   Mjk : J x K column-major (j fastest)   -- "M layout" (the reference's M)
+// For F32/BF16 inputs both are FP32 (BF16 A uses Ub16, K x J bf16).
+struct dt_factor_views {
+    const float *U32;     // K x J
+    const float *M32;     // J x K
+    const uint16_t *Ub16; // K x J (only for BF16 A)
+    const double *M64;    // J x K (only for F64 A)
+};
+int dt_ttm_views(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t K, int64_t B,
+                 const dt_factor_views &fv, int64_t J, void *Y, int dtY);
+int dt_gram_internal(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t I, int64_t B,
+                     double *S);
+int dt_sumsq_dev(dt_ctx *ctx, const void *A, int dt, size_t n, double *out_host);
+int dt_transpose_f64(dt_ctx *ctx, double *B, const double *A, size_t m, size_t n);
+int dt_f64_to_views(dt_ctx *ctx, const double *U64, size_t K, size_t J, float *U32, float *M32,
+                    uint16_t *Ub16, double *M64);
+int dt_chol_upper(dt_ctx *ctx, const double *G, size_t n, double *R, double *Rinv, int *info_host);
+int dt_gemm_f64_internal(dt_ctx *ctx, int tA, int tB, size_t m, size_t n, size_t k, double alpha,
+                         const double *A, size_t lda, const double *B, size_t ldb, double beta,
+                         double *C, size_t ldc);
+int dt_jacobi_eig(dt_ctx *ctx, double *H, size_t n, double *V, double *w);
+int dt_copy_f64(dt_ctx *ctx, double *dst, const double *src, size_t n);
+int dt_fill_normal_f64(dt_ctx *ctx, double *out, size_t n, uint64_t seed, uint64_t stream);

@@ -1,0 +1,556 @@
+// Mode-n tensor-times-matrix without unfolding copies.
+//
+// Reference: Function::ttm (template/function/tensor.tpp:272-335) builds
+// A_(n)^T with Operator::tenmatt (operator_cpu.tpp:83-103), multiplies by M^T
+// with cblas_dgemm NT (operator_double.cpp:176-194) and folds the result back
+// with mattten (operator_cpu.tpp:115-135).  Here the column-major tensor is
+// viewed as A[f, k, b] with f < F = prod_{i<n} I_i, k < K = I_n,
+// b < B = prod_{i>n} I_i, and the product is taken in place by addressing:
+//     Y[f, j, b] = sum_k A[f, k, b] * M[j, k]
+// which is exactly tenmatt -> matmulNT -> mattten of the reference.
+//
+// Kernels (gfx950, wave64, 256 threads = 4 waves per workgroup):
+//   ttm_m0_f32 / ttm_m0_bf16  F == 1: Y(J x B) = M (J x K) . A(K x B).  Each lane
+//       streams 16-byte k-contiguous slices of A columns straight into the MFMA
+//       B operand (v_mfma_f32_16x16x4_f32 / v_mfma_f32_16x16x32_bf16).
+//   ttm_mk_f32 / ttm_mk_bf16  F > 1.  Each lane loads 16-byte f-contiguous
+//       slices; MFMA number t consumes slice element t, so a 16x16 output block
+//       holds f values strided by 4 (fp32) or 8 (bf16) and the stores are
+//       16-byte f-contiguous again.  The bf16 variant transposes 8x8 bf16 blocks
+//       in registers with v_perm_b32.
+//   ttm_generic  any shape and type (F64 path, odd sizes, J > 64).
+// All variants accumulate in FP32 (FP64 for the generic FP64 instantiation).
+#include "dt_internal.hpp"
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ floatx4 mfma_f32_4(float a, float b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ floatx4 mfma_bf16_32(u32x4 a, u32x4 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+template <typename T>
+__device__ __forceinline__ float ld_f(const T *p);
+template <>
+__device__ __forceinline__ float ld_f<float>(const float *p) { return *p; }
+template <>
+__device__ __forceinline__ float ld_f<uint16_t>(const uint16_t *p) { return bf16_to_f32(*p); }
+
+// ---------------------------------------------------------------------------
+// Generic: one thread per output element.
+template <typename TA, typename TM, typename TY, typename TACC>
+__global__ __launch_bounds__(256) void ttm_generic(const TA *__restrict__ A, const TM *__restrict__ M,
+                                                   TY *__restrict__ Y, int64_t F, int64_t K,
+                                                   int64_t B, int64_t J) {
+    const int64_t total = F * J * B;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        int64_t f = idx % F;
+        int64_t t = idx / F;
+        int64_t j = t % J;
+        int64_t b = t / J;
+        const TA *a = A + f + F * K * b;
+        TACC acc = 0;
+        for (int64_t k = 0; k < K; k++) {
+            TACC av;
+            if constexpr (sizeof(TA) == 2)
+                av = (TACC)bf16_to_f32(((const uint16_t *)a)[F * k]);
+            else
+                av = (TACC)a[F * k];
+            acc += av * (TACC)M[j + J * k];
+        }
+        Y[idx] = (TY)acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// F == 1, FP32.  Wave = 64 columns of A (4 blocks of 16), all J rows.
+//   U: K x J column-major (k fastest).  Requires K % 4 == 0.
+template <int JB>
+__global__ __launch_bounds__(256) void ttm_m0_f32(const float *__restrict__ A,
+                                                  const float *__restrict__ U,
+                                                  float *__restrict__ Y, int K, int64_t B, int J) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int c = lane & 15;
+    const int g = lane >> 4;
+    const int64_t bw = ((int64_t)blockIdx.x * 4 + wave) * 64;
+    if (bw >= B) return;
+
+    floatx4 acc[JB][4];
+#pragma unroll
+    for (int jb = 0; jb < JB; jb++)
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++) acc[jb][cb] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        float4 a[4][2];
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++) {
+            const int64_t b = bw + 16 * cb + c;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int k = k0 + 16 * h + 4 * g;
+                if (b < B && k < K)
+                    a[cb][h] = *(const float4 *)(A + k + (int64_t)K * b);
+                else
+                    a[cb][h] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+        float4 u[JB][2];
+#pragma unroll
+        for (int jb = 0; jb < JB; jb++) {
+            const int j = 16 * jb + c;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int k = k0 + 16 * h + 4 * g;
+                if (j < J && k < K)
+                    u[jb][h] = *(const float4 *)(U + k + (int64_t)K * j);
+                else
+                    u[jb][h] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int jb = 0; jb < JB; jb++)
+#pragma unroll
+                for (int cb = 0; cb < 4; cb++) {
+                    acc[jb][cb] = mfma_f32_4(u[jb][h].x, a[cb][h].x, acc[jb][cb]);
+                    acc[jb][cb] = mfma_f32_4(u[jb][h].y, a[cb][h].y, acc[jb][cb]);
+                    acc[jb][cb] = mfma_f32_4(u[jb][h].z, a[cb][h].z, acc[jb][cb]);
+                    acc[jb][cb] = mfma_f32_4(u[jb][h].w, a[cb][h].w, acc[jb][cb]);
+                }
+    }
+    // D[i][col]: i = 4g + v (row of the 16-block), col = c.
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++) {
+        const int64_t b = bw + 16 * cb + c;
+        if (b >= B) continue;
+#pragma unroll
+        for (int jb = 0; jb < JB; jb++) {
+            const int j = 16 * jb + 4 * g;
+            float *y = Y + j + (int64_t)J * b;
+            if ((J & 3) == 0 && j + 3 < J) {
+                *(float4 *)y = make_float4(acc[jb][cb][0], acc[jb][cb][1], acc[jb][cb][2],
+                                           acc[jb][cb][3]);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    if (j + v < J) y[v] = acc[jb][cb][v];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// F > 1, FP32.  Wave = FB*64 consecutive f for one b, all J columns.
+//   M: J x K column-major (j fastest).  Requires F % 4 == 0.
+template <int JB, int FB>
+__global__ __launch_bounds__(256) void ttm_mk_f32(const float *__restrict__ A,
+                                                  const float *__restrict__ M,
+                                                  float *__restrict__ Y, int64_t F, int K,
+                                                  int64_t B, int J, int64_t tiles_per_b) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int c = lane & 15;
+    const int g = lane >> 4;
+    const int64_t w = (int64_t)blockIdx.x * 4 + wave;
+    const int64_t b = w / tiles_per_b;
+    if (b >= B) return;
+    const int64_t f0 = (w - b * tiles_per_b) * (64 * FB);
+    const float *Ab = A + F * (int64_t)K * b;
+
+    floatx4 acc[JB][FB][4];
+#pragma unroll
+    for (int jb = 0; jb < JB; jb++)
+#pragma unroll
+        for (int fb = 0; fb < FB; fb++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[jb][fb][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll 4
+    for (int k0 = 0; k0 < K; k0 += 4) {
+        const int k = k0 + g;
+        float4 a[FB];
+#pragma unroll
+        for (int fb = 0; fb < FB; fb++) {
+            const int64_t f = f0 + 64 * fb + 4 * c;
+            if (k < K && f < F)
+                a[fb] = *(const float4 *)(Ab + f + F * k);
+            else
+                a[fb] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        float u[JB];
+#pragma unroll
+        for (int jb = 0; jb < JB; jb++) {
+            const int j = 16 * jb + c;
+            u[jb] = (k < K && j < J) ? M[j + (int64_t)J * k] : 0.f;
+        }
+#pragma unroll
+        for (int jb = 0; jb < JB; jb++)
+#pragma unroll
+            for (int fb = 0; fb < FB; fb++) {
+                acc[jb][fb][0] = mfma_f32_4(u[jb], a[fb].x, acc[jb][fb][0]);
+                acc[jb][fb][1] = mfma_f32_4(u[jb], a[fb].y, acc[jb][fb][1]);
+                acc[jb][fb][2] = mfma_f32_4(u[jb], a[fb].z, acc[jb][fb][2]);
+                acc[jb][fb][3] = mfma_f32_4(u[jb], a[fb].w, acc[jb][fb][3]);
+            }
+    }
+    // acc[jb][fb][t][v] = Y[f0 + 64fb + 4c + t, 16jb + 4g + v, b]
+#pragma unroll
+    for (int fb = 0; fb < FB; fb++) {
+        const int64_t f = f0 + 64 * fb + 4 * c;
+        if (f >= F) continue;
+#pragma unroll
+        for (int jb = 0; jb < JB; jb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int j = 16 * jb + 4 * g + v;
+                if (j < J)
+                    *(float4 *)(Y + f + F * (j + (int64_t)J * b)) =
+                        make_float4(acc[jb][fb][0][v], acc[jb][fb][1][v], acc[jb][fb][2][v],
+                                    acc[jb][fb][3][v]);
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// F == 1, BF16 input, FP32 accumulate/output.  Ub: K x J bf16 (k fastest).
+// Requires K % 8 == 0.  Wave = 64 columns, k-step 64 (two 128-byte halves of
+// each column line per lane group).
+template <int JB>
+__global__ __launch_bounds__(256) void ttm_m0_bf16(const uint16_t *__restrict__ A,
+                                                   const uint16_t *__restrict__ Ub,
+                                                   float *__restrict__ Y, int K, int64_t B, int J) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int c = lane & 15;
+    const int g = lane >> 4;
+    const int64_t bw = ((int64_t)blockIdx.x * 4 + wave) * 64;
+    if (bw >= B) return;
+
+    floatx4 acc[JB][4];
+#pragma unroll
+    for (int jb = 0; jb < JB; jb++)
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++) acc[jb][cb] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    const u32x4 zero4 = {0u, 0u, 0u, 0u};
+    for (int k0 = 0; k0 < K; k0 += 64) {
+        u32x4 a[4][2];
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++) {
+            const int64_t b = bw + 16 * cb + c;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int k = k0 + 32 * h + 8 * g;
+                a[cb][h] = (b < B && k < K) ? *(const u32x4 *)(A + k + (int64_t)K * b) : zero4;
+            }
+        }
+        u32x4 u[JB][2];
+#pragma unroll
+        for (int jb = 0; jb < JB; jb++) {
+            const int j = 16 * jb + c;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int k = k0 + 32 * h + 8 * g;
+                u[jb][h] = (j < J && k < K) ? *(const u32x4 *)(Ub + k + (int64_t)K * j) : zero4;
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int jb = 0; jb < JB; jb++)
+#pragma unroll
+                for (int cb = 0; cb < 4; cb++)
+                    acc[jb][cb] = mfma_bf16_32(u[jb][h], a[cb][h], acc[jb][cb]);
+    }
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++) {
+        const int64_t b = bw + 16 * cb + c;
+        if (b >= B) continue;
+#pragma unroll
+        for (int jb = 0; jb < JB; jb++) {
+            const int j = 16 * jb + 4 * g;
+            float *y = Y + j + (int64_t)J * b;
+            if ((J & 3) == 0 && j + 3 < J) {
+                *(float4 *)y = make_float4(acc[jb][cb][0], acc[jb][cb][1], acc[jb][cb][2],
+                                           acc[jb][cb][3]);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    if (j + v < J) y[v] = acc[jb][cb][v];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// F > 1, BF16 input.  Wave = 128 consecutive f (8 per lane column) for one b.
+// Each lane loads 8 k-rows of 8 f-contiguous bf16 and transposes the 8x8 block
+// in registers so that MFMA t receives the 8 k values of f = f0 + 8c + t.
+// Requires F % 8 == 0 and K % 8 == 0.
+template <int JB>
+__global__ __launch_bounds__(256) void ttm_mk_bf16(const uint16_t *__restrict__ A,
+                                                   const uint16_t *__restrict__ Ub,
+                                                   float *__restrict__ Y, int64_t F, int K,
+                                                   int64_t B, int J, int64_t tiles_per_b) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int c = lane & 15;
+    const int g = lane >> 4;
+    const int64_t w = (int64_t)blockIdx.x * 4 + wave;
+    const int64_t b = w / tiles_per_b;
+    if (b >= B) return;
+    const int64_t f = (w - b * tiles_per_b) * 128 + 8 * c;
+    const uint16_t *Ab = A + F * (int64_t)K * b;
+    const bool fin = f < F;
+
+    floatx4 acc[JB][8];
+#pragma unroll
+    for (int jb = 0; jb < JB; jb++)
+#pragma unroll
+        for (int t = 0; t < 8; t++) acc[jb][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    const u32x4 zero4 = {0u, 0u, 0u, 0u};
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        const int kb = k0 + 8 * g;
+        const bool kin = kb < K;
+        u32x4 r[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+            r[e] = (kin && fin) ? *(const u32x4 *)(Ab + f + F * (int64_t)(kb + e)) : zero4;
+        u32x4 u[JB];
+#pragma unroll
+        for (int jb = 0; jb < JB; jb++) {
+            const int j = 16 * jb + c;
+            u[jb] = (kin && j < J) ? *(const u32x4 *)(Ub + kb + (int64_t)K * j) : zero4;
+        }
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const uint32_t sel = (t & 1) ? 0x07060302u : 0x05040100u;
+            u32x4 bv;
+            bv.x = __builtin_amdgcn_perm(r[1][t >> 1], r[0][t >> 1], sel);
+            bv.y = __builtin_amdgcn_perm(r[3][t >> 1], r[2][t >> 1], sel);
+            bv.z = __builtin_amdgcn_perm(r[5][t >> 1], r[4][t >> 1], sel);
+            bv.w = __builtin_amdgcn_perm(r[7][t >> 1], r[6][t >> 1], sel);
+#pragma unroll
+            for (int jb = 0; jb < JB; jb++) acc[jb][t] = mfma_bf16_32(u[jb], bv, acc[jb][t]);
+        }
+    }
+    if (!fin) return;
+#pragma unroll
+    for (int jb = 0; jb < JB; jb++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int j = 16 * jb + 4 * g + v;
+            if (j >= J) continue;
+            float *y = Y + f + F * (j + (int64_t)J * b);
+            *(float4 *)y = make_float4(acc[jb][0][v], acc[jb][1][v], acc[jb][2][v], acc[jb][3][v]);
+            *(float4 *)(y + 4) =
+                make_float4(acc[jb][4][v], acc[jb][5][v], acc[jb][6][v], acc[jb][7][v]);
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Transpose with conversion: dst (n x m) = src (m x n)^T, column-major.
+template <typename TD, typename TS>
+__global__ void transpose_convert(TD *__restrict__ dst, const TS *__restrict__ src, int64_t m,
+                                  int64_t n) {
+    const int64_t total = m * n;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        int64_t i = idx % m, j = idx / m;  // src(i, j)
+        double v = (double)src[idx];
+        if constexpr (sizeof(TD) == 2)
+            dst[j + n * i] = f32_to_bf16((float)v);
+        else
+            dst[j + n * i] = (TD)v;
+    }
+}
+template <typename TD, typename TS>
+__global__ void copy_convert(TD *__restrict__ dst, const TS *__restrict__ src, int64_t total) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        double v = (double)src[idx];
+        if constexpr (sizeof(TD) == 2)
+            dst[idx] = f32_to_bf16((float)v);
+        else
+            dst[idx] = (TD)v;
+    }
+}
+
+static inline dim3 gs_grid(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    if (g > 262144) g = 262144;
+    if (g < 1) g = 1;
+    return dim3((unsigned)g);
+}
+
+int dt_transpose_f64(dt_ctx *ctx, double *B, const double *A, size_t m, size_t n) {
+    DT_LAUNCH("transpose", (transpose_convert<double, double>), gs_grid(m * n), dim3(256), 0, B, A,
+              (int64_t)m, (int64_t)n);
+    return DT_OK;
+}
+
+// From an FP64 K x J factor build the kernel views (any pointer may be null).
+int dt_f64_to_views(dt_ctx *ctx, const double *U64, size_t K, size_t J, float *U32, float *M32,
+                    uint16_t *Ub16, double *M64) {
+    const int64_t n = (int64_t)(K * J);
+    if (U32) {
+        DT_LAUNCH("factor_views", (copy_convert<float, double>), gs_grid(n), dim3(256), 0, U32, U64, n);
+    }
+    if (M32) {
+        DT_LAUNCH("factor_views", (transpose_convert<float, double>), gs_grid(n), dim3(256), 0, M32,
+                  U64, (int64_t)K, (int64_t)J);
+    }
+    if (Ub16) {
+        DT_LAUNCH("factor_views", (copy_convert<uint16_t, double>), gs_grid(n), dim3(256), 0, Ub16,
+                  U64, n);
+    }
+    if (M64) {
+        DT_LAUNCH("factor_views", (transpose_convert<double, double>), gs_grid(n), dim3(256), 0, M64,
+                  U64, (int64_t)K, (int64_t)J);
+    }
+    return DT_OK;
+}
+
+// ---------------------------------------------------------------------------
+int dt_ttm_views(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t K, int64_t B,
+                 const dt_factor_views &fv, int64_t J, void *Y, int dtY) {
+    const int64_t total = F * J * B;
+    if (total == 0) return DT_OK;
+    if (dtA == DT_F64) {
+        DT_CHECK_ARG(dtY == DT_F64 && fv.M64, "ttm: F64 input needs F64 output and factor");
+        DT_LAUNCH("ttm_generic_f64", (ttm_generic<double, double, double, double>), gs_grid(total),
+                  dim3(256), 0, (const double *)A, fv.M64, (double *)Y, F, K, B, J);
+        return DT_OK;
+    }
+    DT_CHECK_ARG(dtY == DT_F32, "ttm: F32/BF16 input needs F32 output");
+    const int jbn = (int)ceil_div(J, 16);
+    if (dtA == DT_F32) {
+        if (J <= 64 && F == 1 && (K % 4) == 0 && fv.U32) {
+            dim3 grid((unsigned)ceil_div(B, 256));
+#define M0F(JBV)                                                                                \
+    DT_LAUNCH("ttm_m0_f32", ttm_m0_f32<JBV>, grid, dim3(256), 0, (const float *)A, fv.U32,      \
+              (float *)Y, (int)K, B, (int)J);                                                   \
+    return DT_OK;
+            if (jbn == 1) { M0F(1) }
+            if (jbn == 2) { M0F(2) }
+            M0F(4)
+#undef M0F
+        }
+        if (J <= 64 && F > 1 && (F % 4) == 0 && fv.M32) {
+            // f tile per wave: 64*FB; keep JB*FB <= 4.
+            int fb = 1;
+            if (jbn == 1) fb = F >= 256 ? 4 : (F >= 128 ? 2 : 1);
+            else if (jbn == 2) fb = F >= 128 ? 2 : 1;
+            const int64_t tiles = ceil_div(F, 64 * fb);
+            const int64_t waves = tiles * B;
+            dim3 grid((unsigned)ceil_div(waves, 4));
+#define MKF(JBV, FBV)                                                                          \
+    DT_LAUNCH("ttm_mk_f32", (ttm_mk_f32<JBV, FBV>), grid, dim3(256), 0, (const float *)A,      \
+              fv.M32, (float *)Y, F, (int)K, B, (int)J, tiles);                                \
+    return DT_OK;
+            if (jbn == 1 && fb == 4) { MKF(1, 4) }
+            if (jbn == 1 && fb == 2) { MKF(1, 2) }
+            if (jbn == 1) { MKF(1, 1) }
+            if (jbn == 2 && fb == 2) { MKF(2, 2) }
+            if (jbn == 2) { MKF(2, 1) }
+            MKF(4, 1)
+#undef MKF
+        }
+        DT_CHECK_ARG(fv.M32, "ttm: missing FP32 factor");
+        DT_LAUNCH("ttm_generic_f32", (ttm_generic<float, float, float, float>), gs_grid(total),
+                  dim3(256), 0, (const float *)A, fv.M32, (float *)Y, F, K, B, J);
+        return DT_OK;
+    }
+    if (dtA == DT_BF16) {
+        if (J <= 64 && F == 1 && (K % 8) == 0 && fv.Ub16) {
+            dim3 grid((unsigned)ceil_div(B, 256));
+#define M0B(JBV)                                                                                 \
+    DT_LAUNCH("ttm_m0_bf16", ttm_m0_bf16<JBV>, grid, dim3(256), 0, (const uint16_t *)A, fv.Ub16, \
+              (float *)Y, (int)K, B, (int)J);                                                    \
+    return DT_OK;
+            if (jbn == 1) { M0B(1) }
+            if (jbn == 2) { M0B(2) }
+            M0B(4)
+#undef M0B
+        }
+        if (J <= 64 && F > 1 && (F % 8) == 0 && (K % 8) == 0 && fv.Ub16) {
+            const int64_t tiles = ceil_div(F, 128);
+            dim3 grid((unsigned)ceil_div(tiles * B, 4));
+#define MKB(JBV)                                                                                 \
+    DT_LAUNCH("ttm_mk_bf16", ttm_mk_bf16<JBV>, grid, dim3(256), 0, (const uint16_t *)A, fv.Ub16, \
+              (float *)Y, F, (int)K, B, (int)J, tiles);                                          \
+    return DT_OK;
+            if (jbn == 1) { MKB(1) }
+            if (jbn == 2) { MKB(2) }
+            MKB(4)
+#undef MKB
+        }
+        DT_CHECK_ARG(fv.M32, "ttm: missing FP32 factor");
+        DT_LAUNCH("ttm_generic_bf16", (ttm_generic<uint16_t, float, float, float>), gs_grid(total),
+                  dim3(256), 0, (const uint16_t *)A, fv.M32, (float *)Y, F, K, B, J);
+        return DT_OK;
+    }
+    return dt_fail(ctx, DT_EUNSUPPORTED, "ttm: unsupported input type %d", dtA);
+}
+
+static void mode_split(int ndim, const size_t *shape, int n, int64_t &F, int64_t &K, int64_t &B) {
+    F = 1;
+    B = 1;
+    for (int i = 0; i < n; i++) F *= (int64_t)shape[i];
+    K = (int64_t)shape[n];
+    for (int i = n + 1; i < ndim; i++) B *= (int64_t)shape[i];
+}
+
+extern "C" int dt_ttm(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape,
+                      const void *M, int dtM, size_t J, int n, void *Y, int dtY) {
+    DT_CHECK_ARG(ctx && A && M && Y && shape, "dt_ttm: null argument");
+    DT_CHECK_ARG(ndim >= 1 && ndim <= DT_MAX_NDIM && n >= 0 && n < ndim, "dt_ttm: bad mode %d of %d",
+                 n, ndim);
+    DT_CHECK_ARG(J >= 1, "dt_ttm: J must be positive");
+    DT_CHECK_ARG(dtM == DT_F64 || dtM == DT_F32, "dt_ttm: factor must be F64 or F32");
+    int64_t F, K, B;
+    mode_split(ndim, shape, n, F, K, B);
+    const int64_t mk = (int64_t)J * K;
+    dt_factor_views fv = {};
+    if (dtA == DT_F64) {
+        if (dtM == DT_F64) {
+            fv.M64 = (const double *)M;
+        } else {
+            double *m64 = (double *)dt_ws(ctx, WS_TTM_M, mk * sizeof(double));
+            if (!m64) return DT_ENOMEM;
+            DT_LAUNCH("factor_views", (copy_convert<double, float>), gs_grid(mk), dim3(256), 0, m64,
+                      (const float *)M, mk);
+            fv.M64 = m64;
+        }
+    } else {
+        // workspace: M32 (J x K) | U32 (K x J) | Ub16 (K x J)
+        char *ws = (char *)dt_ws(ctx, WS_TTM_M, mk * (4 + 4 + 2) + 512);
+        if (!ws) return DT_ENOMEM;
+        float *m32 = (float *)ws;
+        float *u32 = (float *)(ws + ((mk * 4 + 255) & ~255LL));
+        uint16_t *ub = (uint16_t *)((char *)u32 + ((mk * 4 + 255) & ~255LL));
+        if (dtM == DT_F64) {
+            DT_LAUNCH("factor_views", (copy_convert<float, double>), gs_grid(mk), dim3(256), 0, m32,
+                      (const double *)M, mk);
+        } else {
+            DT_HIP(hipMemcpyAsync(m32, M, mk * 4, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        // M is J x K; U = M^T is K x J.
+        DT_LAUNCH("factor_views", (transpose_convert<float, float>), gs_grid(mk), dim3(256), 0, u32,
+                  m32, (int64_t)J, K);
+        DT_LAUNCH("factor_views", (copy_convert<uint16_t, float>), gs_grid(mk), dim3(256), 0, ub, u32,
+                  mk);
+        fv.M32 = m32;
+        fv.U32 = u32;
+        fv.Ub16 = ub;
+    }
+    return dt_ttm_views(ctx, A, dtA, F, K, B, fv, (int64_t)J, Y, dtY);
+}

@@ -1,0 +1,439 @@
+"""ctypes binding of libdiana_hip.so (the C ABI declared in include/diana_hip.h).
+
+Used by the test-suite and bench.py to drive the HIP library through exactly the
+entry points a foreign host would bind.  Nothing here computes: every numerical
+call goes to the shared library, and loading fails loudly when it is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "libdiana_hip.so")
+
+DT_OK, DT_EINVAL, DT_ENOMEM, DT_EHIP, DT_ENUMERIC, DT_EUNSUPPORTED = range(6)
+DT_F64, DT_F32, DT_BF16, DT_I32 = range(4)
+DT_MAX_NDIM = 8
+INIT_GIVEN, INIT_QR, INIT_HOSVD, INIT_RANDOM, INIT_LEARNED = range(5)
+SCHED_REFERENCE, SCHED_TREE = 0, 1
+FLAG_DIRECT_RESIDUAL, FLAG_FP32_POLISH = 1, 2
+
+_NP_OF = {DT_F64: np.float64, DT_F32: np.float32, DT_BF16: np.uint16, DT_I32: np.int32}
+_SIZE_OF = {DT_F64: 8, DT_F32: 4, DT_BF16: 2, DT_I32: 4}
+
+
+class DTError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"diana_hip error {code}: {msg}")
+        self.code = code
+
+
+class TuckerSpec(C.Structure):
+    _fields_ = [("ndim", C.c_int32), ("shape", C.c_size_t * DT_MAX_NDIM),
+                ("ranks", C.c_size_t * DT_MAX_NDIM), ("noise", C.c_double),
+                ("decay", C.c_double), ("seed", C.c_uint64)]
+
+
+class HooiArgs(C.Structure):
+    _fields_ = [("ndim", C.c_int32), ("dtype", C.c_int32), ("A", C.c_void_p),
+                ("A_f32", C.c_void_p), ("shape", C.c_size_t * DT_MAX_NDIM),
+                ("ranks", C.c_size_t * DT_MAX_NDIM), ("max_iter", C.c_int32),
+                ("inner_iter", C.c_int32), ("init", C.c_int32), ("schedule", C.c_int32),
+                ("tol", C.c_double), ("U_init", C.c_void_p * DT_MAX_NDIM),
+                ("seed", C.c_uint64), ("flags", C.c_int32), ("model", C.c_void_p)]
+
+
+class HooiResult(C.Structure):
+    _fields_ = [("U", C.c_void_p * DT_MAX_NDIM), ("G", C.c_void_p),
+                ("residual_per_sweep", C.POINTER(C.c_double)), ("sweeps", C.c_int32),
+                ("init_used", C.c_int32), ("A_norm", C.c_double), ("G_norm", C.c_double),
+                ("residual", C.c_double), ("residual_direct", C.c_double),
+                ("init_ms", C.c_double), ("sweeps_ms", C.c_double),
+                ("learned_first_res", C.c_double), ("hosvd_first_res", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the shared library once; raise if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing: build it with `make -C diana-tucker_amd` "
+                "(or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp, sz, i32, dbl, u64 = C.c_void_p, C.c_size_t, C.c_int, C.c_double, C.c_uint64
+        sig = {
+            "dt_create": [C.POINTER(vp), i32], "dt_destroy": [vp], "dt_last_error": [vp],
+            "dt_malloc": [vp, C.POINTER(vp), sz], "dt_free": [vp, vp],
+            "dt_h2d": [vp, vp, vp, sz], "dt_d2h": [vp, vp, vp, sz], "dt_d2d": [vp, vp, vp, sz],
+            "dt_zero": [vp, vp, sz], "dt_sync": [vp],
+            "dt_convert": [vp, vp, i32, vp, i32, sz],
+            "dt_timing_enable": [vp, i32], "dt_timing_reset": [vp],
+            "dt_timing_get": [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(dbl)],
+            "dt_timing_list": [vp, C.c_char_p, sz],
+            "dt_timer_start": [vp], "dt_timer_stop": [vp, C.POINTER(dbl)],
+            "dt_ttm": [vp, vp, i32, i32, C.POINTER(sz), vp, i32, sz, i32, vp, i32],
+            "dt_gram": [vp, vp, i32, i32, C.POINTER(sz), i32, vp, i32],
+            "dt_ttt_except": [vp, vp, vp, i32, i32, C.POINTER(sz), sz, i32, vp],
+            "dt_sumsq": [vp, vp, i32, sz, C.POINTER(dbl)],
+            "dt_gemm_f64": [vp, i32, i32, sz, sz, sz, dbl, vp, sz, vp, sz, dbl, vp, sz],
+            "dt_qr_f64": [vp, vp, vp, vp, sz, sz],
+            "dt_inverse_f64": [vp, vp, vp, sz],
+            "dt_syev_top_f64": [vp, vp, sz, sz, vp, vp],
+            "dt_orth_rows_f64": [vp, vp, sz, sz],
+            "dt_gen_normal": [vp, vp, i32, sz, u64, u64],
+            "dt_gen_uniform": [vp, vp, i32, sz, u64, u64],
+            "dt_gen_tucker": [vp, C.POINTER(TuckerSpec), vp, vp, C.POINTER(vp),
+                              C.POINTER(dbl), C.POINTER(dbl)],
+            "dt_hooi_dense": [vp, C.POINTER(HooiArgs), C.POINTER(HooiResult)],
+            "dt_hooi_create": [vp, C.POINTER(HooiArgs), C.POINTER(vp)],
+            "dt_hooi_sweep": [vp, i32, C.POINTER(dbl)],
+            "dt_hooi_get": [vp, C.POINTER(HooiResult)],
+            "dt_hooi_destroy": [vp],
+        }
+        for name, argtypes in sig.items():
+            f = getattr(L, name)
+            f.argtypes = argtypes
+            f.restype = C.c_char_p if name == "dt_last_error" else C.c_int
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names this binding expects the library to export."""
+    return ["dt_create", "dt_destroy", "dt_last_error", "dt_malloc", "dt_free", "dt_h2d",
+            "dt_d2h", "dt_d2d", "dt_zero", "dt_sync", "dt_convert", "dt_timing_enable",
+            "dt_timing_reset", "dt_timing_get", "dt_timing_list", "dt_timer_start",
+            "dt_timer_stop", "dt_ttm", "dt_gram", "dt_ttt_except", "dt_sumsq", "dt_gemm_f64",
+            "dt_qr_f64", "dt_inverse_f64", "dt_syev_top_f64", "dt_orth_rows_f64",
+            "dt_gen_normal", "dt_gen_uniform", "dt_gen_tucker", "dt_hooi_dense",
+            "dt_hooi_create", "dt_hooi_sweep", "dt_hooi_get", "dt_hooi_destroy"]
+
+
+class DeviceArray:
+    """A device allocation with a numpy-like shape (column-major semantics)."""
+
+    def __init__(self, ctx: "Context", shape, dtype: int):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = dtype
+        self.size = int(np.prod(self.shape)) if self.shape else 1
+        self.nbytes = self.size * _SIZE_OF[dtype]
+        self.ptr = ctx.malloc(self.nbytes)
+
+    def upload(self, host: np.ndarray):
+        h = np.asarray(host)
+        want = _NP_OF[self.dtype]
+        if h.dtype != want:
+            h = h.astype(want)
+        flat = np.ascontiguousarray(h.reshape(-1, order="F"))
+        assert flat.size == self.size, (flat.size, self.size)
+        self.ctx.h2d(self.ptr, flat.ctypes.data, self.nbytes)
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.size, dtype=_NP_OF[self.dtype])
+        self.ctx.d2h(out.ctypes.data, self.ptr, self.nbytes)
+        return out.reshape(self.shape, order="F")
+
+    def free(self):
+        if self.ptr:
+            self.ctx.free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.handle:
+                self.ctx.free(self.ptr)
+        except Exception:
+            pass
+
+
+def bf16_from_f32(x: np.ndarray) -> np.ndarray:
+    """Round-to-nearest-even FP32 -> BF16 bit patterns (uint16)."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    lsb = (u >> 16) & 1
+    r = ((u + 0x7FFF + lsb) >> 16).astype(np.uint16)
+    return r.reshape(np.shape(x))
+
+
+def f32_from_bf16(b: np.ndarray) -> np.ndarray:
+    return (np.asarray(b, dtype=np.uint32) << 16).view(np.float32)
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.dt_create(C.byref(h), device)
+        if rc != DT_OK:
+            raise DTError(rc, f"dt_create(device={device}) failed")
+        self.handle = h
+
+    def check(self, rc):
+        if rc != DT_OK:
+            msg = self.L.dt_last_error(self.handle)
+            raise DTError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if self.handle:
+            self.L.dt_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # memory
+    def malloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        self.check(self.L.dt_malloc(self.handle, C.byref(p), nbytes))
+        return p.value
+
+    def free(self, ptr):
+        self.check(self.L.dt_free(self.handle, ptr))
+
+    def h2d(self, dst, src, nbytes):
+        self.check(self.L.dt_h2d(self.handle, dst, src, nbytes))
+
+    def d2h(self, dst, src, nbytes):
+        self.check(self.L.dt_d2h(self.handle, dst, src, nbytes))
+
+    def sync(self):
+        self.check(self.L.dt_sync(self.handle))
+
+    def array(self, shape, dtype=DT_F32) -> DeviceArray:
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, host: np.ndarray, dtype=None) -> DeviceArray:
+        h = np.asarray(host)
+        if dtype is None:
+            dtype = {np.dtype(np.float64): DT_F64, np.dtype(np.float32): DT_F32,
+                     np.dtype(np.uint16): DT_BF16, np.dtype(np.int32): DT_I32}[h.dtype]
+        return DeviceArray(self, h.shape, dtype).upload(h)
+
+    # timing
+    def timing(self, on=True):
+        self.check(self.L.dt_timing_enable(self.handle, 1 if on else 0))
+
+    def timing_reset(self):
+        self.check(self.L.dt_timing_reset(self.handle))
+
+    def timing_get(self, name: str):
+        cnt = C.c_int64()
+        ms = C.c_double()
+        self.check(self.L.dt_timing_get(self.handle, name.encode(), C.byref(cnt), C.byref(ms)))
+        return cnt.value, ms.value
+
+    def timing_list(self):
+        buf = C.create_string_buffer(1 << 16)
+        self.check(self.L.dt_timing_list(self.handle, buf, len(buf)))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms = line.split()
+            out[name] = (int(cnt), float(ms))
+        return out
+
+    def timer_start(self):
+        self.check(self.L.dt_timer_start(self.handle))
+
+    def timer_stop(self) -> float:
+        ms = C.c_double()
+        self.check(self.L.dt_timer_stop(self.handle, C.byref(ms)))
+        return ms.value
+
+    # operators
+    def ttm(self, A: DeviceArray, M: DeviceArray, n: int) -> DeviceArray:
+        J = M.shape[0]
+        assert M.shape[1] == A.shape[n]
+        out_shape = list(A.shape)
+        out_shape[n] = J
+        ydt = DT_F64 if A.dtype == DT_F64 else DT_F32
+        Y = self.array(out_shape, ydt)
+        shp = (C.c_size_t * len(A.shape))(*A.shape)
+        self.check(self.L.dt_ttm(self.handle, A.ptr, A.dtype, len(A.shape), shp, M.ptr, M.dtype,
+                                 J, n, Y.ptr, ydt))
+        return Y
+
+    def gram(self, A: DeviceArray, n: int, dtS=DT_F64) -> DeviceArray:
+        I = A.shape[n]
+        S = self.array((I, I), dtS)
+        shp = (C.c_size_t * len(A.shape))(*A.shape)
+        self.check(self.L.dt_gram(self.handle, A.ptr, A.dtype, len(A.shape), shp, n, S.ptr, dtS))
+        return S
+
+    def ttt_except(self, A: DeviceArray, B: DeviceArray, n: int) -> DeviceArray:
+        assert A.dtype == B.dtype
+        Cm = self.array((A.shape[n], B.shape[n]), DT_F64)
+        shp = (C.c_size_t * len(A.shape))(*A.shape)
+        self.check(self.L.dt_ttt_except(self.handle, A.ptr, B.ptr, A.dtype, len(A.shape), shp,
+                                        B.shape[n], n, Cm.ptr))
+        return Cm
+
+    def sumsq(self, A: DeviceArray) -> float:
+        out = C.c_double()
+        self.check(self.L.dt_sumsq(self.handle, A.ptr, A.dtype, A.size, C.byref(out)))
+        return out.value
+
+    def gemm(self, tA, tB, A: DeviceArray, B: DeviceArray, alpha=1.0) -> DeviceArray:
+        m = A.shape[1] if tA else A.shape[0]
+        k = A.shape[0] if tA else A.shape[1]
+        n = B.shape[0] if tB else B.shape[1]
+        Cm = self.array((m, n), DT_F64)
+        self.check(self.L.dt_gemm_f64(self.handle, int(tA), int(tB), m, n, k, alpha, A.ptr,
+                                      A.shape[0], B.ptr, B.shape[0], 0.0, Cm.ptr, m))
+        return Cm
+
+    def qr(self, A: DeviceArray, want_r=True):
+        m, n = A.shape
+        Q = self.array((m, n), DT_F64)
+        R = self.array((n, n), DT_F64) if want_r else None
+        self.check(self.L.dt_qr_f64(self.handle, Q.ptr, R.ptr if R else None, A.ptr, m, n))
+        return Q, R
+
+    def inverse(self, A: DeviceArray) -> DeviceArray:
+        n = A.shape[0]
+        Cm = self.array((n, n), DT_F64)
+        self.check(self.L.dt_inverse_f64(self.handle, Cm.ptr, A.ptr, n))
+        return Cm
+
+    def syev_top(self, S: DeviceArray, r: int):
+        n = S.shape[0]
+        V = self.array((n, r), DT_F64)
+        w = self.array((r,), DT_F64)
+        self.check(self.L.dt_syev_top_f64(self.handle, S.ptr, n, r, V.ptr, w.ptr))
+        return V, w
+
+    def orth_rows(self, A: DeviceArray):
+        self.check(self.L.dt_orth_rows_f64(self.handle, A.ptr, A.shape[0], A.shape[1]))
+        return A
+
+    def gen_normal(self, n, dtype, seed, stream) -> DeviceArray:
+        X = self.array((n,), dtype)
+        self.check(self.L.dt_gen_normal(self.handle, X.ptr, dtype, n, seed, stream))
+        return X
+
+    def gen_uniform(self, n, dtype, seed, stream) -> DeviceArray:
+        X = self.array((n,), dtype)
+        self.check(self.L.dt_gen_uniform(self.handle, X.ptr, dtype, n, seed, stream))
+        return X
+
+    def gen_tucker(self, shape, ranks, noise, seed, decay=0.0, f32=True, bf16=False,
+                   want_factors=False):
+        spec = TuckerSpec()
+        spec.ndim = len(shape)
+        for i, (s, r) in enumerate(zip(shape, ranks)):
+            spec.shape[i] = s
+            spec.ranks[i] = r
+        spec.noise = noise
+        spec.decay = decay
+        spec.seed = seed
+        A = self.array(shape, DT_F32)
+        Ab = self.array(shape, DT_BF16) if bf16 else None
+        Us = [self.array((s, r), DT_F64) for s, r in zip(shape, ranks)] if want_factors else None
+        uptr = (C.c_void_p * len(shape))(*[u.ptr for u in Us]) if Us else None
+        nx, na = C.c_double(), C.c_double()
+        self.check(self.L.dt_gen_tucker(self.handle, C.byref(spec), A.ptr, Ab.ptr if Ab else None,
+                                        uptr, C.byref(nx), C.byref(na)))
+        return {"A": A, "A_bf16": Ab, "U": Us, "normX": nx.value, "normA": na.value}
+
+    # HOOI
+    def _hooi_args(self, A, ranks, max_iter=10, inner_iter=5, init=INIT_HOSVD,
+                   schedule=SCHED_TREE, tol=0.0, U_init=None, seed=0, flags=0, A_f32=None,
+                   model=None):
+        args = HooiArgs()
+        args.ndim = len(A.shape)
+        args.dtype = A.dtype
+        args.A = A.ptr
+        args.A_f32 = A_f32.ptr if A_f32 is not None else None
+        for i, (s, r) in enumerate(zip(A.shape, ranks)):
+            args.shape[i] = s
+            args.ranks[i] = r
+        args.max_iter = max_iter
+        args.inner_iter = inner_iter
+        args.init = init
+        args.schedule = schedule
+        args.tol = tol
+        if U_init is not None:
+            for i, u in enumerate(U_init):
+                args.U_init[i] = u.ptr
+        args.seed = seed
+        args.flags = flags
+        args.model = model
+        return args
+
+    def hooi_dense(self, A: DeviceArray, ranks, want_factors=True, **kw):
+        args = self._hooi_args(A, ranks, **kw)
+        N = len(A.shape)
+        res = HooiResult()
+        Us = [self.array((A.shape[i], ranks[i]), DT_F64) for i in range(N)] if want_factors else []
+        G = self.array(tuple(ranks), DT_F64) if want_factors else None
+        for i, u in enumerate(Us):
+            res.U[i] = u.ptr
+        res.G = G.ptr if G else None
+        per = (C.c_double * max(1, args.max_iter))()
+        res.residual_per_sweep = per
+        self.check(self.L.dt_hooi_dense(self.handle, C.byref(args), C.byref(res)))
+        out = {k: getattr(res, k) for k in ("sweeps", "init_used", "A_norm", "G_norm", "residual",
+                                            "residual_direct", "init_ms", "sweeps_ms",
+                                            "learned_first_res", "hosvd_first_res")}
+        out["residuals"] = [per[i] for i in range(res.sweeps)]
+        out["U"] = [u.download() for u in Us]
+        out["G"] = G.download() if G else None
+        return out
+
+    def hooi_session(self, A: DeviceArray, ranks, **kw):
+        return HooiSession(self, A, ranks, **kw)
+
+
+class HooiSession:
+    """dt_hooi_create / dt_hooi_sweep / dt_hooi_get: one sweep per call."""
+
+    def __init__(self, ctx: Context, A: DeviceArray, ranks, **kw):
+        self.ctx = ctx
+        self.A = A
+        self.ranks = list(ranks)
+        self.args = ctx._hooi_args(A, ranks, **kw)
+        h = C.c_void_p()
+        ctx.check(ctx.L.dt_hooi_create(ctx.handle, C.byref(self.args), C.byref(h)))
+        self.h = h
+
+    def sweep(self, fp32=False) -> float:
+        r = C.c_double()
+        self.ctx.check(self.ctx.L.dt_hooi_sweep(self.h, 1 if fp32 else 0, C.byref(r)))
+        return r.value
+
+    def result(self, want_factors=True):
+        N = len(self.A.shape)
+        res = HooiResult()
+        Us = [self.ctx.array((self.A.shape[i], self.ranks[i]), DT_F64) for i in range(N)] \
+            if want_factors else []
+        G = self.ctx.array(tuple(self.ranks), DT_F64) if want_factors else None
+        for i, u in enumerate(Us):
+            res.U[i] = u.ptr
+        res.G = G.ptr if G else None
+        self.ctx.check(self.ctx.L.dt_hooi_get(self.h, C.byref(res)))
+        out = {k: getattr(res, k) for k in ("sweeps", "init_used", "A_norm", "G_norm", "residual",
+                                            "init_ms", "sweeps_ms")}
+        out["U"] = [u.download() for u in Us]
+        out["G"] = G.download() if G else None
+        return out
+
+    def close(self):
+        if self.h:
+            self.ctx.L.dt_hooi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

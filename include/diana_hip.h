@@ -1,0 +1,231 @@
+/*
+ * diana_hip.h -- C ABI of the MI355X (gfx950) HOOI-ALS library `libdiana_hip.so`.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and returns a
+ * status code (DT_OK = 0).  No entry point calls exit()/abort(); the text of the
+ * last failure is available from dt_last_error().  Tensors are column-major with
+ * the first index fastest, exactly as the reference stores them
+ * (template/tensor_mpi.tpp:49-53).  Matrices are column-major with implicit
+ * leading dimension unless an ld argument is given.
+ *
+ * Each entry point names the reference interface it replaces.  The reference
+ * calls these through `A.op()->...` (an Operator<Ty> instance owned by each
+ * Tensor, include/operator.hpp:8-70) and through the Function:: templates
+ * (include/function.hpp:6-94); the C++ mirror in diana-tucker_amd/host keeps
+ * those names and forwards here.
+ *
+ * Device memory: pointers marked "device" are allocations made with dt_malloc()
+ * (or any hipMalloc'd memory on the context's device).  All work is enqueued on
+ * the context's single HIP stream; entry points that return host values
+ * synchronise that stream.  A context must be used from one host thread at a
+ * time (the reference is driven by one thread, spmain.cpp:10-15).
+ */
+#ifndef DIANA_HIP_H
+#define DIANA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    DT_OK = 0,
+    DT_EINVAL = 1,       /* bad argument (shape, rank > mode size, null pointer) */
+    DT_ENOMEM = 2,       /* device allocation failed */
+    DT_EHIP = 3,         /* HIP runtime error */
+    DT_ENUMERIC = 4,     /* non-finite value or breakdown (singular Gram, ||A|| = 0) */
+    DT_EUNSUPPORTED = 5  /* combination not implemented */
+};
+
+/* ---- element types ----------------------------------------------------- */
+enum { DT_F64 = 0, DT_F32 = 1, DT_BF16 = 2, DT_I32 = 3 };
+
+#define DT_MAX_NDIM 8
+
+typedef struct dt_ctx dt_ctx;
+
+/* ---- context, memory, stream ------------------------------------------ */
+/* Creates a context on HIP device `device` with one stream.  Replaces the
+ * per-tensor `new Operator<Ty>()` (template/tensor_mpi.tpp:26) and the
+ * communicator set-up (src/communicator.cpp:3-15) of a single rank. */
+int dt_create(dt_ctx **ctx, int device);
+int dt_destroy(dt_ctx *ctx);
+const char *dt_last_error(const dt_ctx *ctx);
+/* Operator<Ty>::alloc / free (template/operator/operator_cpu.tpp:6-12). */
+int dt_malloc(dt_ctx *ctx, void **ptr, size_t bytes);
+int dt_free(dt_ctx *ctx, void *ptr);
+/* Operator<Ty>::mcpy (operator_cpu.tpp:14-17), split by direction. */
+int dt_h2d(dt_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
+int dt_d2h(dt_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
+int dt_d2d(dt_ctx *ctx, void *dst_dev, const void *src_dev, size_t bytes);
+/* Operator<Ty>::constant(A, 0, n) (src/operator/operator_double.cpp:51-56). */
+int dt_zero(dt_ctx *ctx, void *dev, size_t bytes);
+int dt_sync(dt_ctx *ctx);
+/* Element type conversion dst[i] = (dt_dst) src[i] on the device (F64/F32/BF16). */
+int dt_convert(dt_ctx *ctx, void *dst, int dt_dst, const void *src, int dt_src, size_t n);
+
+/* ---- timing ------------------------------------------------------------
+ * With timing enabled every kernel launch of the library is bracketed by HIP
+ * events on the context stream; dt_timing_get returns the launch count and the
+ * summed event time for one kernel name (names: see DESIGN.md kernel table).
+ * dt_timer_* time a region of the stream with a pair of events.  Replaces the
+ * MPI_Wtime-based Summary timers (src/summary.cpp:29-69). */
+int dt_timing_enable(dt_ctx *ctx, int on);
+int dt_timing_reset(dt_ctx *ctx);
+int dt_timing_get(dt_ctx *ctx, const char *kernel, int64_t *count, double *total_ms);
+int dt_timing_list(dt_ctx *ctx, char *buf, size_t buflen); /* "name count ms\n" lines */
+int dt_timer_start(dt_ctx *ctx);
+int dt_timer_stop(dt_ctx *ctx, double *elapsed_ms);
+
+/* ---- operator level ----------------------------------------------------- */
+/* Y = A x_n M.  Replaces Function::ttm (template/function/tensor.tpp:272-335)
+ * = Operator::tenmatt + matmulNT + mattten (operator_cpu.tpp:83-135,
+ * operator_double.cpp:176-194) at one rank, without the unfold copies.
+ * A: device, `ndim` modes of sizes shape[], element type dtA (F64/F32/BF16).
+ * M: device, J x shape[n] column-major, element type dtM (F64/F32).
+ * Y: device, shape with shape[n] replaced by J, element type dtY (F64/F32).
+ * F64 A requires F64 Y and runs in FP64; F32/BF16 A accumulate in FP32. */
+int dt_ttm(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape,
+           const void *M, int dtM, size_t J, int n, void *Y, int dtY);
+
+/* S = A_(n) A_(n)^T (shape[n] x shape[n], column-major, dtS F32 or F64).
+ * Replaces Function::gram(A, n) (template/function/tensor.tpp:18-120) =
+ * tenmat + matmulNT at one rank.  FP32/BF16 inputs accumulate in FP32 on
+ * MFMA; F64 input accumulates in FP64. */
+int dt_gram(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n,
+            void *S, int dtS);
+
+/* C = A_(n) B_(n)^T where A and B agree on every mode except n (C is
+ * shapeA[n] x shapeB[n], FP64).  Replaces Function::ttt_except
+ * (template/function/tensor.tpp:132-259).  A and B are F64 or F32. */
+int dt_ttt_except(dt_ctx *ctx, const void *A, const void *B, int dt, int ndim,
+                  const size_t *shapeA, size_t JB, int n, double *C);
+
+/* sum of squares in FP64 of n elements (host result).  sqrt of it replaces
+ * Operator<double>::fnorm (src/operator/operator_double.cpp:74-82). */
+int dt_sumsq(dt_ctx *ctx, const void *A, int dt, size_t n, double *out_host);
+
+/* ---- FP64 small dense (device pointers) --------------------------------- */
+/* C = alpha op(A) op(B) + beta C, column-major with leading dimensions.
+ * Replaces Operator<double>::matmulNN/NT/TN (operator_double.cpp:157-216),
+ * which call cblas_dgemm with implicit leading dimensions. */
+int dt_gemm_f64(dt_ctx *ctx, int transA, int transB, size_t m, size_t n, size_t k,
+                double alpha, const double *A, size_t lda, const double *B, size_t ldb,
+                double beta, double *C, size_t ldc);
+/* Reduced QR of an m x n (m >= n) matrix: Q m x n with orthonormal columns,
+ * R n x n upper triangular with positive diagonal (R may be NULL).
+ * Replaces Operator<double>::QR (operator_double.cpp:131-155, dgeqrf+dorgqr);
+ * the column signs differ from LAPACK's, the spans are equal. */
+int dt_qr_f64(dt_ctx *ctx, double *Q, double *R, const double *A, size_t m, size_t n);
+/* C = A^{-1}, n x n.  Replaces Operator<double>::inverse (operator_double.cpp:84-103). */
+int dt_inverse_f64(dt_ctx *ctx, double *C, const double *A, size_t n);
+/* Leading r eigenpairs of the symmetric n x n matrix S (descending): V n x r,
+ * w r values (device).  No reference counterpart; used for HOSVD start factors. */
+int dt_syev_top_f64(dt_ctx *ctx, const double *S, size_t n, size_t r, double *V, double *w);
+/* Row orthonormalisation of a len1 x len2 column-major matrix (rows made
+ * orthonormal, positive diagonal of the triangular factor).  Replaces
+ * Operator<double>::orth2 (operator_double.cpp:248-277); orth1 (:218-246) is
+ * dt_qr_f64 with R = NULL. */
+int dt_orth_rows_f64(dt_ctx *ctx, double *A, size_t len1, size_t len2);
+
+/* ---- seeded generators --------------------------------------------------
+ * Counter-based Philox4x32-10; element i of stream `stream` is a function of
+ * (seed, stream, i) only, so host and device generate identical values. */
+int dt_gen_normal(dt_ctx *ctx, void *out, int dt, size_t n, uint64_t seed, uint64_t stream);
+int dt_gen_uniform(dt_ctx *ctx, void *out, int dt, size_t n, uint64_t seed, uint64_t stream);
+
+/* Synthetic dense Tucker model A = G x_1 U_1 ... x_N U_N + E with
+ * ||E|| = noise * ||X||.  G ~ N(0,1) (then scaled by decay^k along every
+ * mode when decay > 0), U_n = Q of the QR of an N(0,1) I_n x R_n matrix.
+ * Writes A in F32 (A_f32) and/or BF16 (A_bf16); either may be NULL.  The true
+ * factors (FP64, I_n x R_n) are written to U_true[n] when non-NULL. */
+typedef struct dt_tucker_spec {
+    int32_t ndim;
+    size_t shape[DT_MAX_NDIM];
+    size_t ranks[DT_MAX_NDIM];
+    double noise;     /* relative Frobenius norm of E */
+    double decay;     /* geometric decay of the core along each mode (0 = none) */
+    uint64_t seed;
+} dt_tucker_spec;
+int dt_gen_tucker(dt_ctx *ctx, const dt_tucker_spec *spec, float *A_f32, uint16_t *A_bf16,
+                  double *const *U_true, double *normX_host, double *normA_host);
+
+/* ---- dense HOOI --------------------------------------------------------- */
+enum {
+    DT_INIT_GIVEN = 0,   /* U_init[n] are orthonormal start factors */
+    DT_INIT_QR = 1,      /* U_init[n] are raw matrices, orthonormalised first
+                            (the reference's randn + reduced_QR, hooi_als.tpp:76-81) */
+    DT_INIT_HOSVD = 2,   /* leading R_n eigenvectors of A_(n) A_(n)^T */
+    DT_INIT_RANDOM = 3,  /* Philox N(0,1) matrices from `seed`, then QR */
+    DT_INIT_LEARNED = 4  /* sketch -> MLP -> QR; falls back to HOSVD (see dt_hooi_result.init_used) */
+};
+enum {
+    DT_SCHED_REFERENCE = 0, /* per mode a full ascending TTM chain, core by a
+                               further chain: N+1 full-tensor passes per sweep */
+    DT_SCHED_TREE = 1       /* two-level dimension tree: 2 full-tensor passes per
+                               sweep, core from the last Y */
+};
+enum {
+    DT_FLAG_DIRECT_RESIDUAL = 1, /* after the last sweep compute ||A - [[G;U]]|| in FP64 */
+    DT_FLAG_FP32_POLISH = 2      /* mixed precision: finish with one FP32 sweep on A_f32 */
+};
+
+typedef struct dt_hooi_args {
+    int32_t ndim;
+    int32_t dtype;              /* element type of A: DT_F32, DT_BF16 or DT_F64 */
+    const void *A;              /* device */
+    const float *A_f32;         /* optional FP32 copy (mixed precision polish / residual) */
+    size_t shape[DT_MAX_NDIM];
+    size_t ranks[DT_MAX_NDIM];
+    int32_t max_iter;           /* sweeps (hooi_als.tpp:91) */
+    int32_t inner_iter;         /* ALS_ inner iterations (hooi_als.tpp:9, default 5) */
+    int32_t init;               /* DT_INIT_* */
+    int32_t schedule;           /* DT_SCHED_* */
+    double tol;                 /* stop early when |res_k - res_{k-1}| < tol; 0 = never */
+    const double *U_init[DT_MAX_NDIM]; /* device FP64 I_n x R_n (GIVEN / QR) */
+    uint64_t seed;              /* DT_INIT_RANDOM and the learned-init sketch */
+    int32_t flags;              /* DT_FLAG_* */
+    const void *model;          /* dt_mlp handle for DT_INIT_LEARNED */
+} dt_hooi_args;
+
+typedef struct dt_hooi_result {
+    double *U[DT_MAX_NDIM];     /* device outputs FP64 I_n x R_n (NULL: not copied out) */
+    double *G;                  /* device output core FP64, prod R_n (NULL: not copied) */
+    double *residual_per_sweep; /* host array of max_iter entries (may be NULL) */
+    int32_t sweeps;             /* sweeps run */
+    int32_t init_used;          /* DT_INIT_* that produced the start factors */
+    double A_norm;              /* ||A||_F */
+    double G_norm;              /* ||G||_F after the last sweep */
+    double residual;            /* sqrt(1 - ||G||^2/||A||^2) after the last sweep */
+    double residual_direct;     /* ||A - G x_1 U_1 ... ||/||A|| (FP64), if requested */
+    double init_ms;             /* device time of the start-factor computation */
+    double sweeps_ms;           /* device time of all sweeps */
+    double learned_first_res;   /* learned init: residual after its first sweep */
+    double hosvd_first_res;     /* learned init: residual of the HOSVD comparison sweep */
+} dt_hooi_result;
+
+/* Algorithm::Tucker::HOOI_ALS (include/algorithm.hpp:12-14,
+ * template/algorithm/tucker/hooi_als.tpp:64-123) on one GPU: create, max_iter
+ * sweeps (early stop on tol), copy out, destroy. */
+int dt_hooi_dense(dt_ctx *ctx, const dt_hooi_args *args, dt_hooi_result *res);
+
+/* The same loop one sweep at a time (the reference's `for iter < max_iter`
+ * body, hooi_als.tpp:91-115), for callers that time or steer sweeps.
+ * dt_hooi_create computes ||A|| and the start factors (hooi_als.tpp:76-87);
+ * dt_hooi_sweep runs one sweep (on the FP32 copy when fp32 != 0 and A is BF16)
+ * and returns sqrt(1 - ||G||^2/||A||^2); dt_hooi_get copies U and G out (when
+ * no sweep ran it forms G from the start factors). */
+typedef struct dt_hooi dt_hooi;
+int dt_hooi_create(dt_ctx *ctx, const dt_hooi_args *args, dt_hooi **out);
+int dt_hooi_sweep(dt_hooi *h, int fp32, double *residual);
+int dt_hooi_get(dt_hooi *h, dt_hooi_result *res);
+int dt_hooi_destroy(dt_hooi *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIANA_HIP_H */
