@@ -240,6 +240,26 @@ static int sumsq_to_dev(dt_ctx *ctx, const void *A, int dt, size_t n, double *ou
     return DT_OK;
 }
 
+__global__ __launch_bounds__(256) void dot_f64_kernel(const double *__restrict__ a,
+                                                      const double *__restrict__ b, int64_t n,
+                                                      double *__restrict__ part) {
+    double s = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        s = fma(a[i], b[i], s);
+    s = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+int dt_dot_f64_dev(dt_ctx *ctx, const double *a, const double *b, size_t n, double *out_dev) {
+    const int nb = 256;
+    double *part = (double *)dt_ws(ctx, WS_SUMSQ, 2048 * sizeof(double));
+    if (!part) return DT_ENOMEM;
+    DT_LAUNCH("dot", dot_f64_kernel, dim3(nb), dim3(256), 0, a, b, (int64_t)n, part);
+    DT_LAUNCH("dot", sum_partials, dim3(1), dim3(256), 0, part, nb, out_dev);
+    return DT_OK;
+}
+
 int dt_sumsq_dev(dt_ctx *ctx, const void *A, int dt, size_t n, double *out_host) {
     if (n == 0) {
         *out_host = 0.0;
@@ -272,9 +292,12 @@ int dt_gram_internal(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t I, 
     DT_CHECK_ARG(dtA == DT_F32 || dtA == DT_BF16, "gram: unsupported type %d", dtA);
     const int nt = (int)ceil_div(I, 64);
     const int ntiles = nt * (nt + 1) / 2;
-    int64_t nsplit = 2048 / ntiles;
+    // ~512 workgroups in total, at least 256 columns per split so the FP32
+    // partial tiles stay small next to the data read.
+    int64_t nsplit = 512 / ntiles;
     if (nsplit < 1) nsplit = 1;
-    int64_t maxsplit = ceil_div(Ctot, 32);
+    int64_t maxsplit = ceil_div(Ctot, 256);
+    if (maxsplit < 1) maxsplit = 1;
     if (nsplit > maxsplit) nsplit = maxsplit;
     if (nsplit > 256) nsplit = 256;
     int64_t cps = ceil_div(ceil_div(Ctot, nsplit), 32) * 32;

@@ -47,6 +47,14 @@ struct dt_hooi {
     double *S = nullptr;   // In x In
     double *L = nullptr, *P = nullptr, *W = nullptr;
     double *Gm = nullptr, *Ginv = nullptr, *GR = nullptr, *GRinv = nullptr;
+    double *work = nullptr;     // small FP64 scratch (R x R systems, QR passes)
+    double *G64 = nullptr, *T64a = nullptr, *T64b = nullptr;  // core in FP64 + scratch
+    double *Cfix[DT_MAX_NDIM] = {};  // (X_nᵀ X_n)^{-1} of the factor actually used
+    double *Xfix = nullptr;      // FP64 copy of one kernel factor
+    double *dscal = nullptr;     // device scalars: [0] = ||Ahat||^2
+    int *dflag = nullptr;        // device breakdown flags
+    void *fbuf[2] = {};          // FP64 full-pass buffers (DT_FLAG_FP64_FIT)
+    int bf16_core_mode = -1;     // mode whose core-chain product used the bf16 factor
     std::vector<void *> owned;
     int sweeps = 0;
     double last_res = -1.0;
@@ -83,7 +91,8 @@ static int contract(dt_hooi *h, const void *src, int src_dt, int64_t *sh, int n,
     for (int i = 0; i < n; i++) F *= sh[i];
     for (int i = n + 1; i < h->N; i++) B *= sh[i];
     dt_factor_views fv{h->U32[n], h->M32[n], h->Ub16[n], h->M64[n]};
-    DT_TRY(dt_ttm_views(ctx, src, src_dt, F, sh[n], B, fv, h->R[n], dst, h->work_dt));
+    const bool full = src == h->args.A || src == (const void *)h->args.A_f32;
+    DT_TRY(dt_ttm_views(ctx, src, src_dt, F, sh[n], B, fv, h->R[n], dst, h->work_dt, full));
     sh[n] = h->R[n];
     return DT_OK;
 }
@@ -107,37 +116,122 @@ static int chain(dt_hooi *h, const void *src, int src_dt, int64_t *sh, const std
 }
 
 // ALS_ on Y (shape sh, mode n): U64[n] <- Q of QR(L after inner_iter steps).
+// Small problems (I R^2 <= 2^20) run the R x R steps fused in one workgroup;
+// larger ones use multi-workgroup GEMMs for the I x R products.  No step
+// synchronises with the host; breakdowns set bits in h->dflag.
 static int als_update(dt_hooi *h, const void *Y, int ydt, const int64_t *sh, int n) {
     dt_ctx *ctx = h->ctx;
     int64_t F = 1, B = 1;
     for (int i = 0; i < n; i++) F *= sh[i];
     for (int i = n + 1; i < h->N; i++) B *= sh[i];
     const size_t In = (size_t)sh[n], Rn = (size_t)h->R[n];
+    const bool small = (double)In * Rn * Rn <= (double)(1 << 20);
     DT_TRY(dt_gram_internal(ctx, Y, ydt, F, (int64_t)In, B, h->S));
     DT_TRY(dt_copy_f64(ctx, h->L, h->U64[n], In * Rn));
     for (int it = 0; it < h->args.inner_iter; it++) {
         const double *Pp = h->L;
         if (it > 0) {
-            DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, Rn, Rn, In, 1.0, h->L, In, h->L, In, 0.0, h->Gm, Rn));
-            DT_TRY(dt_inverse_f64(ctx, h->Ginv, h->Gm, Rn));
-            DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, In, Rn, Rn, 1.0, h->L, In, h->Ginv, Rn, 0.0, h->P, In));
+            if (small) {
+                DT_TRY(dt_als_prep(ctx, h->L, (int)In, (int)Rn, h->P, h->work, h->dflag));
+            } else {
+                DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, Rn, Rn, In, 1.0, h->L, In, h->L, In, 0.0, h->Gm, Rn));
+                DT_TRY(dt_inverse_flag(ctx, h->Gm, (int)Rn, h->Ginv, h->work, h->dflag, 1));
+                DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, In, Rn, Rn, 1.0, h->L, In, h->Ginv, Rn, 0.0, h->P, In));
+            }
             Pp = h->P;
         }
         DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, In, Rn, In, 1.0, h->S, In, Pp, In, 0.0, h->W, In));
-        DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, Rn, Rn, In, 1.0, Pp, In, h->W, In, 0.0, h->GR, Rn));
-        DT_TRY(dt_inverse_f64(ctx, h->GRinv, h->GR, Rn));
-        DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, In, Rn, Rn, 1.0, h->W, In, h->GRinv, Rn, 0.0, h->L, In));
+        if (small) {
+            DT_TRY(dt_als_finish(ctx, Pp, h->W, (int)In, (int)Rn, h->L, h->work, h->dflag));
+        } else {
+            DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, Rn, Rn, In, 1.0, Pp, In, h->W, In, 0.0, h->GR, Rn));
+            DT_TRY(dt_inverse_flag(ctx, h->GR, (int)Rn, h->GRinv, h->work, h->dflag, 2));
+            DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, In, Rn, Rn, 1.0, h->W, In, h->GRinv, Rn, 0.0, h->L, In));
+        }
     }
-    DT_TRY(dt_qr_f64(ctx, h->U64[n], nullptr, h->L, In, Rn));
+    DT_TRY(dt_copy_f64(ctx, h->U64[n], h->L, In * Rn));
+    DT_TRY(dt_orth_dev(ctx, h->U64[n], In, Rn, h->work, h->dflag));
     return update_views(h, n);
 }
 
-static int core_residual(dt_hooi *h, double *res) {
+// ||Ahat||^2 for the factors the kernels actually used.  With FP32 (or bf16)
+// factor copies X_n, G = A x_n X_nᵀ and the projection of A onto
+// span(X_1 (x) ... (x) X_N) has squared norm <G, G x_1 C_1 ... x_N C_N> with
+// C_n = (X_nᵀ X_n)^{-1}; this removes the O(1e-8) bias that the rounding of
+// the factors to FP32 puts into 1 - ||G||^2/||A||^2.  With DT_FLAG_FP64_FIT the
+// core is instead re-formed from A with FP64 accumulation and FP64 factors.
+static int ahat_sq_dev(dt_hooi *h, const void *A, int dt) {
     dt_ctx *ctx = h->ctx;
+    const int N = h->N;
     int64_t csz = 1;
-    for (int n = 0; n < h->N; n++) csz *= h->R[n];
-    double g2 = 0.0;
-    DT_TRY(dt_sumsq_dev(ctx, h->Gbuf, h->work_dt, (size_t)csz, &g2));
+    for (int n = 0; n < N; n++) csz *= h->R[n];
+    if (h->args.flags & DT_FLAG_FP64_FIT) {
+        // FP64 chain from A: highest mode first (first step FP64-accumulating)
+        int64_t sh[DT_MAX_NDIM];
+        for (int i = 0; i < N; i++) sh[i] = h->I[i];
+        const void *cur = A;
+        int cdt = dt;
+        int which = 0;
+        for (int m = N - 1; m >= 0; m--) {
+            int64_t F = 1, B = 1;
+            for (int i = 0; i < m; i++) F *= sh[i];
+            for (int i = m + 1; i < N; i++) B *= sh[i];
+            double *dst = m == 0 ? h->G64 : (double *)h->fbuf[which];
+            DT_TRY(dt_f64_to_views(ctx, h->U64[m], (size_t)h->I[m], (size_t)h->R[m], nullptr, nullptr,
+                                   nullptr, h->Xfix));  // Xfix = M64 (R x I)
+            DT_TRY(dt_ttm_f64acc(ctx, cur, cdt, F, sh[m], B, h->Xfix, h->R[m], dst));
+            sh[m] = h->R[m];
+            cur = dst;
+            cdt = DT_F64;
+            which ^= 1;
+        }
+        return dt_dot_f64_dev(ctx, h->G64, h->G64, (size_t)csz, h->dscal);
+    }
+    DT_TRY(dt_convert(ctx, h->G64, DT_F64, h->Gbuf, h->work_dt, (size_t)csz));
+    if (h->work_dt == DT_F64) return dt_dot_f64_dev(ctx, h->G64, h->G64, (size_t)csz, h->dscal);
+    // T = G x_n C_n for all n (generic FP64 mode products on the small core)
+    int64_t sh[DT_MAX_NDIM];
+    for (int i = 0; i < N; i++) sh[i] = h->R[i];
+    const double *cur = h->G64;
+    for (int n = 0; n < N; n++) {
+        const size_t In = (size_t)h->I[n], Rn = (size_t)h->R[n];
+        if (n == h->bf16_core_mode && h->Ub16[n])
+            DT_TRY(dt_convert(ctx, h->Xfix, DT_F64, h->Ub16[n], DT_BF16, In * Rn));
+        else
+            DT_TRY(dt_convert(ctx, h->Xfix, DT_F64, h->U32[n], DT_F32, In * Rn));
+        if ((double)In * Rn * Rn <= (double)(1 << 20)) {
+            DT_TRY(dt_gram_inverse_dev(ctx, h->Xfix, (int)In, (int)Rn, h->Cfix[n], h->work, h->dflag));
+        } else {
+            DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, Rn, Rn, In, 1.0, h->Xfix, In, h->Xfix, In, 0.0, h->Gm, Rn));
+            DT_TRY(dt_inverse_flag(ctx, h->Gm, (int)Rn, h->Cfix[n], h->work, h->dflag, 8));
+        }
+        int64_t F = 1, B = 1;
+        for (int i = 0; i < n; i++) F *= sh[i];
+        for (int i = n + 1; i < N; i++) B *= sh[i];
+        double *dst = (n % 2 == 0) ? h->T64a : h->T64b;
+        dt_factor_views fv{nullptr, nullptr, nullptr, h->Cfix[n]};
+        DT_TRY(dt_ttm_views(ctx, cur, DT_F64, F, (int64_t)Rn, B, fv, (int64_t)Rn, dst, DT_F64));
+        cur = dst;
+    }
+    return dt_dot_f64_dev(ctx, cur, h->G64, (size_t)csz, h->dscal);
+}
+
+static int core_residual(dt_hooi *h, const void *A, int dt, double *res) {
+    dt_ctx *ctx = h->ctx;
+    DT_TRY(ahat_sq_dev(h, A, dt));
+    // one read-back per sweep: ||Ahat||^2 and the breakdown flags
+    DT_HIP(hipMemcpyAsync(ctx->pinned, h->dscal, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    DT_HIP(hipMemcpyAsync((int *)(ctx->pinned + 1), h->dflag, sizeof(int), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    DT_HIP(hipStreamSynchronize(ctx->stream));
+    const double g2 = ctx->pinned[0];
+    const int flag = ((int *)(ctx->pinned + 1))[0];
+    if (flag) {
+        return dt_fail(ctx, DT_ENUMERIC, "hooi: numerical breakdown (flags 0x%x: %s%s%s%s)", flag,
+                       (flag & 1) ? "singular L^T L; " : "", (flag & 2) ? "singular P^T S P; " : "",
+                       (flag & 4) ? "rank-deficient factor in QR; " : "",
+                       (flag & 8) ? "singular factor Gram; " : "");
+    }
     h->G_norm = std::sqrt(g2);
     const double ratio = h->normA2 > 0.0 ? g2 / h->normA2 : 1.0;
     *res = std::sqrt(std::max(0.0, 1.0 - ratio));
@@ -150,8 +244,10 @@ static int sweep_once(dt_hooi *h, const void *A, int dt) {
     int64_t sh[DT_MAX_NDIM];
     const void *Y;
     int ydt;
+    h->bf16_core_mode = -1;
     if (h->args.schedule == DT_SCHED_TREE && N >= 2) {
         const int hm = N / 2;
+        if (dt == DT_BF16) h->bf16_core_mode = hm - 1;  // first product of the T_R chain
         std::vector<int> left, right;
         for (int i = 0; i < hm; i++) left.push_back(i);
         for (int i = hm; i < N; i++) right.push_back(i);
@@ -194,6 +290,7 @@ static int sweep_once(dt_hooi *h, const void *A, int dt) {
         return DT_OK;
     }
     // reference schedule
+    if (dt == DT_BF16) h->bf16_core_mode = 0;
     for (int n = 0; n < N; n++) {
         for (int i = 0; i < N; i++) sh[i] = h->I[i];
         std::vector<int> modes;
@@ -213,6 +310,7 @@ static int sweep_once(dt_hooi *h, const void *A, int dt) {
 
 // Core from the current factors (reference chain), for max_iter == 0.
 static int core_only(dt_hooi *h) {
+    h->bf16_core_mode = h->args.dtype == DT_BF16 ? 0 : -1;
     int64_t sh[DT_MAX_NDIM];
     for (int i = 0; i < h->N; i++) sh[i] = h->I[i];
     std::vector<int> all;
@@ -309,6 +407,7 @@ int dt_hooi_create(dt_ctx *ctx, const dt_hooi_args *args, dt_hooi **out) {
     for (int b = 0; b < 4 && st == DT_OK; b++) st = hooi_alloc(h, &h->buf[b], maxe * wsz);
     int64_t csz = 1;
     for (int n = 0; n < N; n++) csz *= h->R[n];
+    csz = std::max<int64_t>(csz, 64);
     if (st == DT_OK) st = hooi_alloc(h, &h->Gbuf, (size_t)csz * wsz);
     if (st == DT_OK) st = hooi_alloc(h, (void **)&h->S, (size_t)(maxI * maxI) * 8);
     const size_t ir = (size_t)(maxI * maxR) * 8, rr = (size_t)(maxR * maxR) * 8;
@@ -319,6 +418,18 @@ int dt_hooi_create(dt_ctx *ctx, const dt_hooi_args *args, dt_hooi **out) {
     if (st == DT_OK) st = hooi_alloc(h, (void **)&h->Ginv, rr);
     if (st == DT_OK) st = hooi_alloc(h, (void **)&h->GR, rr);
     if (st == DT_OK) st = hooi_alloc(h, (void **)&h->GRinv, rr);
+    if (st == DT_OK) st = hooi_alloc(h, (void **)&h->work, std::max(4 * rr, 2 * rr + ir) + 1024);
+    if (st == DT_OK) st = hooi_alloc(h, (void **)&h->G64, (size_t)csz * 8);
+    if (st == DT_OK) st = hooi_alloc(h, (void **)&h->T64a, (size_t)csz * 8);
+    if (st == DT_OK) st = hooi_alloc(h, (void **)&h->T64b, (size_t)csz * 8);
+    if (st == DT_OK) st = hooi_alloc(h, (void **)&h->Xfix, ir);
+    if (st == DT_OK) st = hooi_alloc(h, (void **)&h->dscal, 256);
+    if (st == DT_OK) st = hooi_alloc(h, (void **)&h->dflag, 256);
+    for (int n = 0; n < N && st == DT_OK; n++) st = hooi_alloc(h, (void **)&h->Cfix[n], rr);
+    if (st == DT_OK && (args->flags & DT_FLAG_FP64_FIT)) {
+        st = hooi_alloc(h, &h->fbuf[0], maxe * 8);
+        if (st == DT_OK) st = hooi_alloc(h, &h->fbuf[1], maxe * 8);
+    }
     for (int n = 0; n < N && st == DT_OK; n++) {
         const size_t e = (size_t)(h->I[n] * h->R[n]);
         st = hooi_alloc(h, (void **)&h->U64[n], e * 8);
@@ -367,9 +478,12 @@ int dt_hooi_sweep(dt_hooi *h, int fp32, double *residual) {
     }
     hipEvent_t e0 = dt_get_event(ctx), e1 = dt_get_event(ctx);
     hipEventRecord(e0, ctx->stream);
-    int st = sweep_once(h, A, dt);
+    int st = dt_zero(ctx, h->dflag, sizeof(int));
+    if (st == DT_OK) st = sweep_once(h, A, dt);
     double res = 0.0;
-    if (st == DT_OK) st = core_residual(h, &res);
+    const void *fitA = h->args.A_f32 ? (const void *)h->args.A_f32 : A;
+    const int fitdt = h->args.A_f32 ? DT_F32 : dt;
+    if (st == DT_OK) st = core_residual(h, fitA, fitdt, &res);
     hipEventRecord(e1, ctx->stream);
     hipEventSynchronize(e1);
     float ms = 0.f;
@@ -388,9 +502,11 @@ int dt_hooi_sweep(dt_hooi *h, int fp32, double *residual) {
 int dt_hooi_get(dt_hooi *h, dt_hooi_result *res) {
     dt_ctx *ctx = h->ctx;
     if (h->sweeps == 0) {
+        DT_TRY(dt_zero(ctx, h->dflag, sizeof(int)));
         DT_TRY(core_only(h));
         double r = 0.0;
-        DT_TRY(core_residual(h, &r));
+        const void *fitA = h->args.A_f32 ? (const void *)h->args.A_f32 : h->args.A;
+        DT_TRY(core_residual(h, fitA, h->args.A_f32 ? DT_F32 : h->args.dtype, &r));
         h->last_res = r;
     }
     int64_t csz = 1;

@@ -159,8 +159,10 @@ struct dt_factor_views {
     const uint16_t *Ub16; // K x J (only for BF16 A)
     const double *M64;    // J x K (only for F64 A)
 };
+// stream = true: A is read exactly once (the full input tensor); its loads are
+// non-temporal and the launch is timed under the "<kernel>_stream" name.
 int dt_ttm_views(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t K, int64_t B,
-                 const dt_factor_views &fv, int64_t J, void *Y, int dtY);
+                 const dt_factor_views &fv, int64_t J, void *Y, int dtY, bool stream = false);
 int dt_gram_internal(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t I, int64_t B,
                      double *S);
 int dt_sumsq_dev(dt_ctx *ctx, const void *A, int dt, size_t n, double *out_host);
@@ -174,3 +176,14 @@ int dt_gemm_f64_internal(dt_ctx *ctx, int tA, int tB, size_t m, size_t n, size_t
 int dt_jacobi_eig(dt_ctx *ctx, double *H, size_t n, double *V, double *w);
 int dt_copy_f64(dt_ctx *ctx, double *dst, const double *src, size_t n);
 int dt_fill_normal_f64(dt_ctx *ctx, double *out, size_t n, uint64_t seed, uint64_t stream);
+// fused single-workgroup FP64 steps (dt_small.hip); failures OR bits into *flag
+int dt_als_prep(dt_ctx *ctx, const double *L, int I, int R, double *P, double *work, int *flag);
+int dt_als_finish(dt_ctx *ctx, const double *P, const double *W, int I, int R, double *L,
+                  double *work, int *flag);
+int dt_cholqr3_dev(dt_ctx *ctx, double *X, int m, int n, double *work, int *flag);
+int dt_gram_inverse_dev(dt_ctx *ctx, const double *X, int m, int n, double *C, double *work, int *flag);
+int dt_inverse_flag(dt_ctx *ctx, const double *G, int n, double *C, double *work, int *flag, int bit);
+int dt_orth_dev(dt_ctx *ctx, double *X, size_t m, size_t n, double *work, int *flag);
+int dt_ttm_f64acc(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t K, int64_t B,
+                  const double *M64, int64_t J, double *Y);
+int dt_dot_f64_dev(dt_ctx *ctx, const double *a, const double *b, size_t n, double *out_dev);

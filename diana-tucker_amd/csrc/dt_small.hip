@@ -1,0 +1,392 @@
+// Fused single-workgroup FP64 kernels for the per-mode factor update.
+//
+// The reference's ALS_ (template/algorithm/tucker/hooi_als.tpp:7-34) is a
+// sequence of small BLAS/LAPACK calls (matmulTN, inverse = dgetrf+dgetri,
+// matmulNN, reduced_QR = dgeqrf+dorgqr).  Launched one by one with a host
+// check after every LAPACK-style call they cost more than the full-tensor
+// passes, so here each group of small steps is one launch of one workgroup,
+// and failures (singular R x R systems, non-positive Cholesky pivots) raise a
+// device flag that the host reads once per sweep:
+//   als_prep_kernel   : Gm = LᵀL, Gm^{-1} (in-place Gauss-Jordan, partial
+//                       pivoting), P = L Gm^{-1}                (hooi_als.tpp:21-23)
+//   als_finish_kernel : GR = Pᵀ W, GR^{-1}, L = W GR^{-1}        (hooi_als.tpp:28-30)
+//   cholqr3_kernel    : Q = orth(L) by a shifted Cholesky-QR pass followed by
+//                       two plain passes (Fukaya, Kannan, Nakatsukasa, Yamamoto,
+//                       Yanagisawa, SISC 2020); positive R diagonal  (hooi_als.tpp:32)
+// The I x R panels stream through LDS in row chunks with coalesced loads;
+// the R x R matrix lives in LDS (R <= 64).  The W = S P product stays a
+// multi-workgroup GEMM.
+#include "dt_internal.hpp"
+
+#include <cmath>
+
+#define SMALL_THREADS 512
+constexpr int kMaxR = 64;
+
+// Out (R x R, LDS) = Xᵀ Y for X, Y (I x R, column-major, global).  The rows
+// are staged CH at a time through LDS (stage: 2 * (CH+1) * R doubles).
+__device__ void wg_xty(const double *__restrict__ X, const double *__restrict__ Y, int I, int R,
+                       double *__restrict__ Out, double *__restrict__ stage, int CH) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int ld = CH + 1;  // odd stride in doubles: column reads hit distinct banks
+    double *Xs = stage, *Ys = stage + ld * R;
+    // each thread owns up to 8 outputs
+    double acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) acc[q] = 0.0;
+    const int nout = R * R;
+    for (int t0 = 0; t0 < I; t0 += CH) {
+        const int ch = (I - t0) < CH ? (I - t0) : CH;
+        for (int e = tid; e < CH * R; e += nt) {
+            const int r = e % CH, c = e / CH;
+            Xs[r + ld * c] = r < ch ? X[(t0 + r) + (size_t)I * c] : 0.0;
+            Ys[r + ld * c] = r < ch ? Y[(t0 + r) + (size_t)I * c] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int o = tid + q * nt;
+            if (o < nout) {
+                const int i = o % R, j = o / R;
+                const double *xa = Xs + ld * i, *yb = Ys + ld * j;
+                double s = acc[q];
+                for (int r = 0; r < CH; r++) s = fma(xa[r], yb[r], s);
+                acc[q] = s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int o = tid + q * nt;
+        if (o < nout) Out[o] = acc[q];
+    }
+    __syncthreads();
+}
+
+// Out (I x R, global) = X (I x R, global) M (R x R, LDS).  Rows of X are
+// staged CH at a time through LDS before any output of the chunk is written,
+// so Out may alias X.
+__device__ void wg_xm(const double *X, const double *__restrict__ M, int I, int R, double *Out,
+                      double *__restrict__ stage, int CH) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int ld = CH + 1;
+    for (int t0 = 0; t0 < I; t0 += CH) {
+        const int ch = (I - t0) < CH ? (I - t0) : CH;
+        for (int e = tid; e < CH * R; e += nt) {
+            const int r = e % CH, c = e / CH;
+            stage[r + ld * c] = r < ch ? X[(t0 + r) + (size_t)I * c] : 0.0;
+        }
+        __syncthreads();
+        for (int e = tid; e < CH * R; e += nt) {
+            const int r = e % CH, j = e / CH;
+            if (r < ch) {
+                double s = 0.0;
+                for (int k = 0; k < R; k++) s = fma(stage[r + ld * k], M[k + R * j], s);
+                Out[(t0 + r) + (size_t)I * j] = s;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// In-place Gauss-Jordan inverse with partial pivoting of the n x n LDS matrix A.
+// Returns nonzero on a zero pivot (uniform across the workgroup).
+__device__ int wg_inverse_inplace(double *__restrict__ A, int n, int *__restrict__ perm) {
+    __shared__ int piv_s, bad_s;
+    __shared__ double pinv_s;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (tid == 0) bad_s = 0;
+    __syncthreads();
+    for (int k = 0; k < n; k++) {
+        if (tid == 0) {
+            int p = k;
+            double best = fabs(A[k + n * k]);
+            for (int i = k + 1; i < n; i++) {
+                const double v = fabs(A[i + n * k]);
+                if (v > best) {
+                    best = v;
+                    p = i;
+                }
+            }
+            piv_s = p;
+            perm[k] = p;
+            if (!(best > 0.0) || !isfinite(best)) bad_s = k + 1;
+        }
+        __syncthreads();
+        if (bad_s) return bad_s;
+        const int p = piv_s;
+        if (p != k)
+            for (int j = tid; j < n; j += nt) {
+                const double t = A[k + n * j];
+                A[k + n * j] = A[p + n * j];
+                A[p + n * j] = t;
+            }
+        __syncthreads();
+        if (tid == 0) {
+            pinv_s = 1.0 / A[k + n * k];
+            A[k + n * k] = 1.0;
+        }
+        __syncthreads();
+        const double pinv = pinv_s;
+        for (int j = tid; j < n; j += nt) A[k + n * j] *= pinv;
+        __syncthreads();
+        // eliminate column k from the other rows (column k holds the multipliers)
+        for (int idx = tid; idx < n * n; idx += nt) {
+            const int i = idx % n, j = idx / n;
+            if (i == k || j == k) continue;
+            A[i + n * j] -= A[i + n * k] * A[k + n * j];
+        }
+        __syncthreads();
+        for (int i = tid; i < n; i += nt)
+            if (i != k) A[i + n * k] = -A[i + n * k] * A[k + n * k];
+        __syncthreads();
+    }
+    // undo the row interchanges as column interchanges, last first
+    for (int k = n - 1; k >= 0; k--) {
+        const int p = perm[k];
+        if (p != k)
+            for (int i = tid; i < n; i += nt) {
+                const double t = A[i + n * k];
+                A[i + n * k] = A[i + n * p];
+                A[i + n * p] = t;
+            }
+        __syncthreads();
+    }
+    return 0;
+}
+
+// G (n x n SPD, LDS) + shift I -> upper Cholesky factor R (in place in G);
+// Xo (n x n, LDS) <- R^{-1} (upper).  Returns nonzero on a non-positive pivot.
+__device__ int wg_chol_rinv(double *__restrict__ G, int n, double shift, double *__restrict__ Xo) {
+    __shared__ int bad_s;
+    __shared__ double rjj_s;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int i = tid; i < n; i += nt) G[i + n * i] += shift;
+    if (tid == 0) bad_s = 0;
+    __syncthreads();
+    for (int j = 0; j < n; j++) {
+        if (tid == 0) {
+            const double d = G[j + n * j];
+            if (!(d > 0.0) || !isfinite(d)) {
+                bad_s = j + 1;
+                rjj_s = 1.0;
+            } else {
+                rjj_s = sqrt(d);
+                G[j + n * j] = rjj_s;
+            }
+        }
+        __syncthreads();
+        if (bad_s) return bad_s;
+        const double rjj = rjj_s;
+        for (int k = j + 1 + tid; k < n; k += nt) G[j + n * k] /= rjj;
+        __syncthreads();
+        const int mm = n - j - 1;
+        for (int idx = tid; idx < mm * mm; idx += nt) {
+            const int i = j + 1 + idx % mm, k = j + 1 + idx / mm;
+            if (i <= k) G[i + n * k] -= G[j + n * i] * G[j + n * k];
+        }
+        __syncthreads();
+    }
+    // Xo = R^{-1}: column j solved by thread j (back substitution), written
+    // straight into Xo, which no thread reads during the solve.
+    for (int idx = tid; idx < n * n; idx += nt) Xo[idx] = 0.0;
+    __syncthreads();
+    const int j = tid;
+    if (j < n) {
+        for (int i = j; i >= 0; i--) {
+            double s = (i == j) ? 1.0 : 0.0;
+            for (int k = i + 1; k <= j; k++) s -= G[i + n * k] * Xo[k + n * j];
+            Xo[i + n * j] = s / G[i + n * i];
+        }
+    }
+    __syncthreads();
+    return 0;
+}
+
+__device__ inline double *lds_base() {
+    extern __shared__ double dyn_lds[];
+    return dyn_lds;
+}
+
+__device__ __host__ inline int stage_rows(int R) { return R <= 16 ? 64 : (R <= 32 ? 32 : 16); }
+
+// LDS layout: M (R*R) | stage (2*(CH+1)*R); perm in static shared.
+__global__ __launch_bounds__(SMALL_THREADS) void als_prep_kernel(const double *__restrict__ L, int I,
+                                                                 int R, double *__restrict__ P,
+                                                                 int *__restrict__ flag) {
+    __shared__ int perm[kMaxR];
+    double *M = lds_base(), *stage = M + R * R;
+    wg_xty(L, L, I, R, M, stage, stage_rows(R));
+    if (wg_inverse_inplace(M, R, perm)) {
+        if (threadIdx.x == 0) atomicOr(flag, 1);
+        return;
+    }
+    wg_xm(L, M, I, R, P, stage, stage_rows(R));
+}
+
+__global__ __launch_bounds__(SMALL_THREADS) void als_finish_kernel(const double *__restrict__ P,
+                                                                   const double *__restrict__ Wm,
+                                                                   int I, int R,
+                                                                   double *__restrict__ L,
+                                                                   int *__restrict__ flag) {
+    __shared__ int perm[kMaxR];
+    double *M = lds_base(), *stage = M + R * R;
+    wg_xty(P, Wm, I, R, M, stage, stage_rows(R));
+    if (wg_inverse_inplace(M, R, perm)) {
+        if (threadIdx.x == 0) atomicOr(flag, 2);
+        return;
+    }
+    wg_xm(Wm, M, I, R, L, stage, stage_rows(R));
+}
+
+// X (m x n) <- orth(X): shifted Cholesky-QR pass + two plain passes.
+// LDS: G (n*n) | R^{-1} (n*n) | stage (2*(CH+1)*n).
+__global__ __launch_bounds__(SMALL_THREADS) void cholqr3_kernel(double *X, int m, int n,
+                                                                int *__restrict__ flag) {
+    __shared__ double shift_s;
+    double *G = lds_base(), *Ri = G + n * n, *stage = Ri + n * n;
+    for (int pass = 0; pass < 3; pass++) {
+        wg_xty(X, X, m, n, G, stage, stage_rows(n));
+        if (threadIdx.x == 0) {
+            double shift = 0.0;
+            if (pass == 0) {
+                double tr = 0.0;
+                for (int i = 0; i < n; i++) tr += G[i + n * i];
+                shift = 11.0 * ((double)m * n + (double)n * (n + 1)) * 1.1102230246251565e-16 * tr;
+            }
+            shift_s = shift;
+        }
+        __syncthreads();
+        if (wg_chol_rinv(G, n, shift_s, Ri)) {
+            if (threadIdx.x == 0) atomicOr(flag, 4);
+            return;
+        }
+        wg_xm(X, Ri, m, n, X, stage, stage_rows(n));
+    }
+}
+
+// C (n x n) = (XᵀX)^{-1}; X m x n.
+__global__ __launch_bounds__(SMALL_THREADS) void gram_inverse_kernel(const double *__restrict__ X,
+                                                                     int m, int n,
+                                                                     double *__restrict__ C,
+                                                                     int *__restrict__ flag) {
+    __shared__ int perm[kMaxR];
+    double *G = lds_base(), *stage = G + n * n;
+    wg_xty(X, X, m, n, G, stage, stage_rows(n));
+    if (wg_inverse_inplace(G, n, perm)) {
+        if (threadIdx.x == 0) atomicOr(flag, 8);
+        return;
+    }
+    for (int i = threadIdx.x; i < n * n; i += blockDim.x) C[i] = G[i];
+}
+
+// inverse of G (n x n, global) into C (global)
+__global__ __launch_bounds__(SMALL_THREADS) void inverse_flag_kernel(const double *__restrict__ G,
+                                                                     int n, double *__restrict__ C,
+                                                                     int *__restrict__ flag,
+                                                                     int bit) {
+    __shared__ int perm[kMaxR];
+    double *M = lds_base();
+    for (int i = threadIdx.x; i < n * n; i += blockDim.x) M[i] = G[i];
+    __syncthreads();
+    if (wg_inverse_inplace(M, n, perm)) {
+        if (threadIdx.x == 0) atomicOr(flag, bit);
+        return;
+    }
+    for (int i = threadIdx.x; i < n * n; i += blockDim.x) C[i] = M[i];
+}
+
+// G (n x n SPD, global) -> Rinv (upper inverse Cholesky factor), optional shift.
+// LDS: M (n*n) | Xo (n*n).
+__global__ __launch_bounds__(SMALL_THREADS) void chol_rinv_kernel(const double *__restrict__ G,
+                                                                  int n, int shifted,
+                                                                  double m_rows,
+                                                                  double *__restrict__ Rinv,
+                                                                  int *__restrict__ flag) {
+    __shared__ double shift_s;
+    double *M = lds_base(), *Xo = M + n * n;
+    for (int i = threadIdx.x; i < n * n; i += blockDim.x) M[i] = G[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double shift = 0.0;
+        if (shifted) {
+            double tr = 0.0;
+            for (int i = 0; i < n; i++) tr += M[i + n * i];
+            shift = 11.0 * (m_rows * n + (double)n * (n + 1)) * 1.1102230246251565e-16 * tr;
+        }
+        shift_s = shift;
+    }
+    __syncthreads();
+    if (wg_chol_rinv(M, n, shift_s, Xo)) {
+        if (threadIdx.x == 0) atomicOr(flag, 4);
+        return;
+    }
+    for (int i = threadIdx.x; i < n * n; i += blockDim.x) Rinv[i] = Xo[i];
+}
+
+// ---- host wrappers -----------------------------------------------------------
+static inline size_t lds_bytes(int R) {
+    const int ch = stage_rows(R);
+    return (size_t)(R * R + 2 * (ch + 1) * R) * sizeof(double);
+}
+
+static int check_r(dt_ctx *ctx, int R) {
+    if (R < 1 || R > kMaxR) return dt_fail(ctx, DT_EUNSUPPORTED, "rank %d above the supported %d", R, kMaxR);
+    return DT_OK;
+}
+
+int dt_als_prep(dt_ctx *ctx, const double *L, int I, int R, double *P, double *work, int *flag) {
+    (void)work;
+    DT_TRY(check_r(ctx, R));
+    DT_LAUNCH("als_small", als_prep_kernel, dim3(1), dim3(SMALL_THREADS), lds_bytes(R), L, I, R, P, flag);
+    return DT_OK;
+}
+int dt_als_finish(dt_ctx *ctx, const double *P, const double *W, int I, int R, double *L,
+                  double *work, int *flag) {
+    (void)work;
+    DT_TRY(check_r(ctx, R));
+    DT_LAUNCH("als_small", als_finish_kernel, dim3(1), dim3(SMALL_THREADS), lds_bytes(R), P, W, I, R, L,
+              flag);
+    return DT_OK;
+}
+int dt_cholqr3_dev(dt_ctx *ctx, double *X, int m, int n, double *work, int *flag) {
+    (void)work;
+    DT_TRY(check_r(ctx, n));
+    const size_t bytes = (size_t)(2 * n * n + 2 * (stage_rows(n) + 1) * n) * sizeof(double);
+    DT_LAUNCH("cholqr_small", cholqr3_kernel, dim3(1), dim3(SMALL_THREADS), bytes, X, m, n, flag);
+    return DT_OK;
+}
+int dt_gram_inverse_dev(dt_ctx *ctx, const double *X, int m, int n, double *C, double *work,
+                        int *flag) {
+    (void)work;
+    DT_TRY(check_r(ctx, n));
+    DT_LAUNCH("fit_small", gram_inverse_kernel, dim3(1), dim3(SMALL_THREADS), lds_bytes(n), X, m, n, C,
+              flag);
+    return DT_OK;
+}
+int dt_inverse_flag(dt_ctx *ctx, const double *G, int n, double *C, double *work, int *flag, int bit) {
+    (void)work;
+    DT_TRY(check_r(ctx, n));
+    DT_LAUNCH("als_small", inverse_flag_kernel, dim3(1), dim3(SMALL_THREADS), (size_t)n * n * 8, G, n, C,
+              flag, bit);
+    return DT_OK;
+}
+
+// Device-only orthonormalisation of X (m x n): shifted + 2 plain Cholesky-QR
+// passes.  One workgroup when m n^2 is small, otherwise the Gram and the
+// X R^{-1} products run as multi-workgroup GEMMs.  work: >= 2 n*n + m*n doubles.
+int dt_orth_dev(dt_ctx *ctx, double *X, size_t m, size_t n, double *work, int *flag) {
+    if ((double)m * n * n <= (double)(1 << 21) && n <= 32)
+        return dt_cholqr3_dev(ctx, X, (int)m, (int)n, work, flag);
+    DT_TRY(check_r(ctx, (int)n));
+    double *G = work, *Rinv = work + n * n, *T = work + 2 * n * n;
+    for (int pass = 0; pass < 3; pass++) {
+        DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, n, n, m, 1.0, X, m, X, m, 0.0, G, n));
+        DT_LAUNCH("cholqr_small", chol_rinv_kernel, dim3(1), dim3(SMALL_THREADS), 2 * n * n * 8, G, (int)n,
+                  pass == 0 ? 1 : 0, (double)m, Rinv, flag);
+        DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, m, n, n, 1.0, X, m, Rinv, n, 0.0, T, m));
+        DT_HIP(hipMemcpyAsync(X, T, m * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    return DT_OK;
+}

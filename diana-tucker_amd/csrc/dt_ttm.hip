@@ -69,7 +69,12 @@ __global__ __launch_bounds__(256) void ttm_generic(const TA *__restrict__ A, con
 // ---------------------------------------------------------------------------
 // F == 1, FP32.  Wave = 64 columns of A (4 blocks of 16), all J rows.
 //   U: K x J column-major (k fastest).  Requires K % 4 == 0.
-template <int JB>
+template <typename T>
+__device__ __forceinline__ T ld_a(const T *p, bool nt) {
+    return nt ? __builtin_nontemporal_load(p) : *p;
+}
+
+template <int JB, bool NT>
 __global__ __launch_bounds__(256) void ttm_m0_f32(const float *__restrict__ A,
                                                   const float *__restrict__ U,
                                                   float *__restrict__ Y, int K, int64_t B, int J) {
@@ -94,10 +99,12 @@ __global__ __launch_bounds__(256) void ttm_m0_f32(const float *__restrict__ A,
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int k = k0 + 16 * h + 4 * g;
-                if (b < B && k < K)
-                    a[cb][h] = *(const float4 *)(A + k + (int64_t)K * b);
-                else
+                if (b < B && k < K) {
+                    const floatx4 v = ld_a((const floatx4 *)(A + k + (int64_t)K * b), NT);
+                    a[cb][h] = make_float4(v[0], v[1], v[2], v[3]);
+                } else {
                     a[cb][h] = make_float4(0.f, 0.f, 0.f, 0.f);
+                }
             }
         }
         float4 u[JB][2];
@@ -149,7 +156,7 @@ __global__ __launch_bounds__(256) void ttm_m0_f32(const float *__restrict__ A,
 // ---------------------------------------------------------------------------
 // F > 1, FP32.  Wave = FB*64 consecutive f for one b, all J columns.
 //   M: J x K column-major (j fastest).  Requires F % 4 == 0.
-template <int JB, int FB>
+template <int JB, int FB, bool NT>
 __global__ __launch_bounds__(256) void ttm_mk_f32(const float *__restrict__ A,
                                                   const float *__restrict__ M,
                                                   float *__restrict__ Y, int64_t F, int K,
@@ -179,10 +186,12 @@ __global__ __launch_bounds__(256) void ttm_mk_f32(const float *__restrict__ A,
 #pragma unroll
         for (int fb = 0; fb < FB; fb++) {
             const int64_t f = f0 + 64 * fb + 4 * c;
-            if (k < K && f < F)
-                a[fb] = *(const float4 *)(Ab + f + F * k);
-            else
+            if (k < K && f < F) {
+                const floatx4 v = ld_a((const floatx4 *)(Ab + f + F * k), NT);
+                a[fb] = make_float4(v[0], v[1], v[2], v[3]);
+            } else {
                 a[fb] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
         }
         float u[JB];
 #pragma unroll
@@ -222,7 +231,7 @@ __global__ __launch_bounds__(256) void ttm_mk_f32(const float *__restrict__ A,
 // F == 1, BF16 input, FP32 accumulate/output.  Ub: K x J bf16 (k fastest).
 // Requires K % 8 == 0.  Wave = 64 columns, k-step 64 (two 128-byte halves of
 // each column line per lane group).
-template <int JB>
+template <int JB, bool NT>
 __global__ __launch_bounds__(256) void ttm_m0_bf16(const uint16_t *__restrict__ A,
                                                    const uint16_t *__restrict__ Ub,
                                                    float *__restrict__ Y, int K, int64_t B, int J) {
@@ -248,7 +257,7 @@ __global__ __launch_bounds__(256) void ttm_m0_bf16(const uint16_t *__restrict__ 
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int k = k0 + 32 * h + 8 * g;
-                a[cb][h] = (b < B && k < K) ? *(const u32x4 *)(A + k + (int64_t)K * b) : zero4;
+                a[cb][h] = (b < B && k < K) ? ld_a((const u32x4 *)(A + k + (int64_t)K * b), NT) : zero4;
             }
         }
         u32x4 u[JB][2];
@@ -294,7 +303,7 @@ __global__ __launch_bounds__(256) void ttm_m0_bf16(const uint16_t *__restrict__ 
 // Each lane loads 8 k-rows of 8 f-contiguous bf16 and transposes the 8x8 block
 // in registers so that MFMA t receives the 8 k values of f = f0 + 8c + t.
 // Requires F % 8 == 0 and K % 8 == 0.
-template <int JB>
+template <int JB, bool NT>
 __global__ __launch_bounds__(256) void ttm_mk_bf16(const uint16_t *__restrict__ A,
                                                    const uint16_t *__restrict__ Ub,
                                                    float *__restrict__ Y, int64_t F, int K,
@@ -323,7 +332,7 @@ __global__ __launch_bounds__(256) void ttm_mk_bf16(const uint16_t *__restrict__ 
         u32x4 r[8];
 #pragma unroll
         for (int e = 0; e < 8; e++)
-            r[e] = (kin && fin) ? *(const u32x4 *)(Ab + f + F * (int64_t)(kb + e)) : zero4;
+            r[e] = (kin && fin) ? ld_a((const u32x4 *)(Ab + f + F * (int64_t)(kb + e)), NT) : zero4;
         u32x4 u[JB];
 #pragma unroll
         for (int jb = 0; jb < JB; jb++) {
@@ -354,6 +363,59 @@ __global__ __launch_bounds__(256) void ttm_mk_bf16(const uint16_t *__restrict__ 
             *(float4 *)(y + 4) =
                 make_float4(acc[jb][4][v], acc[jb][5][v], acc[jb][6][v], acc[jb][7][v]);
         }
+}
+
+// ---------------------------------------------------------------------------
+// FP64-accumulating mode product for the accurate core: one thread per (f, b),
+// 16 output columns j per pass, A read as FP32 or BF16 and widened to FP64.
+template <typename TA>
+__global__ __launch_bounds__(256) void ttm_f64acc(const TA *__restrict__ A,
+                                                  const double *__restrict__ M,
+                                                  double *__restrict__ Y, int64_t F, int64_t K,
+                                                  int64_t B, int64_t J) {
+    const int64_t total = F * B;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        const int64_t f = idx % F, b = idx / F;
+        const TA *a = A + f + F * K * b;
+        for (int64_t j0 = 0; j0 < J; j0 += 16) {
+            double acc[16];
+#pragma unroll
+            for (int q = 0; q < 16; q++) acc[q] = 0.0;
+            const int jn = (int)((J - j0) < 16 ? (J - j0) : 16);
+            for (int64_t k = 0; k < K; k++) {
+                double av;
+                if constexpr (sizeof(TA) == 2)
+                    av = (double)bf16_to_f32(((const uint16_t *)a)[F * k]);
+                else
+                    av = (double)a[F * k];
+                const double *m = M + j0 + J * k;
+#pragma unroll
+                for (int q = 0; q < 16; q++)
+                    if (q < jn) acc[q] = fma(av, m[q], acc[q]);
+            }
+            for (int q = 0; q < jn; q++) Y[f + F * ((j0 + q) + J * b)] = acc[q];
+        }
+    }
+}
+
+int dt_ttm_f64acc(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t K, int64_t B,
+                  const double *M64, int64_t J, double *Y) {
+    const int64_t total = F * B;
+    int64_t g = (total + 255) / 256;
+    if (g > 262144) g = 262144;
+    if (g < 1) g = 1;
+    if (dtA == DT_F32) {
+        DT_LAUNCH("ttm_f64acc", ttm_f64acc<float>, dim3((unsigned)g), dim3(256), 0, (const float *)A,
+                  M64, Y, F, K, B, J);
+    } else if (dtA == DT_BF16) {
+        DT_LAUNCH("ttm_f64acc", ttm_f64acc<uint16_t>, dim3((unsigned)g), dim3(256), 0,
+                  (const uint16_t *)A, M64, Y, F, K, B, J);
+    } else {
+        DT_LAUNCH("ttm_f64acc", ttm_f64acc<double>, dim3((unsigned)g), dim3(256), 0, (const double *)A,
+                  M64, Y, F, K, B, J);
+    }
+    return DT_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -421,7 +483,7 @@ int dt_f64_to_views(dt_ctx *ctx, const double *U64, size_t K, size_t J, float *U
 
 // ---------------------------------------------------------------------------
 int dt_ttm_views(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t K, int64_t B,
-                 const dt_factor_views &fv, int64_t J, void *Y, int dtY) {
+                 const dt_factor_views &fv, int64_t J, void *Y, int dtY, bool stream) {
     const int64_t total = F * J * B;
     if (total == 0) return DT_OK;
     if (dtA == DT_F64) {
@@ -435,9 +497,14 @@ int dt_ttm_views(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t K, int6
     if (dtA == DT_F32) {
         if (J <= 64 && F == 1 && (K % 4) == 0 && fv.U32) {
             dim3 grid((unsigned)ceil_div(B, 256));
-#define M0F(JBV)                                                                                \
-    DT_LAUNCH("ttm_m0_f32", ttm_m0_f32<JBV>, grid, dim3(256), 0, (const float *)A, fv.U32,      \
-              (float *)Y, (int)K, B, (int)J);                                                   \
+#define M0F(JBV)                                                                          \
+    if (stream) {                                                                         \
+        DT_LAUNCH("ttm_m0_f32_stream", (ttm_m0_f32<JBV, true>), grid, dim3(256), 0,       \
+                  (const float *)A, fv.U32, (float *)Y, (int)K, B, (int)J);               \
+    } else {                                                                              \
+        DT_LAUNCH("ttm_m0_f32", (ttm_m0_f32<JBV, false>), grid, dim3(256), 0,             \
+                  (const float *)A, fv.U32, (float *)Y, (int)K, B, (int)J);               \
+    }                                                                                     \
     return DT_OK;
             if (jbn == 1) { M0F(1) }
             if (jbn == 2) { M0F(2) }
@@ -452,9 +519,14 @@ int dt_ttm_views(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t K, int6
             const int64_t tiles = ceil_div(F, 64 * fb);
             const int64_t waves = tiles * B;
             dim3 grid((unsigned)ceil_div(waves, 4));
-#define MKF(JBV, FBV)                                                                          \
-    DT_LAUNCH("ttm_mk_f32", (ttm_mk_f32<JBV, FBV>), grid, dim3(256), 0, (const float *)A,      \
-              fv.M32, (float *)Y, F, (int)K, B, (int)J, tiles);                                \
+#define MKF(JBV, FBV)                                                                     \
+    if (stream) {                                                                         \
+        DT_LAUNCH("ttm_mk_f32_stream", (ttm_mk_f32<JBV, FBV, true>), grid, dim3(256), 0,  \
+                  (const float *)A, fv.M32, (float *)Y, F, (int)K, B, (int)J, tiles);     \
+    } else {                                                                              \
+        DT_LAUNCH("ttm_mk_f32", (ttm_mk_f32<JBV, FBV, false>), grid, dim3(256), 0,        \
+                  (const float *)A, fv.M32, (float *)Y, F, (int)K, B, (int)J, tiles);     \
+    }                                                                                     \
     return DT_OK;
             if (jbn == 1 && fb == 4) { MKF(1, 4) }
             if (jbn == 1 && fb == 2) { MKF(1, 2) }
@@ -472,9 +544,14 @@ int dt_ttm_views(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t K, int6
     if (dtA == DT_BF16) {
         if (J <= 64 && F == 1 && (K % 8) == 0 && fv.Ub16) {
             dim3 grid((unsigned)ceil_div(B, 256));
-#define M0B(JBV)                                                                                 \
-    DT_LAUNCH("ttm_m0_bf16", ttm_m0_bf16<JBV>, grid, dim3(256), 0, (const uint16_t *)A, fv.Ub16, \
-              (float *)Y, (int)K, B, (int)J);                                                    \
+#define M0B(JBV)                                                                          \
+    if (stream) {                                                                         \
+        DT_LAUNCH("ttm_m0_bf16_stream", (ttm_m0_bf16<JBV, true>), grid, dim3(256), 0,     \
+                  (const uint16_t *)A, fv.Ub16, (float *)Y, (int)K, B, (int)J);           \
+    } else {                                                                              \
+        DT_LAUNCH("ttm_m0_bf16", (ttm_m0_bf16<JBV, false>), grid, dim3(256), 0,           \
+                  (const uint16_t *)A, fv.Ub16, (float *)Y, (int)K, B, (int)J);           \
+    }                                                                                     \
     return DT_OK;
             if (jbn == 1) { M0B(1) }
             if (jbn == 2) { M0B(2) }
@@ -484,9 +561,14 @@ int dt_ttm_views(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t K, int6
         if (J <= 64 && F > 1 && (F % 8) == 0 && (K % 8) == 0 && fv.Ub16) {
             const int64_t tiles = ceil_div(F, 128);
             dim3 grid((unsigned)ceil_div(tiles * B, 4));
-#define MKB(JBV)                                                                                 \
-    DT_LAUNCH("ttm_mk_bf16", ttm_mk_bf16<JBV>, grid, dim3(256), 0, (const uint16_t *)A, fv.Ub16, \
-              (float *)Y, F, (int)K, B, (int)J, tiles);                                          \
+#define MKB(JBV)                                                                          \
+    if (stream) {                                                                         \
+        DT_LAUNCH("ttm_mk_bf16_stream", (ttm_mk_bf16<JBV, true>), grid, dim3(256), 0,     \
+                  (const uint16_t *)A, fv.Ub16, (float *)Y, F, (int)K, B, (int)J, tiles); \
+    } else {                                                                              \
+        DT_LAUNCH("ttm_mk_bf16", (ttm_mk_bf16<JBV, false>), grid, dim3(256), 0,           \
+                  (const uint16_t *)A, fv.Ub16, (float *)Y, F, (int)K, B, (int)J, tiles); \
+    }                                                                                     \
     return DT_OK;
             if (jbn == 1) { MKB(1) }
             if (jbn == 2) { MKB(2) }
@@ -552,5 +634,5 @@ extern "C" int dt_ttm(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_
         fv.U32 = u32;
         fv.Ub16 = ub;
     }
-    return dt_ttm_views(ctx, A, dtA, F, K, B, fv, (int64_t)J, Y, dtY);
+    return dt_ttm_views(ctx, A, dtA, F, K, B, fv, (int64_t)J, Y, dtY, true);
 }

@@ -19,7 +19,7 @@ DT_F64, DT_F32, DT_BF16, DT_I32 = range(4)
 DT_MAX_NDIM = 8
 INIT_GIVEN, INIT_QR, INIT_HOSVD, INIT_RANDOM, INIT_LEARNED = range(5)
 SCHED_REFERENCE, SCHED_TREE = 0, 1
-FLAG_DIRECT_RESIDUAL, FLAG_FP32_POLISH = 1, 2
+FLAG_DIRECT_RESIDUAL, FLAG_FP32_POLISH, FLAG_FP64_FIT = 1, 2, 4
 
 _NP_OF = {DT_F64: np.float64, DT_F32: np.float32, DT_BF16: np.uint16, DT_I32: np.int32}
 _SIZE_OF = {DT_F64: 8, DT_F32: 4, DT_BF16: 2, DT_I32: 4}

@@ -70,7 +70,7 @@ int dt_convert(dt_ctx *ctx, void *dst, int dt_dst, const void *src, int dt_src, 
 /* ---- timing ------------------------------------------------------------
  * With timing enabled every kernel launch of the library is bracketed by HIP
  * events on the context stream; dt_timing_get returns the launch count and the
- * summed event time for one kernel name (names: see DESIGN.md kernel table).
+ * summed event time for one kernel name (dt_timing_list lists the names seen).
  * dt_timer_* time a region of the stream with a pair of events.  Replaces the
  * MPI_Wtime-based Summary timers (src/summary.cpp:29-69). */
 int dt_timing_enable(dt_ctx *ctx, int on);
@@ -170,7 +170,11 @@ enum {
 };
 enum {
     DT_FLAG_DIRECT_RESIDUAL = 1, /* after the last sweep compute ||A - [[G;U]]|| in FP64 */
-    DT_FLAG_FP32_POLISH = 2      /* mixed precision: finish with one FP32 sweep on A_f32 */
+    DT_FLAG_FP32_POLISH = 2,     /* mixed precision: finish with one FP32 sweep on A_f32 */
+    DT_FLAG_FP64_FIT = 4         /* form the core for the residual with FP64 accumulation
+                                    (one extra pass over A per sweep); needed when the
+                                    residual is below ~1e-3, where FP32 rounding of the
+                                    core dominates 1 - ||G||^2/||A||^2 */
 };
 
 typedef struct dt_hooi_args {

@@ -1,0 +1,330 @@
+"""GPU parity of the dense path: every call goes through libdiana_hip.so's C ABI
+and is checked against the FP64 oracle on the same bytes."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import dtucker as dt
+from oracle import dense as D
+from oracle import philox
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "reference_goldens.json")
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def gold():
+    with open(GOLD) as f:
+        g = json.load(f)
+    A = np.array(g["tensor_colmajor"]).reshape(g["tensor_shape"], order="F")
+    m1 = np.array(g["m1_colmajor"], float).reshape(g["m1_shape"], order="F")
+    m2 = np.array(g["m2_colmajor"], float).reshape(g["m2_shape"], order="F")
+    return g, A, m1, m2
+
+
+@pytest.mark.parametrize("dtype", [dt.DT_F64, dt.DT_F32])
+def test_ttm1_golden_on_gpu(ctx, dtype):
+    g, A, m1, m2 = gold()
+    dA = ctx.to_device(A.astype(np.float64 if dtype == dt.DT_F64 else np.float32))
+    t = ctx.ttm(dA, ctx.to_device(m1), 1)
+    t = ctx.ttm(t, ctx.to_device(m2), 2)
+    got = t.download()
+    assert np.array_equal(got.ravel(order="F").astype(np.float64), np.array(g["ttm1_colmajor"]))
+
+
+@pytest.mark.parametrize("dtype", [dt.DT_F64, dt.DT_F32])
+def test_gram1_golden_on_gpu(ctx, dtype):
+    g, A, _, _ = gold()
+    dA = ctx.to_device(A.astype(np.float64 if dtype == dt.DT_F64 else np.float32))
+    S = ctx.gram(dA, 1).download()
+    assert np.array_equal(S.ravel(order="F"), np.array(g["gram1_colmajor"]))
+
+
+TTM_CASES = [
+    # shape, mode, J -- first, middle, last modes; MFMA (J<=64, aligned) and generic
+    ((64, 48, 40), 0, 16), ((64, 48, 40), 1, 16), ((64, 48, 40), 2, 16),
+    ((256, 32, 24), 0, 64), ((32, 256, 24), 1, 64), ((32, 24, 256), 2, 64),
+    ((96, 80, 16), 0, 8), ((96, 80, 16), 2, 8), ((40, 36, 20), 1, 33), ((40, 36, 20), 0, 70),
+    ((16, 256, 64), 1, 16), ((5, 4, 3), 1, 4), ((7, 9, 6, 5), 2, 3), ((128, 64, 8, 8), 3, 48),
+]
+
+
+@pytest.mark.parametrize("shape,n,J", TTM_CASES)
+def test_ttm_f32_vs_oracle(ctx, shape, n, J):
+    print("ttm f32", shape, n, J, flush=True)
+    rng = np.random.default_rng(hash((shape, n, J)) % 2**32)
+    A = rng.standard_normal(shape).astype(np.float32)
+    M = rng.standard_normal((J, shape[n])).astype(np.float32)
+    Y = ctx.ttm(ctx.to_device(A), ctx.to_device(M), n).download()
+    ref = D.ttm(A.astype(np.float64), M.astype(np.float64), n)
+    assert rel(Y, ref) < 2e-6
+
+
+@pytest.mark.parametrize("shape,n,J", TTM_CASES)
+def test_ttm_bf16_vs_oracle(ctx, shape, n, J):
+    print("ttm bf16", shape, n, J, flush=True)
+    rng = np.random.default_rng(1 + hash((shape, n, J)) % 2**32)
+    A = rng.standard_normal(shape).astype(np.float32)
+    Ab = dt.bf16_from_f32(A)
+    M = rng.standard_normal((J, shape[n])).astype(np.float32)
+    dA = ctx.array(shape, dt.DT_BF16).upload(Ab)
+    Y = ctx.ttm(dA, ctx.to_device(M), n).download()
+    # the kernel rounds M to bf16 too (MFMA path); compare with both operands rounded
+    Mb = dt.f32_from_bf16(dt.bf16_from_f32(M)).astype(np.float64)
+    ref = D.ttm(dt.f32_from_bf16(Ab).astype(np.float64), Mb, n)
+    ref_generic = D.ttm(dt.f32_from_bf16(Ab).astype(np.float64), M.astype(np.float64), n)
+    assert min(rel(Y, ref), rel(Y, ref_generic)) < 2e-6
+
+
+@pytest.mark.parametrize("shape,n", [((64, 48, 40), 0), ((64, 48, 40), 1), ((30, 20, 10), 2),
+                                     ((256, 16, 16), 0), ((16, 256, 16), 1), ((7, 5, 130), 2)])
+def test_gram_vs_oracle(ctx, shape, n):
+    print("gram", shape, n, flush=True)
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal(shape).astype(np.float32)
+    S = ctx.gram(ctx.to_device(A), n).download()
+    ref = D.gram(A.astype(np.float64), n)
+    assert rel(S, ref) < 2e-6
+    S64 = ctx.gram(ctx.to_device(A.astype(np.float64)), n).download()
+    assert rel(S64, ref) < 1e-13
+
+
+def test_ttt_except_vs_oracle(ctx):
+    rng = np.random.default_rng(8)
+    A = rng.standard_normal((6, 7, 5))
+    B = rng.standard_normal((6, 3, 5))
+    Cm = ctx.ttt_except(ctx.to_device(A), ctx.to_device(B), 1).download()
+    assert rel(Cm, D.ttt_except(A, B, 1)) < 1e-13
+
+
+def test_sumsq(ctx):
+    rng = np.random.default_rng(9)
+    for n in (1, 1000, 123457):
+        x = rng.standard_normal(n)
+        assert abs(ctx.sumsq(ctx.to_device(x)) - np.sum(x * x)) < 1e-9 * n
+        x32 = x.astype(np.float32)
+        assert abs(ctx.sumsq(ctx.to_device(x32)) - np.sum(x32.astype(np.float64) ** 2)) < 1e-9 * n
+
+
+@pytest.mark.parametrize("tA,tB,m,n,k", [(0, 0, 33, 17, 70), (1, 0, 16, 16, 5000),
+                                         (0, 1, 64, 40, 8), (1, 1, 9, 70, 31)])
+def test_gemm_f64(ctx, tA, tB, m, n, k):
+    rng = np.random.default_rng(10)
+    A = rng.standard_normal((k, m) if tA else (m, k))
+    B = rng.standard_normal((n, k) if tB else (k, n))
+    Cm = ctx.gemm(tA, tB, ctx.to_device(A), ctx.to_device(B)).download()
+    ref = (A.T if tA else A) @ (B.T if tB else B)
+    assert rel(Cm, ref) < 1e-14
+
+
+@pytest.mark.parametrize("m,n", [(64, 8), (1024, 64), (300, 1), (50, 50), (20000, 16)])
+def test_qr_f64(ctx, m, n):
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((m, n))
+    Q, R = ctx.qr(ctx.to_device(A))
+    Q, R = Q.download(), R.download()
+    assert np.allclose(Q.T @ Q, np.eye(n), atol=1e-13)
+    assert np.allclose(np.triu(R), R)
+    assert np.all(np.diag(R) > 0)
+    assert rel(Q @ R, A) < 1e-13
+    Qn = D.orth_posdiag(A)
+    assert np.allclose(Q, Qn, atol=1e-10)
+
+
+def test_qr_ill_conditioned_uses_shift(ctx):
+    rng = np.random.default_rng(12)
+    U = np.linalg.qr(rng.standard_normal((500, 10)))[0]
+    V = np.linalg.qr(rng.standard_normal((10, 10)))[0]
+    A = U @ np.diag(np.logspace(0, -9, 10)) @ V.T
+    Q, R = ctx.qr(ctx.to_device(A))
+    Q = Q.download()
+    assert np.allclose(Q.T @ Q, np.eye(10), atol=1e-10)
+    assert D.sin_max_angle(Q, A) < 1e-6
+
+
+def test_inverse_f64(ctx):
+    rng = np.random.default_rng(13)
+    for n in (1, 8, 64, 100):
+        A = rng.standard_normal((n, n)) + n * np.eye(n)
+        Ci = ctx.inverse(ctx.to_device(A)).download()
+        assert np.allclose(Ci @ A, np.eye(n), atol=1e-11)
+
+
+def test_inverse_singular_reports_enumeric(ctx):
+    A = np.ones((4, 4))
+    with pytest.raises(dt.DTError) as e:
+        ctx.inverse(ctx.to_device(A))
+    assert e.value.code == dt.DT_ENUMERIC
+
+
+@pytest.mark.parametrize("n,r", [(8, 3), (64, 8), (150, 20), (256, 16), (1024, 64)])
+def test_syev_top(ctx, n, r):
+    print("syev", n, r, flush=True)
+    rng = np.random.default_rng(14)
+    Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    lam = np.concatenate([np.logspace(3, 1, r), rng.uniform(0, 1, n - r)])
+    S = (Q * lam) @ Q.T
+    S = (S + S.T) / 2
+    V, w = ctx.syev_top(ctx.to_device(S), r)
+    V, w = V.download(), w.download()
+    ev, EV = np.linalg.eigh(S)
+    assert np.allclose(w, ev[::-1][:r], rtol=1e-10)
+    assert D.sin_max_angle(V, EV[:, ::-1][:, :r]) < 1e-8
+
+
+def test_orth_rows(ctx):
+    rng = np.random.default_rng(15)
+    A = rng.standard_normal((5, 300))
+    out = ctx.orth_rows(ctx.to_device(A)).download()
+    assert np.allclose(out @ out.T, np.eye(5), atol=1e-13)
+    assert D.sin_max_angle(out.T, A.T) < 1e-12
+
+
+def test_generators_match_oracle(ctx):
+    x = ctx.gen_normal(10001, dt.DT_F64, 20000905, 3).download()
+    ref = philox.normal(10001, 20000905, 3)
+    assert np.allclose(x, ref, rtol=1e-13, atol=1e-13)
+    u = ctx.gen_uniform(777, dt.DT_F64, 5, 9).download()
+    assert np.allclose(u, philox.uniform(777, 5, 9), rtol=0, atol=0)
+
+
+def test_gen_tucker_matches_oracle(ctx):
+    shape, ranks = (20, 16, 12), (4, 3, 2)
+    g = ctx.gen_tucker(shape, ranks, 1e-2, seed=99, decay=0.9, want_factors=True)
+    A = g["A"].download()
+    Aref, Uref, _ = D.tucker_model(shape, ranks, 1e-2, 99, decay=0.9)
+    assert rel(A, Aref) < 1e-6
+    for U, Ur in zip(g["U"], Uref):
+        assert np.allclose(U.download(), Ur, atol=1e-12)
+    assert abs(g["normA"] - np.linalg.norm(A.astype(np.float64).ravel())) < 1e-6 * g["normA"]
+
+
+# ---------------------------------------------------------------------------
+# HOOI end to end
+def run_gpu_hooi(ctx, A, ranks, sweeps, init, U_raw=None, schedule=dt.SCHED_TREE, dtype=dt.DT_F32,
+                 flags=0):
+    dA = ctx.to_device(A.astype(np.float64 if dtype == dt.DT_F64 else np.float32))
+    Uin = [ctx.to_device(u) for u in U_raw] if U_raw is not None else None
+    return ctx.hooi_dense(dA, ranks, max_iter=sweeps, init=init, U_init=Uin, schedule=schedule,
+                          flags=flags)
+
+
+@pytest.mark.parametrize("schedule,flags", [(dt.SCHED_REFERENCE, 0), (dt.SCHED_TREE, 0),
+                                            (dt.SCHED_REFERENCE, dt.FLAG_FP64_FIT),
+                                            (dt.SCHED_TREE, dt.FLAG_FP64_FIT)])
+def test_hooi_c1_reference_random_init(ctx, schedule, flags):
+    """C1: 64^3, core 8^3, noise 1e-3, ranks 8, 10 sweeps, the reference's own
+    random start (srand(20000905), Util::randn, reduced_QR)."""
+    shape, ranks = (64, 64, 64), (8, 8, 8)
+    g = ctx.gen_tucker(shape, ranks, 1e-3, seed=20000905)
+    A = g["A"].download()
+    raws, U0 = D.reference_random_factors(shape, ranks, 20000905)
+    Gr, Ur, res_ref = D.hooi_als(A.astype(np.float64), ranks, 10, U0)
+    out = run_gpu_hooi(ctx, A, ranks, 10, dt.INIT_QR, raws, schedule, flags=flags)
+    fit_gpu = 1 - np.array(out["residuals"])
+    fit_ref = 1 - np.array(res_ref)
+    print("residuals gpu", out["residuals"][-1], "ref", res_ref[-1], flush=True)
+    # the true fit of the GPU factors, evaluated by the oracle in FP64
+    G_true = D.core(A.astype(np.float64), out["U"])
+    res_true = np.sqrt(1 - np.sum(G_true ** 2) / np.sum(A.astype(np.float64) ** 2))
+    assert abs((1 - res_true) - fit_ref[-1]) / fit_ref[-1] < 1e-5
+    # the residual the GPU reports: FP64-accumulated core -> 1e-5; FP32 core -> 5e-5
+    tol = 1e-5 if flags & dt.FLAG_FP64_FIT else 5e-5
+    assert np.max(np.abs(fit_gpu - fit_ref) / fit_ref) < tol
+    for n in range(3):
+        assert D.sin_max_angle(out["U"][n], Ur[n]) < 1e-3
+    # core agrees with the oracle core on the GPU factors
+    Gg = D.core(A.astype(np.float64), out["U"])
+    assert rel(out["G"], Gg) < 1e-5
+
+
+def test_hooi_c1_hosvd_and_fp64(ctx):
+    shape, ranks = (64, 64, 64), (8, 8, 8)
+    g = ctx.gen_tucker(shape, ranks, 1e-3, seed=7)
+    A = g["A"].download().astype(np.float64)
+    Uh = D.hosvd_factors(A, ranks)
+    Gr, Ur, res_ref = D.hooi_als(A, ranks, 10, Uh)
+    for dtype in (dt.DT_F32, dt.DT_F64):
+        out = run_gpu_hooi(ctx, A, ranks, 10, dt.INIT_HOSVD, dtype=dtype, flags=dt.FLAG_FP64_FIT)
+        print("dtype", dtype, "fit gpu", 1 - out["residuals"][-1], "ref", 1 - res_ref[-1], flush=True)
+        f_gpu, f_ref = 1 - out["residuals"][-1], 1 - res_ref[-1]
+        assert abs(f_gpu - f_ref) / f_ref < (1e-5 if dtype == dt.DT_F32 else 1e-10)
+        for n in range(3):
+            assert D.sin_max_angle(out["U"][n], Ur[n]) < (1e-3 if dtype == dt.DT_F32 else 1e-8)
+
+
+def test_hooi_4way_tree_matches_reference_schedule(ctx):
+    shape, ranks = (32, 24, 20, 16), (4, 5, 3, 2)
+    g = ctx.gen_tucker(shape, ranks, 1e-2, seed=5)
+    A = g["A"].download()
+    r1 = run_gpu_hooi(ctx, A, ranks, 5, dt.INIT_HOSVD, schedule=dt.SCHED_REFERENCE,
+                      flags=dt.FLAG_FP64_FIT)
+    r2 = run_gpu_hooi(ctx, A, ranks, 5, dt.INIT_HOSVD, schedule=dt.SCHED_TREE, flags=dt.FLAG_FP64_FIT)
+    assert np.allclose(r1["residuals"], r2["residuals"], rtol=1e-6)
+    for n in range(4):
+        assert D.sin_max_angle(r1["U"][n], r2["U"][n]) < 1e-4
+    Uh = D.hosvd_factors(A.astype(np.float64), ranks)
+    _, _, res_ref = D.hooi_als(A.astype(np.float64), ranks, 5, Uh)
+    assert np.allclose(r2["residuals"], res_ref, rtol=1e-4)
+
+
+def test_hooi_bf16_close_to_fp32(ctx):
+    shape, ranks = (128, 128, 128), (16, 16, 16)
+    g = ctx.gen_tucker(shape, ranks, 1e-3, seed=3, decay=0.93, bf16=True)
+    r32 = ctx.hooi_dense(g["A"], ranks, max_iter=4, init=dt.INIT_HOSVD)
+    rbf = ctx.hooi_dense(g["A_bf16"], ranks, max_iter=4, init=dt.INIT_HOSVD, A_f32=g["A"],
+                         flags=dt.FLAG_FP32_POLISH)
+    assert abs((1 - rbf["residual"]) - (1 - r32["residual"])) < 1e-4
+    for n in range(3):
+        assert D.sin_max_angle(rbf["U"][n], r32["U"][n]) < 0.05
+
+
+def test_hooi_session_matches_one_shot(ctx):
+    shape, ranks = (48, 40, 32), (6, 5, 4)
+    g = ctx.gen_tucker(shape, ranks, 1e-2, seed=13)
+    one = ctx.hooi_dense(g["A"], ranks, max_iter=3, init=dt.INIT_RANDOM, seed=4)
+    s = ctx.hooi_session(g["A"], ranks, init=dt.INIT_RANDOM, seed=4)
+    res = [s.sweep() for _ in range(3)]
+    out = s.result()
+    s.close()
+    assert np.allclose(res, one["residuals"], rtol=1e-12)
+    assert rel(out["G"], one["G"]) < 1e-12
+
+
+def test_hooi_rank_above_size_is_einval(ctx):
+    dA = ctx.to_device(np.ones((4, 5, 6), np.float32))
+    with pytest.raises(dt.DTError) as e:
+        ctx.hooi_dense(dA, (5, 2, 2), max_iter=1, init=dt.INIT_RANDOM)
+    assert e.value.code == dt.DT_EINVAL
+
+
+def test_hooi_zero_tensor_is_enumeric(ctx):
+    dA = ctx.to_device(np.zeros((8, 8, 8), np.float32))
+    with pytest.raises(dt.DTError) as e:
+        ctx.hooi_dense(dA, (2, 2, 2), max_iter=1, init=dt.INIT_RANDOM)
+    assert e.value.code == dt.DT_ENUMERIC
+
+
+def test_hooi_c2_full_size_properties(ctx):
+    """C2 at its real shape: 256^4 FP32 (17.2 GB), core 16^4, noise 1e-2,
+    HOSVD init, 20 sweeps.  Size-independent checks: the residual reaches the
+    noise floor eps*sqrt(1 - prod R/prod I)/sqrt(1+eps^2) and the factors
+    recover the true subspaces to the noise level."""
+    shape, ranks, eps = (256,) * 4, (16,) * 4, 1e-2
+    print("C2 generate", flush=True)
+    g = ctx.gen_tucker(shape, ranks, eps, seed=20000905, want_factors=True)
+    print("C2 hooi", flush=True)
+    out = ctx.hooi_dense(g["A"], ranks, max_iter=20, init=dt.INIT_HOSVD)
+    pred = eps * np.sqrt(1 - np.prod(ranks) / np.prod(shape)) / np.sqrt(1 + eps * eps)
+    print("C2 residual", out["residual"], "pred", pred, "sweep ms", out["sweeps_ms"] / 20, flush=True)
+    assert abs(out["residual"] - pred) / pred < 1e-3
+    for n in range(4):
+        assert D.sin_max_angle(out["U"][n], g["U"][n].download()) < 5e-3
+    del g

@@ -1,0 +1,178 @@
+"""CPU tests pinning the oracle: reference golden vectors, libc's own rand(),
+Philox known answers, literal loop transcriptions and closed forms."""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import dense as D
+from oracle import philox
+from oracle.glibc_rand import GlibcRand, util_randn
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "reference_goldens.json")
+
+
+def load_gold():
+    with open(GOLD) as f:
+        g = json.load(f)
+    A = np.array(g["tensor_colmajor"]).reshape(g["tensor_shape"], order="F")
+    m1 = np.array(g["m1_colmajor"], float).reshape(g["m1_shape"], order="F")
+    m2 = np.array(g["m2_colmajor"], float).reshape(g["m2_shape"], order="F")
+    return g, A, m1, m2
+
+
+def test_reference_ttm1_golden():
+    # tests/testcases/function/distributed/ttm.cpp:7-37
+    g, A, m1, m2 = load_gold()
+    t = D.ttm(D.ttm(A, m1, 1), m2, 2)
+    assert list(t.shape) == g["ttm1_shape"]
+    assert np.array_equal(t.ravel(order="F"), np.array(g["ttm1_colmajor"]))
+
+
+def test_reference_gram1_golden():
+    # tests/testcases/function/distributed/gram.cpp:7-19
+    g, A, _, _ = load_gold()
+    S = D.gram(A, 1)
+    assert np.array_equal(S.ravel(order="F"), np.array(g["gram1_colmajor"]))
+
+
+def test_fixture_matches_per_rank_rule():
+    # FunctionDistributedTest.cpp:8-15 with ceil-div blocks: rank 0's block is
+    # the 3x2x3 corner, filled 0..17 in column-major order.
+    _, A, _, _ = load_gold()
+    assert np.array_equal(A[:3, :2, :].ravel(order="F"), np.arange(18.0))
+
+
+@pytest.mark.parametrize("shape", [(5, 4, 3), (2, 3, 4, 2), (7, 1, 3), (3, 5)])
+def test_tenmat_matches_loop_transcription(shape):
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal(shape)
+    flat = list(A.ravel(order="F"))
+    for n in range(len(shape)):
+        lo = np.array(D.tenmat_loops(flat, list(shape), n))
+        assert np.array_equal(D.tenmat(A, n).ravel(order="F"), lo)
+        lt = np.array(D.tenmatt_loops(flat, list(shape), n))
+        assert np.array_equal(D.tenmatt(A, n).ravel(order="F"), lt)
+        # mattten inverts tenmatt
+        assert np.array_equal(D.mattten(D.tenmatt(A, n), shape, n), A)
+
+
+@pytest.mark.parametrize("shape,n,J", [((5, 4, 3), 0, 2), ((5, 4, 3), 1, 6), ((5, 4, 3), 2, 3),
+                                       ((3, 4, 2, 5), 2, 4), ((6, 7), 1, 3)])
+def test_ttm_brute_force(shape, n, J):
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal(shape)
+    M = rng.standard_normal((J, shape[n]))
+    Y = D.ttm(A, M, n)
+    # brute force element by element
+    out_shape = list(shape)
+    out_shape[n] = J
+    ref = np.zeros(out_shape)
+    for idx in np.ndindex(*out_shape):
+        s = 0.0
+        for k in range(shape[n]):
+            src = list(idx)
+            src[n] = k
+            s += A[tuple(src)] * M[idx[n], k]
+        ref[idx] = s
+    assert np.allclose(Y, ref, rtol=1e-13, atol=1e-13)
+
+
+def test_ttt_except_and_gram_closed_form():
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((4, 5, 3))
+    B = rng.standard_normal((4, 2, 3))
+    C = D.ttt_except(A, B, 1)
+    ref = np.einsum("ajc,abc->jb", A, B)
+    assert np.allclose(C, ref)
+    S = D.gram(A, 2)
+    assert np.allclose(S, S.T)
+    assert np.all(np.linalg.eigvalsh(S) > -1e-12)
+
+
+def test_glibc_rand_matches_libc():
+    libc = ctypes.CDLL("libc.so.6")
+    for seed in (1, 20000905, 0, 424242):
+        libc.srand(ctypes.c_uint(seed))
+        g = GlibcRand(seed)
+        assert [libc.rand() for _ in range(1000)] == [g.rand() for _ in range(1000)]
+
+
+def test_util_randn_polar_moments():
+    g = GlibcRand(7)
+    x = np.array([util_randn(g) for _ in range(20000)])
+    assert abs(x.mean()) < 0.03 and abs(x.std() - 1) < 0.03
+
+
+def test_philox_known_answers():
+    # Salmon et al. SC'11 / Random123 known-answer vectors for philox4x32-10
+    cases = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+             ((0xffffffff,) * 4, (0xffffffff, 0xffffffff),
+              (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+             ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+              (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for c, k, want in cases:
+        got = tuple(int(x) for x in philox.philox4x32_10(*c, *k))
+        assert got == want
+
+
+def test_philox_normal_moments():
+    x = philox.normal(200000, 123, 5)
+    assert abs(x.mean()) < 0.01 and abs(x.std() - 1) < 0.01
+    u = philox.uniform(100000, 1, 2)
+    assert u.min() > 0 and u.max() <= 1 and abs(u.mean() - 0.5) < 0.01
+
+
+def test_als_step_is_power_iteration_span():
+    """ALS_ returns an orthonormal basis of span(S^k L0), S = Y_(n) Y_(n)^T
+    (exact-arithmetic closed form of hooi_als.tpp:7-34)."""
+    rng = np.random.default_rng(4)
+    Y = rng.standard_normal((9, 4, 5))
+    L0 = np.linalg.qr(rng.standard_normal((9, 3)))[0]
+    for k in (1, 3, 5):
+        Q = D.als_(Y, 0, L0, k)
+        S = D.gram(Y, 0)
+        ref = np.linalg.matrix_power(S, k) @ L0
+        assert np.allclose(Q.T @ Q, np.eye(3), atol=1e-12)
+        assert D.sin_max_angle(Q, ref) < 1e-9
+
+
+def test_sin_max_angle_rotation_invariant():
+    rng = np.random.default_rng(5)
+    U = np.linalg.qr(rng.standard_normal((20, 4)))[0]
+    Rot = np.linalg.qr(rng.standard_normal((4, 4)))[0]
+    assert D.sin_max_angle(U, U @ Rot) < 1e-12
+    V = np.linalg.qr(rng.standard_normal((20, 4)))[0]
+    assert D.sin_max_angle(U, V) > 0.1
+
+
+def test_hooi_noise_free_exact():
+    A, Us, G = D.tucker_model((12, 10, 8), (3, 2, 2), 0.0, seed=11)
+    _, U0 = D.reference_random_factors(A.shape, (3, 2, 2), 20000905)
+    Gh, Uh, res = D.hooi_als(A.astype(np.float64), (3, 2, 2), 4, U0)
+    assert res[-1] < 1e-3   # FP32 storage of X limits this
+    for n in range(3):
+        assert D.sin_max_angle(Uh[n], Us[n]) < 1e-5
+
+
+def test_hooi_noise_closed_form():
+    """With isotropic noise eps*||X|| the converged residual is close to
+    eps*sqrt(1 - prod R / prod I) (noise outside the Tucker subspace)."""
+    shape, ranks, eps = (24, 20, 16), (4, 4, 3), 1e-2
+    A, Us, G = D.tucker_model(shape, ranks, eps, seed=3)
+    Uh = D.hosvd_factors(A.astype(np.float64), ranks)
+    _, U, res = D.hooi_als(A.astype(np.float64), ranks, 6, Uh)
+    pred = eps * np.sqrt(1 - np.prod(ranks) / np.prod(shape)) / np.sqrt(1 + eps * eps)
+    assert abs(res[-1] - pred) / pred < 0.05
+    # residual is non-increasing across sweeps (HOOI monotonicity)
+    assert all(res[i + 1] <= res[i] + 1e-12 for i in range(len(res) - 1))
+
+
+def test_reference_random_init_orthonormal_and_repeatable():
+    raws, Us = D.reference_random_factors((6, 5), (2, 3), 20000905)
+    raws2, _ = D.reference_random_factors((6, 5), (2, 3), 20000905)
+    assert all(np.array_equal(a, b) for a, b in zip(raws, raws2))
+    for U in Us:
+        assert np.allclose(U.T @ U, np.eye(U.shape[1]), atol=1e-12)
