@@ -161,6 +161,11 @@ int dt_timing_enable(dt_ctx *ctx, int on) {
     return DT_OK;
 }
 
+int dt_timing_filter(dt_ctx *ctx, const char *prefix) {
+    ctx->timing_prefix = prefix ? prefix : "";
+    return DT_OK;
+}
+
 int dt_timing_reset(dt_ctx *ctx) {
     DT_HIP(hipStreamSynchronize(ctx->stream));
     dt_collect_timing(ctx);

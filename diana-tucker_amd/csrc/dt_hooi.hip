@@ -55,6 +55,8 @@ struct dt_hooi {
     int *dflag = nullptr;        // device breakdown flags
     void *fbuf[2] = {};          // FP64 full-pass buffers (DT_FLAG_FP64_FIT)
     int bf16_core_mode = -1;     // mode whose core-chain product used the bf16 factor
+    bool args_reference_als = false;  // run ALS_ literally (inverse-normalised steps)
+    double *alsws = nullptr;     // workspace of dt_als_update
     std::vector<void *> owned;
     int sweeps = 0;
     double last_res = -1.0;
@@ -127,6 +129,17 @@ static int als_update(dt_hooi *h, const void *Y, int ydt, const int64_t *sh, int
     const size_t In = (size_t)sh[n], Rn = (size_t)h->R[n];
     const bool small = (double)In * Rn * Rn <= (double)(1 << 20);
     DT_TRY(dt_gram_internal(ctx, Y, ydt, F, (int64_t)In, B, h->S));
+    if (!h->args_reference_als) {
+        // span-equivalent normalised iteration, row blocks over many workgroups;
+        // writes the FP64 factor and its kernel views
+        const bool f64 = h->args.dtype == DT_F64;
+        DT_TRY(dt_als_update(ctx, h->S, h->U64[n], (int)In, (int)Rn, h->args.inner_iter, h->alsws,
+                             h->dflag, f64 ? nullptr : h->U32[n], f64 ? nullptr : h->M32[n],
+                             h->args.dtype == DT_BF16 ? h->Ub16[n] : nullptr));
+        if (f64) DT_TRY(update_views(h, n));
+        return DT_OK;
+    }
+    (void)small;
     DT_TRY(dt_copy_f64(ctx, h->L, h->U64[n], In * Rn));
     for (int it = 0; it < h->args.inner_iter; it++) {
         const double *Pp = h->L;
@@ -425,6 +438,8 @@ int dt_hooi_create(dt_ctx *ctx, const dt_hooi_args *args, dt_hooi **out) {
     if (st == DT_OK) st = hooi_alloc(h, (void **)&h->Xfix, ir);
     if (st == DT_OK) st = hooi_alloc(h, (void **)&h->dscal, 256);
     if (st == DT_OK) st = hooi_alloc(h, (void **)&h->dflag, 256);
+    if (st == DT_OK) st = hooi_alloc(h, (void **)&h->alsws, dt_als_ws_doubles(maxI, maxR) * 8);
+    h->args_reference_als = (args->flags & DT_FLAG_REFERENCE_ALS) != 0;
     for (int n = 0; n < N && st == DT_OK; n++) st = hooi_alloc(h, (void **)&h->Cfix[n], rr);
     if (st == DT_OK && (args->flags & DT_FLAG_FP64_FIT)) {
         st = hooi_alloc(h, &h->fbuf[0], maxe * 8);

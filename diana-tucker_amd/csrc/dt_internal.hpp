@@ -31,6 +31,7 @@ struct dt_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     bool timing = false;
+    std::string timing_prefix;  // when non-empty only launches with this name prefix are timed
     std::vector<dt_pending> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, dt_kstat> stats;
@@ -102,14 +103,17 @@ struct dt_ktimer {
     dt_ctx *c;
     const char *name;
     hipEvent_t a = nullptr;
+    bool on = false;
     dt_ktimer(dt_ctx *c_, const char *n) : c(c_), name(n) {
-        if (c->timing) {
+        on = c->timing && (c->timing_prefix.empty() ||
+                           strncmp(n, c->timing_prefix.c_str(), c->timing_prefix.size()) == 0);
+        if (on) {
             a = dt_get_event(c);
             hipEventRecord(a, c->stream);
         }
     }
     ~dt_ktimer() {
-        if (c->timing) {
+        if (on) {
             hipEvent_t b = dt_get_event(c);
             hipEventRecord(b, c->stream);
             c->pending.push_back({name, a, b});
@@ -184,6 +188,11 @@ int dt_cholqr3_dev(dt_ctx *ctx, double *X, int m, int n, double *work, int *flag
 int dt_gram_inverse_dev(dt_ctx *ctx, const double *X, int m, int n, double *C, double *work, int *flag);
 int dt_inverse_flag(dt_ctx *ctx, const double *G, int n, double *C, double *work, int *flag, int bit);
 int dt_orth_dev(dt_ctx *ctx, double *X, size_t m, size_t n, double *work, int *flag);
+size_t dt_als_ws_doubles(size_t I, size_t R);
+int dt_als_update(dt_ctx *ctx, const double *S, double *U, int I, int R, int inner, double *ws,
+                  int *flag, float *U32, float *M32, uint16_t *Ub16);
+int dt_als_fused(dt_ctx *ctx, const double *S, double *U, int I, int R, int inner, double *L,
+                 double *P, double *W, int *flag);
 int dt_ttm_f64acc(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t K, int64_t B,
                   const double *M64, int64_t J, double *Y);
 int dt_dot_f64_dev(dt_ctx *ctx, const double *a, const double *b, size_t n, double *out_dev);

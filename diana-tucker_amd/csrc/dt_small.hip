@@ -50,6 +50,7 @@ __device__ void wg_xty(const double *__restrict__ X, const double *__restrict__ 
                 const int i = o % R, j = o / R;
                 const double *xa = Xs + ld * i, *yb = Ys + ld * j;
                 double s = acc[q];
+#pragma unroll 8
                 for (int r = 0; r < CH; r++) s = fma(xa[r], yb[r], s);
                 acc[q] = s;
             }
@@ -99,19 +100,24 @@ __device__ int wg_inverse_inplace(double *__restrict__ A, int n, int *__restrict
     if (tid == 0) bad_s = 0;
     __syncthreads();
     for (int k = 0; k < n; k++) {
-        if (tid == 0) {
-            int p = k;
-            double best = fabs(A[k + n * k]);
-            for (int i = k + 1; i < n; i++) {
-                const double v = fabs(A[i + n * k]);
-                if (v > best) {
-                    best = v;
-                    p = i;
+        if (tid < 64) {
+            // argmax |A[i][k]| over i >= k in one wave; ties keep the lower row
+            const int i = k + tid;
+            double v = (i < n) ? fabs(A[i + n * k]) : -1.0;
+            int p = (i < n) ? i : n;
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ov = __shfl_xor(v, off, 64);
+                const int op = __shfl_xor(p, off, 64);
+                if (ov > v || (ov == v && op < p)) {
+                    v = ov;
+                    p = op;
                 }
             }
-            piv_s = p;
-            perm[k] = p;
-            if (!(best > 0.0) || !isfinite(best)) bad_s = k + 1;
+            if (tid == 0) {
+                piv_s = p;
+                perm[k] = p;
+                if (!(v > 0.0) || !isfinite(v)) bad_s = k + 1;
+            }
         }
         __syncthreads();
         if (bad_s) return bad_s;
@@ -325,7 +331,137 @@ __global__ __launch_bounds__(SMALL_THREADS) void chol_rinv_kernel(const double *
     for (int i = threadIdx.x; i < n * n; i += blockDim.x) Rinv[i] = Xo[i];
 }
 
+// Out (I x R, global) = S (I x I, symmetric, global) X (I x R, global).
+// Thread i streams column i of S (= row i, contiguous) with 16-byte loads;
+// the rows of X are staged CH at a time through LDS and read as broadcasts.
+template <int RMAX>
+__device__ void wg_sx(const double *__restrict__ S, const double *__restrict__ X, int I, int R,
+                      double *__restrict__ Out, double *__restrict__ stage, int CH) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int ld = CH + 1;
+    const bool vec = (I % 2) == 0;
+    for (int i0 = 0; i0 < I; i0 += nt) {
+        const int i = i0 + tid;
+        double acc[RMAX];
+#pragma unroll
+        for (int j = 0; j < RMAX; j++) acc[j] = 0.0;
+        for (int t0 = 0; t0 < I; t0 += CH) {
+            const int ch = (I - t0) < CH ? (I - t0) : CH;
+            for (int e = tid; e < CH * R; e += nt) {
+                const int r = e % CH, c = e / CH;
+                stage[r + ld * c] = r < ch ? X[(t0 + r) + (size_t)I * c] : 0.0;
+            }
+            __syncthreads();
+            if (i < I) {
+                const double *col = S + (size_t)I * i + t0;
+                int r = 0;
+                if (vec) {
+#pragma unroll 4
+                    for (; r + 1 < ch; r += 2) {
+                        const double2 v = *(const double2 *)(col + r);
+#pragma unroll
+                        for (int j = 0; j < RMAX; j++)
+                            if (j < R) {
+                                acc[j] = fma(v.x, stage[r + ld * j], acc[j]);
+                                acc[j] = fma(v.y, stage[r + 1 + ld * j], acc[j]);
+                            }
+                    }
+                }
+                for (; r < ch; r++) {
+                    const double sv = col[r];
+#pragma unroll
+                    for (int j = 0; j < RMAX; j++)
+                        if (j < R) acc[j] = fma(sv, stage[r + ld * j], acc[j]);
+                }
+            }
+            __syncthreads();
+        }
+        if (i < I) {
+#pragma unroll
+            for (int j = 0; j < RMAX; j++)
+                if (j < R) Out[i + (size_t)I * j] = acc[j];
+        }
+    }
+    __syncthreads();
+}
+
+// The whole per-mode update of ALS_ (hooi_als.tpp:7-34) in one workgroup:
+// `inner` steps of P = L (LᵀL)^{-1}, W = S P, L = W (Pᵀ W)^{-1}, then the
+// orthonormal basis of span(L) by shifted Cholesky-QR3.  U: in = L_initial
+// (orthonormal start), out = Q.  L, P, W: I x R global scratch.
+// LDS: M (R*R) | Ri (R*R) | stage (2*(CH+1)*R).
+template <int RMAX>
+__global__ __launch_bounds__(SMALL_THREADS) void als_fused_kernel(const double *__restrict__ S,
+                                                                  double *__restrict__ U, int I,
+                                                                  int R, int inner,
+                                                                  double *__restrict__ L,
+                                                                  double *__restrict__ P,
+                                                                  double *__restrict__ W,
+                                                                  int *__restrict__ flag) {
+    __shared__ int perm[kMaxR];
+    __shared__ double shift_s;
+    const int CH = stage_rows(R);
+    double *M = lds_base(), *Ri = M + R * R, *stage = Ri + R * R;
+    for (int e = threadIdx.x; e < I * R; e += blockDim.x) L[e] = U[e];
+    __syncthreads();
+    for (int it = 0; it < inner; it++) {
+        const double *Pp = L;
+        if (it > 0) {
+            wg_xty(L, L, I, R, M, stage, CH);
+            if (wg_inverse_inplace(M, R, perm)) {
+                if (threadIdx.x == 0) atomicOr(flag, 1);
+                return;
+            }
+            wg_xm(L, M, I, R, P, stage, CH);
+            Pp = P;
+        }
+        wg_sx<RMAX>(S, Pp, I, R, W, stage, CH);
+        wg_xty(Pp, W, I, R, M, stage, CH);
+        if (wg_inverse_inplace(M, R, perm)) {
+            if (threadIdx.x == 0) atomicOr(flag, 2);
+            return;
+        }
+        wg_xm(W, M, I, R, L, stage, CH);
+    }
+    for (int pass = 0; pass < 3; pass++) {
+        wg_xty(L, L, I, R, M, stage, CH);
+        if (threadIdx.x == 0) {
+            double shift = 0.0;
+            if (pass == 0) {
+                double tr = 0.0;
+                for (int i = 0; i < R; i++) tr += M[i + R * i];
+                shift = 11.0 * ((double)I * R + (double)R * (R + 1)) * 1.1102230246251565e-16 * tr;
+            }
+            shift_s = shift;
+        }
+        __syncthreads();
+        if (wg_chol_rinv(M, R, shift_s, Ri)) {
+            if (threadIdx.x == 0) atomicOr(flag, 4);
+            return;
+        }
+        wg_xm(L, Ri, I, R, pass == 2 ? U : L, stage, CH);
+    }
+}
+
 // ---- host wrappers -----------------------------------------------------------
+// One launch for the whole ALS_ + QR when R <= 32 (returns DT_EUNSUPPORTED otherwise).
+int dt_als_fused(dt_ctx *ctx, const double *S, double *U, int I, int R, int inner, double *L,
+                 double *P, double *W, int *flag) {
+    if (R > 32) return DT_EUNSUPPORTED;
+    const size_t bytes = (size_t)(2 * R * R + 2 * (stage_rows(R) + 1) * R) * sizeof(double);
+    if (R <= 8) {
+        DT_LAUNCH("als_fused", als_fused_kernel<8>, dim3(1), dim3(SMALL_THREADS), bytes, S, U, I, R, inner,
+                  L, P, W, flag);
+    } else if (R <= 16) {
+        DT_LAUNCH("als_fused", als_fused_kernel<16>, dim3(1), dim3(SMALL_THREADS), bytes, S, U, I, R, inner,
+                  L, P, W, flag);
+    } else {
+        DT_LAUNCH("als_fused", als_fused_kernel<32>, dim3(1), dim3(SMALL_THREADS), bytes, S, U, I, R, inner,
+                  L, P, W, flag);
+    }
+    return DT_OK;
+}
+
 static inline size_t lds_bytes(int R) {
     const int ch = stage_rows(R);
     return (size_t)(R * R + 2 * (ch + 1) * R) * sizeof(double);

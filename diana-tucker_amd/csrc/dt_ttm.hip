@@ -74,10 +74,17 @@ __device__ __forceinline__ T ld_a(const T *p, bool nt) {
     return nt ? __builtin_nontemporal_load(p) : *p;
 }
 
+// Every MFMA kernel has a fast path for tiles that are fully inside the tensor
+// and a K that is a multiple of the pipelined step: its loads carry no bounds
+// checks and run PD steps ahead of the MFMAs through a ring of registers, so
+// each wave keeps 16 KB of the streamed tensor in flight.  Edge tiles take the
+// guarded loop.
+
 template <int JB, bool NT>
 __global__ __launch_bounds__(256) void ttm_m0_f32(const float *__restrict__ A,
                                                   const float *__restrict__ U,
                                                   float *__restrict__ Y, int K, int64_t B, int J) {
+    constexpr int PD = 2;  // pipeline depth in 32-k steps
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int c = lane & 15;
@@ -91,46 +98,95 @@ __global__ __launch_bounds__(256) void ttm_m0_f32(const float *__restrict__ A,
 #pragma unroll
         for (int cb = 0; cb < 4; cb++) acc[jb][cb] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        float4 a[4][2];
+    // factor operand: clamp the row so the load is unconditional, mask the value
+    int jrow[JB];
+    float jmask[JB];
 #pragma unroll
-        for (int cb = 0; cb < 4; cb++) {
-            const int64_t b = bw + 16 * cb + c;
+    for (int jb = 0; jb < JB; jb++) {
+        const int j = 16 * jb + c;
+        jrow[jb] = j < J ? j : J - 1;
+        jmask[jb] = j < J ? 1.f : 0.f;
+    }
+
+    const bool fast = (bw + 64 <= B) && (K % (32 * PD) == 0);
+    if (fast) {
+        const float *Ab = A + (int64_t)K * (bw + c) + 4 * g;
+        floatx4 a[PD][4][2];
+        floatx4 u[PD][JB][2];
+        auto load = [&](int p, int k0) {
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int k = k0 + 16 * h + 4 * g;
-                if (b < B && k < K) {
-                    const floatx4 v = ld_a((const floatx4 *)(A + k + (int64_t)K * b), NT);
-                    a[cb][h] = make_float4(v[0], v[1], v[2], v[3]);
-                } else {
-                    a[cb][h] = make_float4(0.f, 0.f, 0.f, 0.f);
-                }
-            }
-        }
-        float4 u[JB][2];
+            for (int cb = 0; cb < 4; cb++)
 #pragma unroll
-        for (int jb = 0; jb < JB; jb++) {
-            const int j = 16 * jb + c;
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int k = k0 + 16 * h + 4 * g;
-                if (j < J && k < K)
-                    u[jb][h] = *(const float4 *)(U + k + (int64_t)K * j);
-                else
-                    u[jb][h] = make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-        }
-#pragma unroll
-        for (int h = 0; h < 2; h++)
+                for (int h = 0; h < 2; h++)
+                    a[p][cb][h] = ld_a((const floatx4 *)(Ab + (int64_t)K * 16 * cb + k0 + 16 * h), NT);
 #pragma unroll
             for (int jb = 0; jb < JB; jb++)
 #pragma unroll
-                for (int cb = 0; cb < 4; cb++) {
-                    acc[jb][cb] = mfma_f32_4(u[jb][h].x, a[cb][h].x, acc[jb][cb]);
-                    acc[jb][cb] = mfma_f32_4(u[jb][h].y, a[cb][h].y, acc[jb][cb]);
-                    acc[jb][cb] = mfma_f32_4(u[jb][h].z, a[cb][h].z, acc[jb][cb]);
-                    acc[jb][cb] = mfma_f32_4(u[jb][h].w, a[cb][h].w, acc[jb][cb]);
+                for (int h = 0; h < 2; h++)
+                    u[p][jb][h] = *(const floatx4 *)(U + k0 + 16 * h + 4 * g + (int64_t)K * jrow[jb]) *
+                                  jmask[jb];
+        };
+#pragma unroll
+        for (int p = 0; p < PD; p++) load(p, 32 * p);
+        auto step = [&](int p) {
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int jb = 0; jb < JB; jb++)
+#pragma unroll
+                    for (int cb = 0; cb < 4; cb++) {
+                        acc[jb][cb] = mfma_f32_4(u[p][jb][h][0], a[p][cb][h][0], acc[jb][cb]);
+                        acc[jb][cb] = mfma_f32_4(u[p][jb][h][1], a[p][cb][h][1], acc[jb][cb]);
+                        acc[jb][cb] = mfma_f32_4(u[p][jb][h][2], a[p][cb][h][2], acc[jb][cb]);
+                        acc[jb][cb] = mfma_f32_4(u[p][jb][h][3], a[p][cb][h][3], acc[jb][cb]);
+                    }
+        };
+        // steady state refills unconditionally; the last PD steps only compute
+        int kb = 0;
+        for (; kb + 32 * PD < K; kb += 32 * PD) {
+#pragma unroll
+            for (int p = 0; p < PD; p++) {
+                step(p);
+                load(p, kb + 32 * (p + PD));
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < PD; p++) step(p);
+    } else {
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            floatx4 a[4][2];
+#pragma unroll
+            for (int cb = 0; cb < 4; cb++) {
+                const int64_t b = bw + 16 * cb + c;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int k = k0 + 16 * h + 4 * g;
+                    a[cb][h] = (b < B && k < K) ? ld_a((const floatx4 *)(A + k + (int64_t)K * b), NT)
+                                                : floatx4{0.f, 0.f, 0.f, 0.f};
                 }
+            }
+            floatx4 u[JB][2];
+#pragma unroll
+            for (int jb = 0; jb < JB; jb++) {
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int k = k0 + 16 * h + 4 * g;
+                    u[jb][h] = (k < K) ? *(const floatx4 *)(U + k + (int64_t)K * jrow[jb]) * jmask[jb]
+                                       : floatx4{0.f, 0.f, 0.f, 0.f};
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int jb = 0; jb < JB; jb++)
+#pragma unroll
+                    for (int cb = 0; cb < 4; cb++) {
+                        acc[jb][cb] = mfma_f32_4(u[jb][h][0], a[cb][h][0], acc[jb][cb]);
+                        acc[jb][cb] = mfma_f32_4(u[jb][h][1], a[cb][h][1], acc[jb][cb]);
+                        acc[jb][cb] = mfma_f32_4(u[jb][h][2], a[cb][h][2], acc[jb][cb]);
+                        acc[jb][cb] = mfma_f32_4(u[jb][h][3], a[cb][h][3], acc[jb][cb]);
+                    }
+        }
     }
     // D[i][col]: i = 4g + v (row of the 16-block), col = c.
 #pragma unroll
@@ -161,6 +217,7 @@ __global__ __launch_bounds__(256) void ttm_mk_f32(const float *__restrict__ A,
                                                   const float *__restrict__ M,
                                                   float *__restrict__ Y, int64_t F, int K,
                                                   int64_t B, int J, int64_t tiles_per_b) {
+    constexpr int PD = FB >= 4 ? 4 : 8;  // pipeline depth in 4-k steps
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int c = lane & 15;
@@ -179,35 +236,73 @@ __global__ __launch_bounds__(256) void ttm_mk_f32(const float *__restrict__ A,
 #pragma unroll
             for (int t = 0; t < 4; t++) acc[jb][fb][t] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-#pragma unroll 4
-    for (int k0 = 0; k0 < K; k0 += 4) {
-        const int k = k0 + g;
-        float4 a[FB];
+    int jcol[JB];
+    float jmask[JB];
 #pragma unroll
-        for (int fb = 0; fb < FB; fb++) {
-            const int64_t f = f0 + 64 * fb + 4 * c;
-            if (k < K && f < F) {
-                const floatx4 v = ld_a((const floatx4 *)(Ab + f + F * k), NT);
-                a[fb] = make_float4(v[0], v[1], v[2], v[3]);
-            } else {
-                a[fb] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int jb = 0; jb < JB; jb++) {
+        const int j = 16 * jb + c;
+        jcol[jb] = j < J ? j : J - 1;
+        jmask[jb] = j < J ? 1.f : 0.f;
+    }
+
+    const bool fast = (f0 + 64 * FB <= F) && (K % (4 * PD) == 0);
+    if (fast) {
+        const float *Ap = Ab + f0 + 4 * c + F * g;
+        floatx4 a[PD][FB];
+        float u[PD][JB];
+        auto load = [&](int p, int k0) {
+#pragma unroll
+            for (int fb = 0; fb < FB; fb++)
+                a[p][fb] = ld_a((const floatx4 *)(Ap + 64 * fb + F * (int64_t)k0), NT);
+#pragma unroll
+            for (int jb = 0; jb < JB; jb++) u[p][jb] = M[jcol[jb] + (int64_t)J * (k0 + g)] * jmask[jb];
+        };
+#pragma unroll
+        for (int p = 0; p < PD; p++) load(p, 4 * p);
+        auto step = [&](int p) {
+#pragma unroll
+            for (int jb = 0; jb < JB; jb++)
+#pragma unroll
+                for (int fb = 0; fb < FB; fb++) {
+                    acc[jb][fb][0] = mfma_f32_4(u[p][jb], a[p][fb][0], acc[jb][fb][0]);
+                    acc[jb][fb][1] = mfma_f32_4(u[p][jb], a[p][fb][1], acc[jb][fb][1]);
+                    acc[jb][fb][2] = mfma_f32_4(u[p][jb], a[p][fb][2], acc[jb][fb][2]);
+                    acc[jb][fb][3] = mfma_f32_4(u[p][jb], a[p][fb][3], acc[jb][fb][3]);
+                }
+        };
+        int kb = 0;
+        for (; kb + 4 * PD < K; kb += 4 * PD) {
+#pragma unroll
+            for (int p = 0; p < PD; p++) {
+                step(p);
+                load(p, kb + 4 * (p + PD));
             }
         }
-        float u[JB];
 #pragma unroll
-        for (int jb = 0; jb < JB; jb++) {
-            const int j = 16 * jb + c;
-            u[jb] = (k < K && j < J) ? M[j + (int64_t)J * k] : 0.f;
-        }
-#pragma unroll
-        for (int jb = 0; jb < JB; jb++)
+        for (int p = 0; p < PD; p++) step(p);
+    } else {
+        for (int k0 = 0; k0 < K; k0 += 4) {
+            const int k = k0 + g;
+            floatx4 a[FB];
 #pragma unroll
             for (int fb = 0; fb < FB; fb++) {
-                acc[jb][fb][0] = mfma_f32_4(u[jb], a[fb].x, acc[jb][fb][0]);
-                acc[jb][fb][1] = mfma_f32_4(u[jb], a[fb].y, acc[jb][fb][1]);
-                acc[jb][fb][2] = mfma_f32_4(u[jb], a[fb].z, acc[jb][fb][2]);
-                acc[jb][fb][3] = mfma_f32_4(u[jb], a[fb].w, acc[jb][fb][3]);
+                const int64_t f = f0 + 64 * fb + 4 * c;
+                a[fb] = (k < K && f < F) ? ld_a((const floatx4 *)(Ab + f + F * k), NT)
+                                         : floatx4{0.f, 0.f, 0.f, 0.f};
             }
+            float u[JB];
+#pragma unroll
+            for (int jb = 0; jb < JB; jb++) u[jb] = (k < K) ? M[jcol[jb] + (int64_t)J * k] * jmask[jb] : 0.f;
+#pragma unroll
+            for (int jb = 0; jb < JB; jb++)
+#pragma unroll
+                for (int fb = 0; fb < FB; fb++) {
+                    acc[jb][fb][0] = mfma_f32_4(u[jb], a[fb][0], acc[jb][fb][0]);
+                    acc[jb][fb][1] = mfma_f32_4(u[jb], a[fb][1], acc[jb][fb][1]);
+                    acc[jb][fb][2] = mfma_f32_4(u[jb], a[fb][2], acc[jb][fb][2]);
+                    acc[jb][fb][3] = mfma_f32_4(u[jb], a[fb][3], acc[jb][fb][3]);
+                }
+        }
     }
     // acc[jb][fb][t][v] = Y[f0 + 64fb + 4c + t, 16jb + 4g + v, b]
 #pragma unroll
@@ -235,6 +330,7 @@ template <int JB, bool NT>
 __global__ __launch_bounds__(256) void ttm_m0_bf16(const uint16_t *__restrict__ A,
                                                    const uint16_t *__restrict__ Ub,
                                                    float *__restrict__ Y, int K, int64_t B, int J) {
+    constexpr int PD = 2;  // pipeline depth in 64-k steps
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int c = lane & 15;
@@ -247,36 +343,83 @@ __global__ __launch_bounds__(256) void ttm_m0_bf16(const uint16_t *__restrict__ 
     for (int jb = 0; jb < JB; jb++)
 #pragma unroll
         for (int cb = 0; cb < 4; cb++) acc[jb][cb] = floatx4{0.f, 0.f, 0.f, 0.f};
-
+    int jrow[JB];
+    bool jin[JB];
+#pragma unroll
+    for (int jb = 0; jb < JB; jb++) {
+        const int j = 16 * jb + c;
+        jrow[jb] = j < J ? j : J - 1;
+        jin[jb] = j < J;
+    }
     const u32x4 zero4 = {0u, 0u, 0u, 0u};
-    for (int k0 = 0; k0 < K; k0 += 64) {
-        u32x4 a[4][2];
+    const bool fast = (bw + 64 <= B) && (K % (64 * PD) == 0);
+    if (fast) {
+        const uint16_t *Ab = A + (int64_t)K * (bw + c) + 8 * g;
+        u32x4 a[PD][4][2], u[PD][JB][2];
+        auto load = [&](int p, int k0) {
 #pragma unroll
-        for (int cb = 0; cb < 4; cb++) {
-            const int64_t b = bw + 16 * cb + c;
+            for (int cb = 0; cb < 4; cb++)
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int k = k0 + 32 * h + 8 * g;
-                a[cb][h] = (b < B && k < K) ? ld_a((const u32x4 *)(A + k + (int64_t)K * b), NT) : zero4;
-            }
-        }
-        u32x4 u[JB][2];
-#pragma unroll
-        for (int jb = 0; jb < JB; jb++) {
-            const int j = 16 * jb + c;
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int k = k0 + 32 * h + 8 * g;
-                u[jb][h] = (j < J && k < K) ? *(const u32x4 *)(Ub + k + (int64_t)K * j) : zero4;
-            }
-        }
-#pragma unroll
-        for (int h = 0; h < 2; h++)
+                for (int h = 0; h < 2; h++)
+                    a[p][cb][h] = ld_a((const u32x4 *)(Ab + (int64_t)K * 16 * cb + k0 + 32 * h), NT);
 #pragma unroll
             for (int jb = 0; jb < JB; jb++)
 #pragma unroll
-                for (int cb = 0; cb < 4; cb++)
-                    acc[jb][cb] = mfma_bf16_32(u[jb][h], a[cb][h], acc[jb][cb]);
+                for (int h = 0; h < 2; h++) {
+                    const u32x4 v = *(const u32x4 *)(Ub + k0 + 32 * h + 8 * g + (int64_t)K * jrow[jb]);
+                    u[p][jb][h] = jin[jb] ? v : zero4;
+                }
+        };
+#pragma unroll
+        for (int p = 0; p < PD; p++) load(p, 64 * p);
+        auto step = [&](int p) {
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int jb = 0; jb < JB; jb++)
+#pragma unroll
+                    for (int cb = 0; cb < 4; cb++)
+                        acc[jb][cb] = mfma_bf16_32(u[p][jb][h], a[p][cb][h], acc[jb][cb]);
+        };
+        int kb = 0;
+        for (; kb + 64 * PD < K; kb += 64 * PD) {
+#pragma unroll
+            for (int p = 0; p < PD; p++) {
+                step(p);
+                load(p, kb + 64 * (p + PD));
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < PD; p++) step(p);
+    } else {
+        for (int k0 = 0; k0 < K; k0 += 64) {
+            u32x4 a[4][2];
+#pragma unroll
+            for (int cb = 0; cb < 4; cb++) {
+                const int64_t b = bw + 16 * cb + c;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int k = k0 + 32 * h + 8 * g;
+                    a[cb][h] = (b < B && k < K) ? ld_a((const u32x4 *)(A + k + (int64_t)K * b), NT) : zero4;
+                }
+            }
+            u32x4 u[JB][2];
+#pragma unroll
+            for (int jb = 0; jb < JB; jb++) {
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int k = k0 + 32 * h + 8 * g;
+                    u[jb][h] = (jin[jb] && k < K) ? *(const u32x4 *)(Ub + k + (int64_t)K * jrow[jb]) : zero4;
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int jb = 0; jb < JB; jb++)
+#pragma unroll
+                    for (int cb = 0; cb < 4; cb++)
+                        acc[jb][cb] = mfma_bf16_32(u[jb][h], a[cb][h], acc[jb][cb]);
+        }
     }
 #pragma unroll
     for (int cb = 0; cb < 4; cb++) {
@@ -303,11 +446,28 @@ __global__ __launch_bounds__(256) void ttm_m0_bf16(const uint16_t *__restrict__ 
 // Each lane loads 8 k-rows of 8 f-contiguous bf16 and transposes the 8x8 block
 // in registers so that MFMA t receives the 8 k values of f = f0 + 8c + t.
 // Requires F % 8 == 0 and K % 8 == 0.
+template <int JB>
+__device__ __forceinline__ void mk_bf16_step(const u32x4 (&r)[8], const u32x4 (&u)[JB],
+                                             floatx4 (&acc)[JB][8]) {
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const uint32_t sel = (t & 1) ? 0x07060302u : 0x05040100u;
+        u32x4 bv;
+        bv.x = __builtin_amdgcn_perm(r[1][t >> 1], r[0][t >> 1], sel);
+        bv.y = __builtin_amdgcn_perm(r[3][t >> 1], r[2][t >> 1], sel);
+        bv.z = __builtin_amdgcn_perm(r[5][t >> 1], r[4][t >> 1], sel);
+        bv.w = __builtin_amdgcn_perm(r[7][t >> 1], r[6][t >> 1], sel);
+#pragma unroll
+        for (int jb = 0; jb < JB; jb++) acc[jb][t] = mfma_bf16_32(u[jb], bv, acc[jb][t]);
+    }
+}
+
 template <int JB, bool NT>
 __global__ __launch_bounds__(256) void ttm_mk_bf16(const uint16_t *__restrict__ A,
                                                    const uint16_t *__restrict__ Ub,
                                                    float *__restrict__ Y, int64_t F, int K,
                                                    int64_t B, int J, int64_t tiles_per_b) {
+    constexpr int PD = 2;  // pipeline depth in 32-k steps
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int c = lane & 15;
@@ -324,31 +484,54 @@ __global__ __launch_bounds__(256) void ttm_mk_bf16(const uint16_t *__restrict__ 
     for (int jb = 0; jb < JB; jb++)
 #pragma unroll
         for (int t = 0; t < 8; t++) acc[jb][t] = floatx4{0.f, 0.f, 0.f, 0.f};
-
+    int jrow[JB];
+    bool jin[JB];
+#pragma unroll
+    for (int jb = 0; jb < JB; jb++) {
+        const int j = 16 * jb + c;
+        jrow[jb] = j < J ? j : J - 1;
+        jin[jb] = j < J;
+    }
     const u32x4 zero4 = {0u, 0u, 0u, 0u};
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        const int kb = k0 + 8 * g;
-        const bool kin = kb < K;
-        u32x4 r[8];
+    const int64_t ftile0 = (w - b * tiles_per_b) * 128;
+    const bool fast = (ftile0 + 128 <= F) && (K % (32 * PD) == 0);
+    if (fast) {
+        const uint16_t *Ap = Ab + f + F * (int64_t)(8 * g);
+        u32x4 r[PD][8], u[PD][JB];
+        auto load = [&](int p, int k0) {
 #pragma unroll
-        for (int e = 0; e < 8; e++)
-            r[e] = (kin && fin) ? ld_a((const u32x4 *)(Ab + f + F * (int64_t)(kb + e)), NT) : zero4;
-        u32x4 u[JB];
+            for (int e = 0; e < 8; e++) r[p][e] = ld_a((const u32x4 *)(Ap + F * (int64_t)(k0 + e)), NT);
 #pragma unroll
-        for (int jb = 0; jb < JB; jb++) {
-            const int j = 16 * jb + c;
-            u[jb] = (kin && j < J) ? *(const u32x4 *)(Ub + kb + (int64_t)K * j) : zero4;
+            for (int jb = 0; jb < JB; jb++) {
+                const u32x4 v = *(const u32x4 *)(Ub + k0 + 8 * g + (int64_t)K * jrow[jb]);
+                u[p][jb] = jin[jb] ? v : zero4;
+            }
+        };
+#pragma unroll
+        for (int p = 0; p < PD; p++) load(p, 32 * p);
+        int kb = 0;
+        for (; kb + 32 * PD < K; kb += 32 * PD) {
+#pragma unroll
+            for (int p = 0; p < PD; p++) {
+                mk_bf16_step<JB>(r[p], u[p], acc);
+                load(p, kb + 32 * (p + PD));
+            }
         }
 #pragma unroll
-        for (int t = 0; t < 8; t++) {
-            const uint32_t sel = (t & 1) ? 0x07060302u : 0x05040100u;
-            u32x4 bv;
-            bv.x = __builtin_amdgcn_perm(r[1][t >> 1], r[0][t >> 1], sel);
-            bv.y = __builtin_amdgcn_perm(r[3][t >> 1], r[2][t >> 1], sel);
-            bv.z = __builtin_amdgcn_perm(r[5][t >> 1], r[4][t >> 1], sel);
-            bv.w = __builtin_amdgcn_perm(r[7][t >> 1], r[6][t >> 1], sel);
+        for (int p = 0; p < PD; p++) mk_bf16_step<JB>(r[p], u[p], acc);
+    } else {
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            const int kb = k0 + 8 * g;
+            const bool kin = kb < K;
+            u32x4 r[8];
 #pragma unroll
-            for (int jb = 0; jb < JB; jb++) acc[jb][t] = mfma_bf16_32(u[jb], bv, acc[jb][t]);
+            for (int e = 0; e < 8; e++)
+                r[e] = (kin && fin) ? ld_a((const u32x4 *)(Ab + f + F * (int64_t)(kb + e)), NT) : zero4;
+            u32x4 u[JB];
+#pragma unroll
+            for (int jb = 0; jb < JB; jb++)
+                u[jb] = (kin && jin[jb]) ? *(const u32x4 *)(Ub + kb + (int64_t)K * jrow[jb]) : zero4;
+            mk_bf16_step<JB>(r, u, acc);
         }
     }
     if (!fin) return;

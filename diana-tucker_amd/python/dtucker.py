@@ -19,7 +19,7 @@ DT_F64, DT_F32, DT_BF16, DT_I32 = range(4)
 DT_MAX_NDIM = 8
 INIT_GIVEN, INIT_QR, INIT_HOSVD, INIT_RANDOM, INIT_LEARNED = range(5)
 SCHED_REFERENCE, SCHED_TREE = 0, 1
-FLAG_DIRECT_RESIDUAL, FLAG_FP32_POLISH, FLAG_FP64_FIT = 1, 2, 4
+FLAG_DIRECT_RESIDUAL, FLAG_FP32_POLISH, FLAG_FP64_FIT, FLAG_REFERENCE_ALS = 1, 2, 4, 8
 
 _NP_OF = {DT_F64: np.float64, DT_F32: np.float32, DT_BF16: np.uint16, DT_I32: np.int32}
 _SIZE_OF = {DT_F64: 8, DT_F32: 4, DT_BF16: 2, DT_I32: 4}
@@ -74,7 +74,7 @@ def lib():
             "dt_h2d": [vp, vp, vp, sz], "dt_d2h": [vp, vp, vp, sz], "dt_d2d": [vp, vp, vp, sz],
             "dt_zero": [vp, vp, sz], "dt_sync": [vp],
             "dt_convert": [vp, vp, i32, vp, i32, sz],
-            "dt_timing_enable": [vp, i32], "dt_timing_reset": [vp],
+            "dt_timing_enable": [vp, i32], "dt_timing_reset": [vp], "dt_timing_filter": [vp, C.c_char_p],
             "dt_timing_get": [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(dbl)],
             "dt_timing_list": [vp, C.c_char_p, sz],
             "dt_timer_start": [vp], "dt_timer_stop": [vp, C.POINTER(dbl)],
@@ -109,7 +109,7 @@ def exported_symbols():
     """Names this binding expects the library to export."""
     return ["dt_create", "dt_destroy", "dt_last_error", "dt_malloc", "dt_free", "dt_h2d",
             "dt_d2h", "dt_d2d", "dt_zero", "dt_sync", "dt_convert", "dt_timing_enable",
-            "dt_timing_reset", "dt_timing_get", "dt_timing_list", "dt_timer_start",
+            "dt_timing_reset", "dt_timing_filter", "dt_timing_get", "dt_timing_list", "dt_timer_start",
             "dt_timer_stop", "dt_ttm", "dt_gram", "dt_ttt_except", "dt_sumsq", "dt_gemm_f64",
             "dt_qr_f64", "dt_inverse_f64", "dt_syev_top_f64", "dt_orth_rows_f64",
             "dt_gen_normal", "dt_gen_uniform", "dt_gen_tucker", "dt_hooi_dense",
@@ -223,6 +223,9 @@ class Context:
     # timing
     def timing(self, on=True):
         self.check(self.L.dt_timing_enable(self.handle, 1 if on else 0))
+
+    def timing_filter(self, prefix: str = ""):
+        self.check(self.L.dt_timing_filter(self.handle, prefix.encode()))
 
     def timing_reset(self):
         self.check(self.L.dt_timing_reset(self.handle))

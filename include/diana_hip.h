@@ -75,6 +75,9 @@ int dt_convert(dt_ctx *ctx, void *dst, int dt_dst, const void *src, int dt_src, 
  * MPI_Wtime-based Summary timers (src/summary.cpp:29-69). */
 int dt_timing_enable(dt_ctx *ctx, int on);
 int dt_timing_reset(dt_ctx *ctx);
+/* Only launches whose kernel name starts with `prefix` are bracketed (NULL or
+ * "" = all); keeps the event overhead off the kernels that are not measured. */
+int dt_timing_filter(dt_ctx *ctx, const char *prefix);
 int dt_timing_get(dt_ctx *ctx, const char *kernel, int64_t *count, double *total_ms);
 int dt_timing_list(dt_ctx *ctx, char *buf, size_t buflen); /* "name count ms\n" lines */
 int dt_timer_start(dt_ctx *ctx);
@@ -171,6 +174,9 @@ enum {
 enum {
     DT_FLAG_DIRECT_RESIDUAL = 1, /* after the last sweep compute ||A - [[G;U]]|| in FP64 */
     DT_FLAG_FP32_POLISH = 2,     /* mixed precision: finish with one FP32 sweep on A_f32 */
+    DT_FLAG_REFERENCE_ALS = 8,   /* per-mode update exactly as ALS_ writes it (LU inverses
+                                    of LᵀL and PᵀSP); default: the span-equivalent
+                                    Cholesky-normalised iteration */
     DT_FLAG_FP64_FIT = 4         /* form the core for the residual with FP64 accumulation
                                     (one extra pass over A per sweep); needed when the
                                     residual is below ~1e-3, where FP32 rounding of the

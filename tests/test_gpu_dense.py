@@ -216,9 +216,11 @@ def run_gpu_hooi(ctx, A, ranks, sweeps, init, U_raw=None, schedule=dt.SCHED_TREE
                           flags=flags)
 
 
-@pytest.mark.parametrize("schedule,flags", [(dt.SCHED_REFERENCE, 0), (dt.SCHED_TREE, 0),
-                                            (dt.SCHED_REFERENCE, dt.FLAG_FP64_FIT),
-                                            (dt.SCHED_TREE, dt.FLAG_FP64_FIT)])
+@pytest.mark.parametrize("schedule,flags", [
+    (dt.SCHED_REFERENCE, 0), (dt.SCHED_TREE, 0),
+    (dt.SCHED_REFERENCE, dt.FLAG_FP64_FIT), (dt.SCHED_TREE, dt.FLAG_FP64_FIT),
+    (dt.SCHED_REFERENCE, dt.FLAG_FP64_FIT | dt.FLAG_REFERENCE_ALS),
+    (dt.SCHED_TREE, dt.FLAG_FP64_FIT | dt.FLAG_REFERENCE_ALS)])
 def test_hooi_c1_reference_random_init(ctx, schedule, flags):
     """C1: 64^3, core 8^3, noise 1e-3, ranks 8, 10 sweeps, the reference's own
     random start (srand(20000905), Util::randn, reduced_QR)."""
