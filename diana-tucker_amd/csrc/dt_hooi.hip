@@ -284,7 +284,25 @@ static int sweep_once(dt_hooi *h, const void *A, int dt) {
         const void *TR;
         int trdt;
         std::vector<int> ldesc(left.rbegin(), left.rend());
-        DT_TRY(chain(h, A, dt, sh, ldesc, h->buf[0], h->buf[1], &TR, &trdt));
+        int fused = DT_EUNSUPPORTED;
+        if (hm >= 2 && h->work_dt == DT_F32 && (dt == DT_F32 || dt == DT_BF16)) {
+            // modes 0 and 1 in one pass: no intermediate of size |A| R_1 / I_1
+            int64_t B = 1;
+            for (int i = 2; i < N; i++) B *= sh[i];
+            const void *U0 = dt == DT_BF16 ? (const void *)h->Ub16[0] : (const void *)h->U32[0];
+            fused = dt_ttm2_lead(h->ctx, A, dt, sh[0], sh[1], B, U0, h->M32[1], h->R[0], h->R[1],
+                                 (float *)h->buf[0]);
+            if (fused != DT_OK && fused != DT_EUNSUPPORTED) return fused;
+        }
+        if (fused == DT_OK) {
+            if (dt == DT_BF16) h->bf16_core_mode = 0;
+            sh[0] = h->R[0];
+            sh[1] = h->R[1];
+            std::vector<int> rest(ldesc.begin(), ldesc.end() - 2);  // modes hm-1 .. 2
+            DT_TRY(chain(h, h->buf[0], DT_F32, sh, rest, h->buf[1], h->buf[0], &TR, &trdt));
+        } else {
+            DT_TRY(chain(h, A, dt, sh, ldesc, h->buf[0], h->buf[1], &TR, &trdt));
+        }
         int64_t shR[DT_MAX_NDIM];
         const void *Ylast = nullptr;
         int ylastdt = 0;

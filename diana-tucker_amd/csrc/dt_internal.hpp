@@ -63,6 +63,9 @@ enum dt_ws_slot {
     WS_TTT,            // ttt_except staging
     WS_GEN,            // generator scratch
     WS_GEN2,
+    WS_TTMC0,          // dt_ttmc intermediates
+    WS_TTMC1,
+    WS_TTMC_M,         // dt_ttmc fused-pair factor views
     WS_SPARSE0,
     WS_SPARSE1,
     WS_SPARSE2,
@@ -193,6 +196,11 @@ int dt_als_update(dt_ctx *ctx, const double *S, double *U, int I, int R, int inn
                   int *flag, float *U32, float *M32, uint16_t *Ub16);
 int dt_als_fused(dt_ctx *ctx, const double *S, double *U, int I, int R, int inner, double *L,
                  double *P, double *W, int *flag);
+// T (J0 x J1 x B) = A (K0 x K1 x B) x_0 U0ᵀ x_1 U1ᵀ in one pass over A.
+// U0: K0 x J0 (FP32, or bf16 for a BF16 A); M1: J1 x K1 FP32.  Returns
+// DT_EUNSUPPORTED when the shape does not fit the fused kernel.
+int dt_ttm2_lead(dt_ctx *ctx, const void *A, int dtA, int64_t K0, int64_t K1, int64_t B, const void *U0,
+                 const float *M1, int64_t J0, int64_t J1, float *T);
 int dt_ttm_f64acc(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t K, int64_t B,
                   const double *M64, int64_t J, double *Y);
 int dt_dot_f64_dev(dt_ctx *ctx, const double *a, const double *b, size_t n, double *out_dev);

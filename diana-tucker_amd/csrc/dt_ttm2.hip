@@ -34,137 +34,66 @@ __device__ __forceinline__ T nt_load(const T *p) {
 
 constexpr int L2_YS = 68;  // LDS row stride (floats) of the J0 x 64 stage-1 block
 
-// Stage 1 for one 64-column block of the slab, FP32 input.
-// Ab: slab base + K0 * (column block start); returns acc[jb][cb] with
-// D[j0 = 16 jb + 4g + v][i1 = 16 cb + c].
-template <int JB>
-__device__ __forceinline__ void lead_stage1_f32(const float *__restrict__ Ab, const float *__restrict__ U0,
-                                                int K0, int J0, floatx4 (&acc)[JB][4], int c, int g) {
-    constexpr int PD = 2;
-    int jrow[JB];
-    float jmask[JB];
-#pragma unroll
-    for (int jb = 0; jb < JB; jb++) {
-        const int j = 16 * jb + c;
-        jrow[jb] = j < J0 ? j : J0 - 1;
-        jmask[jb] = j < J0 ? 1.f : 0.f;
-    }
-    const float *Ap = Ab + (int64_t)K0 * c + 4 * g;
-    floatx4 a[PD][4][2], u[PD][JB][2];
-    auto load = [&](int p, int k0) {
-#pragma unroll
-        for (int cb = 0; cb < 4; cb++)
-#pragma unroll
-            for (int h = 0; h < 2; h++)
-                a[p][cb][h] = nt_load((const floatx4 *)(Ap + (int64_t)K0 * 16 * cb + k0 + 16 * h));
-#pragma unroll
-        for (int jb = 0; jb < JB; jb++)
-#pragma unroll
-            for (int h = 0; h < 2; h++)
-                u[p][jb][h] = *(const floatx4 *)(U0 + k0 + 16 * h + 4 * g + (int64_t)K0 * jrow[jb]) * jmask[jb];
-    };
-    auto step = [&](int p) {
-#pragma unroll
-        for (int h = 0; h < 2; h++)
-#pragma unroll
-            for (int jb = 0; jb < JB; jb++)
-#pragma unroll
-                for (int cb = 0; cb < 4; cb++) {
-                    acc[jb][cb] = mfma4(u[p][jb][h][0], a[p][cb][h][0], acc[jb][cb]);
-                    acc[jb][cb] = mfma4(u[p][jb][h][1], a[p][cb][h][1], acc[jb][cb]);
-                    acc[jb][cb] = mfma4(u[p][jb][h][2], a[p][cb][h][2], acc[jb][cb]);
-                    acc[jb][cb] = mfma4(u[p][jb][h][3], a[p][cb][h][3], acc[jb][cb]);
-                }
-    };
-#pragma unroll
-    for (int p = 0; p < PD; p++) load(p, 32 * p);
-    int kb = 0;
-    for (; kb + 32 * PD < K0; kb += 32 * PD) {
-#pragma unroll
-        for (int p = 0; p < PD; p++) {
-            step(p);
-            load(p, kb + 32 * (p + PD));
-        }
-    }
-#pragma unroll
-    for (int p = 0; p < PD; p++) step(p);
-}
-
-// Stage 1, BF16 input (v_mfma_f32_16x16x32_bf16), U0 in bf16 K0 x J0.
-template <int JB>
-__device__ __forceinline__ void lead_stage1_bf16(const uint16_t *__restrict__ Ab,
-                                                 const uint16_t *__restrict__ U0, int K0, int J0,
-                                                 floatx4 (&acc)[JB][4], int c, int g) {
-    constexpr int PD = 2;
-    int jrow[JB];
-    bool jin[JB];
-#pragma unroll
-    for (int jb = 0; jb < JB; jb++) {
-        const int j = 16 * jb + c;
-        jrow[jb] = j < J0 ? j : J0 - 1;
-        jin[jb] = j < J0;
-    }
-    const u32x4 zero4 = {0u, 0u, 0u, 0u};
-    const uint16_t *Ap = Ab + (int64_t)K0 * c + 8 * g;
-    u32x4 a[PD][4][2], u[PD][JB][2];
-    auto load = [&](int p, int k0) {
-#pragma unroll
-        for (int cb = 0; cb < 4; cb++)
-#pragma unroll
-            for (int h = 0; h < 2; h++)
-                a[p][cb][h] = nt_load((const u32x4 *)(Ap + (int64_t)K0 * 16 * cb + k0 + 32 * h));
-#pragma unroll
-        for (int jb = 0; jb < JB; jb++)
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const u32x4 v = *(const u32x4 *)(U0 + k0 + 32 * h + 8 * g + (int64_t)K0 * jrow[jb]);
-                u[p][jb][h] = jin[jb] ? v : zero4;
-            }
-    };
-    auto step = [&](int p) {
-#pragma unroll
-        for (int h = 0; h < 2; h++)
-#pragma unroll
-            for (int jb = 0; jb < JB; jb++)
-#pragma unroll
-                for (int cb = 0; cb < 4; cb++) acc[jb][cb] = mfma_bf(u[p][jb][h], a[p][cb][h], acc[jb][cb]);
-    };
-#pragma unroll
-    for (int p = 0; p < PD; p++) load(p, 64 * p);
-    int kb = 0;
-    for (; kb + 64 * PD < K0; kb += 64 * PD) {
-#pragma unroll
-        for (int p = 0; p < PD; p++) {
-            step(p);
-            load(p, kb + 64 * (p + PD));
-        }
-    }
-#pragma unroll
-    for (int p = 0; p < PD; p++) step(p);
-}
+// Per-type stage-1 operand traffic.  One "step" covers KSTEP consecutive i0
+// of 64 columns i1: lane (c, g) holds 2 x 16 bytes of A for each of the four
+// 16-column groups and 2 x 16 bytes of U0 for each 16-row group of j0.
+template <bool BF16>
+struct LeadTraits;
+template <>
+struct LeadTraits<false> {
+    static constexpr int KSTEP = 32;
+    typedef floatx4 vec;
+    typedef float elem;
+    static constexpr int LANE_K = 4;  // elements per lane per 16-byte load
+};
+template <>
+struct LeadTraits<true> {
+    static constexpr int KSTEP = 64;
+    typedef u32x4 vec;
+    typedef uint16_t elem;
+    static constexpr int LANE_K = 8;
+};
 
 // Requirements (checked by the host): K1 % 64 == 0, K0 % 64 (FP32) or
 // K0 % 128 (BF16) == 0, J0 <= 16 * JB0 <= 32, J1 <= 16 * JB1 <= 64.
+//
+// Every wave works alone on whole slabs b = wave id, wave id + #waves, ...
+// A slab is processed as K1 / 64 units of 64 columns; a unit is K0 / KSTEP
+// steps.  The loads run PD steps ahead of the MFMAs on one continuous
+// cursor over (slab, unit, step), so the next unit's first loads are in
+// flight while a unit's stage-1 block goes through LDS into stage 2, and no
+// workgroup barrier is ever needed.
 template <int JB0, int JB1, bool BF16>
 __global__ __launch_bounds__(256) void ttm2_lead(const void *__restrict__ Av, const void *__restrict__ U0v,
-                                                 const float *__restrict__ M1, float *__restrict__ T,
-                                                 int K0, int K1, int J0, int J1) {
-    // stage-1 blocks per wave: [4][16 JB0][L2_YS]; reused for the final
-    // cross-wave reduction [4][16 JB0 * 16 JB1] (fits: JB1 <= 4 < L2_YS / 16)
-    __shared__ float smem[4 * 16 * JB0 * L2_YS];
-    float (*Ys)[16 * JB0][L2_YS] = (float (*)[16 * JB0][L2_YS])smem;
+                                                 const float *__restrict__ M1, float *__restrict__ T, int K0,
+                                                 int K1, int J0, int J1, int64_t B) {
+    typedef LeadTraits<BF16> Tr;
+    typedef typename Tr::vec vec;
+    typedef typename Tr::elem elem;
+    constexpr int PD = 2;
+    constexpr int KSTEP = Tr::KSTEP;
+    __shared__ float Ysm[4][16 * JB0][L2_YS];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int c = lane & 15;
     const int g = lane >> 4;
-    const int64_t b = blockIdx.x;
+    const int64_t nw = (int64_t)gridDim.x * 4;
+    const int64_t w0 = (int64_t)blockIdx.x * 4 + wave;
+    if (w0 >= B) return;  // waves never synchronise with each other
+    float(*Ys)[L2_YS] = Ysm[wave];
+    const elem *A = (const elem *)Av;
+    const elem *U0 = (const elem *)U0v;
     const int64_t slab = (int64_t)K0 * K1;
+    const int units = K1 / 64;
 
-    floatx4 acc2[JB0][JB1];
+    int jrow[JB0];
+    bool jin[JB0];
 #pragma unroll
-    for (int i = 0; i < JB0; i++)
-#pragma unroll
-        for (int j = 0; j < JB1; j++) acc2[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int jb = 0; jb < JB0; jb++) {
+        const int j = 16 * jb + c;
+        jrow[jb] = j < J0 ? j : J0 - 1;
+        jin[jb] = j < J0;
+    }
     int j1col[JB1];
     float j1mask[JB1];
 #pragma unroll
@@ -173,64 +102,145 @@ __global__ __launch_bounds__(256) void ttm2_lead(const void *__restrict__ Av, co
         j1col[jb] = j < J1 ? j : J1 - 1;
         j1mask[jb] = j < J1 ? 1.f : 0.f;
     }
+    // this lane's U0 row offsets (k added per step)
+    const elem *Ul[JB0];
+#pragma unroll
+    for (int jb = 0; jb < JB0; jb++) Ul[jb] = U0 + Tr::LANE_K * g + (int64_t)K0 * jrow[jb];
 
-    for (int cbase = 64 * wave; cbase < K1; cbase += 256) {
-        floatx4 acc[JB0][4];
-#pragma unroll
-        for (int jb = 0; jb < JB0; jb++)
-#pragma unroll
-            for (int cb = 0; cb < 4; cb++) acc[jb][cb] = floatx4{0.f, 0.f, 0.f, 0.f};
-        if (BF16)
-            lead_stage1_bf16<JB0>((const uint16_t *)Av + slab * b + (int64_t)K0 * cbase,
-                                  (const uint16_t *)U0v, K0, J0, acc, c, g);
-        else
-            lead_stage1_f32<JB0>((const float *)Av + slab * b + (int64_t)K0 * cbase, (const float *)U0v,
-                                 K0, J0, acc, c, g);
-        // stage-1 block -> LDS as Ys[j0][i1 - cbase]
-#pragma unroll
-        for (int jb = 0; jb < JB0; jb++)
+    vec a[PD][4][2], u[PD][JB0][2];
+    // load cursor: slab lb, unit lu, step offset lk
+    int64_t lb = w0;
+    int lu = 0, lk = 0;
+    const elem *lbase = A + slab * lb + (int64_t)K0 * c + Tr::LANE_K * g;  // + K0 * 64 lu
+    auto issue = [&](int p) {
+        if (lb < B) {
+            const elem *Ap = lbase + (int64_t)K0 * 64 * lu + lk;
 #pragma unroll
             for (int cb = 0; cb < 4; cb++)
 #pragma unroll
-                for (int v = 0; v < 4; v++) Ys[wave][16 * jb + 4 * g + v][16 * cb + c] = acc[jb][cb][v];
-        // the block is read back by other lanes of the same wave only
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // stage 2: acc2[j0][j1] += sum_i1 Ys[j0][i1] M1[j1, i1]
-#pragma unroll 4
-        for (int ks = 0; ks < 64; ks += 4) {
-            float a[JB0], m[JB1];
+                for (int h = 0; h < 2; h++)
+                    a[p][cb][h] = nt_load((const vec *)(Ap + (int64_t)K0 * 16 * cb + (KSTEP / 2) * h));
 #pragma unroll
-            for (int jb = 0; jb < JB0; jb++) a[jb] = Ys[wave][16 * jb + c][ks + g];
+            for (int jb = 0; jb < JB0; jb++)
 #pragma unroll
-            for (int jb = 0; jb < JB1; jb++) m[jb] = M1[j1col[jb] + (int64_t)J1 * (cbase + ks + g)] * j1mask[jb];
-#pragma unroll
-            for (int i = 0; i < JB0; i++)
-#pragma unroll
-                for (int j = 0; j < JB1; j++) acc2[i][j] = mfma4(a[i], m[j], acc2[i][j]);
+                for (int h = 0; h < 2; h++) {
+                    vec v = *(const vec *)(Ul[jb] + lk + (KSTEP / 2) * h);
+                    if constexpr (BF16) {
+                        const vec z = {0u, 0u, 0u, 0u};
+                        u[p][jb][h] = jin[jb] ? v : z;
+                    } else {
+                        u[p][jb][h] = v * (jin[jb] ? 1.f : 0.f);
+                    }
+                }
         }
-        // all lanes' reads of this block precede the next block's writes
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        lk += KSTEP;
+        if (lk == K0) {
+            lk = 0;
+            if (++lu == units) {
+                lu = 0;
+                lb += nw;
+                lbase += slab * nw;
+            }
+        }
+    };
+    floatx4 acc[JB0][4];
+    auto step = [&](int p) {
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int jb = 0; jb < JB0; jb++)
+#pragma unroll
+                for (int cb = 0; cb < 4; cb++) {
+                    if constexpr (BF16) {
+                        acc[jb][cb] = mfma_bf(u[p][jb][h], a[p][cb][h], acc[jb][cb]);
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 4; q++)
+                            acc[jb][cb] = mfma4(u[p][jb][h][q], a[p][cb][h][q], acc[jb][cb]);
+                    }
+                }
+    };
+
+#pragma unroll
+    for (int p = 0; p < PD; p++) issue(p);
+    for (int64_t b = w0; b < B; b += nw) {
+        floatx4 acc2[JB0][JB1];
+#pragma unroll
+        for (int i = 0; i < JB0; i++)
+#pragma unroll
+            for (int j = 0; j < JB1; j++) acc2[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int un = 0; un < units; un++) {
+            const int cbase = 64 * un;
+            // this unit's stage-2 factor values, fetched while stage 1 runs
+            float m[16][JB1];
+#pragma unroll
+            for (int q = 0; q < 16; q++)
+#pragma unroll
+                for (int jb = 0; jb < JB1; jb++) m[q][jb] = M1[j1col[jb] + (int64_t)J1 * (cbase + 4 * q + g)];
+#pragma unroll
+            for (int jb = 0; jb < JB0; jb++)
+#pragma unroll
+                for (int cb = 0; cb < 4; cb++) acc[jb][cb] = floatx4{0.f, 0.f, 0.f, 0.f};
+            for (int ks = 0; ks < K0; ks += KSTEP * PD) {
+#pragma unroll
+                for (int p = 0; p < PD; p++) {
+                    step(p);
+                    issue(p);
+                }
+            }
+            // stage-1 block -> LDS as Ys[j0][i1 - cbase] (read back by this wave only)
+#pragma unroll
+            for (int jb = 0; jb < JB0; jb++)
+#pragma unroll
+                for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) Ys[16 * jb + 4 * g + v][16 * cb + c] = acc[jb][cb][v];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // stage 2: acc2[j0][j1] += sum_i1 Ys[j0][i1] M1[j1, i1]
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                float y[JB0];
+#pragma unroll
+                for (int jb = 0; jb < JB0; jb++) y[jb] = Ys[16 * jb + c][4 * q + g];
+#pragma unroll
+                for (int i = 0; i < JB0; i++)
+#pragma unroll
+                    for (int j = 0; j < JB1; j++) acc2[i][j] = mfma4(y[i], m[q][j] * j1mask[j], acc2[i][j]);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        // acc2[i][j][v] = T[j0 = 16 i + 4 g + v][j1 = 16 j + c]
+        float *Tb = T + (int64_t)J0 * J1 * b;
+#pragma unroll
+        for (int i = 0; i < JB0; i++)
+#pragma unroll
+            for (int j = 0; j < JB1; j++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int j0 = 16 * i + 4 * g + v, j1 = 16 * j + c;
+                    if (j0 < J0 && j1 < J1) Tb[j0 + (int64_t)J0 * j1] = acc2[i][j][v];
+                }
     }
-    __syncthreads();  // every wave is done with its Ys before the space is reused
-    float (*red)[16 * JB0 * 16 * JB1] = (float (*)[16 * JB0 * 16 * JB1])smem;
-    // reduce the four waves' partials: red[w][j0 + J0p * j1]
-    constexpr int J0p = 16 * JB0;
-#pragma unroll
-    for (int i = 0; i < JB0; i++)
-#pragma unroll
-        for (int j = 0; j < JB1; j++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) red[wave][(16 * i + 4 * g + v) + J0p * (16 * j + c)] = acc2[i][j][v];
-    __syncthreads();
-    for (int e = threadIdx.x; e < J0 * J1; e += 256) {
-        const int j0 = e % J0, j1 = e / J0;
-        const int o = j0 + J0p * j1;
-        T[e + (int64_t)J0 * J1 * b] = (red[0][o] + red[1][o]) + (red[2][o] + red[3][o]);
-    }
+}
+
+// waves resident per CU for a kernel at 256 threads, times the CU count
+template <typename K>
+static int64_t resident_waves(K kernel) {
+    int dev = 0, cus = 0, blocks = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, 256, 0) != hipSuccess) return 0;
+    return (int64_t)cus * blocks * 4;
+}
+
+static bool ttm2_lead_fits(int dtA, int64_t K0, int64_t K1, int64_t B, int64_t J0, int64_t J1) {
+    if (dtA != DT_F32 && dtA != DT_BF16) return false;
+    return K0 > 0 && K1 > 0 && K1 % 64 == 0 && K0 % (dtA == DT_BF16 ? 128 : 64) == 0 && J0 >= 1 &&
+           J0 <= 32 && J1 >= 1 && J1 <= 64 && B >= 1 && K0 <= 0x7fffffff && K1 <= 0x7fffffff;
 }
 
 // T (J0 x J1 x B) = A (K0 x K1 x B) x_0 U0ᵀ x_1 U1ᵀ.  U0 given as K0 x J0
@@ -239,25 +249,128 @@ __global__ __launch_bounds__(256) void ttm2_lead(const void *__restrict__ Av, co
 // runs two single products).
 int dt_ttm2_lead(dt_ctx *ctx, const void *A, int dtA, int64_t K0, int64_t K1, int64_t B,
                  const void *U0, const float *M1, int64_t J0, int64_t J1, float *T) {
+    if (!ttm2_lead_fits(dtA, K0, K1, B, J0, J1)) return DT_EUNSUPPORTED;
     const bool bf = dtA == DT_BF16;
-    if (!(dtA == DT_F32 || bf)) return DT_EUNSUPPORTED;
-    if (K1 % 64 || K0 % (bf ? 128 : 64) || J0 > 32 || J1 > 64 || B < 1 || B > 0x7fffffff)
-        return DT_EUNSUPPORTED;
     const int jb0 = J0 <= 16 ? 1 : 2;
     const int jb1 = J1 <= 16 ? 1 : (J1 <= 32 ? 2 : 4);
-    dim3 grid((unsigned)B);
 #define L2K(A0, A1)                                                                                   \
     if (jb0 == A0 && jb1 == A1) {                                                                     \
-        if (bf) {                                                                                     \
-            DT_LAUNCH("ttm2_lead_bf16_stream", (ttm2_lead<A0, A1, true>), grid, dim3(256), 0, A, U0, M1, T, \
-                      (int)K0, (int)K1, (int)J0, (int)J1);                                            \
-        } else {                                                                                      \
-            DT_LAUNCH("ttm2_lead_f32_stream", (ttm2_lead<A0, A1, false>), grid, dim3(256), 0, A, U0, M1, T, \
-                      (int)K0, (int)K1, (int)J0, (int)J1);                                            \
-        }                                                                                             \
+        auto kern = bf ? ttm2_lead<A0, A1, true> : ttm2_lead<A0, A1, false>;                           \
+        static int64_t waves[2] = {0, 0};                                                             \
+        if (!waves[bf]) waves[bf] = resident_waves(kern);                                             \
+        if (waves[bf] <= 0) return dt_fail(ctx, DT_EHIP, "dt_ttm2_lead: occupancy query failed");    \
+        const int64_t nwave = std::min<int64_t>(B, waves[bf]);                                        \
+        dim3 grid((unsigned)((nwave + 3) / 4));                                                       \
+        DT_LAUNCH(bf ? "ttm2_lead_bf16_stream" : "ttm2_lead_f32_stream", kern, grid, dim3(256), 0, A,  \
+                  U0, M1, T, (int)K0, (int)K1, (int)J0, (int)J1, B);                                  \
         return DT_OK;                                                                                 \
     }
     L2K(1, 1) L2K(1, 2) L2K(1, 4) L2K(2, 1) L2K(2, 2) L2K(2, 4)
 #undef L2K
     return DT_EUNSUPPORTED;
+}
+
+// Factor views for the fused pair from J x K column-major factors (reference
+// convention Y = A x_n M): U0 = M0ᵀ (K0 x J0, FP32 or bf16), M1 FP32 copy.
+template <typename TS, typename TD>
+__global__ void pair_view(TD *__restrict__ dst, const TS *__restrict__ src, int64_t J, int64_t K, bool transpose) {
+    const int64_t total = J * K;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = e % J, k = e / J;  // src[j + J k]
+        const float v = (float)src[e];
+        const int64_t o = transpose ? k + K * j : e;
+        if constexpr (sizeof(TD) == 2)
+            dst[o] = f32_to_bf16(v);
+        else
+            dst[o] = v;
+    }
+}
+
+static inline dim3 pv_grid(int64_t n) { return dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)); }
+
+// Y = A x_{idx[0]} M[0] x_{idx[1]} M[1] ... (Function::ttmc).  When the chain
+// starts with modes {0, 1} of an FP32/BF16 tensor, those two products run as
+// one pass (ttm2_lead); the rest are single products through dt_ttm.
+extern "C" int dt_ttmc(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape,
+                       const void *const *M, int dtM, const size_t *J, const int *idx, int nm, void *Y,
+                       int dtY) {
+    DT_CHECK_ARG(ctx && A && M && J && idx && Y && shape, "dt_ttmc: null argument");
+    DT_CHECK_ARG(ndim >= 1 && ndim <= DT_MAX_NDIM, "dt_ttmc: bad ndim %d", ndim);
+    DT_CHECK_ARG(nm >= 1, "dt_ttmc: need at least one factor");
+    DT_CHECK_ARG(dtA == DT_F64 || dtA == DT_F32 || dtA == DT_BF16, "dt_ttmc: bad tensor type %d", dtA);
+    DT_CHECK_ARG(dtM == DT_F64 || dtM == DT_F32, "dt_ttmc: factor must be F64 or F32");
+    DT_CHECK_ARG(dtY == (dtA == DT_F64 ? DT_F64 : DT_F32), "dt_ttmc: output type %d does not match input %d",
+                 dtY, dtA);
+    for (int s = 0; s < nm; s++) {
+        DT_CHECK_ARG(idx[s] >= 0 && idx[s] < ndim, "dt_ttmc: bad mode %d", idx[s]);
+        DT_CHECK_ARG(M[s] && J[s] >= 1, "dt_ttmc: factor %d missing or empty", s);
+    }
+    size_t sh[DT_MAX_NDIM];
+    for (int i = 0; i < ndim; i++) sh[i] = shape[i];
+    const int idt = dtA == DT_F64 ? DT_F64 : DT_F32;  // intermediate type
+    const size_t isz = idt == DT_F64 ? 8 : 4;
+    const void *cur = A;
+    int curdt = dtA;
+    int s = 0;
+    auto scratch = [&](int step, size_t elems) -> void * {
+        return dt_ws(ctx, (step & 1) ? WS_TTMC1 : WS_TTMC0, elems * isz);
+    };
+    const bool pair = nm >= 2 && ndim >= 2 && ((idx[0] == 0 && idx[1] == 1) || (idx[0] == 1 && idx[1] == 0));
+    const int p0 = idx[0] == 0 ? 0 : 1;  // position of the mode-0 factor
+    int64_t B = 1;
+    for (int i = 2; i < ndim; i++) B *= (int64_t)sh[i];
+    if (pair && ttm2_lead_fits(dtA, (int64_t)sh[0], (int64_t)sh[1], B, (int64_t)J[p0], (int64_t)J[1 - p0])) {
+        const int64_t K0 = (int64_t)sh[0], K1 = (int64_t)sh[1], J0 = (int64_t)J[p0], J1 = (int64_t)J[1 - p0];
+        const bool last = nm == 2;
+        // the pair counts as step 1: the next product reads slot 1, writes slot 0
+        float *T = last ? (float *)Y : (float *)scratch(1, (size_t)(J0 * J1 * B));
+        const bool bf = dtA == DT_BF16;
+        const size_t u0b = ((size_t)(K0 * J0) * (bf ? 2 : 4) + 255) & ~(size_t)255;
+        char *ws = (char *)dt_ws(ctx, WS_TTMC_M, u0b + (size_t)(J1 * K1) * 4);
+        if (!T || !ws) return DT_ENOMEM;
+        void *U0 = ws;
+        float *M1 = (float *)(ws + u0b);
+        const void *M0s = M[p0], *M1s = M[1 - p0];
+        // U0 = M0ᵀ (K0 x J0) in the tensor's operand type, M1 as FP32 J1 x K1
+        if (dtM == DT_F64 && bf) {
+            DT_LAUNCH("factor_views", (pair_view<double, uint16_t>), pv_grid(K0 * J0), dim3(256), 0,
+                      (uint16_t *)U0, (const double *)M0s, J0, K0, true);
+        } else if (dtM == DT_F64) {
+            DT_LAUNCH("factor_views", (pair_view<double, float>), pv_grid(K0 * J0), dim3(256), 0,
+                      (float *)U0, (const double *)M0s, J0, K0, true);
+        } else if (bf) {
+            DT_LAUNCH("factor_views", (pair_view<float, uint16_t>), pv_grid(K0 * J0), dim3(256), 0,
+                      (uint16_t *)U0, (const float *)M0s, J0, K0, true);
+        } else {
+            DT_LAUNCH("factor_views", (pair_view<float, float>), pv_grid(K0 * J0), dim3(256), 0,
+                      (float *)U0, (const float *)M0s, J0, K0, true);
+        }
+        if (dtM == DT_F64) {
+            DT_LAUNCH("factor_views", (pair_view<double, float>), pv_grid(K1 * J1), dim3(256), 0, M1,
+                      (const double *)M1s, J1, K1, false);
+        } else {
+            DT_LAUNCH("factor_views", (pair_view<float, float>), pv_grid(K1 * J1), dim3(256), 0, M1,
+                      (const float *)M1s, J1, K1, false);
+        }
+        DT_TRY(dt_ttm2_lead(ctx, A, dtA, K0, K1, B, U0, M1, J0, J1, T));
+        if (last) return DT_OK;
+        sh[0] = (size_t)J0;
+        sh[1] = (size_t)J1;
+        cur = T;
+        curdt = DT_F32;
+        s = 2;
+    }
+    for (; s < nm; s++) {
+        const bool last = s == nm - 1;
+        size_t out = 1;
+        for (int i = 0; i < ndim; i++) out *= i == idx[s] ? J[s] : sh[i];
+        void *dst = last ? Y : scratch(s, out);
+        if (!dst) return DT_ENOMEM;
+        DT_TRY(dt_ttm(ctx, cur, curdt, ndim, sh, M[s], dtM, J[s], idx[s], dst, last ? dtY : idt));
+        sh[idx[s]] = J[s];
+        cur = dst;
+        curdt = last ? dtY : idt;
+    }
+    return DT_OK;
 }

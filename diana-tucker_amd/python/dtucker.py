@@ -79,6 +79,8 @@ def lib():
             "dt_timing_list": [vp, C.c_char_p, sz],
             "dt_timer_start": [vp], "dt_timer_stop": [vp, C.POINTER(dbl)],
             "dt_ttm": [vp, vp, i32, i32, C.POINTER(sz), vp, i32, sz, i32, vp, i32],
+            "dt_ttmc": [vp, vp, i32, i32, C.POINTER(sz), C.POINTER(vp), i32, C.POINTER(sz), C.POINTER(i32),
+                        i32, vp, i32],
             "dt_gram": [vp, vp, i32, i32, C.POINTER(sz), i32, vp, i32],
             "dt_ttt_except": [vp, vp, vp, i32, i32, C.POINTER(sz), sz, i32, vp],
             "dt_sumsq": [vp, vp, i32, sz, C.POINTER(dbl)],
@@ -110,7 +112,7 @@ def exported_symbols():
     return ["dt_create", "dt_destroy", "dt_last_error", "dt_malloc", "dt_free", "dt_h2d",
             "dt_d2h", "dt_d2d", "dt_zero", "dt_sync", "dt_convert", "dt_timing_enable",
             "dt_timing_reset", "dt_timing_filter", "dt_timing_get", "dt_timing_list", "dt_timer_start",
-            "dt_timer_stop", "dt_ttm", "dt_gram", "dt_ttt_except", "dt_sumsq", "dt_gemm_f64",
+            "dt_timer_stop", "dt_ttm", "dt_ttmc", "dt_gram", "dt_ttt_except", "dt_sumsq", "dt_gemm_f64",
             "dt_qr_f64", "dt_inverse_f64", "dt_syev_top_f64", "dt_orth_rows_f64",
             "dt_gen_normal", "dt_gen_uniform", "dt_gen_tucker", "dt_hooi_dense",
             "dt_hooi_create", "dt_hooi_sweep", "dt_hooi_get", "dt_hooi_destroy"]
@@ -264,6 +266,25 @@ class Context:
         shp = (C.c_size_t * len(A.shape))(*A.shape)
         self.check(self.L.dt_ttm(self.handle, A.ptr, A.dtype, len(A.shape), shp, M.ptr, M.dtype,
                                  J, n, Y.ptr, ydt))
+        return Y
+
+    def ttmc(self, A: DeviceArray, Ms, idx) -> DeviceArray:
+        """Y = A x_{idx[0]} Ms[0] x_{idx[1]} Ms[1] ... (all Ms of one dtype)."""
+        assert len(Ms) == len(idx) and len(Ms) >= 1
+        shape = list(A.shape)
+        for M, n in zip(Ms, idx):
+            if 0 <= n < len(shape):  # a bad mode is left for the library to reject
+                assert M.shape[1] == shape[n] and M.dtype == Ms[0].dtype
+                shape[n] = M.shape[0]
+        ydt = DT_F64 if A.dtype == DT_F64 else DT_F32
+        Y = self.array(shape, ydt)
+        k = len(Ms)
+        shp = (C.c_size_t * len(A.shape))(*A.shape)
+        mp = (C.c_void_p * k)(*[M.ptr for M in Ms])
+        js = (C.c_size_t * k)(*[M.shape[0] for M in Ms])
+        ix = (C.c_int * k)(*idx)
+        self.check(self.L.dt_ttmc(self.handle, A.ptr, A.dtype, len(A.shape), shp, mp, Ms[0].dtype, js, ix,
+                                  k, Y.ptr, ydt))
         return Y
 
     def gram(self, A: DeviceArray, n: int, dtS=DT_F64) -> DeviceArray:

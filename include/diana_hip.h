@@ -94,6 +94,18 @@ int dt_timer_stop(dt_ctx *ctx, double *elapsed_ms);
 int dt_ttm(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape,
            const void *M, int dtM, size_t J, int n, void *Y, int dtY);
 
+/* Y = A x_{idx[0]} M[0] x_{idx[1]} M[1] ... x_{idx[nm-1]} M[nm-1].
+ * Replaces Function::ttmc (template/function/tensor.tpp:337-348), which
+ * applies Function::ttm in sequence.  M[s]: device, J[s] x (current size of
+ * mode idx[s]) column-major, all of type dtM (F64/F32).  dtY is F64 for an
+ * F64 tensor, F32 otherwise.  A chain that starts with modes 0 and 1 of an
+ * F32/BF16 tensor runs those two products in one pass over A (no
+ * intermediate of A's size is written) when shape[0] % 64 (F32) or % 128
+ * (BF16), shape[1] % 64 == 0, J <= 32 for mode 0 and J <= 64 for mode 1. */
+int dt_ttmc(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape,
+            const void *const *M, int dtM, const size_t *J, const int *idx, int nm,
+            void *Y, int dtY);
+
 /* S = A_(n) A_(n)^T (shape[n] x shape[n], column-major, dtS F32 or F64).
  * Replaces Function::gram(A, n) (template/function/tensor.tpp:18-120) =
  * tenmat + matmulNT at one rank.  FP32/BF16 inputs accumulate in FP32 on

@@ -83,6 +83,112 @@ def test_ttm_bf16_vs_oracle(ctx, shape, n, J):
     assert min(rel(Y, ref), rel(Y, ref_generic)) < 2e-6
 
 
+TTMC_CASES = [
+    # shape, modes, J -- the first five start with modes {0, 1} in a shape the
+    # fused two-mode pass accepts (both FP32 and BF16 for shape[0] % 128 == 0);
+    # shape[1] = 64 and 192 leave waves of the workgroup without columns
+    ((128, 64, 6), (0, 1), (16, 16)),
+    ((128, 128, 3, 4), (1, 0, 3), (5, 30, 2)),
+    ((256, 192, 5), (0, 1, 2), (32, 64, 3)),
+    ((128, 320, 2), (0, 1), (20, 40)),
+    ((64, 64, 2), (0, 1), (8, 40)),
+    # not fusable: shape[0] % 64 != 0, modes not {0, 1} first, one product
+    ((60, 64, 3), (0, 1), (8, 8)),
+    ((128, 64, 7), (2, 0), (3, 4)),
+    ((32, 24, 20), (1,), (5,)),
+]
+
+
+def ttmc_oracle(A, Ms, modes):
+    Y = A
+    for M, n in zip(Ms, modes):
+        Y = D.ttm(Y, M, n)
+    return Y
+
+
+@pytest.mark.parametrize("shape,modes,J", TTMC_CASES)
+def test_ttmc_f32_vs_oracle(ctx, shape, modes, J):
+    """Function::ttmc: chained products; FP32 accumulation over at most two
+    contractions of <= 320 terms each gives relative error ~1e-6."""
+    print("ttmc f32", shape, modes, J, flush=True)
+    rng = np.random.default_rng(11 + len(shape) + sum(J))
+    A = rng.standard_normal(shape).astype(np.float32)
+    cur = list(shape)
+    Ms = []
+    for n, j in zip(modes, J):
+        Ms.append(rng.standard_normal((j, cur[n])).astype(np.float32))
+        cur[n] = j
+    Y = ctx.ttmc(ctx.to_device(A), [ctx.to_device(M) for M in Ms], list(modes)).download()
+    ref = ttmc_oracle(A.astype(np.float64), [M.astype(np.float64) for M in Ms], modes)
+    assert Y.shape == ref.shape
+    assert rel(Y, ref) < 3e-6
+    # F64 factors give the same FP32 result path
+    Y64 = ctx.ttmc(ctx.to_device(A), [ctx.to_device(M.astype(np.float64)) for M in Ms], list(modes))
+    assert rel(Y64.download(), ref) < 3e-6
+
+
+@pytest.mark.parametrize("shape,modes,J", TTMC_CASES)
+def test_ttmc_bf16_vs_oracle(ctx, shape, modes, J):
+    """BF16 tensor: the product that reads the tensor rounds its factor to
+    bf16 (MFMA operand); later products run on the FP32 intermediate."""
+    print("ttmc bf16", shape, modes, J, flush=True)
+    rng = np.random.default_rng(29 + len(shape) + sum(J))
+    A = rng.standard_normal(shape).astype(np.float32)
+    Ab = dt.bf16_from_f32(A)
+    cur = list(shape)
+    Ms = []
+    for n, j in zip(modes, J):
+        Ms.append(rng.standard_normal((j, cur[n])).astype(np.float32))
+        cur[n] = j
+    dA = ctx.array(shape, dt.DT_BF16).upload(Ab)
+    Y = ctx.ttmc(dA, [ctx.to_device(M) for M in Ms], list(modes)).download()
+    Af = dt.f32_from_bf16(Ab).astype(np.float64)
+    # fused pair {0, 1}: the mode-0 factor is the bf16 operand, whichever comes first
+    first = 0 if set(modes[:2]) == {0, 1} else None
+    Mr = []
+    for M, n in zip(Ms, modes):
+        if (first is not None and n == first) or (first is None and len(Mr) == 0):
+            Mr.append(dt.f32_from_bf16(dt.bf16_from_f32(M)).astype(np.float64))
+        else:
+            Mr.append(M.astype(np.float64))
+    if first is not None and modes[0] == 1:
+        # mode 0 is contracted first inside the fused pass
+        order = [1, 0] + list(range(2, len(modes)))
+        ref = ttmc_oracle(Af, [Mr[i] for i in order], [modes[i] for i in order])
+    else:
+        ref = ttmc_oracle(Af, Mr, modes)
+    assert rel(Y, ref) < 3e-6 or not (first is not None and shape[0] % 128 == 0)
+    if not (first is not None and shape[0] % 128 == 0):
+        # unfused chain: the first product in sequence reads the tensor; its
+        # factor is a bf16 MFMA operand on aligned shapes, FP32 on the generic path
+        Mr2 = [dt.f32_from_bf16(dt.bf16_from_f32(Ms[0])).astype(np.float64)] + \
+              [M.astype(np.float64) for M in Ms[1:]]
+        Mr3 = [M.astype(np.float64) for M in Ms]
+        assert min(rel(Y, ttmc_oracle(Af, Mr2, modes)), rel(Y, ttmc_oracle(Af, Mr3, modes))) < 3e-6
+
+
+def test_ttmc_fused_pass_taken(ctx):
+    """The {0, 1} pair of a fusable shape runs as one kernel launch."""
+    rng = np.random.default_rng(3)
+    A = ctx.to_device(rng.standard_normal((128, 64, 4)).astype(np.float32))
+    Ms = [ctx.to_device(rng.standard_normal((8, 128)).astype(np.float32)),
+          ctx.to_device(rng.standard_normal((8, 64)).astype(np.float32))]
+    ctx.timing(True)
+    ctx.timing_reset()
+    ctx.ttmc(A, Ms, [0, 1])
+    names = ctx.timing_list()
+    ctx.timing(False)
+    assert "ttm2_lead_f32_stream" in names and not any(k.startswith("ttm_m") for k in names)
+
+
+def test_ttmc_bad_arguments(ctx):
+    A = ctx.to_device(np.ones((4, 5, 6), np.float32))
+    M = ctx.to_device(np.ones((2, 4), np.float32))
+    with pytest.raises(dt.DTError) as e:
+        ctx.ttmc(A, [M], [3])
+    assert e.value.code == dt.DT_EINVAL
+
+
 @pytest.mark.parametrize("shape,n", [((64, 48, 40), 0), ((64, 48, 40), 1), ((30, 20, 10), 2),
                                      ((256, 16, 16), 0), ((16, 256, 16), 1), ((7, 5, 130), 2)])
 def test_gram_vs_oracle(ctx, shape, n):
@@ -274,6 +380,28 @@ def test_hooi_4way_tree_matches_reference_schedule(ctx):
         assert D.sin_max_angle(r1["U"][n], r2["U"][n]) < 1e-4
     Uh = D.hosvd_factors(A.astype(np.float64), ranks)
     _, _, res_ref = D.hooi_als(A.astype(np.float64), ranks, 5, Uh)
+    assert np.allclose(r2["residuals"], res_ref, rtol=1e-4)
+
+
+def test_hooi_4way_tree_fused_pair_matches_reference_schedule(ctx):
+    """Shape where the tree schedule's second phase contracts modes 0 and 1 in
+    one pass; the reference schedule runs single products only."""
+    shape, ranks = (128, 64, 12, 10), (8, 12, 3, 4)
+    g = ctx.gen_tucker(shape, ranks, 1e-2, seed=9)
+    A = g["A"].download()
+    r1 = run_gpu_hooi(ctx, A, ranks, 4, dt.INIT_HOSVD, schedule=dt.SCHED_REFERENCE,
+                      flags=dt.FLAG_FP64_FIT)
+    ctx.timing(True)
+    ctx.timing_reset()
+    r2 = run_gpu_hooi(ctx, A, ranks, 4, dt.INIT_HOSVD, schedule=dt.SCHED_TREE, flags=dt.FLAG_FP64_FIT)
+    names = ctx.timing_list()
+    ctx.timing(False)
+    assert "ttm2_lead_f32_stream" in names
+    assert np.allclose(r1["residuals"], r2["residuals"], rtol=1e-6)
+    for n in range(4):
+        assert D.sin_max_angle(r1["U"][n], r2["U"][n]) < 1e-4
+    Uh = D.hosvd_factors(A.astype(np.float64), ranks)
+    _, _, res_ref = D.hooi_als(A.astype(np.float64), ranks, 4, Uh)
     assert np.allclose(r2["residuals"], res_ref, rtol=1e-4)
 
 
