@@ -162,7 +162,15 @@ int dt_timing_enable(dt_ctx *ctx, int on) {
 }
 
 int dt_timing_filter(dt_ctx *ctx, const char *prefix) {
-    ctx->timing_prefix = prefix ? prefix : "";
+    ctx->timing_prefixes.clear();
+    const std::string all = prefix ? prefix : "";
+    size_t a = 0;
+    while (a < all.size()) {
+        size_t b = all.find(',', a);
+        if (b == std::string::npos) b = all.size();
+        if (b > a) ctx->timing_prefixes.push_back(all.substr(a, b - a));
+        a = b + 1;
+    }
     return DT_OK;
 }
 

@@ -31,7 +31,8 @@ struct dt_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     bool timing = false;
-    std::string timing_prefix;  // when non-empty only launches with this name prefix are timed
+    std::vector<std::string> timing_prefixes;  // when non-empty only launches whose name starts
+                                               // with one of these are timed
     std::vector<dt_pending> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, dt_kstat> stats;
@@ -108,8 +109,10 @@ struct dt_ktimer {
     hipEvent_t a = nullptr;
     bool on = false;
     dt_ktimer(dt_ctx *c_, const char *n) : c(c_), name(n) {
-        on = c->timing && (c->timing_prefix.empty() ||
-                           strncmp(n, c->timing_prefix.c_str(), c->timing_prefix.size()) == 0);
+        on = c->timing && c->timing_prefixes.empty();
+        if (c->timing && !on)
+            for (const std::string &p : c->timing_prefixes)
+                if (strncmp(n, p.c_str(), p.size()) == 0) on = true;
         if (on) {
             a = dt_get_event(c);
             hipEventRecord(a, c->stream);

@@ -215,6 +215,13 @@ class Context:
     def array(self, shape, dtype=DT_F32) -> DeviceArray:
         return DeviceArray(self, shape, dtype)
 
+    def zero(self, A: DeviceArray):
+        self.check(self.L.dt_zero(self.handle, A.ptr, A.nbytes))
+
+    def convert(self, dst: DeviceArray, src: DeviceArray):
+        assert dst.size == src.size
+        self.check(self.L.dt_convert(self.handle, dst.ptr, dst.dtype, src.ptr, src.dtype, src.size))
+
     def to_device(self, host: np.ndarray, dtype=None) -> DeviceArray:
         h = np.asarray(host)
         if dtype is None:

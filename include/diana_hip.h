@@ -75,8 +75,9 @@ int dt_convert(dt_ctx *ctx, void *dst, int dt_dst, const void *src, int dt_src, 
  * MPI_Wtime-based Summary timers (src/summary.cpp:29-69). */
 int dt_timing_enable(dt_ctx *ctx, int on);
 int dt_timing_reset(dt_ctx *ctx);
-/* Only launches whose kernel name starts with `prefix` are bracketed (NULL or
- * "" = all); keeps the event overhead off the kernels that are not measured. */
+/* Only launches whose kernel name starts with one of the comma-separated
+ * prefixes are bracketed (NULL or "" = all); keeps the event overhead off
+ * the kernels that are not measured. */
 int dt_timing_filter(dt_ctx *ctx, const char *prefix);
 int dt_timing_get(dt_ctx *ctx, const char *kernel, int64_t *count, double *total_ms);
 int dt_timing_list(dt_ctx *ctx, char *buf, size_t buflen); /* "name count ms\n" lines */
