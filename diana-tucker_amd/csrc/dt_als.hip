@@ -15,10 +15,11 @@
 // the parity tests compare fits and principal angles with the oracle, which
 // runs the reference's own formulation.
 //
-//   als_rows_kernel  : RB rows per workgroup: T = S[rows,:] Y (S symmetric, so a
-//                      row of S is a contiguous column), out = T Rinv, plus the
-//                      workgroup's partial Gram outᵀ out.  Optionally writes the
+//   als_rows_kernel  : RB = 256 / R rows per workgroup (one output per thread):
+//                      T = S[rows,:] Y (S symmetric, so a row of S is a
+//                      contiguous column), out = T Rinv.  Optionally writes the
 //                      FP32 / bf16 / transposed kernel views of the result.
+//   als_gram_kernel  : partial Grams outᵀ out of 128-row blocks.
 //   als_chol_kernel  : one workgroup: G = sum of the partials (fixed order),
 //                      optional shift 11 (I R + R(R+1)) u tr(G) (Fukaya et al.
 //                      2020), G = RᵀR, writes R^{-1}.
@@ -26,59 +27,91 @@
 
 #include <cmath>
 
-constexpr int ALS_RB = 16;      // rows per workgroup
 constexpr int ALS_THREADS = 256;
-constexpr int ALS_CH = 32;      // k-chunk staged in LDS
+constexpr int ALS_CH = 128;     // k-chunk staged in LDS
 constexpr int ALS_MAXR = 64;
+constexpr int ALS_GR = 128;     // rows per workgroup of the partial Gram
+
+// rows per workgroup of als_rows_kernel: one output element per thread
+template <int RMAX>
+struct AlsRows {
+    static constexpr int RB = RMAX <= 16 ? 16 : (RMAX <= 32 ? 8 : 4);
+};
 
 // mode 0: out = (S[rows,:] Y) Rinv      (S: I x I symmetric)
 // mode 1: out = Y[rows,:] Rinv          (no S)
-// Rinv == nullptr means identity.  part (R x R per workgroup) receives outᵀout
-// of this workgroup's rows when non-null.
+// Rinv == nullptr means identity.
 template <int RMAX>
 __global__ __launch_bounds__(ALS_THREADS) void als_rows_kernel(
     int mode, const double *__restrict__ S, const double *__restrict__ Y, int I, int R,
-    const double *__restrict__ Rinv, double *__restrict__ out, double *__restrict__ part,
-    float *__restrict__ U32, float *__restrict__ M32, uint16_t *__restrict__ Ub16) {
-    __shared__ double Ss[ALS_CH][ALS_RB + 1];     // S[t0+c, row0+r]
-    __shared__ double Ys[ALS_CH][RMAX + 1];       // Y[t0+c, j]
-    __shared__ double T[ALS_RB][RMAX + 1];        // rows x R
+    const double *__restrict__ Rinv, double *__restrict__ out, float *__restrict__ U32,
+    float *__restrict__ M32, uint16_t *__restrict__ Ub16) {
+    constexpr int RB = AlsRows<RMAX>::RB;
+    __shared__ double Ss[ALS_CH][RB + 1];     // S[t0+c, row0+r]
+    __shared__ double Ys[ALS_CH][RMAX + 1];   // Y[t0+c, j]
+    __shared__ double T[RB][RMAX + 1];        // rows x R
     __shared__ double Ri[RMAX][RMAX + 1];
     const int tid = threadIdx.x;
-    const int row0 = blockIdx.x * ALS_RB;
-    const int nrow = (I - row0) < ALS_RB ? (I - row0) : ALS_RB;
+    const int row0 = blockIdx.x * RB;
+    const int nrow = (I - row0) < RB ? (I - row0) : RB;
 
     for (int e = tid; e < R * R; e += ALS_THREADS) {
         const int i = e % R, j = e / R;
         Ri[i][j] = Rinv ? Rinv[e] : (i == j ? 1.0 : 0.0);
     }
     // outputs owned by this thread: (r, j) = (o % RB, o / RB), o = tid + q*256
-    constexpr int NQ = (ALS_RB * RMAX + ALS_THREADS - 1) / ALS_THREADS;
+    constexpr int NQ = (RB * RMAX + ALS_THREADS - 1) / ALS_THREADS;
     double acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; q++) acc[q] = 0.0;
 
     if (mode == 0) {
-        for (int t0 = 0; t0 < I; t0 += ALS_CH) {
+        // chunks of ALS_CH rows of S and Y go through LDS; the next chunk's
+        // global loads are issued before the current chunk's FMAs
+        constexpr int NS = (ALS_CH * RB + ALS_THREADS - 1) / ALS_THREADS;
+        constexpr int NY = (ALS_CH * RMAX + ALS_THREADS - 1) / ALS_THREADS;
+        double sreg[NS], yreg[NY];
+        auto fetch = [&](int t0) {
             const int ch = (I - t0) < ALS_CH ? (I - t0) : ALS_CH;
-            for (int e = tid; e < ALS_CH * ALS_RB; e += ALS_THREADS) {
+#pragma unroll
+            for (int q = 0; q < NS; q++) {
+                const int e = tid + ALS_THREADS * q;
                 const int c = e % ALS_CH, r = e / ALS_CH;
-                Ss[c][r] = (c < ch && r < nrow) ? S[(t0 + c) + (size_t)I * (row0 + r)] : 0.0;
+                sreg[q] = (e < ALS_CH * RB && c < ch && r < nrow) ? S[(t0 + c) + (size_t)I * (row0 + r)] : 0.0;
             }
-            for (int e = tid; e < ALS_CH * R; e += ALS_THREADS) {
+#pragma unroll
+            for (int q = 0; q < NY; q++) {
+                const int e = tid + ALS_THREADS * q;
                 const int c = e % ALS_CH, j = e / ALS_CH;
-                Ys[c][j] = c < ch ? Y[(t0 + c) + (size_t)I * j] : 0.0;
+                yreg[q] = (j < R && c < ch) ? Y[(t0 + c) + (size_t)I * j] : 0.0;
+            }
+        };
+        fetch(0);
+        for (int t0 = 0; t0 < I; t0 += ALS_CH) {
+#pragma unroll
+            for (int q = 0; q < NS; q++) {
+                const int e = tid + ALS_THREADS * q;
+                if (e < ALS_CH * RB) Ss[e % ALS_CH][e / ALS_CH] = sreg[q];
+            }
+#pragma unroll
+            for (int q = 0; q < NY; q++) {
+                const int e = tid + ALS_THREADS * q;
+                Ys[e % ALS_CH][e / ALS_CH] = yreg[q];
             }
             __syncthreads();
+            if (t0 + ALS_CH < I) fetch(t0 + ALS_CH);
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
                 const int o = tid + q * ALS_THREADS;
-                const int r = o % ALS_RB, j = o / ALS_RB;
+                const int r = o % RB, j = o / RB;
                 if (j < R) {
-                    double s = acc[q];
-#pragma unroll 8
-                    for (int c = 0; c < ALS_CH; c++) s = fma(Ss[c][r], Ys[c][j], s);
-                    acc[q] = s;
+                    double s0 = acc[q], s1 = 0.0;
+#pragma unroll 16
+                    for (int c = 0; c < ALS_CH; c += 2) {
+                        s0 = fma(Ss[c][r], Ys[c][j], s0);
+                        s1 = fma(Ss[c + 1][r], Ys[c + 1][j], s1);
+                    }
+                    acc[q] = s0 + s1;
                 }
             }
             __syncthreads();
@@ -87,14 +120,14 @@ __global__ __launch_bounds__(ALS_THREADS) void als_rows_kernel(
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
             const int o = tid + q * ALS_THREADS;
-            const int r = o % ALS_RB, j = o / ALS_RB;
+            const int r = o % RB, j = o / RB;
             if (j < R && r < nrow) acc[q] = Y[(row0 + r) + (size_t)I * j];
         }
     }
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
         const int o = tid + q * ALS_THREADS;
-        const int r = o % ALS_RB, j = o / ALS_RB;
+        const int r = o % RB, j = o / RB;
         if (j < R) T[r][j] = acc[q];
     }
     __syncthreads();
@@ -102,56 +135,113 @@ __global__ __launch_bounds__(ALS_THREADS) void als_rows_kernel(
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
         const int o = tid + q * ALS_THREADS;
-        const int r = o % ALS_RB, j = o / ALS_RB;
+        const int r = o % RB, j = o / RB;
         if (j < R) {
             double s = 0.0;
             for (int m = 0; m < R; m++) s = fma(T[r][m], Ri[m][j], s);
             acc[q] = s;
         }
     }
-    __syncthreads();
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
         const int o = tid + q * ALS_THREADS;
-        const int r = o % ALS_RB, j = o / ALS_RB;
-        if (j < R) {
-            T[r][j] = r < nrow ? acc[q] : 0.0;
-            if (r < nrow) {
-                const size_t gi = (size_t)(row0 + r) + (size_t)I * j;
-                out[gi] = acc[q];
-                if (U32) U32[gi] = (float)acc[q];
-                if (Ub16) Ub16[gi] = f32_to_bf16((float)acc[q]);
-                if (M32) M32[j + (size_t)R * (row0 + r)] = (float)acc[q];
-            }
-        }
-    }
-    __syncthreads();
-    if (part) {
-        double *p = part + (size_t)blockIdx.x * R * R;
-        for (int e = tid; e < R * R; e += ALS_THREADS) {
-            const int i = e % R, j = e / R;
-            double s = 0.0;
-#pragma unroll
-            for (int r = 0; r < ALS_RB; r++) s = fma(T[r][i], T[r][j], s);
-            p[e] = s;
+        const int r = o % RB, j = o / RB;
+        if (j < R && r < nrow) {
+            const size_t gi = (size_t)(row0 + r) + (size_t)I * j;
+            out[gi] = acc[q];
+            if (U32) U32[gi] = (float)acc[q];
+            if (Ub16) Ub16[gi] = f32_to_bf16((float)acc[q]);
+            if (M32) M32[j + (size_t)R * (row0 + r)] = (float)acc[q];
         }
     }
 }
 
-// One workgroup: G = sum_w part[w]; optional shift; Rinv = chol(G)^{-1}.
+// part[w] = X[rows_w, :]ᵀ X[rows_w, :] for ALS_GR-row blocks w of X (I x R),
+// staged through LDS 64 rows at a time.
+template <int RMAX>
+__global__ __launch_bounds__(ALS_THREADS) void als_gram_kernel(const double *__restrict__ X, int I, int R,
+                                                               double *__restrict__ part) {
+    constexpr int CR = 64;
+    constexpr int NB = (RMAX + 15) / 16;  // entries (tx + 16 a, ty + 16 b), a, b < NB
+    __shared__ double Xs[CR][RMAX + 1];
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const int rbeg = blockIdx.x * ALS_GR;
+    const int rend = (rbeg + ALS_GR) < I ? (rbeg + ALS_GR) : I;
+    double acc[NB][NB];
+#pragma unroll
+    for (int a = 0; a < NB; a++)
+#pragma unroll
+        for (int b = 0; b < NB; b++) acc[a][b] = 0.0;
+    for (int r0 = rbeg; r0 < rend; r0 += CR) {
+        for (int e = tid; e < CR * R; e += ALS_THREADS) {
+            const int r = e % CR, j = e / CR;
+            Xs[r][j] = r0 + r < rend ? X[(size_t)(r0 + r) + (size_t)I * j] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int r = 0; r < CR; r++) {
+            double xi[NB], xj[NB];
+#pragma unroll
+            for (int a = 0; a < NB; a++) {
+                xi[a] = Xs[r][(tx + 16 * a) < R ? tx + 16 * a : 0];
+                xj[a] = Xs[r][(ty + 16 * a) < R ? ty + 16 * a : 0];
+            }
+#pragma unroll
+            for (int a = 0; a < NB; a++)
+#pragma unroll
+                for (int b = 0; b < NB; b++) acc[a][b] = fma(xi[a], xj[b], acc[a][b]);
+        }
+        __syncthreads();
+    }
+    double *p = part + (size_t)blockIdx.x * R * R;
+#pragma unroll
+    for (int a = 0; a < NB; a++)
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const int i = tx + 16 * a, j = ty + 16 * b;
+            if (i < R && j < R) p[i + (size_t)R * j] = acc[a][b];
+        }
+}
+
+// v_readlane of a double: lane `lane`'s value, uniform across the wave
+__device__ __forceinline__ double rl_f64(double v, int lane) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, lane);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// One workgroup: G = sum_w part[w] (fixed order); optional shift; Rinv = chol(G)^{-1}.
 __global__ __launch_bounds__(ALS_THREADS) void als_chol_kernel(const double *__restrict__ part,
                                                                int nparts, int R, int I,
                                                                int shifted,
                                                                double *__restrict__ Rinv_out,
                                                                int *__restrict__ flag) {
     __shared__ double G[ALS_MAXR][ALS_MAXR + 1];
-    __shared__ double shift_s, rjj_s;
-    __shared__ int bad_s;
+    __shared__ double X[ALS_MAXR][ALS_MAXR + 1];
+    __shared__ double shift_s;
     const int tid = threadIdx.x;
-    for (int e = tid; e < R * R; e += ALS_THREADS) {
-        double s = 0.0;
-        for (int w = 0; w < nparts; w++) s += part[(size_t)w * R * R + e];
-        G[e % R][e / R] = s;
+    {
+        // G = sum of the partials in a fixed order; each pass over w issues
+        // this thread's loads for all its entries together
+        constexpr int NE = (ALS_MAXR * ALS_MAXR + ALS_THREADS - 1) / ALS_THREADS;
+        const size_t RR = (size_t)R * R;
+        double acc[NE];
+#pragma unroll
+        for (int q = 0; q < NE; q++) acc[q] = 0.0;
+        for (int w = 0; w < nparts; w++) {
+#pragma unroll
+            for (int q = 0; q < NE; q++) {
+                const int e = tid + ALS_THREADS * q;
+                if (e < R * R) acc[q] += part[(size_t)w * RR + e];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NE; q++) {
+            const int e = tid + ALS_THREADS * q;
+            if (e < R * R) G[e % R][e / R] = acc[q];
+        }
     }
     __syncthreads();
     if (tid == 0) {
@@ -162,49 +252,60 @@ __global__ __launch_bounds__(ALS_THREADS) void als_chol_kernel(const double *__r
             shift = 11.0 * ((double)I * R + (double)R * (R + 1)) * 1.1102230246251565e-16 * tr;
         }
         shift_s = shift;
-        bad_s = 0;
     }
     __syncthreads();
     if (tid < R) G[tid][tid] += shift_s;
-    __syncthreads();
-    for (int j = 0; j < R; j++) {
-        if (tid == 0) {
-            const double d = G[j][j];
-            if (!(d > 0.0) || !isfinite(d)) {
-                bad_s = j + 1;
-                rjj_s = 1.0;
-            } else {
-                rjj_s = sqrt(d);
-                G[j][j] = rjj_s;
-            }
-        }
-        __syncthreads();
-        if (bad_s) {
-            if (tid == 0) atomicOr(flag, 4);
-            return;
-        }
-        const double rjj = rjj_s;
-        for (int k = j + 1 + tid; k < R; k += ALS_THREADS) G[j][k] /= rjj;
-        __syncthreads();
-        const int mm = R - j - 1;
-        for (int idx = tid; idx < mm * mm; idx += ALS_THREADS) {
-            const int i = j + 1 + idx % mm, k = j + 1 + idx / mm;
-            if (i <= k) G[i][k] -= G[j][i] * G[j][k];
-        }
-        __syncthreads();
+    // pad to ALS_MAXR with the identity: chol(diag(G, I)) = diag(chol(G), I)
+    for (int e = tid; e < ALS_MAXR * ALS_MAXR; e += ALS_THREADS) {
+        const int i = e % ALS_MAXR, j = e / ALS_MAXR;
+        if (i >= R || j >= R) G[i][j] = (i == j) ? 1.0 : 0.0;
     }
-    // R^{-1}: column j by thread j, back substitution straight into the
-    // output (each thread only reads back its own column).
-    for (int e = tid; e < R * R; e += ALS_THREADS)
-        if (e % R > e / R) Rinv_out[e] = 0.0;
-    if (tid < R) {
-        const int j = tid;
-        double *x = Rinv_out + (size_t)R * j;
-        for (int i = j; i >= 0; i--) {
-            double s = (i == j) ? 1.0 : 0.0;
-            for (int k = i + 1; k <= j; k++) s -= G[i][k] * x[k];
-            x[i] = s / G[i][i];
+    __syncthreads();
+    if (tid < 64) {
+        // One wave, lane k holding column k in registers; every index below is
+        // a compile-time constant after unrolling, so row j of the trailing
+        // update comes from v_readlane (lane i's col[j] = G[j][i]).
+        const int k = tid;
+        double col[ALS_MAXR];
+#pragma unroll
+        for (int i = 0; i < ALS_MAXR; i++) col[i] = G[i][k];
+        int bad = 0;
+#pragma unroll
+        for (int j = 0; j < ALS_MAXR; j++) {
+            double d = rl_f64(col[j], j);
+            if (!(d > 0.0) || !isfinite(d)) {
+                if (!bad) bad = j + 1;
+                d = 1.0;
+            }
+            const double rjj = sqrt(d), inv_d = 1.0 / d;
+            const double gjk = col[j];
+#pragma unroll
+            for (int i = j + 1; i < ALS_MAXR; i++) col[i] -= rl_f64(col[j], i) * gjk * inv_d;
+            col[j] = (k == j) ? rjj : gjk / rjj;  // row j of R (meaningful for k >= j)
         }
+#pragma unroll
+        for (int i = 0; i < ALS_MAXR; i++) X[i][k] = col[i];  // X = R (upper part)
+        if (bad && k == 0) atomicOr(flag, 4);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // column k of R^{-1}: x[i] = (delta_ik - sum_{m>i} R[i][m] x[m]) / R[i][i];
+        // x[m] = 0 for m > k falls out of the recurrence
+        double x[ALS_MAXR];
+#pragma unroll
+        for (int i = ALS_MAXR - 1; i >= 0; i--) {
+            double s0 = (i == k) ? 1.0 : 0.0, s1 = 0.0;
+#pragma unroll
+            for (int m = i + 1; m < ALS_MAXR; m += 2) {
+                s0 -= X[i][m] * x[m];
+                if (m + 1 < ALS_MAXR) s1 -= X[i][m + 1] * x[m + 1];
+            }
+            x[i] = (s0 + s1) / X[i][i];
+        }
+        if (!bad && k < R)
+#pragma unroll
+            for (int i = 0; i < ALS_MAXR; i++)
+                if (i < R) Rinv_out[i + (size_t)R * k] = x[i];
     }
 }
 
@@ -212,9 +313,13 @@ template <int RMAX>
 static int launch_rows(dt_ctx *ctx, int mode, const double *S, const double *Y, int I, int R,
                        const double *Rinv, double *out, double *part, float *U32, float *M32,
                        uint16_t *Ub16) {
-    const int nb = (I + ALS_RB - 1) / ALS_RB;
+    const int nb = (I + AlsRows<RMAX>::RB - 1) / AlsRows<RMAX>::RB;
     DT_LAUNCH("als_rows", als_rows_kernel<RMAX>, dim3(nb), dim3(ALS_THREADS), 0, mode, S, Y, I, R, Rinv,
-              out, part, U32, M32, Ub16);
+              out, U32, M32, Ub16);
+    if (part) {
+        const int ng = (I + ALS_GR - 1) / ALS_GR;
+        DT_LAUNCH("als_gram", als_gram_kernel<RMAX>, dim3(ng), dim3(ALS_THREADS), 0, out, I, R, part);
+    }
     return DT_OK;
 }
 
@@ -226,9 +331,9 @@ static int rows(dt_ctx *ctx, int mode, const double *S, const double *Y, int I, 
     return launch_rows<64>(ctx, mode, S, Y, I, R, Rinv, out, part, U32, M32, Ub16);
 }
 
-// Workspace needs: part (ceil(I/16) R^2) + 2 Rinv (R^2) + 2 panels (I R) doubles.
+// Workspace needs: part (ceil(I/64) R^2) + 2 Rinv (R^2) + 2 panels (I R) doubles.
 size_t dt_als_ws_doubles(size_t I, size_t R) {
-    return ((I + ALS_RB - 1) / ALS_RB) * R * R + 2 * R * R + 2 * I * R + 64;
+    return ((I + ALS_GR - 1) / ALS_GR) * R * R + 2 * R * R + 2 * I * R + 64;
 }
 
 // U (I x R, orthonormal, FP64) <- orthonormal basis of span(S^inner U).
@@ -236,7 +341,7 @@ size_t dt_als_ws_doubles(size_t I, size_t R) {
 int dt_als_update(dt_ctx *ctx, const double *S, double *U, int I, int R, int inner, double *ws,
                   int *flag, float *U32, float *M32, uint16_t *Ub16) {
     if (R < 1 || R > ALS_MAXR) return dt_fail(ctx, DT_EUNSUPPORTED, "ALS update: rank %d > %d", R, ALS_MAXR);
-    const int nb = (I + ALS_RB - 1) / ALS_RB;
+    const int nb = (I + ALS_GR - 1) / ALS_GR;  // partial Grams
     double *part = ws;
     double *Ri = part + (size_t)nb * R * R;
     double *Yb[2] = {Ri + 2 * R * R, Ri + 2 * R * R + (size_t)I * R};

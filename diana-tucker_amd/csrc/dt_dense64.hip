@@ -213,10 +213,74 @@ __global__ __launch_bounds__(256) void chol_upper_kernel(const double *__restric
     }
 }
 
+// The same factorisation with the matrix held in LDS (n <= LDS_NMAX): one
+// workgroup, two barriers per column instead of global-memory round trips.
+constexpr int LDS_NMAX = 96;
+__global__ __launch_bounds__(512) void chol_lds_kernel(const double *__restrict__ G, int n, double shift,
+                                                      double *__restrict__ R, double *__restrict__ Rinv,
+                                                      int *__restrict__ info) {
+    __shared__ double W[LDS_NMAX * LDS_NMAX];
+    __shared__ double X[LDS_NMAX * LDS_NMAX];
+    __shared__ int bad;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int idx = tid; idx < n * n; idx += nt) {
+        const int i = idx % n, j = idx / n;
+        W[idx] = (i <= j) ? G[idx] + (i == j ? shift : 0.0) : 0.0;
+    }
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    for (int j = 0; j < n; j++) {
+        const double d = W[j + n * j];
+        const bool ok = d > 0.0 && isfinite(d);
+        if (!ok) {
+            if (tid == 0) bad = j + 1;
+            break;  // uniform: every thread read the same d
+        }
+        const double rjj = sqrt(d);
+        // scale row j right of the diagonal (threads own distinct k)
+        for (int k = j + 1 + tid; k < n; k += nt) W[j + n * k] /= rjj;
+        __syncthreads();
+        if (tid == 0) W[j + n * j] = rjj;
+        const int m = n - j - 1;
+        for (int idx = tid; idx < m * m; idx += nt) {
+            const int i = j + 1 + idx % m, k = j + 1 + idx / m;
+            if (i <= k) W[i + n * k] -= W[j + n * i] * W[j + n * k];
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (tid == 0) info[0] = bad;
+    if (bad) return;
+    // Rinv by columns (thread c: upper-triangular back substitution in LDS)
+    for (int c = tid; c < n; c += nt) {
+        for (int i = c + 1; i < n; i++) X[i + n * c] = 0.0;
+        for (int i = c; i >= 0; i--) {
+            double sacc = (i == c) ? 1.0 : 0.0;
+            for (int k = i + 1; k <= c; k++) sacc -= W[i + n * k] * X[k + n * c];
+            X[i + n * c] = sacc / W[i + n * i];
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < n * n; idx += nt) {
+        R[idx] = W[idx];
+        Rinv[idx] = X[idx];
+    }
+}
+
+static int launch_chol(dt_ctx *ctx, const double *G, size_t n, double shift, double *R, double *Rinv,
+                       int *dinfo) {
+    if (n <= (size_t)LDS_NMAX) {
+        DT_LAUNCH("chol", chol_lds_kernel, dim3(1), dim3(512), 0, G, (int)n, shift, R, Rinv, dinfo);
+    } else {
+        DT_LAUNCH("chol", chol_upper_kernel, dim3(1), dim3(256), 0, G, (int)n, shift, R, Rinv, dinfo);
+    }
+    return DT_OK;
+}
+
 int dt_chol_upper(dt_ctx *ctx, const double *G, size_t n, double *R, double *Rinv, int *info_host) {
     int *dinfo = (int *)dt_ws(ctx, WS_STATUS, 64);
     if (!dinfo) return DT_ENOMEM;
-    DT_LAUNCH("chol", chol_upper_kernel, dim3(1), dim3(256), 0, G, (int)n, 0.0, R, Rinv, dinfo);
+    DT_TRY(launch_chol(ctx, G, n, 0.0, R, Rinv, dinfo));
     int *pin = (int *)ctx->pinned;
     DT_HIP(hipMemcpyAsync(pin, dinfo, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     DT_HIP(hipStreamSynchronize(ctx->stream));
@@ -282,7 +346,7 @@ extern "C" int dt_qr_f64(dt_ctx *ctx, double *Q, double *R, const double *A, siz
                 return dt_fail(ctx, DT_ENUMERIC, "dt_qr_f64: zero or non-finite matrix");
             const double shift = 11.0 * ((double)m * n + (double)n * (n + 1)) * 1.1102230246251565e-16 * tr;
             int *dinfo = (int *)dt_ws(ctx, WS_STATUS, 64);
-            DT_LAUNCH("chol", chol_upper_kernel, dim3(1), dim3(256), 0, G, (int)n, shift, Rk, Rki, dinfo);
+            DT_TRY(launch_chol(ctx, G, n, shift, Rk, Rki, dinfo));
             int *pin = (int *)ctx->pinned;
             DT_HIP(hipMemcpyAsync(pin, dinfo, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
             DT_HIP(hipStreamSynchronize(ctx->stream));
@@ -500,11 +564,128 @@ __global__ __launch_bounds__(256) void jacobi_kernel(double *__restrict__ H, int
     for (int idx = tid; idx < n * n; idx += nt) V[idx] = H[idx];
 }
 
+// Cyclic Jacobi with H and V resident in LDS (n <= LDS_NMAX), same
+// round-robin ordering; the off-diagonal norm is a block reduction.
+__global__ __launch_bounds__(512) void jacobi_lds_kernel(const double *__restrict__ Hin, int n,
+                                                        double *__restrict__ Vout, double *__restrict__ w) {
+    __shared__ double H[LDS_NMAX * LDS_NMAX];
+    __shared__ double V[LDS_NMAX * LDS_NMAX];
+    __shared__ double rc[LDS_NMAX / 2], rsn[LDS_NMAX / 2];
+    __shared__ int rp[LDS_NMAX / 2], rq[LDS_NMAX / 2];
+    __shared__ double red_off[16], red_tot[16];
+    __shared__ int order[LDS_NMAX];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int np = (n + 1) & ~1;
+    for (int idx = tid; idx < n * n; idx += nt) {
+        H[idx] = Hin[idx];
+        V[idx] = (idx % n == idx / n) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int sweep = 0; sweep < 40; sweep++) {
+        double off = 0.0, tot = 0.0;
+        for (int idx = tid; idx < n * n; idx += nt) {
+            const double h = H[idx];
+            tot += h * h;
+            if (idx % n != idx / n) off += h * h;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            off += __shfl_xor(off, o);
+            tot += __shfl_xor(tot, o);
+        }
+        if (lane == 0) {
+            red_off[wv] = off;
+            red_tot[wv] = tot;
+        }
+        __syncthreads();
+        off = 0.0;
+        tot = 0.0;
+        for (int i = 0; i < nt / 64; i++) {
+            off += red_off[i];
+            tot += red_tot[i];
+        }
+        __syncthreads();  // red_* reused next sweep
+        if (!(off > 1e-30 * tot) || tot == 0.0) break;  // uniform
+        for (int round = 0; round < np - 1; round++) {
+            for (int q = tid; q < np / 2; q += nt) {
+                int a, b;
+                if (q == 0) {
+                    a = 0;
+                    b = 1 + (round % (np - 1));
+                } else {
+                    a = 1 + ((round + q) % (np - 1));
+                    b = 1 + ((round + np - 1 - q) % (np - 1));
+                }
+                const int p = a < b ? a : b, qq = a < b ? b : a;
+                double c = 1.0, sn = 0.0;
+                if (qq < n) {
+                    const double apq = H[p + n * qq];
+                    if (fabs(apq) > 1e-300) {
+                        const double app = H[p + n * p], aqq = H[qq + n * qq];
+                        const double tau = (aqq - app) / (2.0 * apq);
+                        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        sn = t * c;
+                    }
+                }
+                rp[q] = p;
+                rq[q] = qq;
+                rc[q] = c;
+                rsn[q] = sn;
+            }
+            __syncthreads();
+            for (int idx = tid; idx < (np / 2) * n; idx += nt) {
+                const int q = idx / n, r = idx % n;
+                const int p = rp[q], qq = rq[q];
+                if (qq >= n) continue;
+                const double c = rc[q], sn = rsn[q];
+                const double hp = H[r + n * p], hq = H[r + n * qq];
+                H[r + n * p] = c * hp - sn * hq;
+                H[r + n * qq] = sn * hp + c * hq;
+                const double vp = V[r + n * p], vq = V[r + n * qq];
+                V[r + n * p] = c * vp - sn * vq;
+                V[r + n * qq] = sn * vp + c * vq;
+            }
+            __syncthreads();
+            for (int idx = tid; idx < (np / 2) * n; idx += nt) {
+                const int q = idx / n, r = idx % n;
+                const int p = rp[q], qq = rq[q];
+                if (qq >= n) continue;
+                const double c = rc[q], sn = rsn[q];
+                const double hp = H[p + n * r], hq = H[qq + n * r];
+                H[p + n * r] = c * hp - sn * hq;
+                H[qq + n * r] = sn * hp + c * hq;
+            }
+            __syncthreads();
+        }
+    }
+    // descending order by rank (ties broken by index)
+    for (int i = tid; i < n; i += nt) {
+        const double di = H[i + n * i];
+        int rank = 0;
+        for (int j = 0; j < n; j++) {
+            const double dj = H[j + n * j];
+            rank += (dj > di) || (dj == di && j < i);
+        }
+        order[rank] = i;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += nt) w[i] = H[order[i] + n * order[i]];
+    for (int idx = tid; idx < n * n; idx += nt) {
+        const int r = idx % n, j = idx / n;
+        Vout[idx] = V[r + n * order[j]];
+    }
+}
+
 int dt_jacobi_eig(dt_ctx *ctx, double *H, size_t n, double *V, double *w) {
     const size_t np = (n + 1) & ~(size_t)1;
     double *rot = (double *)dt_ws(ctx, WS_EIG_H, (2 * np + 8) * sizeof(double) + n * sizeof(int) + 256);
     if (!rot) return DT_ENOMEM;
     int *order = (int *)(rot + 2 * np + 8);
+    if (n <= (size_t)LDS_NMAX) {
+        DT_LAUNCH("jacobi", jacobi_lds_kernel, dim3(1), dim3(512), 0, H, (int)n, V, w);
+        return DT_OK;
+    }
     DT_LAUNCH("jacobi", jacobi_kernel, dim3(1), dim3(256), 0, H, (int)n, V, w, rot, order);
     return DT_OK;
 }
@@ -514,7 +695,7 @@ int dt_jacobi_eig(dt_ctx *ctx, double *H, size_t n, double *V, double *w) {
 extern "C" int dt_syev_top_f64(dt_ctx *ctx, const double *S, size_t n, size_t r, double *V,
                                double *w) {
     DT_CHECK_ARG(ctx && S && V && r >= 1 && r <= n, "dt_syev_top_f64: bad argument");
-    if (n <= 160) {
+    if (n <= (size_t)LDS_NMAX) {
         double *H = (double *)dt_ws(ctx, WS_EIG_X, 2 * n * n * sizeof(double) + n * sizeof(double));
         if (!H) return DT_ENOMEM;
         double *Vall = H + n * n;
@@ -526,7 +707,8 @@ extern "C" int dt_syev_top_f64(dt_ctx *ctx, const double *S, size_t n, size_t r,
         return DT_OK;
     }
     // Oversampled subspace iteration with Rayleigh-Ritz every few steps.
-    const size_t p = std::min(n, std::max(2 * r, r + 16));
+    // oversampling: r + 16 keeps the Rayleigh-Ritz problem in LDS for r <= 80
+    const size_t p = std::min(n, r + 16 <= (size_t)LDS_NMAX ? r + 16 : std::max(2 * r, r + 16));
     double *X = (double *)dt_ws(ctx, WS_EIG_X, (2 * n * p + 2 * p * p + 2 * p) * sizeof(double));
     if (!X) return DT_ENOMEM;
     double *Y = X + n * p;
