@@ -115,6 +115,98 @@ __global__ __launch_bounds__(256) void gram_mfma(const TA *__restrict__ A, float
             }
 }
 
+// Cross product C = A_(n) B_(n)ᵀ (IA x JB) of two tensors that differ only in
+// mode n, on the same MFMA tiling as gram_mfma: workgroup = one 64x64 tile
+// (ti over IA, tj over JB) and one column range (split-K).  FP32 partial
+// tiles, summed in FP64 by cross_reduce.
+template <typename TA, bool F1>
+__global__ __launch_bounds__(256) void cross_mfma(const TA *__restrict__ A, const TA *__restrict__ Bt,
+                                                  float *__restrict__ Cp, int64_t F, int64_t IA, int64_t JB,
+                                                  int64_t Ctot, int64_t cols_per_split, int ntI) {
+    __shared__ float Xa[64 * GS];
+    __shared__ float Xb[64 * GS];
+    const int ti = (int)(blockIdx.x % ntI), tj = (int)(blockIdx.x / ntI);
+    const int64_t c_begin = (int64_t)blockIdx.y * cols_per_split;
+    int64_t c_end = c_begin + cols_per_split;
+    if (c_end > Ctot) c_end = Ctot;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int c = lane & 15;
+    const int g = lane >> 4;
+    const int wi = (wave >> 1) * 32;
+    const int wj = (wave & 1) * 32;
+    floatx4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int64_t c0 = c_begin; c0 < c_end; c0 += 32) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int e = threadIdx.x + 256 * q;
+            int r, kc;
+            if (F1) {
+                r = e & 63;
+                kc = e >> 6;
+            } else {
+                kc = e & 31;
+                r = e >> 5;
+            }
+            const int64_t kk = c0 + kc;
+            float va = 0.f, vb = 0.f;
+            if (kk < c_end) {
+                int64_t fa, bb;
+                if (F1) {
+                    fa = 0;
+                    bb = kk;
+                } else {
+                    bb = kk / F;
+                    fa = kk - bb * F;
+                }
+                const int64_t ia = (int64_t)ti * 64 + r;
+                const int64_t jb = (int64_t)tj * 64 + r;
+                if (ia < IA) va = ldg_f<TA>(A + fa + F * (ia + IA * bb));
+                if (jb < JB) vb = ldg_f<TA>(Bt + fa + F * (jb + JB * bb));
+            }
+            Xa[r * GS + kc] = va;
+            Xb[r * GS + kc] = vb;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 32; ks += 4) {
+            const float a0 = Xa[(wi + c) * GS + ks + g];
+            const float a1 = Xa[(wi + 16 + c) * GS + ks + g];
+            const float b0 = Xb[(wj + c) * GS + ks + g];
+            const float b1 = Xb[(wj + 16 + c) * GS + ks + g];
+            acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    float *Cs = Cp + (int64_t)blockIdx.y * IA * JB;
+#pragma unroll
+    for (int mi = 0; mi < 2; mi++)
+#pragma unroll
+        for (int nj = 0; nj < 2; nj++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int64_t i = (int64_t)ti * 64 + wi + 16 * mi + 4 * g + v;
+                const int64_t j = (int64_t)tj * 64 + wj + 16 * nj + c;
+                if (i < IA && j < JB) Cs[i + IA * j] = acc[mi][nj][v];
+            }
+}
+
+__global__ void cross_reduce(const float *__restrict__ Cp, double *__restrict__ C, int64_t total, int nsplit) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        double s = 0.0;
+        for (int q = 0; q < nsplit; q++) s += (double)Cp[(int64_t)q * total + idx];
+        C[idx] = s;
+    }
+}
+
 // S[i, j] = sum over splits of the partial stored in the lower tile triangle.
 __global__ void gram_reduce(const float *__restrict__ Sp, double *__restrict__ S, int64_t I,
                             int nsplit) {
@@ -342,6 +434,43 @@ extern "C" int dt_gram(dt_ctx *ctx, const void *A, int dtA, int ndim, const size
     return dt_convert(ctx, S, DT_F32, tmp, DT_F64, (size_t)(I * I));
 }
 
+// C (IA x JB, FP64) = A_(n) B_(n)ᵀ for FP32 / bf16 tensors (split-K MFMA).
+int dt_cross_internal(dt_ctx *ctx, const void *A, const void *B, int dt, int64_t F, int64_t IA, int64_t JB,
+                      int64_t Bk, double *C) {
+    const int64_t Ctot = F * Bk;
+    const int ntI = (int)ceil_div(IA, 64), ntJ = (int)ceil_div(JB, 64);
+    const int ntiles = ntI * ntJ;
+    int64_t nsplit = 512 / ntiles;
+    if (nsplit < 1) nsplit = 1;
+    int64_t maxsplit = ceil_div(Ctot, 256);
+    if (maxsplit < 1) maxsplit = 1;
+    if (nsplit > maxsplit) nsplit = maxsplit;
+    if (nsplit > 256) nsplit = 256;
+    int64_t cps = ceil_div(ceil_div(Ctot, nsplit), 32) * 32;
+    nsplit = ceil_div(Ctot, cps);
+    float *Cp = (float *)dt_ws(ctx, WS_TTT, (size_t)(nsplit * IA * JB) * sizeof(float));
+    if (!Cp) return DT_ENOMEM;
+    dim3 grid((unsigned)ntiles, (unsigned)nsplit);
+#define CROSS(T, F1) \
+    DT_LAUNCH("cross_mfma", (cross_mfma<T, F1>), grid, dim3(256), 0, (const T *)A, (const T *)B, Cp, F, IA, JB, Ctot, cps, ntI)
+    if (dt == DT_F32) {
+        if (F == 1) {
+            CROSS(float, true);
+        } else {
+            CROSS(float, false);
+        }
+    } else {
+        if (F == 1) {
+            CROSS(uint16_t, true);
+        } else {
+            CROSS(uint16_t, false);
+        }
+    }
+#undef CROSS
+    DT_LAUNCH("cross_reduce", cross_reduce, gs_grid(IA * JB), dim3(256), 0, Cp, C, IA * JB, (int)nsplit);
+    return DT_OK;
+}
+
 extern "C" int dt_ttt_except(dt_ctx *ctx, const void *A, const void *B, int dt, int ndim,
                              const size_t *shapeA, size_t JB, int n, double *C) {
     DT_CHECK_ARG(ctx && A && B && C && shapeA, "dt_ttt_except: null argument");
@@ -353,11 +482,8 @@ extern "C" int dt_ttt_except(dt_ctx *ctx, const void *A, const void *B, int dt, 
     if (dt == DT_F64) {
         DT_LAUNCH("ttt_except", cross_gram_generic<double>, gs_grid(IA * (int64_t)JB), dim3(256), 0,
                   (const double *)A, (const double *)B, C, F, IA, (int64_t)JB, Bk);
-    } else if (dt == DT_F32) {
-        DT_LAUNCH("ttt_except", cross_gram_generic<float>, gs_grid(IA * (int64_t)JB), dim3(256), 0,
-                  (const float *)A, (const float *)B, C, F, IA, (int64_t)JB, Bk);
-    } else {
-        return dt_fail(ctx, DT_EUNSUPPORTED, "dt_ttt_except: type %d", dt);
+        return DT_OK;
     }
-    return DT_OK;
+    DT_CHECK_ARG(dt == DT_F32 || dt == DT_BF16, "dt_ttt_except: type %d", dt);
+    return dt_cross_internal(ctx, A, B, dt, F, IA, (int64_t)JB, Bk, C);
 }

@@ -565,7 +565,49 @@ int dt_hooi_dense(dt_ctx *ctx, const dt_hooi_args *args, dt_hooi_result *res) {
     int st = DT_OK;
     double prev = -1.0;
     const bool polish = (args->flags & DT_FLAG_FP32_POLISH) && args->dtype == DT_BF16 && args->A_f32;
-    for (int it = 0; it < args->max_iter; it++) {
+    int first = 0;
+    if (args->init == DT_INIT_LEARNED && args->max_iter > 0) {
+        // Learned start, then its first sweep.  Fallback to the HOSVD start when
+        // that sweep's fit is not a number in (0, 1], and, with
+        // DT_FLAG_LEARNED_CHECK, when it is worse than the first sweep from the
+        // HOSVD start.
+        double rl = 0.0;
+        st = dt_hooi_sweep(h, polish && args->max_iter == 1, &rl);
+        if (st != DT_OK) {
+            dt_hooi_destroy(h);
+            return st;
+        }
+        res->learned_first_res = rl;
+        res->hosvd_first_res = -1.0;
+        bool fallback = !(rl >= 0.0 && rl < 1.0);
+        dt_hooi *hh = nullptr;
+        double rh = 0.0;
+        if (fallback || (args->flags & DT_FLAG_LEARNED_CHECK)) {
+            dt_hooi_args a2 = *args;
+            a2.init = DT_INIT_HOSVD;
+            st = dt_hooi_create(ctx, &a2, &hh);
+            if (st == DT_OK) st = dt_hooi_sweep(hh, polish && args->max_iter == 1, &rh);
+            if (st != DT_OK) {
+                dt_hooi_destroy(hh);
+                dt_hooi_destroy(h);
+                return st;
+            }
+            res->hosvd_first_res = rh;
+            if (rh < rl) fallback = true;
+        }
+        if (fallback) {
+            dt_hooi_destroy(h);
+            h = hh;
+            h->init_used = DT_INIT_HOSVD;
+        } else if (hh) {
+            dt_hooi_destroy(hh);
+        }
+        const double r = fallback ? rh : rl;
+        if (res->residual_per_sweep) res->residual_per_sweep[0] = r;
+        prev = r;
+        first = 1;
+    }
+    for (int it = first; it < args->max_iter; it++) {
         const bool last = it == args->max_iter - 1;
         double r = 0.0;
         st = dt_hooi_sweep(h, polish && last, &r);

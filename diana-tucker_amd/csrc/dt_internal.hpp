@@ -184,6 +184,9 @@ int dt_gemm_f64_internal(dt_ctx *ctx, int tA, int tB, size_t m, size_t n, size_t
                          const double *A, size_t lda, const double *B, size_t ldb, double beta,
                          double *C, size_t ldc);
 int dt_jacobi_eig(dt_ctx *ctx, double *H, size_t n, double *V, double *w);
+// C (IA x JB, FP64) = A_(n) B_(n)ᵀ, FP32 or bf16 tensors of mode-n sizes IA and JB
+int dt_cross_internal(dt_ctx *ctx, const void *A, const void *B, int dt, int64_t F, int64_t IA, int64_t JB,
+                      int64_t Bk, double *C);
 int dt_copy_f64(dt_ctx *ctx, double *dst, const double *src, size_t n);
 int dt_fill_normal_f64(dt_ctx *ctx, double *out, size_t n, uint64_t seed, uint64_t stream);
 // fused single-workgroup FP64 steps (dt_small.hip); failures OR bits into *flag

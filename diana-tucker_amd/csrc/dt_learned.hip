@@ -1,9 +1,160 @@
-// Learned start factors (sketch -> MLP -> QR).  Placeholder until the MLP
-// kernels are in: reports DT_EUNSUPPORTED so callers fall back explicitly.
+// Learned start factors: Gaussian sketch -> features -> row-wise MLP -> QR.
+//
+// No counterpart in the reference, which starts HOOI from random factors
+// (template/algorithm/tucker/hooi_als.tpp:76-81).  For mode n the sketch is
+// Y = A_(n) Ω with Ω a Gaussian (prod_{m!=n} I_m) x S matrix stored as a
+// tensor of A's shape with mode n replaced by S (Philox stream 5000 + n), so
+// Y is the cross product A_(n) Ω_(n)ᵀ (dt_cross_internal, split-K MFMA).
+// With YᵀY = V diag(σ²) Vᵀ (σ descending), the features are
+//     X = sqrt(I_n) / σ_1 · Y V          (I_n x S, row-major FP32),
+// i.e. the sketch's left singular vectors scaled by σ_k / σ_1, each column's
+// sign fixed so that its entry of largest magnitude is positive.  The first
+// R columns span the randomized range finder's estimate of the leading
+// subspace; the MLP's residual form (z = x[:R] + ...) starts from exactly
+// that estimate.
 #include "dt_internal.hpp"
 
+#include <cmath>
+#include <vector>
+
+int dt_mlp_forward_dev(dt_ctx *ctx, dt_mlp *m, const float *X, int64_t rows, float *Z);
+
+namespace {
+
+// sg[k] = sign of the entry of largest magnitude of column k of YV (first
+// index on ties), so the features do not depend on the eigen-solver's signs.
+__global__ __launch_bounds__(256) void column_signs(const double *__restrict__ YV, int64_t I,
+                                                   double *__restrict__ sg) {
+    __shared__ double bv[256];
+    __shared__ int64_t bi[256];
+    const int k = blockIdx.x;
+    double best = -1.0;
+    int64_t arg = 0;
+    for (int64_t i = threadIdx.x; i < I; i += 256) {
+        const double a = fabs(YV[i + I * k]);
+        if (a > best) {
+            best = a;
+            arg = i;
+        }
+    }
+    bv[threadIdx.x] = best;
+    bi[threadIdx.x] = arg;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            const double v2 = bv[threadIdx.x + o];
+            const int64_t i2 = bi[threadIdx.x + o];
+            if (v2 > bv[threadIdx.x] || (v2 == bv[threadIdx.x] && i2 < bi[threadIdx.x])) {
+                bv[threadIdx.x] = v2;
+                bi[threadIdx.x] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sg[k] = YV[bi[0] + I * k] < 0.0 ? -1.0 : 1.0;
+}
+
+// X[i * S + k] = sg[k] * scale * YV[i + I k], scale = sqrt(I) / sqrt(w[0])
+__global__ void sketch_rows(const double *__restrict__ YV, const double *__restrict__ w,
+                           const double *__restrict__ sg, int64_t I, int S, float *__restrict__ X) {
+    const double s1 = sqrt(w[0]);
+    const double scale = s1 > 0.0 ? sqrt((double)I) / s1 : 0.0;
+    const int64_t total = I * S;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / S;
+        const int k = (int)(e % S);
+        X[e] = (float)(sg[k] * scale * YV[i + I * k]);
+    }
+}
+
+// U (I x R, column-major FP64) = Z rows (row-major FP32, stride R)
+__global__ void rows_to_cols(const float *__restrict__ Z, int64_t I, int R, double *__restrict__ U) {
+    const int64_t total = I * R;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e % I;
+        const int j = (int)(e / I);
+        U[e] = (double)Z[i * R + j];
+    }
+}
+
+inline dim3 lg_grid(int64_t n) { return dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)); }
+
+}  // namespace
+
+extern "C" int dt_sketch_features(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n,
+                                  int S, uint64_t seed, float *X) {
+    DT_CHECK_ARG(ctx && A && shape && X, "dt_sketch_features: null argument");
+    DT_CHECK_ARG(ndim >= 2 && ndim <= DT_MAX_NDIM && n >= 0 && n < ndim, "dt_sketch_features: bad mode");
+    DT_CHECK_ARG(dtA == DT_F32 || dtA == DT_BF16, "dt_sketch_features: tensor must be F32 or BF16");
+    DT_CHECK_ARG(S >= 1 && S <= 96, "dt_sketch_features: sketch width %d not in [1, 96]", S);
+    int64_t F = 1, Bk = 1;
+    for (int i = 0; i < n; i++) F *= (int64_t)shape[i];
+    for (int i = n + 1; i < ndim; i++) Bk *= (int64_t)shape[i];
+    const int64_t I = (int64_t)shape[n];
+    const int64_t nom = F * S * Bk;
+    const size_t esz = dtA == DT_BF16 ? 2 : 4;
+    void *Om = dt_ws(ctx, WS_MLP0, (size_t)nom * esz);
+    double *Y = (double *)dt_ws(ctx, WS_MLP1, (size_t)(2 * I * S + 2 * S * S + 2 * S + 64) * sizeof(double));
+    if (!Om || !Y) return DT_ENOMEM;
+    double *YV = Y + I * S;
+    double *G = YV + I * S;
+    double *V = G + (int64_t)S * S;
+    double *w = V + (int64_t)S * S;
+    double *sg = w + S;
+    DT_TRY(dt_gen_normal(ctx, Om, dtA, (size_t)nom, seed, 5000 + (uint64_t)n));
+    DT_TRY(dt_cross_internal(ctx, A, Om, dtA, F, I, S, Bk, Y));
+    DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, (size_t)S, (size_t)S, (size_t)I, 1.0, Y, (size_t)I, Y, (size_t)I, 0.0, G,
+                                (size_t)S));
+    DT_TRY(dt_jacobi_eig(ctx, G, (size_t)S, V, w));
+    DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, (size_t)I, (size_t)S, (size_t)S, 1.0, Y, (size_t)I, V, (size_t)S, 0.0, YV,
+                                (size_t)I));
+    DT_LAUNCH("sketch_rows", column_signs, dim3((unsigned)S), dim3(256), 0, YV, I, sg);
+    DT_LAUNCH("sketch_rows", sketch_rows, lg_grid(I * S), dim3(256), 0, YV, w, sg, I, S, X);
+    return DT_OK;
+}
+
+extern "C" int dt_learned_start(dt_ctx *ctx, dt_mlp *mlp, const void *A, int dtA, int ndim, const size_t *shape,
+                                const size_t *ranks, uint64_t seed, double *const *U) {
+    DT_CHECK_ARG(ctx && mlp && A && shape && ranks && U, "dt_learned_start: null argument");
+    DT_CHECK_ARG(ndim >= 2 && ndim <= DT_MAX_NDIM, "dt_learned_start: bad ndim");
+    int din = 0, hid = 0, dout = 0;
+    int64_t np = 0;
+    dt_mlp_shape(mlp, &din, &hid, &dout, &np);
+    int64_t rows = 0;
+    for (int n = 0; n < ndim; n++) {
+        DT_CHECK_ARG(U[n], "dt_learned_start: factor %d missing", n);
+        DT_CHECK_ARG((int)ranks[n] == dout, "dt_learned_start: rank %zu of mode %d differs from the model's %d",
+                     ranks[n], n, dout);
+        DT_CHECK_ARG(ranks[n] <= shape[n], "dt_learned_start: rank above mode size");
+        rows += (int64_t)shape[n];
+    }
+    float *X = (float *)dt_ws(ctx, WS_MLP2, (size_t)rows * (din + dout) * sizeof(float));
+    if (!X) return DT_ENOMEM;
+    float *Z = X + rows * din;
+    int64_t off = 0;
+    for (int n = 0; n < ndim; n++) {
+        DT_TRY(dt_sketch_features(ctx, A, dtA, ndim, shape, n, din, seed, X + off * din));
+        off += (int64_t)shape[n];
+    }
+    DT_TRY(dt_mlp_forward_dev(ctx, mlp, X, rows, Z));
+    off = 0;
+    for (int n = 0; n < ndim; n++) {
+        const int64_t I = (int64_t)shape[n];
+        DT_LAUNCH("learned_rows", rows_to_cols, lg_grid(I * dout), dim3(256), 0, Z + off * dout, I, dout, U[n]);
+        DT_TRY(dt_qr_f64(ctx, U[n], nullptr, U[n], (size_t)I, (size_t)dout));
+        off += I;
+    }
+    return DT_OK;
+}
+
+// HOOI hook (dt_hooi.hip): learned start for every mode.
 int dt_learned_init(dt_ctx *ctx, const void *model, const void *A, int dtA, int N, const int64_t *I,
                     const int64_t *R, uint64_t seed, double *const *U64) {
-    (void)model; (void)A; (void)dtA; (void)N; (void)I; (void)R; (void)seed; (void)U64;
-    return dt_fail(ctx, DT_EUNSUPPORTED, "learned init: no model support built");
+    if (!model) return dt_fail(ctx, DT_EINVAL, "learned init: no model (dt_hooi_args.model)");
+    size_t sh[DT_MAX_NDIM], rk[DT_MAX_NDIM];
+    for (int n = 0; n < N; n++) {
+        sh[n] = (size_t)I[n];
+        rk[n] = (size_t)R[n];
+    }
+    return dt_learned_start(ctx, (dt_mlp *)model, A, dtA, N, sh, rk, seed, U64);
 }

@@ -20,6 +20,7 @@ DT_MAX_NDIM = 8
 INIT_GIVEN, INIT_QR, INIT_HOSVD, INIT_RANDOM, INIT_LEARNED = range(5)
 SCHED_REFERENCE, SCHED_TREE = 0, 1
 FLAG_DIRECT_RESIDUAL, FLAG_FP32_POLISH, FLAG_FP64_FIT, FLAG_REFERENCE_ALS = 1, 2, 4, 8
+FLAG_LEARNED_CHECK = 16
 
 _NP_OF = {DT_F64: np.float64, DT_F32: np.float32, DT_BF16: np.uint16, DT_I32: np.int32}
 _SIZE_OF = {DT_F64: 8, DT_F32: 4, DT_BF16: 2, DT_I32: 4}
@@ -98,6 +99,16 @@ def lib():
             "dt_hooi_sweep": [vp, i32, C.POINTER(dbl)],
             "dt_hooi_get": [vp, C.POINTER(HooiResult)],
             "dt_hooi_destroy": [vp],
+            "dt_mlp_create": [vp, C.POINTER(vp), i32, i32, i32, u64],
+            "dt_mlp_destroy": [vp, vp],
+            "dt_mlp_shape": [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(C.c_int64)],
+            "dt_mlp_get_params": [vp, vp, C.POINTER(C.c_float), C.c_int64],
+            "dt_mlp_set_params": [vp, vp, C.POINTER(C.c_float), C.c_int64],
+            "dt_mlp_forward": [vp, vp, vp, C.c_int64, vp],
+            "dt_mlp_loss_grad": [vp, vp, vp, vp, C.c_int64, C.POINTER(dbl), C.POINTER(C.c_float)],
+            "dt_mlp_adam_step": [vp, vp, C.c_float],
+            "dt_sketch_features": [vp, vp, i32, i32, C.POINTER(sz), i32, i32, u64, vp],
+            "dt_learned_start": [vp, vp, vp, i32, i32, C.POINTER(sz), C.POINTER(sz), u64, C.POINTER(vp)],
         }
         for name, argtypes in sig.items():
             f = getattr(L, name)
@@ -115,7 +126,10 @@ def exported_symbols():
             "dt_timer_stop", "dt_ttm", "dt_ttmc", "dt_gram", "dt_ttt_except", "dt_sumsq", "dt_gemm_f64",
             "dt_qr_f64", "dt_inverse_f64", "dt_syev_top_f64", "dt_orth_rows_f64",
             "dt_gen_normal", "dt_gen_uniform", "dt_gen_tucker", "dt_hooi_dense",
-            "dt_hooi_create", "dt_hooi_sweep", "dt_hooi_get", "dt_hooi_destroy"]
+            "dt_hooi_create", "dt_hooi_sweep", "dt_hooi_get", "dt_hooi_destroy",
+            "dt_mlp_create", "dt_mlp_destroy", "dt_mlp_shape", "dt_mlp_get_params", "dt_mlp_set_params",
+            "dt_mlp_forward", "dt_mlp_loss_grad", "dt_mlp_adam_step", "dt_sketch_features",
+            "dt_learned_start"]
 
 
 class DeviceArray:
@@ -401,6 +415,27 @@ class Context:
         args.model = model
         return args
 
+    # learned start
+    def rows_array(self, rows, width) -> DeviceArray:
+        """Row-major rows x width FP32 device buffer (stored as a width x rows column-major array)."""
+        return self.array((width, rows), DT_F32)
+
+    def sketch_features(self, A: DeviceArray, n: int, S: int, seed: int) -> np.ndarray:
+        X = self.rows_array(A.shape[n], S)
+        shp = (C.c_size_t * len(A.shape))(*A.shape)
+        self.check(self.L.dt_sketch_features(self.handle, A.ptr, A.dtype, len(A.shape), shp, n, S, seed,
+                                             X.ptr))
+        return X.download().T.copy()
+
+    def learned_start(self, mlp: "MLP", A: DeviceArray, ranks, seed: int):
+        N = len(A.shape)
+        Us = [self.array((A.shape[i], ranks[i]), DT_F64) for i in range(N)]
+        shp = (C.c_size_t * N)(*A.shape)
+        rk = (C.c_size_t * N)(*ranks)
+        up = (C.c_void_p * N)(*[u.ptr for u in Us])
+        self.check(self.L.dt_learned_start(self.handle, mlp.handle, A.ptr, A.dtype, N, shp, rk, seed, up))
+        return Us
+
     def hooi_dense(self, A: DeviceArray, ranks, want_factors=True, **kw):
         args = self._hooi_args(A, ranks, **kw)
         N = len(A.shape)
@@ -468,3 +503,60 @@ class HooiSession:
             self.close()
         except Exception:
             pass
+
+
+class MLP:
+    """Row-wise 3-layer MLP of the learned start (dt_mlp_* entry points)."""
+
+    def __init__(self, ctx: Context, d_in: int, hidden: int, d_out: int, seed: int = 0):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.check(ctx.L.dt_mlp_create(ctx.handle, C.byref(h), d_in, hidden, d_out, seed))
+        self.handle = h
+        a, b, c, n = C.c_int(), C.c_int(), C.c_int(), C.c_int64()
+        ctx.L.dt_mlp_shape(h, C.byref(a), C.byref(b), C.byref(c), C.byref(n))
+        self.d_in, self.hidden, self.d_out, self.nparam = a.value, b.value, c.value, n.value
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.nparam, dtype=np.float32)
+        self.ctx.check(self.ctx.L.dt_mlp_get_params(self.ctx.handle, self.handle,
+                                                     out.ctypes.data_as(C.POINTER(C.c_float)), self.nparam))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparam
+        self.ctx.check(self.ctx.L.dt_mlp_set_params(self.ctx.handle, self.handle,
+                                                     p.ctypes.data_as(C.POINTER(C.c_float)), self.nparam))
+
+    def forward(self, X: DeviceArray, rows: int) -> DeviceArray:
+        Z = self.ctx.rows_array(rows, self.d_out)
+        self.ctx.check(self.ctx.L.dt_mlp_forward(self.ctx.handle, self.handle, X.ptr, rows, Z.ptr))
+        return Z
+
+    def loss_grad(self, X: DeviceArray, T: DeviceArray, rows: int, want_grad=False):
+        loss = C.c_double()
+        g = np.empty(self.nparam, dtype=np.float32) if want_grad else None
+        gp = g.ctypes.data_as(C.POINTER(C.c_float)) if want_grad else None
+        self.ctx.check(self.ctx.L.dt_mlp_loss_grad(self.ctx.handle, self.handle, X.ptr, T.ptr, rows,
+                                                    C.byref(loss), gp))
+        return (loss.value, g) if want_grad else loss.value
+
+    def adam_step(self, lr: float):
+        self.ctx.check(self.ctx.L.dt_mlp_adam_step(self.ctx.handle, self.handle, lr))
+
+    def close(self):
+        if self.handle:
+            self.ctx.L.dt_mlp_destroy(self.ctx.handle, self.handle)
+            self.handle = None
+
+
+def upload_rows(ctx: Context, X: np.ndarray) -> DeviceArray:
+    """Upload a host (rows x width) array as a row-major FP32 device buffer."""
+    X = np.asarray(X, dtype=np.float32)
+    D = ctx.rows_array(X.shape[0], X.shape[1])
+    return D.upload(X.T)
+
+
+def download_rows(D: DeviceArray) -> np.ndarray:
+    return D.download().T.copy()

@@ -169,6 +169,39 @@ typedef struct dt_tucker_spec {
 int dt_gen_tucker(dt_ctx *ctx, const dt_tucker_spec *spec, float *A_f32, uint16_t *A_bf16,
                   double *const *U_true, double *normX_host, double *normA_host);
 
+/* ---- learned start ------------------------------------------------------
+ * No counterpart in the reference (it starts from random factors,
+ * hooi_als.tpp:76-81): the learned initialisation that replaces the HOSVD
+ * start.  Per mode n, a Gaussian sketch Y = A_(n) Ω (Ω: prod_{m!=n} I_m x S,
+ * Philox stream 5000 + n) gives features X = sqrt(I_n) / σ_1 · Y V (rows of the
+ * sketch in the frame of its right singular vectors V, ordered by singular
+ * value σ); a row-wise MLP maps each row of X (S values) to R values,
+ *     z = x[:R] + W3ᵀ relu(W2ᵀ relu(W1ᵀ x + b1) + b2) + b3,
+ * and the start factor is Q of QR(Z).  Weights FP32 (training master copy),
+ * rounded to bf16 in every product; products on bf16 MFMA with FP32
+ * accumulation.  Parameter order: W1 (S x H, row-major), b1, W2 (H x H), b2,
+ * W3 (H x R), b3. */
+typedef struct dt_mlp dt_mlp;
+int dt_mlp_create(dt_ctx *ctx, dt_mlp **mlp, int d_in, int hidden, int d_out, uint64_t seed);
+int dt_mlp_destroy(dt_ctx *ctx, dt_mlp *mlp);
+int dt_mlp_shape(const dt_mlp *mlp, int *d_in, int *hidden, int *d_out, int64_t *nparam);
+int dt_mlp_get_params(dt_ctx *ctx, const dt_mlp *mlp, float *host, int64_t n);
+int dt_mlp_set_params(dt_ctx *ctx, dt_mlp *mlp, const float *host, int64_t n);
+/* Z (rows x d_out) = MLP(X (rows x d_in)); device, row-major FP32. */
+int dt_mlp_forward(dt_ctx *ctx, dt_mlp *mlp, const float *X, int64_t rows, float *Z);
+/* loss = sum_rows |MLP(x) - t|^2 / rows and its gradient (kept for
+ * dt_mlp_adam_step; copied to grad_host, nparam floats, when non-NULL). */
+int dt_mlp_loss_grad(dt_ctx *ctx, dt_mlp *mlp, const float *X, const float *T, int64_t rows,
+                     double *loss, float *grad_host);
+int dt_mlp_adam_step(dt_ctx *ctx, dt_mlp *mlp, float lr);
+/* Sketch features of mode n: X (shape[n] x S, device, row-major FP32). */
+int dt_sketch_features(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n,
+                       int S, uint64_t seed, float *X);
+/* Learned start factors U[n] (device FP64, shape[n] x ranks[n], orthonormal);
+ * mlp d_in = sketch width, d_out = every ranks[n]. */
+int dt_learned_start(dt_ctx *ctx, dt_mlp *mlp, const void *A, int dtA, int ndim,
+                     const size_t *shape, const size_t *ranks, uint64_t seed, double *const *U);
+
 /* ---- dense HOOI --------------------------------------------------------- */
 enum {
     DT_INIT_GIVEN = 0,   /* U_init[n] are orthonormal start factors */
@@ -190,10 +223,12 @@ enum {
     DT_FLAG_REFERENCE_ALS = 8,   /* per-mode update exactly as ALS_ writes it (LU inverses
                                     of LᵀL and PᵀSP); default: the span-equivalent
                                     Cholesky-normalised iteration */
-    DT_FLAG_FP64_FIT = 4         /* form the core for the residual with FP64 accumulation
+    DT_FLAG_FP64_FIT = 4,        /* form the core for the residual with FP64 accumulation
                                     (one extra pass over A per sweep); needed when the
                                     residual is below ~1e-3, where FP32 rounding of the
                                     core dominates 1 - ||G||^2/||A||^2 */
+    DT_FLAG_LEARNED_CHECK = 16   /* DT_INIT_LEARNED: also run the first sweep from the HOSVD
+                                    start and keep whichever fits better (the fallback) */
 };
 
 typedef struct dt_hooi_args {

@@ -1,0 +1,102 @@
+"""Learned start restated in numpy -- TEST INFRASTRUCTURE ONLY.
+
+The reference has no learned component (it starts HOOI from random factors,
+template/algorithm/tucker/hooi_als.tpp:76-81); this restates the build's own
+definition so the GPU kernels can be checked:
+
+* sketch features of mode n (diana-tucker_amd/csrc/dt_learned.hip):
+  Ω = Philox normals (stream 5000 + n) laid out as a tensor of A's shape with
+  mode n replaced by S; Y = A_(n) Ω_(n)ᵀ; YᵀY = V diag(σ²) Vᵀ (σ descending);
+  X = sqrt(I_n) / σ_1 · Y V with each column's sign set so its entry of
+  largest magnitude is positive;
+* the row-wise MLP (diana-tucker_amd/csrc/dt_mlp.hip):
+  h1 = relu(x W1 + b1), h2 = relu(h1 W2 + b2), z = x[:R] + h2 W3 + b3,
+  loss = sum_rows |z - t|^2 / rows, with every product's operands rounded to
+  bf16 (round to nearest even) and accumulated here in FP64.
+
+The backward pass is pinned by central finite differences in
+tests/test_oracle_cpu.py (with rounding off).
+"""
+import numpy as np
+
+from . import dense as D
+from . import philox
+
+
+def bf16_round(x):
+    """Round-to-nearest-even to bf16, returned as float64."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    lsb = (u >> np.uint64(16)) & np.uint64(1)
+    r = ((u + np.uint64(0x7FFF) + lsb) >> np.uint64(16)) << np.uint64(16)
+    return r.astype(np.uint32).view(np.float32).astype(np.float64).reshape(np.shape(x))
+
+
+def unpack(params, din, hid, dout):
+    p = np.asarray(params, dtype=np.float64)
+    o = 0
+    out = []
+    for shape in ((din, hid), (hid,), (hid, hid), (hid,), (hid, dout), (dout,)):
+        n = int(np.prod(shape))
+        out.append(p[o:o + n].reshape(shape))  # row-major, as the device stores them
+        o += n
+    assert o == p.size
+    return out
+
+
+def pack(parts):
+    return np.concatenate([np.asarray(x, dtype=np.float64).ravel() for x in parts])
+
+
+def forward(params, X, din, hid, dout, rounding=True):
+    W1, b1, W2, b2, W3, b3 = unpack(params, din, hid, dout)
+    q = bf16_round if rounding else (lambda a: np.asarray(a, dtype=np.float64))
+    X = np.asarray(X, dtype=np.float64)
+    a1 = q(X) @ q(W1) + b1
+    h1 = np.maximum(a1, 0.0)
+    a2 = q(h1) @ q(W2) + b2
+    h2 = np.maximum(a2, 0.0)
+    z = X[:, :dout] + q(h2) @ q(W3) + b3
+    return z, (X, h1, h2)
+
+
+def loss_grad(params, X, T, din, hid, dout, rounding=True):
+    """loss and gradient (same parameter order as dt_mlp_get_params)."""
+    W1, b1, W2, b2, W3, b3 = unpack(params, din, hid, dout)
+    q = bf16_round if rounding else (lambda a: np.asarray(a, dtype=np.float64))
+    z, (X, h1, h2) = forward(params, X, din, hid, dout, rounding)
+    rows = X.shape[0]
+    d = z - np.asarray(T, dtype=np.float64)
+    loss = float(np.sum(d * d) / rows)
+    dz = 2.0 * d / rows
+    gW3 = q(h2).T @ q(dz)
+    gb3 = dz.sum(axis=0)
+    dh2 = (q(dz) @ q(W3).T) * (h2 > 0)
+    gW2 = q(h1).T @ q(dh2)
+    gb2 = dh2.sum(axis=0)
+    dh1 = (q(dh2) @ q(W2).T) * (h1 > 0)
+    gW1 = q(X).T @ q(dh1)
+    gb1 = dh1.sum(axis=0)
+    return loss, pack([gW1, gb1, gW2, gb2, gW3, gb3])
+
+
+def sketch_features(A, n, S, seed):
+    """X (I_n x S) of dt_sketch_features for an FP32/FP64 tensor A (values as given)."""
+    A = np.asarray(A, dtype=np.float64)
+    shape = list(A.shape)
+    oshape = shape.copy()
+    oshape[n] = S
+    om = philox.normal(int(np.prod(oshape)), seed, 5000 + n).astype(np.float32).astype(np.float64)
+    Om = om.reshape(oshape, order="F")
+    Y = D.ttt_except(A, Om, n)                    # A_(n) Ω_(n)ᵀ, I_n x S
+    w, V = np.linalg.eigh(Y.T @ Y)
+    order = np.argsort(-w, kind="stable")
+    w, V = w[order], V[:, order]
+    YV = Y @ V
+    idx = np.argmax(np.abs(YV), axis=0)
+    sg = np.where(YV[idx, np.arange(S)] < 0, -1.0, 1.0)
+    return np.sqrt(A.shape[n]) / np.sqrt(w[0]) * YV * sg
+
+
+def start_from_features(Z):
+    """Orthonormal basis (positive R diagonal) of the columns of Z."""
+    return D.orth_posdiag(np.asarray(Z, dtype=np.float64))

@@ -176,3 +176,56 @@ def test_reference_random_init_orthonormal_and_repeatable():
     assert all(np.array_equal(a, b) for a, b in zip(raws, raws2))
     for U in Us:
         assert np.allclose(U.T @ U, np.eye(U.shape[1]), atol=1e-12)
+
+
+# ---- learned start (no reference counterpart; oracle/learned.py) -----------
+
+def test_mlp_oracle_backward_matches_finite_differences():
+    """Central differences pin the oracle's hand-written backward (rounding off)."""
+    from oracle import learned as Lr
+    rng = np.random.default_rng(0)
+    din, hid, dout = 6, 5, 3
+    n = din * hid + hid + hid * hid + hid + hid * dout + dout
+    p = rng.standard_normal(n)
+    X = rng.standard_normal((9, din))
+    T = rng.standard_normal((9, dout))
+    _, g = Lr.loss_grad(p, X, T, din, hid, dout, rounding=False)
+    eps = 1e-6
+    gn = np.empty(n)
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = eps
+        gn[i] = (Lr.loss_grad(p + e, X, T, din, hid, dout, False)[0]
+                 - Lr.loss_grad(p - e, X, T, din, hid, dout, False)[0]) / (2 * eps)
+    assert np.max(np.abs(g - gn)) < 1e-6 * max(1.0, np.max(np.abs(g)))
+
+
+def test_bf16_round_matches_binding():
+    import sys
+    from oracle import learned as Lr
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "diana-tucker_amd", "python"))
+    import dtucker as dt
+    x = np.random.default_rng(1).standard_normal(1000).astype(np.float32) * 100
+    x[:4] = [1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -1.0 - 2 ** -8, 65504.0]  # ties round to even
+    assert np.array_equal(Lr.bf16_round(x), dt.f32_from_bf16(dt.bf16_from_f32(x)).astype(np.float64))
+    assert Lr.bf16_round(np.float32(1.0 + 2 ** -8)) == 1.0
+
+
+def test_sketch_features_closed_form():
+    """X = sqrt(I)/σ1 · Y V: orthogonal columns with norms sqrt(I) σ_k/σ_1 spanning
+    range(Y); the leading R columns recover an exactly rank-R tensor's subspace."""
+    from oracle import learned as Lr
+    rng = np.random.default_rng(2)
+    G = rng.standard_normal((3, 3, 3))
+    Us = [np.linalg.qr(rng.standard_normal((I, 3)))[0] for I in (12, 10, 8)]
+    A = D.reconstruct(G, Us)
+    for n in range(3):
+        X = Lr.sketch_features(A, n, 6, 5)
+        XtX = X.T @ X
+        assert np.allclose(XtX - np.diag(np.diag(XtX)), 0, atol=1e-8 * XtX[0, 0])
+        assert np.isclose(XtX[0, 0], A.shape[n])
+        assert D.sin_max_angle(np.linalg.qr(X[:, :3])[0], Us[n]) < 1e-8
+        # sign convention: largest-magnitude entry of each column positive
+        idx = np.argmax(np.abs(X[:, :3]), axis=0)
+        assert np.all(X[idx, np.arange(3)] > 0)
