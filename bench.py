@@ -104,6 +104,110 @@ def dense_c2(ctx, dt, steps, warmup, shape=(256,) * 4, ranks=(16,) * 4, noise=1e
             "kbytes": kbytes, "init_ms": out["init_ms"], "A": g["A"], "total": total}
 
 
+def _median_ms(fn, reps=3):
+    vals = sorted(fn() for _ in range(reps))
+    return vals[len(vals) // 2]
+
+
+def mixed_c3(ctx, dt, shape=(1024,) * 3, ranks=(64,) * 3, noise=1e-3, decay=0.93, seed=20000905 + 2,
+             max_sweeps=4):
+    """Time-to-fit on the 1024^3 rank-64 workload (core 64^3 with mode spectra
+    decaying as 0.93^k, noise 1e-3): wall time, including the start, until
+    the fit is within 1e-4 of the reference fit (FP32 HOOI from the HOSVD
+    start with FP64-accumulated fit, 3 sweeps).  The sweep count each path
+    needs is found in a run that evaluates the fit in FP64 after every sweep;
+    the timed runs then do exactly that many sweeps:
+      plain FP32  : HOSVD start on the FP32 tensor + FP32 sweeps (their own residual);
+      mixed+learned: learned start (Kronecker sketch of the BF16 tensor -> MLP
+                    -> QR) + BF16 sweeps + one fit of the factors on the FP32 copy."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "diana-tucker_amd", "python"))
+    import learned as LN
+    log(f"[bench] mixed precision: generating {shape} BF16 + FP32 (ranks {ranks})")
+    g = ctx.gen_tucker(shape, ranks, noise, seed=seed, decay=decay, bf16=True, want_factors=True)
+    A32, A16 = g["A"], g["A_bf16"]
+    ref = ctx.hooi_dense(A32, ranks, max_iter=3, init=dt.INIT_HOSVD, flags=dt.FLAG_FP64_FIT)
+    target = 1.0 - ref["residual"]
+    model = LN.load_model(ctx, os.path.join(LN.MODEL_DIR, "learned_c3.npz"))
+    kron = dt.FLAG_SKETCH_KRON
+
+    def sweeps_needed(A, init, **kw):
+        s = ctx.hooi_session(A, ranks, init=init, **kw)
+        res = []
+        for k in range(max_sweeps):
+            res.append(s.sweep())
+            if 1.0 - res[-1] >= target - 1e-4:
+                break
+        out = s.result(want_factors=True)
+        s.close()
+        return len(res), res, out["U"]
+
+    k_plain, res_plain, U_plain = sweeps_needed(A32, dt.INIT_HOSVD, flags=dt.FLAG_FP64_FIT)
+    k_mixed, res_mixed, U_mixed = sweeps_needed(A16, dt.INIT_LEARNED, model=model.handle, A_f32=A32,
+                                                flags=dt.FLAG_FP64_FIT | kron, seed=seed)
+
+    def run_plain():
+        ctx.sync()
+        t0 = time.perf_counter()
+        s = ctx.hooi_session(A32, ranks, init=dt.INIT_HOSVD)
+        for _ in range(k_plain):
+            s.sweep()
+        ctx.sync()
+        ms = (time.perf_counter() - t0) * 1e3
+        s.close()
+        return ms
+
+    fit_mixed_reported = []
+
+    def run_mixed():
+        ctx.sync()
+        t0 = time.perf_counter()
+        s = ctx.hooi_session(A16, ranks, init=dt.INIT_LEARNED, model=model.handle, A_f32=A32, flags=kron,
+                             seed=seed)
+        for _ in range(k_mixed):
+            s.sweep()
+        fit_mixed_reported.append(1.0 - s.fit(fp64=False))
+        ctx.sync()
+        ms = (time.perf_counter() - t0) * 1e3
+        s.close()
+        return ms
+
+    run_plain()
+    run_mixed()  # first launches of every kernel
+    plain_ms = _median_ms(run_plain)
+    mixed_ms = _median_ms(run_mixed)
+    # breakdown of one mixed run
+    ctx.timing(True)
+    ctx.timing_reset()
+    run_mixed()
+    kt = {k: round(v[1], 3) for k, v in sorted(ctx.timing_list().items(), key=lambda kv: -kv[1][1])[:12]}
+    ctx.timing(False)
+    model.close()
+
+    def angle(Ua, Ub):
+        """sine of the largest principal angle between span(Ua) and span(Ub)"""
+        return float(np.linalg.svd(Ua - Ub @ (Ub.T @ Ua), compute_uv=False)[0])
+
+    Ut = [u.download() for u in g["U"]]
+    out = {"workload": "dense HOOI 1024^3, core 64^3 decaying 0.93^k, noise 1e-3, ranks (64,64,64)",
+           "reference_fit": target,
+           "closed_form_fit": 1.0 - noise * math.sqrt(1 - np.prod(ranks) / np.prod(shape)) / math.sqrt(1 + noise ** 2),
+           "plain_fp32": {"sweeps": k_plain, "ms": plain_ms, "fit": 1.0 - res_plain[-1],
+                          "max_sin_angle_vs_reference": max(angle(a, b) for a, b in zip(U_plain, ref["U"])),
+                          "max_sin_angle_vs_true": max(angle(a, b) for a, b in zip(U_plain, Ut))},
+           "mixed_learned": {"sweeps_bf16": k_mixed, "ms": mixed_ms, "fit": 1.0 - res_mixed[-1],
+                             "fit_reported_fp32": fit_mixed_reported[-1],
+                             "max_sin_angle_vs_reference": max(angle(a, b) for a, b in zip(U_mixed, ref["U"])),
+                             "max_sin_angle_vs_true": max(angle(a, b) for a, b in zip(U_mixed, Ut)),
+                             "kernel_ms": kt},
+           "speedup": plain_ms / mixed_ms}
+    for a in (A32, A16):
+        a.free()
+    log(f"[bench] time-to-fit plain FP32 {plain_ms:.1f} ms ({k_plain} sweeps), mixed+learned "
+        f"{mixed_ms:.1f} ms ({k_mixed} BF16 sweeps): {plain_ms / mixed_ms:.2f}x")
+    return out
+
+
 def copy_calibration(ctx, dt, nbytes=1 << 30):
     """A plain FP32 copy of known size (the `convert` kernel reads and writes
     nbytes each); profiler counter runs use it to scale FETCH_SIZE/WRITE_SIZE."""
@@ -168,6 +272,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build keeps one rank's block on one "
@@ -179,6 +284,7 @@ def main():
         del r["A"]
         calib = copy_calibration(ctx, dt)
         log(f"[bench] copy calibration: {calib['GB_per_s']:.1f} GB/s")
+        c3 = None if args.no_mixed else mixed_c3(ctx, dt)
     steps = args.steps
     value = steps / r["wall"]
     passes = {}
@@ -218,7 +324,8 @@ def main():
                    "init_ms": r["init_ms"],
                    "kernel_ms_per_sweep": r["breakdown"],
                    "full_passes": passes,
-                   "copy_calibration": calib},
+                   "copy_calibration": calib,
+                   "time_to_fit_mixed_learned_vs_fp32": c3},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": measured_traffic(dom_name) if dom_name else None,

@@ -339,21 +339,24 @@ static int sweep_once(dt_hooi *h, const void *A, int dt) {
     return contract(h, T, tdt, sh, N - 1, h->Gbuf);
 }
 
-// Core from the current factors (reference chain), for max_iter == 0.
-static int core_only(dt_hooi *h) {
-    h->bf16_core_mode = h->args.dtype == DT_BF16 ? 0 : -1;
+// Core from the current factors (reference chain) of tensor src (type sdt).
+static int core_from(dt_hooi *h, const void *src, int sdt) {
+    h->bf16_core_mode = sdt == DT_BF16 ? 0 : -1;
     int64_t sh[DT_MAX_NDIM];
     for (int i = 0; i < h->N; i++) sh[i] = h->I[i];
     std::vector<int> all;
     for (int i = 0; i < h->N - 1; i++) all.push_back(i);
     const void *T;
     int tdt;
-    DT_TRY(chain(h, h->args.A, h->args.dtype, sh, all, h->buf[0], h->buf[1], &T, &tdt));
+    DT_TRY(chain(h, src, sdt, sh, all, h->buf[0], h->buf[1], &T, &tdt));
     return contract(h, T, tdt, sh, h->N - 1, h->Gbuf);
 }
 
+// Core from the current factors, for max_iter == 0.
+static int core_only(dt_hooi *h) { return core_from(h, h->args.A, h->args.dtype); }
+
 int dt_learned_init(dt_ctx *ctx, const void *model, const void *A, int dtA, int N, const int64_t *I,
-                    const int64_t *R, uint64_t seed, double *const *U64);
+                    const int64_t *R, int sketch, uint64_t seed, double *const *U64);
 
 static int init_factors(dt_hooi *h) {
     dt_ctx *ctx = h->ctx;
@@ -384,7 +387,9 @@ static int init_factors(dt_hooi *h) {
             DT_TRY(dt_syev_top_f64(ctx, h->S, (size_t)h->I[n], (size_t)h->R[n], h->U64[n], nullptr));
         }
     } else if (a.init == DT_INIT_LEARNED) {
-        DT_TRY(dt_learned_init(ctx, a.model, a.A, a.dtype, N, h->I, h->R, a.seed, h->U64));
+        DT_TRY(dt_learned_init(ctx, a.model, a.A, a.dtype, N, h->I, h->R,
+                               (a.flags & DT_FLAG_SKETCH_KRON) ? DT_SKETCH_KRONECKER : DT_SKETCH_GAUSSIAN, a.seed,
+                               h->U64));
     } else {
         return dt_fail(ctx, DT_EINVAL, "hooi: unknown init %d", a.init);
     }
@@ -530,6 +535,25 @@ int dt_hooi_sweep(dt_hooi *h, int fp32, double *residual) {
     h->last_res = res;
     if (residual) *residual = res;
     return DT_OK;
+}
+
+int dt_hooi_fit(dt_hooi *h, int fp64, double *residual) {
+    dt_ctx *ctx = h->ctx;
+    DT_CHECK_ARG(residual, "dt_hooi_fit: null residual");
+    const void *fitA = h->args.A_f32 ? (const void *)h->args.A_f32 : h->args.A;
+    const int fitdt = h->args.A_f32 ? DT_F32 : h->args.dtype;
+    DT_TRY(dt_zero(ctx, h->dflag, sizeof(int)));
+    const int saved = h->args.flags;
+    if (fp64) {
+        h->args.flags |= DT_FLAG_FP64_FIT;
+    } else {
+        h->args.flags &= ~DT_FLAG_FP64_FIT;
+        DT_TRY(core_from(h, fitA, fitdt));
+    }
+    const int st = core_residual(h, fitA, fitdt, residual);
+    h->args.flags = saved;
+    if (st == DT_OK) h->last_res = *residual;
+    return st;
 }
 
 int dt_hooi_get(dt_hooi *h, dt_hooi_result *res) {

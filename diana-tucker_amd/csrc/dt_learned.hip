@@ -5,6 +5,9 @@
 // Y = A_(n) Ω with Ω a Gaussian (prod_{m!=n} I_m) x S matrix stored as a
 // tensor of A's shape with mode n replaced by S (Philox stream 5000 + n), so
 // Y is the cross product A_(n) Ω_(n)ᵀ (dt_cross_internal, split-K MFMA).
+// The Kronecker sketch instead takes Ω = ⊗_{m != n} Ω_m with Ω_m Gaussian
+// I_m x q: Y_(n) = (A x_{m != n} Ω_mᵀ)_(n) is a chain of mode products on
+// the streaming TTM kernels (one pass over A per mode), S = q^(N-1).
 // With YᵀY = V diag(σ²) Vᵀ (σ descending), the features are
 //     X = sqrt(I_n) / σ_1 · Y V          (I_n x S, row-major FP32),
 // i.e. the sketch's left singular vectors scaled by σ_k / σ_1, each column's
@@ -81,6 +84,25 @@ inline dim3 lg_grid(int64_t n) { return dim3((unsigned)std::min<int64_t>((n + 25
 
 }  // namespace
 
+// X from the sketch Y (I x S, column-major FP64) held at the start of ws
+// (which has room for 2 I S + 2 S^2 + 2 S doubles).
+static int features_from_sketch(dt_ctx *ctx, double *ws, int64_t I, int S, float *X) {
+    double *Y = ws;
+    double *YV = Y + I * S;
+    double *G = YV + I * S;
+    double *V = G + (int64_t)S * S;
+    double *w = V + (int64_t)S * S;
+    double *sg = w + S;
+    DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, (size_t)S, (size_t)S, (size_t)I, 1.0, Y, (size_t)I, Y, (size_t)I, 0.0, G,
+                                (size_t)S));
+    DT_TRY(dt_jacobi_eig(ctx, G, (size_t)S, V, w));
+    DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, (size_t)I, (size_t)S, (size_t)S, 1.0, Y, (size_t)I, V, (size_t)S, 0.0, YV,
+                                (size_t)I));
+    DT_LAUNCH("sketch_rows", column_signs, dim3((unsigned)S), dim3(256), 0, YV, I, sg);
+    DT_LAUNCH("sketch_rows", sketch_rows, lg_grid(I * S), dim3(256), 0, YV, w, sg, I, S, X);
+    return DT_OK;
+}
+
 extern "C" int dt_sketch_features(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n,
                                   int S, uint64_t seed, float *X) {
     DT_CHECK_ARG(ctx && A && shape && X, "dt_sketch_features: null argument");
@@ -94,32 +116,93 @@ extern "C" int dt_sketch_features(dt_ctx *ctx, const void *A, int dtA, int ndim,
     const int64_t nom = F * S * Bk;
     const size_t esz = dtA == DT_BF16 ? 2 : 4;
     void *Om = dt_ws(ctx, WS_MLP0, (size_t)nom * esz);
-    double *Y = (double *)dt_ws(ctx, WS_MLP1, (size_t)(2 * I * S + 2 * S * S + 2 * S + 64) * sizeof(double));
-    if (!Om || !Y) return DT_ENOMEM;
-    double *YV = Y + I * S;
-    double *G = YV + I * S;
-    double *V = G + (int64_t)S * S;
-    double *w = V + (int64_t)S * S;
-    double *sg = w + S;
+    double *ws = (double *)dt_ws(ctx, WS_MLP1, (size_t)(2 * I * S + 2 * S * S + 2 * S + 64) * sizeof(double));
+    if (!Om || !ws) return DT_ENOMEM;
     DT_TRY(dt_gen_normal(ctx, Om, dtA, (size_t)nom, seed, 5000 + (uint64_t)n));
-    DT_TRY(dt_cross_internal(ctx, A, Om, dtA, F, I, S, Bk, Y));
-    DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, (size_t)S, (size_t)S, (size_t)I, 1.0, Y, (size_t)I, Y, (size_t)I, 0.0, G,
-                                (size_t)S));
-    DT_TRY(dt_jacobi_eig(ctx, G, (size_t)S, V, w));
-    DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, (size_t)I, (size_t)S, (size_t)S, 1.0, Y, (size_t)I, V, (size_t)S, 0.0, YV,
-                                (size_t)I));
-    DT_LAUNCH("sketch_rows", column_signs, dim3((unsigned)S), dim3(256), 0, YV, I, sg);
-    DT_LAUNCH("sketch_rows", sketch_rows, lg_grid(I * S), dim3(256), 0, YV, w, sg, I, S, X);
-    return DT_OK;
+    DT_TRY(dt_cross_internal(ctx, A, Om, dtA, F, I, S, Bk, ws));
+    return features_from_sketch(ctx, ws, I, S, X);
+}
+
+// Y_(n) (I x S column-major FP64) of a tensor T whose mode n has size I and
+// every other mode size q: column = Kolda index of the remaining modes.
+__global__ void unfold_to_f64(const float *__restrict__ T, int ndim, int n, int64_t I, int q, int64_t total,
+                              double *__restrict__ Y) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        int64_t rem = e, col = 0, stride = 1, i = 0;
+        for (int m = 0; m < ndim; m++) {
+            if (m == n) {
+                i = rem % I;
+                rem /= I;
+            } else {
+                col += (rem % q) * stride;
+                rem /= q;
+                stride *= q;
+            }
+        }
+        Y[i + I * col] = (double)T[e];
+    }
+}
+
+extern "C" int dt_sketch_features_kron(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n,
+                                       int q, uint64_t seed, float *X) {
+    DT_CHECK_ARG(ctx && A && shape && X, "dt_sketch_features_kron: null argument");
+    DT_CHECK_ARG(ndim >= 2 && ndim <= DT_MAX_NDIM && n >= 0 && n < ndim, "dt_sketch_features_kron: bad mode");
+    DT_CHECK_ARG(dtA == DT_F32 || dtA == DT_BF16, "dt_sketch_features_kron: tensor must be F32 or BF16");
+    int64_t S = 1;
+    for (int m = 0; m < ndim - 1; m++) S *= q;
+    DT_CHECK_ARG(q >= 1 && S <= 96, "dt_sketch_features_kron: q^(ndim-1) = %lld not in [1, 96]", (long long)S);
+    const int64_t I = (int64_t)shape[n];
+    // factors Ω_m (q x I_m, column-major FP32), Philox stream 6000 + m
+    size_t off[DT_MAX_NDIM], tot = 0;
+    for (int m = 0; m < ndim; m++) {
+        off[m] = tot;
+        tot += m == n ? 0 : (size_t)q * shape[m];
+    }
+    float *Om = (float *)dt_ws(ctx, WS_MLP0, tot * sizeof(float) + 256);
+    double *ws = (double *)dt_ws(ctx, WS_MLP1, (size_t)(2 * I * S + 2 * S * S + 2 * S + 64) * sizeof(double) +
+                                                   (size_t)(I * S) * sizeof(float));
+    if (!Om || !ws) return DT_ENOMEM;
+    float *Yt = (float *)(ws + 2 * I * S + 2 * S * S + 2 * S + 64);
+    for (int m = 0; m < ndim; m++)
+        if (m != n) DT_TRY(dt_gen_normal(ctx, Om + off[m], DT_F32, (size_t)q * shape[m], seed, 6000 + (uint64_t)m));
+    // contraction order: the fused leading pair when n >= 2, else highest mode first
+    int idx[DT_MAX_NDIM], nm = 0;
+    if (n >= 2) {
+        idx[nm++] = 0;
+        idx[nm++] = 1;
+        for (int m = ndim - 1; m >= 2; m--)
+            if (m != n) idx[nm++] = m;
+    } else {
+        for (int m = ndim - 1; m >= 0; m--)
+            if (m != n) idx[nm++] = m;
+    }
+    const void *Ms[DT_MAX_NDIM];
+    size_t Js[DT_MAX_NDIM];
+    for (int k = 0; k < nm; k++) {
+        Ms[k] = Om + off[idx[k]];
+        Js[k] = (size_t)q;
+    }
+    DT_TRY(dt_ttmc(ctx, A, dtA, ndim, shape, Ms, DT_F32, Js, idx, nm, Yt, DT_F32));
+    DT_LAUNCH("sketch_unfold", unfold_to_f64, lg_grid(I * S), dim3(256), 0, Yt, ndim, n, I, q, I * S, ws);
+    return features_from_sketch(ctx, ws, I, (int)S, X);
 }
 
 extern "C" int dt_learned_start(dt_ctx *ctx, dt_mlp *mlp, const void *A, int dtA, int ndim, const size_t *shape,
-                                const size_t *ranks, uint64_t seed, double *const *U) {
+                                const size_t *ranks, int sketch, uint64_t seed, double *const *U) {
     DT_CHECK_ARG(ctx && mlp && A && shape && ranks && U, "dt_learned_start: null argument");
     DT_CHECK_ARG(ndim >= 2 && ndim <= DT_MAX_NDIM, "dt_learned_start: bad ndim");
     int din = 0, hid = 0, dout = 0;
     int64_t np = 0;
     dt_mlp_shape(mlp, &din, &hid, &dout, &np);
+    DT_CHECK_ARG(sketch == DT_SKETCH_GAUSSIAN || sketch == DT_SKETCH_KRONECKER, "dt_learned_start: bad sketch %d",
+                 sketch);
+    int q = 0;
+    if (sketch == DT_SKETCH_KRONECKER) {
+        q = (int)std::lround(std::pow((double)din, 1.0 / (ndim - 1)));
+        int64_t S = 1;
+        for (int m = 0; m < ndim - 1; m++) S *= q;
+        DT_CHECK_ARG(S == din, "dt_learned_start: model input %d is not q^%d", din, ndim - 1);
+    }
     int64_t rows = 0;
     for (int n = 0; n < ndim; n++) {
         DT_CHECK_ARG(U[n], "dt_learned_start: factor %d missing", n);
@@ -133,7 +216,10 @@ extern "C" int dt_learned_start(dt_ctx *ctx, dt_mlp *mlp, const void *A, int dtA
     float *Z = X + rows * din;
     int64_t off = 0;
     for (int n = 0; n < ndim; n++) {
-        DT_TRY(dt_sketch_features(ctx, A, dtA, ndim, shape, n, din, seed, X + off * din));
+        if (sketch == DT_SKETCH_KRONECKER)
+            DT_TRY(dt_sketch_features_kron(ctx, A, dtA, ndim, shape, n, q, seed, X + off * din));
+        else
+            DT_TRY(dt_sketch_features(ctx, A, dtA, ndim, shape, n, din, seed, X + off * din));
         off += (int64_t)shape[n];
     }
     DT_TRY(dt_mlp_forward_dev(ctx, mlp, X, rows, Z));
@@ -149,12 +235,12 @@ extern "C" int dt_learned_start(dt_ctx *ctx, dt_mlp *mlp, const void *A, int dtA
 
 // HOOI hook (dt_hooi.hip): learned start for every mode.
 int dt_learned_init(dt_ctx *ctx, const void *model, const void *A, int dtA, int N, const int64_t *I,
-                    const int64_t *R, uint64_t seed, double *const *U64) {
+                    const int64_t *R, int sketch, uint64_t seed, double *const *U64) {
     if (!model) return dt_fail(ctx, DT_EINVAL, "learned init: no model (dt_hooi_args.model)");
     size_t sh[DT_MAX_NDIM], rk[DT_MAX_NDIM];
     for (int n = 0; n < N; n++) {
         sh[n] = (size_t)I[n];
         rk[n] = (size_t)R[n];
     }
-    return dt_learned_start(ctx, (dt_mlp *)model, A, dtA, N, sh, rk, seed, U64);
+    return dt_learned_start(ctx, (dt_mlp *)model, A, dtA, N, sh, rk, sketch, seed, U64);
 }

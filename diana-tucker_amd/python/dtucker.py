@@ -20,7 +20,8 @@ DT_MAX_NDIM = 8
 INIT_GIVEN, INIT_QR, INIT_HOSVD, INIT_RANDOM, INIT_LEARNED = range(5)
 SCHED_REFERENCE, SCHED_TREE = 0, 1
 FLAG_DIRECT_RESIDUAL, FLAG_FP32_POLISH, FLAG_FP64_FIT, FLAG_REFERENCE_ALS = 1, 2, 4, 8
-FLAG_LEARNED_CHECK = 16
+FLAG_LEARNED_CHECK, FLAG_SKETCH_KRON = 16, 32
+SKETCH_GAUSSIAN, SKETCH_KRONECKER = 0, 1
 
 _NP_OF = {DT_F64: np.float64, DT_F32: np.float32, DT_BF16: np.uint16, DT_I32: np.int32}
 _SIZE_OF = {DT_F64: 8, DT_F32: 4, DT_BF16: 2, DT_I32: 4}
@@ -98,6 +99,7 @@ def lib():
             "dt_hooi_create": [vp, C.POINTER(HooiArgs), C.POINTER(vp)],
             "dt_hooi_sweep": [vp, i32, C.POINTER(dbl)],
             "dt_hooi_get": [vp, C.POINTER(HooiResult)],
+            "dt_hooi_fit": [vp, i32, C.POINTER(dbl)],
             "dt_hooi_destroy": [vp],
             "dt_mlp_create": [vp, C.POINTER(vp), i32, i32, i32, u64],
             "dt_mlp_destroy": [vp, vp],
@@ -108,7 +110,8 @@ def lib():
             "dt_mlp_loss_grad": [vp, vp, vp, vp, C.c_int64, C.POINTER(dbl), C.POINTER(C.c_float)],
             "dt_mlp_adam_step": [vp, vp, C.c_float],
             "dt_sketch_features": [vp, vp, i32, i32, C.POINTER(sz), i32, i32, u64, vp],
-            "dt_learned_start": [vp, vp, vp, i32, i32, C.POINTER(sz), C.POINTER(sz), u64, C.POINTER(vp)],
+            "dt_sketch_features_kron": [vp, vp, i32, i32, C.POINTER(sz), i32, i32, u64, vp],
+            "dt_learned_start": [vp, vp, vp, i32, i32, C.POINTER(sz), C.POINTER(sz), i32, u64, C.POINTER(vp)],
         }
         for name, argtypes in sig.items():
             f = getattr(L, name)
@@ -126,10 +129,10 @@ def exported_symbols():
             "dt_timer_stop", "dt_ttm", "dt_ttmc", "dt_gram", "dt_ttt_except", "dt_sumsq", "dt_gemm_f64",
             "dt_qr_f64", "dt_inverse_f64", "dt_syev_top_f64", "dt_orth_rows_f64",
             "dt_gen_normal", "dt_gen_uniform", "dt_gen_tucker", "dt_hooi_dense",
-            "dt_hooi_create", "dt_hooi_sweep", "dt_hooi_get", "dt_hooi_destroy",
+            "dt_hooi_create", "dt_hooi_sweep", "dt_hooi_fit", "dt_hooi_get", "dt_hooi_destroy",
             "dt_mlp_create", "dt_mlp_destroy", "dt_mlp_shape", "dt_mlp_get_params", "dt_mlp_set_params",
             "dt_mlp_forward", "dt_mlp_loss_grad", "dt_mlp_adam_step", "dt_sketch_features",
-            "dt_learned_start"]
+            "dt_sketch_features_kron", "dt_learned_start"]
 
 
 class DeviceArray:
@@ -427,13 +430,21 @@ class Context:
                                              X.ptr))
         return X.download().T.copy()
 
-    def learned_start(self, mlp: "MLP", A: DeviceArray, ranks, seed: int):
+    def sketch_features_kron(self, A: DeviceArray, n: int, q: int, seed: int) -> np.ndarray:
+        S = q ** (len(A.shape) - 1)
+        X = self.rows_array(A.shape[n], S)
+        shp = (C.c_size_t * len(A.shape))(*A.shape)
+        self.check(self.L.dt_sketch_features_kron(self.handle, A.ptr, A.dtype, len(A.shape), shp, n, q, seed,
+                                                  X.ptr))
+        return X.download().T.copy()
+
+    def learned_start(self, mlp: "MLP", A: DeviceArray, ranks, seed: int, sketch: int = SKETCH_GAUSSIAN):
         N = len(A.shape)
         Us = [self.array((A.shape[i], ranks[i]), DT_F64) for i in range(N)]
         shp = (C.c_size_t * N)(*A.shape)
         rk = (C.c_size_t * N)(*ranks)
         up = (C.c_void_p * N)(*[u.ptr for u in Us])
-        self.check(self.L.dt_learned_start(self.handle, mlp.handle, A.ptr, A.dtype, N, shp, rk, seed, up))
+        self.check(self.L.dt_learned_start(self.handle, mlp.handle, A.ptr, A.dtype, N, shp, rk, sketch, seed, up))
         return Us
 
     def hooi_dense(self, A: DeviceArray, ranks, want_factors=True, **kw):
@@ -475,6 +486,12 @@ class HooiSession:
     def sweep(self, fp32=False) -> float:
         r = C.c_double()
         self.ctx.check(self.ctx.L.dt_hooi_sweep(self.h, 1 if fp32 else 0, C.byref(r)))
+        return r.value
+
+    def fit(self, fp64=False) -> float:
+        """Residual of the current factors on A_f32 (or A), core formed again."""
+        r = C.c_double()
+        self.ctx.check(self.ctx.L.dt_hooi_fit(self.h, 1 if fp64 else 0, C.byref(r)))
         return r.value
 
     def result(self, want_factors=True):

@@ -33,12 +33,16 @@ import dtucker as dt  # noqa: E402
 MODEL_DIR = os.path.join(os.path.dirname(HERE), "models")
 
 C5 = {"name": "c5", "shape": (128, 128, 128), "ranks": (16, 16, 16), "S": 32, "hidden": 512,
-      "base_seed": 20000905, "n_train": 3584, "n_total": 4096, "decay": (0.7, 0.95), "noise": (1e-3, 1e-1)}
+      "base_seed": 20000905, "n_train": 3584, "n_total": 4096, "decay": (0.7, 0.95), "noise": (1e-3, 1e-1),
+      "sketch": dt.SKETCH_GAUSSIAN}
 # The 1024^3 rank-64 mixed-precision workload's family (core decay 0.93, noise
 # 1e-3) at 256^3: the network is row-wise with features normalised by
-# sqrt(I_n), so it is trained on smaller tensors of the same model.
-C3 = {"name": "c3", "shape": (256, 256, 256), "ranks": (64, 64, 64), "S": 96, "hidden": 512,
-      "base_seed": 30000905, "n_train": 192, "n_total": 224, "decay": (0.93, 0.93), "noise": (1e-3, 1e-3)}
+# sqrt(I_n), so it is trained on smaller tensors of the same model.  Its
+# sketch is the Kronecker one (q = 9 per mode, S = 81): at 1024^3 a dense
+# Gaussian Ω would have 2^20 x S entries per mode.
+C3 = {"name": "c3", "shape": (256, 256, 256), "ranks": (64, 64, 64), "S": 81, "q": 9, "hidden": 512,
+      "base_seed": 30000905, "n_train": 192, "n_total": 224, "decay": (0.93, 0.93), "noise": (1e-3, 1e-3),
+      "sketch": dt.SKETCH_KRONECKER}
 CONFIGS = {"c5": C5, "c3": C3}
 
 
@@ -56,7 +60,13 @@ def generate(ctx, t, cfg, **kw):
 
 
 def features(ctx, A, cfg, seed):
-    return [ctx.sketch_features(A, n, cfg["S"], seed) for n in range(len(cfg["shape"]))]
+    if cfg["sketch"] == dt.SKETCH_KRONECKER:
+        return [ctx.sketch_features_kron(A, n, cfg["q"], seed) for n in range(len(A.shape))]
+    return [ctx.sketch_features(A, n, cfg["S"], seed) for n in range(len(A.shape))]
+
+
+def learned_flags(cfg):
+    return dt.FLAG_SKETCH_KRON if cfg["sketch"] == dt.SKETCH_KRONECKER else 0
 
 
 def make_examples(ctx, ids, cfg, sweeps=12, log=print):
@@ -138,7 +148,7 @@ def sweeps_to_fit(ctx, A, cfg, init, model=None, max_sweeps=10, target=None, see
     """Per-sweep residuals (FP64-accumulated core) from `init`; the first sweep
     whose fit is within 1e-4 of the target fit (or None)."""
     s = ctx.hooi_session(A, cfg["ranks"], init=init, model=model.handle if model else None, seed=seed,
-                         flags=dt.FLAG_FP64_FIT)
+                         flags=dt.FLAG_FP64_FIT | learned_flags(cfg))
     res = [s.sweep() for _ in range(max_sweeps)]
     s.close()
     hit = None
@@ -154,7 +164,8 @@ def timed_run(ctx, A, cfg, init, sweeps, model=None, seed=0):
     """Wall time (ms) of start + `sweeps` plain sweeps, host clock around synchronised calls."""
     ctx.sync()
     t0 = time.perf_counter()
-    s = ctx.hooi_session(A, cfg["ranks"], init=init, model=model.handle if model else None, seed=seed)
+    s = ctx.hooi_session(A, cfg["ranks"], init=init, model=model.handle if model else None, seed=seed,
+                         flags=learned_flags(cfg))
     for _ in range(sweeps):
         s.sweep()
     ctx.sync()

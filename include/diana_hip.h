@@ -197,10 +197,18 @@ int dt_mlp_adam_step(dt_ctx *ctx, dt_mlp *mlp, float lr);
 /* Sketch features of mode n: X (shape[n] x S, device, row-major FP32). */
 int dt_sketch_features(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n,
                        int S, uint64_t seed, float *X);
+/* The same features from the Kronecker sketch Ω = ⊗_{m != n} Ω_m (Ω_m
+ * Gaussian shape[m] x q, Philox stream 6000 + m), S = q^(ndim-1): one chain
+ * of mode products (dt_ttmc) instead of a dense (prod_{m!=n} I_m) x S Ω. */
+int dt_sketch_features_kron(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape,
+                            int n, int q, uint64_t seed, float *X);
+enum { DT_SKETCH_GAUSSIAN = 0, DT_SKETCH_KRONECKER = 1 };
 /* Learned start factors U[n] (device FP64, shape[n] x ranks[n], orthonormal);
- * mlp d_in = sketch width, d_out = every ranks[n]. */
+ * mlp d_in = sketch width S (= q^(ndim-1) for the Kronecker sketch),
+ * d_out = every ranks[n]. */
 int dt_learned_start(dt_ctx *ctx, dt_mlp *mlp, const void *A, int dtA, int ndim,
-                     const size_t *shape, const size_t *ranks, uint64_t seed, double *const *U);
+                     const size_t *shape, const size_t *ranks, int sketch, uint64_t seed,
+                     double *const *U);
 
 /* ---- dense HOOI --------------------------------------------------------- */
 enum {
@@ -227,8 +235,9 @@ enum {
                                     (one extra pass over A per sweep); needed when the
                                     residual is below ~1e-3, where FP32 rounding of the
                                     core dominates 1 - ||G||^2/||A||^2 */
-    DT_FLAG_LEARNED_CHECK = 16   /* DT_INIT_LEARNED: also run the first sweep from the HOSVD
+    DT_FLAG_LEARNED_CHECK = 16,  /* DT_INIT_LEARNED: also run the first sweep from the HOSVD
                                     start and keep whichever fits better (the fallback) */
+    DT_FLAG_SKETCH_KRON = 32     /* DT_INIT_LEARNED: Kronecker sketch (model input q^(N-1)) */
 };
 
 typedef struct dt_hooi_args {
@@ -279,6 +288,13 @@ int dt_hooi_dense(dt_ctx *ctx, const dt_hooi_args *args, dt_hooi_result *res);
 typedef struct dt_hooi dt_hooi;
 int dt_hooi_create(dt_ctx *ctx, const dt_hooi_args *args, dt_hooi **out);
 int dt_hooi_sweep(dt_hooi *h, int fp32, double *residual);
+/* Residual of the current factors on A_f32 (or A): the core is formed again
+ * from that tensor, in FP32 with the factor-copy correction, or (fp64 != 0)
+ * with FP64 accumulation.  One pass over the tensor; the factors do not
+ * change.  Lets a BF16 run report its fit on the FP32 copy without an FP32
+ * sweep.  No reference counterpart (hooi_als.tpp:112-114 prints the residual
+ * of the sweep's own core). */
+int dt_hooi_fit(dt_hooi *h, int fp64, double *residual);
 int dt_hooi_get(dt_hooi *h, dt_hooi_result *res);
 int dt_hooi_destroy(dt_hooi *h);
 

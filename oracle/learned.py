@@ -4,11 +4,14 @@ The reference has no learned component (it starts HOOI from random factors,
 template/algorithm/tucker/hooi_als.tpp:76-81); this restates the build's own
 definition so the GPU kernels can be checked:
 
-* sketch features of mode n (diana-tucker_amd/csrc/dt_learned.hip):
+* sketch features of mode n (diana-tucker_amd/csrc/dt_learned.hip), dense
+  Gaussian sketch:
   Ω = Philox normals (stream 5000 + n) laid out as a tensor of A's shape with
   mode n replaced by S; Y = A_(n) Ω_(n)ᵀ; YᵀY = V diag(σ²) Vᵀ (σ descending);
   X = sqrt(I_n) / σ_1 · Y V with each column's sign set so its entry of
-  largest magnitude is positive;
+  largest magnitude is positive; the Kronecker sketch replaces Y by the
+  mode-n unfolding of A x_{m != n} Ω_m with Ω_m Gaussian q x I_m (stream
+  6000 + m);
 * the row-wise MLP (diana-tucker_amd/csrc/dt_mlp.hip):
   h1 = relu(x W1 + b1), h2 = relu(h1 W2 + b2), z = x[:R] + h2 W3 + b3,
   loss = sum_rows |z - t|^2 / rows, with every product's operands rounded to
@@ -79,6 +82,17 @@ def loss_grad(params, X, T, din, hid, dout, rounding=True):
     return loss, pack([gW1, gb1, gW2, gb2, gW3, gb3])
 
 
+def _features(Y, In):
+    S = Y.shape[1]
+    w, V = np.linalg.eigh(Y.T @ Y)
+    order = np.argsort(-w, kind="stable")
+    w, V = w[order], V[:, order]
+    YV = Y @ V
+    idx = np.argmax(np.abs(YV), axis=0)
+    sg = np.where(YV[idx, np.arange(S)] < 0, -1.0, 1.0)
+    return np.sqrt(In) / np.sqrt(w[0]) * YV * sg
+
+
 def sketch_features(A, n, S, seed):
     """X (I_n x S) of dt_sketch_features for an FP32/FP64 tensor A (values as given)."""
     A = np.asarray(A, dtype=np.float64)
@@ -87,14 +101,19 @@ def sketch_features(A, n, S, seed):
     oshape[n] = S
     om = philox.normal(int(np.prod(oshape)), seed, 5000 + n).astype(np.float32).astype(np.float64)
     Om = om.reshape(oshape, order="F")
-    Y = D.ttt_except(A, Om, n)                    # A_(n) Ω_(n)ᵀ, I_n x S
-    w, V = np.linalg.eigh(Y.T @ Y)
-    order = np.argsort(-w, kind="stable")
-    w, V = w[order], V[:, order]
-    YV = Y @ V
-    idx = np.argmax(np.abs(YV), axis=0)
-    sg = np.where(YV[idx, np.arange(S)] < 0, -1.0, 1.0)
-    return np.sqrt(A.shape[n]) / np.sqrt(w[0]) * YV * sg
+    return _features(D.ttt_except(A, Om, n), A.shape[n])   # A_(n) Ω_(n)ᵀ, I_n x S
+
+
+def sketch_features_kron(A, n, q, seed):
+    """X of dt_sketch_features_kron: Y_(n) of A x_{m != n} Ω_m, Ω_m = q x I_m
+    column-major Philox normals (stream 6000 + m)."""
+    A = np.asarray(A, dtype=np.float64)
+    Y = A
+    for m in range(A.ndim):
+        if m != n:
+            om = philox.normal(q * A.shape[m], seed, 6000 + m).astype(np.float32).astype(np.float64)
+            Y = D.ttm(Y, om.reshape((q, A.shape[m]), order="F"), m)
+    return _features(D.tenmat(Y, n), A.shape[n])
 
 
 def start_from_features(Z):

@@ -104,6 +104,30 @@ def test_sketch_features_vs_oracle(ctx, shape, n, S):
     assert np.allclose(np.linalg.norm(X, axis=0), np.linalg.norm(Xr, axis=0), rtol=1e-4)
 
 
+@pytest.mark.parametrize("shape,n,q,dtype", [((64, 48, 40), 0, 4, dt.DT_F32), ((64, 48, 40), 1, 4, dt.DT_F32),
+                                             ((128, 64, 40), 2, 5, dt.DT_F32), ((128, 64, 40), 2, 5, dt.DT_BF16),
+                                             ((16, 12, 10, 8), 1, 3, dt.DT_F32)])
+def test_sketch_features_kron_vs_oracle(ctx, shape, n, q, dtype):
+    """Kronecker sketch through dt_ttmc (the fused leading pair for n >= 2)."""
+    print("kron sketch", shape, n, q, dtype, flush=True)
+    rng = np.random.default_rng(sum(shape) + 7 * n)
+    r = 3
+    G = 50.0 * rng.standard_normal((r,) * len(shape))  # signal well above the noise
+    Us = [np.linalg.qr(rng.standard_normal((I, r)))[0] for I in shape]
+    A = (D.reconstruct(G, Us) + 0.005 * rng.standard_normal(shape)).astype(np.float32)
+    if dtype == dt.DT_BF16:
+        Ab = dt.bf16_from_f32(A)
+        dA = ctx.array(shape, dt.DT_BF16).upload(Ab)
+        A = dt.f32_from_bf16(Ab)
+    else:
+        dA = ctx.to_device(A)
+    X = ctx.sketch_features_kron(dA, n, q, seed=11)
+    Xr = Lr.sketch_features_kron(A.astype(np.float64), n, q, 11)
+    tol = 1e-4 if dtype == dt.DT_F32 else 1e-2  # bf16 rounds the factor of the first product
+    assert rel(X[:, :r], Xr[:, :r]) < tol
+    assert D.sin_max_angle(np.linalg.qr(X[:, :r])[0], Us[n]) < 0.1
+
+
 def test_learned_start_with_untrained_model_is_the_sketch_range(ctx):
     """A fresh model (zero last layer) gives QR of the first R feature columns:
     the randomized range finder's leading subspace."""
