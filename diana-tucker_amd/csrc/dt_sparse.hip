@@ -1,0 +1,835 @@
+// Sparse HOOI: Zipf COO generator with duplicate merging, the SpTensor
+// mode sorts, the TTMc and the one-step-per-mode sparse HOOI.
+//
+// Reference path (spmain.cpp:55 -> Algorithm::SpTucker::Sp_HOOI_ALS,
+// template/algorithm/sp_tucker/sp_hooi_als.tpp:67-118): per sweep and mode n,
+// Sp_TTMC_ALS_ (:48-65) forms L = Y_(n)ᵀ U_n with Y = A x_{m != n} U_mᵀ
+// (ttmc_mTR, template/function/sp_tensor.tpp:236-286), then U_n <- orth(Y_(n) L)
+// (ttmc_mTL :288-365 and orth2, operator_double.cpp:218-277): one subspace
+// step U_n <- orth(S U_n) with S = Y_(n) Y_(n)ᵀ never formed.  The final core
+// is L of the last mode (:110) and the residual sqrt(1 - ||G||^2/||A||^2).
+//
+// Layout on the device (dt_sp): the COO as given (int32 indices per mode,
+// FP32 values) and, per mode n, a copy sorted stably by index n (the
+// reference's SpTensor constructor counting sort, sp_tensor.hpp:117-147,
+// which it applies to its slice mode): the permutation, the sorted mode-n
+// index of every nonzero, the other modes' indices and the values in that
+// order, row pointers (I_n + 1), and a compact numbering ("slot") of the
+// non-empty rows.  Sorting uses rocPRIM's LSD radix sort (hipcub), which is
+// stable, so the permutation is exactly the reference's.
+//
+// TTMc (dt_sp_ttmc_dev): Y_c[slot(i), :] = sum over the nonzeros of row i of
+// val * (U_a[i_a, :] ⊗ U_b[i_b, :]) for the two other modes a < b (entry
+// r_a + R_a r_b, the reference's add_outer_product_all order).  Every wave
+// takes chunks of T consecutive nonzeros of the mode-sorted stream; four
+// nonzeros at a time are one v_mfma_f32_16x16x4_f32 (A operand = val *
+// U_a rows, B operand = U_b rows, so D accumulates the 16 x 16 outer-product
+// sum of a row segment).  A row wholly inside a chunk is written directly; a
+// row cut by chunk boundaries leaves FP32 partials in per-chunk carry slots
+// that sp_fixup sums in chunk order (deterministic).  The same pass
+// accumulates L = Y_cᵀ U_n from the segment sums (linear, so partial rows
+// need no fixup): per-wave registers, reduced per workgroup in LDS, then
+// across workgroups in a fixed order in FP64.
+#include "dt_internal.hpp"
+#include "dt_philox.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace {
+
+constexpr int SP_T = 2048;   // nonzeros per chunk (multiple of 4)
+constexpr int SP_E = 256;    // Kronecker row length handled per wave (16 x 16)
+
+int bits_for(int64_t dim) {
+    int b = 0;
+    while (b < 62 && ((int64_t)1 << b) < dim) b++;
+    return b < 1 ? 1 : b;
+}
+
+inline dim3 sp_grid(int64_t n) { return dim3((unsigned)std::min<int64_t>((n + 255) / 256, 65535)); }
+
+// ---- generator -------------------------------------------------------------
+
+// key(d) = sum_n idx_n(d) << shift_n, idx_n = first k with cdf_n[k] >= u,
+// u = uniform(seed, 7000 + n, d); value(d) = uniform(seed, 7100, d) in FP32.
+__global__ void zipf_draw(int64_t ndraws, int ndim, const double *__restrict__ cdf, const int64_t *__restrict__ coff,
+                          const int64_t *__restrict__ dims, const int *__restrict__ shift, uint64_t seed,
+                          uint64_t *__restrict__ key, int32_t *__restrict__ pos, float *__restrict__ vdraw) {
+    for (int64_t d = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; d < ndraws; d += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t k = 0;
+        for (int n = 0; n < ndim; n++) {
+            const double u = uniform_at(seed, 7000 + (uint64_t)n, (uint64_t)d);
+            const double *c = cdf + coff[n];
+            int64_t lo = 0, hi = dims[n] - 1;  // cdf[dims-1] == 1 >= u
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (c[mid] >= u)
+                    hi = mid;
+                else
+                    lo = mid + 1;
+            }
+            k |= (uint64_t)lo << shift[n];
+        }
+        key[d] = k;
+        pos[d] = (int32_t)d;
+        vdraw[d] = (float)uniform_at(seed, 7100, (uint64_t)d);
+    }
+}
+
+__global__ void run_heads(const uint64_t *__restrict__ key, int64_t n, int32_t *__restrict__ flag) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+        flag[p] = (p == 0 || key[p] != key[p - 1]) ? 1 : 0;
+}
+
+__global__ void scatter_heads(const int32_t *__restrict__ flag, const int32_t *__restrict__ uidx, int64_t n,
+                              int32_t *__restrict__ heads) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+        if (flag[p]) heads[uidx[p]] = (int32_t)p;
+}
+
+constexpr int LONG_RUN = 4096;
+
+// unique entry j: indices from the key, value = FP64 sum of its draws in draw
+// order, rounded once to FP32; runs longer than LONG_RUN are queued.
+__global__ void merge_runs(const uint64_t *__restrict__ key, const int32_t *__restrict__ pos,
+                           const float *__restrict__ vdraw, const int32_t *__restrict__ heads, int64_t nu, int64_t n,
+                           int ndim, const int *__restrict__ shift, const int *__restrict__ nbits,
+                           int32_t *const *__restrict__ idx, float *__restrict__ val, int32_t *__restrict__ longq,
+                           int32_t *__restrict__ nlong) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nu; j += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t h = heads[j], e = j + 1 < nu ? heads[j + 1] : n;
+        const uint64_t k = key[h];
+        for (int m = 0; m < ndim; m++) idx[m][j] = (int32_t)((k >> shift[m]) & ((1ull << nbits[m]) - 1));
+        if (e - h > LONG_RUN) {
+            longq[atomicAdd(nlong, 1)] = (int32_t)j;
+            continue;
+        }
+        double s = 0.0;
+        for (int64_t q = h; q < e; q++) s += (double)vdraw[pos[q]];
+        val[j] = (float)s;
+    }
+}
+
+// one workgroup per long run: 256 strided FP64 partial sums, fixed tree
+__global__ __launch_bounds__(256) void merge_long(const int32_t *__restrict__ pos, const float *__restrict__ vdraw,
+                                                  const int32_t *__restrict__ heads, int64_t nu, int64_t n,
+                                                  const int32_t *__restrict__ longq, float *__restrict__ val) {
+    __shared__ double red[256];
+    const int64_t j = longq[blockIdx.x];
+    const int64_t h = heads[j], e = j + 1 < nu ? heads[j + 1] : n;
+    double s = 0.0;
+    for (int64_t q = h + threadIdx.x; q < e; q += 256) s += (double)vdraw[pos[q]];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) val[j] = (float)red[0];
+}
+
+// ---- mode sort ---------------------------------------------------------------
+
+__global__ void iota32(int32_t *__restrict__ p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = (int32_t)i;
+}
+
+__global__ void count_rows(const int32_t *__restrict__ k, int64_t n, int32_t *__restrict__ cnt) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&cnt[k[i]], 1);
+}
+
+__global__ void nonempty_flags(const int32_t *__restrict__ rowptr, int64_t I, int32_t *__restrict__ f) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < I; i += (int64_t)gridDim.x * blockDim.x)
+        f[i] = rowptr[i + 1] > rowptr[i] ? 1 : 0;
+}
+
+__global__ void rows_list(const int32_t *__restrict__ f, const int32_t *__restrict__ slot, int64_t I,
+                          int32_t *__restrict__ rows) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < I; i += (int64_t)gridDim.x * blockDim.x)
+        if (f[i]) rows[slot[i]] = (int32_t)i;
+}
+
+template <typename T>
+__global__ void gather_by(const T *__restrict__ src, const int32_t *__restrict__ perm, int64_t n, T *__restrict__ dst) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[perm[i]];
+}
+
+// ---- TTMc --------------------------------------------------------------------
+
+struct SpModeArgs {
+    const int32_t *row;     // sorted mode-n index of each nonzero
+    const int32_t *ia;      // index in mode a (first other mode)
+    const int32_t *ib;      // index in mode b
+    const float *val;
+    const int32_t *rowptr;  // I_n + 1
+    const int32_t *slot;    // compact row number
+    int64_t nnz;
+};
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Flush the segment of row r (acc: lane (c, g) holds entries 4g+v + 16c).
+__device__ __forceinline__ void sp_flush(const SpModeArgs &a, int r, int64_t p0, int64_t p1, int first_row, int64_t c,
+                                         floatx4 acc, float *__restrict__ Yc, float *__restrict__ carry,
+                                         int32_t *__restrict__ carry_row, const float *__restrict__ Un,
+                                         float (&Lp)[4][16], int c16, int g) {
+    const int64_t rs = a.rowptr[r], re = a.rowptr[r + 1];
+    const int e0 = 4 * g + 16 * c16;
+    float *dst;
+    if (rs >= p0 && re <= p1) {
+        dst = Yc + (int64_t)a.slot[r] * SP_E;
+    } else {
+        const int s = r == first_row ? 0 : 1;
+        dst = carry + (c * 2 + s) * SP_E;
+        if (c16 == 0 && g == 0) carry_row[c * 2 + s] = r;
+    }
+    *(floatx4 *)(dst + e0) = acc;
+    if (Un) {
+        const floatx4 *u = (const floatx4 *)(Un + (int64_t)r * 16);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const floatx4 uq = u[q];
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                Lp[v][4 * q + 0] += acc[v] * uq[0];
+                Lp[v][4 * q + 1] += acc[v] * uq[1];
+                Lp[v][4 * q + 2] += acc[v] * uq[2];
+                Lp[v][4 * q + 3] += acc[v] * uq[3];
+            }
+        }
+    }
+}
+
+// One wave per chunk (persistent over chunks); Ua: I_a x 16, Ub: I_b x 16,
+// Un: I_n x 16 (row-major FP32) or null; Lwg: per-workgroup L partial
+// (256 x 16, entry-major) when Un is given.
+__global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__restrict__ Ua,
+                                                 const float *__restrict__ Ub, const float *__restrict__ Un,
+                                                 float *__restrict__ Yc, float *__restrict__ carry,
+                                                 int32_t *__restrict__ carry_row, float *__restrict__ Lwg,
+                                                 int64_t nchunks) {
+    __shared__ float Ls[SP_E * 16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c16 = lane & 15, g = lane >> 4;
+    float Lp[4][16];
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+#pragma unroll
+        for (int k = 0; k < 16; k++) Lp[v][k] = 0.f;
+    const int64_t nw = (int64_t)gridDim.x * 4;
+    for (int64_t c = (int64_t)blockIdx.x * 4 + wave; c < nchunks; c += nw) {
+        const int64_t p0 = c * SP_T;
+        const int64_t p1 = p0 + SP_T < a.nnz ? p0 + SP_T : a.nnz;
+        const int first_row = a.row[p0];
+        int cur = first_row;
+        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int64_t b0 = p0; b0 < p1; b0 += 64) {
+            // this lane's nonzero of the batch
+            const int64_t pz = b0 + lane;
+            const bool ok = pz < p1;
+            const int zr = ok ? a.row[pz] : -1;
+            const int zi = ok ? a.ia[pz] : 0;
+            const int zj = ok ? a.ib[pz] : 0;
+            const float zv = ok ? a.val[pz] : 0.f;
+            // gathers for all 16 groups of the batch, then the MFMAs
+            float av[16], bv[16];
+            int rw[16];
+#pragma unroll
+            for (int t = 0; t < 16; t++) {
+                const int src = 4 * t + g;  // lane holding nonzero b0 + 4t + g
+                const int ri = __builtin_amdgcn_ds_bpermute(src * 4, zr);
+                const int ii = __builtin_amdgcn_ds_bpermute(src * 4, zi);
+                const int jj = __builtin_amdgcn_ds_bpermute(src * 4, zj);
+                const float vv = __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src * 4, __builtin_bit_cast(int, zv)));
+                rw[t] = ri;
+                av[t] = ri >= 0 ? vv * Ua[(int64_t)ii * 16 + c16] : 0.f;
+                bv[t] = ri >= 0 ? Ub[(int64_t)jj * 16 + c16] : 0.f;
+            }
+#pragma unroll
+            for (int t = 0; t < 16; t++) {
+                if (b0 + 4 * t >= p1) break;  // uniform
+                // rows of the group's four nonzeros (g = 0..3), -1 past the end
+                const int r0 = __builtin_amdgcn_readlane(rw[t], 0);
+                const int r3 = __builtin_amdgcn_readlane(rw[t], 48);
+                if (r0 == cur && r3 == cur) {
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t], bv[t], acc, 0, 0, 0);
+                } else {
+                    // up to four rows in this group, in order
+#pragma unroll
+                    for (int gg = 0; gg < 4; gg++) {
+                        const int r = __builtin_amdgcn_readlane(rw[t], 16 * gg);
+                        if (r < 0) break;
+                        if (gg > 0 && r == __builtin_amdgcn_readlane(rw[t], 16 * (gg - 1))) continue;
+                        if (r != cur) {
+                            sp_flush(a, cur, p0, p1, first_row, c, acc, Yc, carry, carry_row, Un, Lp, c16, g);
+                            cur = r;
+                            acc = floatx4{0.f, 0.f, 0.f, 0.f};
+                        }
+                        const bool mine = rw[t] == r;
+                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(mine ? av[t] : 0.f, mine ? bv[t] : 0.f, acc, 0,
+                                                                   0, 0);
+                    }
+                }
+            }
+        }
+        sp_flush(a, cur, p0, p1, first_row, c, acc, Yc, carry, carry_row, Un, Lp, c16, g);
+    }
+    if (!Un) return;
+    // L partial of the workgroup: Ls[e * 16 + k], waves added in order
+    for (int w = 0; w < 4; w++) {
+        if (wave == w) {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+#pragma unroll
+                for (int k = 0; k < 16; k++) {
+                    const int e = 4 * g + v + 16 * c16;
+                    if (w == 0)
+                        Ls[e * 16 + k] = Lp[v][k];
+                    else
+                        Ls[e * 16 + k] += Lp[v][k];
+                }
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < SP_E * 16; i += 256) Lwg[(int64_t)blockIdx.x * SP_E * 16 + i] = Ls[i];
+}
+
+// Rows cut by chunk boundaries: the chunk where row r starts sums the
+// partials of r over its chunks, in chunk order.
+__global__ __launch_bounds__(64) void sp_fixup(SpModeArgs a, const float *__restrict__ carry,
+                                               const int32_t *__restrict__ carry_row, int64_t nchunks,
+                                               float *__restrict__ Yc) {
+    const int64_t c = blockIdx.x;
+    const int64_t p0 = c * SP_T;
+    const int64_t p1 = p0 + SP_T < a.nnz ? p0 + SP_T : a.nnz;
+    const int r = a.row[p1 - 1];  // last row of the chunk
+    const int64_t rs = a.rowptr[r], re = a.rowptr[r + 1];
+    if (re <= p1) return;          // ends inside this chunk: complete, or owned by its first chunk
+    if (rs / SP_T != c) return;    // owned by the chunk where it starts
+    (void)p0;
+    const int64_t ce = (re - 1) / SP_T;
+    floatx4 s = {0.f, 0.f, 0.f, 0.f};
+    for (int64_t q = c; q <= ce; q++) {
+        const int first = a.row[q * SP_T];
+        const int sl = (first == r) ? 0 : 1;
+        const floatx4 v = *(const floatx4 *)(carry + (q * 2 + sl) * SP_E + 4 * threadIdx.x);
+        s += v;
+    }
+    *(floatx4 *)(Yc + (int64_t)a.slot[r] * SP_E + 4 * threadIdx.x) = s;
+}
+
+// L (256 x 16, FP64) = sum over workgroups of the partials, fixed order.
+__global__ void sp_lreduce(const float *__restrict__ Lwg, int nwg, double *__restrict__ L) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= SP_E * 16) return;
+    double s = 0.0;
+    for (int w = 0; w < nwg; w++) s += (double)Lwg[(int64_t)w * SP_E * 16 + i];
+    L[i] = s;
+}
+
+// W (nrows x 16) = Y_c L, scattered to U (I x 16 row-major FP64) rows; U is
+// pre-zeroed for empty rows.  One wave per row (lane: 4 entries).
+__global__ __launch_bounds__(256) void sp_times_l(const float *__restrict__ Yc, const int32_t *__restrict__ rows,
+                                                  int64_t nrows, const double *__restrict__ L,
+                                                  double *__restrict__ Uout) {
+    __shared__ float Lsh[SP_E * 16];
+    for (int i = threadIdx.x; i < SP_E * 16; i += 256) Lsh[i] = (float)L[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = (int64_t)gridDim.x * 4;
+    for (int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); s < nrows; s += nw) {
+        const floatx4 y = *(const floatx4 *)(Yc + s * SP_E + 4 * lane);
+        float o[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            o[k] = y[0] * Lsh[(4 * lane + 0) * 16 + k] + y[1] * Lsh[(4 * lane + 1) * 16 + k] +
+                   y[2] * Lsh[(4 * lane + 2) * 16 + k] + y[3] * Lsh[(4 * lane + 3) * 16 + k];
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) o[k] += __shfl_xor(o[k], off);
+        if (lane < 16) Uout[(int64_t)rows[s] * 16 + lane] = (double)o[lane];
+    }
+}
+
+// Y (I x 256 row-major FP32, zeros for empty rows) from Y_c.
+__global__ void sp_expand(const float *__restrict__ Yc, const int32_t *__restrict__ rows, int64_t nrows,
+                          float *__restrict__ Y) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nrows * SP_E; e += (int64_t)gridDim.x * blockDim.x)
+        Y[(int64_t)rows[e / SP_E] * SP_E + e % SP_E] = Yc[e];
+}
+
+__global__ void rowmajor_to_colmajor(const double *__restrict__ Ur, int64_t I, int R, double *__restrict__ U) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < I * R; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e % I;
+        const int r = (int)(e / I);
+        U[e] = Ur[i * R + r];
+    }
+}
+
+__global__ void sumsq_f32_sp(const float *__restrict__ v, int64_t n, double *__restrict__ part) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        s += (double)v[i] * v[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+}  // namespace
+
+// ---- the sparse tensor ---------------------------------------------------------
+
+struct dt_sp_mode {
+    int32_t *perm = nullptr, *row = nullptr, *ia = nullptr, *ib = nullptr, *rowptr = nullptr, *slot = nullptr,
+            *rows = nullptr;
+    float *val = nullptr;
+    int64_t nrows = 0;
+};
+
+struct dt_sp {
+    int ndim = 0;
+    int64_t dims[DT_MAX_NDIM] = {};
+    int64_t nnz = 0;
+    int slice_mode = 0;
+    int32_t *idx[DT_MAX_NDIM] = {};
+    float *val = nullptr;
+    dt_sp_mode mode[DT_MAX_NDIM];
+    double normA2 = 0.0;
+    std::vector<void *> owned;
+};
+
+static void *sp_alloc(dt_sp *sp, size_t bytes) {
+    void *p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) return nullptr;
+    sp->owned.push_back(p);
+    return p;
+}
+
+// temp storage for hipcub calls (grow-only workspace slot)
+static void *cub_tmp(dt_ctx *ctx, size_t bytes) { return dt_ws(ctx, WS_SPARSE3, std::max<size_t>(bytes, 256)); }
+
+static int sort_mode_internal(dt_ctx *ctx, int64_t nnz, const int32_t *keys, int64_t dim, int32_t *perm,
+                              int32_t *sorted_keys, int32_t *rowptr) {
+    int32_t *iota = (int32_t *)dt_ws(ctx, WS_SPARSE2, (size_t)nnz * 4 + 256);
+    if (!iota) return DT_ENOMEM;
+    DT_LAUNCH("sp_sort", iota32, sp_grid(nnz), dim3(256), 0, iota, nnz);
+    const int nb = bits_for(dim);
+    size_t tb = 0;
+    DT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (const uint32_t *)keys, (uint32_t *)sorted_keys, iota,
+                                              perm, (int)nnz, 0, nb, ctx->stream));
+    void *tmp = cub_tmp(ctx, tb);
+    if (!tmp) return DT_ENOMEM;
+    DT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, (const uint32_t *)keys, (uint32_t *)sorted_keys, iota, perm,
+                                              (int)nnz, 0, nb, ctx->stream));
+    // row pointers: histogram, exclusive scan (int32: nnz < 2^31)
+    int32_t *cnt = (int32_t *)dt_ws(ctx, WS_SPARSE1, (size_t)(dim + 1) * 4 + 256);
+    if (!cnt) return DT_ENOMEM;
+    DT_TRY(dt_zero(ctx, cnt, (size_t)(dim + 1) * 4));
+    DT_LAUNCH("sp_sort", count_rows, sp_grid(nnz), dim3(256), 0, keys, nnz, cnt);
+    size_t tb2 = 0;
+    DT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, cnt, rowptr, (int)(dim + 1), ctx->stream));
+    tmp = cub_tmp(ctx, tb2);
+    if (!tmp) return DT_ENOMEM;
+    DT_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb2, cnt, rowptr, (int)(dim + 1), ctx->stream));
+    return DT_OK;
+}
+
+extern "C" int dt_sp_sort_mode(dt_ctx *ctx, int64_t nnz, const int32_t *keys, size_t dim, int32_t *perm,
+                               int32_t *rowptr) {
+    DT_CHECK_ARG(ctx && keys && perm && rowptr, "dt_sp_sort_mode: null argument");
+    DT_CHECK_ARG(nnz >= 1 && nnz < ((int64_t)1 << 31) && dim >= 1 && dim < ((size_t)1 << 31),
+                 "dt_sp_sort_mode: nnz %lld / dim %zu out of range", (long long)nnz, dim);
+    int32_t *sk = (int32_t *)dt_ws(ctx, WS_SPARSE0, (size_t)nnz * 4 + 256);
+    if (!sk) return DT_ENOMEM;
+    return sort_mode_internal(ctx, nnz, keys, (int64_t)dim, perm, sk, rowptr);
+}
+
+extern "C" int dt_sp_gen_zipf(dt_ctx *ctx, int ndim, const size_t *dims, int64_t ndraws, double exponent,
+                              uint64_t seed, int32_t *const *idx_out, float *val_out, int64_t *nnz_out) {
+    DT_CHECK_ARG(ctx && dims && idx_out && val_out && nnz_out, "dt_sp_gen_zipf: null argument");
+    DT_CHECK_ARG(ndim >= 1 && ndim <= DT_MAX_NDIM, "dt_sp_gen_zipf: bad ndim");
+    DT_CHECK_ARG(ndraws >= 1 && ndraws < ((int64_t)1 << 31), "dt_sp_gen_zipf: draws out of range");
+    int shift[DT_MAX_NDIM], nbits[DT_MAX_NDIM], tot = 0;
+    int64_t dd[DT_MAX_NDIM], coff[DT_MAX_NDIM], ctot = 0;
+    for (int n = 0; n < ndim; n++) {
+        DT_CHECK_ARG(dims[n] >= 1 && dims[n] < ((size_t)1 << 31), "dt_sp_gen_zipf: bad dim %d", n);
+        DT_CHECK_ARG(idx_out[n], "dt_sp_gen_zipf: index output %d missing", n);
+        dd[n] = (int64_t)dims[n];
+        nbits[n] = bits_for(dd[n]);
+        shift[n] = tot;
+        tot += nbits[n];
+        coff[n] = ctot;
+        ctot += dd[n];
+    }
+    DT_CHECK_ARG(tot <= 64, "dt_sp_gen_zipf: composite key needs %d > 64 bits", tot);
+    // CDF tables on the host: P(k) ~ (k+1)^-s, sequential FP64 sums
+    std::vector<double> cdf((size_t)ctot);
+    for (int n = 0; n < ndim; n++) {
+        double s = 0.0;
+        for (int64_t k = 0; k < dd[n]; k++) {
+            s += std::pow((double)(k + 1), -exponent);
+            cdf[(size_t)(coff[n] + k)] = s;
+        }
+        for (int64_t k = 0; k < dd[n]; k++) cdf[(size_t)(coff[n] + k)] /= s;
+    }
+    double *dcdf = (double *)dt_ws(ctx, WS_SPARSE0, (size_t)ctot * 8 + 256);
+    int64_t *dmeta = (int64_t *)dt_ws(ctx, WS_SPARSE1, 64 * 8 * 4);
+    if (!dcdf || !dmeta) return DT_ENOMEM;
+    int64_t *d_coff = dmeta, *d_dims = dmeta + DT_MAX_NDIM;
+    int *d_shift = (int *)(dmeta + 2 * DT_MAX_NDIM), *d_nbits = d_shift + DT_MAX_NDIM;
+    int32_t **d_idx = (int32_t **)(dmeta + 4 * DT_MAX_NDIM);
+    int32_t *d_nlong = (int32_t *)(dmeta + 5 * DT_MAX_NDIM);
+    DT_HIP(hipMemcpyAsync(dcdf, cdf.data(), (size_t)ctot * 8, hipMemcpyHostToDevice, ctx->stream));
+    DT_HIP(hipMemcpyAsync(d_coff, coff, sizeof(coff), hipMemcpyHostToDevice, ctx->stream));
+    DT_HIP(hipMemcpyAsync(d_dims, dd, sizeof(dd), hipMemcpyHostToDevice, ctx->stream));
+    DT_HIP(hipMemcpyAsync(d_shift, shift, sizeof(shift), hipMemcpyHostToDevice, ctx->stream));
+    DT_HIP(hipMemcpyAsync(d_nbits, nbits, sizeof(nbits), hipMemcpyHostToDevice, ctx->stream));
+    DT_HIP(hipMemcpyAsync(d_idx, idx_out, sizeof(int32_t *) * ndim, hipMemcpyHostToDevice, ctx->stream));
+    DT_TRY(dt_zero(ctx, d_nlong, 4));
+    // draws: keys, positions, values; then a stable radix sort of (key, position)
+    const int64_t n = ndraws;
+    char *buf = (char *)dt_ws(ctx, WS_GEN, (size_t)n * (8 + 8 + 4 + 4 + 4 + 4 + 4 + 4) + 4096);
+    if (!buf) return DT_ENOMEM;
+    uint64_t *key = (uint64_t *)buf;
+    uint64_t *key2 = key + n;
+    int32_t *pos = (int32_t *)(key2 + n);
+    int32_t *pos2 = pos + n;
+    float *vdraw = (float *)(pos2 + n);
+    int32_t *flag = (int32_t *)(vdraw + n);
+    int32_t *uidx = flag + n;
+    int32_t *heads = uidx + n;
+    DT_LAUNCH("sp_gen", zipf_draw, sp_grid(n), dim3(256), 0, n, ndim, dcdf, d_coff, d_dims, d_shift, seed, key, pos,
+              vdraw);
+    size_t tb = 0;
+    DT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, key2, pos, pos2, (int)n, 0, tot, ctx->stream));
+    void *tmp = cub_tmp(ctx, tb);
+    if (!tmp) return DT_ENOMEM;
+    DT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, key, key2, pos, pos2, (int)n, 0, tot, ctx->stream));
+    DT_LAUNCH("sp_gen", run_heads, sp_grid(n), dim3(256), 0, key2, n, flag);
+    size_t tb2 = 0;
+    DT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, flag, uidx, (int)n, ctx->stream));
+    tmp = cub_tmp(ctx, tb2);
+    if (!tmp) return DT_ENOMEM;
+    DT_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb2, flag, uidx, (int)n, ctx->stream));
+    int32_t last[2];
+    DT_HIP(hipMemcpyAsync(&last[0], uidx + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+    DT_HIP(hipMemcpyAsync(&last[1], flag + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+    DT_HIP(hipStreamSynchronize(ctx->stream));
+    const int64_t nu = (int64_t)last[0] + last[1];
+    DT_LAUNCH("sp_gen", scatter_heads, sp_grid(n), dim3(256), 0, flag, uidx, n, heads);
+    int32_t *longq = uidx;  // reused: the run numbering is no longer needed
+    DT_LAUNCH("sp_gen", merge_runs, sp_grid(nu), dim3(256), 0, key2, pos2, vdraw, heads, nu, n, ndim, d_shift,
+              d_nbits, d_idx, val_out, longq, d_nlong);
+    int32_t nlong = 0;
+    DT_HIP(hipMemcpyAsync(&nlong, d_nlong, 4, hipMemcpyDeviceToHost, ctx->stream));
+    DT_HIP(hipStreamSynchronize(ctx->stream));
+    if (nlong > 0)
+        DT_LAUNCH("sp_gen", merge_long, dim3((unsigned)nlong), dim3(256), 0, pos2, vdraw, heads, nu, n, longq, val_out);
+    *nnz_out = nu;
+    return DT_OK;
+}
+
+extern "C" int dt_sp_destroy(dt_sp *sp) {
+    if (!sp) return DT_OK;
+    for (void *p : sp->owned) (void)hipFree(p);
+    delete sp;
+    return DT_OK;
+}
+
+extern "C" int dt_sp_create(dt_ctx *ctx, int ndim, const size_t *dims, int64_t nnz, const int32_t *const *idx,
+                            const float *val, dt_sp **out) {
+    DT_CHECK_ARG(ctx && dims && idx && val && out, "dt_sp_create: null argument");
+    DT_CHECK_ARG(ndim >= 2 && ndim <= DT_MAX_NDIM, "dt_sp_create: bad ndim %d", ndim);
+    DT_CHECK_ARG(nnz >= 1 && nnz < ((int64_t)1 << 31), "dt_sp_create: nnz out of range");
+    dt_sp *sp = new dt_sp();
+    sp->ndim = ndim;
+    sp->nnz = nnz;
+    int st = DT_OK;
+    // slice mode: the largest mode, first on ties (sp_tensor.hpp:107-115)
+    int64_t best = 0;
+    for (int n = 0; n < ndim; n++) {
+        sp->dims[n] = (int64_t)dims[n];
+        if (best < sp->dims[n]) {
+            best = sp->dims[n];
+            sp->slice_mode = n;
+        }
+    }
+    auto fail = [&](int code, const char *msg) {
+        dt_sp_destroy(sp);
+        return dt_fail(ctx, code, "%s", msg);
+    };
+    for (int n = 0; n < ndim; n++) {
+        sp->idx[n] = (int32_t *)sp_alloc(sp, (size_t)nnz * 4);
+        if (!sp->idx[n]) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
+        if (hipMemcpyAsync(sp->idx[n], idx[n], (size_t)nnz * 4, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+            return fail(DT_EHIP, "dt_sp_create: copy failed");
+    }
+    sp->val = (float *)sp_alloc(sp, (size_t)nnz * 4);
+    if (!sp->val) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
+    if (hipMemcpyAsync(sp->val, val, (size_t)nnz * 4, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+        return fail(DT_EHIP, "dt_sp_create: copy failed");
+    for (int n = 0; n < ndim; n++) {
+        dt_sp_mode &m = sp->mode[n];
+        const int64_t I = sp->dims[n];
+        m.perm = (int32_t *)sp_alloc(sp, (size_t)nnz * 4);
+        m.row = (int32_t *)sp_alloc(sp, (size_t)nnz * 4);
+        m.val = (float *)sp_alloc(sp, (size_t)nnz * 4);
+        m.rowptr = (int32_t *)sp_alloc(sp, (size_t)(I + 1) * 4);
+        m.slot = (int32_t *)sp_alloc(sp, (size_t)(I + 1) * 4);
+        if (!m.perm || !m.row || !m.val || !m.rowptr || !m.slot) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
+        if (ndim == 3) {
+            m.ia = (int32_t *)sp_alloc(sp, (size_t)nnz * 4);
+            m.ib = (int32_t *)sp_alloc(sp, (size_t)nnz * 4);
+            if (!m.ia || !m.ib) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
+        }
+        st = sort_mode_internal(ctx, nnz, sp->idx[n], I, m.perm, m.row, m.rowptr);
+        if (st != DT_OK) return fail(st, "dt_sp_create: mode sort failed");
+        if (ndim == 3) {
+            const int a = n == 0 ? 1 : 0, b = n == 2 ? 1 : 2;
+            hipLaunchKernelGGL(gather_by<int32_t>, sp_grid(nnz), dim3(256), 0, ctx->stream, sp->idx[a], m.perm, nnz, m.ia);
+            hipLaunchKernelGGL(gather_by<int32_t>, sp_grid(nnz), dim3(256), 0, ctx->stream, sp->idx[b], m.perm, nnz, m.ib);
+        }
+        hipLaunchKernelGGL(gather_by<float>, sp_grid(nnz), dim3(256), 0, ctx->stream, sp->val, m.perm, nnz, m.val);
+        // compact numbering of the non-empty rows
+        int32_t *f = (int32_t *)dt_ws(ctx, WS_SPARSE1, (size_t)(I + 1) * 4 + 256);
+        if (!f) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
+        hipLaunchKernelGGL(nonempty_flags, sp_grid(I), dim3(256), 0, ctx->stream, m.rowptr, I, f);
+        size_t tb = 0;
+        if (hipcub::DeviceScan::ExclusiveSum(nullptr, tb, f, m.slot, (int)I, ctx->stream) != hipSuccess)
+            return fail(DT_EHIP, "dt_sp_create: scan failed");
+        void *tmp = cub_tmp(ctx, tb);
+        if (!tmp || hipcub::DeviceScan::ExclusiveSum(tmp, tb, f, m.slot, (int)I, ctx->stream) != hipSuccess)
+            return fail(DT_EHIP, "dt_sp_create: scan failed");
+        int32_t lastv[2];
+        if (hipMemcpyAsync(&lastv[0], m.slot + I - 1, 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipMemcpyAsync(&lastv[1], f + I - 1, 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            return fail(DT_EHIP, "dt_sp_create: readback failed");
+        m.nrows = (int64_t)lastv[0] + lastv[1];
+        m.rows = (int32_t *)sp_alloc(sp, (size_t)std::max<int64_t>(m.nrows, 1) * 4);
+        if (!m.rows) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
+        hipLaunchKernelGGL(rows_list, sp_grid(I), dim3(256), 0, ctx->stream, f, m.slot, I, m.rows);
+        if (hipGetLastError() != hipSuccess) return fail(DT_EHIP, "dt_sp_create: launch failed");
+    }
+    // ||A||^2 in FP64
+    double *part = (double *)dt_ws(ctx, WS_SUMSQ, 1024 * 8);
+    if (!part) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
+    hipLaunchKernelGGL(sumsq_f32_sp, dim3(1024), dim3(256), 0, ctx->stream, sp->val, nnz, part);
+    std::vector<double> hp(1024);
+    if (hipMemcpyAsync(hp.data(), part, 1024 * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return fail(DT_EHIP, "dt_sp_create: readback failed");
+    for (double v : hp) sp->normA2 += v;
+    *out = sp;
+    return DT_OK;
+}
+
+extern "C" int dt_sp_info(const dt_sp *sp, int *slice_mode, int64_t *nnz, double *norm) {
+    if (!sp) return DT_EINVAL;
+    if (slice_mode) *slice_mode = sp->slice_mode;
+    if (nnz) *nnz = sp->nnz;
+    if (norm) *norm = std::sqrt(sp->normA2);
+    return DT_OK;
+}
+
+extern "C" int dt_sp_mode_arrays(const dt_sp *sp, int n, const int32_t **perm, const int32_t **rowptr,
+                                 int64_t *nonempty_rows) {
+    if (!sp || n < 0 || n >= sp->ndim) return DT_EINVAL;
+    if (perm) *perm = sp->mode[n].perm;
+    if (rowptr) *rowptr = sp->mode[n].rowptr;
+    if (nonempty_rows) *nonempty_rows = sp->mode[n].nrows;
+    return DT_OK;
+}
+
+// ---- TTMc driver ---------------------------------------------------------------
+
+// Yc (nrows x 256) and, with Un, L (256 x 16 FP64) for mode n of a 3-way
+// tensor with ranks 16 in the other modes.  Ur[m]: row-major FP32 I_m x 16.
+static int sp_ttmc_dev(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, float *Yc, double *L) {
+    DT_CHECK_ARG(sp->ndim == 3, "sparse TTMc: 3-way tensors only (got %d)", sp->ndim);
+    const dt_sp_mode &m = sp->mode[n];
+    const int a = n == 0 ? 1 : 0, b = n == 2 ? 1 : 2;
+    SpModeArgs args{m.row, m.ia, m.ib, m.val, m.rowptr, m.slot, sp->nnz};
+    const int64_t nchunks = (sp->nnz + SP_T - 1) / SP_T;
+    float *carry = (float *)dt_ws(ctx, WS_SPARSE0, (size_t)nchunks * 2 * SP_E * 4 + (size_t)nchunks * 2 * 4 + 512);
+    if (!carry) return DT_ENOMEM;
+    int32_t *carry_row = (int32_t *)(carry + nchunks * 2 * SP_E);
+    static int blocks_per_cu = 0, cus = 0;
+    if (!blocks_per_cu) {
+        int dev = 0;
+        DT_HIP(hipGetDevice(&dev));
+        DT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        DT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, sp_ttmc16, 256, 0));
+        if (blocks_per_cu < 1) blocks_per_cu = 1;
+    }
+    const int64_t nwg = std::min<int64_t>((nchunks + 3) / 4, (int64_t)cus * blocks_per_cu);
+    float *Lwg = nullptr;
+    if (L) {
+        Lwg = (float *)dt_ws(ctx, WS_SPARSE1, (size_t)nwg * SP_E * 16 * 4 + 256);
+        if (!Lwg) return DT_ENOMEM;
+    }
+    DT_LAUNCH("sp_ttmc", sp_ttmc16, dim3((unsigned)nwg), dim3(256), 0, args, Ur[a], Ur[b], L ? Ur[n] : nullptr, Yc,
+              carry, carry_row, Lwg, nchunks);
+    DT_LAUNCH("sp_fixup", sp_fixup, dim3((unsigned)nchunks), dim3(64), 0, args, carry, carry_row, nchunks, Yc);
+    if (L) DT_LAUNCH("sp_lreduce", sp_lreduce, dim3((SP_E * 16 + 255) / 256), dim3(256), 0, Lwg, (int)nwg, L);
+    return DT_OK;
+}
+
+extern "C" int dt_sp_ttmc(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, const size_t *ranks, float *Y) {
+    DT_CHECK_ARG(ctx && sp && Ur && ranks && Y, "dt_sp_ttmc: null argument");
+    DT_CHECK_ARG(n >= 0 && n < sp->ndim, "dt_sp_ttmc: bad mode %d", n);
+    DT_CHECK_ARG(sp->ndim == 3, "dt_sp_ttmc: 3-way tensors only");
+    for (int m = 0; m < 3; m++)
+        if (m != n) DT_CHECK_ARG(ranks[m] == 16 && Ur[m], "dt_sp_ttmc: ranks of the other modes must be 16");
+    const dt_sp_mode &md = sp->mode[n];
+    float *Yc = (float *)dt_ws(ctx, WS_SPARSE2, (size_t)std::max<int64_t>(md.nrows, 1) * SP_E * 4);
+    if (!Yc) return DT_ENOMEM;
+    DT_TRY(sp_ttmc_dev(ctx, sp, n, Ur, Yc, nullptr));
+    DT_TRY(dt_zero(ctx, Y, (size_t)sp->dims[n] * SP_E * 4));
+    DT_LAUNCH("sp_expand", sp_expand, sp_grid(md.nrows * SP_E), dim3(256), 0, Yc, md.rows, md.nrows, Y);
+    return DT_OK;
+}
+
+// ---- sparse HOOI -----------------------------------------------------------------
+
+extern "C" int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int max_iter, uint64_t seed,
+                              double *const *U_out, double *G_out, double *residual_per_sweep, double *residual,
+                              double *sweeps_ms) {
+    DT_CHECK_ARG(ctx && sp && ranks && residual, "dt_hooi_sparse: null argument");
+    DT_CHECK_ARG(sp->ndim == 3, "dt_hooi_sparse: 3-way tensors only");
+    for (int m = 0; m < 3; m++) DT_CHECK_ARG(ranks[m] == 16, "dt_hooi_sparse: ranks must be 16 (got %zu)", ranks[m]);
+    DT_CHECK_ARG(max_iter >= 1, "dt_hooi_sparse: max_iter >= 1");
+    const int R = 16;
+    // factors: FP64 I x R column-major (orthonormalised), FP32 row-major views
+    std::vector<void *> mine;
+    auto alloc = [&](size_t bytes) -> void * {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) return nullptr;
+        mine.push_back(p);
+        return p;
+    };
+    auto cleanup = [&] {
+        (void)hipStreamSynchronize(ctx->stream);
+        for (void *p : mine) (void)hipFree(p);
+    };
+    double *U64[3], *Urow64 = nullptr;
+    float *U32r[3];
+    int64_t maxI = 0, maxrows = 0;
+    for (int m = 0; m < 3; m++) {
+        maxI = std::max(maxI, sp->dims[m]);
+        maxrows = std::max(maxrows, sp->mode[m].nrows);
+    }
+    for (int m = 0; m < 3; m++) {
+        U64[m] = (double *)alloc((size_t)sp->dims[m] * R * 8);
+        U32r[m] = (float *)alloc((size_t)sp->dims[m] * R * 4);
+        if (!U64[m] || !U32r[m]) {
+            cleanup();
+            return dt_fail(ctx, DT_ENOMEM, "dt_hooi_sparse: out of memory");
+        }
+    }
+    Urow64 = (double *)alloc((size_t)maxI * R * 8);
+    float *Yc = (float *)alloc((size_t)std::max<int64_t>(maxrows, 1) * SP_E * 4);
+    double *L = (double *)alloc(SP_E * 16 * 8);
+    double *G64 = (double *)alloc(SP_E * 16 * 8);
+    if (!Urow64 || !Yc || !L || !G64) {
+        cleanup();
+        return dt_fail(ctx, DT_ENOMEM, "dt_hooi_sparse: out of memory");
+    }
+    int st = DT_OK;
+    auto views = [&](int m) {
+        return dt_f64_to_views(ctx, U64[m], (size_t)sp->dims[m], (size_t)R, nullptr, U32r[m], nullptr, nullptr);
+    };
+    // random orthonormal start (Philox stream 200 + m)
+    for (int m = 0; m < 3 && st == DT_OK; m++) {
+        st = dt_fill_normal_f64(ctx, U64[m], (size_t)sp->dims[m] * R, seed, 200 + (uint64_t)m);
+        if (st == DT_OK) st = dt_qr_f64(ctx, U64[m], nullptr, U64[m], (size_t)sp->dims[m], (size_t)R);
+        if (st == DT_OK) st = views(m);
+    }
+    hipEvent_t e0 = dt_get_event(ctx), e1 = dt_get_event(ctx);
+    double total_ms = 0.0, res = 0.0;
+    for (int it = 0; it < max_iter && st == DT_OK; it++) {
+        (void)hipEventRecord(e0, ctx->stream);
+        for (int n = 0; n < 3 && st == DT_OK; n++) {
+            const dt_sp_mode &md = sp->mode[n];
+            const int64_t I = sp->dims[n];
+            const float *Ur[3] = {U32r[0], U32r[1], U32r[2]};
+            st = sp_ttmc_dev(ctx, sp, n, Ur, Yc, L);           // Y_c and L = Y_cᵀ U_n
+            if (st == DT_OK) st = dt_zero(ctx, Urow64, (size_t)I * R * 8);
+            if (st == DT_OK) {
+                hipLaunchKernelGGL(sp_times_l, dim3((unsigned)std::min<int64_t>((md.nrows + 3) / 4, 4096)), dim3(256),
+                                   0, ctx->stream, Yc, md.rows, md.nrows, L, Urow64);
+                hipLaunchKernelGGL(rowmajor_to_colmajor, sp_grid(I * R), dim3(256), 0, ctx->stream, Urow64, I, R,
+                                   U64[n]);
+                st = hipGetLastError() == hipSuccess ? DT_OK : DT_EHIP;
+            }
+            if (st == DT_OK) st = dt_qr_f64(ctx, U64[n], nullptr, U64[n], (size_t)I, (size_t)R);  // orth(S U_n)
+            if (st == DT_OK) st = views(n);
+        }
+        // core = Y_(2)ᵀ U_2 with the updated U_2 (Y_c of the last mode is current)
+        if (st == DT_OK) {
+            const int n = 2;
+            const dt_sp_mode &md = sp->mode[n];
+            const float *Ur[3] = {U32r[0], U32r[1], U32r[2]};
+            st = sp_ttmc_dev(ctx, sp, n, Ur, Yc, G64);
+            (void)md;
+        }
+        (void)hipEventRecord(e1, ctx->stream);
+        if (st != DT_OK) break;
+        std::vector<double> g(SP_E * 16);
+        if (hipMemcpyAsync(g.data(), G64, g.size() * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            st = DT_EHIP;
+            break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        total_ms += ms;
+        double g2 = 0.0;
+        for (double v : g) g2 += v * v;
+        res = std::sqrt(std::max(0.0, 1.0 - g2 / sp->normA2));
+        if (residual_per_sweep) residual_per_sweep[it] = res;
+    }
+    ctx->event_pool.push_back(e0);
+    ctx->event_pool.push_back(e1);
+    if (st == DT_OK) {
+        *residual = res;
+        if (sweeps_ms) *sweeps_ms = total_ms;
+        for (int m = 0; m < 3; m++)
+            if (U_out && U_out[m])
+                if (hipMemcpyAsync(U_out[m], U64[m], (size_t)sp->dims[m] * R * 8, hipMemcpyDeviceToDevice,
+                                   ctx->stream) != hipSuccess)
+                    st = DT_EHIP;
+        // G (16 x 16 x 16, column-major): G[r0 + 16 r1 + 256 r2] = L[(r0 + 16 r1) * 16 + r2]
+        if (G_out && st == DT_OK) {
+            std::vector<double> g(SP_E * 16), gc(SP_E * 16);
+            if (hipMemcpyAsync(g.data(), G64, g.size() * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess)
+                st = DT_EHIP;
+            for (int e = 0; e < SP_E; e++)
+                for (int k = 0; k < 16; k++) gc[(size_t)(e + SP_E * k)] = g[(size_t)(e * 16 + k)];
+            if (st == DT_OK &&
+                hipMemcpyAsync(G_out, gc.data(), gc.size() * 8, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+                st = DT_EHIP;
+        }
+    }
+    cleanup();
+    if (st != DT_OK && ctx->err.empty()) return dt_fail(ctx, st, "dt_hooi_sparse: failed");
+    return st;
+}

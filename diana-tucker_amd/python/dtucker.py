@@ -112,6 +112,16 @@ def lib():
             "dt_sketch_features": [vp, vp, i32, i32, C.POINTER(sz), i32, i32, u64, vp],
             "dt_sketch_features_kron": [vp, vp, i32, i32, C.POINTER(sz), i32, i32, u64, vp],
             "dt_learned_start": [vp, vp, vp, i32, i32, C.POINTER(sz), C.POINTER(sz), i32, u64, C.POINTER(vp)],
+            "dt_sp_gen_zipf": [vp, i32, C.POINTER(sz), C.c_int64, dbl, u64, C.POINTER(vp), vp,
+                               C.POINTER(C.c_int64)],
+            "dt_sp_sort_mode": [vp, C.c_int64, vp, sz, vp, vp],
+            "dt_sp_create": [vp, i32, C.POINTER(sz), C.c_int64, C.POINTER(vp), vp, C.POINTER(vp)],
+            "dt_sp_destroy": [vp],
+            "dt_sp_info": [vp, C.POINTER(i32), C.POINTER(C.c_int64), C.POINTER(dbl)],
+            "dt_sp_mode_arrays": [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_int64)],
+            "dt_sp_ttmc": [vp, vp, i32, C.POINTER(vp), C.POINTER(sz), vp],
+            "dt_hooi_sparse": [vp, vp, C.POINTER(sz), i32, u64, C.POINTER(vp), vp, C.POINTER(dbl),
+                               C.POINTER(dbl), C.POINTER(dbl)],
         }
         for name, argtypes in sig.items():
             f = getattr(L, name)
@@ -132,7 +142,9 @@ def exported_symbols():
             "dt_hooi_create", "dt_hooi_sweep", "dt_hooi_fit", "dt_hooi_get", "dt_hooi_destroy",
             "dt_mlp_create", "dt_mlp_destroy", "dt_mlp_shape", "dt_mlp_get_params", "dt_mlp_set_params",
             "dt_mlp_forward", "dt_mlp_loss_grad", "dt_mlp_adam_step", "dt_sketch_features",
-            "dt_sketch_features_kron", "dt_learned_start"]
+            "dt_sketch_features_kron", "dt_learned_start", "dt_sp_gen_zipf", "dt_sp_sort_mode",
+            "dt_sp_create", "dt_sp_destroy", "dt_sp_info", "dt_sp_mode_arrays", "dt_sp_ttmc",
+            "dt_hooi_sparse"]
 
 
 class DeviceArray:
@@ -577,3 +589,80 @@ def upload_rows(ctx: Context, X: np.ndarray) -> DeviceArray:
 
 def download_rows(D: DeviceArray) -> np.ndarray:
     return D.download().T.copy()
+
+
+class SpTensor:
+    """dt_sp: a 3-way COO tensor on the device with its per-mode sorted copies."""
+
+    def __init__(self, ctx: Context, dims, idx, val, nnz: int):
+        self.ctx = ctx
+        self.dims = tuple(int(d) for d in dims)
+        self.nnz = int(nnz)
+        h = C.c_void_p()
+        nd = len(self.dims)
+        dd = (C.c_size_t * nd)(*self.dims)
+        ip = (C.c_void_p * nd)(*[a.ptr for a in idx])
+        ctx.check(ctx.L.dt_sp_create(ctx.handle, nd, dd, self.nnz, ip, val.ptr, C.byref(h)))
+        self.handle = h
+        sm, nz, nrm = C.c_int(), C.c_int64(), C.c_double()
+        ctx.L.dt_sp_info(h, C.byref(sm), C.byref(nz), C.byref(nrm))
+        self.slice_mode, self.norm = sm.value, nrm.value
+
+    @staticmethod
+    def from_host(ctx: Context, dims, idx_lists, vals):
+        idx = [ctx.to_device(np.ascontiguousarray(a, dtype=np.int32)) for a in idx_lists]
+        val = ctx.to_device(np.ascontiguousarray(vals, dtype=np.float32))
+        return SpTensor(ctx, dims, idx, val, len(vals))
+
+    def mode_arrays(self, n: int):
+        """(perm, rowptr, nonempty_rows) of mode n, downloaded."""
+        pp, rp, nr = C.c_void_p(), C.c_void_p(), C.c_int64()
+        self.ctx.check(self.ctx.L.dt_sp_mode_arrays(self.handle, n, C.byref(pp), C.byref(rp), C.byref(nr)))
+        perm = np.empty(self.nnz, np.int32)
+        rowptr = np.empty(self.dims[n] + 1, np.int32)
+        self.ctx.d2h(perm.ctypes.data, pp.value, perm.nbytes)
+        self.ctx.d2h(rowptr.ctypes.data, rp.value, rowptr.nbytes)
+        return perm, rowptr, nr.value
+
+    def ttmc(self, n: int, Ur):
+        """Y_(n) (dims[n] x 256) for row-major FP32 device factors Ur[m] (dims[m] x 16)."""
+        Y = self.ctx.rows_array(self.dims[n], 256)
+        ptrs = (C.c_void_p * 3)(*[u.ptr if u is not None else None for u in Ur])
+        rk = (C.c_size_t * 3)(16, 16, 16)
+        self.ctx.check(self.ctx.L.dt_sp_ttmc(self.ctx.handle, self.handle, n, ptrs, rk, Y.ptr))
+        return Y
+
+    def hooi(self, max_iter=10, seed=0, want_factors=True):
+        U = [self.ctx.array((d, 16), DT_F64) for d in self.dims] if want_factors else []
+        G = self.ctx.array((16, 16, 16), DT_F64) if want_factors else None
+        up = (C.c_void_p * 3)(*[u.ptr for u in U]) if U else None
+        per = (C.c_double * max_iter)()
+        res, ms = C.c_double(), C.c_double()
+        rk = (C.c_size_t * 3)(16, 16, 16)
+        self.ctx.check(self.ctx.L.dt_hooi_sparse(self.ctx.handle, self.handle, rk, max_iter, seed, up,
+                                                  G.ptr if G else None, per, C.byref(res), C.byref(ms)))
+        return {"U": [u.download() for u in U], "G": G.download() if G else None,
+                "residuals": list(per), "residual": res.value, "sweeps_ms": ms.value}
+
+    def close(self):
+        if self.handle:
+            self.ctx.L.dt_sp_destroy(self.handle)
+            self.handle = None
+
+
+def sp_gen_zipf(ctx: Context, dims, ndraws: int, exponent: float, seed: int):
+    """(idx DeviceArrays, val DeviceArray, nnz): merged Zipf COO (capacity ndraws)."""
+    idx = [ctx.array((ndraws,), DT_I32) for _ in dims]
+    val = ctx.array((ndraws,), DT_F32)
+    nnz = C.c_int64()
+    dd = (C.c_size_t * len(dims))(*dims)
+    ip = (C.c_void_p * len(dims))(*[a.ptr for a in idx])
+    ctx.check(ctx.L.dt_sp_gen_zipf(ctx.handle, len(dims), dd, ndraws, exponent, seed, ip, val.ptr, C.byref(nnz)))
+    return idx, val, nnz.value
+
+
+def sp_sort_mode(ctx: Context, keys: DeviceArray, nnz: int, dim: int):
+    perm = ctx.array((nnz,), DT_I32)
+    rowptr = ctx.array((dim + 1,), DT_I32)
+    ctx.check(ctx.L.dt_sp_sort_mode(ctx.handle, nnz, keys.ptr, dim, perm.ptr, rowptr.ptr))
+    return perm, rowptr

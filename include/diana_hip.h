@@ -169,6 +169,49 @@ typedef struct dt_tucker_spec {
 int dt_gen_tucker(dt_ctx *ctx, const dt_tucker_spec *spec, float *A_f32, uint16_t *A_bf16,
                   double *const *U_true, double *normX_host, double *normA_host);
 
+/* ---- sparse ---------------------------------------------------------------
+ * 3-way sparse HOOI of Algorithm::SpTucker::Sp_HOOI_ALS
+ * (template/algorithm/sp_tucker/sp_hooi_als.tpp:67-118): random orthonormal
+ * start, then per sweep and mode n one step U_n <- orth(Y_(n) Y_(n)ᵀ U_n)
+ * (Sp_TTMC_ALS_, :48-65 = ttmc_mTR + ttmc_mTL + orth2), core = Y_(N-1)ᵀ U_{N-1}
+ * (:110).  Indices int32, values FP32, all device pointers. */
+typedef struct dt_sp dt_sp;
+/* COO from independent per-mode truncated Zipf draws P(k) ~ (k+1)^-s (inverse
+ * CDF on an FP64 table; Philox uniform streams 7000 + n, values stream 7100,
+ * U(0,1] in FP32); duplicates merged: entries in increasing composite-key
+ * order (mode 0 in the lowest bits), value = FP64 sum of the draws in draw
+ * order rounded to FP32.  idx_out[n] and val_out hold ndraws entries;
+ * *nnz_out receives the merged count.  No reference counterpart (the
+ * reference reads .tns files, src/read_data.cpp:36-129). */
+int dt_sp_gen_zipf(dt_ctx *ctx, int ndim, const size_t *dims, int64_t ndraws, double exponent,
+                   uint64_t seed, int32_t *const *idx_out, float *val_out, int64_t *nnz_out);
+/* Stable sort of nonzeros by one index list: perm (nnz) and row pointers
+ * (dim + 1).  The SpTensor constructor's counting sort (sp_tensor.hpp:117-147)
+ * as a stable LSD radix sort: perm is identical. */
+int dt_sp_sort_mode(dt_ctx *ctx, int64_t nnz, const int32_t *keys, size_t dim, int32_t *perm,
+                    int32_t *rowptr);
+/* SpTensor(data_buffer*) (sp_tensor.hpp:72-240): copies the COO and builds a
+ * stably sorted copy per mode; slice mode = the largest mode, first on ties. */
+int dt_sp_create(dt_ctx *ctx, int ndim, const size_t *dims, int64_t nnz, const int32_t *const *idx,
+                 const float *val, dt_sp **sp);
+int dt_sp_destroy(dt_sp *sp);
+int dt_sp_info(const dt_sp *sp, int *slice_mode, int64_t *nnz, double *norm);
+int dt_sp_mode_arrays(const dt_sp *sp, int n, const int32_t **perm, const int32_t **rowptr,
+                      int64_t *nonempty_rows);
+/* Y_(n) of A x_{m != n} U_mᵀ: Y (shape[n] x 256, row-major FP32; row i =
+ * sum over the nonzeros of slice i of val * U_a[i_a,:] ⊗ U_b[i_b,:], entry
+ * r_a + 16 r_b, a < b the other modes), Ur[m] row-major FP32 shape[m] x 16.
+ * Replaces the Kronecker accumulation of ttmc_mTR / ttmc_mTL
+ * (sp_tensor.tpp:236-365, add_outer_product_all :367-417); ranks 16 only. */
+int dt_sp_ttmc(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, const size_t *ranks, float *Y);
+/* Sparse HOOI: ranks (16,16,16), max_iter sweeps from the Philox random start
+ * (streams 200 + n).  U_out[n] (shape[n] x 16 FP64 column-major) and G_out
+ * (16^3 FP64 column-major) may be NULL; residual_per_sweep holds max_iter
+ * entries (or NULL); sweeps_ms = device time of the sweeps. */
+int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int max_iter, uint64_t seed,
+                   double *const *U_out, double *G_out, double *residual_per_sweep,
+                   double *residual, double *sweeps_ms);
+
 /* ---- learned start ------------------------------------------------------
  * No counterpart in the reference (it starts from random factors,
  * hooi_als.tpp:76-81): the learned initialisation that replaces the HOSVD

@@ -1,0 +1,157 @@
+"""Sparse path on the GPU: Zipf COO generator and duplicate merging, the
+stable per-mode sort (bit-exact against the reference constructor's counting
+sort), the TTMc and the sparse HOOI, all against oracle/sparse.py."""
+import numpy as np
+import pytest
+
+import dtucker as dt
+from oracle import dense as D
+from oracle import philox
+from oracle import sparse as S
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+@pytest.mark.parametrize("dims,ndraws,seed", [((500, 300, 200), 20000, 1),
+                                              ((1 << 20, 1 << 18, 1 << 16), 200000, 20000905),
+                                              ((64, 64, 64, 8), 30000, 5)])
+def test_zipf_generator_bit_exact(ctx, dims, ndraws, seed):
+    """Indices, merge order and merged values identical to the oracle."""
+    print("zipf", dims, ndraws, flush=True)
+    idx, val, nnz = dt.sp_gen_zipf(ctx, dims, ndraws, 1.1, seed)
+    ridx, rval = S.gen_zipf(dims, ndraws, 1.1, seed)
+    assert nnz == len(rval)
+    for n in range(len(dims)):
+        assert np.array_equal(idx[n].download()[:nnz], ridx[n])
+    assert np.array_equal(val.download()[:nnz], rval)
+
+
+@pytest.mark.parametrize("n,dim,seed", [(1000, 7, 1), (100000, 1000, 2), (3000000, 1 << 20, 3)])
+def test_sort_mode_bit_exact(ctx, n, dim, seed):
+    """perm == argsort(kind=stable) (the constructor's counting sort), row pointers == histogram scan."""
+    rng = np.random.default_rng(seed)
+    keys = np.minimum((rng.pareto(1.1, n) * 3).astype(np.int64), dim - 1).astype(np.int32)
+    perm, rowptr = dt.sp_sort_mode(ctx, ctx.to_device(keys), n, dim)
+    rp, rr = S.sort_mode(keys, dim)
+    assert np.array_equal(perm.download(), rp)
+    assert np.array_equal(rowptr.download(), rr)
+
+
+def make_coo(rng, dims, nnz, long_row=None):
+    """Skewed random COO (duplicates merged) with empty rows and, optionally, one
+    mode-0 row longer than a TTMc chunk."""
+    idx = [np.minimum((rng.pareto(1.0, nnz) * d / 8).astype(np.int64), d - 1) for d in dims]
+    if long_row is not None:
+        idx[0][:long_row] = 3
+    vals = rng.random(nnz).astype(np.float32)
+    return S.merge_duplicates(dims, idx, vals)
+
+
+def test_sp_create_modes_and_slice_mode(ctx):
+    rng = np.random.default_rng(7)
+    dims = (120, 300, 300)
+    idx, vals = make_coo(rng, dims, 5000)
+    sp = dt.SpTensor.from_host(ctx, dims, idx, vals)
+    try:
+        assert sp.slice_mode == 1  # largest mode, first on ties
+        for n in range(3):
+            perm, rowptr, nrows = sp.mode_arrays(n)
+            rp, rr = S.sort_mode(idx[n], dims[n])
+            assert np.array_equal(perm, rp) and np.array_equal(rowptr, rr)
+            assert nrows == int(np.count_nonzero(np.diff(rr)))
+        assert np.isclose(sp.norm, np.sqrt(np.sum(vals.astype(np.float64) ** 2)), rtol=1e-12)
+    finally:
+        sp.close()
+
+
+@pytest.mark.parametrize("dims,nnz,long_row", [((300, 200, 100), 40000, 9000), ((64, 4096, 512), 20000, None),
+                                               ((2000, 50, 40), 3000, 2500)])
+def test_sp_ttmc_vs_oracle(ctx, dims, nnz, long_row):
+    """Every mode's TTMc (rows across chunk boundaries, empty rows) against the
+    oracle's Kronecker accumulation; FP32 accumulation: 2e-6 relative."""
+    print("sp_ttmc", dims, nnz, long_row, flush=True)
+    rng = np.random.default_rng(sum(dims))
+    idx, vals = make_coo(rng, dims, nnz, long_row)
+    Us = [rng.standard_normal((d, 16)).astype(np.float32) for d in dims]
+    sp = dt.SpTensor.from_host(ctx, dims, idx, vals)
+    try:
+        dU = [dt.upload_rows(ctx, u) for u in Us]
+        for n in range(3):
+            Y = dt.download_rows(sp.ttmc(n, dU))
+            Yr = S.ttmc(dims, idx, vals, [u.astype(np.float64) for u in Us], n)
+            assert rel(Y, Yr) < 2e-6, n
+    finally:
+        sp.close()
+
+
+def test_sparse_hooi_vs_oracle(ctx):
+    """Three sweeps from the same start (Philox normals, positive-diagonal QR):
+    residuals to 1e-5 relative, factors by principal angles."""
+    rng = np.random.default_rng(11)
+    dims = (400, 300, 200)
+    idx, vals = make_coo(rng, dims, 60000)
+    seed = 77
+    sp = dt.SpTensor.from_host(ctx, dims, idx, vals)
+    try:
+        out = sp.hooi(max_iter=3, seed=seed)
+    finally:
+        sp.close()
+    U0 = [D.orth_posdiag(philox.normal(d * 16, seed, 200 + m).reshape((d, 16), order="F"))
+          for m, d in enumerate(dims)]
+    Ur, Gr, res = S.hooi(dims, idx, vals, U0, 3)
+    print("residuals gpu", out["residuals"], "oracle", res, flush=True)
+    assert np.allclose(out["residuals"], res, rtol=1e-5)
+    for n in range(3):
+        assert D.sin_max_angle(out["U"][n], Ur[n]) < 1e-3
+    G = out["G"]
+    assert np.isclose(np.linalg.norm(G), np.linalg.norm(Gr), rtol=1e-5)
+
+
+def test_sparse_bad_ranks_einval(ctx):
+    idx, vals = make_coo(np.random.default_rng(1), (40, 30, 20), 500)
+    sp = dt.SpTensor.from_host(ctx, (40, 30, 20), idx, vals)
+    try:
+        with pytest.raises(dt.DTError) as e:
+            ctx.check(ctx.L.dt_hooi_sparse(ctx.handle, sp.handle, (dt.C.c_size_t * 3)(8, 8, 8), 1, 0, None, None,
+                                           None, dt.C.byref(dt.C.c_double()), None))
+        assert e.value.code == dt.DT_EINVAL
+    finally:
+        sp.close()
+
+
+def test_c4_full_size_generator_and_sort(ctx):
+    """The sparse workload at its real size: 2^20 x 2^18 x 2^16, 1e8 Zipf(1.1)
+    draws.  Size-independent checks: ~65.6 % of the draws survive the merge,
+    merged keys strictly increase, values stay in (0, #draws of the key], the
+    slice-mode permutation is the stable argsort (keys non-decreasing along it
+    and, within equal keys, positions increasing) and the row pointers count it."""
+    dims = (1 << 20, 1 << 18, 1 << 16)
+    print("C4 generate", flush=True)
+    idx, val, nnz = dt.sp_gen_zipf(ctx, dims, 100_000_000, 1.1, 20000905)
+    print("C4 nnz", nnz, flush=True)
+    assert 0.60 < nnz / 1e8 < 0.71
+    i0 = idx[0].download()[:nnz].astype(np.int64)
+    i1 = idx[1].download()[:nnz].astype(np.int64)
+    i2 = idx[2].download()[:nnz].astype(np.int64)
+    key = i0 | (i1 << 20) | (i2 << 38)
+    assert np.all(np.diff(key) > 0)
+    v = val.download()[:nnz]
+    assert np.all(v > 0)
+    sp = dt.SpTensor(ctx, dims, idx, val, nnz)
+    try:
+        assert sp.slice_mode == 0
+        perm, rowptr, nrows = sp.mode_arrays(0)
+        ks = i0[perm]
+        assert np.all(np.diff(ks) >= 0)
+        same = np.diff(ks) == 0
+        assert np.all(np.diff(perm.astype(np.int64))[same] > 0)
+        assert np.array_equal(np.diff(rowptr.astype(np.int64)), np.bincount(i0, minlength=dims[0]))
+        print("C4 mode-0 rows touched", nrows, flush=True)
+    finally:
+        sp.close()

@@ -229,3 +229,69 @@ def test_sketch_features_closed_form():
         # sign convention: largest-magnitude entry of each column positive
         idx = np.argmax(np.abs(X[:, :3]), axis=0)
         assert np.all(X[idx, np.arange(3)] > 0)
+
+
+# ---- sparse path (oracle/sparse.py) -------------------------------------------
+
+def test_sparse_ttmc_oracle_equals_dense_chain():
+    """The per-nonzero Kronecker accumulation equals the dense TTM chain on the
+    densified tensor (duplicates summed first: linearity)."""
+    from oracle import sparse as S
+    rng = np.random.default_rng(3)
+    dims = (9, 7, 6)
+    idx = [rng.integers(0, d, 80) for d in dims]
+    vals = rng.random(80)
+    Us = [rng.standard_normal((d, r)) for d, r in zip(dims, (3, 4, 2))]
+    A = S.densify(dims, idx, vals)
+    for n in range(3):
+        Y = S.ttmc(dims, idx, vals, Us, n)
+        T = A
+        for m in range(3):
+            if m != n:
+                T = D.ttm(T, Us[m].T, m)
+        assert np.allclose(Y, D.tenmat(T, n), rtol=1e-12, atol=1e-12)
+
+
+def test_sparse_stable_sort_matches_counting_sort():
+    """argsort(kind=stable) == the constructor's histogram/scan/scatter in input order."""
+    from oracle import sparse as S
+    keys = np.random.default_rng(4).integers(0, 17, 500)
+    perm, rowptr = S.sort_mode(keys, 17)
+    tmp = rowptr[:-1].copy()
+    out = np.empty(500, dtype=np.int64)
+    for i, k in enumerate(keys):  # sp_tensor.hpp:128-147
+        out[tmp[k]] = i
+        tmp[k] += 1
+    assert np.array_equal(perm, out)
+
+
+def test_sparse_hooi_oracle_recovers_low_rank():
+    """An exactly rank-(2,2,2) tensor stored as COO (all entries): one-step-per-mode
+    subspace iteration reaches fit 1 and the true subspaces."""
+    from oracle import sparse as S
+    rng = np.random.default_rng(5)
+    dims, r = (8, 7, 6), 2
+    G = rng.standard_normal((r, r, r)) * 5
+    Ut = [np.linalg.qr(rng.standard_normal((d, r)))[0] for d in dims]
+    A = D.reconstruct(G, Ut)
+    idx = [a.ravel(order="F") for a in np.meshgrid(*[np.arange(d) for d in dims], indexing="ij")]
+    vals = A.ravel(order="F")
+    U0 = [np.linalg.qr(rng.standard_normal((d, r)))[0] for d in dims]
+    Us, Gc, res = S.hooi(dims, idx, vals, U0, 25)
+    assert res[-1] < 1e-6
+    for n in range(3):
+        assert D.sin_max_angle(Us[n], Ut[n]) < 1e-6
+
+
+def test_zipf_cdf_and_merge_small():
+    """CDF: monotone, last entry 1, P(0) = 1/H; merged COO: unique keys, sums preserved."""
+    from oracle import sparse as S
+    c = S.zipf_cdf(1000, 1.1)
+    assert np.all(np.diff(c) > 0) and c[-1] == 1.0
+    assert np.isclose(c[0], 1.0 / np.sum(np.arange(1, 1001, dtype=float) ** -1.1))
+    dims = (50, 40, 30)
+    idx, vals = S.zipf_draws(dims, 5000, 1.1, 9)
+    uidx, uval = S.merge_duplicates(dims, idx, vals)
+    keys = set(zip(*[u.tolist() for u in uidx]))
+    assert len(keys) == len(uval) < 5000
+    assert np.isclose(float(np.sum(uval, dtype=np.float64)), float(np.sum(vals, dtype=np.float64)), rtol=1e-6)
