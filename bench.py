@@ -208,6 +208,56 @@ def mixed_c3(ctx, dt, shape=(1024,) * 3, ranks=(64,) * 3, noise=1e-3, decay=0.93
     return out
 
 
+def sparse_c4(ctx, dt, dims=(1 << 20, 1 << 18, 1 << 16), ndraws=100_000_000, seed=20000905, sweeps=10):
+    """Sparse HOOI on the 2^20 x 2^18 x 2^16 COO tensor from 1e8 Zipf(1.1)
+    draws per mode (duplicates merged), ranks (16,16,16), 10 sweeps.  The
+    TTMc kernel of each mode is timed with HIP events inside the sweeps; its
+    algorithmic bytes are nnz (4N + 4) + I_n prod_{m!=n} R_m 4 + sum_{m!=n} I_m R_m 4
+    (read the mode-sorted nonzeros, write Y_(n), read the factors once)."""
+    log(f"[bench] sparse: generating {dims}, {ndraws:.0e} Zipf(1.1) draws")
+    ctx.sync()
+    t0 = time.perf_counter()
+    idx, val, nnz = dt.sp_gen_zipf(ctx, dims, ndraws, 1.1, seed)
+    ctx.sync()
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sp = dt.SpTensor(ctx, dims, idx, val, nnz)
+    ctx.sync()
+    build_s = time.perf_counter() - t0
+    for a in idx + [val]:
+        a.free()
+    log(f"[bench] sparse: nnz {nnz} (gen {gen_s:.2f} s, sorted copies {build_s:.2f} s)")
+    sp.hooi(max_iter=1, seed=seed, want_factors=False)  # first launches
+    ctx.timing_filter("sp_ttmc_m")
+    ctx.timing(True)
+    ctx.timing_reset()
+    out = sp.hooi(max_iter=sweeps, seed=seed, want_factors=False)
+    kt = ctx.timing_list()
+    ctx.timing(False)
+    ctx.timing_filter("")
+    ctx.timing(True)
+    ctx.timing_reset()
+    sp.hooi(max_iter=1, seed=seed, want_factors=False)
+    breakdown = {k: round(v[1], 3) for k, v in sorted(ctx.timing_list().items(), key=lambda kv: -kv[1][1])[:10]}
+    ctx.timing(False)
+    sp.close()
+    modes = {}
+    for n in range(3):
+        cnt, ms = kt.get(f"sp_ttmc_m{n}", (0, 0.0))
+        if not cnt:
+            continue
+        others = [m for m in range(3) if m != n]
+        nbytes = nnz * 16 + dims[n] * 256 * 4 + sum(dims[m] * 16 * 4 for m in others)
+        avg = ms / cnt
+        modes[f"mode{n}"] = {"launches": cnt, "avg_ms": avg, "bytes": nbytes,
+                             "GB_per_s": nbytes / (avg * 1e-3) / 1e9,
+                             "frac_of_8TBs": nbytes / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    return {"workload": "sparse HOOI 2^20 x 2^18 x 2^16, 1e8 Zipf(1.1) draws per mode merged, ranks (16,16,16)",
+            "nnz": nnz, "sweeps": sweeps, "ms_per_sweep": out["sweeps_ms"] / sweeps,
+            "residual": out["residual"], "residuals": out["residuals"], "ttmc": modes,
+            "kernel_ms_one_sweep": breakdown, "generate_s": gen_s, "sorted_copies_s": build_s}
+
+
 def copy_calibration(ctx, dt, nbytes=1 << 30):
     """A plain FP32 copy of known size (the `convert` kernel reads and writes
     nbytes each); profiler counter runs use it to scale FETCH_SIZE/WRITE_SIZE."""
@@ -273,6 +323,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-sparse", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build keeps one rank's block on one "
@@ -285,6 +336,7 @@ def main():
         calib = copy_calibration(ctx, dt)
         log(f"[bench] copy calibration: {calib['GB_per_s']:.1f} GB/s")
         c3 = None if args.no_mixed else mixed_c3(ctx, dt)
+        c4 = None if args.no_sparse else sparse_c4(ctx, dt)
     steps = args.steps
     value = steps / r["wall"]
     passes = {}
@@ -325,7 +377,8 @@ def main():
                    "kernel_ms_per_sweep": r["breakdown"],
                    "full_passes": passes,
                    "copy_calibration": calib,
-                   "time_to_fit_mixed_learned_vs_fp32": c3},
+                   "time_to_fit_mixed_learned_vs_fp32": c3,
+                   "sparse_ttmc": c4},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": measured_traffic(dom_name) if dom_name else None,

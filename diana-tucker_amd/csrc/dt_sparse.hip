@@ -18,18 +18,16 @@
 // non-empty rows.  Sorting uses rocPRIM's LSD radix sort (hipcub), which is
 // stable, so the permutation is exactly the reference's.
 //
-// TTMc (dt_sp_ttmc_dev): Y_c[slot(i), :] = sum over the nonzeros of row i of
+// TTMc (sp_ttmc_dev): Y_c[slot(i), :] = sum over the nonzeros of row i of
 // val * (U_a[i_a, :] ⊗ U_b[i_b, :]) for the two other modes a < b (entry
 // r_a + R_a r_b, the reference's add_outer_product_all order).  Every wave
-// takes chunks of T consecutive nonzeros of the mode-sorted stream; four
-// nonzeros at a time are one v_mfma_f32_16x16x4_f32 (A operand = val *
-// U_a rows, B operand = U_b rows, so D accumulates the 16 x 16 outer-product
-// sum of a row segment).  A row wholly inside a chunk is written directly; a
-// row cut by chunk boundaries leaves FP32 partials in per-chunk carry slots
-// that sp_fixup sums in chunk order (deterministic).  The same pass
-// accumulates L = Y_cᵀ U_n from the segment sums (linear, so partial rows
-// need no fixup): per-wave registers, reduced per workgroup in LDS, then
-// across workgroups in a fixed order in FP64.
+// takes one contiguous range of the mode-sorted stream; four nonzeros at a
+// time are one v_mfma_f32_16x16x4_f32 (A operand = val * U_a rows, B operand
+// = U_b rows, so D accumulates the 16 x 16 outer-product sum of a row
+// segment).  A row wholly inside a range is written directly; a row cut by
+// range boundaries leaves FP32 partials in carry slots that sp_fixup adds in
+// range order (deterministic).  L = Y_cᵀ U_n is a second, dense MFMA pass
+// over Y_c (sp_ytu) with per-workgroup partials summed in a fixed order in FP64.
 #include "dt_internal.hpp"
 #include "dt_philox.hpp"
 
@@ -41,7 +39,6 @@
 
 namespace {
 
-constexpr int SP_T = 2048;   // nonzeros per chunk (multiple of 4)
 constexpr int SP_E = 256;    // Kronecker row length handled per wave (16 x 16)
 
 int bits_for(int64_t dim) {
@@ -179,154 +176,190 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Flush the segment of row r (acc: lane (c, g) holds entries 4g+v + 16c).
-__device__ __forceinline__ void sp_flush(const SpModeArgs &a, int r, int64_t p0, int64_t p1, int first_row, int64_t c,
-                                         floatx4 acc, float *__restrict__ Yc, float *__restrict__ carry,
-                                         int32_t *__restrict__ carry_row, const float *__restrict__ Un,
-                                         float (&Lp)[4][16], int c16, int g) {
-    const int64_t rs = a.rowptr[r], re = a.rowptr[r + 1];
-    const int e0 = 4 * g + 16 * c16;
-    float *dst;
-    if (rs >= p0 && re <= p1) {
-        dst = Yc + (int64_t)a.slot[r] * SP_E;
-    } else {
-        const int s = r == first_row ? 0 : 1;
-        dst = carry + (c * 2 + s) * SP_E;
-        if (c16 == 0 && g == 0) carry_row[c * 2 + s] = r;
+// Every wave owns the contiguous range [P0, P1) of the mode-sorted nonzeros
+// (equal sizes, multiples of 64).  The index stream is loaded two batches of
+// 64 ahead and the compact row number of every nonzero one batch ahead, so
+// a row change needs no dependent load; the factor gathers of a batch are
+// issued together before its 16 MFMAs.  A segment is written to Y_c when its
+// row lies inside the range; otherwise it goes to carry slot 0 (row started
+// before P0) or 1 (row continues past P1), and sp_fixup adds them.
+struct SpBatch {
+    int r, i, j, sl;
+    float v;
+};
+
+__device__ __forceinline__ SpBatch sp_load(const SpModeArgs &a, int64_t p, int64_t P1) {
+    SpBatch z{-1, 0, 0, 0, 0.f};
+    if (p < P1) {
+        z.r = a.row[p];
+        z.i = a.ia[p];
+        z.j = a.ib[p];
+        z.v = a.val[p];
     }
-    *(floatx4 *)(dst + e0) = acc;
-    if (Un) {
-        const floatx4 *u = (const floatx4 *)(Un + (int64_t)r * 16);
+    return z;
+}
+
+// Factor gathers of one batch: lane (c, g) gets, for group t (nonzeros
+// b0 + 4t .. b0 + 4t + 3), val * U_a[i_a, c] and U_b[i_b, c] of nonzero 4t + g.
+__device__ __forceinline__ void sp_gather(const SpBatch &z, const float *__restrict__ Ua,
+                                          const float *__restrict__ Ub, int c16, int g, float (&av)[16],
+                                          float (&bv)[16]) {
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        const int src = (4 * t + g) * 4;
+        const int ri = __builtin_amdgcn_ds_bpermute(src, z.r);
+        const int ii = __builtin_amdgcn_ds_bpermute(src, z.i);
+        const int jj = __builtin_amdgcn_ds_bpermute(src, z.j);
+        const float vv = __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src, __builtin_bit_cast(int, z.v)));
+        // past the range: gather row 0 (always valid) and zero the product
+        const float m = ri >= 0 ? vv : 0.f;
+        av[t] = m * Ua[(int64_t)(ri >= 0 ? ii : 0) * 16 + c16];
+        bv[t] = Ub[(int64_t)(ri >= 0 ? jj : 0) * 16 + c16];
+    }
+}
+
+// Memory operations of a wave complete in issue order for the purpose of
+// waiting on them, so the loop issues, per batch: the gathers of the NEXT
+// batch, then the index loads two batches ahead and the row slots one batch
+// ahead, and only then the MFMAs and row stores of the current batch.
+__global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__restrict__ Ua,
+                                                 const float *__restrict__ Ub, float *__restrict__ Yc,
+                                                 float *__restrict__ carry, int32_t *__restrict__ carry_row,
+                                                 int64_t per_wave, int64_t nwaves) {
+    const int lane = threadIdx.x & 63;
+    const int c16 = lane & 15, g = lane >> 4;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= nwaves) return;
+    const int64_t P0 = w * per_wave;
+    const int64_t P1 = P0 + per_wave < a.nnz ? P0 + per_wave : a.nnz;
+    if (P0 >= P1) return;
+    const int first_row = a.row[P0];
+    const bool left_inc = P0 > 0 && a.row[P0 - 1] == first_row;
+    const int after = P1 < a.nnz ? a.row[P1] : -1;
+    int cur = first_row;
+    int cur_slot = a.slot[first_row];
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int e0 = 4 * g + 16 * c16;
+
+    auto flush = [&](bool last) {
+        const bool linc = cur == first_row && left_inc;
+        const bool rinc = last && after == cur;
+        float *dst;
+        if (!linc && !rinc) {
+            dst = Yc + (int64_t)cur_slot * SP_E;
+        } else {
+            const int s = linc ? 0 : 1;
+            dst = carry + (w * 2 + s) * SP_E;
+            if (lane == 0) carry_row[w * 2 + s] = cur;
+        }
+        *(floatx4 *)(dst + e0) = acc;
+    };
+
+    SpBatch zP = sp_load(a, P0 + lane, P1);       // batch being processed
+    SpBatch zQ = sp_load(a, P0 + 64 + lane, P1);  // next batch
+    float aP[16], bP[16], aQ[16], bQ[16];
+    sp_gather(zP, Ua, Ub, c16, g, aP, bP);
+    if (zP.r >= 0) zP.sl = a.slot[zP.r];
+    for (int64_t b0 = P0; b0 < P1; b0 += 64) {
+        const bool more = b0 + 64 < P1;
+        if (more) sp_gather(zQ, Ua, Ub, c16, g, aQ, bQ);
+        SpBatch zR = sp_load(a, b0 + 128 + lane, P1);
+        if (zQ.r >= 0) zQ.sl = a.slot[zQ.r];
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            if (b0 + 4 * t >= P1) break;  // uniform
+            const int r0 = __builtin_amdgcn_readlane(zP.r, 4 * t);
+            const int r3 = __builtin_amdgcn_readlane(zP.r, 4 * t + 3);
+            if (r0 == cur && r3 == cur) {
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aP[t], bP[t], acc, 0, 0, 0);
+            } else {
+                const int myrow = __builtin_amdgcn_ds_bpermute((4 * t + g) * 4, zP.r);
+#pragma unroll 1
+                for (int gg = 0; gg < 4; gg++) {
+                    const int r = __builtin_amdgcn_readlane(zP.r, 4 * t + gg);
+                    if (r < 0) break;
+                    if (gg > 0 && r == __builtin_amdgcn_readlane(zP.r, 4 * t + gg - 1)) continue;
+                    if (r != cur) {
+                        flush(false);
+                        cur = r;
+                        cur_slot = __builtin_amdgcn_readlane(zP.sl, 4 * t + gg);
+                        acc = floatx4{0.f, 0.f, 0.f, 0.f};
+                    }
+                    const bool mine = myrow == r;
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(mine ? aP[t] : 0.f, mine ? bP[t] : 0.f, acc, 0,
+                                                               0, 0);
+                }
+            }
+        }
+        zP = zQ;
+        zQ = zR;
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            aP[t] = aQ[t];
+            bP[t] = bQ[t];
+        }
+    }
+    flush(true);
+}
+
+// Rows cut by wave ranges: the wave whose range holds the row's first nonzero
+// adds slot 1 of its own range and slot 0 of every later range the row
+// reaches, in range order (four 64-thread groups take every fourth range,
+// then are added in group order).
+__global__ __launch_bounds__(256) void sp_fixup(SpModeArgs a, const float *__restrict__ carry, int64_t per_wave,
+                                                int64_t nwaves, float *__restrict__ Yc) {
+    __shared__ floatx4 part[4][64];
+    const int64_t w = blockIdx.x;
+    const int64_t P0 = w * per_wave;
+    const int64_t P1 = P0 + per_wave < a.nnz ? P0 + per_wave : a.nnz;
+    if (P0 >= P1) return;
+    const int r = a.row[P1 - 1];
+    const int64_t rs = a.rowptr[r], re = a.rowptr[r + 1];
+    if (re <= P1 || rs < P0) return;  // ends inside, or owned by an earlier range
+    const int64_t we = (re - 1) / per_wave;
+    const int grp = threadIdx.x >> 6, t = threadIdx.x & 63;
+    floatx4 s = {0.f, 0.f, 0.f, 0.f};
+    for (int64_t q = w + grp; q <= we; q += 4) {
+        const int sl = q == w ? 1 : 0;
+        s += *(const floatx4 *)(carry + (q * 2 + sl) * SP_E + 4 * t);
+    }
+    part[grp][t] = s;
+    __syncthreads();
+    if (grp == 0) {
+        const floatx4 tot = ((part[0][t] + part[1][t]) + part[2][t]) + part[3][t];
+        *(floatx4 *)(Yc + (int64_t)a.slot[r] * SP_E + 4 * t) = tot;
+    }
+}
+
+// Per-workgroup partial L = Y_cᵀ U_n[rows] (256 x 16) on v_mfma_f32_16x16x4_f32:
+// wave w covers entry tiles 4w..4w+3; k-steps of 4 compact rows.
+__global__ __launch_bounds__(256) void sp_ytu(const float *__restrict__ Yc, const int32_t *__restrict__ rows,
+                                              int64_t nrows, const float *__restrict__ Un, int64_t rows_per_wg,
+                                              float *__restrict__ Lwg) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int64_t s0 = (int64_t)blockIdx.x * rows_per_wg;
+    const int64_t s1 = s0 + rows_per_wg < nrows ? s0 + rows_per_wg : nrows;
+    floatx4 acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[q] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int64_t s = s0; s < s1; s += 4) {
+        const int64_t sg = s + g;
+        const bool ok = sg < s1;
+        const float u = ok ? Un[(int64_t)rows[sg] * 16 + c] : 0.f;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const floatx4 uq = u[q];
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                Lp[v][4 * q + 0] += acc[v] * uq[0];
-                Lp[v][4 * q + 1] += acc[v] * uq[1];
-                Lp[v][4 * q + 2] += acc[v] * uq[2];
-                Lp[v][4 * q + 3] += acc[v] * uq[3];
-            }
+            const int e = 16 * (4 * wave + q) + c;
+            const float y = ok ? Yc[sg * SP_E + e] : 0.f;
+            acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(y, u, acc[q], 0, 0, 0);
         }
     }
-}
-
-// One wave per chunk (persistent over chunks); Ua: I_a x 16, Ub: I_b x 16,
-// Un: I_n x 16 (row-major FP32) or null; Lwg: per-workgroup L partial
-// (256 x 16, entry-major) when Un is given.
-__global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__restrict__ Ua,
-                                                 const float *__restrict__ Ub, const float *__restrict__ Un,
-                                                 float *__restrict__ Yc, float *__restrict__ carry,
-                                                 int32_t *__restrict__ carry_row, float *__restrict__ Lwg,
-                                                 int64_t nchunks) {
-    __shared__ float Ls[SP_E * 16];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int c16 = lane & 15, g = lane >> 4;
-    float Lp[4][16];
+    // D[i = 4g + v][j = c] of tile q: entry 16 (4 wave + q) + 4g + v, column k = c
+    float *L = Lwg + (int64_t)blockIdx.x * SP_E * 16;
 #pragma unroll
-    for (int v = 0; v < 4; v++)
+    for (int q = 0; q < 4; q++)
 #pragma unroll
-        for (int k = 0; k < 16; k++) Lp[v][k] = 0.f;
-    const int64_t nw = (int64_t)gridDim.x * 4;
-    for (int64_t c = (int64_t)blockIdx.x * 4 + wave; c < nchunks; c += nw) {
-        const int64_t p0 = c * SP_T;
-        const int64_t p1 = p0 + SP_T < a.nnz ? p0 + SP_T : a.nnz;
-        const int first_row = a.row[p0];
-        int cur = first_row;
-        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-        for (int64_t b0 = p0; b0 < p1; b0 += 64) {
-            // this lane's nonzero of the batch
-            const int64_t pz = b0 + lane;
-            const bool ok = pz < p1;
-            const int zr = ok ? a.row[pz] : -1;
-            const int zi = ok ? a.ia[pz] : 0;
-            const int zj = ok ? a.ib[pz] : 0;
-            const float zv = ok ? a.val[pz] : 0.f;
-            // gathers for all 16 groups of the batch, then the MFMAs
-            float av[16], bv[16];
-            int rw[16];
-#pragma unroll
-            for (int t = 0; t < 16; t++) {
-                const int src = 4 * t + g;  // lane holding nonzero b0 + 4t + g
-                const int ri = __builtin_amdgcn_ds_bpermute(src * 4, zr);
-                const int ii = __builtin_amdgcn_ds_bpermute(src * 4, zi);
-                const int jj = __builtin_amdgcn_ds_bpermute(src * 4, zj);
-                const float vv = __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src * 4, __builtin_bit_cast(int, zv)));
-                rw[t] = ri;
-                av[t] = ri >= 0 ? vv * Ua[(int64_t)ii * 16 + c16] : 0.f;
-                bv[t] = ri >= 0 ? Ub[(int64_t)jj * 16 + c16] : 0.f;
-            }
-#pragma unroll
-            for (int t = 0; t < 16; t++) {
-                if (b0 + 4 * t >= p1) break;  // uniform
-                // rows of the group's four nonzeros (g = 0..3), -1 past the end
-                const int r0 = __builtin_amdgcn_readlane(rw[t], 0);
-                const int r3 = __builtin_amdgcn_readlane(rw[t], 48);
-                if (r0 == cur && r3 == cur) {
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t], bv[t], acc, 0, 0, 0);
-                } else {
-                    // up to four rows in this group, in order
-#pragma unroll
-                    for (int gg = 0; gg < 4; gg++) {
-                        const int r = __builtin_amdgcn_readlane(rw[t], 16 * gg);
-                        if (r < 0) break;
-                        if (gg > 0 && r == __builtin_amdgcn_readlane(rw[t], 16 * (gg - 1))) continue;
-                        if (r != cur) {
-                            sp_flush(a, cur, p0, p1, first_row, c, acc, Yc, carry, carry_row, Un, Lp, c16, g);
-                            cur = r;
-                            acc = floatx4{0.f, 0.f, 0.f, 0.f};
-                        }
-                        const bool mine = rw[t] == r;
-                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(mine ? av[t] : 0.f, mine ? bv[t] : 0.f, acc, 0,
-                                                                   0, 0);
-                    }
-                }
-            }
-        }
-        sp_flush(a, cur, p0, p1, first_row, c, acc, Yc, carry, carry_row, Un, Lp, c16, g);
-    }
-    if (!Un) return;
-    // L partial of the workgroup: Ls[e * 16 + k], waves added in order
-    for (int w = 0; w < 4; w++) {
-        if (wave == w) {
-#pragma unroll
-            for (int v = 0; v < 4; v++)
-#pragma unroll
-                for (int k = 0; k < 16; k++) {
-                    const int e = 4 * g + v + 16 * c16;
-                    if (w == 0)
-                        Ls[e * 16 + k] = Lp[v][k];
-                    else
-                        Ls[e * 16 + k] += Lp[v][k];
-                }
-        }
-        __syncthreads();
-    }
-    for (int i = threadIdx.x; i < SP_E * 16; i += 256) Lwg[(int64_t)blockIdx.x * SP_E * 16 + i] = Ls[i];
-}
-
-// Rows cut by chunk boundaries: the chunk where row r starts sums the
-// partials of r over its chunks, in chunk order.
-__global__ __launch_bounds__(64) void sp_fixup(SpModeArgs a, const float *__restrict__ carry,
-                                               const int32_t *__restrict__ carry_row, int64_t nchunks,
-                                               float *__restrict__ Yc) {
-    const int64_t c = blockIdx.x;
-    const int64_t p0 = c * SP_T;
-    const int64_t p1 = p0 + SP_T < a.nnz ? p0 + SP_T : a.nnz;
-    const int r = a.row[p1 - 1];  // last row of the chunk
-    const int64_t rs = a.rowptr[r], re = a.rowptr[r + 1];
-    if (re <= p1) return;          // ends inside this chunk: complete, or owned by its first chunk
-    if (rs / SP_T != c) return;    // owned by the chunk where it starts
-    (void)p0;
-    const int64_t ce = (re - 1) / SP_T;
-    floatx4 s = {0.f, 0.f, 0.f, 0.f};
-    for (int64_t q = c; q <= ce; q++) {
-        const int first = a.row[q * SP_T];
-        const int sl = (first == r) ? 0 : 1;
-        const floatx4 v = *(const floatx4 *)(carry + (q * 2 + sl) * SP_E + 4 * threadIdx.x);
-        s += v;
-    }
-    *(floatx4 *)(Yc + (int64_t)a.slot[r] * SP_E + 4 * threadIdx.x) = s;
+        for (int v = 0; v < 4; v++) L[(16 * (4 * wave + q) + 4 * g + v) * 16 + c] = acc[q][v];
 }
 
 // L (256 x 16, FP64) = sum over workgroups of the partials, fixed order.
@@ -660,17 +693,13 @@ extern "C" int dt_sp_mode_arrays(const dt_sp *sp, int n, const int32_t **perm, c
 
 // ---- TTMc driver ---------------------------------------------------------------
 
-// Yc (nrows x 256) and, with Un, L (256 x 16 FP64) for mode n of a 3-way
-// tensor with ranks 16 in the other modes.  Ur[m]: row-major FP32 I_m x 16.
+// Yc (nrows x 256) and, with L, L = Y_cᵀ U_n (256 x 16 FP64) for mode n of a
+// 3-way tensor with ranks 16 in the other modes.  Ur[m]: row-major FP32 I_m x 16.
 static int sp_ttmc_dev(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, float *Yc, double *L) {
     DT_CHECK_ARG(sp->ndim == 3, "sparse TTMc: 3-way tensors only (got %d)", sp->ndim);
     const dt_sp_mode &m = sp->mode[n];
     const int a = n == 0 ? 1 : 0, b = n == 2 ? 1 : 2;
     SpModeArgs args{m.row, m.ia, m.ib, m.val, m.rowptr, m.slot, sp->nnz};
-    const int64_t nchunks = (sp->nnz + SP_T - 1) / SP_T;
-    float *carry = (float *)dt_ws(ctx, WS_SPARSE0, (size_t)nchunks * 2 * SP_E * 4 + (size_t)nchunks * 2 * 4 + 512);
-    if (!carry) return DT_ENOMEM;
-    int32_t *carry_row = (int32_t *)(carry + nchunks * 2 * SP_E);
     static int blocks_per_cu = 0, cus = 0;
     if (!blocks_per_cu) {
         int dev = 0;
@@ -679,16 +708,28 @@ static int sp_ttmc_dev(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, fl
         DT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, sp_ttmc16, 256, 0));
         if (blocks_per_cu < 1) blocks_per_cu = 1;
     }
-    const int64_t nwg = std::min<int64_t>((nchunks + 3) / 4, (int64_t)cus * blocks_per_cu);
-    float *Lwg = nullptr;
+    // equal contiguous ranges, multiples of 64, one per resident wave
+    const int64_t maxw = (int64_t)cus * blocks_per_cu * 4;
+    int64_t per_wave = ((sp->nnz + maxw - 1) / maxw + 63) / 64 * 64;
+    if (per_wave < 64) per_wave = 64;
+    const int64_t nwaves = (sp->nnz + per_wave - 1) / per_wave;
+    float *carry = (float *)dt_ws(ctx, WS_SPARSE0, (size_t)nwaves * 2 * SP_E * 4 + (size_t)nwaves * 2 * 4 + 512);
+    if (!carry) return DT_ENOMEM;
+    int32_t *carry_row = (int32_t *)(carry + nwaves * 2 * SP_E);
+    static const char *names[3] = {"sp_ttmc_m0", "sp_ttmc_m1", "sp_ttmc_m2"};
+    DT_LAUNCH(names[n], sp_ttmc16, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, args, Ur[a], Ur[b], Yc, carry,
+              carry_row, per_wave, nwaves);
+    DT_LAUNCH("sp_fixup", sp_fixup, dim3((unsigned)nwaves), dim3(256), 0, args, carry, per_wave, nwaves, Yc);
     if (L) {
-        Lwg = (float *)dt_ws(ctx, WS_SPARSE1, (size_t)nwg * SP_E * 16 * 4 + 256);
+        const int64_t nwg = std::min<int64_t>(1024, std::max<int64_t>(1, (m.nrows + 255) / 256));
+        int64_t rpw = (m.nrows + nwg - 1) / nwg;
+        rpw = (rpw + 3) / 4 * 4;
+        const int64_t ng = std::max<int64_t>(1, (m.nrows + rpw - 1) / rpw);
+        float *Lwg = (float *)dt_ws(ctx, WS_SPARSE1, (size_t)ng * SP_E * 16 * 4 + 256);
         if (!Lwg) return DT_ENOMEM;
+        DT_LAUNCH("sp_ytu", sp_ytu, dim3((unsigned)ng), dim3(256), 0, Yc, m.rows, m.nrows, Ur[n], rpw, Lwg);
+        DT_LAUNCH("sp_lreduce", sp_lreduce, dim3((SP_E * 16 + 255) / 256), dim3(256), 0, Lwg, (int)ng, L);
     }
-    DT_LAUNCH("sp_ttmc", sp_ttmc16, dim3((unsigned)nwg), dim3(256), 0, args, Ur[a], Ur[b], L ? Ur[n] : nullptr, Yc,
-              carry, carry_row, Lwg, nchunks);
-    DT_LAUNCH("sp_fixup", sp_fixup, dim3((unsigned)nchunks), dim3(64), 0, args, carry, carry_row, nchunks, Yc);
-    if (L) DT_LAUNCH("sp_lreduce", sp_lreduce, dim3((SP_E * 16 + 255) / 256), dim3(256), 0, Lwg, (int)nwg, L);
     return DT_OK;
 }
 
