@@ -403,11 +403,21 @@ __global__ void sp_expand(const float *__restrict__ Yc, const int32_t *__restric
         Y[(int64_t)rows[e / SP_E] * SP_E + e % SP_E] = Yc[e];
 }
 
+// U (I x R column-major) = the first R columns of Ur (I x 16 row-major)
 __global__ void rowmajor_to_colmajor(const double *__restrict__ Ur, int64_t I, int R, double *__restrict__ U) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < I * R; e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = e % I;
         const int r = (int)(e / I);
-        U[e] = Ur[i * R + r];
+        U[e] = Ur[i * 16 + r];
+    }
+}
+
+// row-major FP32 view with 16 columns, columns R..15 zero
+__global__ void rows16_view(const double *__restrict__ U, int64_t I, int R, float *__restrict__ Ur) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < I * 16; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / 16;
+        const int r = (int)(e % 16);
+        Ur[e] = r < R ? (float)U[i + I * r] : 0.f;
     }
 }
 
@@ -751,13 +761,18 @@ extern "C" int dt_sp_ttmc(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur,
 // ---- sparse HOOI -----------------------------------------------------------------
 
 extern "C" int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int max_iter, uint64_t seed,
-                              double *const *U_out, double *G_out, double *residual_per_sweep, double *residual,
-                              double *sweeps_ms) {
+                              const double *const *U_init, double *const *U_out, double *G_out,
+                              double *residual_per_sweep, double *residual, double *sweeps_ms) {
     DT_CHECK_ARG(ctx && sp && ranks && residual, "dt_hooi_sparse: null argument");
     DT_CHECK_ARG(sp->ndim == 3, "dt_hooi_sparse: 3-way tensors only");
-    for (int m = 0; m < 3; m++) DT_CHECK_ARG(ranks[m] == 16, "dt_hooi_sparse: ranks must be 16 (got %zu)", ranks[m]);
+    int Rk[3];
+    for (int m = 0; m < 3; m++) {
+        DT_CHECK_ARG(ranks[m] >= 1 && ranks[m] <= 16 && (int64_t)ranks[m] <= sp->dims[m],
+                     "dt_hooi_sparse: rank %zu of mode %d not in [1, min(16, I)]", ranks[m], m);
+        Rk[m] = (int)ranks[m];
+    }
     DT_CHECK_ARG(max_iter >= 1, "dt_hooi_sparse: max_iter >= 1");
-    const int R = 16;
+    const int R = 16;  // padded width of the device views
     // factors: FP64 I x R column-major (orthonormalised), FP32 row-major views
     std::vector<void *> mine;
     auto alloc = [&](size_t bytes) -> void * {
@@ -778,7 +793,7 @@ extern "C" int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int m
         maxrows = std::max(maxrows, sp->mode[m].nrows);
     }
     for (int m = 0; m < 3; m++) {
-        U64[m] = (double *)alloc((size_t)sp->dims[m] * R * 8);
+        U64[m] = (double *)alloc((size_t)sp->dims[m] * Rk[m] * 8);
         U32r[m] = (float *)alloc((size_t)sp->dims[m] * R * 4);
         if (!U64[m] || !U32r[m]) {
             cleanup();
@@ -795,12 +810,21 @@ extern "C" int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int m
     }
     int st = DT_OK;
     auto views = [&](int m) {
-        return dt_f64_to_views(ctx, U64[m], (size_t)sp->dims[m], (size_t)R, nullptr, U32r[m], nullptr, nullptr);
+        hipLaunchKernelGGL(rows16_view, sp_grid(sp->dims[m] * 16), dim3(256), 0, ctx->stream, U64[m], sp->dims[m],
+                           Rk[m], U32r[m]);
+        return hipGetLastError() == hipSuccess ? DT_OK : DT_EHIP;
     };
-    // random orthonormal start (Philox stream 200 + m)
+    // start: the caller's factors, or random orthonormal (Philox stream 200 + m);
+    // QR with positive R diagonal, the basis orth2's Gram-Schmidt gives
     for (int m = 0; m < 3 && st == DT_OK; m++) {
-        st = dt_fill_normal_f64(ctx, U64[m], (size_t)sp->dims[m] * R, seed, 200 + (uint64_t)m);
-        if (st == DT_OK) st = dt_qr_f64(ctx, U64[m], nullptr, U64[m], (size_t)sp->dims[m], (size_t)R);
+        const size_t sz = (size_t)sp->dims[m] * Rk[m];
+        if (U_init && U_init[m]) {
+            if (hipMemcpyAsync(U64[m], U_init[m], sz * 8, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+                st = DT_EHIP;
+        } else {
+            st = dt_fill_normal_f64(ctx, U64[m], sz, seed, 200 + (uint64_t)m);
+        }
+        if (st == DT_OK) st = dt_qr_f64(ctx, U64[m], nullptr, U64[m], (size_t)sp->dims[m], (size_t)Rk[m]);
         if (st == DT_OK) st = views(m);
     }
     hipEvent_t e0 = dt_get_event(ctx), e1 = dt_get_event(ctx);
@@ -816,11 +840,11 @@ extern "C" int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int m
             if (st == DT_OK) {
                 hipLaunchKernelGGL(sp_times_l, dim3((unsigned)std::min<int64_t>((md.nrows + 3) / 4, 4096)), dim3(256),
                                    0, ctx->stream, Yc, md.rows, md.nrows, L, Urow64);
-                hipLaunchKernelGGL(rowmajor_to_colmajor, sp_grid(I * R), dim3(256), 0, ctx->stream, Urow64, I, R,
-                                   U64[n]);
+                hipLaunchKernelGGL(rowmajor_to_colmajor, sp_grid(I * Rk[n]), dim3(256), 0, ctx->stream, Urow64, I,
+                                   Rk[n], U64[n]);
                 st = hipGetLastError() == hipSuccess ? DT_OK : DT_EHIP;
             }
-            if (st == DT_OK) st = dt_qr_f64(ctx, U64[n], nullptr, U64[n], (size_t)I, (size_t)R);  // orth(S U_n)
+            if (st == DT_OK) st = dt_qr_f64(ctx, U64[n], nullptr, U64[n], (size_t)I, (size_t)Rk[n]);  // orth(S U_n)
             if (st == DT_OK) st = views(n);
         }
         // core = Y_(2)ᵀ U_2 with the updated U_2 (Y_c of the last mode is current)
@@ -854,20 +878,23 @@ extern "C" int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int m
         if (sweeps_ms) *sweeps_ms = total_ms;
         for (int m = 0; m < 3; m++)
             if (U_out && U_out[m])
-                if (hipMemcpyAsync(U_out[m], U64[m], (size_t)sp->dims[m] * R * 8, hipMemcpyDeviceToDevice,
+                if (hipMemcpyAsync(U_out[m], U64[m], (size_t)sp->dims[m] * Rk[m] * 8, hipMemcpyDeviceToDevice,
                                    ctx->stream) != hipSuccess)
                     st = DT_EHIP;
-        // G (16 x 16 x 16, column-major): G[r0 + 16 r1 + 256 r2] = L[(r0 + 16 r1) * 16 + r2]
+        // G (R0 x R1 x R2, column-major): G[r0 + R0 r1 + R0 R1 r2] = L[(r0 + 16 r1) * 16 + r2]
         if (G_out && st == DT_OK) {
-            std::vector<double> g(SP_E * 16), gc(SP_E * 16);
+            std::vector<double> g(SP_E * 16), gc((size_t)Rk[0] * Rk[1] * Rk[2]);
             if (hipMemcpyAsync(g.data(), G64, g.size() * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
                 hipStreamSynchronize(ctx->stream) != hipSuccess)
                 st = DT_EHIP;
-            for (int e = 0; e < SP_E; e++)
-                for (int k = 0; k < 16; k++) gc[(size_t)(e + SP_E * k)] = g[(size_t)(e * 16 + k)];
+            for (int r2 = 0; r2 < Rk[2]; r2++)
+                for (int r1 = 0; r1 < Rk[1]; r1++)
+                    for (int r0 = 0; r0 < Rk[0]; r0++)
+                        gc[(size_t)(r0 + Rk[0] * (r1 + Rk[1] * r2))] = g[(size_t)((r0 + 16 * r1) * 16 + r2)];
             if (st == DT_OK &&
                 hipMemcpyAsync(G_out, gc.data(), gc.size() * 8, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
                 st = DT_EHIP;
+            if (st == DT_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) st = DT_EHIP;
         }
     }
     cleanup();

@@ -11,6 +11,7 @@
 #include <type_traits>
 #include <vector>
 #include "function.hpp"
+#include "sp_tensor.hpp"
 
 namespace diana {
 // Knobs the reference does not have; defaults reproduce its behaviour on the GPU.
@@ -123,5 +124,78 @@ std::tuple<Tensor<Ty>, std::vector<Tensor<Ty>>> HOOI_ALS(const Tensor<Ty> &A, co
     return std::make_tuple(G, U);
 }
 }  // namespace Tucker
+
+namespace SpTucker {
+// Sp_HOOI_ALS (include/algorithm.hpp:16-19, sp_hooi_als.tpp:67-118) on one
+// GPU: the reference's start (per mode, Tensor {R_n, I_n}.randn() after the
+// caller's srand, orthonormalised; the transposed factor U_n = Ut_nᵀ), then
+// per sweep and mode U_n <- orth(Y_(n) Y_(n)ᵀ U_n) and the core from the
+// last mode, with the reference's output lines.  Returns (G, Ut) with Ut[n]
+// stored as {R_n, I_n} like the reference.  Ranks up to 16, 3-way tensors.
+template <typename Ty>
+std::tuple<Tensor<Ty>, std::vector<Tensor<Ty>>> Sp_HOOI_ALS(const SpTensor<Ty> &A, const shape_t &R,
+                                                            size_t max_iter) {
+    const size_t N = A.ndim();
+    if (R.size() != N) error("Sp_HOOI_ALS: |R| != ndim");
+    const shape_t &I = A.shape();
+    dt_ctx *ctx = diana::Device::ctx();
+    double a2 = 0.0;
+    {
+        int sm = 0;
+        int64_t nnz = 0;
+        double nrm = 0.0;
+        dt_sp_info(A.device(), &sm, &nnz, &nrm);
+        a2 = nrm * nrm;
+    }
+    output("||A||_F = " + std::to_string(std::sqrt(a2)));
+    auto dglobal = new DistributionGlobal();
+    std::vector<diana::DevBuf *> dU0, dU;
+    std::vector<const double *> u0p;
+    std::vector<double *> up;
+    for (size_t n = 0; n < N; n++) {
+        Tensor<double> Ut(dglobal, {R[n], I[n]}, false);
+        Ut.randn();
+        std::vector<double> U(I[n] * R[n]);
+        for (size_t i = 0; i < I[n]; i++)
+            for (size_t r = 0; r < R[n]; r++) U[i + I[n] * r] = Ut[r + R[n] * i];
+        dU0.push_back(new diana::DevBuf(U.data(), U.size() * 8));
+        dU.push_back(new diana::DevBuf(U.size() * 8));
+        u0p.push_back((const double *)dU0.back()->p);
+        up.push_back((double *)dU.back()->p);
+    }
+    size_t csz = 1;
+    for (auto r : R) csz *= r;
+    diana::DevBuf dG(csz * 8);
+    std::vector<double> per(max_iter ? max_iter : 1);
+    double res = 0.0, ms = 0.0;
+    for (size_t it = 0; it < max_iter; it++) output("Calculating iteration " + std::to_string(it + 1) + " ...");
+    diana::check(dt_hooi_sparse(ctx, A.device(), R.data(), (int)max_iter, 0, u0p.data(), up.data(),
+                                (double *)dG.p, per.data(), &res, &ms),
+                 "Sp_HOOI_ALS");
+    std::vector<Tensor<Ty>> Ut;
+    std::vector<double> tmp;
+    for (size_t n = 0; n < N; n++) {
+        tmp.resize(I[n] * R[n]);
+        dU[n]->to_host(tmp.data(), tmp.size() * 8);
+        Tensor<Ty> ut(dglobal, {R[n], I[n]}, false);
+        for (size_t i = 0; i < I[n]; i++)
+            for (size_t r = 0; r < R[n]; r++) ut[r + R[n] * i] = (Ty)tmp[i + I[n] * r];
+        Ut.push_back(ut);
+        delete dU0[n];
+        delete dU[n];
+    }
+    Tensor<Ty> G(dglobal, R, false);
+    tmp.resize(csz);
+    dG.to_host(tmp.data(), csz * 8);
+    double g2 = 0.0;
+    for (size_t i = 0; i < csz; i++) {
+        G[i] = (Ty)tmp[i];
+        g2 += tmp[i] * tmp[i];
+    }
+    output("||G||_F = " + std::to_string(std::sqrt(g2)));
+    output("Residual: sqrt(1 - ||G||_F^2 / ||A||_F^2) = " + std::to_string(std::sqrt(std::max(0.0, 1 - g2 / a2))));
+    return std::make_tuple(G, Ut);
+}
+}  // namespace SpTucker
 }  // namespace Algorithm
 #endif

@@ -2,16 +2,25 @@
 // status codes onto the reference's error() convention.
 #ifndef DIANA_AMD_DEVICE_HPP
 #define DIANA_AMD_DEVICE_HPP
+#include <cstdlib>
 #include <string>
 #include "diana_hip.h"
 #include "logger.hpp"
 
 namespace diana {
 struct Device {
+    // The HIP runtime's initialisation draws from the C library's rand()
+    // stream.  The reference seeds that stream (srand in main.cpp:16,
+    // spmain.cpp:17) and draws its start factors from it later, so the
+    // context is created on a private generator state and the caller's state
+    // is put back untouched.
     static dt_ctx *ctx() {
         static dt_ctx *c = [] {
+            static char priv[256];
+            char *caller = initstate(1u, priv, sizeof(priv));
             dt_ctx *p = nullptr;
             int st = dt_create(&p, 0);
+            setstate(caller);
             if (st != DT_OK) error("dt_create failed (no HIP device?), status " + std::to_string(st));
             return p;
         }();

@@ -120,8 +120,8 @@ def lib():
             "dt_sp_info": [vp, C.POINTER(i32), C.POINTER(C.c_int64), C.POINTER(dbl)],
             "dt_sp_mode_arrays": [vp, i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_int64)],
             "dt_sp_ttmc": [vp, vp, i32, C.POINTER(vp), C.POINTER(sz), vp],
-            "dt_hooi_sparse": [vp, vp, C.POINTER(sz), i32, u64, C.POINTER(vp), vp, C.POINTER(dbl),
-                               C.POINTER(dbl), C.POINTER(dbl)],
+            "dt_hooi_sparse": [vp, vp, C.POINTER(sz), i32, u64, C.POINTER(vp), C.POINTER(vp), vp,
+                               C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)],
         }
         for name, argtypes in sig.items():
             f = getattr(L, name)
@@ -632,14 +632,16 @@ class SpTensor:
         self.ctx.check(self.ctx.L.dt_sp_ttmc(self.ctx.handle, self.handle, n, ptrs, rk, Y.ptr))
         return Y
 
-    def hooi(self, max_iter=10, seed=0, want_factors=True):
-        U = [self.ctx.array((d, 16), DT_F64) for d in self.dims] if want_factors else []
-        G = self.ctx.array((16, 16, 16), DT_F64) if want_factors else None
+    def hooi(self, max_iter=10, seed=0, want_factors=True, ranks=(16, 16, 16), U_init=None):
+        U = [self.ctx.array((d, r), DT_F64) for d, r in zip(self.dims, ranks)] if want_factors else []
+        G = self.ctx.array(tuple(ranks), DT_F64) if want_factors else None
         up = (C.c_void_p * 3)(*[u.ptr for u in U]) if U else None
+        dinit = [self.ctx.to_device(np.asarray(u, dtype=np.float64)) for u in U_init] if U_init is not None else None
+        ip = (C.c_void_p * 3)(*[u.ptr for u in dinit]) if dinit else None
         per = (C.c_double * max_iter)()
         res, ms = C.c_double(), C.c_double()
-        rk = (C.c_size_t * 3)(16, 16, 16)
-        self.ctx.check(self.ctx.L.dt_hooi_sparse(self.ctx.handle, self.handle, rk, max_iter, seed, up,
+        rk = (C.c_size_t * 3)(*ranks)
+        self.ctx.check(self.ctx.L.dt_hooi_sparse(self.ctx.handle, self.handle, rk, max_iter, seed, ip, up,
                                                   G.ptr if G else None, per, C.byref(res), C.byref(ms)))
         return {"U": [u.download() for u in U], "G": G.download() if G else None,
                 "residuals": list(per), "residual": res.value, "sweeps_ms": ms.value}

@@ -204,13 +204,15 @@ int dt_sp_mode_arrays(const dt_sp *sp, int n, const int32_t **perm, const int32_
  * Replaces the Kronecker accumulation of ttmc_mTR / ttmc_mTL
  * (sp_tensor.tpp:236-365, add_outer_product_all :367-417); ranks 16 only. */
 int dt_sp_ttmc(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, const size_t *ranks, float *Y);
-/* Sparse HOOI: ranks (16,16,16), max_iter sweeps from the Philox random start
- * (streams 200 + n).  U_out[n] (shape[n] x 16 FP64 column-major) and G_out
- * (16^3 FP64 column-major) may be NULL; residual_per_sweep holds max_iter
- * entries (or NULL); sweeps_ms = device time of the sweeps. */
+/* Sparse HOOI: ranks R_n <= 16, max_iter sweeps.  Start: U_init[n] (device
+ * FP64 shape[n] x R_n column-major, orthonormalised here), or, when U_init is
+ * NULL, Philox normals (streams 200 + n).  U_out[n] (shape[n] x R_n FP64
+ * column-major) and G_out (R_0 x R_1 x R_2 FP64 column-major) may be NULL;
+ * residual_per_sweep holds max_iter entries (or NULL); sweeps_ms = device
+ * time of the sweeps. */
 int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int max_iter, uint64_t seed,
-                   double *const *U_out, double *G_out, double *residual_per_sweep,
-                   double *residual, double *sweeps_ms);
+                   const double *const *U_init, double *const *U_out, double *G_out,
+                   double *residual_per_sweep, double *residual, double *sweeps_ms);
 
 /* ---- learned start ------------------------------------------------------
  * No counterpart in the reference (it starts from random factors,

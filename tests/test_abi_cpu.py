@@ -39,3 +39,17 @@ def test_binding_covers_declared_symbols():
     import dtucker
     names = set(declared_functions())
     assert set(dtucker.exported_symbols()) <= names
+
+
+def test_tns_reader_fixture():
+    """read_data rules (src/read_data.cpp:13-33, 80-127): space-separated,
+    1-based indices, ndim = spaces on the first line, dims = max index + 1."""
+    import subprocess
+    exe = os.path.join(ROOT, "diana-tucker_amd", "host", "bin", "tns_dump")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "diana-tucker_amd", "host")], check=True)
+    out = subprocess.run([exe, os.path.join(ROOT, "tests", "golden", "tiny.tns")], check=True,
+                         capture_output=True, text=True).stdout.split("\n")
+    assert out[0] == "3 4"
+    assert out[1] == "dims 4 3 3"
+    assert out[2:6] == ["0 0 0 1.5", "1 2 0 0.25", "3 0 1 2", "2 2 2 10.125"]

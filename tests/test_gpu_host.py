@@ -25,11 +25,58 @@ def test_reference_function_tests_through_host_layer():
 
 
 def test_main_driver_prints_reference_lines():
-    p = run("main", 60, 50, 40, 8)
+    """main.cpp's workload: srand(20000905), the tensor from Tensor::randn, then
+    HOOI_ALS's own random start from the same rand() stream.  The printed
+    residuals equal the FP64 oracle's from the identical tensor and start
+    (glibc rand() and Util::randn restated)."""
+    import sys
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from oracle import dense as D
+    from oracle.glibc_rand import GlibcRand, randn_fill
+    shape, R = (60, 50, 40), 8
+    p = run("main", *shape, R)
     assert p.returncode == 0, p.stderr
     lines = p.stderr.splitlines()
     assert lines[0].startswith("Start Tucker::HOOI_ALS decomposition.. with max_iter = 5")
     res = [float(l.split("=")[-1]) for l in lines if l.startswith("Residual")]
     assert len(res) == 5
-    # a Gaussian tensor is far from low rank: the residual stays large and decreases
-    assert 0.5 < res[-1] < 1.0 and res[-1] <= res[0] + 1e-9
+    g = GlibcRand(20000905)
+    A = np.array(randn_fill(g, int(np.prod(shape)))).reshape(shape, order="F")
+    U0 = [D.reduced_qr(np.array(randn_fill(g, I * R)).reshape((I, R), order="F"))[0] for I in shape]
+    _, _, rr = D.hooi_als(A, (R,) * 3, 5, U0)
+    print("main residuals", res, "oracle", rr, flush=True)
+    assert np.allclose(res, rr, atol=2e-6)  # printed with 6 decimals
+
+
+def test_spmain_driver_matches_oracle_from_the_reference_start(tmp_path):
+    """spmain.cpp's workload through the host layer: read a .tns file, build the
+    SpTensor, Sp_HOOI_ALS with truncation 5 from the reference's start (srand(1),
+    Tensor {R, I}.randn() per mode); the printed residual equals the sparse
+    oracle's from the same start (glibc rand() restated)."""
+    import sys
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from oracle import dense as D
+    from oracle import sparse as S
+    from oracle.glibc_rand import GlibcRand, randn_fill
+    dims, R, sweeps = (60, 50, 40), 5, 2
+    idx, vals = S.gen_zipf(dims, 6000, 1.1, 4)
+    vals = np.round(vals.astype(np.float64), 6)  # what the text file holds
+    dims = tuple(int(i.max()) + 1 for i in idx)  # read_data: dims = largest index + 1
+    path = tmp_path / "small.tns"
+    with open(path, "w") as f:
+        for k in range(len(vals)):
+            f.write(" ".join(str(int(idx[n][k]) + 1) for n in range(3)) + f" {vals[k]:.6f}\n")
+    p = run("spmain", path, 1, R, sweeps)
+    assert p.returncode == 0, p.stderr
+    res = [float(l.split("=")[-1]) for l in p.stderr.splitlines() if l.startswith("Residual")]
+    assert len(res) == 1
+    g = GlibcRand(1)
+    U0 = []
+    for I in dims:
+        ut = np.array(randn_fill(g, R * I)).reshape((R, I), order="F")  # Tensor {R, I}
+        U0.append(D.orth_posdiag(ut.T))
+    _, _, rr = S.hooi(dims, idx, vals.astype(np.float32), U0, sweeps)
+    print("spmain residual", res[0], "oracle", rr[-1], flush=True)
+    assert abs(res[0] - rr[-1]) < 5e-6   # printed with 6 decimals; FP32 TTMc

@@ -113,13 +113,33 @@ def test_sparse_hooi_vs_oracle(ctx):
     assert np.isclose(np.linalg.norm(G), np.linalg.norm(Gr), rtol=1e-5)
 
 
+def test_sparse_hooi_ranks_below_16_from_given_start(ctx):
+    """Ranks (10, 7, 12) (the reference's default truncation is 10) from the
+    caller's start factors, against the oracle."""
+    rng = np.random.default_rng(12)
+    dims, ranks = (300, 250, 200), (10, 7, 12)
+    idx, vals = make_coo(rng, dims, 40000)
+    U0 = [D.orth_posdiag(rng.standard_normal((d, r))) for d, r in zip(dims, ranks)]
+    sp = dt.SpTensor.from_host(ctx, dims, idx, vals)
+    try:
+        out = sp.hooi(max_iter=3, ranks=ranks, U_init=U0)
+    finally:
+        sp.close()
+    Ur, Gr, res = S.hooi(dims, idx, vals, U0, 3)
+    print("ranks", ranks, "residuals", out["residuals"], res, flush=True)
+    assert np.allclose(out["residuals"], res, rtol=1e-5)
+    assert out["G"].shape == ranks
+    for n in range(3):
+        assert D.sin_max_angle(out["U"][n], Ur[n]) < 1e-3
+
+
 def test_sparse_bad_ranks_einval(ctx):
     idx, vals = make_coo(np.random.default_rng(1), (40, 30, 20), 500)
     sp = dt.SpTensor.from_host(ctx, (40, 30, 20), idx, vals)
     try:
         with pytest.raises(dt.DTError) as e:
-            ctx.check(ctx.L.dt_hooi_sparse(ctx.handle, sp.handle, (dt.C.c_size_t * 3)(8, 8, 8), 1, 0, None, None,
-                                           None, dt.C.byref(dt.C.c_double()), None))
+            ctx.check(ctx.L.dt_hooi_sparse(ctx.handle, sp.handle, (dt.C.c_size_t * 3)(17, 8, 8), 1, 0, None, None,
+                                           None, None, dt.C.byref(dt.C.c_double()), None))
         assert e.value.code == dt.DT_EINVAL
     finally:
         sp.close()
