@@ -313,6 +313,116 @@ __global__ void set_identity(double *__restrict__ A, int n) {
         A[idx] = (idx % n == idx / n) ? 1.0 : 0.0;
 }
 
+// ---- tall-skinny pieces of Cholesky-QR (n <= TS_NMAX, m large) ---------------
+// Partial Grams: workgroup w sums Q[r, i] Q[r, j] in FP64 over its contiguous
+// row range (rows staged TS_ROWS at a time through LDS with column-coalesced
+// loads) into part[w][i + n j]; thread t owns entries t, t + 256, ... < n^2.
+constexpr int TS_NMAX = 32, TS_ROWS = 128;
+__global__ __launch_bounds__(256) void ts_gram_kernel(const double *__restrict__ Q, int64_t m, int n,
+                                                      int64_t rows_per_wg, double *__restrict__ part) {
+    __shared__ double Qs[TS_ROWS][TS_NMAX + 1];
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_wg;
+    const int64_t r1 = r0 + rows_per_wg < m ? r0 + rows_per_wg : m;
+    const int nn = n * n;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    int ei[4], ej[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int e = threadIdx.x + 256 * q;
+        ei[q] = e % n;
+        ej[q] = e / n;
+    }
+    for (int64_t rb = r0; rb < r1; rb += TS_ROWS) {
+        const int nr = rb + TS_ROWS < r1 ? TS_ROWS : (int)(r1 - rb);
+        __syncthreads();
+        for (int e = threadIdx.x; e < TS_ROWS * n; e += 256) {
+            const int r = e % TS_ROWS, c = e / TS_ROWS;
+            Qs[r][c] = r < nr ? Q[rb + r + m * c] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int r = 0; r < TS_ROWS; r++) {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if ((int)threadIdx.x + 256 * q < nn) acc[q] = fma(Qs[r][ei[q]], Qs[r][ej[q]], acc[q]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int e = threadIdx.x + 256 * q;
+        if (e < nn) part[(int64_t)blockIdx.x * nn + e] = acc[q];
+    }
+}
+
+// out[e] = sum over p of part[p][e], e < ne: 16 entries per workgroup, 16
+// groups of threads each summing every 16th partial, combined in group order.
+__global__ __launch_bounds__(256) void part_reduce_kernel(const double *__restrict__ part, int nparts, int ne,
+                                                          double *__restrict__ out) {
+    __shared__ double red[16][17];
+    const int c = threadIdx.x & 15, grp = threadIdx.x >> 4;
+    const int e = blockIdx.x * 16 + c;
+    double s = 0.0;
+    if (e < ne)
+        for (int p = grp; p < nparts; p += 16) s += part[(int64_t)p * ne + e];
+    red[grp][c] = s;
+    __syncthreads();
+    if (grp == 0 && e < ne) {
+        double t = 0.0;
+        for (int g = 0; g < 16; g++) t += red[g][c];
+        out[e] = t;
+    }
+}
+
+// Q[r, :] <- Q[r, :] Rinv in place (Rinv upper triangular n x n, column-major),
+// one row per thread.
+template <int NMAX>
+__global__ __launch_bounds__(256) void ts_rmul_kernel(double *__restrict__ Q, int64_t m, int n,
+                                                      const double *__restrict__ Rinv) {
+    __shared__ double Rs[NMAX][NMAX + 1];  // Rs[k][j] = Rinv[k + n j]
+    for (int e = threadIdx.x; e < NMAX * NMAX; e += 256) {
+        const int k = e % NMAX, j = e / NMAX;
+        Rs[k][j] = (k < n && j < n && k <= j) ? Rinv[k + n * j] : 0.0;
+    }
+    __syncthreads();
+    for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < m; r += (int64_t)gridDim.x * 256) {
+        double q[NMAX];
+#pragma unroll
+        for (int c = 0; c < NMAX; c++) q[c] = c < n ? Q[r + m * c] : 0.0;
+#pragma unroll
+        for (int j = 0; j < NMAX; j++) {
+            if (j < n) {
+                double s = 0.0;
+#pragma unroll
+                for (int k = 0; k <= j; k++) s = fma(q[k], Rs[k][j], s);
+                Q[r + m * j] = s;
+            }
+        }
+    }
+}
+
+// G = QᵀQ (n x n) for a tall-skinny column-major Q, FP64, fixed summation order.
+static int ts_gram(dt_ctx *ctx, const double *Q, size_t m, size_t n, double *G) {
+    const int64_t nwg = std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div((int64_t)m, 1024)));
+    const int64_t rpw = ceil_div((int64_t)m, nwg);
+    const int64_t ng = ceil_div((int64_t)m, rpw);
+    double *part = (double *)dt_ws(ctx, WS_GEMM_PART, (size_t)(ng * n * n) * sizeof(double));
+    if (!part) return DT_ENOMEM;
+    DT_LAUNCH("ts_gram", ts_gram_kernel, dim3((unsigned)ng), dim3(256), 0, Q, (int64_t)m, (int)n, rpw, part);
+    DT_LAUNCH("ts_gram", part_reduce_kernel, dim3((unsigned)ceil_div((int64_t)(n * n), 16)), dim3(256), 0,
+              (const double *)part, (int)ng, (int)(n * n), G);
+    return DT_OK;
+}
+
+static int ts_rmul(dt_ctx *ctx, double *Q, size_t m, size_t n, const double *Rinv) {
+    const dim3 grid((unsigned)std::min<int64_t>(ceil_div((int64_t)m, 256), 8192));
+    if (n <= 16) {
+        DT_LAUNCH("ts_rmul", ts_rmul_kernel<16>, grid, dim3(256), 0, Q, (int64_t)m, (int)n, Rinv);
+    } else {
+        DT_LAUNCH("ts_rmul", ts_rmul_kernel<32>, grid, dim3(256), 0, Q, (int64_t)m, (int)n, Rinv);
+    }
+    return DT_OK;
+}
+
 // Reduced QR through Cholesky-QR2 (a shifted Cholesky-QR first pass when the
 // plain one breaks down).  Q may alias A.
 extern "C" int dt_qr_f64(dt_ctx *ctx, double *Q, double *R, const double *A, size_t m, size_t n) {
@@ -322,7 +432,9 @@ extern "C" int dt_qr_f64(dt_ctx *ctx, double *Q, double *R, const double *A, siz
     double *G = (double *)dt_ws(ctx, WS_SMALL0, nn * sizeof(double));
     double *Rk = (double *)dt_ws(ctx, WS_SMALL1, nn * sizeof(double));
     double *Rki = (double *)dt_ws(ctx, WS_SMALL2, nn * sizeof(double));
-    double *Racc = (double *)dt_ws(ctx, WS_QR_T1, 2 * nn * sizeof(double) + m * n * sizeof(double));
+    // tall-skinny: streaming Gram and in-place row update instead of the tiled GEMM
+    const bool ts = n <= (size_t)TS_NMAX && m >= 4096;
+    double *Racc = (double *)dt_ws(ctx, WS_QR_T1, 2 * nn * sizeof(double) + (ts ? 0 : m * n * sizeof(double)));
     if (!G || !Rk || !Rki || !Racc) return DT_ENOMEM;
     double *Rtmp = Racc + nn;
     double *T = Rtmp + nn;  // m x n temporary
@@ -331,7 +443,11 @@ extern "C" int dt_qr_f64(dt_ctx *ctx, double *Q, double *R, const double *A, siz
     int passes = 2;
     bool shifted = false;
     for (int pass = 0; pass < passes; pass++) {
-        DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, n, n, m, 1.0, Q, m, Q, m, 0.0, G, n));
+        if (ts) {
+            DT_TRY(ts_gram(ctx, Q, m, n, G));
+        } else {
+            DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, n, n, m, 1.0, Q, m, Q, m, 0.0, G, n));
+        }
         int info = 0;
         DT_TRY(dt_chol_upper(ctx, G, n, Rk, Rki, &info));
         if (info != 0) {
@@ -355,8 +471,12 @@ extern "C" int dt_qr_f64(dt_ctx *ctx, double *Q, double *R, const double *A, siz
             shifted = true;
             passes = 3;
         }
-        DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, m, n, n, 1.0, Q, m, Rki, n, 0.0, T, m));
-        DT_HIP(hipMemcpyAsync(Q, T, m * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        if (ts) {
+            DT_TRY(ts_rmul(ctx, Q, m, n, Rki));
+        } else {
+            DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, m, n, n, 1.0, Q, m, Rki, n, 0.0, T, m));
+            DT_HIP(hipMemcpyAsync(Q, T, m * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        }
         if (R) {
             DT_LAUNCH("qr_small", small_matmul, dim3(16), dim3(256), 0, Rk, Racc, Rtmp, (int)n);
             DT_HIP(hipMemcpyAsync(Racc, Rtmp, nn * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));

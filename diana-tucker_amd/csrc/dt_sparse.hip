@@ -160,15 +160,24 @@ __global__ void gather_by(const T *__restrict__ src, const int32_t *__restrict__
 
 // ---- TTMc --------------------------------------------------------------------
 
+typedef int intx4 __attribute__((ext_vector_type(4)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+typedef int intx2 __attribute__((ext_vector_type(2)));
+
 struct SpModeArgs {
     const int32_t *row;     // sorted mode-n index of each nonzero
-    const int32_t *ia;      // index in mode a (first other mode)
-    const int32_t *ib;      // index in mode b
-    const float *val;
+    const intx4 *rec;       // per sorted nonzero: {index in mode a, index in mode b, compact row, value bits}
     const int32_t *rowptr;  // I_n + 1
     const int32_t *slot;    // compact row number
     int64_t nnz;
 };
+
+__global__ void build_rec(const int32_t *__restrict__ row, const int32_t *__restrict__ ia,
+                          const int32_t *__restrict__ ib, const float *__restrict__ val,
+                          const int32_t *__restrict__ slot, int64_t n, intx4 *__restrict__ rec) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+        rec[p] = intx4{ia[p], ib[p], slot[row[p]], __float_as_int(val[p])};
+}
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -176,129 +185,191 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Every wave owns the contiguous range [P0, P1) of the mode-sorted nonzeros
-// (equal sizes, multiples of 64).  The index stream is loaded two batches of
-// 64 ahead and the compact row number of every nonzero one batch ahead, so
-// a row change needs no dependent load; the factor gathers of a batch are
-// issued together before its 16 MFMAs.  A segment is written to Y_c when its
-// row lies inside the range; otherwise it goes to carry slot 0 (row started
-// before P0) or 1 (row continues past P1), and sp_fixup adds them.
-struct SpBatch {
-    int r, i, j, sl;
-    float v;
+// Per-wave LDS: the staged batch's U_a and U_b rows [nonzero][column] and a
+// ring of three batches' (compact row, value) pairs.
+constexpr int SP_RING = 3;
+struct alignas(16) SpLds {
+    float a[64 * 16];
+    float b[64 * 16];
+    intx2 sv[SP_RING][64];
 };
 
-__device__ __forceinline__ SpBatch sp_load(const SpModeArgs &a, int64_t p, int64_t P1) {
-    SpBatch z{-1, 0, 0, 0, 0.f};
-    if (p < P1) {
-        z.r = a.row[p];
-        z.i = a.ia[p];
-        z.j = a.ib[p];
-        z.v = a.val[p];
-    }
-    return z;
+// Record of nonzero p of the range; past P1 the last record of the range is
+// loaded (its indices are valid gather addresses) and the (row, value) pair
+// is replaced by (-1, 0) where it is staged.  The load is issued for every
+// lane and its result is not touched until it is needed, so no wait for it is
+// forced early.
+__device__ __forceinline__ intx4 sp_rec_load(const intx4 *__restrict__ rec, int64_t p, int64_t P1) {
+    return __builtin_nontemporal_load(rec + (p < P1 ? p : P1 - 1));
 }
 
-// Factor gathers of one batch: lane (c, g) gets, for group t (nonzeros
-// b0 + 4t .. b0 + 4t + 3), val * U_a[i_a, c] and U_b[i_b, c] of nonzero 4t + g.
-__device__ __forceinline__ void sp_gather(const SpBatch &z, const float *__restrict__ Ua,
-                                          const float *__restrict__ Ub, int c16, int g, float (&av)[16],
-                                          float (&bv)[16]) {
+// Gathers of one batch: instruction s, lane (p4, c4) loads columns 4c4..4c4+3
+// of U_a[i_a] and U_b[i_b] of nonzero 16 s + p4 (whole 64-byte rows, four lanes
+// each).
+__device__ __forceinline__ void sp_issue(const intx4 &z, const float *__restrict__ Ua, const float *__restrict__ Ub,
+                                         int p4, int c4, floatx4 (&ga)[4], floatx4 (&gb)[4]) {
 #pragma unroll
-    for (int t = 0; t < 16; t++) {
-        const int src = (4 * t + g) * 4;
-        const int ri = __builtin_amdgcn_ds_bpermute(src, z.r);
-        const int ii = __builtin_amdgcn_ds_bpermute(src, z.i);
-        const int jj = __builtin_amdgcn_ds_bpermute(src, z.j);
-        const float vv = __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src, __builtin_bit_cast(int, z.v)));
-        // past the range: gather row 0 (always valid) and zero the product
-        const float m = ri >= 0 ? vv : 0.f;
-        av[t] = m * Ua[(int64_t)(ri >= 0 ? ii : 0) * 16 + c16];
-        bv[t] = Ub[(int64_t)(ri >= 0 ? jj : 0) * 16 + c16];
+    for (int s = 0; s < 4; s++) {
+        const int src = (16 * s + p4) * 4;
+        const int ia = __builtin_amdgcn_ds_bpermute(src, z.x);
+        const int ib = __builtin_amdgcn_ds_bpermute(src, z.y);
+        ga[s] = *(const floatx4 *)(Ua + (int64_t)ia * 16 + 4 * c4);
+        gb[s] = *(const floatx4 *)(Ub + (int64_t)ib * 16 + 4 * c4);
     }
 }
 
-// Memory operations of a wave complete in issue order for the purpose of
-// waiting on them, so the loop issues, per batch: the gathers of the NEXT
-// batch, then the index loads two batches ahead and the row slots one batch
-// ahead, and only then the MFMAs and row stores of the current batch.
+// Every wave owns the contiguous range [P0, P1) of the mode-sorted records
+// (equal sizes, multiples of 64) and walks it in batches of 64 nonzeros.  For
+// batch k it stages the gathered rows into LDS, issues the gathers of batch
+// k + 2 into the registers just freed and the record load of batch k + 4, then
+// runs batch k from LDS; two register sets alternate (the loop body is
+// unrolled by two, so no register holding a load in flight is ever copied).
+// A batch where at most one row starts runs 16 MFMA steps: step t, lane (c, g)
+// reads U_a[i_a, c] and U_b[i_b, c] of nonzero 4t + g and its (row, value);
+// when the four nonzeros share the current row one MFMA adds
+// val * U_a-row ⊗ U_b-row, otherwise one MFMA per row present with the other
+// lanes zeroed.  A batch where more rows start (the short rows of a Zipf
+// tail) adds one nonzero at a time with VALU FMAs in the same accumulator
+// layout, so a row end never waits for an MFMA.  A finished row is written to Y_c when it lies
+// inside the range; otherwise it goes to carry slot 0 (row started before
+// P0) or 1 (row continues past P1), and sp_fixup adds them.
 __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__restrict__ Ua,
                                                  const float *__restrict__ Ub, float *__restrict__ Yc,
-                                                 float *__restrict__ carry, int32_t *__restrict__ carry_row,
-                                                 int64_t per_wave, int64_t nwaves) {
-    const int lane = threadIdx.x & 63;
-    const int c16 = lane & 15, g = lane >> 4;
-    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+                                                 float *__restrict__ carry, int64_t per_wave, int64_t nwaves) {
+    __shared__ SpLds lds_all[4];
+    // the wave index is uniform: readfirstlane lets the compiler keep the
+    // range bounds and the batch loop scalar
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    SpLds &L = lds_all[wv];
+    const int c = lane & 15, g = lane >> 4;     // MFMA operand layout
+    const int p4 = lane >> 2, c4 = lane & 3;    // gather layout
+    const int64_t w = (int64_t)blockIdx.x * 4 + wv;
     if (w >= nwaves) return;
     const int64_t P0 = w * per_wave;
     const int64_t P1 = P0 + per_wave < a.nnz ? P0 + per_wave : a.nnz;
     if (P0 >= P1) return;
-    const int first_row = a.row[P0];
-    const bool left_inc = P0 > 0 && a.row[P0 - 1] == first_row;
-    const int after = P1 < a.nnz ? a.row[P1] : -1;
+    const intx4 *__restrict__ rec = a.rec;
+    const int first_row = rec[P0].z;
+    const bool left_inc = P0 > 0 && rec[P0 - 1].z == first_row;
+    const int after = P1 < a.nnz ? rec[P1].z : -1;
     int cur = first_row;
-    int cur_slot = a.slot[first_row];
-    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-    const int e0 = 4 * g + 16 * c16;
+    // partial sums of the current row: two MFMA accumulators and one VALU
+    // accumulator, all in the MFMA D layout (lane (c, g), v -> entry 4g + v + 16c)
+    floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f}, vacc = {0.f, 0.f, 0.f, 0.f};
+    const int e0 = 4 * g + 16 * c;
 
     auto flush = [&](bool last) {
         const bool linc = cur == first_row && left_inc;
         const bool rinc = last && after == cur;
-        float *dst;
-        if (!linc && !rinc) {
-            dst = Yc + (int64_t)cur_slot * SP_E;
-        } else {
-            const int s = linc ? 0 : 1;
-            dst = carry + (w * 2 + s) * SP_E;
-            if (lane == 0) carry_row[w * 2 + s] = cur;
-        }
-        *(floatx4 *)(dst + e0) = acc;
+        float *dst = (!linc && !rinc) ? Yc + (int64_t)cur * SP_E : carry + (w * 2 + (linc ? 0 : 1)) * SP_E;
+        *(floatx4 *)(dst + e0) = (acc0 + acc1) + vacc;
     };
 
-    SpBatch zP = sp_load(a, P0 + lane, P1);       // batch being processed
-    SpBatch zQ = sp_load(a, P0 + 64 + lane, P1);  // next batch
-    float aP[16], bP[16], aQ[16], bQ[16];
-    sp_gather(zP, Ua, Ub, c16, g, aP, bP);
-    if (zP.r >= 0) zP.sl = a.slot[zP.r];
-    for (int64_t b0 = P0; b0 < P1; b0 += 64) {
-        const bool more = b0 + 64 < P1;
-        if (more) sp_gather(zQ, Ua, Ub, c16, g, aQ, bQ);
-        SpBatch zR = sp_load(a, b0 + 128 + lane, P1);
-        if (zQ.r >= 0) zQ.sl = a.slot[zQ.r];
-#pragma unroll
+    auto compute = [&](int64_t b0, int ks) {
+        // rows starting in this batch (lane k: nonzero k; padding has row -1)
+        const intx2 svl = L.sv[ks][lane];
+        const int prevr = __builtin_amdgcn_ds_bpermute((lane == 0 ? 0 : lane - 1) * 4, svl.x);
+        const uint64_t heads = __ballot(svl.x >= 0 && svl.x != (lane == 0 ? cur : prevr));
+        if (__popcll(heads) > 1) {
+            // many short rows: one nonzero at a time on the VALU, one segment
+            // per row, no MFMA result to wait for at a row end
+            const int nvalid = __popcll(__ballot(svl.x >= 0));
+            uint64_t hm = heads;
+            int k = 0;
+            for (;;) {
+                const int h = hm ? __builtin_ctzll(hm) : nvalid;
+                for (int kk = k; kk < h; kk++) {
+                    const float vv = __int_as_float(__builtin_amdgcn_readlane(svl.y, kk));
+                    const floatx4 ua = *(const floatx4 *)(L.a + kk * 16 + 4 * g);
+                    const float wb = vv * L.b[kk * 16 + c];
+                    vacc[0] = fmaf(ua[0], wb, vacc[0]);
+                    vacc[1] = fmaf(ua[1], wb, vacc[1]);
+                    vacc[2] = fmaf(ua[2], wb, vacc[2]);
+                    vacc[3] = fmaf(ua[3], wb, vacc[3]);
+                }
+                if (h >= nvalid) break;
+                flush(false);
+                cur = __builtin_amdgcn_readlane(svl.x, h);
+                acc0 = floatx4{0.f, 0.f, 0.f, 0.f};
+                acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
+                vacc = floatx4{0.f, 0.f, 0.f, 0.f};
+                hm &= hm - 1;
+                k = h;
+            }
+            return;
+        }
+#pragma unroll 2
         for (int t = 0; t < 16; t++) {
-            if (b0 + 4 * t >= P1) break;  // uniform
-            const int r0 = __builtin_amdgcn_readlane(zP.r, 4 * t);
-            const int r3 = __builtin_amdgcn_readlane(zP.r, 4 * t + 3);
+            const float ua = L.a[(4 * t + g) * 16 + c], bv = L.b[(4 * t + g) * 16 + c];
+            const intx2 sv = L.sv[ks][4 * t + g];
+            const float av = __int_as_float(sv.y) * ua;
+            const int r0 = __builtin_amdgcn_readlane(sv.x, 0);
+            const int r3 = __builtin_amdgcn_readlane(sv.x, 48);
             if (r0 == cur && r3 == cur) {
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aP[t], bP[t], acc, 0, 0, 0);
+                if (t & 1)
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc1, 0, 0, 0);
+                else
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc0, 0, 0, 0);
             } else {
-                const int myrow = __builtin_amdgcn_ds_bpermute((4 * t + g) * 4, zP.r);
 #pragma unroll 1
                 for (int gg = 0; gg < 4; gg++) {
-                    const int r = __builtin_amdgcn_readlane(zP.r, 4 * t + gg);
+                    const int r = __builtin_amdgcn_readlane(sv.x, 16 * gg);
                     if (r < 0) break;
-                    if (gg > 0 && r == __builtin_amdgcn_readlane(zP.r, 4 * t + gg - 1)) continue;
+                    if (gg > 0 && r == __builtin_amdgcn_readlane(sv.x, 16 * gg - 16)) continue;
                     if (r != cur) {
                         flush(false);
                         cur = r;
-                        cur_slot = __builtin_amdgcn_readlane(zP.sl, 4 * t + gg);
-                        acc = floatx4{0.f, 0.f, 0.f, 0.f};
+                        acc0 = floatx4{0.f, 0.f, 0.f, 0.f};
+                        acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
+                        vacc = floatx4{0.f, 0.f, 0.f, 0.f};
                     }
-                    const bool mine = myrow == r;
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(mine ? aP[t] : 0.f, mine ? bP[t] : 0.f, acc, 0,
-                                                               0, 0);
+                    const bool mine = sv.x == r;
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(mine ? av : 0.f, mine ? bv : 0.f, acc0, 0, 0, 0);
                 }
             }
         }
-        zP = zQ;
-        zQ = zR;
+    };
+
+    // batch k (first nonzero b0, ring slot ks) with register set (z, ga, gb):
+    // z holds the records of batch k + 2, ga/gb the gathers of batch k
+    auto step = [&](int64_t b0, int ks, intx4 &z, floatx4 (&ga)[4], floatx4 (&gb)[4]) {
+        wave_sync();  // the previous batch's LDS reads are done
 #pragma unroll
-        for (int t = 0; t < 16; t++) {
-            aP[t] = aQ[t];
-            bP[t] = bQ[t];
+        for (int s = 0; s < 4; s++) {
+            *(floatx4 *)(L.a + (16 * s + p4) * 16 + 4 * c4) = ga[s];
+            *(floatx4 *)(L.b + (16 * s + p4) * 16 + 4 * c4) = gb[s];
         }
+        const int ks2 = ks == 0 ? 2 : ks - 1;  // (ks + 2) mod 3
+        const bool ok = b0 + 128 + lane < P1;
+        L.sv[ks2][lane] = ok ? intx2{z.z, z.w} : intx2{-1, 0};
+        sp_issue(z, Ua, Ub, p4, c4, ga, gb);
+        z = sp_rec_load(rec, b0 + 256 + lane, P1);
+        wave_sync();
+        compute(b0, ks);
+    };
+
+    intx4 zX = sp_rec_load(rec, P0 + lane, P1);
+    intx4 zY = sp_rec_load(rec, P0 + 64 + lane, P1);
+    floatx4 gaX[4], gbX[4], gaY[4], gbY[4];
+    L.sv[0][lane] = P0 + lane < P1 ? intx2{zX.z, zX.w} : intx2{-1, 0};
+    sp_issue(zX, Ua, Ub, p4, c4, gaX, gbX);
+    zX = sp_rec_load(rec, P0 + 128 + lane, P1);
+    L.sv[1][lane] = P0 + 64 + lane < P1 ? intx2{zY.z, zY.w} : intx2{-1, 0};
+    sp_issue(zY, Ua, Ub, p4, c4, gaY, gbY);
+    zY = sp_rec_load(rec, P0 + 192 + lane, P1);
+    // a counted loop over pairs of batches (padding past P1 has row -1 and
+    // value 0, so the last batch needs no bounds test), then the odd batch
+    const int64_t nb = (P1 - P0 + 63) / 64;
+    int ks = 0;
+    int64_t b0 = P0;
+    for (int64_t i = 0; i < nb / 2; i++) {
+        step(b0, ks, zX, gaX, gbX);
+        ks = ks == 2 ? 0 : ks + 1;
+        step(b0 + 64, ks, zY, gaY, gbY);
+        ks = ks == 2 ? 0 : ks + 1;
+        b0 += 128;
     }
+    if (nb & 1) step(b0, ks, zX, gaX, gbX);
     flush(true);
 }
 
@@ -362,13 +433,22 @@ __global__ __launch_bounds__(256) void sp_ytu(const float *__restrict__ Yc, cons
         for (int v = 0; v < 4; v++) L[(16 * (4 * wave + q) + 4 * g + v) * 16 + c] = acc[q][v];
 }
 
-// L (256 x 16, FP64) = sum over workgroups of the partials, fixed order.
-__global__ void sp_lreduce(const float *__restrict__ Lwg, int nwg, double *__restrict__ L) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= SP_E * 16) return;
+// L (256 x 16, FP64) = sum over workgroups of the partials, fixed order: 16
+// entries per workgroup, 16 groups of threads each summing every 16th partial,
+// then the groups added in order.
+__global__ __launch_bounds__(256) void sp_lreduce(const float *__restrict__ Lwg, int nwg, double *__restrict__ L) {
+    __shared__ double red[16][17];
+    const int c = threadIdx.x & 15, grp = threadIdx.x >> 4;
+    const int i = blockIdx.x * 16 + c;
     double s = 0.0;
-    for (int w = 0; w < nwg; w++) s += (double)Lwg[(int64_t)w * SP_E * 16 + i];
-    L[i] = s;
+    for (int w = grp; w < nwg; w += 16) s += (double)Lwg[(int64_t)w * SP_E * 16 + i];
+    red[grp][c] = s;
+    __syncthreads();
+    if (grp == 0) {
+        double t = 0.0;
+        for (int g = 0; g < 16; g++) t += red[g][c];
+        L[i] = t;
+    }
 }
 
 // W (nrows x 16) = Y_c L, scattered to U (I x 16 row-major FP64) rows; U is
@@ -403,21 +483,29 @@ __global__ void sp_expand(const float *__restrict__ Yc, const int32_t *__restric
         Y[(int64_t)rows[e / SP_E] * SP_E + e % SP_E] = Yc[e];
 }
 
-// U (I x R column-major) = the first R columns of Ur (I x 16 row-major)
+// U (I x R column-major) = the first R columns of Ur (I x 16 row-major); one
+// row per thread, column-coalesced writes.
 __global__ void rowmajor_to_colmajor(const double *__restrict__ Ur, int64_t I, int R, double *__restrict__ U) {
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < I * R; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = e % I;
-        const int r = (int)(e / I);
-        U[e] = Ur[i * 16 + r];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < I; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 *src = (const double2 *)(Ur + i * 16);
+#pragma unroll
+        for (int h = 0; h < 8; h++) {
+            const double2 v = src[h];
+            if (2 * h < R) U[i + I * (2 * h)] = v.x;
+            if (2 * h + 1 < R) U[i + I * (2 * h + 1)] = v.y;
+        }
     }
 }
 
-// row-major FP32 view with 16 columns, columns R..15 zero
+// row-major FP32 view with 16 columns, columns R..15 zero; one row per thread
 __global__ void rows16_view(const double *__restrict__ U, int64_t I, int R, float *__restrict__ Ur) {
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < I * 16; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = e / 16;
-        const int r = (int)(e % 16);
-        Ur[e] = r < R ? (float)U[i + I * r] : 0.f;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < I; i += (int64_t)gridDim.x * blockDim.x) {
+        float v[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) v[r] = r < R ? (float)U[i + I * r] : 0.f;
+        floatx4 *dst = (floatx4 *)(Ur + i * 16);
+#pragma unroll
+        for (int q = 0; q < 4; q++) dst[q] = floatx4{v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]};
     }
 }
 
@@ -440,9 +528,8 @@ __global__ void sumsq_f32_sp(const float *__restrict__ v, int64_t n, double *__r
 // ---- the sparse tensor ---------------------------------------------------------
 
 struct dt_sp_mode {
-    int32_t *perm = nullptr, *row = nullptr, *ia = nullptr, *ib = nullptr, *rowptr = nullptr, *slot = nullptr,
-            *rows = nullptr;
-    float *val = nullptr;
+    int32_t *perm = nullptr, *row = nullptr, *rowptr = nullptr, *slot = nullptr, *rows = nullptr;
+    intx4 *rec = nullptr;  // 3-way tensors: TTMc records in this mode's order
     int64_t nrows = 0;
 };
 
@@ -633,23 +720,15 @@ extern "C" int dt_sp_create(dt_ctx *ctx, int ndim, const size_t *dims, int64_t n
         const int64_t I = sp->dims[n];
         m.perm = (int32_t *)sp_alloc(sp, (size_t)nnz * 4);
         m.row = (int32_t *)sp_alloc(sp, (size_t)nnz * 4);
-        m.val = (float *)sp_alloc(sp, (size_t)nnz * 4);
         m.rowptr = (int32_t *)sp_alloc(sp, (size_t)(I + 1) * 4);
         m.slot = (int32_t *)sp_alloc(sp, (size_t)(I + 1) * 4);
-        if (!m.perm || !m.row || !m.val || !m.rowptr || !m.slot) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
+        if (!m.perm || !m.row || !m.rowptr || !m.slot) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
         if (ndim == 3) {
-            m.ia = (int32_t *)sp_alloc(sp, (size_t)nnz * 4);
-            m.ib = (int32_t *)sp_alloc(sp, (size_t)nnz * 4);
-            if (!m.ia || !m.ib) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
+            m.rec = (intx4 *)sp_alloc(sp, (size_t)nnz * 16);
+            if (!m.rec) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
         }
         st = sort_mode_internal(ctx, nnz, sp->idx[n], I, m.perm, m.row, m.rowptr);
         if (st != DT_OK) return fail(st, "dt_sp_create: mode sort failed");
-        if (ndim == 3) {
-            const int a = n == 0 ? 1 : 0, b = n == 2 ? 1 : 2;
-            hipLaunchKernelGGL(gather_by<int32_t>, sp_grid(nnz), dim3(256), 0, ctx->stream, sp->idx[a], m.perm, nnz, m.ia);
-            hipLaunchKernelGGL(gather_by<int32_t>, sp_grid(nnz), dim3(256), 0, ctx->stream, sp->idx[b], m.perm, nnz, m.ib);
-        }
-        hipLaunchKernelGGL(gather_by<float>, sp_grid(nnz), dim3(256), 0, ctx->stream, sp->val, m.perm, nnz, m.val);
         // compact numbering of the non-empty rows
         int32_t *f = (int32_t *)dt_ws(ctx, WS_SPARSE1, (size_t)(I + 1) * 4 + 256);
         if (!f) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
@@ -669,6 +748,19 @@ extern "C" int dt_sp_create(dt_ctx *ctx, int ndim, const size_t *dims, int64_t n
         m.rows = (int32_t *)sp_alloc(sp, (size_t)std::max<int64_t>(m.nrows, 1) * 4);
         if (!m.rows) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
         hipLaunchKernelGGL(rows_list, sp_grid(I), dim3(256), 0, ctx->stream, f, m.slot, I, m.rows);
+        if (ndim == 3) {
+            // TTMc records: the other modes' indices and the value in this mode's order
+            const int a = n == 0 ? 1 : 0, b = n == 2 ? 1 : 2;
+            int32_t *tia = (int32_t *)dt_ws(ctx, WS_SPARSE0, (size_t)nnz * 12 + 256);
+            if (!tia) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
+            int32_t *tib = tia + nnz;
+            float *tval = (float *)(tib + nnz);
+            hipLaunchKernelGGL(gather_by<int32_t>, sp_grid(nnz), dim3(256), 0, ctx->stream, sp->idx[a], m.perm, nnz, tia);
+            hipLaunchKernelGGL(gather_by<int32_t>, sp_grid(nnz), dim3(256), 0, ctx->stream, sp->idx[b], m.perm, nnz, tib);
+            hipLaunchKernelGGL(gather_by<float>, sp_grid(nnz), dim3(256), 0, ctx->stream, sp->val, m.perm, nnz, tval);
+            hipLaunchKernelGGL(build_rec, sp_grid(nnz), dim3(256), 0, ctx->stream, m.row, tia, tib, tval, m.slot, nnz,
+                               m.rec);
+        }
         if (hipGetLastError() != hipSuccess) return fail(DT_EHIP, "dt_sp_create: launch failed");
     }
     // ||A||^2 in FP64
@@ -703,13 +795,28 @@ extern "C" int dt_sp_mode_arrays(const dt_sp *sp, int n, const int32_t **perm, c
 
 // ---- TTMc driver ---------------------------------------------------------------
 
+// L = Y_cᵀ U_n (256 x 16 FP64) from the compact rows of mode n; Un row-major
+// FP32 I_n x 16.
+static int sp_ytu_dev(dt_ctx *ctx, dt_sp *sp, int n, const float *Un, const float *Yc, double *L) {
+    const dt_sp_mode &m = sp->mode[n];
+    const int64_t nwg = std::min<int64_t>(1024, std::max<int64_t>(1, (m.nrows + 255) / 256));
+    int64_t rpw = (m.nrows + nwg - 1) / nwg;
+    rpw = (rpw + 3) / 4 * 4;
+    const int64_t ng = std::max<int64_t>(1, (m.nrows + rpw - 1) / rpw);
+    float *Lwg = (float *)dt_ws(ctx, WS_SPARSE1, (size_t)ng * SP_E * 16 * 4 + 256);
+    if (!Lwg) return DT_ENOMEM;
+    DT_LAUNCH("sp_ytu", sp_ytu, dim3((unsigned)ng), dim3(256), 0, Yc, m.rows, m.nrows, Un, rpw, Lwg);
+    DT_LAUNCH("sp_lreduce", sp_lreduce, dim3(SP_E * 16 / 16), dim3(256), 0, (const float *)Lwg, (int)ng, L);
+    return DT_OK;
+}
+
 // Yc (nrows x 256) and, with L, L = Y_cᵀ U_n (256 x 16 FP64) for mode n of a
 // 3-way tensor with ranks 16 in the other modes.  Ur[m]: row-major FP32 I_m x 16.
 static int sp_ttmc_dev(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, float *Yc, double *L) {
     DT_CHECK_ARG(sp->ndim == 3, "sparse TTMc: 3-way tensors only (got %d)", sp->ndim);
     const dt_sp_mode &m = sp->mode[n];
     const int a = n == 0 ? 1 : 0, b = n == 2 ? 1 : 2;
-    SpModeArgs args{m.row, m.ia, m.ib, m.val, m.rowptr, m.slot, sp->nnz};
+    SpModeArgs args{m.row, m.rec, m.rowptr, m.slot, sp->nnz};
     static int blocks_per_cu = 0, cus = 0;
     if (!blocks_per_cu) {
         int dev = 0;
@@ -718,28 +825,20 @@ static int sp_ttmc_dev(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, fl
         DT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, sp_ttmc16, 256, 0));
         if (blocks_per_cu < 1) blocks_per_cu = 1;
     }
-    // equal contiguous ranges, multiples of 64, one per resident wave
-    const int64_t maxw = (int64_t)cus * blocks_per_cu * 4;
+    // equal contiguous ranges, multiples of 64, two per resident wave (the
+    // hardware hands a range to whichever wave finishes first, which evens
+    // out ranges of different cost)
+    const int64_t maxw = (int64_t)cus * blocks_per_cu * 4 * 2;
     int64_t per_wave = ((sp->nnz + maxw - 1) / maxw + 63) / 64 * 64;
     if (per_wave < 64) per_wave = 64;
     const int64_t nwaves = (sp->nnz + per_wave - 1) / per_wave;
-    float *carry = (float *)dt_ws(ctx, WS_SPARSE0, (size_t)nwaves * 2 * SP_E * 4 + (size_t)nwaves * 2 * 4 + 512);
+    float *carry = (float *)dt_ws(ctx, WS_SPARSE0, (size_t)nwaves * 2 * SP_E * 4 + 512);
     if (!carry) return DT_ENOMEM;
-    int32_t *carry_row = (int32_t *)(carry + nwaves * 2 * SP_E);
     static const char *names[3] = {"sp_ttmc_m0", "sp_ttmc_m1", "sp_ttmc_m2"};
     DT_LAUNCH(names[n], sp_ttmc16, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, args, Ur[a], Ur[b], Yc, carry,
-              carry_row, per_wave, nwaves);
+              per_wave, nwaves);
     DT_LAUNCH("sp_fixup", sp_fixup, dim3((unsigned)nwaves), dim3(256), 0, args, carry, per_wave, nwaves, Yc);
-    if (L) {
-        const int64_t nwg = std::min<int64_t>(1024, std::max<int64_t>(1, (m.nrows + 255) / 256));
-        int64_t rpw = (m.nrows + nwg - 1) / nwg;
-        rpw = (rpw + 3) / 4 * 4;
-        const int64_t ng = std::max<int64_t>(1, (m.nrows + rpw - 1) / rpw);
-        float *Lwg = (float *)dt_ws(ctx, WS_SPARSE1, (size_t)ng * SP_E * 16 * 4 + 256);
-        if (!Lwg) return DT_ENOMEM;
-        DT_LAUNCH("sp_ytu", sp_ytu, dim3((unsigned)ng), dim3(256), 0, Yc, m.rows, m.nrows, Ur[n], rpw, Lwg);
-        DT_LAUNCH("sp_lreduce", sp_lreduce, dim3((SP_E * 16 + 255) / 256), dim3(256), 0, Lwg, (int)ng, L);
-    }
+    if (L) DT_TRY(sp_ytu_dev(ctx, sp, n, Ur[n], Yc, L));
     return DT_OK;
 }
 
@@ -810,7 +909,7 @@ extern "C" int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int m
     }
     int st = DT_OK;
     auto views = [&](int m) {
-        hipLaunchKernelGGL(rows16_view, sp_grid(sp->dims[m] * 16), dim3(256), 0, ctx->stream, U64[m], sp->dims[m],
+        hipLaunchKernelGGL(rows16_view, sp_grid(sp->dims[m]), dim3(256), 0, ctx->stream, U64[m], sp->dims[m],
                            Rk[m], U32r[m]);
         return hipGetLastError() == hipSuccess ? DT_OK : DT_EHIP;
     };
@@ -840,21 +939,16 @@ extern "C" int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int m
             if (st == DT_OK) {
                 hipLaunchKernelGGL(sp_times_l, dim3((unsigned)std::min<int64_t>((md.nrows + 3) / 4, 4096)), dim3(256),
                                    0, ctx->stream, Yc, md.rows, md.nrows, L, Urow64);
-                hipLaunchKernelGGL(rowmajor_to_colmajor, sp_grid(I * Rk[n]), dim3(256), 0, ctx->stream, Urow64, I,
+                hipLaunchKernelGGL(rowmajor_to_colmajor, sp_grid(I), dim3(256), 0, ctx->stream, Urow64, I,
                                    Rk[n], U64[n]);
                 st = hipGetLastError() == hipSuccess ? DT_OK : DT_EHIP;
             }
             if (st == DT_OK) st = dt_qr_f64(ctx, U64[n], nullptr, U64[n], (size_t)I, (size_t)Rk[n]);  // orth(S U_n)
             if (st == DT_OK) st = views(n);
         }
-        // core = Y_(2)ᵀ U_2 with the updated U_2 (Y_c of the last mode is current)
-        if (st == DT_OK) {
-            const int n = 2;
-            const dt_sp_mode &md = sp->mode[n];
-            const float *Ur[3] = {U32r[0], U32r[1], U32r[2]};
-            st = sp_ttmc_dev(ctx, sp, n, Ur, Yc, G64);
-            (void)md;
-        }
+        // core = Y_(2)ᵀ U_2 with the updated U_2: Y_(2) does not depend on U_2, so
+        // the last mode's Y_c is reused
+        if (st == DT_OK) st = sp_ytu_dev(ctx, sp, 2, U32r[2], Yc, G64);
         (void)hipEventRecord(e1, ctx->stream);
         if (st != DT_OK) break;
         std::vector<double> g(SP_E * 16);
