@@ -314,43 +314,67 @@ __global__ void set_identity(double *__restrict__ A, int n) {
 }
 
 // ---- tall-skinny pieces of Cholesky-QR (n <= TS_NMAX, m large) ---------------
-// Partial Grams: workgroup w sums Q[r, i] Q[r, j] in FP64 over its contiguous
-// row range (rows staged TS_ROWS at a time through LDS with column-coalesced
-// loads) into part[w][i + n j]; thread t owns entries t, t + 256, ... < n^2.
+// Partial Grams on v_mfma_f64_16x16x4f64: workgroup w covers a contiguous row
+// range, staged TS_ROWS rows at a time through LDS with column-coalesced loads
+// (columns n..TS_NMAX-1 zero); wave v takes rows 32v..32v+31 of each block,
+// four rows per MFMA, lane (c, g) holding Q[r_g, c] (and Q[r_g, 16 + c] when
+// n > 16) as both operands, so D accumulates the 16 x 16 Gram tiles.  The four
+// waves' tiles are added in wave order and written to part[w][i + n j].
 constexpr int TS_NMAX = 32, TS_ROWS = 128;
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+template <bool HI>
 __global__ __launch_bounds__(256) void ts_gram_kernel(const double *__restrict__ Q, int64_t m, int n,
                                                       int64_t rows_per_wg, double *__restrict__ part) {
-    __shared__ double Qs[TS_ROWS][TS_NMAX + 1];
+    constexpr int NC = HI ? 32 : 16;
+    constexpr int NT = HI ? 3 : 1;  // tiles (lo,lo), (lo,hi), (hi,hi)
+    __shared__ double Qs[TS_ROWS][NC + 1];
+    __shared__ double red[4][NT][4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
     const int64_t r0 = (int64_t)blockIdx.x * rows_per_wg;
     const int64_t r1 = r0 + rows_per_wg < m ? r0 + rows_per_wg : m;
-    const int nn = n * n;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    int ei[4], ej[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int e = threadIdx.x + 256 * q;
-        ei[q] = e % n;
-        ej[q] = e / n;
-    }
+    doublex4 dll = {0.0, 0.0, 0.0, 0.0}, dlh = {0.0, 0.0, 0.0, 0.0}, dhh = {0.0, 0.0, 0.0, 0.0};
     for (int64_t rb = r0; rb < r1; rb += TS_ROWS) {
         const int nr = rb + TS_ROWS < r1 ? TS_ROWS : (int)(r1 - rb);
         __syncthreads();
-        for (int e = threadIdx.x; e < TS_ROWS * n; e += 256) {
-            const int r = e % TS_ROWS, c = e / TS_ROWS;
-            Qs[r][c] = r < nr ? Q[rb + r + m * c] : 0.0;
+        for (int e = threadIdx.x; e < TS_ROWS * NC; e += 256) {
+            const int r = e % TS_ROWS, col = e / TS_ROWS;
+            Qs[r][col] = (r < nr && col < n) ? Q[rb + r + m * col] : 0.0;
         }
         __syncthreads();
-#pragma unroll 4
-        for (int r = 0; r < TS_ROWS; r++) {
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-                if ((int)threadIdx.x + 256 * q < nn) acc[q] = fma(Qs[r][ei[q]], Qs[r][ej[q]], acc[q]);
+        for (int s = 0; s < 8; s++) {
+            const int r = 32 * wv + 4 * s + g;
+            const double lo = Qs[r][c];
+            dll = __builtin_amdgcn_mfma_f64_16x16x4f64(lo, lo, dll, 0, 0, 0);
+            if (HI) {
+                const double hi = Qs[r][16 + c];
+                dlh = __builtin_amdgcn_mfma_f64_16x16x4f64(lo, hi, dlh, 0, 0, 0);
+                dhh = __builtin_amdgcn_mfma_f64_16x16x4f64(hi, hi, dhh, 0, 0, 0);
+            }
         }
     }
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int e = threadIdx.x + 256 * q;
-        if (e < nn) part[(int64_t)blockIdx.x * nn + e] = acc[q];
+    for (int v = 0; v < 4; v++) {
+        red[wv][0][v][lane] = dll[v];
+        if (HI) {
+            red[wv][1][v][lane] = dlh[v];
+            red[wv][2][v][lane] = dhh[v];
+        }
+    }
+    __syncthreads();
+    // thread (v, lane): register v of lane (c, g) holds D[4v + g][c] (the FP64
+    // 16x16x4 output layout, unlike the FP32 one's D[4g + v][c])
+    const int v = threadIdx.x >> 6;
+    double *out = part + (int64_t)blockIdx.x * n * n;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const double sum = ((red[0][t][v][lane] + red[1][t][v][lane]) + red[2][t][v][lane]) + red[3][t][v][lane];
+        const int i = 4 * v + g + (t == 2 ? 16 : 0), j = c + (t >= 1 ? 16 : 0);
+        if (i < n && j < n) {
+            out[i + n * j] = sum;
+            if (t == 1) out[j + n * i] = sum;
+        }
     }
 }
 
@@ -407,7 +431,11 @@ static int ts_gram(dt_ctx *ctx, const double *Q, size_t m, size_t n, double *G) 
     const int64_t ng = ceil_div((int64_t)m, rpw);
     double *part = (double *)dt_ws(ctx, WS_GEMM_PART, (size_t)(ng * n * n) * sizeof(double));
     if (!part) return DT_ENOMEM;
-    DT_LAUNCH("ts_gram", ts_gram_kernel, dim3((unsigned)ng), dim3(256), 0, Q, (int64_t)m, (int)n, rpw, part);
+    if (n <= 16) {
+        DT_LAUNCH("ts_gram", ts_gram_kernel<false>, dim3((unsigned)ng), dim3(256), 0, Q, (int64_t)m, (int)n, rpw, part);
+    } else {
+        DT_LAUNCH("ts_gram", ts_gram_kernel<true>, dim3((unsigned)ng), dim3(256), 0, Q, (int64_t)m, (int)n, rpw, part);
+    }
     DT_LAUNCH("ts_gram", part_reduce_kernel, dim3((unsigned)ceil_div((int64_t)(n * n), 16)), dim3(256), 0,
               (const double *)part, (int)ng, (int)(n * n), G);
     return DT_OK;

@@ -402,35 +402,46 @@ __global__ __launch_bounds__(256) void sp_fixup(SpModeArgs a, const float *__res
     }
 }
 
-// Per-workgroup partial L = Y_cᵀ U_n[rows] (256 x 16) on v_mfma_f32_16x16x4_f32:
-// wave w covers entry tiles 4w..4w+3; k-steps of 4 compact rows.
-__global__ __launch_bounds__(256) void sp_ytu(const float *__restrict__ Yc, const int32_t *__restrict__ rows,
-                                              int64_t nrows, const float *__restrict__ Un, int64_t rows_per_wg,
-                                              float *__restrict__ Lwg) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int c = lane & 15, g = lane >> 4;
+// Per-wave partial L = Y_cᵀ U_n[rows] (256 x 16): one 64-thread workgroup per
+// contiguous run of compact rows; lane l owns entries 4l..4l+3 of the row for
+// all 16 columns (64 FP32 accumulators), reading each Y_c row once (1 KB,
+// coalesced) and the factor row once per wave.
+__global__ __launch_bounds__(64) void sp_ytu(const float *__restrict__ Yc, const int32_t *__restrict__ rows,
+                                             int64_t nrows, const float *__restrict__ Un, int64_t rows_per_wg,
+                                             float *__restrict__ Lwg) {
+    const int lane = threadIdx.x;
     const int64_t s0 = (int64_t)blockIdx.x * rows_per_wg;
     const int64_t s1 = s0 + rows_per_wg < nrows ? s0 + rows_per_wg : nrows;
-    floatx4 acc[4];
+    float acc[4][16];
 #pragma unroll
-    for (int q = 0; q < 4; q++) acc[q] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int64_t s = s0; s < s1; s += 4) {
-        const int64_t sg = s + g;
-        const bool ok = sg < s1;
-        const float u = ok ? Un[(int64_t)rows[sg] * 16 + c] : 0.f;
+    for (int q = 0; q < 4; q++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int e = 16 * (4 * wave + q) + c;
-            const float y = ok ? Yc[sg * SP_E + e] : 0.f;
-            acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(y, u, acc[q], 0, 0, 0);
+        for (int k = 0; k < 16; k++) acc[q][k] = 0.f;
+#pragma unroll 2
+    for (int64_t s = s0; s < s1; s++) {
+        const floatx4 y = *(const floatx4 *)(Yc + s * SP_E + 4 * lane);
+        const floatx4 *u4 = (const floatx4 *)(Un + (int64_t)rows[s] * 16);
+        float u[16];
+#pragma unroll
+        for (int h = 0; h < 4; h++) {
+            const floatx4 t = u4[h];
+            u[4 * h] = t[0];
+            u[4 * h + 1] = t[1];
+            u[4 * h + 2] = t[2];
+            u[4 * h + 3] = t[3];
         }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int k = 0; k < 16; k++) acc[q][k] = fmaf(y[q], u[k], acc[q][k]);
     }
-    // D[i = 4g + v][j = c] of tile q: entry 16 (4 wave + q) + 4g + v, column k = c
     float *L = Lwg + (int64_t)blockIdx.x * SP_E * 16;
 #pragma unroll
     for (int q = 0; q < 4; q++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) L[(16 * (4 * wave + q) + 4 * g + v) * 16 + c] = acc[q][v];
+        for (int h = 0; h < 4; h++)
+            *(floatx4 *)(L + (4 * lane + q) * 16 + 4 * h) =
+                floatx4{acc[q][4 * h], acc[q][4 * h + 1], acc[q][4 * h + 2], acc[q][4 * h + 3]};
 }
 
 // L (256 x 16, FP64) = sum over workgroups of the partials, fixed order: 16
@@ -799,13 +810,12 @@ extern "C" int dt_sp_mode_arrays(const dt_sp *sp, int n, const int32_t **perm, c
 // FP32 I_n x 16.
 static int sp_ytu_dev(dt_ctx *ctx, dt_sp *sp, int n, const float *Un, const float *Yc, double *L) {
     const dt_sp_mode &m = sp->mode[n];
-    const int64_t nwg = std::min<int64_t>(1024, std::max<int64_t>(1, (m.nrows + 255) / 256));
-    int64_t rpw = (m.nrows + nwg - 1) / nwg;
-    rpw = (rpw + 3) / 4 * 4;
+    const int64_t nwg = std::min<int64_t>(2048, std::max<int64_t>(1, (m.nrows + 63) / 64));
+    const int64_t rpw = (m.nrows + nwg - 1) / nwg;
     const int64_t ng = std::max<int64_t>(1, (m.nrows + rpw - 1) / rpw);
     float *Lwg = (float *)dt_ws(ctx, WS_SPARSE1, (size_t)ng * SP_E * 16 * 4 + 256);
     if (!Lwg) return DT_ENOMEM;
-    DT_LAUNCH("sp_ytu", sp_ytu, dim3((unsigned)ng), dim3(256), 0, Yc, m.rows, m.nrows, Un, rpw, Lwg);
+    DT_LAUNCH("sp_ytu", sp_ytu, dim3((unsigned)ng), dim3(64), 0, Yc, m.rows, m.nrows, Un, rpw, Lwg);
     DT_LAUNCH("sp_lreduce", sp_lreduce, dim3(SP_E * 16 / 16), dim3(256), 0, (const float *)Lwg, (int)ng, L);
     return DT_OK;
 }

@@ -230,7 +230,8 @@ def test_gemm_f64(ctx, tA, tB, m, n, k):
     assert rel(Cm, ref) < 1e-14
 
 
-@pytest.mark.parametrize("m,n", [(64, 8), (1024, 64), (300, 1), (50, 50), (20000, 16)])
+@pytest.mark.parametrize("m,n", [(64, 8), (1024, 64), (300, 1), (50, 50), (20000, 16), (9001, 29), (4097, 3),
+                                 (65537, 32)])
 def test_qr_f64(ctx, m, n):
     rng = np.random.default_rng(11)
     A = rng.standard_normal((m, n))
@@ -244,9 +245,12 @@ def test_qr_f64(ctx, m, n):
     assert np.allclose(Q, Qn, atol=1e-10)
 
 
-def test_qr_ill_conditioned_uses_shift(ctx):
+@pytest.mark.parametrize("m", [500, 6000])
+def test_qr_ill_conditioned_uses_shift(ctx, m):
+    """Condition 1e9 breaks the plain Cholesky-QR; the shifted pass recovers
+    (m = 6000 runs the tall-skinny Gram and row update)."""
     rng = np.random.default_rng(12)
-    U = np.linalg.qr(rng.standard_normal((500, 10)))[0]
+    U = np.linalg.qr(rng.standard_normal((m, 10)))[0]
     V = np.linalg.qr(rng.standard_normal((10, 10)))[0]
     A = U @ np.diag(np.logspace(0, -9, 10)) @ V.T
     Q, R = ctx.qr(ctx.to_device(A))
