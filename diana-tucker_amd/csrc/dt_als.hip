@@ -213,19 +213,21 @@ __device__ __forceinline__ double rl_f64(double v, int lane) {
 }
 
 // One workgroup: G = sum_w part[w] (fixed order); optional shift; Rinv = chol(G)^{-1}.
+// CM (16, 32 or 64 >= R) is the padded order of the register Cholesky.
+template <int CM>
 __global__ __launch_bounds__(ALS_THREADS) void als_chol_kernel(const double *__restrict__ part,
                                                                int nparts, int R, int I,
                                                                int shifted,
                                                                double *__restrict__ Rinv_out,
                                                                int *__restrict__ flag) {
-    __shared__ double G[ALS_MAXR][ALS_MAXR + 1];
-    __shared__ double X[ALS_MAXR][ALS_MAXR + 1];
+    __shared__ double G[CM][CM + 1];
+    __shared__ double X[CM][CM + 1];
     __shared__ double shift_s;
     const int tid = threadIdx.x;
     {
         // G = sum of the partials in a fixed order; each pass over w issues
         // this thread's loads for all its entries together
-        constexpr int NE = (ALS_MAXR * ALS_MAXR + ALS_THREADS - 1) / ALS_THREADS;
+        constexpr int NE = (CM * CM + ALS_THREADS - 1) / ALS_THREADS;
         const size_t RR = (size_t)R * R;
         double acc[NE];
 #pragma unroll
@@ -255,23 +257,24 @@ __global__ __launch_bounds__(ALS_THREADS) void als_chol_kernel(const double *__r
     }
     __syncthreads();
     if (tid < R) G[tid][tid] += shift_s;
-    // pad to ALS_MAXR with the identity: chol(diag(G, I)) = diag(chol(G), I)
-    for (int e = tid; e < ALS_MAXR * ALS_MAXR; e += ALS_THREADS) {
-        const int i = e % ALS_MAXR, j = e / ALS_MAXR;
+    // pad to CM with the identity: chol(diag(G, I)) = diag(chol(G), I)
+    for (int e = tid; e < CM * CM; e += ALS_THREADS) {
+        const int i = e % CM, j = e / CM;
         if (i >= R || j >= R) G[i][j] = (i == j) ? 1.0 : 0.0;
     }
     __syncthreads();
     if (tid < 64) {
-        // One wave, lane k holding column k in registers; every index below is
-        // a compile-time constant after unrolling, so row j of the trailing
+        // One wave, lane k < CM holding column k in registers (lanes past CM
+        // repeat column CM - 1 and write nothing); every index below is a
+        // compile-time constant after unrolling, so row j of the trailing
         // update comes from v_readlane (lane i's col[j] = G[j][i]).
-        const int k = tid;
-        double col[ALS_MAXR];
+        const int k = tid < CM ? tid : CM - 1;
+        double col[CM];
 #pragma unroll
-        for (int i = 0; i < ALS_MAXR; i++) col[i] = G[i][k];
+        for (int i = 0; i < CM; i++) col[i] = G[i][k];
         int bad = 0;
 #pragma unroll
-        for (int j = 0; j < ALS_MAXR; j++) {
+        for (int j = 0; j < CM; j++) {
             double d = rl_f64(col[j], j);
             if (!(d > 0.0) || !isfinite(d)) {
                 if (!bad) bad = j + 1;
@@ -280,33 +283,49 @@ __global__ __launch_bounds__(ALS_THREADS) void als_chol_kernel(const double *__r
             const double rjj = sqrt(d), inv_d = 1.0 / d;
             const double gjk = col[j];
 #pragma unroll
-            for (int i = j + 1; i < ALS_MAXR; i++) col[i] -= rl_f64(col[j], i) * gjk * inv_d;
+            for (int i = j + 1; i < CM; i++) col[i] -= rl_f64(col[j], i) * gjk * inv_d;
             col[j] = (k == j) ? rjj : gjk / rjj;  // row j of R (meaningful for k >= j)
         }
+        if (tid < CM)
 #pragma unroll
-        for (int i = 0; i < ALS_MAXR; i++) X[i][k] = col[i];  // X = R (upper part)
-        if (bad && k == 0) atomicOr(flag, 4);
+            for (int i = 0; i < CM; i++) X[i][k] = col[i];  // X = R (upper part)
+        if (bad && tid == 0) atomicOr(flag, 4);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // column k of R^{-1}: x[i] = (delta_ik - sum_{m>i} R[i][m] x[m]) / R[i][i];
         // x[m] = 0 for m > k falls out of the recurrence
-        double x[ALS_MAXR];
+        double x[CM];
 #pragma unroll
-        for (int i = ALS_MAXR - 1; i >= 0; i--) {
+        for (int i = CM - 1; i >= 0; i--) {
             double s0 = (i == k) ? 1.0 : 0.0, s1 = 0.0;
 #pragma unroll
-            for (int m = i + 1; m < ALS_MAXR; m += 2) {
+            for (int m = i + 1; m < CM; m += 2) {
                 s0 -= X[i][m] * x[m];
-                if (m + 1 < ALS_MAXR) s1 -= X[i][m + 1] * x[m + 1];
+                if (m + 1 < CM) s1 -= X[i][m + 1] * x[m + 1];
             }
             x[i] = (s0 + s1) / X[i][i];
         }
-        if (!bad && k < R)
+        if (!bad && tid < R)
 #pragma unroll
-            for (int i = 0; i < ALS_MAXR; i++)
+            for (int i = 0; i < CM; i++)
                 if (i < R) Rinv_out[i + (size_t)R * k] = x[i];
     }
+}
+
+static int launch_chol(dt_ctx *ctx, const double *part, int nparts, int R, int I, int shifted, double *ri,
+                       int *flag) {
+    if (R <= 16) {
+        DT_LAUNCH("als_chol", als_chol_kernel<16>, dim3(1), dim3(ALS_THREADS), 0, part, nparts, R, I, shifted, ri,
+                  flag);
+    } else if (R <= 32) {
+        DT_LAUNCH("als_chol", als_chol_kernel<32>, dim3(1), dim3(ALS_THREADS), 0, part, nparts, R, I, shifted, ri,
+                  flag);
+    } else {
+        DT_LAUNCH("als_chol", als_chol_kernel<64>, dim3(1), dim3(ALS_THREADS), 0, part, nparts, R, I, shifted, ri,
+                  flag);
+    }
+    return DT_OK;
 }
 
 template <int RMAX>
@@ -351,13 +370,13 @@ int dt_als_update(dt_ctx *ctx, const double *S, double *U, int I, int R, int inn
         double *nxt = Yb[k & 1];
         DT_TRY(rows(ctx, 0, S, cur, I, R, curRi, nxt, part));
         double *ri = Ri + (size_t)(k & 1) * R * R;
-        DT_LAUNCH("als_chol", als_chol_kernel, dim3(1), dim3(ALS_THREADS), 0, part, nb, R, I, 1, ri, flag);
+        DT_TRY(launch_chol(ctx, part, nb, R, I, 1, ri, flag));
         cur = nxt;
         curRi = ri;
     }
     // Q = Y R^{-1} (orthonormal to ~cond(Y) u), then one plain Cholesky-QR pass
     DT_TRY(rows(ctx, 1, nullptr, cur, I, R, curRi, U, part));
     double *ri = Ri + (size_t)(inner & 1) * R * R;
-    DT_LAUNCH("als_chol", als_chol_kernel, dim3(1), dim3(ALS_THREADS), 0, part, nb, R, I, 0, ri, flag);
+    DT_TRY(launch_chol(ctx, part, nb, R, I, 0, ri, flag));
     return rows(ctx, 1, nullptr, U, I, R, ri, U, nullptr, U32, M32, Ub16);
 }

@@ -179,6 +179,22 @@ __global__ void build_rec(const int32_t *__restrict__ row, const int32_t *__rest
         rec[p] = intx4{ia[p], ib[p], slot[row[p]], __float_as_int(val[p])};
 }
 
+// Sort key of a record: (compact row, index in mode a); the position rides along.
+__global__ void rec_keys(const intx4 *__restrict__ rec, int64_t n, int ba, uint64_t *__restrict__ key,
+                         int32_t *__restrict__ pos) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const intx4 r = rec[p];
+        key[p] = ((uint64_t)(uint32_t)r.z << ba) | (uint32_t)r.x;
+        pos[p] = (int32_t)p;
+    }
+}
+
+__global__ void rec_gather(const intx4 *__restrict__ src, const int32_t *__restrict__ pos, int64_t n,
+                           intx4 *__restrict__ dst) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+        dst[p] = src[pos[p]];
+}
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -188,6 +204,7 @@ __device__ __forceinline__ void wave_sync() {
 // Per-wave LDS: the staged batch's U_a and U_b rows [nonzero][column] and a
 // ring of three batches' (compact row, value) pairs.
 constexpr int SP_RING = 3;
+constexpr int SP_PF = 8;  // MFMA steps whose operands are read from LDS together
 struct alignas(16) SpLds {
     float a[64 * 16];
     float b[64 * 16];
@@ -233,6 +250,7 @@ __device__ __forceinline__ void sp_issue(const intx4 &z, const float *__restrict
 // layout, so a row end never waits for an MFMA.  A finished row is written to Y_c when it lies
 // inside the range; otherwise it goes to carry slot 0 (row started before
 // P0) or 1 (row continues past P1), and sp_fixup adds them.
+template <int PF>
 __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__restrict__ Ua,
                                                  const float *__restrict__ Ub, float *__restrict__ Yc,
                                                  float *__restrict__ carry, int64_t per_wave, int64_t nwaves) {
@@ -262,12 +280,13 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
         const bool linc = cur == first_row && left_inc;
         const bool rinc = last && after == cur;
         float *dst = (!linc && !rinc) ? Yc + (int64_t)cur * SP_E : carry + (w * 2 + (linc ? 0 : 1)) * SP_E;
-        *(floatx4 *)(dst + e0) = (acc0 + acc1) + vacc;
+        __builtin_nontemporal_store((acc0 + acc1) + vacc, (floatx4 *)(dst + e0));
     };
 
-    auto compute = [&](int64_t b0, int ks) {
-        // rows starting in this batch (lane k: nonzero k; padding has row -1)
-        const intx2 svl = L.sv[ks][lane];
+    // svl: lane k holds (row, value) of nonzero k of the batch; the staged U_a
+    // rows are already scaled by the values
+    auto compute = [&](const intx2 &svl) {
+        // rows starting in this batch (padding has row -1)
         const int prevr = __builtin_amdgcn_ds_bpermute((lane == 0 ? 0 : lane - 1) * 4, svl.x);
         const uint64_t heads = __ballot(svl.x >= 0 && svl.x != (lane == 0 ? cur : prevr));
         if (__popcll(heads) > 1) {
@@ -279,9 +298,8 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
             for (;;) {
                 const int h = hm ? __builtin_ctzll(hm) : nvalid;
                 for (int kk = k; kk < h; kk++) {
-                    const float vv = __int_as_float(__builtin_amdgcn_readlane(svl.y, kk));
                     const floatx4 ua = *(const floatx4 *)(L.a + kk * 16 + 4 * g);
-                    const float wb = vv * L.b[kk * 16 + c];
+                    const float wb = L.b[kk * 16 + c];
                     vacc[0] = fmaf(ua[0], wb, vacc[0]);
                     vacc[1] = fmaf(ua[1], wb, vacc[1]);
                     vacc[2] = fmaf(ua[2], wb, vacc[2]);
@@ -298,24 +316,32 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
             }
             return;
         }
-#pragma unroll 2
-        for (int t = 0; t < 16; t++) {
-            const float ua = L.a[(4 * t + g) * 16 + c], bv = L.b[(4 * t + g) * 16 + c];
-            const intx2 sv = L.sv[ks][4 * t + g];
-            const float av = __int_as_float(sv.y) * ua;
-            const int r0 = __builtin_amdgcn_readlane(sv.x, 0);
-            const int r3 = __builtin_amdgcn_readlane(sv.x, 48);
+        // operand reads of PF steps at a time, so their LDS latency overlaps
+#pragma unroll
+        for (int t0 = 0; t0 < 16; t0 += PF) {
+        float av[PF], bv[PF];
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            av[u] = L.a[(4 * (t0 + u) + g) * 16 + c];
+            bv[u] = L.b[(4 * (t0 + u) + g) * 16 + c];
+        }
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int t = t0 + u;
+            const int r0 = __builtin_amdgcn_readlane(svl.x, 4 * t);
+            const int r3 = __builtin_amdgcn_readlane(svl.x, 4 * t + 3);
             if (r0 == cur && r3 == cur) {
                 if (t & 1)
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc1, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], acc1, 0, 0, 0);
                 else
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc0, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], acc0, 0, 0, 0);
             } else {
+                const int myrow = __builtin_amdgcn_ds_bpermute((4 * t + g) * 4, svl.x);
 #pragma unroll 1
                 for (int gg = 0; gg < 4; gg++) {
-                    const int r = __builtin_amdgcn_readlane(sv.x, 16 * gg);
+                    const int r = __builtin_amdgcn_readlane(svl.x, 4 * t + gg);
                     if (r < 0) break;
-                    if (gg > 0 && r == __builtin_amdgcn_readlane(sv.x, 16 * gg - 16)) continue;
+                    if (gg > 0 && r == __builtin_amdgcn_readlane(svl.x, 4 * t + gg - 1)) continue;
                     if (r != cur) {
                         flush(false);
                         cur = r;
@@ -323,10 +349,12 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
                         acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
                         vacc = floatx4{0.f, 0.f, 0.f, 0.f};
                     }
-                    const bool mine = sv.x == r;
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(mine ? av : 0.f, mine ? bv : 0.f, acc0, 0, 0, 0);
+                    const bool mine = myrow == r;
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(mine ? av[u] : 0.f, mine ? bv[u] : 0.f, acc0, 0, 0,
+                                                                0);
                 }
             }
+        }
         }
     };
 
@@ -334,9 +362,11 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
     // z holds the records of batch k + 2, ga/gb the gathers of batch k
     auto step = [&](int64_t b0, int ks, intx4 &z, floatx4 (&ga)[4], floatx4 (&gb)[4]) {
         wave_sync();  // the previous batch's LDS reads are done
+        const intx2 svl = L.sv[ks][lane];
 #pragma unroll
         for (int s = 0; s < 4; s++) {
-            *(floatx4 *)(L.a + (16 * s + p4) * 16 + 4 * c4) = ga[s];
+            const float vs = __int_as_float(__builtin_amdgcn_ds_bpermute((16 * s + p4) * 4, svl.y));
+            *(floatx4 *)(L.a + (16 * s + p4) * 16 + 4 * c4) = ga[s] * vs;
             *(floatx4 *)(L.b + (16 * s + p4) * 16 + 4 * c4) = gb[s];
         }
         const int ks2 = ks == 0 ? 2 : ks - 1;  // (ks + 2) mod 3
@@ -345,7 +375,7 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
         sp_issue(z, Ua, Ub, p4, c4, ga, gb);
         z = sp_rec_load(rec, b0 + 256 + lane, P1);
         wave_sync();
-        compute(b0, ks);
+        compute(svl);
     };
 
     intx4 zX = sp_rec_load(rec, P0 + lane, P1);
@@ -592,6 +622,38 @@ static int sort_mode_internal(dt_ctx *ctx, int64_t nnz, const int32_t *keys, int
     return DT_OK;
 }
 
+static int sort_rows_by_a(dt_ctx *ctx, int64_t nnz, int ba, int bs, intx4 *rec) {
+    void *buf = nullptr;
+    const size_t bytes = (size_t)nnz * (8 + 8 + 4 + 4 + 16) + 1024;
+    if (hipMalloc(&buf, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return dt_fail(ctx, DT_ENOMEM, "dt_sp_create: hipMalloc(%zu) for the record sort failed", bytes);
+    }
+    uint64_t *k1 = (uint64_t *)buf, *k2 = k1 + nnz;
+    int32_t *p1 = (int32_t *)(k2 + nnz), *p2 = p1 + nnz;
+    intx4 *tmp = (intx4 *)(((uintptr_t)(p2 + nnz) + 255) & ~(uintptr_t)255);
+    int st = DT_OK;
+    hipLaunchKernelGGL(rec_keys, sp_grid(nnz), dim3(256), 0, ctx->stream, rec, nnz, ba, k1, p1);
+    size_t tb = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k1, k2, p1, p2, (int)nnz, 0, ba + bs, ctx->stream) !=
+        hipSuccess)
+        st = DT_EHIP;
+    void *ctmp = st == DT_OK ? cub_tmp(ctx, tb) : nullptr;
+    if (st == DT_OK && !ctmp) st = DT_ENOMEM;
+    if (st == DT_OK &&
+        hipcub::DeviceRadixSort::SortPairs(ctmp, tb, k1, k2, p1, p2, (int)nnz, 0, ba + bs, ctx->stream) != hipSuccess)
+        st = DT_EHIP;
+    if (st == DT_OK) {
+        hipLaunchKernelGGL(rec_gather, sp_grid(nnz), dim3(256), 0, ctx->stream, (const intx4 *)rec, p2, nnz, tmp);
+        if (hipMemcpyAsync(rec, tmp, (size_t)nnz * 16, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+            st = DT_EHIP;
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess && st == DT_OK) st = DT_EHIP;
+    (void)hipFree(buf);
+    if (st != DT_OK && ctx->err.empty()) return dt_fail(ctx, st, "dt_sp_create: record sort failed");
+    return st;
+}
+
 extern "C" int dt_sp_sort_mode(dt_ctx *ctx, int64_t nnz, const int32_t *keys, size_t dim, int32_t *perm,
                                int32_t *rowptr) {
     DT_CHECK_ARG(ctx && keys && perm && rowptr, "dt_sp_sort_mode: null argument");
@@ -771,6 +833,13 @@ extern "C" int dt_sp_create(dt_ctx *ctx, int ndim, const size_t *dims, int64_t n
             hipLaunchKernelGGL(gather_by<float>, sp_grid(nnz), dim3(256), 0, ctx->stream, sp->val, m.perm, nnz, tval);
             hipLaunchKernelGGL(build_rec, sp_grid(nnz), dim3(256), 0, ctx->stream, m.row, tia, tib, tval, m.slot, nnz,
                                m.rec);
+            // inside each row, order the records by the index in mode a (the
+            // larger of the two gathered factors): a long row's U_a gathers
+            // then sweep U_a in order instead of jumping across it.  Rows keep
+            // their place in the stream, so the mode permutation and the row
+            // pointers are unchanged.
+            st = sort_rows_by_a(ctx, nnz, bits_for(sp->dims[a]), bits_for(std::max<int64_t>(m.nrows, 1)), m.rec);
+            if (st != DT_OK) return fail(st, "dt_sp_create: record sort failed");
         }
         if (hipGetLastError() != hipSuccess) return fail(DT_EHIP, "dt_sp_create: launch failed");
     }
@@ -832,7 +901,7 @@ static int sp_ttmc_dev(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, fl
         int dev = 0;
         DT_HIP(hipGetDevice(&dev));
         DT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        DT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, sp_ttmc16, 256, 0));
+        DT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, sp_ttmc16<SP_PF>, 256, 0));
         if (blocks_per_cu < 1) blocks_per_cu = 1;
     }
     // equal contiguous ranges, multiples of 64, two per resident wave (the
@@ -845,7 +914,7 @@ static int sp_ttmc_dev(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, fl
     float *carry = (float *)dt_ws(ctx, WS_SPARSE0, (size_t)nwaves * 2 * SP_E * 4 + 512);
     if (!carry) return DT_ENOMEM;
     static const char *names[3] = {"sp_ttmc_m0", "sp_ttmc_m1", "sp_ttmc_m2"};
-    DT_LAUNCH(names[n], sp_ttmc16, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, args, Ur[a], Ur[b], Yc, carry,
+    DT_LAUNCH(names[n], sp_ttmc16<SP_PF>, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, args, Ur[a], Ur[b], Yc, carry,
               per_wave, nwaves);
     DT_LAUNCH("sp_fixup", sp_fixup, dim3((unsigned)nwaves), dim3(256), 0, args, carry, per_wave, nwaves, Yc);
     if (L) DT_TRY(sp_ytu_dev(ctx, sp, n, Ur[n], Yc, L));
