@@ -131,14 +131,29 @@ __global__ __launch_bounds__(256) void merge_long(const int32_t *__restrict__ po
 
 // ---- mode sort ---------------------------------------------------------------
 
+// flag |= 1 when an index lies outside [0, dim)
+__global__ void index_check(const int32_t *__restrict__ idx, int64_t n, int64_t dim, int32_t *__restrict__ flag) {
+    int bad = 0;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+        bad |= idx[p] < 0 || (int64_t)idx[p] >= dim;
+    if (__ballot(bad) != 0 && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
 __global__ void iota32(int32_t *__restrict__ p, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = (int32_t)i;
 }
 
-__global__ void count_rows(const int32_t *__restrict__ k, int64_t n, int32_t *__restrict__ cnt) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd(&cnt[k[i]], 1);
+// rowptr[i] = first position whose sorted key is >= i, rowptr[dim] = n: the
+// boundary at position p (keys prev < next around it) writes the pointers of
+// rows prev + 1 .. next, so every entry is written once, without atomics.
+__global__ void row_bounds(const int32_t *__restrict__ sorted, int64_t n, int64_t dim, int32_t *__restrict__ rowptr) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p <= n; p += (int64_t)gridDim.x * blockDim.x) {
+        // clamped to [-1, dim] so that keys out of range cannot write past rowptr
+        const int64_t prev = p == 0 ? -1 : min((int64_t)sorted[p - 1], dim);
+        const int64_t next = p == n ? dim : min(max((int64_t)sorted[p], (int64_t)-1), dim);
+        for (int64_t i = prev + 1; i <= next; i++) rowptr[i] = (int32_t)p;
+    }
 }
 
 __global__ void nonempty_flags(const int32_t *__restrict__ rowptr, int64_t I, int32_t *__restrict__ f) {
@@ -204,7 +219,6 @@ __device__ __forceinline__ void wave_sync() {
 // Per-wave LDS: the staged batch's U_a and U_b rows [nonzero][column] and a
 // ring of three batches' (compact row, value) pairs.
 constexpr int SP_RING = 3;
-constexpr int SP_PF = 8;  // MFMA steps whose operands are read from LDS together
 struct alignas(16) SpLds {
     float a[64 * 16];
     float b[64 * 16];
@@ -250,7 +264,6 @@ __device__ __forceinline__ void sp_issue(const intx4 &z, const float *__restrict
 // layout, so a row end never waits for an MFMA.  A finished row is written to Y_c when it lies
 // inside the range; otherwise it goes to carry slot 0 (row started before
 // P0) or 1 (row continues past P1), and sp_fixup adds them.
-template <int PF>
 __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__restrict__ Ua,
                                                  const float *__restrict__ Ub, float *__restrict__ Yc,
                                                  float *__restrict__ carry, int64_t per_wave, int64_t nwaves) {
@@ -283,10 +296,9 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
         __builtin_nontemporal_store((acc0 + acc1) + vacc, (floatx4 *)(dst + e0));
     };
 
-    // svl: lane k holds (row, value) of nonzero k of the batch; the staged U_a
-    // rows are already scaled by the values
-    auto compute = [&](const intx2 &svl) {
-        // rows starting in this batch (padding has row -1)
+    auto compute = [&](int64_t b0, int ks) {
+        // rows starting in this batch (lane k: nonzero k; padding has row -1)
+        const intx2 svl = L.sv[ks][lane];
         const int prevr = __builtin_amdgcn_ds_bpermute((lane == 0 ? 0 : lane - 1) * 4, svl.x);
         const uint64_t heads = __ballot(svl.x >= 0 && svl.x != (lane == 0 ? cur : prevr));
         if (__popcll(heads) > 1) {
@@ -298,8 +310,9 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
             for (;;) {
                 const int h = hm ? __builtin_ctzll(hm) : nvalid;
                 for (int kk = k; kk < h; kk++) {
+                    const float vv = __int_as_float(__builtin_amdgcn_readlane(svl.y, kk));
                     const floatx4 ua = *(const floatx4 *)(L.a + kk * 16 + 4 * g);
-                    const float wb = L.b[kk * 16 + c];
+                    const float wb = vv * L.b[kk * 16 + c];
                     vacc[0] = fmaf(ua[0], wb, vacc[0]);
                     vacc[1] = fmaf(ua[1], wb, vacc[1]);
                     vacc[2] = fmaf(ua[2], wb, vacc[2]);
@@ -316,32 +329,24 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
             }
             return;
         }
-        // operand reads of PF steps at a time, so their LDS latency overlaps
-#pragma unroll
-        for (int t0 = 0; t0 < 16; t0 += PF) {
-        float av[PF], bv[PF];
-#pragma unroll
-        for (int u = 0; u < PF; u++) {
-            av[u] = L.a[(4 * (t0 + u) + g) * 16 + c];
-            bv[u] = L.b[(4 * (t0 + u) + g) * 16 + c];
-        }
-#pragma unroll
-        for (int u = 0; u < PF; u++) {
-            const int t = t0 + u;
-            const int r0 = __builtin_amdgcn_readlane(svl.x, 4 * t);
-            const int r3 = __builtin_amdgcn_readlane(svl.x, 4 * t + 3);
+#pragma unroll 2
+        for (int t = 0; t < 16; t++) {
+            const float ua = L.a[(4 * t + g) * 16 + c], bv = L.b[(4 * t + g) * 16 + c];
+            const intx2 sv = L.sv[ks][4 * t + g];
+            const float av = __int_as_float(sv.y) * ua;
+            const int r0 = __builtin_amdgcn_readlane(sv.x, 0);
+            const int r3 = __builtin_amdgcn_readlane(sv.x, 48);
             if (r0 == cur && r3 == cur) {
                 if (t & 1)
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], acc1, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc1, 0, 0, 0);
                 else
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], acc0, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc0, 0, 0, 0);
             } else {
-                const int myrow = __builtin_amdgcn_ds_bpermute((4 * t + g) * 4, svl.x);
 #pragma unroll 1
                 for (int gg = 0; gg < 4; gg++) {
-                    const int r = __builtin_amdgcn_readlane(svl.x, 4 * t + gg);
+                    const int r = __builtin_amdgcn_readlane(sv.x, 16 * gg);
                     if (r < 0) break;
-                    if (gg > 0 && r == __builtin_amdgcn_readlane(svl.x, 4 * t + gg - 1)) continue;
+                    if (gg > 0 && r == __builtin_amdgcn_readlane(sv.x, 16 * gg - 16)) continue;
                     if (r != cur) {
                         flush(false);
                         cur = r;
@@ -349,12 +354,10 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
                         acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
                         vacc = floatx4{0.f, 0.f, 0.f, 0.f};
                     }
-                    const bool mine = myrow == r;
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(mine ? av[u] : 0.f, mine ? bv[u] : 0.f, acc0, 0, 0,
-                                                                0);
+                    const bool mine = sv.x == r;
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(mine ? av : 0.f, mine ? bv : 0.f, acc0, 0, 0, 0);
                 }
             }
-        }
         }
     };
 
@@ -362,11 +365,9 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
     // z holds the records of batch k + 2, ga/gb the gathers of batch k
     auto step = [&](int64_t b0, int ks, intx4 &z, floatx4 (&ga)[4], floatx4 (&gb)[4]) {
         wave_sync();  // the previous batch's LDS reads are done
-        const intx2 svl = L.sv[ks][lane];
 #pragma unroll
         for (int s = 0; s < 4; s++) {
-            const float vs = __int_as_float(__builtin_amdgcn_ds_bpermute((16 * s + p4) * 4, svl.y));
-            *(floatx4 *)(L.a + (16 * s + p4) * 16 + 4 * c4) = ga[s] * vs;
+            *(floatx4 *)(L.a + (16 * s + p4) * 16 + 4 * c4) = ga[s];
             *(floatx4 *)(L.b + (16 * s + p4) * 16 + 4 * c4) = gb[s];
         }
         const int ks2 = ks == 0 ? 2 : ks - 1;  // (ks + 2) mod 3
@@ -375,7 +376,7 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
         sp_issue(z, Ua, Ub, p4, c4, ga, gb);
         z = sp_rec_load(rec, b0 + 256 + lane, P1);
         wave_sync();
-        compute(svl);
+        compute(b0, ks);
     };
 
     intx4 zX = sp_rec_load(rec, P0 + lane, P1);
@@ -405,11 +406,12 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
 
 // Rows cut by wave ranges: the wave whose range holds the row's first nonzero
 // adds slot 1 of its own range and slot 0 of every later range the row
-// reaches, in range order (four 64-thread groups take every fourth range,
-// then are added in group order).
-__global__ __launch_bounds__(256) void sp_fixup(SpModeArgs a, const float *__restrict__ carry, int64_t per_wave,
-                                                int64_t nwaves, float *__restrict__ Yc) {
-    __shared__ floatx4 part[4][64];
+// reaches.  A Zipf-hot row spans thousands of ranges, so sixteen 64-thread
+// groups take every sixteenth range (four loads in flight each) and are then
+// added in group order: a fixed order for a given range size.
+__global__ __launch_bounds__(1024) void sp_fixup(SpModeArgs a, const float *__restrict__ carry, int64_t per_wave,
+                                                 int64_t nwaves, float *__restrict__ Yc) {
+    __shared__ floatx4 part[16][64];
     const int64_t w = blockIdx.x;
     const int64_t P0 = w * per_wave;
     const int64_t P1 = P0 + per_wave < a.nnz ? P0 + per_wave : a.nnz;
@@ -419,15 +421,23 @@ __global__ __launch_bounds__(256) void sp_fixup(SpModeArgs a, const float *__res
     if (re <= P1 || rs < P0) return;  // ends inside, or owned by an earlier range
     const int64_t we = (re - 1) / per_wave;
     const int grp = threadIdx.x >> 6, t = threadIdx.x & 63;
-    floatx4 s = {0.f, 0.f, 0.f, 0.f};
-    for (int64_t q = w + grp; q <= we; q += 4) {
-        const int sl = q == w ? 1 : 0;
-        s += *(const floatx4 *)(carry + (q * 2 + sl) * SP_E + 4 * t);
+    auto slot = [&](int64_t q) {
+        return *(const floatx4 *)(carry + (q * 2 + (q == w ? 1 : 0)) * SP_E + 4 * t);
+    };
+    floatx4 s0 = {0.f, 0.f, 0.f, 0.f}, s1 = s0, s2 = s0, s3 = s0;
+    int64_t q = w + grp;
+    for (; q + 48 <= we; q += 64) {
+        s0 += slot(q);
+        s1 += slot(q + 16);
+        s2 += slot(q + 32);
+        s3 += slot(q + 48);
     }
-    part[grp][t] = s;
+    for (; q <= we; q += 16) s0 += slot(q);
+    part[grp][t] = (s0 + s1) + (s2 + s3);
     __syncthreads();
     if (grp == 0) {
-        const floatx4 tot = ((part[0][t] + part[1][t]) + part[2][t]) + part[3][t];
+        floatx4 tot = part[0][t];
+        for (int gq = 1; gq < 16; gq++) tot += part[gq][t];
         *(floatx4 *)(Yc + (int64_t)a.slot[r] * SP_E + 4 * t) = tot;
     }
 }
@@ -493,27 +503,39 @@ __global__ __launch_bounds__(256) void sp_lreduce(const float *__restrict__ Lwg,
 }
 
 // W (nrows x 16) = Y_c L, scattered to U (I x 16 row-major FP64) rows; U is
-// pre-zeroed for empty rows.  One wave per row (lane: 4 entries).
+// pre-zeroed for empty rows.  v_mfma_f32_16x16x4_f32 over tiles of 16 compact
+// rows (one wave per tile), K = 256: lane (i, g) loads entries
+// 16q + 4g .. 16q + 4g + 3 of row i and feeds entry 16q + 4g + j at step (q, j)
+// as A[i][g], with B[g][n] = L[16q + 4g + j][n] from LDS (rows padded to 20
+// floats so the four groups read disjoint banks).
 __global__ __launch_bounds__(256) void sp_times_l(const float *__restrict__ Yc, const int32_t *__restrict__ rows,
                                                   int64_t nrows, const double *__restrict__ L,
                                                   double *__restrict__ Uout) {
-    __shared__ float Lsh[SP_E * 16];
-    for (int i = threadIdx.x; i < SP_E * 16; i += 256) Lsh[i] = (float)L[i];
+    __shared__ float Ls[SP_E][20];
+    for (int e = threadIdx.x; e < SP_E * 16; e += 256) Ls[e / 16][e % 16] = (float)L[e];
     __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int64_t nw = (int64_t)gridDim.x * 4;
-    for (int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); s < nrows; s += nw) {
-        const floatx4 y = *(const floatx4 *)(Yc + s * SP_E + 4 * lane);
-        float o[16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = lane & 15, g = lane >> 4;
+    const int64_t ntiles = (nrows + 15) / 16;
+    for (int64_t t = (int64_t)blockIdx.x * 4 + wv; t < ntiles; t += (int64_t)gridDim.x * 4) {
+        const int64_t r = t * 16 + i;
+        const bool ok = r < nrows;
+        const floatx4 *yr = (const floatx4 *)(Yc + (ok ? r : nrows - 1) * SP_E);
+        floatx4 y[16];
 #pragma unroll
-        for (int k = 0; k < 16; k++)
-            o[k] = y[0] * Lsh[(4 * lane + 0) * 16 + k] + y[1] * Lsh[(4 * lane + 1) * 16 + k] +
-                   y[2] * Lsh[(4 * lane + 2) * 16 + k] + y[3] * Lsh[(4 * lane + 3) * 16 + k];
+        for (int q = 0; q < 16; q++) y[q] = ok ? yr[4 * q + g] : floatx4{0.f, 0.f, 0.f, 0.f};
+        floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int k = 0; k < 16; k++)
+        for (int q = 0; q < 16; q++)
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) o[k] += __shfl_xor(o[k], off);
-        if (lane < 16) Uout[(int64_t)rows[s] * 16 + lane] = (double)o[lane];
+            for (int j = 0; j < 4; j++)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(y[q][j], Ls[16 * q + 4 * g + j][i], acc, 0, 0, 0);
+        // lane (n = i, g), register v: W[row 16t + 4g + v][n]
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int64_t rr = t * 16 + 4 * g + v;
+            if (rr < nrows) Uout[(int64_t)rows[rr] * 16 + i] = (double)acc[v];
+        }
     }
 }
 
@@ -609,16 +631,8 @@ static int sort_mode_internal(dt_ctx *ctx, int64_t nnz, const int32_t *keys, int
     if (!tmp) return DT_ENOMEM;
     DT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, (const uint32_t *)keys, (uint32_t *)sorted_keys, iota, perm,
                                               (int)nnz, 0, nb, ctx->stream));
-    // row pointers: histogram, exclusive scan (int32: nnz < 2^31)
-    int32_t *cnt = (int32_t *)dt_ws(ctx, WS_SPARSE1, (size_t)(dim + 1) * 4 + 256);
-    if (!cnt) return DT_ENOMEM;
-    DT_TRY(dt_zero(ctx, cnt, (size_t)(dim + 1) * 4));
-    DT_LAUNCH("sp_sort", count_rows, sp_grid(nnz), dim3(256), 0, keys, nnz, cnt);
-    size_t tb2 = 0;
-    DT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, cnt, rowptr, (int)(dim + 1), ctx->stream));
-    tmp = cub_tmp(ctx, tb2);
-    if (!tmp) return DT_ENOMEM;
-    DT_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb2, cnt, rowptr, (int)(dim + 1), ctx->stream));
+    // row pointers from the sorted keys (int32: nnz < 2^31)
+    DT_LAUNCH("sp_sort", row_bounds, sp_grid(nnz + 1), dim3(256), 0, (const int32_t *)sorted_keys, nnz, dim, rowptr);
     return DT_OK;
 }
 
@@ -784,6 +798,22 @@ extern "C" int dt_sp_create(dt_ctx *ctx, int ndim, const size_t *dims, int64_t n
         if (hipMemcpyAsync(sp->idx[n], idx[n], (size_t)nnz * 4, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
             return fail(DT_EHIP, "dt_sp_create: copy failed");
     }
+    {
+        // every index inside its mode (the kernels gather and scatter by them)
+        int32_t *flag = (int32_t *)dt_ws(ctx, WS_STATUS, 64);
+        if (!flag) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
+        if (dt_zero(ctx, flag, 4) != DT_OK) return fail(DT_EHIP, "dt_sp_create: zero failed");
+        for (int n = 0; n < ndim; n++)
+            hipLaunchKernelGGL(index_check, sp_grid(nnz), dim3(256), 0, ctx->stream, sp->idx[n], nnz, sp->dims[n], flag);
+        int32_t hflag = 0;
+        if (hipMemcpyAsync(&hflag, flag, 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            return fail(DT_EHIP, "dt_sp_create: index check failed");
+        if (hflag) {
+            dt_sp_destroy(sp);
+            return dt_fail(ctx, DT_EINVAL, "dt_sp_create: an index lies outside its mode's range");
+        }
+    }
     sp->val = (float *)sp_alloc(sp, (size_t)nnz * 4);
     if (!sp->val) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
     if (hipMemcpyAsync(sp->val, val, (size_t)nnz * 4, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
@@ -901,22 +931,22 @@ static int sp_ttmc_dev(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, fl
         int dev = 0;
         DT_HIP(hipGetDevice(&dev));
         DT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        DT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, sp_ttmc16<SP_PF>, 256, 0));
+        DT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, sp_ttmc16, 256, 0));
         if (blocks_per_cu < 1) blocks_per_cu = 1;
     }
-    // equal contiguous ranges, multiples of 64, two per resident wave (the
-    // hardware hands a range to whichever wave finishes first, which evens
-    // out ranges of different cost)
-    const int64_t maxw = (int64_t)cus * blocks_per_cu * 4 * 2;
+    // equal contiguous ranges, multiples of 64, about eight per resident wave
+    // (the hardware hands a range to whichever wave finishes first, which
+    // evens out ranges of different cost: short-row stretches take longer)
+    const int64_t maxw = (int64_t)cus * blocks_per_cu * 4 * 8;
     int64_t per_wave = ((sp->nnz + maxw - 1) / maxw + 63) / 64 * 64;
     if (per_wave < 64) per_wave = 64;
     const int64_t nwaves = (sp->nnz + per_wave - 1) / per_wave;
     float *carry = (float *)dt_ws(ctx, WS_SPARSE0, (size_t)nwaves * 2 * SP_E * 4 + 512);
     if (!carry) return DT_ENOMEM;
     static const char *names[3] = {"sp_ttmc_m0", "sp_ttmc_m1", "sp_ttmc_m2"};
-    DT_LAUNCH(names[n], sp_ttmc16<SP_PF>, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, args, Ur[a], Ur[b], Yc, carry,
+    DT_LAUNCH(names[n], sp_ttmc16, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, args, Ur[a], Ur[b], Yc, carry,
               per_wave, nwaves);
-    DT_LAUNCH("sp_fixup", sp_fixup, dim3((unsigned)nwaves), dim3(256), 0, args, carry, per_wave, nwaves, Yc);
+    DT_LAUNCH("sp_fixup", sp_fixup, dim3((unsigned)nwaves), dim3(1024), 0, args, carry, per_wave, nwaves, Yc);
     if (L) DT_TRY(sp_ytu_dev(ctx, sp, n, Ur[n], Yc, L));
     return DT_OK;
 }
@@ -988,6 +1018,7 @@ extern "C" int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int m
     }
     int st = DT_OK;
     auto views = [&](int m) {
+        dt_ktimer kt(ctx, "sp_views");
         hipLaunchKernelGGL(rows16_view, sp_grid(sp->dims[m]), dim3(256), 0, ctx->stream, U64[m], sp->dims[m],
                            Rk[m], U32r[m]);
         return hipGetLastError() == hipSuccess ? DT_OK : DT_EHIP;
@@ -1016,8 +1047,12 @@ extern "C" int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int m
             st = sp_ttmc_dev(ctx, sp, n, Ur, Yc, L);           // Y_c and L = Y_cᵀ U_n
             if (st == DT_OK) st = dt_zero(ctx, Urow64, (size_t)I * R * 8);
             if (st == DT_OK) {
-                hipLaunchKernelGGL(sp_times_l, dim3((unsigned)std::min<int64_t>((md.nrows + 3) / 4, 4096)), dim3(256),
-                                   0, ctx->stream, Yc, md.rows, md.nrows, L, Urow64);
+                {
+                    dt_ktimer kt(ctx, "sp_times_l");
+                    hipLaunchKernelGGL(sp_times_l, dim3((unsigned)std::min<int64_t>((md.nrows + 63) / 64, 4096)),
+                                       dim3(256), 0, ctx->stream, Yc, md.rows, md.nrows, L, Urow64);
+                }
+                dt_ktimer kt(ctx, "sp_views");
                 hipLaunchKernelGGL(rowmajor_to_colmajor, sp_grid(I), dim3(256), 0, ctx->stream, Urow64, I,
                                    Rk[n], U64[n]);
                 st = hipGetLastError() == hipSuccess ? DT_OK : DT_EHIP;

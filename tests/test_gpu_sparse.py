@@ -145,6 +145,21 @@ def test_sparse_bad_ranks_einval(ctx):
         sp.close()
 
 
+def test_sp_create_rejects_out_of_range_index(ctx):
+    """An index outside its mode is refused at creation (the TTMc gathers by it)."""
+    idx, vals = make_coo(np.random.default_rng(2), (40, 30, 20), 500)
+    bad = [i.copy() for i in idx]
+    bad[1][7] = 30
+    with pytest.raises(dt.DTError) as e:
+        dt.SpTensor.from_host(ctx, (40, 30, 20), bad, vals)
+    assert e.value.code == dt.DT_EINVAL
+    bad[1][7] = -1
+    with pytest.raises(dt.DTError):
+        dt.SpTensor.from_host(ctx, (40, 30, 20), bad, vals)
+    sp = dt.SpTensor.from_host(ctx, (40, 30, 20), idx, vals)  # the context still works
+    sp.close()
+
+
 def test_c4_full_size_generator_and_sort(ctx):
     """The sparse workload at its real size: 2^20 x 2^18 x 2^16, 1e8 Zipf(1.1)
     draws.  Size-independent checks: ~65.6 % of the draws survive the merge,
