@@ -112,8 +112,17 @@ __global__ void gemm_reduce(const double *__restrict__ part, int nsplit, int64_t
     const int64_t total = m * n;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
-        double s = 0.0;
-        for (int q = 0; q < nsplit; q++) s += part[(int64_t)q * total + idx];
+        // four independent sums so that four loads are in flight
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int q = 0;
+        for (; q + 3 < nsplit; q += 4) {
+            a0 += part[(int64_t)q * total + idx];
+            a1 += part[(int64_t)(q + 1) * total + idx];
+            a2 += part[(int64_t)(q + 2) * total + idx];
+            a3 += part[(int64_t)(q + 3) * total + idx];
+        }
+        for (; q < nsplit; q++) a0 += part[(int64_t)q * total + idx];
+        const double s = (a0 + a1) + (a2 + a3);
         const int64_t i = idx % m, j = idx / m;
         const double old = beta == 0.0 ? 0.0 : beta * C[i + ldc * j];
         C[i + ldc * j] = alpha * s + old;
@@ -314,43 +323,48 @@ __global__ void set_identity(double *__restrict__ A, int n) {
 }
 
 // ---- tall-skinny pieces of Cholesky-QR (n <= TS_NMAX, m large) ---------------
-// Partial Grams on v_mfma_f64_16x16x4f64: workgroup w covers a contiguous row
-// range, staged TS_ROWS rows at a time through LDS with column-coalesced loads
-// (columns n..TS_NMAX-1 zero); wave v takes rows 32v..32v+31 of each block,
-// four rows per MFMA, lane (c, g) holding Q[r_g, c] (and Q[r_g, 16 + c] when
-// n > 16) as both operands, so D accumulates the 16 x 16 Gram tiles.  The four
-// waves' tiles are added in wave order and written to part[w][i + n j].
-constexpr int TS_NMAX = 32, TS_ROWS = 128;
+// Partial Grams on v_mfma_f64_16x16x4f64: each wave streams a contiguous run of
+// rows straight from global memory, 16 rows per iteration, lane (c, g) loading
+// Q[r + 4g + j, c] (and Q[r + 4g + j, 16 + c] when n > 16) for j = 0..3 and
+// feeding it as both operands of step j, so D accumulates the 16 x 16 Gram
+// tiles.  The four waves' tiles are added in wave order and written to
+// part[block][i + n j].
+constexpr int TS_NMAX = 32;
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 template <bool HI>
 __global__ __launch_bounds__(256) void ts_gram_kernel(const double *__restrict__ Q, int64_t m, int n,
-                                                      int64_t rows_per_wg, double *__restrict__ part) {
-    constexpr int NC = HI ? 32 : 16;
+                                                      int64_t rows_per_wave, double *__restrict__ part) {
     constexpr int NT = HI ? 3 : 1;  // tiles (lo,lo), (lo,hi), (hi,hi)
-    __shared__ double Qs[TS_ROWS][NC + 1];
     __shared__ double red[4][NT][4][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int c = lane & 15, g = lane >> 4;
-    const int64_t r0 = (int64_t)blockIdx.x * rows_per_wg;
-    const int64_t r1 = r0 + rows_per_wg < m ? r0 + rows_per_wg : m;
+    const int64_t w = (int64_t)blockIdx.x * 4 + wv;
+    const int64_t r0 = w * rows_per_wave;
+    const int64_t r1 = r0 + rows_per_wave < m ? r0 + rows_per_wave : m;
+    const bool clo = c < n, chi = HI && c + 16 < n;
+    const double *__restrict__ qlo = Q + m * (clo ? c : 0);
+    const double *__restrict__ qhi = Q + m * (chi ? c + 16 : 0);
     doublex4 dll = {0.0, 0.0, 0.0, 0.0}, dlh = {0.0, 0.0, 0.0, 0.0}, dhh = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t rb = r0; rb < r1; rb += TS_ROWS) {
-        const int nr = rb + TS_ROWS < r1 ? TS_ROWS : (int)(r1 - rb);
-        __syncthreads();
-        for (int e = threadIdx.x; e < TS_ROWS * NC; e += 256) {
-            const int r = e % TS_ROWS, col = e / TS_ROWS;
-            Qs[r][col] = (r < nr && col < n) ? Q[rb + r + m * col] : 0.0;
-        }
-        __syncthreads();
+    // 64 rows per iteration: every load is issued (address clamped into the
+    // matrix) before any is used, and rows past r1 or columns past n read as 0
+    for (int64_t r = r0; r < r1; r += 64) {
+        double lo[16], hi[16];
 #pragma unroll
-        for (int s = 0; s < 8; s++) {
-            const int r = 32 * wv + 4 * s + g;
-            const double lo = Qs[r][c];
-            dll = __builtin_amdgcn_mfma_f64_16x16x4f64(lo, lo, dll, 0, 0, 0);
+        for (int j = 0; j < 16; j++) {
+            const int64_t rr = r + 16 * (j >> 2) + 4 * g + (j & 3);
+            const int64_t rc = rr < m ? rr : m - 1;
+            lo[j] = qlo[rc];
+            if (HI) hi[j] = qhi[rc];
+        }
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int64_t rr = r + 16 * (j >> 2) + 4 * g + (j & 3);
+            const double a = (clo && rr < r1) ? lo[j] : 0.0;
+            dll = __builtin_amdgcn_mfma_f64_16x16x4f64(a, a, dll, 0, 0, 0);
             if (HI) {
-                const double hi = Qs[r][16 + c];
-                dlh = __builtin_amdgcn_mfma_f64_16x16x4f64(lo, hi, dlh, 0, 0, 0);
-                dhh = __builtin_amdgcn_mfma_f64_16x16x4f64(hi, hi, dhh, 0, 0, 0);
+                const double b = (chi && rr < r1) ? hi[j] : 0.0;
+                dlh = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, dlh, 0, 0, 0);
+                dhh = __builtin_amdgcn_mfma_f64_16x16x4f64(b, b, dhh, 0, 0, 0);
             }
         }
     }
@@ -385,10 +399,16 @@ __global__ __launch_bounds__(256) void part_reduce_kernel(const double *__restri
     __shared__ double red[16][17];
     const int c = threadIdx.x & 15, grp = threadIdx.x >> 4;
     const int e = blockIdx.x * 16 + c;
-    double s = 0.0;
-    if (e < ne)
-        for (int p = grp; p < nparts; p += 16) s += part[(int64_t)p * ne + e];
-    red[grp][c] = s;
+    // eight independent sums per thread so that eight loads are in flight
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (e < ne) {
+        int p = grp;
+        for (; p + 7 * 16 < nparts; p += 8 * 16)
+#pragma unroll
+            for (int u = 0; u < 8; u++) a[u] += part[(int64_t)(p + 16 * u) * ne + e];
+        for (; p < nparts; p += 16) a[0] += part[(int64_t)p * ne + e];
+    }
+    red[grp][c] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     __syncthreads();
     if (grp == 0 && e < ne) {
         double t = 0.0;
@@ -426,9 +446,11 @@ __global__ __launch_bounds__(256) void ts_rmul_kernel(double *__restrict__ Q, in
 
 // G = QᵀQ (n x n) for a tall-skinny column-major Q, FP64, fixed summation order.
 static int ts_gram(dt_ctx *ctx, const double *Q, size_t m, size_t n, double *G) {
-    const int64_t nwg = std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div((int64_t)m, 1024)));
-    const int64_t rpw = ceil_div((int64_t)m, nwg);
-    const int64_t ng = ceil_div((int64_t)m, rpw);
+    // runs of 64-row chunks, at most 4096 waves
+    const int64_t chunks = ceil_div((int64_t)m, 64);
+    const int64_t nwaves = std::max<int64_t>(1, std::min<int64_t>(4096, ceil_div(chunks, 4)));
+    const int64_t rpw = ceil_div(chunks, nwaves) * 64;
+    const int64_t ng = ceil_div(ceil_div((int64_t)m, rpw), 4);
     double *part = (double *)dt_ws(ctx, WS_GEMM_PART, (size_t)(ng * n * n) * sizeof(double));
     if (!part) return DT_ENOMEM;
     if (n <= 16) {

@@ -491,9 +491,14 @@ __global__ __launch_bounds__(256) void sp_lreduce(const float *__restrict__ Lwg,
     __shared__ double red[16][17];
     const int c = threadIdx.x & 15, grp = threadIdx.x >> 4;
     const int i = blockIdx.x * 16 + c;
-    double s = 0.0;
-    for (int w = grp; w < nwg; w += 16) s += (double)Lwg[(int64_t)w * SP_E * 16 + i];
-    red[grp][c] = s;
+    // eight independent sums per thread so that eight loads are in flight
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int w = grp;
+    for (; w + 7 * 16 < nwg; w += 8 * 16)
+#pragma unroll
+        for (int u = 0; u < 8; u++) a[u] += (double)Lwg[(int64_t)(w + 16 * u) * SP_E * 16 + i];
+    for (; w < nwg; w += 16) a[0] += (double)Lwg[(int64_t)w * SP_E * 16 + i];
+    red[grp][c] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     __syncthreads();
     if (grp == 0) {
         double t = 0.0;
