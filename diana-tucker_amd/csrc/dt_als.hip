@@ -328,6 +328,197 @@ static int launch_chol(dt_ctx *ctx, const double *part, int nparts, int R, int I
     return DT_OK;
 }
 
+// ---- the whole update in one workgroup (I * CM <= AF_MAXIR) -------------------
+// Y and W (I x CM, row-major, columns R..CM-1 zero) live in LDS for all the
+// steps; W = S Q runs on v_mfma_f64_16x16x4f64 (wave w takes 16-row tiles
+// w, w + 16, ...; lane (c, g) holds S[row c][k + g] as A and Q[k + g][c] as B;
+// output register v of lane (c, g) is row 4v + g, column c), the Gram, the
+// register Cholesky and the triangular products run on the VALU.  The same
+// steps as the multi-workgroup path:
+//   Q_0 = U;  for k < inner: W = S Q_k, R_k = chol(WᵀW + shift), Q_{k+1} = W R_k^{-1};
+//   then R = chol(QᵀQ), U = Q R^{-1}.
+constexpr int AF_THREADS = 512, AF_MAXIR = 8192;
+
+template <int CM>
+__device__ __forceinline__ void af_chol(double (*G)[CM + 1], double (*Ri)[CM + 1], double (*X)[CM + 1], int R,
+                                        int I, int shifted, int *flag) {
+    // G (R x R, LDS) -> Ri = chol(G + shift)^{-1} (CM x CM, identity-padded); one wave
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        double shift = 0.0;
+        if (shifted) {
+            double tr = 0.0;
+            for (int i = 0; i < R; i++) tr += G[i][i];
+            shift = 11.0 * ((double)I * R + (double)R * (R + 1)) * 1.1102230246251565e-16 * tr;
+        }
+        const int k = tid < CM ? tid : CM - 1;
+        double col[CM];
+#pragma unroll
+        for (int i = 0; i < CM; i++)
+            col[i] = (i < R && k < R) ? G[i][k] + (i == k ? shift : 0.0) : (i == k ? 1.0 : 0.0);
+        int bad = 0;
+#pragma unroll
+        for (int j = 0; j < CM; j++) {
+            double d = rl_f64(col[j], j);
+            if (!(d > 0.0) || !isfinite(d)) {
+                if (!bad) bad = j + 1;
+                d = 1.0;
+            }
+            const double rjj = sqrt(d), inv_d = 1.0 / d;
+            const double gjk = col[j];
+#pragma unroll
+            for (int i = j + 1; i < CM; i++) col[i] -= rl_f64(col[j], i) * gjk * inv_d;
+            col[j] = (k == j) ? rjj : gjk / rjj;
+        }
+        if (tid < CM)
+#pragma unroll
+            for (int i = 0; i < CM; i++) X[i][k] = col[i];
+        if (bad && tid == 0) atomicOr(flag, 4);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double x[CM];
+#pragma unroll
+        for (int i = CM - 1; i >= 0; i--) {
+            double s0 = (i == k) ? 1.0 : 0.0, s1 = 0.0;
+#pragma unroll
+            for (int m = i + 1; m < CM; m += 2) {
+                s0 -= X[i][m] * x[m];
+                if (m + 1 < CM) s1 -= X[i][m + 1] * x[m + 1];
+            }
+            x[i] = (s0 + s1) / X[i][i];
+        }
+        if (tid < CM)
+#pragma unroll
+            for (int i = 0; i < CM; i++) Ri[i][k] = (i < R && k < R) ? x[i] : (i == k ? 1.0 : 0.0);
+    }
+}
+
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+template <int CM>
+__global__ __launch_bounds__(AF_THREADS) void als_fused_kernel(const double *__restrict__ S, double *__restrict__ U,
+                                                               int I, int R, int inner, int *__restrict__ flag,
+                                                               float *__restrict__ U32, float *__restrict__ M32,
+                                                               uint16_t *__restrict__ Ub16) {
+    constexpr int NW = AF_THREADS / 64, NCT = CM / 16;
+    static_assert(NW * NCT * NCT * 256 <= AF_MAXIR, "Gram tiles must fit in a panel");
+    __shared__ double Y[AF_MAXIR], W[AF_MAXIR];
+    __shared__ double G[CM][CM + 1], Ri[CM][CM + 1], X[CM][CM + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int n = I * CM, ntile = (I + 15) / 16;
+    for (int e = tid; e < n; e += AF_THREADS) {
+        const int i = e / CM, r = e % CM;
+        Y[e] = r < R ? U[i + (size_t)I * r] : 0.0;
+    }
+    for (int e = tid; e < CM * CM; e += AF_THREADS) Ri[e / CM][e % CM] = (e / CM == e % CM) ? 1.0 : 0.0;
+    __syncthreads();
+    // dst = src Ri: per 16-row tile and column tile, K = CM on MFMA
+    auto times_ri = [&](const double *src, double *dst) {
+        for (int t = wv; t < ntile * NCT; t += NW) {
+            const int rt = t % ntile, ct = t / ntile;
+            const int row = 16 * rt + c;
+            doublex4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k0 = 0; k0 < CM; k0 += 4) {
+                const double a = row < I ? src[row * CM + k0 + g] : 0.0;
+                d = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Ri[k0 + g][16 * ct + c], d, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = 16 * rt + 4 * v + g;
+                if (i < I) dst[i * CM + 16 * ct + c] = d[v];
+            }
+        }
+    };
+    // G = srcᵀ src (R x R): every wave sums its own rows on MFMA, then the waves
+    // are added in order; the per-wave tiles go to `scratch` (the other panel,
+    // NW * NCT^2 * 256 <= AF_MAXIR doubles)
+    auto gram = [&](const double *src, double *scratch) {
+        double(*Gp)[NCT * NCT][4][64] = (double(*)[NCT * NCT][4][64])scratch;
+        doublex4 d[NCT * NCT];
+#pragma unroll
+        for (int q = 0; q < NCT * NCT; q++) d[q] = doublex4{0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 4 * wv; k0 < I; k0 += 4 * NW) {
+            const int kk = k0 + g;
+            double v[NCT];
+#pragma unroll
+            for (int h = 0; h < NCT; h++) v[h] = kk < I ? src[kk * CM + 16 * h + c] : 0.0;
+#pragma unroll
+            for (int a = 0; a < NCT; a++)
+#pragma unroll
+                for (int b = 0; b < NCT; b++)
+                    d[a * NCT + b] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[a], v[b], d[a * NCT + b], 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < NCT * NCT; q++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) Gp[wv][q][v][lane] = d[q][v];
+        __syncthreads();
+        for (int e = tid; e < NCT * NCT * 256; e += AF_THREADS) {
+            const int q = e / 256, v = (e / 64) % 4, l = e % 64;
+            double sum = 0.0;
+            for (int w = 0; w < NW; w++) sum += Gp[w][q][v][l];
+            // lane l = (col, grp), register v: tile row 4v + grp, column col
+            const int a = (q / NCT) * 16 + 4 * v + (l >> 4), b = (q % NCT) * 16 + (l & 15);
+            if (a < R && b < R) G[a][b] = sum;
+        }
+    };
+    for (int k = 0;; k++) {
+        times_ri(Y, W);  // Q_k = Y R^{-1}
+        __syncthreads();
+        if (k == inner) break;
+        // Y = S Q_k on MFMA, 16 steps of S loads in flight
+        for (int t = wv; t < ntile * NCT; t += NW) {
+            const int rt = t % ntile, ct = t / ntile;
+            const int row = 16 * rt + c;
+            const int rowc = row < I ? row : I - 1;
+            doublex4 d = {0.0, 0.0, 0.0, 0.0};
+            for (int k0 = 0; k0 < I; k0 += 64) {
+                double a[16];
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    const int kk = k0 + 4 * j + g;
+                    a[j] = S[(size_t)rowc + (size_t)I * (kk < I ? kk : I - 1)];
+                }
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    const int kk = k0 + 4 * j + g;
+                    const bool ok = row < I && kk < I;
+                    const double b = kk < I ? W[kk * CM + 16 * ct + c] : 0.0;
+                    d = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? a[j] : 0.0, b, d, 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = 16 * rt + 4 * v + g;
+                if (i < I) Y[i * CM + 16 * ct + c] = d[v];
+            }
+        }
+        __syncthreads();
+        gram(Y, W);
+        __syncthreads();
+        af_chol<CM>(G, Ri, X, R, I, 1, flag);
+        __syncthreads();
+    }
+    // W = Q_inner; R = chol(QᵀQ); U = Q R^{-1}
+    gram(W, Y);
+    __syncthreads();
+    af_chol<CM>(G, Ri, X, R, I, 0, flag);
+    __syncthreads();
+    times_ri(W, Y);
+    __syncthreads();
+    for (int e = tid; e < I * R; e += AF_THREADS) {
+        const int i = e % I, r = e / I;
+        const double v = Y[i * CM + r];
+        U[e] = v;
+        if (U32) U32[e] = (float)v;
+        if (Ub16) Ub16[e] = f32_to_bf16((float)v);
+        if (M32) M32[r + (size_t)R * i] = (float)v;
+    }
+}
+
 template <int RMAX>
 static int launch_rows(dt_ctx *ctx, int mode, const double *S, const double *Y, int I, int R,
                        const double *Rinv, double *out, double *part, float *U32, float *M32,
@@ -360,6 +551,16 @@ size_t dt_als_ws_doubles(size_t I, size_t R) {
 int dt_als_update(dt_ctx *ctx, const double *S, double *U, int I, int R, int inner, double *ws,
                   int *flag, float *U32, float *M32, uint16_t *Ub16) {
     if (R < 1 || R > ALS_MAXR) return dt_fail(ctx, DT_EUNSUPPORTED, "ALS update: rank %d > %d", R, ALS_MAXR);
+    if (R <= 16 && I * 16 <= AF_MAXIR) {
+        DT_LAUNCH("als_fused", als_fused_kernel<16>, dim3(1), dim3(AF_THREADS), 0, S, U, I, R, inner, flag, U32, M32,
+                  Ub16);
+        return DT_OK;
+    }
+    if (R <= 32 && I * 32 <= AF_MAXIR) {
+        DT_LAUNCH("als_fused", als_fused_kernel<32>, dim3(1), dim3(AF_THREADS), 0, S, U, I, R, inner, flag, U32, M32,
+                  Ub16);
+        return DT_OK;
+    }
     const int nb = (I + ALS_GR - 1) / ALS_GR;  // partial Grams
     double *part = ws;
     double *Ri = part + (size_t)nb * R * R;
