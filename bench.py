@@ -23,9 +23,11 @@ sys.path.insert(0, os.path.join(ROOT, "diana-tucker_amd", "python"))
 
 HBM_PEAK_GBS = 8000.0
 # the two full-tensor passes of a tree-schedule sweep at 256^4: phase 1 reads A
-# and writes A x_3 U3^T (ttm_mk_f32_stream); phase 2 reads A and writes
-# A x_0 U0^T x_1 U1^T in one fused pass (ttm2_lead_f32_stream)
-FULL_PASS_KERNELS = ("ttm_mk_f32_stream", "ttm2_lead_f32_stream")
+# and writes A x_2 U2^T x_3 U3^T in one fused pass (ttm2_trail_f32_stream);
+# phase 2 reads A and writes A x_0 U0^T x_1 U1^T in one fused pass
+# (ttm2_lead_f32_stream).  ttm_mk_f32_stream is phase 1's single product when
+# the trailing pair does not fit.
+FULL_PASS_KERNELS = ("ttm2_trail_f32_stream", "ttm2_lead_f32_stream", "ttm_mk_f32_stream")
 
 
 def log(*a):
@@ -46,7 +48,8 @@ def tree_contractions(shape, ranks):
             sh[m] = ranks[m]
         return sh
 
-    TL = chain(shape, list(range(N - 1, hm - 1, -1)))
+    # phase 1: modes N-2 and N-1 in the fused pass (stage 1 over N-2), then the rest
+    TL = chain(shape, [N - 2, N - 1] + list(range(N - 3, hm - 1, -1)))
     for n in range(hm):
         chain(TL, [i for i in range(hm - 1, -1, -1) if i != n])
     TR = chain(shape, list(range(hm - 1, -1, -1)))
@@ -99,6 +102,7 @@ def dense_c2(ctx, dt, steps, warmup, shape=(256,) * 4, ranks=(16,) * 4, noise=1e
     total = math.prod(shape)
     # algorithmic bytes per launch: read A once, write the contracted result
     kbytes = {"ttm_mk_f32_stream": total * 4 + total // shape[-1] * ranks[-1] * 4,
+              "ttm2_trail_f32_stream": total * 4 + total // (shape[-2] * shape[-1]) * ranks[-2] * ranks[-1] * 4,
               "ttm2_lead_f32_stream": total * 4 + total // (shape[0] * shape[1]) * ranks[0] * ranks[1] * 4}
     return {"wall": wall, "res": res, "ktimes": ktimes, "breakdown": breakdown, "flops_per_sweep": flops,
             "kbytes": kbytes, "init_ms": out["init_ms"], "A": g["A"], "total": total}

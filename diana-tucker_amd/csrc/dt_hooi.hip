@@ -269,7 +269,23 @@ static int sweep_once(dt_hooi *h, const void *A, int dt) {
         std::vector<int> rdesc(right.rbegin(), right.rend());
         const void *TL;
         int tldt;
-        DT_TRY(chain(h, A, dt, sh, rdesc, h->buf[0], h->buf[1], &TL, &tldt));
+        int tfused = DT_EUNSUPPORTED;
+        if (rdesc.size() >= 2 && h->work_dt == DT_F32 && dt == DT_F32) {
+            // modes N-1 and N-2 in one pass: no intermediate of size |A| R_{N-1} / I_{N-1}
+            const int m2 = N - 2, m3 = N - 1;
+            int64_t F = 1;
+            for (int i = 0; i < m2; i++) F *= sh[i];
+            tfused = dt_ttm2_trail(h->ctx, (const float *)A, dt, F, sh[m2], sh[m3], h->M32[m2], h->M32[m3],
+                                   h->R[m2], h->R[m3], (float *)h->buf[0]);
+            if (tfused != DT_OK && tfused != DT_EUNSUPPORTED) return tfused;
+            if (tfused == DT_OK) {
+                sh[m2] = h->R[m2];
+                sh[m3] = h->R[m3];
+                std::vector<int> rest(rdesc.begin() + 2, rdesc.end());
+                DT_TRY(chain(h, h->buf[0], DT_F32, sh, rest, h->buf[1], h->buf[0], &TL, &tldt));
+            }
+        }
+        if (tfused != DT_OK) DT_TRY(chain(h, A, dt, sh, rdesc, h->buf[0], h->buf[1], &TL, &tldt));
         int64_t shL[DT_MAX_NDIM];
         for (int n : left) {
             std::copy(sh, sh + N, shL);

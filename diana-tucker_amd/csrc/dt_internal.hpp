@@ -207,6 +207,10 @@ int dt_als_fused(dt_ctx *ctx, const double *S, double *U, int I, int R, int inne
 // DT_EUNSUPPORTED when the shape does not fit the fused kernel.
 int dt_ttm2_lead(dt_ctx *ctx, const void *A, int dtA, int64_t K0, int64_t K1, int64_t B, const void *U0,
                  const float *M1, int64_t J0, int64_t J1, float *T);
+// T (F x J2 x J3) = A (F x K2 x K3) x_1 M2 x_2 M3 (FP32, M = J x K): the last two
+// modes in one pass; DT_EUNSUPPORTED when the shape does not fit
+int dt_ttm2_trail(dt_ctx *ctx, const float *A, int dtA, int64_t F, int64_t K2, int64_t K3, const float *M2,
+                  const float *M3, int64_t J2, int64_t J3, float *T);
 int dt_ttm_f64acc(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t K, int64_t B,
                   const double *M64, int64_t J, double *Y);
 int dt_dot_f64_dev(dt_ctx *ctx, const double *a, const double *b, size_t n, double *out_dev);

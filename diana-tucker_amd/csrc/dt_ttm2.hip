@@ -31,6 +31,9 @@ template <typename T>
 __device__ __forceinline__ T nt_load(const T *p) {
     return __builtin_nontemporal_load(p);
 }
+__device__ __forceinline__ floatx4 ld_nt_f4(const float *p, bool nt) {
+    return nt ? __builtin_nontemporal_load((const floatx4 *)p) : *(const floatx4 *)p;
+}
 
 constexpr int L2_YS = 68;  // LDS row stride (floats) of the J0 x 64 stage-1 block
 
@@ -225,6 +228,132 @@ __global__ __launch_bounds__(256) void ttm2_lead(const void *__restrict__ Av, co
                     if (j0 < J0 && j1 < J1) Tb[j0 + (int64_t)J0 * j1] = acc2[i][j][v];
                 }
     }
+}
+
+// ---- trailing pair ------------------------------------------------------------
+//   ttm2_trail:  T[f, j2, j3] = sum_{i2, i3} U2[i2, j2] U3[i3, j3] A[f, i2, i3]
+// (the last two modes of a tensor viewed as F x K2 x K3, FP32).  One
+// workgroup owns 64 consecutive f, i.e. 256-byte pieces of every (i2, i3)
+// column, and wave w streams i2 in [w K2/4, (w + 1) K2/4) for every i3 on one
+// continuous load cursor PD steps ahead.  Stage 1 (per i3 and wave) is the
+// ttm_mk_f32 MFMA form: A operand U2ᵀ (lane (c, g): M2[c, i2 + g]), B operand
+// a 16-byte f-contiguous piece of A (lane (c, g): A[f0 + 4c + t, i2 + g, i3],
+// t = 0..3 in four MFMAs), giving the wave's partial P_w[f0 + 4c + t, 4g + v].
+// The four partials go to LDS (double-buffered, rows j2 * 64 + f); stage 2 is
+// one MFMA per 16-row tile and i3 with K = the four waves (lane (c, g) reads
+// P_g[row c]) and B = U3[i3, j3 = c] in every group, so the MFMA also adds the
+// partials.  T (64 x J2 x J3 per workgroup) stays in registers until the end:
+// the pass reads A once and writes |A| J2 J3 / (K2 K3) floats.
+constexpr int TR_PSTRIDE = 1024 + 16;  // floats per wave partial (the pad staggers banks)
+
+template <int KQ, bool NT>
+__global__ __launch_bounds__(256) void ttm2_trail(const float *__restrict__ A, const float *__restrict__ M2,
+                                                  const float *__restrict__ M3, float *__restrict__ T, int64_t F,
+                                                  int K2, int K3, int J2, int J3) {
+    // KQ = K2 / 16 load steps (4 i2 each) per i3 and wave; the ring holds one
+    // whole i3 of steps, so the next i3's loads are in flight through the
+    // stage-2 barrier
+    __shared__ float Ps[2][4 * TR_PSTRIDE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int64_t f0 = (int64_t)blockIdx.x * 64;
+    const int i2w = wave * (K2 / 4);
+    const float um = c < J2 ? 1.f : 0.f;
+    const int cj2 = c < J2 ? c : J2 - 1;
+    const float m3m = c < J3 ? 1.f : 0.f;
+    const int cj3 = c < J3 ? c : J3 - 1;
+    // the U2 operand of every step of this wave (its i2 range is fixed)
+    float u[KQ];
+#pragma unroll
+    for (int q = 0; q < KQ; q++) u[q] = M2[cj2 + (int64_t)J2 * (i2w + 4 * q + g)] * um;
+
+    const float *Ab = A + f0 + 4 * c + F * (int64_t)(i2w + g);
+    // two half rings: while the MFMAs run over one half of i3's steps, the
+    // other half's loads for i3 + 1 are already in flight (scheduling barriers
+    // keep the compiler from sinking them below the MFMAs)
+    constexpr int HQ = KQ / 2;
+    floatx4 r0[HQ], r1[HQ];
+    auto ld = [&](int q, int i3) { return ld_nt_f4(Ab + F * ((int64_t)4 * q + (int64_t)K2 * i3), NT); };
+#pragma unroll
+    for (int q = 0; q < HQ; q++) r0[q] = ld(q, 0);
+#pragma unroll
+    for (int q = 0; q < HQ; q++) r1[q] = ld(HQ + q, 0);
+    float b3n = M3[cj3] * m3m;
+    floatx4 acc2[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) acc2[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int i3 = 0; i3 < K3; i3++) {
+        floatx4 acc[4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+        const float b3 = b3n;
+        // unconditional (the last i3 reloads itself) so that every path issues
+        // the same loads and the waits stay exact
+        const int in3 = i3 + 1 < K3 ? i3 + 1 : i3;
+#pragma unroll
+        for (int q = 0; q < HQ; q++) {
+            acc[0] = mfma4(u[q], r0[q][0], acc[0]);
+            acc[1] = mfma4(u[q], r0[q][1], acc[1]);
+            acc[2] = mfma4(u[q], r0[q][2], acc[2]);
+            acc[3] = mfma4(u[q], r0[q][3], acc[3]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < HQ; q++) r0[q] = ld(q, in3);
+        b3n = M3[cj3 + (int64_t)J3 * in3] * m3m;
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < HQ; q++) {
+            acc[0] = mfma4(u[HQ + q], r1[q][0], acc[0]);
+            acc[1] = mfma4(u[HQ + q], r1[q][1], acc[1]);
+            acc[2] = mfma4(u[HQ + q], r1[q][2], acc[2]);
+            acc[3] = mfma4(u[HQ + q], r1[q][3], acc[3]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < HQ; q++) r1[q] = ld(HQ + q, in3);
+        __builtin_amdgcn_sched_barrier(0);
+        // partial of this wave: acc[t][v] = P_w[f0 + 4c + t, j2 = 4g + v]; row j2 * 64 + f
+        float *P = Ps[i3 & 1] + wave * TR_PSTRIDE;
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            *(floatx4 *)(P + (4 * g + v) * 64 + 4 * c) = floatx4{acc[0][v], acc[1][v], acc[2][v], acc[3][v]};
+        __syncthreads();
+        // stage 2: tiles 16 wave .. 16 wave + 15 of the 1024 rows
+        const float *Pg = Ps[i3 & 1] + g * TR_PSTRIDE;
+#pragma unroll
+        for (int t = 0; t < 16; t++) acc2[t] = mfma4(Pg[(16 * wave + t) * 16 + c], b3, acc2[t]);
+    }
+    // acc2[t][v]: row r = 16 (16 wave + t) + 4g + v -> j2 = r / 64, f = f0 + r % 64; column j3 = c
+    if (c < J3) {
+#pragma unroll
+        for (int t = 0; t < 16; t++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = 16 * (16 * wave + t) + 4 * g + v;
+                const int j2 = r >> 6;
+                if (j2 < J2) T[f0 + (r & 63) + F * (j2 + (int64_t)J2 * c)] = acc2[t][v];
+            }
+    }
+}
+
+static bool ttm2_trail_fits(int dtA, int64_t F, int64_t K2, int64_t K3, int64_t J2, int64_t J3) {
+    return dtA == DT_F32 && F >= 64 && F % 64 == 0 && (K2 == 128 || K2 == 256) && K3 >= 1 &&
+           K3 <= 0x7fffffff && J2 >= 1 && J2 <= 16 && J3 >= 1 && J3 <= 16 && F / 64 <= 0x7fffffff;
+}
+
+// T (F x J2 x J3) = A (F x K2 x K3) x_1 M2 x_2 M3 with M2 = J2 x K2 and
+// M3 = J3 x K3 (FP32, j fastest).  DT_EUNSUPPORTED when the shape does not
+// fit (the caller then runs two single products).
+int dt_ttm2_trail(dt_ctx *ctx, const float *A, int dtA, int64_t F, int64_t K2, int64_t K3, const float *M2,
+                  const float *M3, int64_t J2, int64_t J3, float *T) {
+    if (!ttm2_trail_fits(dtA, F, K2, K3, J2, J3)) return DT_EUNSUPPORTED;
+    if (K2 == 256) {
+        DT_LAUNCH("ttm2_trail_f32_stream", (ttm2_trail<16, true>), dim3((unsigned)(F / 64)), dim3(256), 0, A, M2, M3,
+                  T, F, (int)K2, (int)K3, (int)J2, (int)J3);
+    } else {
+        DT_LAUNCH("ttm2_trail_f32_stream", (ttm2_trail<8, true>), dim3((unsigned)(F / 64)), dim3(256), 0, A, M2, M3,
+                  T, F, (int)K2, (int)K3, (int)J2, (int)J3);
+    }
+    return DT_OK;
 }
 
 // waves resident per CU for a kernel at 256 threads, times the CU count

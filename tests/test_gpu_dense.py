@@ -409,6 +409,30 @@ def test_hooi_4way_tree_fused_pair_matches_reference_schedule(ctx):
     assert np.allclose(r2["residuals"], res_ref, rtol=1e-4)
 
 
+@pytest.mark.parametrize("shape,ranks", [((16, 8, 128, 12), (4, 3, 10, 5)), ((8, 8, 256, 20), (4, 4, 16, 16)),
+                                         ((64, 3, 128, 7), (2, 3, 9, 7))])
+def test_hooi_4way_tree_trailing_pair_matches_reference_schedule(ctx, shape, ranks):
+    """Shapes where the tree schedule's first phase contracts modes 3 and 2 in
+    one pass (ranks up to 16, ragged J < 16 included); the reference schedule
+    runs single products only."""
+    g = ctx.gen_tucker(shape, ranks, 1e-2, seed=13)
+    A = g["A"].download()
+    r1 = run_gpu_hooi(ctx, A, ranks, 4, dt.INIT_HOSVD, schedule=dt.SCHED_REFERENCE,
+                      flags=dt.FLAG_FP64_FIT)
+    ctx.timing(True)
+    ctx.timing_reset()
+    r2 = run_gpu_hooi(ctx, A, ranks, 4, dt.INIT_HOSVD, schedule=dt.SCHED_TREE, flags=dt.FLAG_FP64_FIT)
+    names = ctx.timing_list()
+    ctx.timing(False)
+    assert "ttm2_trail_f32_stream" in names
+    assert np.allclose(r1["residuals"], r2["residuals"], rtol=1e-6)
+    for n in range(4):
+        assert D.sin_max_angle(r1["U"][n], r2["U"][n]) < 1e-4
+    Uh = D.hosvd_factors(A.astype(np.float64), ranks)
+    _, _, res_ref = D.hooi_als(A.astype(np.float64), ranks, 4, Uh)
+    assert np.allclose(r2["residuals"], res_ref, rtol=1e-4)
+
+
 def test_hooi_bf16_close_to_fp32(ctx):
     shape, ranks = (128, 128, 128), (16, 16, 16)
     g = ctx.gen_tucker(shape, ranks, 1e-3, seed=3, decay=0.93, bf16=True)
