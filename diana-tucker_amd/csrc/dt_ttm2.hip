@@ -76,6 +76,18 @@ __global__ __launch_bounds__(256) void ttm2_lead(const void *__restrict__ Av, co
     constexpr int PD = 2;
     constexpr int KSTEP = Tr::KSTEP;
     __shared__ float Ysm[4][16 * JB0][L2_YS];
+    // the stage-2 factor as M1s[i1][j1] when it fits (one fill per workgroup):
+    // per-unit global loads would sit behind the in-flight tensor loads
+    constexpr int M1_LDS = 8192;
+    __shared__ float M1s[M1_LDS];
+    const bool m1lds = (int64_t)K1 * 16 * JB1 <= M1_LDS;
+    if (m1lds) {
+        for (int e = threadIdx.x; e < K1 * 16 * JB1; e += 256) {
+            const int i1 = e / (16 * JB1), j = e % (16 * JB1);
+            M1s[e] = j < J1 ? M1[j + (int64_t)J1 * i1] : 0.f;
+        }
+        __syncthreads();
+    }
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int c = lane & 15;
@@ -115,9 +127,12 @@ __global__ __launch_bounds__(256) void ttm2_lead(const void *__restrict__ Av, co
     int64_t lb = w0;
     int lu = 0, lk = 0;
     const elem *lbase = A + slab * lb + (int64_t)K0 * c + Tr::LANE_K * g;  // + K0 * 64 lu
+    // past the last slab the loads repeat the slab before the end (never
+    // used): every path issues the same loads, so the waits stay exact
+    const elem *lclamp = A + slab * (B - 1) + (int64_t)K0 * c + Tr::LANE_K * g;
     auto issue = [&](int p) {
-        if (lb < B) {
-            const elem *Ap = lbase + (int64_t)K0 * 64 * lu + lk;
+        {
+            const elem *Ap = (lb < B ? lbase : lclamp) + (int64_t)K0 * 64 * lu + lk;
 #pragma unroll
             for (int cb = 0; cb < 4; cb++)
 #pragma unroll
@@ -174,12 +189,15 @@ __global__ __launch_bounds__(256) void ttm2_lead(const void *__restrict__ Av, co
             for (int j = 0; j < JB1; j++) acc2[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
         for (int un = 0; un < units; un++) {
             const int cbase = 64 * un;
-            // this unit's stage-2 factor values, fetched while stage 1 runs
+            // this unit's stage-2 factor values (from LDS when it holds M1)
             float m[16][JB1];
+            if (!m1lds) {
 #pragma unroll
-            for (int q = 0; q < 16; q++)
+                for (int q = 0; q < 16; q++)
 #pragma unroll
-                for (int jb = 0; jb < JB1; jb++) m[q][jb] = M1[j1col[jb] + (int64_t)J1 * (cbase + 4 * q + g)];
+                    for (int jb = 0; jb < JB1; jb++)
+                        m[q][jb] = M1[j1col[jb] + (int64_t)J1 * (cbase + 4 * q + g)];
+            }
 #pragma unroll
             for (int jb = 0; jb < JB0; jb++)
 #pragma unroll
@@ -202,6 +220,12 @@ __global__ __launch_bounds__(256) void ttm2_lead(const void *__restrict__ Av, co
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             // stage 2: acc2[j0][j1] += sum_i1 Ys[j0][i1] M1[j1, i1]
+            if (m1lds) {
+#pragma unroll
+                for (int q = 0; q < 16; q++)
+#pragma unroll
+                    for (int jb = 0; jb < JB1; jb++) m[q][jb] = M1s[(cbase + 4 * q + g) * 16 * JB1 + 16 * jb + c];
+            }
 #pragma unroll
             for (int q = 0; q < 16; q++) {
                 float y[JB0];
