@@ -34,6 +34,11 @@ __device__ __forceinline__ T nt_load(const T *p) {
 __device__ __forceinline__ floatx4 ld_nt_f4(const float *p, bool nt) {
     return nt ? __builtin_nontemporal_load((const floatx4 *)p) : *(const floatx4 *)p;
 }
+__device__ __forceinline__ void wave_sync_wl() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 constexpr int L2_YS = 68;  // LDS row stride (floats) of the J0 x 64 stage-1 block
 
@@ -89,7 +94,7 @@ __global__ __launch_bounds__(256) void ttm2_lead(const void *__restrict__ Av, co
         __syncthreads();
     }
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c = lane & 15;
     const int g = lane >> 4;
     const int64_t nw = (int64_t)gridDim.x * 4;
@@ -254,6 +259,115 @@ __global__ __launch_bounds__(256) void ttm2_lead(const void *__restrict__ Av, co
     }
 }
 
+// ---- lead pair with whole-column loads ------------------------------------------
+//   ttm2_lead_wl: the same product as ttm2_lead for FP32, K0 = 256, J0, J1 <= 16.
+// Every load instruction reads one whole 1 KB column (64 lanes x 16 B, the
+// pattern that streams at ~7 TB/s); a group of GC columns goes through a
+// per-wave LDS tile (column stride 260 floats, so the MFMA-layout reads below
+// hit 16 different bank quads) and is read back as lane (c, g) <- column c,
+// i0 = 16 s + 4 g .. +3 for the stage-1 MFMAs (U0 operand in registers).  The
+// next group's loads are issued as soon as the current group is in LDS.  The
+// stage-1 block goes through a small LDS tile to the operand layout of stage
+// 2, which runs against U1 held in LDS.
+constexpr int WL_CS = 260;  // LDS column stride (floats)
+constexpr int WL_GC = 16;   // columns per LDS group
+
+template <int GC, int K1>  // GC = 16 columns per LDS group, K1 = 256
+__global__ __launch_bounds__(256) void ttm2_lead_wl(const float *__restrict__ A, const float *__restrict__ U0,
+                                                    const float *__restrict__ M1, float *__restrict__ T, int J0,
+                                                    int J1, int64_t B) {
+    constexpr int K0 = 256, NG = K1 / GC;
+    __shared__ float Ac[4][GC * WL_CS];
+    __shared__ float Ps[4][GC * 20];
+    __shared__ float M1s[4096];  // M1s[i1 * 16 + j1], K1 <= 256
+    // uniform wave index: the slab loop stays scalar (exact waits across it)
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c = lane & 15, g = lane >> 4;
+    for (int e = threadIdx.x; e < K1 * 16; e += 256) {
+        const int i1 = e >> 4, j = e & 15;
+        M1s[e] = j < J1 ? M1[j + (int64_t)J1 * i1] : 0.f;
+    }
+    __syncthreads();
+    const int64_t nw = (int64_t)gridDim.x * 4;
+    const int64_t w0 = (int64_t)blockIdx.x * 4 + wave;
+    if (w0 >= B) return;  // no workgroup barrier after this point
+    float *Aw = Ac[wave];
+    float *Pw = Ps[wave];
+    // stage-1 operand U0[16 s + 4 g + t, j0 = c], masked past J0 (U0: K0 x J0, i0 fastest)
+    float u0[16][4];
+    const bool jin = c < J0;
+    const int jc = jin ? c : J0 - 1;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const floatx4 v = *(const floatx4 *)(U0 + 16 * q + 4 * g + (int64_t)K0 * jc);
+#pragma unroll
+        for (int t = 0; t < 4; t++) u0[q][t] = jin ? v[t] : 0.f;
+    }
+    constexpr int groups = NG;
+    const int64_t slab = (int64_t)K0 * K1;
+    // load cursor: slab lb, group lgr (past the last slab: reload the last one)
+    int64_t lb = w0;
+    int lgr = 0;
+    // two register sets alternate (groups per slab is even), so the loads of
+    // the next two groups are in flight while a group is computed
+    floatx4 vA[GC], vB[GC];
+    auto issue = [&](floatx4 (&v)[GC]) {
+        const int64_t bb = lb < B ? lb : B - 1;
+        const float *p = A + slab * bb + (int64_t)K0 * GC * lgr + 4 * lane;
+#pragma unroll
+        for (int q = 0; q < GC; q++) v[q] = ld_nt_f4(p + (int64_t)K0 * q, true);
+        if (++lgr == groups) {
+            lgr = 0;
+            lb += nw;
+        }
+    };
+    issue(vA);
+    issue(vB);
+    // per slab: NG = K1 / GC groups, fully unrolled in pairs (static register
+    // sets and a static load sequence, so the waits stay exact)
+    for (int64_t b = w0; b < B; b += nw) {
+        floatx4 acc2 = {0.f, 0.f, 0.f, 0.f};
+        auto group = [&](int gr, floatx4 (&v)[GC]) {
+            wave_sync_wl();  // the previous group's LDS reads are done
+#pragma unroll
+            for (int q = 0; q < GC; q++) *(floatx4 *)(Aw + q * WL_CS + 4 * lane) = v[q];
+            issue(v);
+            wave_sync_wl();
+            // stage 1 over this group's 16 columns
+            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const floatx4 a = *(const floatx4 *)(Aw + c * WL_CS + 16 * q + 4 * g);
+#pragma unroll
+                for (int t = 0; t < 4; t++) acc = mfma4(u0[q][t], a[t], acc);
+            }
+            // acc[v] = P[j0 = 4g + v][column c] -> Ps[column][j0]
+            *(floatx4 *)(Pw + c * 20 + 4 * g) = acc;
+            wave_sync_wl();
+            // stage 2: acc2[j0][j1] += sum over the group's columns P[j0][col] U1[col][j1]
+#pragma unroll
+            for (int kk = 0; kk < GC / 4; kk++) {
+                const float pa = Pw[(4 * kk + g) * 20 + c];
+                const float mb = M1s[(GC * gr + 4 * kk + g) * 16 + c];
+                acc2 = mfma4(pa, mb, acc2);
+            }
+        };
+#pragma unroll
+        for (int gr = 0; gr < NG; gr += 2) {
+            group(gr, vA);
+            group(gr + 1, vB);
+        }
+        // acc2[v] = T[j0 = 4g + v][j1 = c]
+        float *Tb = T + (int64_t)J0 * J1 * b;
+        if (c < J1)
+#pragma unroll
+            for (int vv = 0; vv < 4; vv++) {
+                const int j0 = 4 * g + vv;
+                if (j0 < J0) Tb[j0 + (int64_t)J0 * c] = acc2[vv];
+            }
+    }
+}
+
 // ---- trailing pair ------------------------------------------------------------
 //   ttm2_trail:  T[f, j2, j3] = sum_{i2, i3} U2[i2, j2] U3[i3, j3] A[f, i2, i3]
 // (the last two modes of a tensor viewed as F x K2 x K3, FP32).  One
@@ -403,6 +517,15 @@ static bool ttm2_lead_fits(int dtA, int64_t K0, int64_t K1, int64_t B, int64_t J
 int dt_ttm2_lead(dt_ctx *ctx, const void *A, int dtA, int64_t K0, int64_t K1, int64_t B,
                  const void *U0, const float *M1, int64_t J0, int64_t J1, float *T) {
     if (!ttm2_lead_fits(dtA, K0, K1, B, J0, J1)) return DT_EUNSUPPORTED;
+    if (dtA == DT_F32 && K0 == 256 && K1 == 256 && J0 <= 16 && J1 <= 16) {
+        static int64_t waves = 0;
+        if (!waves) waves = resident_waves(ttm2_lead_wl<WL_GC, 256>);
+        if (waves <= 0) return dt_fail(ctx, DT_EHIP, "dt_ttm2_lead: occupancy query failed");
+        const int64_t nwave = std::min<int64_t>(B, waves);
+        DT_LAUNCH("ttm2_lead_f32_stream", (ttm2_lead_wl<WL_GC, 256>), dim3((unsigned)((nwave + 3) / 4)), dim3(256), 0,
+                  (const float *)A, (const float *)U0, M1, T, (int)J0, (int)J1, B);
+        return DT_OK;
+    }
     const bool bf = dtA == DT_BF16;
     const int jb0 = J0 <= 16 ? 1 : 2;
     const int jb1 = J1 <= 16 ? 1 : (J1 <= 32 ? 2 : 4);

@@ -387,10 +387,12 @@ def test_hooi_4way_tree_matches_reference_schedule(ctx):
     assert np.allclose(r2["residuals"], res_ref, rtol=1e-4)
 
 
-def test_hooi_4way_tree_fused_pair_matches_reference_schedule(ctx):
-    """Shape where the tree schedule's second phase contracts modes 0 and 1 in
-    one pass; the reference schedule runs single products only."""
-    shape, ranks = (128, 64, 12, 10), (8, 12, 3, 4)
+@pytest.mark.parametrize("shape,ranks", [((128, 64, 12, 10), (8, 12, 3, 4)), ((256, 256, 3, 2), (16, 12, 3, 2)),
+                                         ((256, 256, 5, 7), (7, 16, 2, 3))])
+def test_hooi_4way_tree_fused_pair_matches_reference_schedule(ctx, shape, ranks):
+    """Shapes where the tree schedule's second phase contracts modes 0 and 1 in
+    one pass (256 x 256 slabs take the whole-column kernel, ragged ranks
+    included); the reference schedule runs single products only."""
     g = ctx.gen_tucker(shape, ranks, 1e-2, seed=9)
     A = g["A"].download()
     r1 = run_gpu_hooi(ctx, A, ranks, 4, dt.INIT_HOSVD, schedule=dt.SCHED_REFERENCE,
