@@ -736,8 +736,12 @@ __global__ __launch_bounds__(256) void jacobi_kernel(double *__restrict__ H, int
 
 // Cyclic Jacobi with H and V resident in LDS (n <= LDS_NMAX), same
 // round-robin ordering; the off-diagonal norm is a block reduction.
+// One matrix per workgroup: workgroup b solves Hin + b n^2 into Vout + b n^2, w + b n.
 __global__ __launch_bounds__(512) void jacobi_lds_kernel(const double *__restrict__ Hin, int n,
                                                         double *__restrict__ Vout, double *__restrict__ w) {
+    Hin += (size_t)blockIdx.x * n * n;
+    Vout += (size_t)blockIdx.x * n * n;
+    w += (size_t)blockIdx.x * n;
     __shared__ double H[LDS_NMAX * LDS_NMAX];
     __shared__ double V[LDS_NMAX * LDS_NMAX];
     __shared__ double rc[LDS_NMAX / 2], rsn[LDS_NMAX / 2];
@@ -845,6 +849,14 @@ __global__ __launch_bounds__(512) void jacobi_lds_kernel(const double *__restric
         const int r = idx % n, j = idx / n;
         Vout[idx] = V[r + n * order[j]];
     }
+}
+
+// `count` independent n x n problems (n <= LDS_NMAX) stored back to back, one
+// workgroup each.
+int dt_jacobi_eig_batched(dt_ctx *ctx, double *H, size_t n, int count, double *V, double *w) {
+    if (n > (size_t)LDS_NMAX || count < 1) return dt_fail(ctx, DT_EUNSUPPORTED, "batched Jacobi: n %zu", n);
+    DT_LAUNCH("jacobi", jacobi_lds_kernel, dim3((unsigned)count), dim3(512), 0, H, (int)n, V, w);
+    return DT_OK;
 }
 
 int dt_jacobi_eig(dt_ctx *ctx, double *H, size_t n, double *V, double *w) {

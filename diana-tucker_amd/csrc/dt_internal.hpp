@@ -74,6 +74,7 @@ enum dt_ws_slot {
     WS_MLP0,
     WS_MLP1,
     WS_MLP2,
+    WS_SKETCH,         // learned start: every mode's sketch, Gram and eigenvectors
     WS_COUNT_
 };
 static_assert(WS_COUNT_ <= dt_ctx::kSlots, "too many workspace slots");
@@ -184,6 +185,7 @@ int dt_gemm_f64_internal(dt_ctx *ctx, int tA, int tB, size_t m, size_t n, size_t
                          const double *A, size_t lda, const double *B, size_t ldb, double beta,
                          double *C, size_t ldc);
 int dt_jacobi_eig(dt_ctx *ctx, double *H, size_t n, double *V, double *w);
+int dt_jacobi_eig_batched(dt_ctx *ctx, double *H, size_t n, int count, double *V, double *w);
 // C (IA x JB, FP64) = A_(n) B_(n)ᵀ, FP32 or bf16 tensors of mode-n sizes IA and JB
 int dt_cross_internal(dt_ctx *ctx, const void *A, const void *B, int dt, int64_t F, int64_t IA, int64_t JB,
                       int64_t Bk, double *C);

@@ -84,28 +84,42 @@ inline dim3 lg_grid(int64_t n) { return dim3((unsigned)std::min<int64_t>((n + 25
 
 }  // namespace
 
-// X from the sketch Y (I x S, column-major FP64) held at the start of ws
-// (which has room for 2 I S + 2 S^2 + 2 S doubles).
-static int features_from_sketch(dt_ctx *ctx, double *ws, int64_t I, int S, float *X) {
-    double *Y = ws;
-    double *YV = Y + I * S;
-    double *G = YV + I * S;
-    double *V = G + (int64_t)S * S;
-    double *w = V + (int64_t)S * S;
-    double *sg = w + S;
-    DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, (size_t)S, (size_t)S, (size_t)I, 1.0, Y, (size_t)I, Y, (size_t)I, 0.0, G,
-                                (size_t)S));
-    DT_TRY(dt_jacobi_eig(ctx, G, (size_t)S, V, w));
-    DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, (size_t)I, (size_t)S, (size_t)S, 1.0, Y, (size_t)I, V, (size_t)S, 0.0, YV,
-                                (size_t)I));
-    DT_LAUNCH("sketch_rows", column_signs, dim3((unsigned)S), dim3(256), 0, YV, I, sg);
-    DT_LAUNCH("sketch_rows", sketch_rows, lg_grid(I * S), dim3(256), 0, YV, w, sg, I, S, X);
+// Feature pipeline of a sketch Y (I x S, column-major FP64): G = YᵀY, the
+// eigenvectors V of G (Jacobi), X = signs * sqrt(I) / sqrt(w_0) * Y V.  The
+// Jacobi step of several modes runs as one launch (one workgroup per mode).
+struct SketchWs {
+    double *Y, *YV, *sg;  // I x S, I x S, S
+};
+static int sketch_gram(dt_ctx *ctx, const SketchWs &s, int64_t I, int S, double *G) {
+    return dt_gemm_f64_internal(ctx, 1, 0, (size_t)S, (size_t)S, (size_t)I, 1.0, s.Y, (size_t)I, s.Y, (size_t)I, 0.0,
+                                G, (size_t)S);
+}
+static int sketch_finish(dt_ctx *ctx, const SketchWs &s, int64_t I, int S, const double *V, const double *w,
+                         float *X) {
+    DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, (size_t)I, (size_t)S, (size_t)S, 1.0, s.Y, (size_t)I, V, (size_t)S, 0.0,
+                                s.YV, (size_t)I));
+    DT_LAUNCH("sketch_rows", column_signs, dim3((unsigned)S), dim3(256), 0, s.YV, I, s.sg);
+    DT_LAUNCH("sketch_rows", sketch_rows, lg_grid(I * S), dim3(256), 0, s.YV, w, s.sg, I, S, X);
     return DT_OK;
 }
 
-extern "C" int dt_sketch_features(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n,
-                                  int S, uint64_t seed, float *X) {
-    DT_CHECK_ARG(ctx && A && shape && X, "dt_sketch_features: null argument");
+// X from the sketch Y (I x S, column-major FP64) held at the start of ws
+// (which has room for 2 I S + 2 S^2 + 2 S doubles).
+static int features_from_sketch(dt_ctx *ctx, double *ws, int64_t I, int S, float *X) {
+    SketchWs s{ws, ws + I * S, nullptr};
+    double *G = s.YV + I * S;
+    double *V = G + (int64_t)S * S;
+    double *w = V + (int64_t)S * S;
+    s.sg = w + S;
+    DT_TRY(sketch_gram(ctx, s, I, S, G));
+    DT_TRY(dt_jacobi_eig(ctx, G, (size_t)S, V, w));
+    return sketch_finish(ctx, s, I, S, V, w, X);
+}
+
+// Gaussian sketch of mode n (Ydst: as for sketch_kron)
+static int sketch_gauss(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n, int S,
+                        uint64_t seed, float *X, double *Ydst) {
+    DT_CHECK_ARG(ctx && A && shape && (X || Ydst), "dt_sketch_features: null argument");
     DT_CHECK_ARG(ndim >= 2 && ndim <= DT_MAX_NDIM && n >= 0 && n < ndim, "dt_sketch_features: bad mode");
     DT_CHECK_ARG(dtA == DT_F32 || dtA == DT_BF16, "dt_sketch_features: tensor must be F32 or BF16");
     DT_CHECK_ARG(S >= 1 && S <= 96, "dt_sketch_features: sketch width %d not in [1, 96]", S);
@@ -119,8 +133,14 @@ extern "C" int dt_sketch_features(dt_ctx *ctx, const void *A, int dtA, int ndim,
     double *ws = (double *)dt_ws(ctx, WS_MLP1, (size_t)(2 * I * S + 2 * S * S + 2 * S + 64) * sizeof(double));
     if (!Om || !ws) return DT_ENOMEM;
     DT_TRY(dt_gen_normal(ctx, Om, dtA, (size_t)nom, seed, 5000 + (uint64_t)n));
-    DT_TRY(dt_cross_internal(ctx, A, Om, dtA, F, I, S, Bk, ws));
+    DT_TRY(dt_cross_internal(ctx, A, Om, dtA, F, I, S, Bk, Ydst ? Ydst : ws));
+    if (Ydst) return DT_OK;
     return features_from_sketch(ctx, ws, I, S, X);
+}
+
+extern "C" int dt_sketch_features(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n,
+                                  int S, uint64_t seed, float *X) {
+    return sketch_gauss(ctx, A, dtA, ndim, shape, n, S, seed, X, nullptr);
 }
 
 // Y_(n) (I x S column-major FP64) of a tensor T whose mode n has size I and
@@ -143,9 +163,11 @@ __global__ void unfold_to_f64(const float *__restrict__ T, int ndim, int n, int6
     }
 }
 
-extern "C" int dt_sketch_features_kron(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n,
-                                       int q, uint64_t seed, float *X) {
-    DT_CHECK_ARG(ctx && A && shape && X, "dt_sketch_features_kron: null argument");
+// Kronecker sketch of mode n: with Ydst the unfolded sketch (I x S FP64) is
+// written there and nothing else happens; without, the features go to X.
+static int sketch_kron(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n, int q,
+                       uint64_t seed, float *X, double *Ydst) {
+    DT_CHECK_ARG(ctx && A && shape && (X || Ydst), "dt_sketch_features_kron: null argument");
     DT_CHECK_ARG(ndim >= 2 && ndim <= DT_MAX_NDIM && n >= 0 && n < ndim, "dt_sketch_features_kron: bad mode");
     DT_CHECK_ARG(dtA == DT_F32 || dtA == DT_BF16, "dt_sketch_features_kron: tensor must be F32 or BF16");
     int64_t S = 1;
@@ -183,8 +205,15 @@ extern "C" int dt_sketch_features_kron(dt_ctx *ctx, const void *A, int dtA, int 
         Js[k] = (size_t)q;
     }
     DT_TRY(dt_ttmc(ctx, A, dtA, ndim, shape, Ms, DT_F32, Js, idx, nm, Yt, DT_F32));
-    DT_LAUNCH("sketch_unfold", unfold_to_f64, lg_grid(I * S), dim3(256), 0, Yt, ndim, n, I, q, I * S, ws);
+    DT_LAUNCH("sketch_unfold", unfold_to_f64, lg_grid(I * S), dim3(256), 0, Yt, ndim, n, I, q, I * S,
+              Ydst ? Ydst : ws);
+    if (Ydst) return DT_OK;
     return features_from_sketch(ctx, ws, I, (int)S, X);
+}
+
+extern "C" int dt_sketch_features_kron(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n,
+                                       int q, uint64_t seed, float *X) {
+    return sketch_kron(ctx, A, dtA, ndim, shape, n, q, seed, X, nullptr);
 }
 
 extern "C" int dt_learned_start(dt_ctx *ctx, dt_mlp *mlp, const void *A, int dtA, int ndim, const size_t *shape,
@@ -212,15 +241,36 @@ extern "C" int dt_learned_start(dt_ctx *ctx, dt_mlp *mlp, const void *A, int dtA
         rows += (int64_t)shape[n];
     }
     float *X = (float *)dt_ws(ctx, WS_MLP2, (size_t)rows * (din + dout) * sizeof(float));
-    if (!X) return DT_ENOMEM;
+    // per mode: sketch Y and Y V (I_n x S each), signs (S); then all Grams, V, w
+    const int S = din;
+    const size_t per_mode_rows = (size_t)rows;
+    double *sk = (double *)dt_ws(ctx, WS_SKETCH, (2 * per_mode_rows * S + (size_t)ndim * (2 * S * S + 2 * S) + 64) *
+                                                    sizeof(double));
+    if (!X || !sk) return DT_ENOMEM;
     float *Z = X + rows * din;
+    SketchWs sw[DT_MAX_NDIM];
+    double *G = sk + 2 * per_mode_rows * S;
+    double *V = G + (size_t)ndim * S * S;
+    double *w = V + (size_t)ndim * S * S;
+    double *sg = w + (size_t)ndim * S;
     int64_t off = 0;
     for (int n = 0; n < ndim; n++) {
+        const int64_t I = (int64_t)shape[n];
+        sw[n] = SketchWs{sk + 2 * off * S, sk + 2 * off * S + I * S, sg + (size_t)n * S};
         if (sketch == DT_SKETCH_KRONECKER)
-            DT_TRY(dt_sketch_features_kron(ctx, A, dtA, ndim, shape, n, q, seed, X + off * din));
+            DT_TRY(sketch_kron(ctx, A, dtA, ndim, shape, n, q, seed, nullptr, sw[n].Y));
         else
-            DT_TRY(dt_sketch_features(ctx, A, dtA, ndim, shape, n, din, seed, X + off * din));
-        off += (int64_t)shape[n];
+            DT_TRY(sketch_gauss(ctx, A, dtA, ndim, shape, n, S, seed, nullptr, sw[n].Y));
+        DT_TRY(sketch_gram(ctx, sw[n], I, S, G + (size_t)n * S * S));
+        off += I;
+    }
+    // the eigen-decompositions of all modes in one launch
+    DT_TRY(dt_jacobi_eig_batched(ctx, G, (size_t)S, ndim, V, w));
+    off = 0;
+    for (int n = 0; n < ndim; n++) {
+        const int64_t I = (int64_t)shape[n];
+        DT_TRY(sketch_finish(ctx, sw[n], I, S, V + (size_t)n * S * S, w + (size_t)n * S, X + off * din));
+        off += I;
     }
     DT_TRY(dt_mlp_forward_dev(ctx, mlp, X, rows, Z));
     off = 0;
