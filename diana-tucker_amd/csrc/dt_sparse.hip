@@ -236,7 +236,9 @@ __device__ __forceinline__ intx4 sp_rec_load(const intx4 *__restrict__ rec, int6
 
 // Gathers of one batch: instruction s, lane (p4, c4) loads columns 4c4..4c4+3
 // of U_a[i_a] and U_b[i_b] of nonzero 16 s + p4 (whole 64-byte rows, four lanes
-// each).
+// each).  OFF32: both factors are below 4 GiB, so the address is the factor's
+// (scalar) base plus a 32-bit byte offset, one VALU operation per gather.
+template <bool OFF32>
 __device__ __forceinline__ void sp_issue(const intx4 &z, const float *__restrict__ Ua, const float *__restrict__ Ub,
                                          int p4, int c4, floatx4 (&ga)[4], floatx4 (&gb)[4]) {
 #pragma unroll
@@ -244,8 +246,15 @@ __device__ __forceinline__ void sp_issue(const intx4 &z, const float *__restrict
         const int src = (16 * s + p4) * 4;
         const int ia = __builtin_amdgcn_ds_bpermute(src, z.x);
         const int ib = __builtin_amdgcn_ds_bpermute(src, z.y);
-        ga[s] = *(const floatx4 *)(Ua + (int64_t)ia * 16 + 4 * c4);
-        gb[s] = *(const floatx4 *)(Ub + (int64_t)ib * 16 + 4 * c4);
+        if (OFF32) {
+            const uint32_t oa = ((uint32_t)ia << 6) + 16u * (uint32_t)c4;
+            const uint32_t ob = ((uint32_t)ib << 6) + 16u * (uint32_t)c4;
+            ga[s] = *(const floatx4 *)((const char *)Ua + oa);
+            gb[s] = *(const floatx4 *)((const char *)Ub + ob);
+        } else {
+            ga[s] = *(const floatx4 *)(Ua + (int64_t)ia * 16 + 4 * c4);
+            gb[s] = *(const floatx4 *)(Ub + (int64_t)ib * 16 + 4 * c4);
+        }
     }
 }
 
@@ -264,6 +273,7 @@ __device__ __forceinline__ void sp_issue(const intx4 &z, const float *__restrict
 // layout, so a row end never waits for an MFMA.  A finished row is written to Y_c when it lies
 // inside the range; otherwise it goes to carry slot 0 (row started before
 // P0) or 1 (row continues past P1), and sp_fixup adds them.
+template <bool OFF32>
 __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__restrict__ Ua,
                                                  const float *__restrict__ Ub, float *__restrict__ Yc,
                                                  float *__restrict__ carry, int64_t per_wave, int64_t nwaves) {
@@ -300,11 +310,13 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
         // rows starting in this batch (lane k: nonzero k; padding has row -1)
         const intx2 svl = L.sv[ks][lane];
         const int prevr = __builtin_amdgcn_ds_bpermute((lane == 0 ? 0 : lane - 1) * 4, svl.x);
+        const uint64_t valid = __ballot(svl.x >= 0);
         const uint64_t heads = __ballot(svl.x >= 0 && svl.x != (lane == 0 ? cur : prevr));
-        if (__popcll(heads) > 1) {
+        const int nh = __popcll(heads);
+        if (nh > 1) {
             // many short rows: one nonzero at a time on the VALU, one segment
             // per row, no MFMA result to wait for at a row end
-            const int nvalid = __popcll(__ballot(svl.x >= 0));
+            const int nvalid = __popcll(valid);
             uint64_t hm = heads;
             int k = 0;
             for (;;) {
@@ -329,35 +341,42 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
             }
             return;
         }
-#pragma unroll 2
-        for (int t = 0; t < 16; t++) {
-            const float ua = L.a[(4 * t + g) * 16 + c], bv = L.b[(4 * t + g) * 16 + c];
-            const intx2 sv = L.sv[ks][4 * t + g];
-            const float av = __int_as_float(sv.y) * ua;
-            const int r0 = __builtin_amdgcn_readlane(sv.x, 0);
-            const int r3 = __builtin_amdgcn_readlane(sv.x, 48);
-            if (r0 == cur && r3 == cur) {
+        if (nh == 0 && valid == ~0ull) {
+            // the whole batch continues the current row: 16 MFMA steps with
+            // no test between them
+#pragma unroll
+            for (int t = 0; t < 16; t++) {
+                const int k = 4 * t + g;
+                const float av = __int_as_float(L.sv[ks][k].y) * L.a[k * 16 + c];
+                const float bv = L.b[k * 16 + c];
                 if (t & 1)
                     acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc1, 0, 0, 0);
                 else
                     acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc0, 0, 0, 0);
-            } else {
-#pragma unroll 1
-                for (int gg = 0; gg < 4; gg++) {
-                    const int r = __builtin_amdgcn_readlane(sv.x, 16 * gg);
-                    if (r < 0) break;
-                    if (gg > 0 && r == __builtin_amdgcn_readlane(sv.x, 16 * gg - 16)) continue;
-                    if (r != cur) {
-                        flush(false);
-                        cur = r;
-                        acc0 = floatx4{0.f, 0.f, 0.f, 0.f};
-                        acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
-                        vacc = floatx4{0.f, 0.f, 0.f, 0.f};
-                    }
-                    const bool mine = sv.x == r;
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(mine ? av : 0.f, mine ? bv : 0.f, acc0, 0, 0, 0);
-                }
             }
+            return;
+        }
+        // one row start at lane h and/or a partial last batch: the nonzeros
+        // [lo, hi) of one row, in the MFMA steps that hold them, with the
+        // others masked
+        const int nvalid = __popcll(valid);
+        const int h = nh ? __builtin_ctzll(heads) : nvalid;
+        auto segment = [&](int lo, int hi) {
+            for (int t = lo >> 2; t < (hi + 3) >> 2; t++) {
+                const int k = 4 * t + g;
+                const float av = __int_as_float(L.sv[ks][k].y) * L.a[k * 16 + c];
+                const float bv = L.b[k * 16 + c];
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(k >= lo && k < hi ? av : 0.f, bv, acc0, 0, 0, 0);
+            }
+        };
+        segment(0, h);
+        if (nh) {
+            flush(false);
+            cur = __builtin_amdgcn_readlane(svl.x, h);
+            acc0 = floatx4{0.f, 0.f, 0.f, 0.f};
+            acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
+            vacc = floatx4{0.f, 0.f, 0.f, 0.f};
+            segment(h, nvalid);
         }
     };
 
@@ -373,7 +392,7 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
         const int ks2 = ks == 0 ? 2 : ks - 1;  // (ks + 2) mod 3
         const bool ok = b0 + 128 + lane < P1;
         L.sv[ks2][lane] = ok ? intx2{z.z, z.w} : intx2{-1, 0};
-        sp_issue(z, Ua, Ub, p4, c4, ga, gb);
+        sp_issue<OFF32>(z, Ua, Ub, p4, c4, ga, gb);
         z = sp_rec_load(rec, b0 + 256 + lane, P1);
         wave_sync();
         compute(b0, ks);
@@ -383,10 +402,10 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
     intx4 zY = sp_rec_load(rec, P0 + 64 + lane, P1);
     floatx4 gaX[4], gbX[4], gaY[4], gbY[4];
     L.sv[0][lane] = P0 + lane < P1 ? intx2{zX.z, zX.w} : intx2{-1, 0};
-    sp_issue(zX, Ua, Ub, p4, c4, gaX, gbX);
+    sp_issue<OFF32>(zX, Ua, Ub, p4, c4, gaX, gbX);
     zX = sp_rec_load(rec, P0 + 128 + lane, P1);
     L.sv[1][lane] = P0 + 64 + lane < P1 ? intx2{zY.z, zY.w} : intx2{-1, 0};
-    sp_issue(zY, Ua, Ub, p4, c4, gaY, gbY);
+    sp_issue<OFF32>(zY, Ua, Ub, p4, c4, gaY, gbY);
     zY = sp_rec_load(rec, P0 + 192 + lane, P1);
     // a counted loop over pairs of batches (padding past P1 has row -1 and
     // value 0, so the last batch needs no bounds test), then the odd batch
@@ -931,12 +950,14 @@ static int sp_ttmc_dev(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, fl
     const dt_sp_mode &m = sp->mode[n];
     const int a = n == 0 ? 1 : 0, b = n == 2 ? 1 : 2;
     SpModeArgs args{m.row, m.rec, m.rowptr, m.slot, sp->nnz};
+    // 32-bit gather offsets when both gathered factors (rows x 64 B) are below 4 GiB
+    const bool off32 = sp->dims[a] < ((int64_t)1 << 26) && sp->dims[b] < ((int64_t)1 << 26);
     static int blocks_per_cu = 0, cus = 0;
     if (!blocks_per_cu) {
         int dev = 0;
         DT_HIP(hipGetDevice(&dev));
         DT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        DT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, sp_ttmc16, 256, 0));
+        DT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, sp_ttmc16<true>, 256, 0));
         if (blocks_per_cu < 1) blocks_per_cu = 1;
     }
     // equal contiguous ranges, multiples of 64, about eight per resident wave
@@ -949,8 +970,13 @@ static int sp_ttmc_dev(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, fl
     float *carry = (float *)dt_ws(ctx, WS_SPARSE0, (size_t)nwaves * 2 * SP_E * 4 + 512);
     if (!carry) return DT_ENOMEM;
     static const char *names[3] = {"sp_ttmc_m0", "sp_ttmc_m1", "sp_ttmc_m2"};
-    DT_LAUNCH(names[n], sp_ttmc16, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, args, Ur[a], Ur[b], Yc, carry,
-              per_wave, nwaves);
+    if (off32) {
+        DT_LAUNCH(names[n], sp_ttmc16<true>, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, args, Ur[a], Ur[b], Yc,
+                  carry, per_wave, nwaves);
+    } else {
+        DT_LAUNCH(names[n], sp_ttmc16<false>, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, args, Ur[a], Ur[b],
+                  Yc, carry, per_wave, nwaves);
+    }
     DT_LAUNCH("sp_fixup", sp_fixup, dim3((unsigned)nwaves), dim3(1024), 0, args, carry, per_wave, nwaves, Yc);
     if (L) DT_TRY(sp_ytu_dev(ctx, sp, n, Ur[n], Yc, L));
     return DT_OK;
