@@ -273,8 +273,9 @@ __device__ __forceinline__ void sp_issue(const intx4 &z, const float *__restrict
 // layout, so a row end never waits for an MFMA.  A finished row is written to Y_c when it lies
 // inside the range; otherwise it goes to carry slot 0 (row started before
 // P0) or 1 (row continues past P1), and sp_fixup adds them.
+// Four waves per SIMD (128 registers; LDS allows four workgroups per CU).
 template <bool OFF32>
-__global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__restrict__ Ua,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void sp_ttmc16(SpModeArgs a, const float *__restrict__ Ua,
                                                  const float *__restrict__ Ub, float *__restrict__ Yc,
                                                  float *__restrict__ carry, int64_t per_wave, int64_t nwaves) {
     __shared__ SpLds lds_all[4];
@@ -294,16 +295,16 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
     const bool left_inc = P0 > 0 && rec[P0 - 1].z == first_row;
     const int after = P1 < a.nnz ? rec[P1].z : -1;
     int cur = first_row;
-    // partial sums of the current row: two MFMA accumulators and one VALU
-    // accumulator, all in the MFMA D layout (lane (c, g), v -> entry 4g + v + 16c)
-    floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f}, vacc = {0.f, 0.f, 0.f, 0.f};
+    // partial sums of the current row: two MFMA accumulators in the D layout
+    // (lane (c, g), v -> entry 4g + v + 16c)
+    floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
     const int e0 = 4 * g + 16 * c;
 
     auto flush = [&](bool last) {
         const bool linc = cur == first_row && left_inc;
         const bool rinc = last && after == cur;
         float *dst = (!linc && !rinc) ? Yc + (int64_t)cur * SP_E : carry + (w * 2 + (linc ? 0 : 1)) * SP_E;
-        __builtin_nontemporal_store((acc0 + acc1) + vacc, (floatx4 *)(dst + e0));
+        __builtin_nontemporal_store(acc0 + acc1, (floatx4 *)(dst + e0));
     };
 
     auto compute = [&](int64_t b0, int ks) {
@@ -313,42 +314,16 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
         const uint64_t valid = __ballot(svl.x >= 0);
         const uint64_t heads = __ballot(svl.x >= 0 && svl.x != (lane == 0 ? cur : prevr));
         const int nh = __popcll(heads);
-        if (nh > 1) {
-            // many short rows: one nonzero at a time on the VALU, one segment
-            // per row, no MFMA result to wait for at a row end
-            const int nvalid = __popcll(valid);
-            uint64_t hm = heads;
-            int k = 0;
-            for (;;) {
-                const int h = hm ? __builtin_ctzll(hm) : nvalid;
-                for (int kk = k; kk < h; kk++) {
-                    const float vv = __int_as_float(__builtin_amdgcn_readlane(svl.y, kk));
-                    const floatx4 ua = *(const floatx4 *)(L.a + kk * 16 + 4 * g);
-                    const float wb = vv * L.b[kk * 16 + c];
-                    vacc[0] = fmaf(ua[0], wb, vacc[0]);
-                    vacc[1] = fmaf(ua[1], wb, vacc[1]);
-                    vacc[2] = fmaf(ua[2], wb, vacc[2]);
-                    vacc[3] = fmaf(ua[3], wb, vacc[3]);
-                }
-                if (h >= nvalid) break;
-                flush(false);
-                cur = __builtin_amdgcn_readlane(svl.x, h);
-                acc0 = floatx4{0.f, 0.f, 0.f, 0.f};
-                acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
-                vacc = floatx4{0.f, 0.f, 0.f, 0.f};
-                hm &= hm - 1;
-                k = h;
-            }
-            return;
-        }
         if (nh == 0 && valid == ~0ull) {
             // the whole batch continues the current row: 16 MFMA steps with
             // no test between them
+            // one base address per operand; step t is a constant offset from it
+            const float *pa = L.a + g * 16 + c;
+            const float *pb = L.b + g * 16 + c;
 #pragma unroll
             for (int t = 0; t < 16; t++) {
-                const int k = 4 * t + g;
-                const float av = __int_as_float(L.sv[ks][k].y) * L.a[k * 16 + c];
-                const float bv = L.b[k * 16 + c];
+                const float av = pa[64 * t];
+                const float bv = pb[64 * t];
                 if (t & 1)
                     acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc1, 0, 0, 0);
                 else
@@ -356,27 +331,27 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
             }
             return;
         }
-        // one row start at lane h and/or a partial last batch: the nonzeros
-        // [lo, hi) of one row, in the MFMA steps that hold them, with the
-        // others masked
+        // row starts and/or a partial last batch: each row's nonzeros
+        // [lo, hi) in the MFMA steps that hold them, the others masked; a
+        // row that ends here is flushed before the next one starts
         const int nvalid = __popcll(valid);
-        const int h = nh ? __builtin_ctzll(heads) : nvalid;
-        auto segment = [&](int lo, int hi) {
+        uint64_t hm = heads;
+        int lo = 0;
+        for (;;) {
+            const int hi = hm ? __builtin_ctzll(hm) : nvalid;
             for (int t = lo >> 2; t < (hi + 3) >> 2; t++) {
                 const int k = 4 * t + g;
-                const float av = __int_as_float(L.sv[ks][k].y) * L.a[k * 16 + c];
+                const float av = L.a[k * 16 + c];
                 const float bv = L.b[k * 16 + c];
                 acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(k >= lo && k < hi ? av : 0.f, bv, acc0, 0, 0, 0);
             }
-        };
-        segment(0, h);
-        if (nh) {
+            if (hi >= nvalid) break;
             flush(false);
-            cur = __builtin_amdgcn_readlane(svl.x, h);
+            cur = __builtin_amdgcn_readlane(svl.x, hi);
             acc0 = floatx4{0.f, 0.f, 0.f, 0.f};
             acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
-            vacc = floatx4{0.f, 0.f, 0.f, 0.f};
-            segment(h, nvalid);
+            hm &= hm - 1;
+            lo = hi;
         }
     };
 
@@ -384,9 +359,13 @@ __global__ __launch_bounds__(256) void sp_ttmc16(SpModeArgs a, const float *__re
     // z holds the records of batch k + 2, ga/gb the gathers of batch k
     auto step = [&](int64_t b0, int ks, intx4 &z, floatx4 (&ga)[4], floatx4 (&gb)[4]) {
         wave_sync();  // the previous batch's LDS reads are done
+        // U_a rows are staged already scaled by their nonzero's value
+        // (padding has value 0)
+        const intx2 *psv = &L.sv[ks][p4];
 #pragma unroll
         for (int s = 0; s < 4; s++) {
-            *(floatx4 *)(L.a + (16 * s + p4) * 16 + 4 * c4) = ga[s];
+            const float v = __int_as_float(psv[16 * s].y);
+            *(floatx4 *)(L.a + (16 * s + p4) * 16 + 4 * c4) = ga[s] * v;
             *(floatx4 *)(L.b + (16 * s + p4) * 16 + 4 * c4) = gb[s];
         }
         const int ks2 = ks == 0 ? 2 : ks - 1;  // (ks + 2) mod 3
