@@ -463,6 +463,12 @@ static int ts_gram(dt_ctx *ctx, const double *Q, size_t m, size_t n, double *G) 
     return DT_OK;
 }
 
+int dt_part_reduce(dt_ctx *ctx, const double *part, int nparts, int ne, double *out) {
+    DT_LAUNCH("ts_gram", part_reduce_kernel, dim3((unsigned)ceil_div((int64_t)ne, 16)), dim3(256), 0, part, nparts, ne,
+              out);
+    return DT_OK;
+}
+
 static int ts_rmul(dt_ctx *ctx, double *Q, size_t m, size_t n, const double *Rinv) {
     const dim3 grid((unsigned)std::min<int64_t>(ceil_div((int64_t)m, 256), 8192));
     if (n <= 16) {

@@ -199,6 +199,11 @@ int dt_cholqr3_dev(dt_ctx *ctx, double *X, int m, int n, double *work, int *flag
 int dt_gram_inverse_dev(dt_ctx *ctx, const double *X, int m, int n, double *C, double *work, int *flag);
 int dt_inverse_flag(dt_ctx *ctx, const double *G, int n, double *C, double *work, int *flag, int bit);
 int dt_orth_dev(dt_ctx *ctx, double *X, size_t m, size_t n, double *work, int *flag);
+// Rinv (n x n upper, column-major) = inverse Cholesky factor of G (+ the
+// shifted-CholeskyQR shift for m_rows rows when shifted); failure ORs 4 into *flag
+int dt_chol_rinv_dev(dt_ctx *ctx, const double *G, int n, bool shifted, double m_rows, double *Rinv, int *flag);
+// out[e] = sum over p < nparts of part[p * ne + e] in a fixed order
+int dt_part_reduce(dt_ctx *ctx, const double *part, int nparts, int ne, double *out);
 size_t dt_als_ws_doubles(size_t I, size_t R);
 int dt_als_update(dt_ctx *ctx, const double *S, double *U, int I, int R, int inner, double *ws,
                   int *flag, float *U32, float *M32, uint16_t *Ub16);

@@ -512,6 +512,13 @@ int dt_inverse_flag(dt_ctx *ctx, const double *G, int n, double *C, double *work
 // Device-only orthonormalisation of X (m x n): shifted + 2 plain Cholesky-QR
 // passes.  One workgroup when m n^2 is small, otherwise the Gram and the
 // X R^{-1} products run as multi-workgroup GEMMs.  work: >= 2 n*n + m*n doubles.
+int dt_chol_rinv_dev(dt_ctx *ctx, const double *G, int n, bool shifted, double m_rows, double *Rinv, int *flag) {
+    DT_TRY(check_r(ctx, n));
+    DT_LAUNCH("cholqr_small", chol_rinv_kernel, dim3(1), dim3(SMALL_THREADS), 2 * n * n * 8, G, n, shifted ? 1 : 0,
+              m_rows, Rinv, flag);
+    return DT_OK;
+}
+
 int dt_orth_dev(dt_ctx *ctx, double *X, size_t m, size_t n, double *work, int *flag) {
     if ((double)m * n * n <= (double)(1 << 21) && n <= 32)
         return dt_cholqr3_dev(ctx, X, (int)m, (int)n, work, flag);

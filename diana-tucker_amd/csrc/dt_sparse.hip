@@ -505,25 +505,58 @@ __global__ __launch_bounds__(256) void sp_lreduce(const float *__restrict__ Lwg,
     }
 }
 
-// W (nrows x 16) = Y_c L, scattered to U (I x 16 row-major FP64) rows; U is
-// pre-zeroed for empty rows.  v_mfma_f32_16x16x4_f32 over tiles of 16 compact
-// rows (one wave per tile), K = 256: lane (i, g) loads entries
-// 16q + 4g .. 16q + 4g + 3 of row i and feeds entry 16q + 4g + j at step (q, j)
-// as A[i][g], with B[g][n] = L[16q + 4g + j][n] from LDS (rows padded to 20
-// floats so the four groups read disjoint banks).
-__global__ __launch_bounds__(256) void sp_times_l(const float *__restrict__ Yc, const int32_t *__restrict__ rows,
-                                                  int64_t nrows, const double *__restrict__ L,
-                                                  double *__restrict__ Uout) {
+// Y (I x 256 row-major FP32, zeros for empty rows) from Y_c.
+__global__ void sp_expand(const float *__restrict__ Yc, const int32_t *__restrict__ rows, int64_t nrows,
+                          float *__restrict__ Y) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nrows * SP_E; e += (int64_t)gridDim.x * blockDim.x)
+        Y[(int64_t)rows[e / SP_E] * SP_E + e % SP_E] = Yc[e];
+}
+
+// ---- factor update without host round trips ----------------------------------
+// U_n <- orth(Y_(n) L) as shifted Cholesky-QR3 (Fukaya et al. 2020): one shifted
+// pass, then two plain ones.  Every pass streams U once on FP64 MFMA and leaves
+// the Gram of its output as per-workgroup partials (fixed order), so no step
+// waits for the host; a Cholesky breakdown sets a device flag that the sweep
+// reads with its residual.
+
+// Partial Gram of a wave's 16 x 16 FP64 MFMA accumulator (lane (c, g), register
+// v = G[4v + g][c]): the four waves are added in wave order and entry (a, b),
+// a, b < R, goes to part[blockIdx.x * R * R + a + R b].
+__device__ __forceinline__ void sp_gram_out(doublex4 gacc, int R, double *__restrict__ part) {
+    __shared__ double red[4][4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int v = 0; v < 4; v++) red[wv][v][lane] = gacc[v];
+    __syncthreads();
+    const int v = threadIdx.x >> 6, c = lane & 15, g = lane >> 4;
+    const double sum = ((red[0][v][lane] + red[1][v][lane]) + red[2][v][lane]) + red[3][v][lane];
+    const int a = 4 * v + g;
+    if (a < R && c < R) part[(int64_t)blockIdx.x * R * R + a + R * c] = sum;
+}
+
+// U (I x R, FP64 column-major) = Y_(n) L for every row i of the mode (rows
+// without nonzeros give 0) and the partial Grams of U.  A wave takes runs of
+// 16-row tiles: lane (i, g) reads row 16t + i of Y (its compact copy) as
+// float4s 4q + g, one FP32 MFMA step per float4 component against L (FP32 in
+// LDS) leaves W[16t + 4g + v][i] in register v; those values, widened to FP64,
+// are stored and fed to the Gram's FP64 MFMA as both operands.
+__global__ __launch_bounds__(256) void sp_tl_gram(const float *__restrict__ Yc, const int32_t *__restrict__ rowptr,
+                                                  const int32_t *__restrict__ slot, int64_t I, int R,
+                                                  const double *__restrict__ L, int64_t tiles_per_wave,
+                                                  double *__restrict__ U, double *__restrict__ part) {
     __shared__ float Ls[SP_E][20];
     for (int e = threadIdx.x; e < SP_E * 16; e += 256) Ls[e / 16][e % 16] = (float)L[e];
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = lane & 15, g = lane >> 4;
-    const int64_t ntiles = (nrows + 15) / 16;
-    for (int64_t t = (int64_t)blockIdx.x * 4 + wv; t < ntiles; t += (int64_t)gridDim.x * 4) {
+    const int64_t ntiles = (I + 15) / 16;
+    const int64_t t0 = ((int64_t)blockIdx.x * 4 + wv) * tiles_per_wave;
+    const int64_t t1 = t0 + tiles_per_wave < ntiles ? t0 + tiles_per_wave : ntiles;
+    doublex4 gacc = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t t = t0; t < t1; t++) {
         const int64_t r = t * 16 + i;
-        const bool ok = r < nrows;
-        const floatx4 *yr = (const floatx4 *)(Yc + (ok ? r : nrows - 1) * SP_E);
+        const bool ok = r < I && rowptr[r + 1] > rowptr[r];
+        const floatx4 *yr = (const floatx4 *)(Yc + (int64_t)(ok ? slot[r] : 0) * SP_E);
         floatx4 y[16];
 #pragma unroll
         for (int q = 0; q < 16; q++) y[q] = ok ? yr[4 * q + g] : floatx4{0.f, 0.f, 0.f, 0.f};
@@ -533,34 +566,57 @@ __global__ __launch_bounds__(256) void sp_times_l(const float *__restrict__ Yc, 
 #pragma unroll
             for (int j = 0; j < 4; j++)
                 acc = __builtin_amdgcn_mfma_f32_16x16x4f32(y[q][j], Ls[16 * q + 4 * g + j][i], acc, 0, 0, 0);
-        // lane (n = i, g), register v: W[row 16t + 4g + v][n]
 #pragma unroll
         for (int v = 0; v < 4; v++) {
+            const double w = (double)acc[v];
             const int64_t rr = t * 16 + 4 * g + v;
-            if (rr < nrows) Uout[(int64_t)rows[rr] * 16 + i] = (double)acc[v];
+            if (rr < I && i < R) U[rr + I * i] = w;
+            gacc = __builtin_amdgcn_mfma_f64_16x16x4f64(w, w, gacc, 0, 0, 0);
         }
     }
+    sp_gram_out(gacc, R, part);
 }
 
-// Y (I x 256 row-major FP32, zeros for empty rows) from Y_c.
-__global__ void sp_expand(const float *__restrict__ Yc, const int32_t *__restrict__ rows, int64_t nrows,
-                          float *__restrict__ Y) {
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nrows * SP_E; e += (int64_t)gridDim.x * blockDim.x)
-        Y[(int64_t)rows[e / SP_E] * SP_E + e % SP_E] = Yc[e];
-}
-
-// U (I x R column-major) = the first R columns of Ur (I x 16 row-major); one
-// row per thread, column-coalesced writes.
-__global__ void rowmajor_to_colmajor(const double *__restrict__ Ur, int64_t I, int R, double *__restrict__ U) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < I; i += (int64_t)gridDim.x * blockDim.x) {
-        const double2 *src = (const double2 *)(Ur + i * 16);
+// U <- U Rinv in place (U: I x R FP64 column-major, Rinv: R x R upper,
+// column-major) on FP64 MFMA: for a 16-row tile lane (c, g) feeds
+// U[r0 + c][4s + g] as A and Rinv[4s + g][c] as B of step s and receives
+// Unew[r0 + 4v + g][c] in register v.  GRAM: partial Grams of the new U.
+// VIEW: also the FP32 row-major copy, 16 columns (zero past R).
+template <bool GRAM, bool VIEW>
+__global__ __launch_bounds__(256) void sp_rmul(double *__restrict__ U, int64_t I, int R,
+                                               const double *__restrict__ Rinv, int64_t tiles_per_wave,
+                                               double *__restrict__ part, float *__restrict__ Ur) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    double b[4];
 #pragma unroll
-        for (int h = 0; h < 8; h++) {
-            const double2 v = src[h];
-            if (2 * h < R) U[i + I * (2 * h)] = v.x;
-            if (2 * h + 1 < R) U[i + I * (2 * h + 1)] = v.y;
+    for (int s = 0; s < 4; s++) b[s] = (4 * s + g < R && c < R) ? Rinv[(4 * s + g) + R * c] : 0.0;
+    const int64_t ntiles = (I + 15) / 16;
+    const int64_t t0 = ((int64_t)blockIdx.x * 4 + wv) * tiles_per_wave;
+    const int64_t t1 = t0 + tiles_per_wave < ntiles ? t0 + tiles_per_wave : ntiles;
+    doublex4 gacc = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t t = t0; t < t1; t++) {
+        const int64_t r0 = t * 16;
+        double a[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int k = 4 * s + g;
+            a[s] = (k < R && r0 + c < I) ? U[(r0 + c) + I * k] : 0.0;
+        }
+        doublex4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < 4; s++) d = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], d, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int64_t rr = r0 + 4 * v + g;
+            if (rr < I) {
+                if (c < R) U[rr + I * c] = d[v];
+                if (VIEW) Ur[rr * 16 + c] = (float)d[v];
+            }
+            if (GRAM) gacc = __builtin_amdgcn_mfma_f64_16x16x4f64(d[v], d[v], gacc, 0, 0, 0);
         }
     }
+    if (GRAM) sp_gram_out(gacc, R, part);
 }
 
 // row-major FP32 view with 16 columns, columns R..15 zero; one row per thread
@@ -1003,11 +1059,10 @@ extern "C" int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int m
         (void)hipStreamSynchronize(ctx->stream);
         for (void *p : mine) (void)hipFree(p);
     };
-    double *U64[3], *Urow64 = nullptr;
+    double *U64[3];
     float *U32r[3];
-    int64_t maxI = 0, maxrows = 0;
+    int64_t maxrows = 0;
     for (int m = 0; m < 3; m++) {
-        maxI = std::max(maxI, sp->dims[m]);
         maxrows = std::max(maxrows, sp->mode[m].nrows);
     }
     for (int m = 0; m < 3; m++) {
@@ -1018,15 +1073,46 @@ extern "C" int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int m
             return dt_fail(ctx, DT_ENOMEM, "dt_hooi_sparse: out of memory");
         }
     }
-    Urow64 = (double *)alloc((size_t)maxI * R * 8);
     float *Yc = (float *)alloc((size_t)std::max<int64_t>(maxrows, 1) * SP_E * 4);
     double *L = (double *)alloc(SP_E * 16 * 8);
     double *G64 = (double *)alloc(SP_E * 16 * 8);
-    if (!Urow64 || !Yc || !L || !G64) {
+    constexpr int kMaxWg = 2048;  // factor-update workgroups (Gram partials)
+    double *part = (double *)alloc((size_t)kMaxWg * 256 * 8);
+    double *Gs = (double *)alloc(256 * 8);
+    double *Rinv = (double *)alloc(256 * 8);
+    int *dflag = (int *)alloc(256);
+    if (!Yc || !L || !G64 || !part || !Gs || !Rinv || !dflag) {
         cleanup();
         return dt_fail(ctx, DT_ENOMEM, "dt_hooi_sparse: out of memory");
     }
-    int st = DT_OK;
+    int st = hipMemsetAsync(dflag, 0, sizeof(int), ctx->stream) == hipSuccess ? DT_OK : DT_EHIP;
+    // U_n <- orth(Y_(n) L): sp_tl_gram, then three Cholesky-QR passes (the first
+    // shifted) whose Grams come from the previous pass's partials; the last
+    // pass also writes the FP32 view
+    auto update = [&](int n) -> int {
+        const int64_t I = sp->dims[n];
+        const int Rn = Rk[n];
+        const int64_t ntiles = (I + 15) / 16;
+        const int64_t tpw = std::max<int64_t>(1, (ntiles + 4 * kMaxWg - 1) / (4 * kMaxWg));
+        const int ng = (int)(((ntiles + tpw - 1) / tpw + 3) / 4);
+        const dt_sp_mode &md = sp->mode[n];
+        DT_LAUNCH("sp_tl_gram", sp_tl_gram, dim3((unsigned)ng), dim3(256), 0, (const float *)Yc, md.rowptr, md.slot,
+                  I, Rn, (const double *)L, tpw, U64[n], part);
+        auto rmul_gram = sp_rmul<true, false>;
+        auto rmul_view = sp_rmul<false, true>;
+        for (int pass = 0; pass < 3; pass++) {
+            DT_TRY(dt_part_reduce(ctx, part, ng, Rn * Rn, Gs));
+            DT_TRY(dt_chol_rinv_dev(ctx, Gs, Rn, pass == 0, (double)I, Rinv, dflag));
+            if (pass < 2) {
+                DT_LAUNCH("sp_rmul", rmul_gram, dim3((unsigned)ng), dim3(256), 0, U64[n], I, Rn,
+                          (const double *)Rinv, tpw, part, (float *)nullptr);
+            } else {
+                DT_LAUNCH("sp_rmul", rmul_view, dim3((unsigned)ng), dim3(256), 0, U64[n], I, Rn,
+                          (const double *)Rinv, tpw, (double *)nullptr, U32r[n]);
+            }
+        }
+        return DT_OK;
+    };
     auto views = [&](int m) {
         dt_ktimer kt(ctx, "sp_views");
         hipLaunchKernelGGL(rows16_view, sp_grid(sp->dims[m]), dim3(256), 0, ctx->stream, U64[m], sp->dims[m],
@@ -1051,24 +1137,9 @@ extern "C" int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int m
     for (int it = 0; it < max_iter && st == DT_OK; it++) {
         (void)hipEventRecord(e0, ctx->stream);
         for (int n = 0; n < 3 && st == DT_OK; n++) {
-            const dt_sp_mode &md = sp->mode[n];
-            const int64_t I = sp->dims[n];
             const float *Ur[3] = {U32r[0], U32r[1], U32r[2]};
-            st = sp_ttmc_dev(ctx, sp, n, Ur, Yc, L);           // Y_c and L = Y_cᵀ U_n
-            if (st == DT_OK) st = dt_zero(ctx, Urow64, (size_t)I * R * 8);
-            if (st == DT_OK) {
-                {
-                    dt_ktimer kt(ctx, "sp_times_l");
-                    hipLaunchKernelGGL(sp_times_l, dim3((unsigned)std::min<int64_t>((md.nrows + 63) / 64, 4096)),
-                                       dim3(256), 0, ctx->stream, Yc, md.rows, md.nrows, L, Urow64);
-                }
-                dt_ktimer kt(ctx, "sp_views");
-                hipLaunchKernelGGL(rowmajor_to_colmajor, sp_grid(I), dim3(256), 0, ctx->stream, Urow64, I,
-                                   Rk[n], U64[n]);
-                st = hipGetLastError() == hipSuccess ? DT_OK : DT_EHIP;
-            }
-            if (st == DT_OK) st = dt_qr_f64(ctx, U64[n], nullptr, U64[n], (size_t)I, (size_t)Rk[n]);  // orth(S U_n)
-            if (st == DT_OK) st = views(n);
+            st = sp_ttmc_dev(ctx, sp, n, Ur, Yc, L);  // Y_c and L = Y_cᵀ U_n
+            if (st == DT_OK) st = update(n);           // U_n <- orth(Y_(n) L)
         }
         // core = Y_(2)ᵀ U_2 with the updated U_2: Y_(2) does not depend on U_2, so
         // the last mode's Y_c is reused
@@ -1077,9 +1148,16 @@ extern "C" int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int m
         if (st != DT_OK) break;
         std::vector<double> g(SP_E * 16);
         if (hipMemcpyAsync(g.data(), G64, g.size() * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipMemcpyAsync(ctx->pinned, dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
             hipStreamSynchronize(ctx->stream) != hipSuccess) {
             st = DT_EHIP;
             break;
+        }
+        if (*(const int *)ctx->pinned != 0) {
+            cleanup();
+            return dt_fail(ctx, DT_ENUMERIC,
+                           "dt_hooi_sparse: Cholesky breakdown in the factor update of sweep %d (Y_(n) L rank-deficient)",
+                           it);
         }
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
