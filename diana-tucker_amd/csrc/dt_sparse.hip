@@ -440,46 +440,69 @@ __global__ __launch_bounds__(1024) void sp_fixup(SpModeArgs a, const float *__re
     }
 }
 
-// Per-wave partial L = Y_cᵀ U_n[rows] (256 x 16): one 64-thread workgroup per
-// contiguous run of compact rows; lane l owns entries 4l..4l+3 of the row for
-// all 16 columns (64 FP32 accumulators), reading each Y_c row once (1 KB,
-// coalesced) and the factor row once per wave.
-__global__ __launch_bounds__(64) void sp_ytu(const float *__restrict__ Yc, const int32_t *__restrict__ rows,
-                                             int64_t nrows, const float *__restrict__ Un, int64_t rows_per_wg,
-                                             float *__restrict__ Lwg) {
-    const int lane = threadIdx.x;
-    const int64_t s0 = (int64_t)blockIdx.x * rows_per_wg;
-    const int64_t s1 = s0 + rows_per_wg < nrows ? s0 + rows_per_wg : nrows;
+// Partial L = Y_cᵀ U_n[rows] (256 x 16) per 256-thread workgroup: each wave
+// walks its own contiguous run of compact rows, lane l owning entries
+// 4l..4l+3 of the row for all 16 columns (64 FP32 accumulators); four rows are
+// loaded before any is used (each Y_c row once, 1 KB coalesced, and its factor
+// row).  The four waves' sums are then added in wave order through LDS.
+__global__ __launch_bounds__(256) void sp_ytu(const float *__restrict__ Yc, const int32_t *__restrict__ rows,
+                                              int64_t nrows, const float *__restrict__ Un, int64_t rows_per_wave,
+                                              float *__restrict__ Lwg) {
+    __shared__ floatx4 red[16][64];  // one wave's sums: [4q + h][lane]
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t s0 = ((int64_t)blockIdx.x * 4 + wv) * rows_per_wave;
+    const int64_t s1 = s0 + rows_per_wave < nrows ? s0 + rows_per_wave : nrows;
     float acc[4][16];
 #pragma unroll
     for (int q = 0; q < 4; q++)
 #pragma unroll
         for (int k = 0; k < 16; k++) acc[q][k] = 0.f;
-#pragma unroll 2
-    for (int64_t s = s0; s < s1; s++) {
-        const floatx4 y = *(const floatx4 *)(Yc + s * SP_E + 4 * lane);
-        const floatx4 *u4 = (const floatx4 *)(Un + (int64_t)rows[s] * 16);
-        float u[16];
-#pragma unroll
-        for (int h = 0; h < 4; h++) {
-            const floatx4 t = u4[h];
-            u[4 * h] = t[0];
-            u[4 * h + 1] = t[1];
-            u[4 * h + 2] = t[2];
-            u[4 * h + 3] = t[3];
-        }
+    auto add_row = [&](const floatx4 &y, const floatx4 (&u4)[4]) {
 #pragma unroll
         for (int q = 0; q < 4; q++)
 #pragma unroll
-            for (int k = 0; k < 16; k++) acc[q][k] = fmaf(y[q], u[k], acc[q][k]);
+            for (int k = 0; k < 16; k++) acc[q][k] = fmaf(y[q], u4[k >> 2][k & 3], acc[q][k]);
+    };
+    int64_t s = s0;
+    for (; s + 4 <= s1; s += 4) {
+        floatx4 y[4], u[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            y[r] = *(const floatx4 *)(Yc + (s + r) * SP_E + 4 * lane);
+            const floatx4 *u4 = (const floatx4 *)(Un + (int64_t)rows[s + r] * 16);
+#pragma unroll
+            for (int h = 0; h < 4; h++) u[r][h] = u4[h];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) add_row(y[r], u[r]);
+    }
+    for (; s < s1; s++) {
+        floatx4 u[4];
+        const floatx4 y = *(const floatx4 *)(Yc + s * SP_E + 4 * lane);
+        const floatx4 *u4 = (const floatx4 *)(Un + (int64_t)rows[s] * 16);
+#pragma unroll
+        for (int h = 0; h < 4; h++) u[h] = u4[h];
+        add_row(y, u);
+    }
+    // waves 0..3 in order: each adds the running sum in LDS to its own
+    for (int w = 0; w < 4; w++) {
+        if (wv == w) {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int h = 0; h < 4; h++) {
+                    floatx4 v = floatx4{acc[q][4 * h], acc[q][4 * h + 1], acc[q][4 * h + 2], acc[q][4 * h + 3]};
+                    if (w > 0) v = red[4 * q + h][lane] + v;
+                    red[4 * q + h][lane] = v;
+                }
+        }
+        __syncthreads();
     }
     float *L = Lwg + (int64_t)blockIdx.x * SP_E * 16;
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-#pragma unroll
-        for (int h = 0; h < 4; h++)
-            *(floatx4 *)(L + (4 * lane + q) * 16 + 4 * h) =
-                floatx4{acc[q][4 * h], acc[q][4 * h + 1], acc[q][4 * h + 2], acc[q][4 * h + 3]};
+    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+        const int qh = e >> 6, l = e & 63;
+        *(floatx4 *)(L + (4 * l + (qh >> 2)) * 16 + 4 * (qh & 3)) = red[qh][l];
+    }
 }
 
 // L (256 x 16, FP64) = sum over workgroups of the partials, fixed order: 16
@@ -968,12 +991,13 @@ extern "C" int dt_sp_mode_arrays(const dt_sp *sp, int n, const int32_t **perm, c
 // FP32 I_n x 16.
 static int sp_ytu_dev(dt_ctx *ctx, dt_sp *sp, int n, const float *Un, const float *Yc, double *L) {
     const dt_sp_mode &m = sp->mode[n];
-    const int64_t nwg = std::min<int64_t>(2048, std::max<int64_t>(1, (m.nrows + 63) / 64));
-    const int64_t rpw = (m.nrows + nwg - 1) / nwg;
-    const int64_t ng = std::max<int64_t>(1, (m.nrows + rpw - 1) / rpw);
+    // at most 8192 waves of at least 16 rows; one partial per 4-wave workgroup
+    const int64_t nw = std::min<int64_t>(8192, std::max<int64_t>(1, (m.nrows + 15) / 16));
+    const int64_t rpw = (m.nrows + nw - 1) / nw;
+    const int64_t ng = std::max<int64_t>(1, ((m.nrows + rpw - 1) / rpw + 3) / 4);
     float *Lwg = (float *)dt_ws(ctx, WS_SPARSE1, (size_t)ng * SP_E * 16 * 4 + 256);
     if (!Lwg) return DT_ENOMEM;
-    DT_LAUNCH("sp_ytu", sp_ytu, dim3((unsigned)ng), dim3(64), 0, Yc, m.rows, m.nrows, Un, rpw, Lwg);
+    DT_LAUNCH("sp_ytu", sp_ytu, dim3((unsigned)ng), dim3(256), 0, Yc, m.rows, m.nrows, Un, rpw, Lwg);
     DT_LAUNCH("sp_lreduce", sp_lreduce, dim3(SP_E * 16 / 16), dim3(256), 0, (const float *)Lwg, (int)ng, L);
     return DT_OK;
 }
