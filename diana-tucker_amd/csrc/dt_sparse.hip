@@ -26,8 +26,9 @@
 // = U_b rows, so D accumulates the 16 x 16 outer-product sum of a row
 // segment).  A row wholly inside a range is written directly; a row cut by
 // range boundaries leaves FP32 partials in carry slots that sp_fixup adds in
-// range order (deterministic).  L = Y_cᵀ U_n is a second, dense MFMA pass
-// over Y_c (sp_ytu) with per-workgroup partials summed in a fixed order in FP64.
+// range order (deterministic).  L = Y_cᵀ U_n is a second, dense pass over Y_c
+// (sp_ytu) with per-workgroup partials summed in a fixed order in FP64; the
+// factor update (sp_tl_gram, sp_rmul) never waits for the host.
 #include "dt_internal.hpp"
 #include "dt_philox.hpp"
 
@@ -260,20 +261,19 @@ __device__ __forceinline__ void sp_issue(const intx4 &z, const float *__restrict
 
 // Every wave owns the contiguous range [P0, P1) of the mode-sorted records
 // (equal sizes, multiples of 64) and walks it in batches of 64 nonzeros.  For
-// batch k it stages the gathered rows into LDS, issues the gathers of batch
-// k + 2 into the registers just freed and the record load of batch k + 4, then
-// runs batch k from LDS; two register sets alternate (the loop body is
-// unrolled by two, so no register holding a load in flight is ever copied).
-// A batch where at most one row starts runs 16 MFMA steps: step t, lane (c, g)
-// reads U_a[i_a, c] and U_b[i_b, c] of nonzero 4t + g and its (row, value);
-// when the four nonzeros share the current row one MFMA adds
-// val * U_a-row ⊗ U_b-row, otherwise one MFMA per row present with the other
-// lanes zeroed.  A batch where more rows start (the short rows of a Zipf
-// tail) adds one nonzero at a time with VALU FMAs in the same accumulator
-// layout, so a row end never waits for an MFMA.  A finished row is written to Y_c when it lies
-// inside the range; otherwise it goes to carry slot 0 (row started before
-// P0) or 1 (row continues past P1), and sp_fixup adds them.
-// Four waves per SIMD (128 registers; LDS allows four workgroups per CU).
+// batch k it stages the gathered rows into LDS (U_a rows multiplied by their
+// value), issues the gathers of batch k + 2 into the registers just freed and
+// the record load of batch k + 4, then runs batch k from LDS; two register
+// sets alternate (the loop body is unrolled by two, so no register holding a
+// load in flight is ever copied).  MFMA step t, lane (c, g) takes
+// val * U_a[i_a, c] and U_b[i_b, c] of nonzero 4t + g.  A batch inside the
+// current row runs its 16 steps with no test between them; a batch where rows
+// start runs, row by row, the steps that hold the row's nonzeros with the
+// other lanes zeroed and flushes the row before the next one starts.  A
+// finished row is written to Y_c when it lies inside the range; otherwise it
+// goes to carry slot 0 (row started before P0) or 1 (row continues past P1),
+// and sp_fixup adds them.  Four waves per SIMD (128 registers; LDS allows
+// four workgroups per CU).
 template <bool OFF32>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void sp_ttmc16(SpModeArgs a, const float *__restrict__ Ua,
                                                  const float *__restrict__ Ub, float *__restrict__ Yc,
