@@ -202,19 +202,31 @@ static int ahat_sq_dev(dt_hooi *h, const void *A, int dt) {
     }
     DT_TRY(dt_convert(ctx, h->G64, DT_F64, h->Gbuf, h->work_dt, (size_t)csz));
     if (h->work_dt == DT_F64) return dt_dot_f64_dev(ctx, h->G64, h->G64, (size_t)csz, h->dscal);
-    // T = G x_n C_n for all n (generic FP64 mode products on the small core)
+    // T = G x_n C_n for all n (generic FP64 mode products on the small core);
+    // the small C_n come from one launch, straight from the FP32 / bf16 copies
     int64_t sh[DT_MAX_NDIM];
     for (int i = 0; i < N; i++) sh[i] = h->R[i];
     const double *cur = h->G64;
+    auto small = [&](int n) { return (double)h->I[n] * h->R[n] * h->R[n] <= (double)(1 << 20); };
+    dt_gram_inverse_batch gb{};
+    for (int n = 0; n < N; n++) {
+        if (!small(n)) continue;
+        const bool b16 = n == h->bf16_core_mode && h->Ub16[n];
+        gb.X[gb.count] = b16 ? (const void *)h->Ub16[n] : (const void *)h->U32[n];
+        gb.dt[gb.count] = b16 ? DT_BF16 : DT_F32;
+        gb.m[gb.count] = (int)h->I[n];
+        gb.n[gb.count] = (int)h->R[n];
+        gb.C[gb.count] = h->Cfix[n];
+        gb.count++;
+    }
+    DT_TRY(dt_gram_inverse_batch_dev(ctx, gb, h->dflag));
     for (int n = 0; n < N; n++) {
         const size_t In = (size_t)h->I[n], Rn = (size_t)h->R[n];
-        if (n == h->bf16_core_mode && h->Ub16[n])
-            DT_TRY(dt_convert(ctx, h->Xfix, DT_F64, h->Ub16[n], DT_BF16, In * Rn));
-        else
-            DT_TRY(dt_convert(ctx, h->Xfix, DT_F64, h->U32[n], DT_F32, In * Rn));
-        if ((double)In * Rn * Rn <= (double)(1 << 20)) {
-            DT_TRY(dt_gram_inverse_dev(ctx, h->Xfix, (int)In, (int)Rn, h->Cfix[n], h->work, h->dflag));
-        } else {
+        if (!small(n)) {
+            if (n == h->bf16_core_mode && h->Ub16[n])
+                DT_TRY(dt_convert(ctx, h->Xfix, DT_F64, h->Ub16[n], DT_BF16, In * Rn));
+            else
+                DT_TRY(dt_convert(ctx, h->Xfix, DT_F64, h->U32[n], DT_F32, In * Rn));
             DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, Rn, Rn, In, 1.0, h->Xfix, In, h->Xfix, In, 0.0, h->Gm, Rn));
             DT_TRY(dt_inverse_flag(ctx, h->Gm, (int)Rn, h->Cfix[n], h->work, h->dflag, 8));
         }

@@ -197,6 +197,14 @@ int dt_als_finish(dt_ctx *ctx, const double *P, const double *W, int I, int R, d
                   double *work, int *flag);
 int dt_cholqr3_dev(dt_ctx *ctx, double *X, int m, int n, double *work, int *flag);
 int dt_gram_inverse_dev(dt_ctx *ctx, const double *X, int m, int n, double *C, double *work, int *flag);
+// up to DT_MAX_NDIM independent (XᵀX)^{-1} in one launch (one workgroup each)
+struct dt_gram_inverse_batch {
+    const void *X[DT_MAX_NDIM];
+    int dt[DT_MAX_NDIM], m[DT_MAX_NDIM], n[DT_MAX_NDIM];
+    double *C[DT_MAX_NDIM];
+    int count;
+};
+int dt_gram_inverse_batch_dev(dt_ctx *ctx, const dt_gram_inverse_batch &b, int *flag);
 int dt_inverse_flag(dt_ctx *ctx, const double *G, int n, double *C, double *work, int *flag, int bit);
 int dt_orth_dev(dt_ctx *ctx, double *X, size_t m, size_t n, double *work, int *flag);
 // Rinv (n x n upper, column-major) = inverse Cholesky factor of G (+ the

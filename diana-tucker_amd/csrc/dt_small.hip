@@ -25,7 +25,12 @@ constexpr int kMaxR = 64;
 
 // Out (R x R, LDS) = Xᵀ Y for X, Y (I x R, column-major, global).  The rows
 // are staged CH at a time through LDS (stage: 2 * (CH+1) * R doubles).
-__device__ void wg_xty(const double *__restrict__ X, const double *__restrict__ Y, int I, int R,
+__device__ inline double ld_f64(const double *p) { return *p; }
+__device__ inline double ld_f64(const float *p) { return (double)*p; }
+__device__ inline double ld_f64(const uint16_t *p) { return (double)bf16_to_f32(*p); }
+
+template <typename T>
+__device__ void wg_xty(const T *__restrict__ X, const T *__restrict__ Y, int I, int R,
                        double *__restrict__ Out, double *__restrict__ stage, int CH) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const int ld = CH + 1;  // odd stride in doubles: column reads hit distinct banks
@@ -39,8 +44,8 @@ __device__ void wg_xty(const double *__restrict__ X, const double *__restrict__ 
         const int ch = (I - t0) < CH ? (I - t0) : CH;
         for (int e = tid; e < CH * R; e += nt) {
             const int r = e % CH, c = e / CH;
-            Xs[r + ld * c] = r < ch ? X[(t0 + r) + (size_t)I * c] : 0.0;
-            Ys[r + ld * c] = r < ch ? Y[(t0 + r) + (size_t)I * c] : 0.0;
+            Xs[r + ld * c] = r < ch ? ld_f64(X + (t0 + r) + (size_t)I * c) : 0.0;
+            Ys[r + ld * c] = r < ch ? ld_f64(Y + (t0 + r) + (size_t)I * c) : 0.0;
         }
         __syncthreads();
 #pragma unroll
@@ -287,6 +292,26 @@ __global__ __launch_bounds__(SMALL_THREADS) void gram_inverse_kernel(const doubl
     for (int i = threadIdx.x; i < n * n; i += blockDim.x) C[i] = G[i];
 }
 
+// Workgroup q: C_q (n_q x n_q) = (X_qᵀX_q)^{-1}, X_q m_q x n_q column-major in
+// FP64, FP32 or bf16 (widened on load).
+__global__ __launch_bounds__(SMALL_THREADS) void gram_inverse_batch_kernel(dt_gram_inverse_batch b,
+                                                                           int *__restrict__ flag) {
+    __shared__ int perm[kMaxR];
+    const int q = blockIdx.x, m = b.m[q], n = b.n[q];
+    double *G = lds_base(), *stage = G + n * n;
+    if (b.dt[q] == DT_F32)
+        wg_xty((const float *)b.X[q], (const float *)b.X[q], m, n, G, stage, stage_rows(n));
+    else if (b.dt[q] == DT_BF16)
+        wg_xty((const uint16_t *)b.X[q], (const uint16_t *)b.X[q], m, n, G, stage, stage_rows(n));
+    else
+        wg_xty((const double *)b.X[q], (const double *)b.X[q], m, n, G, stage, stage_rows(n));
+    if (wg_inverse_inplace(G, n, perm)) {
+        if (threadIdx.x == 0) atomicOr(flag, 8);
+        return;
+    }
+    for (int i = threadIdx.x; i < n * n; i += blockDim.x) b.C[q][i] = G[i];
+}
+
 // inverse of G (n x n, global) into C (global)
 __global__ __launch_bounds__(SMALL_THREADS) void inverse_flag_kernel(const double *__restrict__ G,
                                                                      int n, double *__restrict__ C,
@@ -509,15 +534,28 @@ int dt_inverse_flag(dt_ctx *ctx, const double *G, int n, double *C, double *work
     return DT_OK;
 }
 
-// Device-only orthonormalisation of X (m x n): shifted + 2 plain Cholesky-QR
-// passes.  One workgroup when m n^2 is small, otherwise the Gram and the
-// X R^{-1} products run as multi-workgroup GEMMs.  work: >= 2 n*n + m*n doubles.
+int dt_gram_inverse_batch_dev(dt_ctx *ctx, const dt_gram_inverse_batch &b, int *flag) {
+    int nmax = 1;
+    for (int q = 0; q < b.count; q++) {
+        DT_TRY(check_r(ctx, b.n[q]));
+        nmax = b.n[q] > nmax ? b.n[q] : nmax;
+    }
+    if (b.count < 1) return DT_OK;
+    DT_LAUNCH("fit_small", gram_inverse_batch_kernel, dim3((unsigned)b.count), dim3(SMALL_THREADS), lds_bytes(nmax),
+              b, flag);
+    return DT_OK;
+}
+
 int dt_chol_rinv_dev(dt_ctx *ctx, const double *G, int n, bool shifted, double m_rows, double *Rinv, int *flag) {
     DT_TRY(check_r(ctx, n));
     DT_LAUNCH("cholqr_small", chol_rinv_kernel, dim3(1), dim3(SMALL_THREADS), 2 * n * n * 8, G, n, shifted ? 1 : 0,
               m_rows, Rinv, flag);
     return DT_OK;
 }
+
+// Device-only orthonormalisation of X (m x n): shifted + 2 plain Cholesky-QR
+// passes.  One workgroup when m n^2 is small, otherwise the Gram and the
+// X R^{-1} products run as multi-workgroup GEMMs.  work: >= 2 n*n + m*n doubles.
 
 int dt_orth_dev(dt_ctx *ctx, double *X, size_t m, size_t n, double *work, int *flag) {
     if ((double)m * n * n <= (double)(1 << 21) && n <= 32)
