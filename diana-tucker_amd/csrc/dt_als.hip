@@ -273,6 +273,7 @@ __global__ __launch_bounds__(ALS_THREADS) void als_chol_kernel(const double *__r
 #pragma unroll
         for (int i = 0; i < CM; i++) col[i] = G[i][k];
         int bad = 0;
+        double invr[CM];  // 1 / R[j][j]: one division per pivot, none in the back substitution
 #pragma unroll
         for (int j = 0; j < CM; j++) {
             double d = rl_f64(col[j], j);
@@ -280,11 +281,13 @@ __global__ __launch_bounds__(ALS_THREADS) void als_chol_kernel(const double *__r
                 if (!bad) bad = j + 1;
                 d = 1.0;
             }
-            const double rjj = sqrt(d), inv_d = 1.0 / d;
+            const double rjj = sqrt(d), ir = 1.0 / rjj;
+            invr[j] = ir;
             const double gjk = col[j];
+            const double f = gjk * (ir * ir);
 #pragma unroll
-            for (int i = j + 1; i < CM; i++) col[i] -= rl_f64(col[j], i) * gjk * inv_d;
-            col[j] = (k == j) ? rjj : gjk / rjj;  // row j of R (meaningful for k >= j)
+            for (int i = j + 1; i < CM; i++) col[i] -= rl_f64(col[j], i) * f;
+            col[j] = (k == j) ? rjj : gjk * ir;  // row j of R (meaningful for k >= j)
         }
         if (tid < CM)
 #pragma unroll
@@ -304,7 +307,7 @@ __global__ __launch_bounds__(ALS_THREADS) void als_chol_kernel(const double *__r
                 s0 -= X[i][m] * x[m];
                 if (m + 1 < CM) s1 -= X[i][m + 1] * x[m + 1];
             }
-            x[i] = (s0 + s1) / X[i][i];
+            x[i] = (s0 + s1) * invr[i];
         }
         if (!bad && tid < R)
 #pragma unroll
@@ -357,6 +360,7 @@ __device__ __forceinline__ void af_chol(double (*G)[CM + 1], double (*Ri)[CM + 1
         for (int i = 0; i < CM; i++)
             col[i] = (i < R && k < R) ? G[i][k] + (i == k ? shift : 0.0) : (i == k ? 1.0 : 0.0);
         int bad = 0;
+        double invr[CM];  // 1 / R[j][j]: one division per pivot, none in the back substitution
 #pragma unroll
         for (int j = 0; j < CM; j++) {
             double d = rl_f64(col[j], j);
@@ -364,11 +368,13 @@ __device__ __forceinline__ void af_chol(double (*G)[CM + 1], double (*Ri)[CM + 1
                 if (!bad) bad = j + 1;
                 d = 1.0;
             }
-            const double rjj = sqrt(d), inv_d = 1.0 / d;
+            const double rjj = sqrt(d), ir = 1.0 / rjj;
+            invr[j] = ir;
             const double gjk = col[j];
+            const double f = gjk * (ir * ir);
 #pragma unroll
-            for (int i = j + 1; i < CM; i++) col[i] -= rl_f64(col[j], i) * gjk * inv_d;
-            col[j] = (k == j) ? rjj : gjk / rjj;
+            for (int i = j + 1; i < CM; i++) col[i] -= rl_f64(col[j], i) * f;
+            col[j] = (k == j) ? rjj : gjk * ir;
         }
         if (tid < CM)
 #pragma unroll
@@ -386,7 +392,7 @@ __device__ __forceinline__ void af_chol(double (*G)[CM + 1], double (*Ri)[CM + 1
                 s0 -= X[i][m] * x[m];
                 if (m + 1 < CM) s1 -= X[i][m + 1] * x[m + 1];
             }
-            x[i] = (s0 + s1) / X[i][i];
+            x[i] = (s0 + s1) * invr[i];
         }
         if (tid < CM)
 #pragma unroll
