@@ -360,7 +360,9 @@ __device__ __forceinline__ void af_chol(double (*G)[CM + 1], double (*Ri)[CM + 1
         for (int i = 0; i < CM; i++)
             col[i] = (i < R && k < R) ? G[i][k] + (i == k ? shift : 0.0) : (i == k ? 1.0 : 0.0);
         int bad = 0;
-        double invr[CM];  // 1 / R[j][j]: one division per pivot, none in the back substitution
+        // CM <= 16: one division per pivot, 1 / R[j][j] kept for the back
+        // substitution; wider panels keep the two-division form (registers)
+        double invr[CM <= 16 ? CM : 1];
 #pragma unroll
         for (int j = 0; j < CM; j++) {
             double d = rl_f64(col[j], j);
@@ -368,13 +370,20 @@ __device__ __forceinline__ void af_chol(double (*G)[CM + 1], double (*Ri)[CM + 1
                 if (!bad) bad = j + 1;
                 d = 1.0;
             }
-            const double rjj = sqrt(d), ir = 1.0 / rjj;
-            invr[j] = ir;
             const double gjk = col[j];
-            const double f = gjk * (ir * ir);
+            if constexpr (CM <= 16) {
+                const double rjj = sqrt(d), ir = 1.0 / rjj;
+                invr[j] = ir;
+                const double f = gjk * (ir * ir);
 #pragma unroll
-            for (int i = j + 1; i < CM; i++) col[i] -= rl_f64(col[j], i) * f;
-            col[j] = (k == j) ? rjj : gjk * ir;
+                for (int i = j + 1; i < CM; i++) col[i] -= rl_f64(col[j], i) * f;
+                col[j] = (k == j) ? rjj : gjk * ir;
+            } else {
+                const double rjj = sqrt(d), inv_d = 1.0 / d;
+#pragma unroll
+                for (int i = j + 1; i < CM; i++) col[i] -= rl_f64(col[j], i) * gjk * inv_d;
+                col[j] = (k == j) ? rjj : gjk / rjj;
+            }
         }
         if (tid < CM)
 #pragma unroll
@@ -392,7 +401,10 @@ __device__ __forceinline__ void af_chol(double (*G)[CM + 1], double (*Ri)[CM + 1
                 s0 -= X[i][m] * x[m];
                 if (m + 1 < CM) s1 -= X[i][m + 1] * x[m + 1];
             }
-            x[i] = (s0 + s1) * invr[i];
+            if constexpr (CM <= 16)
+                x[i] = (s0 + s1) * invr[i];
+            else
+                x[i] = (s0 + s1) / X[i][i];
         }
         if (tid < CM)
 #pragma unroll
