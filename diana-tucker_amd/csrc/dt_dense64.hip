@@ -741,9 +741,14 @@ __global__ __launch_bounds__(256) void jacobi_kernel(double *__restrict__ H, int
 }
 
 // Cyclic Jacobi with H and V resident in LDS (n <= LDS_NMAX), same
-// round-robin ordering; the off-diagonal norm is a block reduction.
+// round-robin ordering; the off-diagonal norm is a block reduction.  A round's
+// rotations act on disjoint index pairs, so H <- Jᵀ H J splits into 2 x 2
+// blocks (pair P's rows, pair Q's columns) that one thread each updates in
+// place, column rotation first and row rotation second (the operations, and
+// so the results, of rotating all columns and then all rows); V <- V J in the
+// same phase.  Two barriers per round.
 // One matrix per workgroup: workgroup b solves Hin + b n^2 into Vout + b n^2, w + b n.
-__global__ __launch_bounds__(512) void jacobi_lds_kernel(const double *__restrict__ Hin, int n,
+__global__ __launch_bounds__(1024) void jacobi_lds_kernel(const double *__restrict__ Hin, int n,
                                                         double *__restrict__ Vout, double *__restrict__ w) {
     Hin += (size_t)blockIdx.x * n * n;
     Vout += (size_t)blockIdx.x * n * n;
@@ -754,21 +759,24 @@ __global__ __launch_bounds__(512) void jacobi_lds_kernel(const double *__restric
     __shared__ int rp[LDS_NMAX / 2], rq[LDS_NMAX / 2];
     __shared__ double red_off[16], red_tot[16];
     __shared__ int order[LDS_NMAX];
+    __shared__ int rotated;  // a rotation of this sweep changed something
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int lane = tid & 63, wv = tid >> 6;
-    const int np = (n + 1) & ~1;
-    for (int idx = tid; idx < n * n; idx += nt) {
-        H[idx] = Hin[idx];
-        V[idx] = (idx % n == idx / n) ? 1.0 : 0.0;
-    }
+    const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+    const int np = (n + 1) & ~1, npair = np / 2;  // npair <= 48 < 64
+    for (int j = wv; j < n; j += nw)
+        for (int i = lane; i < n; i += 64) {
+            H[i + n * j] = Hin[i + n * j];
+            V[i + n * j] = i == j ? 1.0 : 0.0;
+        }
     __syncthreads();
     for (int sweep = 0; sweep < 40; sweep++) {
         double off = 0.0, tot = 0.0;
-        for (int idx = tid; idx < n * n; idx += nt) {
-            const double h = H[idx];
-            tot += h * h;
-            if (idx % n != idx / n) off += h * h;
-        }
+        for (int j = wv; j < n; j += nw)
+            for (int i = lane; i < n; i += 64) {
+                const double h = H[i + n * j];
+                tot += h * h;
+                if (i != j) off += h * h;
+            }
         for (int o = 32; o > 0; o >>= 1) {
             off += __shfl_xor(off, o);
             tot += __shfl_xor(tot, o);
@@ -780,14 +788,16 @@ __global__ __launch_bounds__(512) void jacobi_lds_kernel(const double *__restric
         __syncthreads();
         off = 0.0;
         tot = 0.0;
-        for (int i = 0; i < nt / 64; i++) {
+        for (int i = 0; i < nw; i++) {
             off += red_off[i];
             tot += red_tot[i];
         }
         __syncthreads();  // red_* reused next sweep
         if (!(off > 1e-30 * tot) || tot == 0.0) break;  // uniform
+        if (tid == 0) rotated = 0;
+        __syncthreads();
         for (int round = 0; round < np - 1; round++) {
-            for (int q = tid; q < np / 2; q += nt) {
+            for (int q = tid; q < npair; q += nt) {
                 int a, b;
                 if (q == 0) {
                     a = 0;
@@ -797,11 +807,14 @@ __global__ __launch_bounds__(512) void jacobi_lds_kernel(const double *__restric
                     b = 1 + ((round + np - 1 - q) % (np - 1));
                 }
                 const int p = a < b ? a : b, qq = a < b ? b : a;
-                double c = 1.0, sn = 0.0;
+                double c = 1.0, sn = 0.0;  // identity for the pair holding the padding index n
                 if (qq < n) {
                     const double apq = H[p + n * qq];
-                    if (fabs(apq) > 1e-300) {
-                        const double app = H[p + n * p], aqq = H[qq + n * qq];
+                    const double app = H[p + n * p], aqq = H[qq + n * qq];
+                    // a rotation is skipped when apq is at the rounding level of the
+                    // diagonal pair (a few ulps: rotations leave that much behind)
+                    if (fabs(apq) > 1e-300 && fabs(apq) > 1e-15 * sqrt(fabs(app * aqq))) {
+                        rotated = 1;
                         const double tau = (aqq - app) / (2.0 * apq);
                         const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                         c = 1.0 / sqrt(1.0 + t * t);
@@ -814,30 +827,69 @@ __global__ __launch_bounds__(512) void jacobi_lds_kernel(const double *__restric
                 rsn[q] = sn;
             }
             __syncthreads();
-            for (int idx = tid; idx < (np / 2) * n; idx += nt) {
-                const int q = idx / n, r = idx % n;
-                const int p = rp[q], qq = rq[q];
-                if (qq >= n) continue;
-                const double c = rc[q], sn = rsn[q];
-                const double hp = H[r + n * p], hq = H[r + n * qq];
-                H[r + n * p] = c * hp - sn * hq;
-                H[r + n * qq] = sn * hp + c * hq;
-                const double vp = V[r + n * p], vq = V[r + n * qq];
-                V[r + n * p] = c * vp - sn * vq;
-                V[r + n * qq] = sn * vp + c * vq;
+            // H blocks (pair P of rows, pair Q = lane of columns) and V columns
+            // (pair P, rows lane and lane + 64); the waves take P = wv, wv + 16,
+            // wv + 32 (npair <= 48), and every LDS read of the phase is issued
+            // before the arithmetic
+            constexpr int JP = (LDS_NMAX / 2 + 15) / 16;
+            const int Q = lane < npair ? lane : npair - 1;
+            const int q0 = rp[Q], q1 = rq[Q];
+            const double cQ = rc[Q], sQ = rsn[Q];
+            const bool k1 = q1 < n;
+            int p0[JP], p1[JP];
+            double cP[JP], sP[JP], h[JP][4], v[JP][4];
+#pragma unroll
+            for (int i = 0; i < JP; i++) {
+                const int P = wv + 16 * i < npair ? wv + 16 * i : npair - 1;
+                p0[i] = rp[P];
+                p1[i] = rq[P];
+                cP[i] = rc[P];
+                sP[i] = rsn[P];
+                const int r1 = p1[i] < n ? p1[i] : p0[i];
+                const int c1 = k1 ? q1 : q0;
+                h[i][0] = H[p0[i] + n * q0];
+                h[i][1] = H[p0[i] + n * c1];
+                h[i][2] = H[r1 + n * q0];
+                h[i][3] = H[r1 + n * c1];
+                const int ra = lane < n ? lane : n - 1, rb = lane + 64 < n ? lane + 64 : n - 1;
+                const int v1 = p1[i] < n ? p1[i] : p0[i];
+                v[i][0] = V[ra + n * p0[i]];
+                v[i][1] = V[ra + n * v1];
+                v[i][2] = V[rb + n * p0[i]];
+                v[i][3] = V[rb + n * v1];
             }
-            __syncthreads();
-            for (int idx = tid; idx < (np / 2) * n; idx += nt) {
-                const int q = idx / n, r = idx % n;
-                const int p = rp[q], qq = rq[q];
-                if (qq >= n) continue;
-                const double c = rc[q], sn = rsn[q];
-                const double hp = H[p + n * r], hq = H[qq + n * r];
-                H[p + n * r] = c * hp - sn * hq;
-                H[qq + n * r] = sn * hp + c * hq;
+#pragma unroll
+            for (int i = 0; i < JP; i++) {
+                if (wv + 16 * i >= npair) break;  // uniform
+                const bool r1 = p1[i] < n;
+                if (lane < npair) {
+                    const double h00 = h[i][0], h01 = k1 ? h[i][1] : 0.0;
+                    const double h10 = r1 ? h[i][2] : 0.0, h11 = (r1 && k1) ? h[i][3] : 0.0;
+                    // columns: (x0, x1) = (cQ h0 - sQ h1, sQ h0 + cQ h1) per row
+                    const double x00 = cQ * h00 - sQ * h01, x01 = sQ * h00 + cQ * h01;
+                    const double x10 = cQ * h10 - sQ * h11, x11 = sQ * h10 + cQ * h11;
+                    // rows: (y0, y1) = (cP x0 - sP x1, sP x0 + cP x1) per column
+                    H[p0[i] + n * q0] = cP[i] * x00 - sP[i] * x10;
+                    if (k1) H[p0[i] + n * q1] = cP[i] * x01 - sP[i] * x11;
+                    if (r1) H[p1[i] + n * q0] = sP[i] * x00 + cP[i] * x10;
+                    if (r1 && k1) H[p1[i] + n * q1] = sP[i] * x01 + cP[i] * x11;
+                }
+                // V <- V J on pair P's columns (the padding pair has none to rotate)
+                if (r1) {
+                    const double c = cP[i], sn = sP[i];
+                    if (lane < n) {
+                        V[lane + n * p0[i]] = c * v[i][0] - sn * v[i][1];
+                        V[lane + n * p1[i]] = sn * v[i][0] + c * v[i][1];
+                    }
+                    if (lane + 64 < n) {
+                        V[lane + 64 + n * p0[i]] = c * v[i][2] - sn * v[i][3];
+                        V[lane + 64 + n * p1[i]] = sn * v[i][2] + c * v[i][3];
+                    }
+                }
             }
             __syncthreads();
         }
+        if (!rotated) break;  // uniform: read after the round's barrier
     }
     // descending order by rank (ties broken by index)
     for (int i = tid; i < n; i += nt) {
@@ -861,7 +913,7 @@ __global__ __launch_bounds__(512) void jacobi_lds_kernel(const double *__restric
 // workgroup each.
 int dt_jacobi_eig_batched(dt_ctx *ctx, double *H, size_t n, int count, double *V, double *w) {
     if (n > (size_t)LDS_NMAX || count < 1) return dt_fail(ctx, DT_EUNSUPPORTED, "batched Jacobi: n %zu", n);
-    DT_LAUNCH("jacobi", jacobi_lds_kernel, dim3((unsigned)count), dim3(512), 0, H, (int)n, V, w);
+    DT_LAUNCH("jacobi", jacobi_lds_kernel, dim3((unsigned)count), dim3(1024), 0, H, (int)n, V, w);
     return DT_OK;
 }
 
@@ -871,7 +923,7 @@ int dt_jacobi_eig(dt_ctx *ctx, double *H, size_t n, double *V, double *w) {
     if (!rot) return DT_ENOMEM;
     int *order = (int *)(rot + 2 * np + 8);
     if (n <= (size_t)LDS_NMAX) {
-        DT_LAUNCH("jacobi", jacobi_lds_kernel, dim3(1), dim3(512), 0, H, (int)n, V, w);
+        DT_LAUNCH("jacobi", jacobi_lds_kernel, dim3(1), dim3(1024), 0, H, (int)n, V, w);
         return DT_OK;
     }
     DT_LAUNCH("jacobi", jacobi_kernel, dim3(1), dim3(256), 0, H, (int)n, V, w, rot, order);

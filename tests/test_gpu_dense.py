@@ -274,7 +274,7 @@ def test_inverse_singular_reports_enumeric(ctx):
     assert e.value.code == dt.DT_ENUMERIC
 
 
-@pytest.mark.parametrize("n,r", [(8, 3), (64, 8), (150, 20), (256, 16), (1024, 64)])
+@pytest.mark.parametrize("n,r", [(8, 3), (33, 5), (64, 8), (81, 9), (150, 20), (256, 16), (1024, 64)])
 def test_syev_top(ctx, n, r):
     print("syev", n, r, flush=True)
     rng = np.random.default_rng(14)
