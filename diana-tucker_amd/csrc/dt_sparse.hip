@@ -307,12 +307,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         __builtin_nontemporal_store(acc0 + acc1, (floatx4 *)(dst + e0));
     };
 
-    auto compute = [&](int64_t b0, int ks) {
-        // rows starting in this batch (lane k: nonzero k; padding has row -1)
-        const intx2 svl = L.sv[ks][lane];
-        const int prevr = __builtin_amdgcn_ds_bpermute((lane == 0 ? 0 : lane - 1) * 4, svl.x);
-        const uint64_t valid = __ballot(svl.x >= 0);
-        const uint64_t heads = __ballot(svl.x >= 0 && svl.x != (lane == 0 ? cur : prevr));
+    // heads: the lanes where a row starts (bit k: nonzero k), valid: the
+    // lanes that are not padding; both found when the batch's records were
+    // issued (row_marks), so a batch inside one row reads nothing but its
+    // operands here
+    auto compute = [&](int ks, uint64_t heads, uint64_t valid) {
         const int nh = __popcll(heads);
         if (nh == 0 && valid == ~0ull) {
             // the whole batch continues the current row: 16 MFMA steps with
@@ -347,7 +346,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             }
             if (hi >= nvalid) break;
             flush(false);
-            cur = __builtin_amdgcn_readlane(svl.x, hi);
+            cur = L.sv[ks][hi].x;
             acc0 = floatx4{0.f, 0.f, 0.f, 0.f};
             acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
             hm &= hm - 1;
@@ -355,9 +354,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         }
     };
 
-    // batch k (first nonzero b0, ring slot ks) with register set (z, ga, gb):
-    // z holds the records of batch k + 2, ga/gb the gathers of batch k
-    auto step = [&](int64_t b0, int ks, intx4 &z, floatx4 (&ga)[4], floatx4 (&gb)[4]) {
+    // heads / valid of a batch from its records (row -1 past P1): lane k's
+    // predecessor row comes from lane k - 1 by a wave shift, lane 0's is the
+    // last row of the batch issued before (the first row of the range for
+    // the first batch), which is the current row when the batch is computed
+    int lastq = first_row;
+    auto row_marks = [&](int row, uint64_t &heads, uint64_t &valid) {
+        const int prev = __builtin_amdgcn_update_dpp(lastq, row, 0x138, 0xf, 0xf, false);  // wave_shr:1
+        valid = __ballot(row >= 0);
+        heads = __ballot(row >= 0 && row != prev);
+        lastq = __builtin_amdgcn_readlane(row, 63);
+    };
+
+    // batch k (first nonzero b0, ring slot ks) with register set (z, ga, gb,
+    // hd, vd): z holds the records of batch k + 2, ga/gb the gathers of batch
+    // k, hd/vd its heads and valid lanes
+    auto step = [&](int64_t b0, int ks, intx4 &z, floatx4 (&ga)[4], floatx4 (&gb)[4], uint64_t &hd,
+                    uint64_t &vd) {
         wave_sync();  // the previous batch's LDS reads are done
         // U_a rows are staged already scaled by their nonzero's value
         // (padding has value 0)
@@ -370,20 +383,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         }
         const int ks2 = ks == 0 ? 2 : ks - 1;  // (ks + 2) mod 3
         const bool ok = b0 + 128 + lane < P1;
-        L.sv[ks2][lane] = ok ? intx2{z.z, z.w} : intx2{-1, 0};
+        const int rown = ok ? z.z : -1;
+        L.sv[ks2][lane] = intx2{rown, ok ? z.w : 0};
+        const uint64_t h = hd, v = vd;
+        row_marks(rown, hd, vd);
         sp_issue<OFF32>(z, Ua, Ub, p4, c4, ga, gb);
         z = sp_rec_load(rec, b0 + 256 + lane, P1);
         wave_sync();
-        compute(b0, ks);
+        compute(ks, h, v);
     };
 
     intx4 zX = sp_rec_load(rec, P0 + lane, P1);
     intx4 zY = sp_rec_load(rec, P0 + 64 + lane, P1);
     floatx4 gaX[4], gbX[4], gaY[4], gbY[4];
-    L.sv[0][lane] = P0 + lane < P1 ? intx2{zX.z, zX.w} : intx2{-1, 0};
+    uint64_t hX, vX, hY, vY;
+    {
+        const bool ok = P0 + lane < P1;
+        const int row = ok ? zX.z : -1;
+        L.sv[0][lane] = intx2{row, ok ? zX.w : 0};
+        row_marks(row, hX, vX);
+    }
     sp_issue<OFF32>(zX, Ua, Ub, p4, c4, gaX, gbX);
     zX = sp_rec_load(rec, P0 + 128 + lane, P1);
-    L.sv[1][lane] = P0 + 64 + lane < P1 ? intx2{zY.z, zY.w} : intx2{-1, 0};
+    {
+        const bool ok = P0 + 64 + lane < P1;
+        const int row = ok ? zY.z : -1;
+        L.sv[1][lane] = intx2{row, ok ? zY.w : 0};
+        row_marks(row, hY, vY);
+    }
     sp_issue<OFF32>(zY, Ua, Ub, p4, c4, gaY, gbY);
     zY = sp_rec_load(rec, P0 + 192 + lane, P1);
     // a counted loop over pairs of batches (padding past P1 has row -1 and
@@ -392,13 +419,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     int ks = 0;
     int64_t b0 = P0;
     for (int64_t i = 0; i < nb / 2; i++) {
-        step(b0, ks, zX, gaX, gbX);
+        step(b0, ks, zX, gaX, gbX, hX, vX);
         ks = ks == 2 ? 0 : ks + 1;
-        step(b0 + 64, ks, zY, gaY, gbY);
+        step(b0 + 64, ks, zY, gaY, gbY, hY, vY);
         ks = ks == 2 ? 0 : ks + 1;
         b0 += 128;
     }
-    if (nb & 1) step(b0, ks, zX, gaX, gbX);
+    if (nb & 1) step(b0, ks, zX, gaX, gbX, hX, vX);
     flush(true);
 }
 
