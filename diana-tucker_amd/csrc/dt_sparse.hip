@@ -312,13 +312,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     // issued (row_marks), so a batch inside one row reads nothing but its
     // operands here
     auto compute = [&](int ks, uint64_t heads, uint64_t valid) {
-        const int nh = __popcll(heads);
-        if (nh == 0 && valid == ~0ull) {
+        // one base address per operand; step t is a constant offset from it
+        const float *pa = L.a + g * 16 + c;
+        const float *pb = L.b + g * 16 + c;
+        if (heads == 0 && valid == ~0ull) {
             // the whole batch continues the current row: 16 MFMA steps with
             // no test between them
-            // one base address per operand; step t is a constant offset from it
-            const float *pa = L.a + g * 16 + c;
-            const float *pb = L.b + g * 16 + c;
 #pragma unroll
             for (int t = 0; t < 16; t++) {
                 const float av = pa[64 * t];
@@ -330,27 +329,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             }
             return;
         }
-        // row starts and/or a partial last batch: each row's nonzeros
-        // [lo, hi) in the MFMA steps that hold them, the others masked; a
-        // row that ends here is flushed before the next one starts
+        // row starts and/or a partial last batch: step by step (operands read
+        // four steps at a time), the lanes of the current row in each step,
+        // the others masked; at a row start inside a step the finished row is
+        // flushed and the step goes on with the next row
         const int nvalid = __popcll(valid);
-        uint64_t hm = heads;
-        int lo = 0;
-        for (;;) {
-            const int hi = hm ? __builtin_ctzll(hm) : nvalid;
-            for (int t = lo >> 2; t < (hi + 3) >> 2; t++) {
-                const int k = 4 * t + g;
-                const float av = L.a[k * 16 + c];
-                const float bv = L.b[k * 16 + c];
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(k >= lo && k < hi ? av : 0.f, bv, acc0, 0, 0, 0);
+        for (int t4 = 0; t4 < 4; t4++) {
+            if (16 * t4 >= nvalid) break;  // uniform
+            float a4[4], b4[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                a4[j] = pa[64 * (4 * t4 + j)];
+                b4[j] = pb[64 * (4 * t4 + j)];
             }
-            if (hi >= nvalid) break;
-            flush(false);
-            cur = L.sv[ks][hi].x;
-            acc0 = floatx4{0.f, 0.f, 0.f, 0.f};
-            acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
-            hm &= hm - 1;
-            lo = hi;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int t = 4 * t4 + j;
+                const int k = 4 * t + g;
+                int lo = 4 * t;
+                uint32_t hs = (uint32_t)(heads >> (4 * t)) & 0xfu;
+                while (hs) {
+                    const int h = 4 * t + __builtin_ctz(hs);
+                    if (h > lo)
+                        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(k >= lo && k < h ? a4[j] : 0.f, b4[j], acc0, 0, 0,
+                                                                    0);
+                    flush(false);
+                    cur = L.sv[ks][h].x;
+                    acc0 = floatx4{0.f, 0.f, 0.f, 0.f};
+                    acc1 = floatx4{0.f, 0.f, 0.f, 0.f};
+                    lo = h;
+                    hs &= hs - 1;
+                }
+                const int hi = 4 * t + 4 < nvalid ? 4 * t + 4 : nvalid;
+                if (hi > lo)
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(k >= lo && k < hi ? a4[j] : 0.f, b4[j], acc0, 0, 0, 0);
+            }
         }
     };
 
