@@ -217,6 +217,9 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// ranges per chunk of a long row's chain of carry slots (sp_fixup)
+constexpr int SP_FC = 64;
+
 // Per-wave LDS: the staged batch's U_a and U_b rows [nonzero][column] and a
 // ring of three batches' (compact row, value) pairs.
 constexpr int SP_RING = 3;
@@ -277,7 +280,8 @@ __device__ __forceinline__ void sp_issue(const intx4 &z, const float *__restrict
 template <bool OFF32>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void sp_ttmc16(SpModeArgs a, const float *__restrict__ Ua,
                                                  const float *__restrict__ Ub, float *__restrict__ Yc,
-                                                 float *__restrict__ carry, int64_t per_wave, int64_t nwaves) {
+                                                 float *__restrict__ carry, int64_t per_wave, int64_t nwaves,
+                                                 int *__restrict__ fix_list, int *__restrict__ fix_count) {
     __shared__ SpLds lds_all[4];
     // the wave index is uniform: readfirstlane lets the compiler keep the
     // range bounds and the batch loop scalar
@@ -440,42 +444,117 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     }
     if (nb & 1) step(b0, ks, zX, gaX, gbX, hX, vX);
     flush(true);
+    // ranges with work for sp_fixup: the owner of a row that continues past
+    // P1, and the start of every SP_FC-th range inside a longer row's chain
+    const bool owner = after == cur && !(cur == first_row && left_inc);
+    bool chunk = false;
+    if (left_inc) {
+        const int64_t o = a.rowptr[a.row[P0]] / per_wave;
+        chunk = (w - o) % SP_FC == 0;
+    }
+    if ((owner || chunk) && lane == 0) fix_list[atomicAdd(fix_count, 1)] = (int)w;
 }
 
-// Rows cut by wave ranges: the wave whose range holds the row's first nonzero
-// adds slot 1 of its own range and slot 0 of every later range the row
-// reaches.  A Zipf-hot row spans thousands of ranges, so sixteen 64-thread
-// groups take every sixteenth range (four loads in flight each) and are then
-// added in group order: a fixed order for a given range size.
-__global__ __launch_bounds__(1024) void sp_fixup(SpModeArgs a, const float *__restrict__ carry, int64_t per_wave,
-                                                 int64_t nwaves, float *__restrict__ Yc) {
-    __shared__ floatx4 part[16][64];
-    const int64_t w = blockIdx.x;
-    const int64_t P0 = w * per_wave;
-    const int64_t P1 = P0 + per_wave < a.nnz ? P0 + per_wave : a.nnz;
-    if (P0 >= P1) return;
-    const int r = a.row[P1 - 1];
-    const int64_t rs = a.rowptr[r], re = a.rowptr[r + 1];
-    if (re <= P1 || rs < P0) return;  // ends inside, or owned by an earlier range
-    const int64_t we = (re - 1) / per_wave;
+// Rows cut by wave ranges: the row's partials are slot 1 of the range that
+// holds its first nonzero (the owner) and slot 0 of every later range it
+// reaches.  A Zipf-hot row spans thousands of ranges, so its chain is cut into
+// chunks of SP_FC ranges summed in parallel (sp_fixup) and the chunk sums are
+// added by the owner (sp_fixup_chunks); chains of one chunk are finished in
+// the first kernel.  Every sum runs in a fixed order for a given range size.
+// Range w's chunk of the chain of row r that starts at range s0 = w (owner
+// when w holds the row's first nonzero, else a chunk start inside the chain):
+// the sum of slots [w, min(w + SP_FC, we + 1)), four 64-thread groups taking
+// every fourth slot, added in group order.
+__device__ __forceinline__ floatx4 sp_chunk_sum(const float *__restrict__ carry, int64_t w, bool owner, int64_t we,
+                                                 floatx4 (*part)[64]) {
     const int grp = threadIdx.x >> 6, t = threadIdx.x & 63;
-    auto slot = [&](int64_t q) {
-        return *(const floatx4 *)(carry + (q * 2 + (q == w ? 1 : 0)) * SP_E + 4 * t);
-    };
+    const int64_t qe = w + SP_FC - 1 < we ? w + SP_FC - 1 : we;
     floatx4 s0 = {0.f, 0.f, 0.f, 0.f}, s1 = s0, s2 = s0, s3 = s0;
     int64_t q = w + grp;
-    for (; q + 48 <= we; q += 64) {
+    auto slot = [&](int64_t x) {
+        return *(const floatx4 *)(carry + (x * 2 + (owner && x == w ? 1 : 0)) * SP_E + 4 * t);
+    };
+    for (; q + 12 <= qe; q += 16) {
         s0 += slot(q);
-        s1 += slot(q + 16);
-        s2 += slot(q + 32);
-        s3 += slot(q + 48);
+        s1 += slot(q + 4);
+        s2 += slot(q + 8);
+        s3 += slot(q + 12);
     }
-    for (; q <= we; q += 16) s0 += slot(q);
+    for (; q <= qe; q += 4) s0 += slot(q);
     part[grp][t] = (s0 + s1) + (s2 + s3);
     __syncthreads();
-    if (grp == 0) {
-        floatx4 tot = part[0][t];
-        for (int gq = 1; gq < 16; gq++) tot += part[gq][t];
+    return ((part[0][t] + part[1][t]) + part[2][t]) + part[3][t];
+}
+
+__global__ __launch_bounds__(256) void sp_fixup(SpModeArgs a, const float *__restrict__ carry, int64_t per_wave,
+                                                float *__restrict__ csum, float *__restrict__ Yc,
+                                                const int *__restrict__ fix_list, const int *__restrict__ fix_count) {
+    __shared__ floatx4 part[4][64];
+    const int nlist = *fix_count;
+    for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
+    __syncthreads();  // part is reused across entries
+    const int64_t w = fix_list[li];
+    const int64_t P0 = w * per_wave;
+    const int64_t P1 = P0 + per_wave < a.nnz ? P0 + per_wave : a.nnz;
+    const int t = threadIdx.x & 63;
+    // role 1: owner of the chain of its last row
+    {
+        const int r = a.row[P1 - 1];
+        const int64_t rs = a.rowptr[r], re = a.rowptr[r + 1];
+        if (re > P1 && rs >= P0) {  // uniform
+            const int64_t we = (re - 1) / per_wave;
+            const floatx4 v = sp_chunk_sum(carry, w, true, we, part);
+            if (threadIdx.x < 64) {
+                float *dst = we - w < SP_FC ? Yc + (int64_t)a.slot[r] * SP_E : csum + (w * 2 + 1) * SP_E;
+                *(floatx4 *)(dst + 4 * t) = v;
+            }
+            __syncthreads();  // part is reused below
+        }
+    }
+    // role 2: start of a later chunk of the chain of its first row
+    {
+        const int r = a.row[P0];
+        const int64_t rs = a.rowptr[r], re = a.rowptr[r + 1];
+        if (rs < P0) {  // uniform
+            const int64_t o = rs / per_wave;
+            if ((w - o) % SP_FC == 0) {
+                const int64_t we = (re - 1) / per_wave;
+                const floatx4 v = sp_chunk_sum(carry, w, false, we, part);
+                if (threadIdx.x < 64) *(floatx4 *)(csum + (w * 2) * SP_E + 4 * t) = v;
+            }
+        }
+    }
+    }
+}
+
+// Owners of chains longer than one chunk: the chunk sums in chain order.
+__global__ __launch_bounds__(64) void sp_fixup_chunks(SpModeArgs a, const float *__restrict__ csum,
+                                                      int64_t per_wave, float *__restrict__ Yc,
+                                                      const int *__restrict__ fix_list,
+                                                      const int *__restrict__ fix_count) {
+    const int nlist = *fix_count;
+    for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
+        const int64_t w = fix_list[li];
+        const int64_t P0 = w * per_wave;
+        const int64_t P1 = P0 + per_wave < a.nnz ? P0 + per_wave : a.nnz;
+        const int r = a.row[P1 - 1];
+        const int64_t rs = a.rowptr[r], re = a.rowptr[r + 1];
+        if (re <= P1 || rs < P0) continue;  // not an owner
+        const int64_t we = (re - 1) / per_wave;
+        if (we - w < SP_FC) continue;  // finished by sp_fixup
+        const int t = threadIdx.x;
+        auto chunk = [&](int64_t s) { return *(const floatx4 *)(csum + (s * 2) * SP_E + 4 * t); };
+        // eight chunk sums in flight, added in a fixed order
+        floatx4 acc[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+        int64_t s = w + SP_FC;
+        for (; s + 7 * SP_FC <= we; s += 8 * SP_FC)
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc[u] += chunk(s + u * SP_FC);
+        for (; s <= we; s += SP_FC) acc[0] += chunk(s);
+        const floatx4 tot = *(const floatx4 *)(csum + (w * 2 + 1) * SP_E + 4 * t) +
+                            (((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7])));
         *(floatx4 *)(Yc + (int64_t)a.slot[r] * SP_E + 4 * t) = tot;
     }
 }
@@ -1066,17 +1145,27 @@ static int sp_ttmc_dev(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, fl
     int64_t per_wave = ((sp->nnz + maxw - 1) / maxw + 63) / 64 * 64;
     if (per_wave < 64) per_wave = 64;
     const int64_t nwaves = (sp->nnz + per_wave - 1) / per_wave;
-    float *carry = (float *)dt_ws(ctx, WS_SPARSE0, (size_t)nwaves * 2 * SP_E * 4 + 512);
+    // carry slots (2 per range), the chunk sums of long chains (2 per range),
+    // the list of ranges with fixup work and its length
+    float *carry = (float *)dt_ws(ctx, WS_SPARSE0, (size_t)nwaves * (4 * SP_E * 4 + 4) + 512);
     if (!carry) return DT_ENOMEM;
+    float *csum = carry + (size_t)nwaves * 2 * SP_E;
+    int *fix_list = (int *)(csum + (size_t)nwaves * 2 * SP_E);
+    int *fix_count = fix_list + nwaves;
+    DT_TRY(dt_zero(ctx, fix_count, sizeof(int)));
     static const char *names[3] = {"sp_ttmc_m0", "sp_ttmc_m1", "sp_ttmc_m2"};
     if (off32) {
         DT_LAUNCH(names[n], sp_ttmc16<true>, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, args, Ur[a], Ur[b], Yc,
-                  carry, per_wave, nwaves);
+                  carry, per_wave, nwaves, fix_list, fix_count);
     } else {
         DT_LAUNCH(names[n], sp_ttmc16<false>, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, args, Ur[a], Ur[b],
-                  Yc, carry, per_wave, nwaves);
+                  Yc, carry, per_wave, nwaves, fix_list, fix_count);
     }
-    DT_LAUNCH("sp_fixup", sp_fixup, dim3((unsigned)nwaves), dim3(1024), 0, args, carry, per_wave, nwaves, Yc);
+    const unsigned nfix = (unsigned)std::min<int64_t>(nwaves, 1024);  // workgroups walking the list
+    DT_LAUNCH("sp_fixup", sp_fixup, dim3(nfix), dim3(256), 0, args, carry, per_wave, csum, Yc, (const int *)fix_list,
+              (const int *)fix_count);
+    DT_LAUNCH("sp_fixup", sp_fixup_chunks, dim3(nfix), dim3(64), 0, args, (const float *)csum, per_wave, Yc,
+              (const int *)fix_list, (const int *)fix_count);
     if (L) DT_TRY(sp_ytu_dev(ctx, sp, n, Ur[n], Yc, L));
     return DT_OK;
 }
