@@ -25,8 +25,8 @@
 // time are one v_mfma_f32_16x16x4_f32 (A operand = val * U_a rows, B operand
 // = U_b rows, so D accumulates the 16 x 16 outer-product sum of a row
 // segment).  A row wholly inside a range is written directly; a row cut by
-// range boundaries leaves FP32 partials in carry slots that sp_fixup adds in
-// range order (deterministic).  L = Y_cᵀ U_n is a second, dense pass over Y_c
+// range boundaries leaves FP32 partials in carry slots that sp_fixup and
+// sp_fixup_chunks add in a fixed order (deterministic).  L = Y_cᵀ U_n is a second, dense pass over Y_c
 // (sp_ytu) with per-workgroup partials summed in a fixed order in FP64; the
 // factor update (sp_tl_gram, sp_rmul) never waits for the host.
 #include "dt_internal.hpp"
