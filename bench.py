@@ -374,7 +374,7 @@ def main():
         "data": "synthetic (seeded Tucker model generated on the device)",
         "config": {"workload": "dense HOOI 256x256x256x256 FP32, core 16^4, noise 1e-2, "
                                "ranks (16,16,16,16), HOSVD init, dimension-tree schedule "
-                               "(2 full-tensor passes per sweep, the second fusing modes 0 and 1)",
+                               "(2 full-tensor passes per sweep: modes 2 and 3 fused, then modes 0 and 1 fused)",
                    "residual_after": r["res"][-1],
                    "tflops_per_s": r["flops_per_sweep"] * value / 1e12,
                    "init_ms": r["init_ms"],
