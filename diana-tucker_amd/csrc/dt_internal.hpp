@@ -196,7 +196,6 @@ int dt_als_prep(dt_ctx *ctx, const double *L, int I, int R, double *P, double *w
 int dt_als_finish(dt_ctx *ctx, const double *P, const double *W, int I, int R, double *L,
                   double *work, int *flag);
 int dt_cholqr3_dev(dt_ctx *ctx, double *X, int m, int n, double *work, int *flag);
-int dt_gram_inverse_dev(dt_ctx *ctx, const double *X, int m, int n, double *C, double *work, int *flag);
 // up to DT_MAX_NDIM independent (XᵀX)^{-1} in one launch (one workgroup each)
 struct dt_gram_inverse_batch {
     const void *X[DT_MAX_NDIM];

@@ -277,21 +277,6 @@ __global__ __launch_bounds__(SMALL_THREADS) void cholqr3_kernel(double *X, int m
     }
 }
 
-// C (n x n) = (XᵀX)^{-1}; X m x n.
-__global__ __launch_bounds__(SMALL_THREADS) void gram_inverse_kernel(const double *__restrict__ X,
-                                                                     int m, int n,
-                                                                     double *__restrict__ C,
-                                                                     int *__restrict__ flag) {
-    __shared__ int perm[kMaxR];
-    double *G = lds_base(), *stage = G + n * n;
-    wg_xty(X, X, m, n, G, stage, stage_rows(n));
-    if (wg_inverse_inplace(G, n, perm)) {
-        if (threadIdx.x == 0) atomicOr(flag, 8);
-        return;
-    }
-    for (int i = threadIdx.x; i < n * n; i += blockDim.x) C[i] = G[i];
-}
-
 // Workgroup q: C_q (n_q x n_q) = (X_qᵀX_q)^{-1}, X_q m_q x n_q column-major in
 // FP64, FP32 or bf16 (widened on load).
 __global__ __launch_bounds__(SMALL_THREADS) void gram_inverse_batch_kernel(dt_gram_inverse_batch b,
@@ -516,14 +501,6 @@ int dt_cholqr3_dev(dt_ctx *ctx, double *X, int m, int n, double *work, int *flag
     DT_TRY(check_r(ctx, n));
     const size_t bytes = (size_t)(2 * n * n + 2 * (stage_rows(n) + 1) * n) * sizeof(double);
     DT_LAUNCH("cholqr_small", cholqr3_kernel, dim3(1), dim3(SMALL_THREADS), bytes, X, m, n, flag);
-    return DT_OK;
-}
-int dt_gram_inverse_dev(dt_ctx *ctx, const double *X, int m, int n, double *C, double *work,
-                        int *flag) {
-    (void)work;
-    DT_TRY(check_r(ctx, n));
-    DT_LAUNCH("fit_small", gram_inverse_kernel, dim3(1), dim3(SMALL_THREADS), lds_bytes(n), X, m, n, C,
-              flag);
     return DT_OK;
 }
 int dt_inverse_flag(dt_ctx *ctx, const double *G, int n, double *C, double *work, int *flag, int bit) {
