@@ -207,7 +207,7 @@ static int ahat_sq_dev(dt_hooi *h, const void *A, int dt) {
     int64_t sh[DT_MAX_NDIM];
     for (int i = 0; i < N; i++) sh[i] = h->R[i];
     const double *cur = h->G64;
-    auto small = [&](int n) { return (double)h->I[n] * h->R[n] * h->R[n] <= (double)(1 << 20); };
+    auto small = [&](int n) { return h->R[n] <= 64; };  // every supported rank
     dt_gram_inverse_batch gb{};
     for (int n = 0; n < N; n++) {
         if (!small(n)) continue;

@@ -202,6 +202,7 @@ struct dt_gram_inverse_batch {
     int dt[DT_MAX_NDIM], m[DT_MAX_NDIM], n[DT_MAX_NDIM];
     double *C[DT_MAX_NDIM];
     int count;
+    int ch;  // rows per staged chunk (set by dt_gram_inverse_batch_dev)
 };
 int dt_gram_inverse_batch_dev(dt_ctx *ctx, const dt_gram_inverse_batch &b, int *flag);
 int dt_inverse_flag(dt_ctx *ctx, const double *G, int n, double *C, double *work, int *flag, int bit);

@@ -279,17 +279,73 @@ __global__ __launch_bounds__(SMALL_THREADS) void cholqr3_kernel(double *X, int m
 
 // Workgroup q: C_q (n_q x n_q) = (X_qᵀX_q)^{-1}, X_q m_q x n_q column-major in
 // FP64, FP32 or bf16 (widened on load).
+// out (n x n, LDS) = x y (LDS, n x n column-major); waves over columns, lanes
+// over rows, no integer division
+__device__ void wg_small_mm(const double *__restrict__ x, const double *__restrict__ y, int n,
+                            double *__restrict__ out) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int j = wv; j < n; j += nw)
+        for (int i = lane; i < n; i += 64) {
+            double s0 = 0.0, s1 = 0.0;
+            int k = 0;
+            for (; k + 1 < n; k += 2) {
+                s0 = fma(x[i + n * k], y[k + n * j], s0);
+                s1 = fma(x[i + n * (k + 1)], y[k + 1 + n * j], s1);
+            }
+            if (k < n) s0 = fma(x[i + n * k], y[k + n * j], s0);
+            out[i + n * j] = s0 + s1;
+        }
+}
+
+// Workgroup q: C_q (n_q x n_q) = (X_qᵀX_q)^{-1}, X_q m_q x n_q column-major in
+// FP64, FP32 or bf16 (widened on load).  The Gram of a rounded copy of an
+// orthonormal factor is I + E with E small: then Newton-Schulz from C = 2I - G,
+// C <- C (2I - G C), squares the error each step (5 steps from ||E||_F < 1/4
+// reach the FP64 floor) with two small products per step and no pivoting;
+// otherwise Gauss-Jordan with partial pivoting.
+// LDS: G (n^2) | staging of wg_xty, reused as C, T, T2 (3 n^2).
 __global__ __launch_bounds__(SMALL_THREADS) void gram_inverse_batch_kernel(dt_gram_inverse_batch b,
                                                                            int *__restrict__ flag) {
     __shared__ int perm[kMaxR];
+    __shared__ double red[SMALL_THREADS / 64];
     const int q = blockIdx.x, m = b.m[q], n = b.n[q];
+    const int tid = threadIdx.x, nt = blockDim.x;
     double *G = lds_base(), *stage = G + n * n;
+    // 64-row chunks: the staging (2 x 65 x n) fits in the 3 n^2 that Newton-Schulz
+    // needs anyway for n >= 44, and the launch provides at least that much
+    const int ch = b.ch;
     if (b.dt[q] == DT_F32)
-        wg_xty((const float *)b.X[q], (const float *)b.X[q], m, n, G, stage, stage_rows(n));
+        wg_xty((const float *)b.X[q], (const float *)b.X[q], m, n, G, stage, ch);
     else if (b.dt[q] == DT_BF16)
-        wg_xty((const uint16_t *)b.X[q], (const uint16_t *)b.X[q], m, n, G, stage, stage_rows(n));
+        wg_xty((const uint16_t *)b.X[q], (const uint16_t *)b.X[q], m, n, G, stage, ch);
     else
-        wg_xty((const double *)b.X[q], (const double *)b.X[q], m, n, G, stage, stage_rows(n));
+        wg_xty((const double *)b.X[q], (const double *)b.X[q], m, n, G, stage, ch);
+    // ||G - I||_F^2
+    double e2 = 0.0;
+    for (int e = tid; e < n * n; e += nt) {
+        const double d = G[e] - ((e % n) == (e / n) ? 1.0 : 0.0);
+        e2 += d * d;
+    }
+    for (int o = 32; o > 0; o >>= 1) e2 += __shfl_xor(e2, o);
+    if ((tid & 63) == 0) red[tid >> 6] = e2;
+    __syncthreads();
+    e2 = 0.0;
+    for (int w = 0; w < nt / 64; w++) e2 += red[w];
+    if (e2 < 1.0 / 16.0) {  // uniform
+        double *C = stage, *T = C + n * n, *T2 = T + n * n;
+        for (int e = tid; e < n * n; e += nt) C[e] = ((e % n) == (e / n) ? 2.0 : 0.0) - G[e];
+        __syncthreads();
+        for (int it = 0; it < 5; it++) {
+            wg_small_mm(G, C, n, T);  // T = G C
+            __syncthreads();
+            wg_small_mm(C, T, n, T2);  // T2 = C G C
+            __syncthreads();
+            for (int e = tid; e < n * n; e += nt) C[e] = 2.0 * C[e] - T2[e];
+            __syncthreads();
+        }
+        for (int e = tid; e < n * n; e += nt) b.C[q][e] = C[e];
+        return;
+    }
     if (wg_inverse_inplace(G, n, perm)) {
         if (threadIdx.x == 0) atomicOr(flag, 8);
         return;
@@ -518,8 +574,14 @@ int dt_gram_inverse_batch_dev(dt_ctx *ctx, const dt_gram_inverse_batch &b, int *
         nmax = b.n[q] > nmax ? b.n[q] : nmax;
     }
     if (b.count < 1) return DT_OK;
-    DT_LAUNCH("fit_small", gram_inverse_batch_kernel, dim3((unsigned)b.count), dim3(SMALL_THREADS), lds_bytes(nmax),
-              b, flag);
+    // LDS: the Gram and the row staging of wg_xty (64-row chunks when the
+    // three Newton-Schulz matrices leave room for them), or those matrices
+    dt_gram_inverse_batch bb = b;
+    const size_t ns = (size_t)3 * nmax * nmax;
+    bb.ch = (size_t)2 * 65 * nmax <= ns ? 64 : stage_rows(nmax);
+    const size_t lds = std::max((size_t)nmax * nmax + 2 * (size_t)(bb.ch + 1) * nmax, (size_t)nmax * nmax + ns) *
+                       sizeof(double);
+    DT_LAUNCH("fit_small", gram_inverse_batch_kernel, dim3((unsigned)b.count), dim3(SMALL_THREADS), lds, bb, flag);
     return DT_OK;
 }
 
