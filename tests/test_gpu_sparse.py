@@ -90,6 +90,62 @@ def test_sp_ttmc_vs_oracle(ctx, dims, nnz, long_row):
         sp.close()
 
 
+@pytest.mark.parametrize("case", ["one", "row63", "row64", "row65", "rows130", "two_rows_129"])
+def test_sp_ttmc_tiny_edges(ctx, case):
+    """Batch-boundary edges of the TTMc (batches of 64 records): a single
+    nonzero, one row of 63 / 64 / 65 nonzeros, 130 rows of one nonzero each
+    and two rows whose boundary falls inside the second batch."""
+    rng = np.random.default_rng(5)
+    if case == "one":
+        dims = (5, 4, 3)
+        idx = [np.array([3]), np.array([1]), np.array([2])]
+    elif case.startswith("row"):
+        k = int(case[3:]) if case != "rows130" else 130
+        if case == "rows130":
+            dims = (200, 40, 30)
+            idx = [rng.permutation(200)[:k], rng.integers(0, 40, k), rng.integers(0, 30, k)]
+        else:
+            dims = (4, 50, 40)
+            flat = rng.permutation(50 * 40)[:k]
+            idx = [np.full(k, 2), flat % 50, flat // 50]
+    else:
+        dims = (3, 60, 50)
+        flat = rng.permutation(60 * 50)[:129]
+        idx = [np.where(np.arange(129) < 70, 0, 2), flat % 60, flat // 60]
+    idx = [np.asarray(i, np.int64) for i in idx]
+    vals = rng.random(len(idx[0])).astype(np.float32) + np.float32(0.5)
+    idx, vals = S.merge_duplicates(dims, idx, vals)
+    print("tiny", case, dims, len(vals), flush=True)
+    Us = [rng.standard_normal((d, 16)).astype(np.float32) for d in dims]
+    sp = dt.SpTensor.from_host(ctx, dims, idx, vals)
+    try:
+        dU = [dt.upload_rows(ctx, u) for u in Us]
+        for n in range(3):
+            Y = dt.download_rows(sp.ttmc(n, dU))
+            Yr = S.ttmc(dims, idx, vals, [u.astype(np.float64) for u in Us], n)
+            assert rel(Y, Yr) < 2e-6, (case, n)
+    finally:
+        sp.close()
+
+
+def test_sparse_hooi_tiny_ranks(ctx):
+    """A 20 x 18 x 17 tensor of 200 nonzeros at ranks (3, 4, 5) from a given start."""
+    rng = np.random.default_rng(21)
+    dims, ranks = (20, 18, 17), (3, 4, 5)
+    idx, vals = make_coo(rng, dims, 200)
+    U0 = [D.orth_posdiag(rng.standard_normal((d, r))) for d, r in zip(dims, ranks)]
+    sp = dt.SpTensor.from_host(ctx, dims, idx, vals)
+    try:
+        out = sp.hooi(max_iter=3, ranks=ranks, U_init=U0)
+    finally:
+        sp.close()
+    Ur, Gr, res = S.hooi(dims, idx, vals, U0, 3)
+    print("tiny ranks residuals", out["residuals"], res, flush=True)
+    assert np.allclose(out["residuals"], res, rtol=1e-5)
+    for n in range(3):
+        assert D.sin_max_angle(out["U"][n], Ur[n]) < 1e-3
+
+
 def test_sparse_hooi_vs_oracle(ctx):
     """Three sweeps from the same start (Philox normals, positive-diagonal QR):
     residuals to 1e-5 relative, factors by principal angles."""
@@ -188,5 +244,55 @@ def test_c4_full_size_generator_and_sort(ctx):
         assert np.all(np.diff(perm.astype(np.int64))[same] > 0)
         assert np.array_equal(np.diff(rowptr.astype(np.int64)), np.bincount(i0, minlength=dims[0]))
         print("C4 mode-0 rows touched", nrows, flush=True)
+    finally:
+        sp.close()
+
+
+def test_c4_full_size_ttmc_rows(ctx):
+    """The TTMc at the sparse workload's real size (2^20 x 2^18 x 2^16, 1e8
+    Zipf(1.1) draws, 65.6M nonzeros) for every mode: the 16 longest rows (up to
+    4.2M nonzeros each, split over many ranges and summed by the fix-up) and 2000
+    random rows, each recomputed on the CPU in FP64 as the matrix product
+    (val * U_a[i_a])^T U_b[i_b] over the row's nonzeros (entry r_a + 16 r_b),
+    which is the Kronecker sum written as one GEMM.  FP32 accumulation: 1e-5
+    relative per row (Frobenius).  Every row of Y is finite and the rows with no
+    nonzero are zero."""
+    dims = (1 << 20, 1 << 18, 1 << 16)
+    print("C4 ttmc: generate", flush=True)
+    idx, val, nnz = dt.sp_gen_zipf(ctx, dims, 100_000_000, 1.1, 20000905)
+    hidx = [idx[m].download()[:nnz].astype(np.int64) for m in range(3)]
+    v = val.download()[:nnz].astype(np.float64)
+    sp = dt.SpTensor(ctx, dims, idx, val, nnz)
+    try:
+        rng = np.random.default_rng(99)
+        Us = [rng.standard_normal((d, 16)).astype(np.float32) for d in dims]
+        dU = [dt.upload_rows(ctx, u) for u in Us]
+        U64 = [u.astype(np.float64) for u in Us]
+        for n in range(3):
+            print("C4 ttmc mode", n, flush=True)
+            Y = sp.ttmc(n, dU)
+            Yh = dt.download_rows(Y)
+            Y.free()
+            a, b = [m for m in range(3) if m != n]
+            order = np.argsort(hidx[n], kind="stable")
+            cnt = np.bincount(hidx[n], minlength=dims[n])
+            rp = np.concatenate(([0], np.cumsum(cnt)))
+            assert np.all(np.isfinite(Yh))
+            assert not np.any(Yh[cnt == 0])
+            nonempty = np.flatnonzero(cnt)
+            rows = np.unique(np.concatenate((np.argsort(cnt)[-16:], rng.choice(nonempty, 2000, replace=False))))
+            worst = 0.0
+            for r in rows:
+                sl = order[rp[r]:rp[r + 1]]
+                M = np.zeros((16, 16))
+                for c0 in range(0, len(sl), 1 << 20):
+                    s = sl[c0:c0 + (1 << 20)]
+                    M += (U64[a][hidx[a][s]] * v[s, None]).T @ U64[b][hidx[b][s]]
+                ref = M.ravel(order="F")
+                e = float(np.linalg.norm(Yh[r] - ref) / max(np.linalg.norm(ref), 1e-300))
+                worst = max(worst, e)
+                assert e < 1e-5, (n, int(r), int(cnt[r]), e)
+            print("C4 ttmc mode", n, "rows checked", len(rows), "worst rel", worst, flush=True)
+            del Yh
     finally:
         sp.close()
