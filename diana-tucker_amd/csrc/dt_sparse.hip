@@ -559,68 +559,80 @@ __global__ __launch_bounds__(64) void sp_fixup_chunks(SpModeArgs a, const float 
     }
 }
 
-// Partial L = Y_cᵀ U_n[rows] (256 x 16) per 256-thread workgroup: each wave
-// walks its own contiguous run of compact rows, lane l owning entries
-// 4l..4l+3 of the row for all 16 columns (64 FP32 accumulators); four rows are
-// loaded before any is used (each Y_c row once, 1 KB coalesced, and its factor
-// row).  The four waves' sums are then added in wave order through LDS.
-__global__ __launch_bounds__(256) void sp_ytu(const float *__restrict__ Yc, const int32_t *__restrict__ rows,
-                                              int64_t nrows, const float *__restrict__ Un, int64_t rows_per_wave,
-                                              float *__restrict__ Lwg) {
-    __shared__ floatx4 red[16][64];  // one wave's sums: [4q + h][lane]
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+// Partial L = Y_cᵀ U_n[rows] (256 x 16) per 256-thread workgroup on FP32
+// MFMA.  Each wave walks its own contiguous run [s0, s1) of compact rows four
+// at a time (the MFMA's K): lane (c, g) loads entries 64q + 4c .. 64q + 4c + 3
+// (q = 0..3) of row s + g, so each load instruction reads four 256-byte
+// contiguous pieces, and U_n[rows[s + g], c].  MFMA t = 4q + r takes
+// A[i][g] = Y_c[s + g, 64q + 4i + r] and B[g][j] = U_n[rows[s + g], j], so
+// accumulator t ends as L[64q + 4i + r, j] with i = 4g + v (register v) and
+// j = c.  Two register sets alternate over groups of four rows (one group's
+// loads are in flight while the other's MFMAs run) and each set's row indices
+// are loaded one of its groups ahead.  Rows past s1 are read as row s1 - 1
+// with their U_n value zeroed.  The four waves' sums are then added in wave
+// order through LDS.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void sp_ytu(
+    const float *__restrict__ Yc, const int32_t *__restrict__ rows, int64_t nrows, const float *__restrict__ Un,
+    int64_t rows_per_wave, float *__restrict__ Lwg) {
+    __shared__ floatx4 red[16][64];  // [t][lane]: the running sum of the waves
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c = lane & 15, g = lane >> 4;
     const int64_t s0 = ((int64_t)blockIdx.x * 4 + wv) * rows_per_wave;
     const int64_t s1 = s0 + rows_per_wave < nrows ? s0 + rows_per_wave : nrows;
-    float acc[4][16];
+    floatx4 acc[16];
 #pragma unroll
-    for (int q = 0; q < 4; q++)
+    for (int t = 0; t < 16; t++) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+    if (s0 < s1) {  // uniform
+        const int64_t last = s1 - 1;
+        auto row_of = [&](int64_t s) { return s + g <= last ? s + g : last; };
+        auto ld_y = [&](int64_t s, floatx4(&y)[4]) {
+            const float *p = Yc + row_of(s) * SP_E + 4 * c;
 #pragma unroll
-        for (int k = 0; k < 16; k++) acc[q][k] = 0.f;
-    auto add_row = [&](const floatx4 &y, const floatx4 (&u4)[4]) {
+            for (int q = 0; q < 4; q++) y[q] = __builtin_nontemporal_load((const floatx4 *)(p + 64 * q));
+        };
+        auto ld_u = [&](int64_t s, int r) {
+            const float u = Un[(int64_t)r * 16 + c];
+            return s + g <= last ? u : 0.f;
+        };
+        auto compute = [&](const floatx4(&y)[4], float u) {
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-#pragma unroll
-            for (int k = 0; k < 16; k++) acc[q][k] = fmaf(y[q], u4[k >> 2][k & 3], acc[q][k]);
-    };
-    int64_t s = s0;
-    for (; s + 4 <= s1; s += 4) {
-        floatx4 y[4], u[4][4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            y[r] = *(const floatx4 *)(Yc + (s + r) * SP_E + 4 * lane);
-            const floatx4 *u4 = (const floatx4 *)(Un + (int64_t)rows[s + r] * 16);
-#pragma unroll
-            for (int h = 0; h < 4; h++) u[r][h] = u4[h];
+            for (int t = 0; t < 16; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(y[t >> 2][t & 3], u, acc[t], 0, 0, 0);
+        };
+        const int64_t ngrp = (s1 - s0 + 3) / 4;
+        floatx4 yX[4], yY[4];
+        const int rX0 = rows[row_of(s0)], rY0 = rows[row_of(s0 + 4)];
+        ld_y(s0, yX);
+        float uX = ld_u(s0, rX0);
+        ld_y(s0 + 4, yY);
+        float uY = ld_u(s0 + 4, rY0);
+        int rX = rows[row_of(s0 + 8)], rY = rows[row_of(s0 + 12)];
+        int64_t s = s0;
+        for (int64_t k = 0; k < ngrp / 2; k++) {
+            compute(yX, uX);
+            ld_y(s + 8, yX);
+            uX = ld_u(s + 8, rX);
+            rX = rows[row_of(s + 16)];
+            compute(yY, uY);
+            ld_y(s + 12, yY);
+            uY = ld_u(s + 12, rY);
+            rY = rows[row_of(s + 20)];
+            s += 8;
         }
-#pragma unroll
-        for (int r = 0; r < 4; r++) add_row(y[r], u[r]);
+        if (ngrp & 1) compute(yX, uX);
     }
-    for (; s < s1; s++) {
-        floatx4 u[4];
-        const floatx4 y = *(const floatx4 *)(Yc + s * SP_E + 4 * lane);
-        const floatx4 *u4 = (const floatx4 *)(Un + (int64_t)rows[s] * 16);
-#pragma unroll
-        for (int h = 0; h < 4; h++) u[h] = u4[h];
-        add_row(y, u);
-    }
-    // waves 0..3 in order: each adds the running sum in LDS to its own
+    // waves 0..3 in order: each adds its sums to the running sum in LDS
     for (int w = 0; w < 4; w++) {
         if (wv == w) {
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-#pragma unroll
-                for (int h = 0; h < 4; h++) {
-                    floatx4 v = floatx4{acc[q][4 * h], acc[q][4 * h + 1], acc[q][4 * h + 2], acc[q][4 * h + 3]};
-                    if (w > 0) v = red[4 * q + h][lane] + v;
-                    red[4 * q + h][lane] = v;
-                }
+            for (int t = 0; t < 16; t++) red[t][lane] = w > 0 ? red[t][lane] + acc[t] : acc[t];
         }
         __syncthreads();
     }
+    // partial, row-major 256 x 16: entry e = 64q + 4i + r (accumulator 4q + r), column j
     float *L = Lwg + (int64_t)blockIdx.x * SP_E * 16;
-    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
-        const int qh = e >> 6, l = e & 63;
-        *(floatx4 *)(L + (4 * l + (qh >> 2)) * 16 + 4 * (qh & 3)) = red[qh][l];
+    for (int x = threadIdx.x; x < SP_E * 16; x += 256) {
+        const int j = x & 15, e = x >> 4, i = (e >> 2) & 15, t = 4 * (e >> 6) + (e & 3);
+        L[x] = red[t][j + 16 * (i >> 2)][i & 3];
     }
 }
 
