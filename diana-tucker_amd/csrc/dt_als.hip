@@ -212,6 +212,125 @@ __device__ __forceinline__ double rl_f64(double v, int lane) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// One wave: A (CM x CM SPD, LDS, element (i, k) at A[i * lda + k]) = RᵀR.
+// Writes R (upper part; the lower part holds leftovers) to Rm and R^{-1}
+// (upper, zeros below) to Ri, both with the same element order.  Lane k < CM
+// holds column k in registers and row j of each trailing update comes from
+// v_readlane, as in als_chol_kernel.  Returns 1 + the first bad pivot, or 0.
+template <int CM>
+__device__ int chol_wave(const double *__restrict__ A, int lda, double *__restrict__ Rm, int ldr,
+                         double *__restrict__ Ri, int ldi) {
+    const int lane = threadIdx.x & 63;
+    const int k = lane < CM ? lane : CM - 1;
+    double col[CM];
+#pragma unroll
+    for (int i = 0; i < CM; i++) col[i] = A[i * lda + k];
+    int bad = 0;
+    double invr[CM];
+#pragma unroll
+    for (int j = 0; j < CM; j++) {
+        double d = rl_f64(col[j], j);
+        if (!(d > 0.0) || !isfinite(d)) {
+            if (!bad) bad = j + 1;
+            d = 1.0;
+        }
+        const double rjj = sqrt(d), ir = 1.0 / rjj;
+        invr[j] = ir;
+        const double gjk = col[j];
+        const double f = gjk * (ir * ir);
+#pragma unroll
+        for (int i = j + 1; i < CM; i++) col[i] -= rl_f64(col[j], i) * f;
+        col[j] = (k == j) ? rjj : gjk * ir;
+    }
+    if (lane < CM)
+#pragma unroll
+        for (int i = 0; i < CM; i++) Rm[i * ldr + k] = col[i];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double x[CM];
+#pragma unroll
+    for (int i = CM - 1; i >= 0; i--) {
+        double s0 = (i == k) ? 1.0 : 0.0, s1 = 0.0;
+#pragma unroll
+        for (int m = i + 1; m < CM; m += 2) {
+            s0 -= Rm[i * ldr + m] * x[m];
+            if (m + 1 < CM) s1 -= Rm[i * ldr + m + 1] * x[m + 1];
+        }
+        x[i] = (s0 + s1) * invr[i];
+    }
+    if (lane < CM)
+#pragma unroll
+        for (int i = 0; i < CM; i++) Ri[i * ldi + k] = x[i];
+    return bad;
+}
+
+// Order 64 as 32 x 32 blocks, G = [A B; Bᵀ C] (LDS, row stride 65):
+//   R11 = chol(A), R12 = R11⁻ᵀ B, R22 = chol(C − R12ᵀR12),
+//   R⁻¹ = [R11⁻¹, −R11⁻¹ R12 R22⁻¹; 0, R22⁻¹],
+// the two Cholesky factors by one wave each in turn (chol_wave<32>, whose
+// register broadcasts fit where order 64 would not), the products by the
+// whole workgroup.  Writes the R x R corner of R⁻¹ (column-major) unless a
+// pivot fails, which ORs 4 into *flag.
+__device__ void chol64_blocked(double (*G)[65], int R, double *__restrict__ Rinv_out, int *__restrict__ flag) {
+    __shared__ double Rm[32][33], I11[32][33], I22[32][33], R12[32][33];
+    __shared__ int bad_s;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (tid < 64) {
+        const int b = chol_wave<32>(&G[0][0], 65, &Rm[0][0], 33, &I11[0][0], 33);
+        if (tid == 0) bad_s = b;
+    }
+    __syncthreads();
+    // R12 = I11ᵀ B (I11 upper: m <= i)
+    for (int e = tid; e < 1024; e += nt) {
+        const int i = e >> 5, j = e & 31;
+        double s = 0.0;
+        for (int m = 0; m <= i; m++) s += I11[m][i] * G[m][32 + j];
+        R12[i][j] = s;
+    }
+    __syncthreads();
+    // Schur complement in place of C
+    for (int e = tid; e < 1024; e += nt) {
+        const int i = e >> 5, j = e & 31;
+        double s = G[32 + i][32 + j];
+        for (int m = 0; m < 32; m++) s -= R12[m][i] * R12[m][j];
+        G[32 + i][32 + j] = s;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const int b = chol_wave<32>(&G[32][32], 65, &Rm[0][0], 33, &I22[0][0], 33);
+        if (tid == 0 && !bad_s && b) bad_s = 32 + b;
+    }
+    __syncthreads();
+    if (bad_s) {
+        if (tid == 0) atomicOr(flag, 4);
+        return;
+    }
+    // T = I11 R12 into Rm (I11 upper: m >= i), then the upper-right block
+    // -T I22 (I22 upper: m <= j) straight to the output
+    for (int e = tid; e < 1024; e += nt) {
+        const int i = e >> 5, j = e & 31;
+        double s = 0.0;
+        for (int m = i; m < 32; m++) s += I11[i][m] * R12[m][j];
+        Rm[i][j] = s;
+    }
+    __syncthreads();
+    for (int e = tid; e < R * R; e += nt) {
+        const int i = e % R, k = e / R;  // entry (i, k) of R^{-1}, column-major
+        double v;
+        if (i >= 32) {
+            v = k >= 32 ? I22[i - 32][k - 32] : 0.0;
+        } else if (k < 32) {
+            v = I11[i][k];
+        } else {
+            double s = 0.0;
+            for (int m = 0; m <= k - 32; m++) s += Rm[i][m] * I22[m][k - 32];
+            v = -s;
+        }
+        Rinv_out[e] = v;
+    }
+}
+
 // One workgroup: G = sum_w part[w] (fixed order); optional shift; Rinv = chol(G)^{-1}.
 // CM (16, 32 or 64 >= R) is the padded order of the register Cholesky.
 template <int CM>
@@ -263,6 +382,10 @@ __global__ __launch_bounds__(ALS_THREADS) void als_chol_kernel(const double *__r
         if (i >= R || j >= R) G[i][j] = (i == j) ? 1.0 : 0.0;
     }
     __syncthreads();
+    if constexpr (CM > 32) {
+        chol64_blocked(G, R, Rinv_out, flag);
+        return;
+    }
     if (tid < 64) {
         // One wave, lane k < CM holding column k in registers (lanes past CM
         // repeat column CM - 1 and write nothing); every index below is a

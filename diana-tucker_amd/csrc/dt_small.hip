@@ -200,15 +200,18 @@ __device__ int wg_chol_rinv(double *__restrict__ G, int n, double shift, double 
         __syncthreads();
     }
     // Xo = R^{-1}: column j solved by thread j (back substitution), written
-    // straight into Xo, which no thread reads during the solve.
+    // straight into Xo, which no thread reads during the solve; one division
+    // per diagonal entry, shared by all columns
+    __shared__ double invd[kMaxR];
     for (int idx = tid; idx < n * n; idx += nt) Xo[idx] = 0.0;
+    for (int i = tid; i < n; i += nt) invd[i] = 1.0 / G[i + n * i];
     __syncthreads();
     const int j = tid;
     if (j < n) {
         for (int i = j; i >= 0; i--) {
             double s = (i == j) ? 1.0 : 0.0;
             for (int k = i + 1; k <= j; k++) s -= G[i + n * k] * Xo[k + n * j];
-            Xo[i + n * j] = s / G[i + n * i];
+            Xo[i + n * j] = s * invd[i];
         }
     }
     __syncthreads();

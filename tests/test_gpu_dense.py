@@ -372,6 +372,24 @@ def test_hooi_c1_hosvd_and_fp64(ctx):
             assert D.sin_max_angle(out["U"][n], Ur[n]) < (1e-3 if dtype == dt.DT_F32 else 1e-8)
 
 
+def test_hooi_ranks_above_32(ctx):
+    """Ranks (40, 48, 64): the factor update runs over many workgroups and its
+    order-64 Cholesky is the blocked one (two order-32 wave factorizations);
+    fit and factors against the oracle's ALS_ from the same HOSVD start."""
+    shape, ranks = (96, 80, 72), (40, 48, 64)
+    g = ctx.gen_tucker(shape, ranks, 1e-3, seed=31)
+    A = g["A"].download().astype(np.float64)
+    Uh = D.hosvd_factors(A, ranks)
+    Gr, Ur, res_ref = D.hooi_als(A, ranks, 4, Uh)
+    for dtype in (dt.DT_F64, dt.DT_F32):
+        out = run_gpu_hooi(ctx, A, ranks, 4, dt.INIT_HOSVD, dtype=dtype, flags=dt.FLAG_FP64_FIT)
+        f_gpu, f_ref = 1 - out["residuals"][-1], 1 - res_ref[-1]
+        angles = [D.sin_max_angle(out["U"][n], Ur[n]) for n in range(3)]
+        print("ranks>32 dtype", dtype, "fit gpu", f_gpu, "ref", f_ref, "angles", angles, flush=True)
+        assert abs(f_gpu - f_ref) / f_ref < (1e-10 if dtype == dt.DT_F64 else 1e-5)
+        assert max(angles) < (1e-7 if dtype == dt.DT_F64 else 1e-3)
+
+
 def test_hooi_4way_tree_matches_reference_schedule(ctx):
     shape, ranks = (32, 24, 20, 16), (4, 5, 3, 2)
     g = ctx.gen_tucker(shape, ranks, 1e-2, seed=5)
