@@ -43,6 +43,8 @@ struct dt_ctx {
     size_t ws_size[kSlots] = {};
     // Host-pinned scalar staging area for small device->host results.
     double *pinned = nullptr;
+    // resident workgroups on this context's device per kernel (occupancy x CUs)
+    std::map<const void *, int64_t> resident_wgs;
 };
 
 // Workspace slot ids (one owner each, so nested helpers never alias).
@@ -80,6 +82,20 @@ enum dt_ws_slot {
 static_assert(WS_COUNT_ <= dt_ctx::kSlots, "too many workspace slots");
 
 int dt_fail(dt_ctx *ctx, int code, const char *fmt, ...);
+// workgroups of `threads` threads of `kernel` resident on the context's device
+// at once (cached per context); 0 when the query fails
+template <typename K>
+int64_t dt_resident_wgs(dt_ctx *ctx, K kernel, int threads) {
+    const void *key = (const void *)kernel;
+    auto it = ctx->resident_wgs.find(key);
+    if (it != ctx->resident_wgs.end()) return it->second;
+    int cus = 0, blocks = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, threads, 0) != hipSuccess) return 0;
+    const int64_t n = (int64_t)cus * (blocks < 1 ? 1 : blocks);
+    ctx->resident_wgs[key] = n;
+    return n;
+}
 void *dt_ws(dt_ctx *ctx, int slot, size_t bytes);  // nullptr on failure (error set)
 hipEvent_t dt_get_event(dt_ctx *ctx);
 void dt_collect_timing(dt_ctx *ctx);

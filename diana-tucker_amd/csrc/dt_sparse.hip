@@ -1142,18 +1142,12 @@ static int sp_ttmc_dev(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, fl
     SpModeArgs args{m.row, m.rec, m.rowptr, m.slot, sp->nnz};
     // 32-bit gather offsets when both gathered factors (rows x 64 B) are below 4 GiB
     const bool off32 = sp->dims[a] < ((int64_t)1 << 26) && sp->dims[b] < ((int64_t)1 << 26);
-    static int blocks_per_cu = 0, cus = 0;
-    if (!blocks_per_cu) {
-        int dev = 0;
-        DT_HIP(hipGetDevice(&dev));
-        DT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        DT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, sp_ttmc16<true>, 256, 0));
-        if (blocks_per_cu < 1) blocks_per_cu = 1;
-    }
+    const int64_t wgs = dt_resident_wgs(ctx, sp_ttmc16<true>, 256);
+    if (wgs <= 0) return dt_fail(ctx, DT_EHIP, "sparse TTMc: occupancy query failed");
     // equal contiguous ranges, multiples of 64, about eight per resident wave
     // (the hardware hands a range to whichever wave finishes first, which
     // evens out ranges of different cost: short-row stretches take longer)
-    const int64_t maxw = (int64_t)cus * blocks_per_cu * 4 * 8;
+    const int64_t maxw = wgs * 4 * 8;
     int64_t per_wave = ((sp->nnz + maxw - 1) / maxw + 63) / 64 * 64;
     if (per_wave < 64) per_wave = 64;
     const int64_t nwaves = (sp->nnz + per_wave - 1) / per_wave;

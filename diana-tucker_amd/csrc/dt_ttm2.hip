@@ -494,14 +494,10 @@ int dt_ttm2_trail(dt_ctx *ctx, const float *A, int dtA, int64_t F, int64_t K2, i
     return DT_OK;
 }
 
-// waves resident per CU for a kernel at 256 threads, times the CU count
+// waves resident on the context's device for a kernel at 256 threads
 template <typename K>
-static int64_t resident_waves(K kernel) {
-    int dev = 0, cus = 0, blocks = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, 256, 0) != hipSuccess) return 0;
-    return (int64_t)cus * blocks * 4;
+static int64_t resident_waves(dt_ctx *ctx, K kernel) {
+    return dt_resident_wgs(ctx, kernel, 256) * 4;
 }
 
 static bool ttm2_lead_fits(int dtA, int64_t K0, int64_t K1, int64_t B, int64_t J0, int64_t J1) {
@@ -518,8 +514,7 @@ int dt_ttm2_lead(dt_ctx *ctx, const void *A, int dtA, int64_t K0, int64_t K1, in
                  const void *U0, const float *M1, int64_t J0, int64_t J1, float *T) {
     if (!ttm2_lead_fits(dtA, K0, K1, B, J0, J1)) return DT_EUNSUPPORTED;
     if (dtA == DT_F32 && K0 == 256 && K1 == 256 && J0 <= 16 && J1 <= 16) {
-        static int64_t waves = 0;
-        if (!waves) waves = resident_waves(ttm2_lead_wl<WL_GC, 256>);
+        const int64_t waves = resident_waves(ctx, ttm2_lead_wl<WL_GC, 256>);
         if (waves <= 0) return dt_fail(ctx, DT_EHIP, "dt_ttm2_lead: occupancy query failed");
         const int64_t nwave = std::min<int64_t>(B, waves);
         DT_LAUNCH("ttm2_lead_f32_stream", (ttm2_lead_wl<WL_GC, 256>), dim3((unsigned)((nwave + 3) / 4)), dim3(256), 0,
@@ -532,10 +527,9 @@ int dt_ttm2_lead(dt_ctx *ctx, const void *A, int dtA, int64_t K0, int64_t K1, in
 #define L2K(A0, A1)                                                                                   \
     if (jb0 == A0 && jb1 == A1) {                                                                     \
         auto kern = bf ? ttm2_lead<A0, A1, true> : ttm2_lead<A0, A1, false>;                           \
-        static int64_t waves[2] = {0, 0};                                                             \
-        if (!waves[bf]) waves[bf] = resident_waves(kern);                                             \
-        if (waves[bf] <= 0) return dt_fail(ctx, DT_EHIP, "dt_ttm2_lead: occupancy query failed");    \
-        const int64_t nwave = std::min<int64_t>(B, waves[bf]);                                        \
+        const int64_t waves = resident_waves(ctx, kern);                                              \
+        if (waves <= 0) return dt_fail(ctx, DT_EHIP, "dt_ttm2_lead: occupancy query failed");        \
+        const int64_t nwave = std::min<int64_t>(B, waves);                                            \
         dim3 grid((unsigned)((nwave + 3) / 4));                                                       \
         DT_LAUNCH(bf ? "ttm2_lead_bf16_stream" : "ttm2_lead_f32_stream", kern, grid, dim3(256), 0, A,  \
                   U0, M1, T, (int)K0, (int)K1, (int)J0, (int)J1, B);                                  \
