@@ -1,11 +1,14 @@
 // Dense driver with the shape of the reference's main.cpp (main.cpp:13-29):
 // srand(20000905), a Gaussian tensor filled by Tensor::randn, HOOI_ALS with
-// 5 sweeps.  One rank: partition {1,1,1}.  Optional arguments: I0 I1 I2 R.
+// 5 sweeps, the Summary table at the end.  One rank: partition {1,1,1}.
+// Optional arguments: I0 I1 I2 R.
 #include <cstdlib>
 #include "algorithm.hpp"
+#include "summary.hpp"
 
 int main(int argc, char **argv) {
     mpi_init();
+    Summary::init();
     srand((unsigned int)20000905);
     shape_t I{500, 400, 300};
     size_t r = 100;
@@ -21,6 +24,8 @@ int main(int argc, char **argv) {
     auto [G, U] = Algorithm::Tucker::HOOI_ALS(T, R, 5);
     (void)G;
     (void)U;
+    Summary::finalize();
+    Summary::print_summary();
     mpi_finalize();
     return 0;
 }

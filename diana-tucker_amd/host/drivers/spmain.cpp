@@ -1,18 +1,21 @@
 // spmain.cpp's workload on one GPU: read a .tns file, build the SpTensor,
-// Sp_HOOI_ALS with truncation (default 10) for every mode, one sweep.
+// Sp_HOOI_ALS with truncation (default 10) for every mode, one sweep, then the
+// Summary table.
 //   ./spmain <file.tns> [nthreads (ignored)] [truncation] [sweeps]
 #include <cstdio>
 #include <cstdlib>
 #include "algorithm.hpp"
 #include "read_data.hpp"
 #include "sp_tensor.hpp"
+#include "summary.hpp"
 
 int main(int argc, char *argv[]) {
-    srand((unsigned int)1);
     if (argc < 2) {
         printf("Usage: ./sptucker [filepath]\n");
         exit(1);
     }
+    Summary::init();
+    srand((unsigned int)1);
     auto dbf = (data_buffer<double> *)malloc(sizeof(data_buffer<double>));
     read_data<double>(argv[1], dbf);
     size_t truncation = 10;
@@ -32,5 +35,7 @@ int main(int argc, char *argv[]) {
     auto [G, U] = Algorithm::SpTucker::Sp_HOOI_ALS(A, R, sweeps);
     (void)G;
     (void)U;
+    Summary::finalize();
+    Summary::print_summary();
     return 0;
 }

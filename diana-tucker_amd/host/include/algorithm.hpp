@@ -27,6 +27,87 @@ inline HooiOptions &hooi_options() {
     static HooiOptions o;
     return o;
 }
+
+// Algorithmic work of one dense HOOI sweep for the Summary table: 2 in J flop
+// per mode product (in = elements read, J = the rank it contracts to) and
+// the elements each product reads and writes, A in its own type and the
+// intermediates in the working type (FP32, FP64 for an FP64 tensor).  The
+// reference schedule runs N + 1 chains from A (hooi_als.tpp:94-109); the
+// dimension tree (sweep_once in dt_hooi.hip) runs two, whose first two
+// products are one pass (their intermediate is never stored).  The ALS
+// updates (R x R and I x R work) are left out.
+struct SweepWork {
+    long long flop = 0, bytes = 0;
+};
+inline SweepWork hooi_sweep_work(const shape_t &I, const shape_t &R, int schedule, size_t a_bytes) {
+    const size_t N = I.size(), hm = N / 2, wb = a_bytes == 8 ? 8 : 4;
+    SweepWork w;
+    auto chain = [&](shape_t sh, const std::vector<size_t> &modes, bool from_A, bool fused) {
+        for (size_t k = 0; k < modes.size(); k++) {
+            const size_t m = modes[k];
+            double in = 1.0;
+            for (size_t x : sh) in *= (double)x;
+            const double out = in / (double)sh[m] * (double)R[m];
+            w.flop += (long long)(2.0 * in * (double)R[m]);
+            if (!(fused && k == 1)) w.bytes += (long long)(in * (double)((from_A && k == 0) ? a_bytes : wb));
+            if (!(fused && k == 0)) w.bytes += (long long)(out * (double)wb);
+            sh[m] = R[m];
+        }
+        return sh;
+    };
+    if (schedule == DT_SCHED_REFERENCE || N < 3) {
+        for (size_t n = 0; n < N; n++) {
+            std::vector<size_t> ms;
+            for (size_t i = 0; i < N; i++)
+                if (i != n) ms.push_back(i);
+            chain(I, ms, true, false);
+        }
+        std::vector<size_t> all;
+        for (size_t i = 0; i < N; i++) all.push_back(i);
+        chain(I, all, true, false);
+        return w;
+    }
+    std::vector<size_t> m1{N - 2, N - 1};
+    for (size_t i = N - 3; i + 1 > hm && i < N; i--) m1.push_back(i);
+    const shape_t TL = chain(I, m1, true, true);
+    for (size_t n = 0; n < hm; n++) {
+        std::vector<size_t> ms;
+        for (size_t i = hm; i-- > 0;)
+            if (i != n) ms.push_back(i);
+        chain(TL, ms, false, false);
+    }
+    std::vector<size_t> m2;
+    for (size_t i = hm; i-- > 0;) m2.push_back(i);
+    const shape_t TR = chain(I, m2, true, m2.size() >= 2);
+    shape_t last = TR;
+    for (size_t n = hm; n < N; n++) {
+        std::vector<size_t> ms;
+        for (size_t i = N; i-- > hm;)
+            if (i != n) ms.push_back(i);
+        last = chain(TR, ms, false, false);
+    }
+    chain(last, {N - 1}, false, false);
+    return w;
+}
+
+// One sparse HOOI sweep (3-way, mode-sorted copies): per mode the TTMc reads
+// every nonzero (three indices and a value), writes Y_(n) and reads the two
+// other factors once, nnz (2 prod_{m != n} R_m + R_n) flop (the Kronecker
+// rows and the Y_(n)ᵀU_n product).
+inline SweepWork sp_sweep_work(const shape_t &I, const shape_t &R, long long nnz) {
+    SweepWork w;
+    for (size_t n = 0; n < I.size(); n++) {
+        long long kr = 1, fac = 0;
+        for (size_t m = 0; m < I.size(); m++)
+            if (m != n) {
+                kr *= (long long)R[m];
+                fac += (long long)(I[m] * R[m]) * 4;
+            }
+        w.flop += nnz * (2 * kr + (long long)R[n]);
+        w.bytes += nnz * (4 * (long long)I.size() + 4) + (long long)I[n] * kr * 4 + fac;
+    }
+    return w;
+}
 }  // namespace diana
 
 namespace Algorithm {
@@ -37,6 +118,7 @@ std::tuple<Tensor<Ty>, std::vector<Tensor<Ty>>> HOOI_ALS(const Tensor<Ty> &A, co
     auto &opt = diana::hooi_options();
     const size_t N = A.ndim();
     if (R.size() != N) error("HOOI_ALS: |R| != ndim");
+    SummaryScope region("Tucker::HOOI_ALS()");
     const shape_t &I = A.shape_global();
     if (opt.verbose) output("Start Tucker::HOOI_ALS decomposition.. with max_iter = " + std::to_string(max_iter));
     dt_ctx *ctx = diana::Device::ctx();
@@ -88,10 +170,14 @@ std::tuple<Tensor<Ty>, std::vector<Tensor<Ty>>> HOOI_ALS(const Tensor<Ty> &A, co
     double a2 = 0.0;
     diana::check(dt_sumsq(ctx, dA.p, dtA, A.size(), &a2), "HOOI_ALS fnorm");
     if (opt.verbose) output("||A||_F = " + std::to_string(std::sqrt(a2)));
+    const diana::SweepWork sw = diana::hooi_sweep_work(I, R, opt.schedule, f64 ? 8 : 4);
     for (size_t it = 0; it < max_iter && st == DT_OK; it++) {
         if (opt.verbose) output("Calculating iteration " + std::to_string(it + 1) + " ...");
         double r = 0.0;
-        st = dt_hooi_sweep(h, 0, &r);
+        {
+            SummaryScope sc("Tucker::HOOI_ALS::sweep()", sw.flop, sw.bytes);
+            st = dt_hooi_sweep(h, 0, &r);
+        }
         if (st == DT_OK) {
             opt.residuals.push_back(r);
             if (opt.verbose) {
@@ -137,12 +223,13 @@ std::tuple<Tensor<Ty>, std::vector<Tensor<Ty>>> Sp_HOOI_ALS(const SpTensor<Ty> &
                                                             size_t max_iter) {
     const size_t N = A.ndim();
     if (R.size() != N) error("Sp_HOOI_ALS: |R| != ndim");
+    SummaryScope region("SpTucker::Sp_HOOI_ALS()");
     const shape_t &I = A.shape();
     dt_ctx *ctx = diana::Device::ctx();
     double a2 = 0.0;
+    int64_t nnz = 0;
     {
         int sm = 0;
-        int64_t nnz = 0;
         double nrm = 0.0;
         dt_sp_info(A.device(), &sm, &nnz, &nrm);
         a2 = nrm * nrm;
@@ -169,9 +256,14 @@ std::tuple<Tensor<Ty>, std::vector<Tensor<Ty>>> Sp_HOOI_ALS(const SpTensor<Ty> &
     std::vector<double> per(max_iter ? max_iter : 1);
     double res = 0.0, ms = 0.0;
     for (size_t it = 0; it < max_iter; it++) output("Calculating iteration " + std::to_string(it + 1) + " ...");
-    diana::check(dt_hooi_sparse(ctx, A.device(), R.data(), (int)max_iter, 0, u0p.data(), up.data(),
-                                (double *)dG.p, per.data(), &res, &ms),
-                 "Sp_HOOI_ALS");
+    {
+        const diana::SweepWork sw = diana::sp_sweep_work(I, R, (long long)nnz);
+        SummaryScope sc("SpTucker::Sp_HOOI_ALS::sweeps()", sw.flop * (long long)max_iter,
+                        sw.bytes * (long long)max_iter);
+        diana::check(dt_hooi_sparse(ctx, A.device(), R.data(), (int)max_iter, 0, u0p.data(), up.data(),
+                                    (double *)dG.p, per.data(), &res, &ms),
+                     "Sp_HOOI_ALS");
+    }
     std::vector<Tensor<Ty>> Ut;
     std::vector<double> tmp;
     for (size_t n = 0; n < N; n++) {

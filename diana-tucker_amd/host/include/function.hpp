@@ -5,6 +5,7 @@
 #define DIANA_AMD_FUNCTION_HPP
 #include <tuple>
 #include <vector>
+#include "summary.hpp"
 #include "tensor.hpp"
 
 namespace Function {
@@ -17,6 +18,7 @@ template <typename Ty>
 Tensor<Ty> matmulNN(const Tensor<Ty> &A, const Tensor<Ty> &B) {
     if (!A.is_matrix() || A.shape()[1] != B.shape()[0]) error("matmulNN: shape mismatch");
     size_t m = A.shape()[0], n = B.shape()[1], k = A.shape()[1];
+    SummaryScope sc("Function::matmulNN()", 2LL * m * n * k, (long long)(m * k + k * n + m * n) * sizeof(Ty));
     Tensor<Ty> ret({m, n}, false);
     A.op()->matmulNN(ret.data(), A.data(), B.data(), m, n, k);
     return ret;
@@ -25,6 +27,7 @@ template <typename Ty>
 Tensor<Ty> matmulNT(const Tensor<Ty> &A, const Tensor<Ty> &B) {
     if (!A.is_matrix() || A.shape()[1] != B.shape()[1]) error("matmulNT: shape mismatch");
     size_t m = A.shape()[0], n = B.shape()[0], k = A.shape()[1];
+    SummaryScope sc("Function::matmulNT()", 2LL * m * n * k, (long long)(m * k + k * n + m * n) * sizeof(Ty));
     Tensor<Ty> ret({m, n}, false);
     A.op()->matmulNT(ret.data(), A.data(), B.data(), m, n, k);
     return ret;
@@ -33,6 +36,7 @@ template <typename Ty>
 Tensor<Ty> matmulTN(const Tensor<Ty> &A, const Tensor<Ty> &B) {
     if (!A.is_matrix() || A.shape()[0] != B.shape()[0]) error("matmulTN: shape mismatch");
     size_t m = A.shape()[1], n = B.shape()[1], k = A.shape()[0];
+    SummaryScope sc("Function::matmulTN()", 2LL * m * n * k, (long long)(m * k + k * n + m * n) * sizeof(Ty));
     Tensor<Ty> ret({m, n}, false);
     A.op()->matmulTN(ret.data(), A.data(), B.data(), m, n, k);
     return ret;
@@ -86,6 +90,9 @@ Tensor<Ty> ttm(const Tensor<Ty> &A, const Tensor<Ty> &M, size_t n) {
     const int dt = diana::DType<Ty>::v, wt = detail::work_type(dt);
     shape_t out = A.shape();
     out[n] = M.shape()[0];
+    // 2 |A| J flop; A and Y each moved once
+    SummaryScope sc("Function::ttm()", 2LL * (long long)A.size() * (long long)M.shape()[0],
+                    (long long)(A.size() + A.size() / A.shape()[n] * M.shape()[0]) * (long long)sizeof(Ty));
     Tensor<Ty> ret(A.distribution(), out, false);
     diana::DevBuf dA(A.data(), A.size() * sizeof(Ty)), dM(M.data(), M.size() * sizeof(Ty));
     diana::DevBuf dY(ret.size() * (wt == DT_F64 ? 8 : 4));
@@ -109,6 +116,9 @@ template <typename Ty>
 Tensor<Ty> gram(const Tensor<Ty> &A, size_t n) {
     const int dt = diana::DType<Ty>::v;
     const size_t I = A.shape()[n];
+    // symmetric rank-K update, K = |A| / I: I (I + 1) K flop, A read once
+    SummaryScope sc("Function::gram()", (long long)(I + 1) * (long long)A.size(),
+                    (long long)A.size() * (long long)sizeof(Ty));
     Tensor<Ty> ret({I, I}, false);
     std::vector<double> s(I * I);
     diana::DevBuf dA(A.data(), A.size() * sizeof(Ty)), dS(s.size() * 8);
@@ -125,6 +135,8 @@ Tensor<Ty> ttt_except(const Tensor<Ty> &A, const Tensor<Ty> &B, size_t n) {
     for (size_t i = 0; i < A.ndim(); i++)
         if (i != n && A.shape()[i] != B.shape()[i]) error("ttt_except: shapes differ off mode n");
     const size_t I = A.shape()[n], J = B.shape()[n];
+    SummaryScope sc("Function::ttt_except()", 2LL * (long long)A.size() * (long long)J,
+                    (long long)(A.size() + B.size()) * (long long)sizeof(Ty));
     Tensor<Ty> ret({I, J}, false);
     std::vector<double> c(I * J);
     diana::DevBuf dA(A.data(), A.size() * sizeof(Ty)), dB(B.data(), B.size() * sizeof(Ty)), dC(c.size() * 8);

@@ -47,6 +47,22 @@ def test_main_driver_prints_reference_lines():
     _, _, rr = D.hooi_als(A, (R,) * 3, 5, U0)
     print("main residuals", res, "oracle", rr, flush=True)
     assert np.allclose(res, rr, atol=2e-6)  # printed with 6 decimals
+    check_summary(p.stderr, "Tucker::HOOI_ALS::sweep()", 5)
+
+
+def check_summary(err, region, calls):
+    """The Summary table (src/summary.cpp:98-206 layout): caption row, the
+    region with its call count and positive GFlop/s and bandwidth, and the
+    per-kernel device times after it."""
+    lines = err.splitlines()
+    assert any("Time(s)" in l and "GFlop/s" in l and "Bandw.(GB/s)" in l for l in lines)
+    k = next(i for i, l in enumerate(lines) if l.startswith("| " + region))
+    cells = lines[k + 1].replace("|", " ").split()
+    # time, share in nested regions ("Kernel" when none), number, average, GFlop/s, GB/s
+    assert int(cells[2]) == calls, cells
+    assert float(cells[0]) > 0 and float(cells[4]) > 0 and float(cells[5]) > 0, cells
+    assert any(l.startswith("| {Main}") for l in lines)
+    assert any(l.startswith("GPU kernels") for l in lines)
 
 
 def test_spmain_driver_matches_oracle_from_the_reference_start(tmp_path):
@@ -80,3 +96,4 @@ def test_spmain_driver_matches_oracle_from_the_reference_start(tmp_path):
     _, _, rr = S.hooi(dims, idx, vals.astype(np.float32), U0, sweeps)
     print("spmain residual", res[0], "oracle", rr[-1], flush=True)
     assert abs(res[0] - rr[-1]) < 5e-6   # printed with 6 decimals; FP32 TTMc
+    check_summary(p.stderr, "SpTucker::Sp_HOOI_ALS::sweeps()", 1)
