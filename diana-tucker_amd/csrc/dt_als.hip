@@ -682,6 +682,30 @@ static int rows(dt_ctx *ctx, int mode, const double *S, const double *Y, int I, 
     return launch_rows<64>(ctx, mode, S, Y, I, R, Rinv, out, part, U32, M32, Ub16);
 }
 
+// Q <- the orthonormal factor of X (I x R FP64, R <= 64) with positive R
+// diagonal: shifted Cholesky-QR, then two plain passes (Fukaya et al. 2020),
+// each pass one als_chol (partial Grams summed, R^{-1}) and one als_rows
+// (X <- X R^{-1} in place) that also leaves the next pass's partial Grams.
+// Nothing waits for the host; a Cholesky breakdown ORs 4 into *flag.
+// ws: (ceil(I / 128) + 1) R^2 doubles.
+int dt_cholqr3_rows(dt_ctx *ctx, double *X, int I, int R, double *ws, int *flag) {
+    if (R < 1 || R > ALS_MAXR) return dt_fail(ctx, DT_EUNSUPPORTED, "Cholesky-QR: %d columns > %d", R, ALS_MAXR);
+    const int nb = (I + ALS_GR - 1) / ALS_GR;
+    double *part = ws, *ri = ws + (size_t)nb * R * R;
+    if (R <= 16) {
+        DT_LAUNCH("als_gram", als_gram_kernel<16>, dim3(nb), dim3(ALS_THREADS), 0, X, I, R, part);
+    } else if (R <= 32) {
+        DT_LAUNCH("als_gram", als_gram_kernel<32>, dim3(nb), dim3(ALS_THREADS), 0, X, I, R, part);
+    } else {
+        DT_LAUNCH("als_gram", als_gram_kernel<64>, dim3(nb), dim3(ALS_THREADS), 0, X, I, R, part);
+    }
+    for (int pass = 0; pass < 3; pass++) {
+        DT_TRY(launch_chol(ctx, part, nb, R, I, pass == 0 ? 1 : 0, ri, flag));
+        DT_TRY(rows(ctx, 1, nullptr, X, I, R, ri, X, pass < 2 ? part : nullptr));
+    }
+    return DT_OK;
+}
+
 // Workspace needs: part (ceil(I/64) R^2) + 2 Rinv (R^2) + 2 panels (I R) doubles.
 size_t dt_als_ws_doubles(size_t I, size_t R) {
     return ((I + ALS_GR - 1) / ALS_GR) * R * R + 2 * R * R + 2 * I * R + 64;

@@ -485,6 +485,23 @@ extern "C" int dt_qr_f64(dt_ctx *ctx, double *Q, double *R, const double *A, siz
     DT_CHECK_ARG(ctx && Q && A, "dt_qr_f64: null argument");
     DT_CHECK_ARG(m >= n && n >= 1, "dt_qr_f64: need m >= n >= 1 (got %zu x %zu)", m, n);
     const size_t nn = n * n;
+    // Q only, 16 < n <= 64: shifted Cholesky-QR3 over many workgroups with one
+    // host wait for its breakdown flag (the GEMM path below waits after every
+    // Cholesky and runs a tiled GEMM with few tiles at these shapes)
+    if (!R && n > 16 && n <= 64 && m >= 256 && m < ((size_t)1 << 31) && !(n <= (size_t)TS_NMAX && m >= 4096)) {
+        const size_t nb = (m + 127) / 128;
+        double *ws = (double *)dt_ws(ctx, WS_CHOLQR, ((nb + 1) * nn + 8) * sizeof(double));
+        if (!ws) return DT_ENOMEM;
+        int *dflag = (int *)(ws + (nb + 1) * nn);
+        if (Q != A) DT_HIP(hipMemcpyAsync(Q, A, m * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        DT_TRY(dt_zero(ctx, dflag, sizeof(int)));
+        DT_TRY(dt_cholqr3_rows(ctx, Q, (int)m, (int)n, ws, dflag));
+        int *pin = (int *)ctx->pinned;
+        DT_HIP(hipMemcpyAsync(pin, dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        DT_HIP(hipStreamSynchronize(ctx->stream));
+        if (pin[0] != 0) return dt_fail(ctx, DT_ENUMERIC, "dt_qr_f64: rank-deficient or non-finite input");
+        return DT_OK;
+    }
     double *G = (double *)dt_ws(ctx, WS_SMALL0, nn * sizeof(double));
     double *Rk = (double *)dt_ws(ctx, WS_SMALL1, nn * sizeof(double));
     double *Rki = (double *)dt_ws(ctx, WS_SMALL2, nn * sizeof(double));

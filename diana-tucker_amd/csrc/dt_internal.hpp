@@ -77,6 +77,7 @@ enum dt_ws_slot {
     WS_MLP1,
     WS_MLP2,
     WS_SKETCH,         // learned start: every mode's sketch, Gram and eigenvectors
+    WS_CHOLQR,         // dt_qr_f64's multi-workgroup Cholesky-QR: partial Grams, R^{-1}, flag
     WS_COUNT_
 };
 static_assert(WS_COUNT_ <= dt_ctx::kSlots, "too many workspace slots");
@@ -229,6 +230,10 @@ int dt_chol_rinv_dev(dt_ctx *ctx, const double *G, int n, bool shifted, double m
 // out[e] = sum over p < nparts of part[p * ne + e] in a fixed order
 int dt_part_reduce(dt_ctx *ctx, const double *part, int nparts, int ne, double *out);
 size_t dt_als_ws_doubles(size_t I, size_t R);
+// X (I x R, R <= 64) <- its orthonormal QR factor (positive R diagonal) by
+// shifted Cholesky-QR3 over many workgroups; no host wait, breakdown ORs 4
+// into *flag; ws: (ceil(I / 128) + 1) R^2 doubles
+int dt_cholqr3_rows(dt_ctx *ctx, double *X, int I, int R, double *ws, int *flag);
 int dt_als_update(dt_ctx *ctx, const double *S, double *U, int I, int R, int inner, double *ws,
                   int *flag, float *U32, float *M32, uint16_t *Ub16);
 int dt_als_fused(dt_ctx *ctx, const double *S, double *U, int I, int R, int inner, double *L,

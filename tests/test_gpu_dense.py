@@ -245,6 +245,34 @@ def test_qr_f64(ctx, m, n):
     assert np.allclose(Q, Qn, atol=1e-10)
 
 
+@pytest.mark.parametrize("m,n", [(1024, 64), (700, 40), (300, 17), (5000, 48), (256, 64)])
+def test_qr_f64_q_only(ctx, m, n):
+    """Q only (no R), 16 < n <= 64: shifted Cholesky-QR3 over many workgroups;
+    the same Q as the positive-diagonal QR, also at condition 1e9."""
+    rng = np.random.default_rng(m + n)
+    for cond in (1.0, 1e9):
+        U = np.linalg.qr(rng.standard_normal((m, n)))[0]
+        V = np.linalg.qr(rng.standard_normal((n, n)))[0]
+        A = U @ np.diag(np.logspace(0, -np.log10(cond), n)) @ V.T
+        Q, _ = ctx.qr(ctx.to_device(A), want_r=False)
+        Q = Q.download()
+        assert np.allclose(Q.T @ Q, np.eye(n), atol=1e-12), cond
+        if cond == 1.0:
+            assert np.allclose(Q, D.orth_posdiag(A), atol=1e-10)
+        else:
+            assert D.sin_max_angle(Q, A) < 1e-6
+
+
+def test_qr_f64_q_only_non_finite_is_enumeric(ctx):
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((600, 40))
+    A[5, 7] = np.nan
+    with pytest.raises(dt.DTError) as e:
+        ctx.qr(ctx.to_device(A), want_r=False)
+    assert e.value.code == dt.DT_ENUMERIC
+    Q, _ = ctx.qr(ctx.to_device(rng.standard_normal((600, 40))), want_r=False)  # the context still works
+
+
 @pytest.mark.parametrize("m", [500, 6000])
 def test_qr_ill_conditioned_uses_shift(ctx, m):
     """Condition 1e9 breaks the plain Cholesky-QR; the shifted pass recovers
