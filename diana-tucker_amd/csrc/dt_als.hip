@@ -265,70 +265,75 @@ __device__ int chol_wave(const double *__restrict__ A, int lda, double *__restri
     return bad;
 }
 
-// Order 64 as 32 x 32 blocks, G = [A B; Bᵀ C] (LDS, row stride 65):
-//   R11 = chol(A), R12 = R11⁻ᵀ B, R22 = chol(C − R12ᵀR12),
-//   R⁻¹ = [R11⁻¹, −R11⁻¹ R12 R22⁻¹; 0, R22⁻¹],
-// the two Cholesky factors by one wave each in turn (chol_wave<32>, whose
-// register broadcasts fit where order 64 would not), the products by the
-// whole workgroup.  Writes the R x R corner of R⁻¹ (column-major) unless a
-// pivot fails, which ORs 4 into *flag.
-__device__ void chol64_blocked(double (*G)[65], int R, double *__restrict__ Rinv_out, int *__restrict__ flag) {
-    __shared__ double Rm[32][33], I11[32][33], I22[32][33], R12[32][33];
+// Order 64 as a 4 x 4 grid of 16 x 16 blocks (G: LDS, row stride 65, upper
+// part used), right-looking: for block column k, one wave factors the
+// diagonal block (chol_wave<16>: R_kk and R_kk⁻¹), the workgroup forms the
+// block row R_kj = R_kk⁻ᵀ G_kj and updates the trailing blocks
+// G_ij -= R_kiᵀ R_kj (i <= j).  Then R⁻¹ by block diagonals d = 1..3:
+// X_ij = -R_ii⁻¹ sum_{k = i+1..j} R_ik X_kj, every X_kj of a smaller d being
+// known.  Writes the R x R corner of R⁻¹ (column-major) unless a pivot fails,
+// which ORs 4 into *flag.
+__device__ void chol64_blocked(double (*G)[65], double (*Rb)[65], int R, double *__restrict__ Rinv_out,
+                               int *__restrict__ flag) {
+    __shared__ double Id[4][16][17];  // inverses of the diagonal blocks of R
+    __shared__ double Xi[64][65];     // R^{-1}
+    __shared__ double Tt[3][16][17];
     __shared__ int bad_s;
     const int tid = threadIdx.x, nt = blockDim.x;
-    if (tid < 64) {
-        const int b = chol_wave<32>(&G[0][0], 65, &Rm[0][0], 33, &I11[0][0], 33);
-        if (tid == 0) bad_s = b;
-    }
-    __syncthreads();
-    // R12 = I11ᵀ B (I11 upper: m <= i)
-    for (int e = tid; e < 1024; e += nt) {
-        const int i = e >> 5, j = e & 31;
-        double s = 0.0;
-        for (int m = 0; m <= i; m++) s += I11[m][i] * G[m][32 + j];
-        R12[i][j] = s;
-    }
-    __syncthreads();
-    // Schur complement in place of C
-    for (int e = tid; e < 1024; e += nt) {
-        const int i = e >> 5, j = e & 31;
-        double s = G[32 + i][32 + j];
-        for (int m = 0; m < 32; m++) s -= R12[m][i] * R12[m][j];
-        G[32 + i][32 + j] = s;
-    }
-    __syncthreads();
-    if (tid < 64) {
-        const int b = chol_wave<32>(&G[32][32], 65, &Rm[0][0], 33, &I22[0][0], 33);
-        if (tid == 0 && !bad_s && b) bad_s = 32 + b;
+    if (tid == 0) bad_s = 0;
+    for (int kb = 0; kb < 4; kb++) {
+        const int k0 = 16 * kb;
+        __syncthreads();
+        if (tid < 64) {
+            const int b = chol_wave<16>(&G[k0][k0], 65, &Rb[k0][k0], 65, &Id[kb][0][0], 17);
+            if (tid == 0 && b && !bad_s) bad_s = k0 + b;
+        }
+        __syncthreads();
+        const int nc = 48 - k0;  // columns right of the diagonal block
+        for (int e = tid; e < 16 * nc; e += nt) {
+            const int i = e & 15, j = k0 + 16 + (e >> 4);
+            double s = 0.0;
+            for (int m = 0; m <= i; m++) s += Id[kb][m][i] * G[k0 + m][j];  // (R_kk⁻ᵀ)_im, upper R_kk⁻¹
+            Rb[k0 + i][j] = s;
+        }
+        __syncthreads();
+        for (int e = tid; e < nc * nc; e += nt) {
+            const int i = k0 + 16 + e % nc, j = k0 + 16 + e / nc;
+            if (i <= j) {
+                double s = G[i][j];
+                for (int m = 0; m < 16; m++) s -= Rb[k0 + m][i] * Rb[k0 + m][j];
+                G[i][j] = s;
+            }
+        }
     }
     __syncthreads();
     if (bad_s) {
         if (tid == 0) atomicOr(flag, 4);
         return;
     }
-    // T = I11 R12 into Rm (I11 upper: m >= i), then the upper-right block
-    // -T I22 (I22 upper: m <= j) straight to the output
-    for (int e = tid; e < 1024; e += nt) {
-        const int i = e >> 5, j = e & 31;
-        double s = 0.0;
-        for (int m = i; m < 32; m++) s += I11[i][m] * R12[m][j];
-        Rm[i][j] = s;
+    for (int e = tid; e < 64 * 64; e += nt) {
+        const int i = e & 63, j = e >> 6;
+        Xi[i][j] = (i >> 4) == (j >> 4) ? Id[i >> 4][i & 15][j & 15] : 0.0;
+    }
+    for (int d = 1; d < 4; d++) {
+        __syncthreads();
+        for (int e = tid; e < (4 - d) * 256; e += nt) {
+            const int ib = e >> 8, i = e & 15, j = (e >> 4) & 15, jb = ib + d;
+            double s = 0.0;
+            for (int kb = ib + 1; kb <= jb; kb++)
+                for (int m = 0; m < 16; m++) s += Rb[16 * ib + i][16 * kb + m] * Xi[16 * kb + m][16 * jb + j];
+            Tt[ib][i][j] = s;
+        }
+        __syncthreads();
+        for (int e = tid; e < (4 - d) * 256; e += nt) {
+            const int ib = e >> 8, i = e & 15, j = (e >> 4) & 15, jb = ib + d;
+            double s = 0.0;
+            for (int m = i; m < 16; m++) s += Id[ib][i][m] * Tt[ib][m][j];
+            Xi[16 * ib + i][16 * jb + j] = -s;
+        }
     }
     __syncthreads();
-    for (int e = tid; e < R * R; e += nt) {
-        const int i = e % R, k = e / R;  // entry (i, k) of R^{-1}, column-major
-        double v;
-        if (i >= 32) {
-            v = k >= 32 ? I22[i - 32][k - 32] : 0.0;
-        } else if (k < 32) {
-            v = I11[i][k];
-        } else {
-            double s = 0.0;
-            for (int m = 0; m <= k - 32; m++) s += Rm[i][m] * I22[m][k - 32];
-            v = -s;
-        }
-        Rinv_out[e] = v;
-    }
+    for (int e = tid; e < R * R; e += nt) Rinv_out[e] = Xi[e % R][e / R];
 }
 
 // One workgroup: G = sum_w part[w] (fixed order); optional shift; Rinv = chol(G)^{-1}.
@@ -351,12 +356,23 @@ __global__ __launch_bounds__(ALS_THREADS) void als_chol_kernel(const double *__r
         double acc[NE];
 #pragma unroll
         for (int q = 0; q < NE; q++) acc[q] = 0.0;
-        for (int w = 0; w < nparts; w++) {
+        // four partials' loads in flight at a time, added in partial order
+        // (a partial past nparts reads the last one and adds zero)
+        for (int w0 = 0; w0 < nparts; w0 += 4) {
+            double v[4][NE];
 #pragma unroll
-            for (int q = 0; q < NE; q++) {
-                const int e = tid + ALS_THREADS * q;
-                if (e < R * R) acc[q] += part[(size_t)w * RR + e];
+            for (int u = 0; u < 4; u++) {
+                const int w = w0 + u < nparts ? w0 + u : nparts - 1;
+#pragma unroll
+                for (int q = 0; q < NE; q++) {
+                    const int e = tid + ALS_THREADS * q;
+                    v[u][q] = part[(size_t)w * RR + (e < R * R ? e : 0)];
+                }
             }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int q = 0; q < NE; q++) acc[q] += w0 + u < nparts ? v[u][q] : 0.0;
         }
 #pragma unroll
         for (int q = 0; q < NE; q++) {
@@ -383,7 +399,7 @@ __global__ __launch_bounds__(ALS_THREADS) void als_chol_kernel(const double *__r
     }
     __syncthreads();
     if constexpr (CM > 32) {
-        chol64_blocked(G, R, Rinv_out, flag);
+        chol64_blocked(G, X, R, Rinv_out, flag);
         return;
     }
     if (tid < 64) {
