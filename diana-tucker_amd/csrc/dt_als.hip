@@ -676,13 +676,95 @@ __global__ __launch_bounds__(AF_THREADS) void als_fused_kernel(const double *__r
     }
 }
 
+// Mode 0 on FP64 MFMA for I % 128 == 0: out = (S Y) Rinv, a workgroup owning
+// 16 rows.  Wave w sums k over [w I/4, (w + 1) I/4) in chunks of 16: lane
+// (c, g) loads S[row0 + c][k0 + 4g .. +3] (S symmetric, so that is 32
+// contiguous bytes of column row0 + c) and Y[k0 + 4g .. +3][16t + c] for the
+// four 16-column tiles t; MFMA step s pairs k = k0 + 4g + s in both operands,
+// and register v of accumulator t ends as (S Y)[row0 + 4v + g][16t + c].  Two
+// register sets of chunks alternate.  The waves' partials are added in wave
+// order through LDS, then multiplied by Rinv (identity when null).
+__global__ __launch_bounds__(ALS_THREADS) void als_sy_mfma(const double *__restrict__ S, const double *__restrict__ Y,
+                                                          int I, int R, const double *__restrict__ Rinv,
+                                                          double *__restrict__ out) {
+    __shared__ double P[16][65];
+    __shared__ double Ri[64][65];
+    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int c = lane & 15, g = lane >> 4;
+    const int row0 = blockIdx.x * 16;
+    for (int e = tid; e < 64 * 64; e += ALS_THREADS) {
+        const int i = e & 63, j = e >> 6;
+        Ri[i][j] = (i < R && j < R) ? (Rinv ? Rinv[i + (size_t)R * j] : (i == j ? 1.0 : 0.0)) : 0.0;
+    }
+    const int kw = I / 4, kbeg = wv * kw, kend = kbeg + kw;
+    const double *sp = S + (size_t)(row0 + c) * I + 4 * g;
+    const double *yp[4];
+    bool yin[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int col = 16 * t + c;
+        yin[t] = col < R;
+        yp[t] = Y + (size_t)(yin[t] ? col : R - 1) * I + 4 * g;
+    }
+    doublex4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) acc[t] = doublex4{0.0, 0.0, 0.0, 0.0};
+    auto ld = [&](int k0, doublex4 &a, doublex4 (&b)[4]) {
+        a = *(const doublex4 *)(sp + k0);
+#pragma unroll
+        for (int t = 0; t < 4; t++) b[t] = *(const doublex4 *)(yp[t] + k0);
+    };
+    auto mma = [&](const doublex4 &a, const doublex4 (&b)[4]) {
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], yin[t] ? b[t][s] : 0.0, acc[t], 0, 0, 0);
+    };
+    doublex4 aX, bX[4], aY, bY[4];
+    ld(kbeg, aX, bX);
+    ld(kbeg + 16, aY, bY);
+    for (int k0 = kbeg; k0 < kend; k0 += 32) {
+        // past the range the last chunk is loaded again and not used
+        mma(aX, bX);
+        ld(k0 + 32 < kend ? k0 + 32 : k0, aX, bX);
+        mma(aY, bY);
+        ld(k0 + 48 < kend ? k0 + 48 : k0 + 16, aY, bY);
+    }
+    for (int w = 0; w < 4; w++) {
+        if (wv == w) {
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const double x = acc[t][v];
+                    P[4 * v + g][16 * t + c] = w > 0 ? P[4 * v + g][16 * t + c] + x : x;
+                }
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < 16 * R; e += ALS_THREADS) {
+        const int i = e & 15, j = e >> 4;
+        double s0 = 0.0, s1 = 0.0;
+        for (int m = 0; m < 64; m += 2) {
+            s0 = fma(P[i][m], Ri[m][j], s0);
+            s1 = fma(P[i][m + 1], Ri[m + 1][j], s1);
+        }
+        out[(size_t)(row0 + i) + (size_t)I * j] = s0 + s1;
+    }
+}
+
 template <int RMAX>
 static int launch_rows(dt_ctx *ctx, int mode, const double *S, const double *Y, int I, int R,
                        const double *Rinv, double *out, double *part, float *U32, float *M32,
                        uint16_t *Ub16) {
-    const int nb = (I + AlsRows<RMAX>::RB - 1) / AlsRows<RMAX>::RB;
-    DT_LAUNCH("als_rows", als_rows_kernel<RMAX>, dim3(nb), dim3(ALS_THREADS), 0, mode, S, Y, I, R, Rinv,
-              out, U32, M32, Ub16);
+    if (mode == 0 && R > 32 && I >= 128 && I % 128 == 0 && out != Y && !U32 && !M32 && !Ub16) {
+        DT_LAUNCH("als_rows", als_sy_mfma, dim3(I / 16), dim3(ALS_THREADS), 0, S, Y, I, R, Rinv, out);
+    } else {
+        const int nb = (I + AlsRows<RMAX>::RB - 1) / AlsRows<RMAX>::RB;
+        DT_LAUNCH("als_rows", als_rows_kernel<RMAX>, dim3(nb), dim3(ALS_THREADS), 0, mode, S, Y, I, R, Rinv,
+                  out, U32, M32, Ub16);
+    }
     if (part) {
         const int ng = (I + ALS_GR - 1) / ALS_GR;
         DT_LAUNCH("als_gram", als_gram_kernel<RMAX>, dim3(ng), dim3(ALS_THREADS), 0, out, I, R, part);
