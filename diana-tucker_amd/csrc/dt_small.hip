@@ -280,6 +280,98 @@ __global__ __launch_bounds__(SMALL_THREADS) void cholqr3_kernel(double *X, int m
     }
 }
 
+// FP64 MFMA forms of the two products of the fit correction (n <= 64, 8
+// waves).  Output tiles of 16 x 16 go to waves round-robin; v_mfma_f64_16x16x4f64
+// takes A[i][k] from lane (i, g = k) and B[k][j] from lane (j, g = k), and
+// register v of lane (c, g) holds row 4v + g, column c of the tile.  Indices
+// past n (and rows past m) read as zero.
+__device__ __forceinline__ double ld4_f64(const float *p, int s) { return (double)p[s]; }
+__device__ __forceinline__ double ld4_f64(const uint16_t *p, int s) { return (double)bf16_to_f32(p[s]); }
+__device__ __forceinline__ double ld4_f64(const double *p, int s) { return p[s]; }
+
+// G (n x n, LDS, column-major) = XᵀX for X (m x n, global, column-major).
+// Step s of a 16-row chunk k0 pairs row k = k0 + 4g + s in both operands, so
+// each lane reads four consecutive rows of its column.
+template <typename T>
+__device__ void wg_xtx_mfma(const T *__restrict__ X, int m, int n, double *__restrict__ G) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int c = lane & 15, g = lane >> 4, nt = (n + 15) / 16;
+    for (int tile = wv; tile < nt * nt; tile += nw) {
+        const int ti = tile % nt, tj = tile / nt;
+        const int ca = 16 * ti + c, cb = 16 * tj + c;
+        const T *pa = X + (size_t)(ca < n ? ca : n - 1) * m;
+        const T *pb = X + (size_t)(cb < n ? cb : n - 1) * m;
+        const bool ina = ca < n, inb = cb < n;
+        doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+        int k0 = 0;
+        // four chunks' loads issued before their MFMAs
+        for (; k0 + 64 <= m; k0 += 64) {
+            double a[4][4], b[4][4];
+            if (ina && inb) {
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int s = 0; s < 4; s++) {
+                        a[u][s] = ld4_f64(pa, k0 + 16 * u + 4 * g + s);
+                        b[u][s] = ld4_f64(pb, k0 + 16 * u + 4 * g + s);
+                    }
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int s = 0; s < 4; s++) {
+                        a[u][s] = ina ? ld4_f64(pa, k0 + 16 * u + 4 * g + s) : 0.0;
+                        b[u][s] = inb ? ld4_f64(pb, k0 + 16 * u + 4 * g + s) : 0.0;
+                    }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int s = 0; s < 4; s++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][s], b[u][s], acc, 0, 0, 0);
+        }
+        for (; k0 < m; k0 += 16) {
+            double a[4], b[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const int k = k0 + 4 * g + s, kc = k < m ? k : m - 1;
+                a[s] = (ina && k < m) ? ld4_f64(pa, kc) : 0.0;
+                b[s] = (inb && k < m) ? ld4_f64(pb, kc) : 0.0;
+            }
+#pragma unroll
+            for (int s = 0; s < 4; s++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = 16 * ti + 4 * v + g, j = 16 * tj + c;
+            if (i < n && j < n) G[i + n * j] = acc[v];
+        }
+    }
+    __syncthreads();
+}
+
+// out = x y (n x n, LDS, column-major)
+__device__ void wg_small_mm_mfma(const double *__restrict__ x, const double *__restrict__ y, int n,
+                                 double *__restrict__ out) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int c = lane & 15, g = lane >> 4, nt = (n + 15) / 16;
+    for (int tile = wv; tile < nt * nt; tile += nw) {
+        const int ti = tile % nt, tj = tile / nt;
+        const int i = 16 * ti + c, j = 16 * tj + c;
+        doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < n; k0 += 4) {
+            const int k = k0 + g;
+            const double a = (i < n && k < n) ? x[i + n * k] : 0.0;
+            const double b = (k < n && j < n) ? y[k + n * j] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int r = 16 * ti + 4 * v + g;
+            if (r < n && j < n) out[r + n * j] = acc[v];
+        }
+    }
+}
+
 // Workgroup q: C_q (n_q x n_q) = (X_qᵀX_q)^{-1}, X_q m_q x n_q column-major in
 // FP64, FP32 or bf16 (widened on load).
 // out (n x n, LDS) = x y (LDS, n x n column-major); waves over columns, lanes
@@ -316,13 +408,13 @@ __global__ __launch_bounds__(SMALL_THREADS) void gram_inverse_batch_kernel(dt_gr
     double *G = lds_base(), *stage = G + n * n;
     // 64-row chunks: the staging (2 x 65 x n) fits in the 3 n^2 that Newton-Schulz
     // needs anyway for n >= 44, and the launch provides at least that much
-    const int ch = b.ch;
+    (void)b.ch;
     if (b.dt[q] == DT_F32)
-        wg_xty((const float *)b.X[q], (const float *)b.X[q], m, n, G, stage, ch);
+        wg_xtx_mfma((const float *)b.X[q], m, n, G);
     else if (b.dt[q] == DT_BF16)
-        wg_xty((const uint16_t *)b.X[q], (const uint16_t *)b.X[q], m, n, G, stage, ch);
+        wg_xtx_mfma((const uint16_t *)b.X[q], m, n, G);
     else
-        wg_xty((const double *)b.X[q], (const double *)b.X[q], m, n, G, stage, ch);
+        wg_xtx_mfma((const double *)b.X[q], m, n, G);
     // ||G - I||_F^2
     double e2 = 0.0;
     for (int e = tid; e < n * n; e += nt) {
@@ -339,9 +431,9 @@ __global__ __launch_bounds__(SMALL_THREADS) void gram_inverse_batch_kernel(dt_gr
         for (int e = tid; e < n * n; e += nt) C[e] = ((e % n) == (e / n) ? 2.0 : 0.0) - G[e];
         __syncthreads();
         for (int it = 0; it < 5; it++) {
-            wg_small_mm(G, C, n, T);  // T = G C
+            wg_small_mm_mfma(G, C, n, T);  // T = G C
             __syncthreads();
-            wg_small_mm(C, T, n, T2);  // T2 = C G C
+            wg_small_mm_mfma(C, T, n, T2);  // T2 = C G C
             __syncthreads();
             for (int e = tid; e < n * n; e += nt) C[e] = 2.0 * C[e] - T2[e];
             __syncthreads();
