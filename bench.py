@@ -60,7 +60,7 @@ def tree_contractions(shape, ranks):
     return out
 
 
-def dense_c2(ctx, dt, steps, warmup, shape=(256,) * 4, ranks=(16,) * 4, noise=1e-2,
+def dense_256x4(ctx, dt, steps, warmup, shape=(256,) * 4, ranks=(16,) * 4, noise=1e-2,
              seed=20000905):
     log(f"[bench] generating {shape} FP32 Tucker model (ranks {ranks}, noise {noise})")
     g = ctx.gen_tucker(shape, ranks, noise, seed=seed)
@@ -113,7 +113,7 @@ def _median_ms(fn, reps=3):
     return vals[len(vals) // 2]
 
 
-def mixed_c3(ctx, dt, shape=(1024,) * 3, ranks=(64,) * 3, noise=1e-3, decay=0.93, seed=20000905 + 2,
+def mixed_1024x3(ctx, dt, shape=(1024,) * 3, ranks=(64,) * 3, noise=1e-3, decay=0.93, seed=20000905 + 2,
              max_sweeps=4):
     """Time-to-fit on the 1024^3 rank-64 workload (core 64^3 with mode spectra
     decaying as 0.93^k, noise 1e-3): wall time, including the start, until
@@ -132,7 +132,7 @@ def mixed_c3(ctx, dt, shape=(1024,) * 3, ranks=(64,) * 3, noise=1e-3, decay=0.93
     A32, A16 = g["A"], g["A_bf16"]
     ref = ctx.hooi_dense(A32, ranks, max_iter=3, init=dt.INIT_HOSVD, flags=dt.FLAG_FP64_FIT)
     target = 1.0 - ref["residual"]
-    model = LN.load_model(ctx, os.path.join(LN.MODEL_DIR, "learned_c3.npz"))
+    model = LN.load_model(ctx, os.path.join(LN.MODEL_DIR, "learned_tucker256_r64.npz"))
     kron = dt.FLAG_SKETCH_KRON
 
     def sweeps_needed(A, init, **kw):
@@ -212,7 +212,7 @@ def mixed_c3(ctx, dt, shape=(1024,) * 3, ranks=(64,) * 3, noise=1e-3, decay=0.93
     return out
 
 
-def sparse_c4(ctx, dt, dims=(1 << 20, 1 << 18, 1 << 16), ndraws=100_000_000, seed=20000905, sweeps=10):
+def sparse_zipf(ctx, dt, dims=(1 << 20, 1 << 18, 1 << 16), ndraws=100_000_000, seed=20000905, sweeps=10):
     """Sparse HOOI on the 2^20 x 2^18 x 2^16 COO tensor from 1e8 Zipf(1.1)
     draws per mode (duplicates merged), ranks (16,16,16), 10 sweeps.  The
     TTMc kernel of each mode is timed with HIP events inside the sweeps; its
@@ -286,8 +286,8 @@ def copy_calibration(ctx, dt, nbytes=1 << 30):
 
 def measured_traffic(name):
     """HBM bytes per launch of `name` from the committed counter summary
-    profiles/hbm_traffic_c2.txt (lines `kernel fetch_bytes write_bytes`), or None."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_c2.txt")
+    profiles/hbm_traffic_dense_256x4.txt (lines `kernel fetch_bytes write_bytes`), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_dense_256x4.txt")
     try:
         with open(path) as f:
             for line in f:
@@ -335,12 +335,12 @@ def main():
         sys.exit(2)
     import dtucker as dt
     with dt.Context(0) as ctx:
-        r = dense_c2(ctx, dt, args.steps, args.warmup)
+        r = dense_256x4(ctx, dt, args.steps, args.warmup)
         del r["A"]
         calib = copy_calibration(ctx, dt)
         log(f"[bench] copy calibration: {calib['GB_per_s']:.1f} GB/s")
-        c3 = None if args.no_mixed else mixed_c3(ctx, dt)
-        c4 = None if args.no_sparse else sparse_c4(ctx, dt)
+        c3 = None if args.no_mixed else mixed_1024x3(ctx, dt)
+        c4 = None if args.no_sparse else sparse_zipf(ctx, dt)
     steps = args.steps
     value = steps / r["wall"]
     passes = {}
@@ -360,7 +360,7 @@ def main():
         cpu = cpu_baseline_dense()
         log(f"[bench] CPU baseline {cpu['value']:.4g} iters/s")
     line = {
-        "metric": "HOOI iters/s (dense C2 256^4 FP32 ranks 16^4)",
+        "metric": "HOOI iters/s (dense 256^4 FP32, ranks 16^4)",
         "value": value,
         "unit": "iters/s",
         "n_gpus": 1,

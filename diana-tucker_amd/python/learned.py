@@ -14,8 +14,8 @@ projection of the sketch's leading directions onto span(U*):
 T = U* U*ᵀ X[:, :R].  This fixes a basis of span(U*) in the input's own
 frame, so a row-wise network can be trained with a squared loss.
 
-    python learned.py train [--config c5|c3] [--n-train N] [--epochs E] [--model PATH]
-    python learned.py eval  [--config c5|c3] [--n-eval N] [--model PATH] [--report PATH]
+    python learned.py train [--config tucker128_r16|tucker256_r64] [--n-train N] [--epochs E] [--model PATH]
+    python learned.py eval  [--config tucker128_r16|tucker256_r64] [--n-eval N] [--model PATH] [--report PATH]
 """
 import argparse
 import json
@@ -32,7 +32,7 @@ import dtucker as dt  # noqa: E402
 
 MODEL_DIR = os.path.join(os.path.dirname(HERE), "models")
 
-C5 = {"name": "c5", "shape": (128, 128, 128), "ranks": (16, 16, 16), "S": 32, "hidden": 512,
+FAMILY_128 = {"name": "tucker128_r16", "shape": (128, 128, 128), "ranks": (16, 16, 16), "S": 32, "hidden": 512,
       "base_seed": 20000905, "n_train": 3584, "n_total": 4096, "decay": (0.7, 0.95), "noise": (1e-3, 1e-1),
       "sketch": dt.SKETCH_GAUSSIAN}
 # The 1024^3 rank-64 mixed-precision workload's family (core decay 0.93, noise
@@ -40,10 +40,10 @@ C5 = {"name": "c5", "shape": (128, 128, 128), "ranks": (16, 16, 16), "S": 32, "h
 # sqrt(I_n), so it is trained on smaller tensors of the same model.  Its
 # sketch is the Kronecker one (q = 9 per mode, S = 81): at 1024^3 a dense
 # Gaussian Ω would have 2^20 x S entries per mode.
-C3 = {"name": "c3", "shape": (256, 256, 256), "ranks": (64, 64, 64), "S": 81, "q": 9, "hidden": 512,
+FAMILY_256_R64 = {"name": "tucker256_r64", "shape": (256, 256, 256), "ranks": (64, 64, 64), "S": 81, "q": 9, "hidden": 512,
       "base_seed": 30000905, "n_train": 192, "n_total": 224, "decay": (0.93, 0.93), "noise": (1e-3, 1e-3),
       "sketch": dt.SKETCH_KRONECKER}
-CONFIGS = {"c5": C5, "c3": C3}
+CONFIGS = {"tucker128_r16": FAMILY_128, "tucker256_r64": FAMILY_256_R64}
 
 
 def tensor_params(t, cfg):
@@ -217,7 +217,7 @@ def summarize(rows):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("what", choices=["train", "eval"])
-    ap.add_argument("--config", choices=sorted(CONFIGS), default="c5")
+    ap.add_argument("--config", choices=sorted(CONFIGS), default="tucker128_r16")
     ap.add_argument("--n-train", type=int, default=None)
     ap.add_argument("--n-eval", type=int, default=None)
     ap.add_argument("--epochs", type=int, default=10)

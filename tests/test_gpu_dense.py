@@ -359,8 +359,8 @@ def run_gpu_hooi(ctx, A, ranks, sweeps, init, U_raw=None, schedule=dt.SCHED_TREE
     (dt.SCHED_REFERENCE, dt.FLAG_FP64_FIT), (dt.SCHED_TREE, dt.FLAG_FP64_FIT),
     (dt.SCHED_REFERENCE, dt.FLAG_FP64_FIT | dt.FLAG_REFERENCE_ALS),
     (dt.SCHED_TREE, dt.FLAG_FP64_FIT | dt.FLAG_REFERENCE_ALS)])
-def test_hooi_c1_reference_random_init(ctx, schedule, flags):
-    """C1: 64^3, core 8^3, noise 1e-3, ranks 8, 10 sweeps, the reference's own
+def test_hooi_64cube_reference_random_init(ctx, schedule, flags):
+    """64^3, core 8^3, noise 1e-3, ranks 8, 10 sweeps, the reference's own
     random start (srand(20000905), Util::randn, reduced_QR)."""
     shape, ranks = (64, 64, 64), (8, 8, 8)
     g = ctx.gen_tucker(shape, ranks, 1e-3, seed=20000905)
@@ -385,7 +385,7 @@ def test_hooi_c1_reference_random_init(ctx, schedule, flags):
     assert rel(out["G"], Gg) < 1e-5
 
 
-def test_hooi_c1_hosvd_and_fp64(ctx):
+def test_hooi_64cube_hosvd_and_fp64(ctx):
     shape, ranks = (64, 64, 64), (8, 8, 8)
     g = ctx.gen_tucker(shape, ranks, 1e-3, seed=7)
     A = g["A"].download().astype(np.float64)
@@ -518,18 +518,18 @@ def test_hooi_zero_tensor_is_enumeric(ctx):
     assert e.value.code == dt.DT_ENUMERIC
 
 
-def test_hooi_c2_full_size_properties(ctx):
-    """C2 at its real shape: 256^4 FP32 (17.2 GB), core 16^4, noise 1e-2,
+def test_hooi_256x4_full_size_properties(ctx):
+    """256^4 FP32 at its real size (17.2 GB), core 16^4, noise 1e-2,
     HOSVD init, 20 sweeps.  Size-independent checks: the residual reaches the
     noise floor eps*sqrt(1 - prod R/prod I)/sqrt(1+eps^2) and the factors
     recover the true subspaces to the noise level."""
     shape, ranks, eps = (256,) * 4, (16,) * 4, 1e-2
-    print("C2 generate", flush=True)
+    print("256^4 generate", flush=True)
     g = ctx.gen_tucker(shape, ranks, eps, seed=20000905, want_factors=True)
-    print("C2 hooi", flush=True)
+    print("256^4 hooi", flush=True)
     out = ctx.hooi_dense(g["A"], ranks, max_iter=20, init=dt.INIT_HOSVD)
     pred = eps * np.sqrt(1 - np.prod(ranks) / np.prod(shape)) / np.sqrt(1 + eps * eps)
-    print("C2 residual", out["residual"], "pred", pred, "sweep ms", out["sweeps_ms"] / 20, flush=True)
+    print("256^4 residual", out["residual"], "pred", pred, "sweep ms", out["sweeps_ms"] / 20, flush=True)
     assert abs(out["residual"] - pred) / pred < 1e-3
     for n in range(4):
         assert D.sin_max_angle(out["U"][n], g["U"][n].download()) < 5e-3

@@ -194,7 +194,7 @@ def test_learned_init_without_model_is_einval(ctx):
     assert e.value.code == dt.DT_EINVAL
 
 
-def test_trained_c5_model_against_hosvd_and_sketch_only(ctx):
+def test_trained_model_128cube_against_hosvd_and_sketch_only(ctx):
     """The committed network (trained on tensors 0..3583 by the library's own
     kernels) on 8 held-out tensors of the learned-init workload (128^3, ranks
     16, decay U(0.7, 0.95), noise log-uniform [1e-3, 1e-1]): every start
@@ -208,8 +208,8 @@ def test_trained_c5_model_against_hosvd_and_sketch_only(ctx):
     sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                                     "diana-tucker_amd", "python"))
     import learned as LN
-    cfg = LN.C5
-    m = LN.load_model(ctx, os.path.join(LN.MODEL_DIR, "learned_c5.npz"))
+    cfg = LN.FAMILY_128
+    m = LN.load_model(ctx, os.path.join(LN.MODEL_DIR, "learned_tucker128_r16.npz"))
     try:
         ids = list(range(cfg["n_train"], cfg["n_train"] + 8))
         # warm-up (first launches of every kernel)

@@ -216,16 +216,16 @@ def test_sp_create_rejects_out_of_range_index(ctx):
     sp.close()
 
 
-def test_c4_full_size_generator_and_sort(ctx):
+def test_sparse_full_size_generator_and_sort(ctx):
     """The sparse workload at its real size: 2^20 x 2^18 x 2^16, 1e8 Zipf(1.1)
     draws.  Size-independent checks: ~65.6 % of the draws survive the merge,
     merged keys strictly increase, values stay in (0, #draws of the key], the
     slice-mode permutation is the stable argsort (keys non-decreasing along it
     and, within equal keys, positions increasing) and the row pointers count it."""
     dims = (1 << 20, 1 << 18, 1 << 16)
-    print("C4 generate", flush=True)
+    print("sparse 2^20x2^18x2^16 generate", flush=True)
     idx, val, nnz = dt.sp_gen_zipf(ctx, dims, 100_000_000, 1.1, 20000905)
-    print("C4 nnz", nnz, flush=True)
+    print("sparse 2^20x2^18x2^16 nnz", nnz, flush=True)
     assert 0.60 < nnz / 1e8 < 0.71
     i0 = idx[0].download()[:nnz].astype(np.int64)
     i1 = idx[1].download()[:nnz].astype(np.int64)
@@ -243,12 +243,12 @@ def test_c4_full_size_generator_and_sort(ctx):
         same = np.diff(ks) == 0
         assert np.all(np.diff(perm.astype(np.int64))[same] > 0)
         assert np.array_equal(np.diff(rowptr.astype(np.int64)), np.bincount(i0, minlength=dims[0]))
-        print("C4 mode-0 rows touched", nrows, flush=True)
+        print("sparse 2^20x2^18x2^16 mode-0 rows touched", nrows, flush=True)
     finally:
         sp.close()
 
 
-def test_c4_full_size_ttmc_rows(ctx):
+def test_sparse_full_size_ttmc_rows(ctx):
     """The TTMc at the sparse workload's real size (2^20 x 2^18 x 2^16, 1e8
     Zipf(1.1) draws, 65.6M nonzeros) for every mode: the 16 longest rows (up to
     4.2M nonzeros each, split over many ranges and summed by the fix-up) and 2000
@@ -258,7 +258,7 @@ def test_c4_full_size_ttmc_rows(ctx):
     relative per row (Frobenius).  Every row of Y is finite and the rows with no
     nonzero are zero."""
     dims = (1 << 20, 1 << 18, 1 << 16)
-    print("C4 ttmc: generate", flush=True)
+    print("sparse 2^20x2^18x2^16 ttmc: generate", flush=True)
     idx, val, nnz = dt.sp_gen_zipf(ctx, dims, 100_000_000, 1.1, 20000905)
     hidx = [idx[m].download()[:nnz].astype(np.int64) for m in range(3)]
     v = val.download()[:nnz].astype(np.float64)
@@ -269,7 +269,7 @@ def test_c4_full_size_ttmc_rows(ctx):
         dU = [dt.upload_rows(ctx, u) for u in Us]
         U64 = [u.astype(np.float64) for u in Us]
         for n in range(3):
-            print("C4 ttmc mode", n, flush=True)
+            print("sparse 2^20x2^18x2^16 ttmc mode", n, flush=True)
             Y = sp.ttmc(n, dU)
             Yh = dt.download_rows(Y)
             Y.free()
@@ -292,7 +292,7 @@ def test_c4_full_size_ttmc_rows(ctx):
                 e = float(np.linalg.norm(Yh[r] - ref) / max(np.linalg.norm(ref), 1e-300))
                 worst = max(worst, e)
                 assert e < 1e-5, (n, int(r), int(cnt[r]), e)
-            print("C4 ttmc mode", n, "rows checked", len(rows), "worst rel", worst, flush=True)
+            print("sparse 2^20x2^18x2^16 ttmc mode", n, "rows checked", len(rows), "worst rel", worst, flush=True)
             del Yh
     finally:
         sp.close()
