@@ -418,6 +418,24 @@ def test_hooi_ranks_above_32(ctx):
         assert max(angles) < (1e-7 if dtype == dt.DT_F64 else 1e-3)
 
 
+def test_hooi_ranks_above_32_mode_sizes_multiple_of_128(ctx):
+    """Ranks (40, 48, 64) on 256 x 128 x 128: every mode size is a multiple of
+    128, so the update's (S Y) R^{-1} steps run on FP64 MFMA (K split over four
+    waves) and the start's Q-only QR is the multi-workgroup Cholesky-QR3."""
+    shape, ranks = (256, 128, 128), (40, 48, 64)
+    g = ctx.gen_tucker(shape, ranks, 1e-3, seed=33)
+    A = g["A"].download().astype(np.float64)
+    Uh = D.hosvd_factors(A, ranks)
+    Gr, Ur, res_ref = D.hooi_als(A, ranks, 3, Uh)
+    for dtype in (dt.DT_F64, dt.DT_F32):
+        out = run_gpu_hooi(ctx, A, ranks, 3, dt.INIT_HOSVD, dtype=dtype, flags=dt.FLAG_FP64_FIT)
+        f_gpu, f_ref = 1 - out["residuals"][-1], 1 - res_ref[-1]
+        angles = [D.sin_max_angle(out["U"][n], Ur[n]) for n in range(3)]
+        print("ranks>32 x128 dtype", dtype, "fit gpu", f_gpu, "ref", f_ref, "angles", angles, flush=True)
+        assert abs(f_gpu - f_ref) / f_ref < (1e-10 if dtype == dt.DT_F64 else 1e-5)
+        assert max(angles) < (1e-7 if dtype == dt.DT_F64 else 1e-3)
+
+
 def test_hooi_4way_tree_matches_reference_schedule(ctx):
     shape, ranks = (32, 24, 20, 16), (4, 5, 3, 2)
     g = ctx.gen_tucker(shape, ranks, 1e-2, seed=5)
