@@ -179,6 +179,7 @@ __global__ void gather_by(const T *__restrict__ src, const int32_t *__restrict__
 typedef int intx4 __attribute__((ext_vector_type(4)));
 typedef float floatx2 __attribute__((ext_vector_type(2)));
 typedef int intx2 __attribute__((ext_vector_type(2)));
+typedef double double2x __attribute__((ext_vector_type(2)));
 
 struct SpModeArgs {
     const int32_t *row;     // sorted mode-n index of each nonzero
@@ -693,12 +694,14 @@ __device__ __forceinline__ void sp_gram_out(doublex4 gacc, int R, double *__rest
 // 16-row tiles: lane (i, g) reads row 16t + i of Y (its compact copy) as
 // float4s 4q + g, one FP32 MFMA step per float4 component against L (FP32 in
 // LDS) leaves W[16t + 4g + v][i] in register v; those values, widened to FP64,
-// are stored and fed to the Gram's FP64 MFMA as both operands.
+// are fed to the Gram's FP64 MFMA as both operands and stored column by
+// column through a per-wave LDS tile.
 __global__ __launch_bounds__(256) void sp_tl_gram(const float *__restrict__ Yc, const int32_t *__restrict__ rowptr,
                                                   const int32_t *__restrict__ slot, int64_t I, int R,
                                                   const double *__restrict__ L, int64_t tiles_per_wave,
                                                   double *__restrict__ U, double *__restrict__ part) {
     __shared__ float Ls[SP_E][20];
+    __shared__ double Wt[4][16][18];  // per wave: the tile by column, rows contiguous
     for (int e = threadIdx.x; e < SP_E * 16; e += 256) Ls[e / 16][e % 16] = (float)L[e];
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -723,10 +726,29 @@ __global__ __launch_bounds__(256) void sp_tl_gram(const float *__restrict__ Yc, 
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const double w = (double)acc[v];
-            const int64_t rr = t * 16 + 4 * g + v;
-            if (rr < I && i < R) U[rr + I * i] = w;
+            Wt[wv][i][4 * g + v] = w;
             gacc = __builtin_amdgcn_mfma_f64_16x16x4f64(w, w, gacc, 0, 0, 0);
         }
+        wave_sync();
+        {
+            // the tile goes out by column: lane l stores rows 4 (l % 4) .. +3 of
+            // column l / 4, so four lanes write one 128-byte column piece
+            // (16-byte stores when the column start is 16-byte aligned)
+            const int col = lane >> 2, r4 = 4 * (lane & 3);
+            const int64_t r0 = t * 16 + r4;
+            const double *src = &Wt[wv][col][r4];
+            double *dst = U + r0 + I * col;
+            if (col < R) {
+                if (r0 + 3 < I && (I & 1) == 0) {
+                    *(double2x *)dst = double2x{src[0], src[1]};
+                    *(double2x *)(dst + 2) = double2x{src[2], src[3]};
+                } else {
+                    for (int q = 0; q < 4; q++)
+                        if (r0 + q < I) dst[q] = src[q];
+                }
+            }
+        }
+        wave_sync();  // the tile buffer is rewritten by the next tile
     }
     sp_gram_out(gacc, R, part);
 }
