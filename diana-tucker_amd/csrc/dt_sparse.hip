@@ -762,6 +762,7 @@ template <bool GRAM, bool VIEW>
 __global__ __launch_bounds__(256) void sp_rmul(double *__restrict__ U, int64_t I, int R,
                                                const double *__restrict__ Rinv, int64_t tiles_per_wave,
                                                double *__restrict__ part, float *__restrict__ Ur) {
+    __shared__ double Wt[4][16][18];  // per wave: the new tile by column
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int c = lane & 15, g = lane >> 4;
     double b[4];
@@ -785,12 +786,29 @@ __global__ __launch_bounds__(256) void sp_rmul(double *__restrict__ U, int64_t I
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int64_t rr = r0 + 4 * v + g;
-            if (rr < I) {
-                if (c < R) U[rr + I * c] = d[v];
-                if (VIEW) Ur[rr * 16 + c] = (float)d[v];
-            }
+            Wt[wv][c][4 * v + g] = d[v];
+            if (VIEW && rr < I) Ur[rr * 16 + c] = (float)d[v];
             if (GRAM) gacc = __builtin_amdgcn_mfma_f64_16x16x4f64(d[v], d[v], gacc, 0, 0, 0);
         }
+        wave_sync();
+        {
+            // column-wise stores as in sp_tl_gram: lane l writes rows
+            // 4 (l % 4) .. +3 of column l / 4
+            const int col = lane >> 2, r4 = 4 * (lane & 3);
+            const int64_t rb = r0 + r4;
+            const double *src = &Wt[wv][col][r4];
+            double *dst = U + rb + I * col;
+            if (col < R) {
+                if (rb + 3 < I && (I & 1) == 0) {
+                    *(double2x *)dst = double2x{src[0], src[1]};
+                    *(double2x *)(dst + 2) = double2x{src[2], src[3]};
+                } else {
+                    for (int q = 0; q < 4; q++)
+                        if (rb + q < I) dst[q] = src[q];
+                }
+            }
+        }
+        wave_sync();
     }
     if (GRAM) sp_gram_out(gacc, R, part);
 }
