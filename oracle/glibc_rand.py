@@ -7,7 +7,9 @@ glibc's srand/rand (random_r.c, degree 31, separation 3): state r[0] = seed,
 r[i] = 16807 r[i-1] mod (2^31 - 1) for i = 1..30 (Schrage's method),
 r[i] = r[i-31] for i = 31..33, r[i] = r[i-31] + r[i-3] (mod 2^32) afterwards,
 and the k-th output is r[k + 344] >> 1.  Pinned against the C library's own
-rand() by tests/test_oracle_cpu.py.
+rand() by tests/test_oracle_cpu.py; util_randn is pinned bit for bit against the
+reference's own Util::randn, compiled from src/util.cpp into oracle/_ref/ref_randn
+by oracle/Makefile (10^4 draws per seed) and recorded in tests/golden/ref_randn.json.
 """
 import math
 

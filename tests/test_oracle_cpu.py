@@ -106,6 +106,57 @@ def test_util_randn_polar_moments():
     assert abs(x.mean()) < 0.03 and abs(x.std() - 1) < 0.03
 
 
+REF_RANDN = os.path.join(os.path.dirname(__file__), "golden", "ref_randn.json")
+REF_RANDN_BIN = os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref", "ref_randn")
+
+
+def _bits(x: float) -> str:
+    return "%016x" % int.from_bytes(np.float64(x).tobytes(), "little")
+
+
+def test_util_randn_matches_reference_fixture():
+    # draws of the reference's own Util::randn (src/util.cpp:10-18), compiled
+    # from its sources (tests/golden/make_ref_randn.py); bit for bit
+    with open(REF_RANDN) as f:
+        fx = json.load(f)
+    for seed, want in fx["draws"].items():
+        g = GlibcRand(int(seed))
+        got = [_bits(util_randn(g)) for _ in range(len(want))]
+        assert got == want, seed
+
+
+@pytest.mark.skipif(not os.path.exists(REF_RANDN_BIN), reason="oracle/_ref not built (make -C oracle)")
+def test_util_randn_matches_reference_binary_10k():
+    import subprocess
+    for seed in (20000905, 1, 424242):
+        out = subprocess.run([REF_RANDN_BIN, str(seed), "10000"], check=True, capture_output=True,
+                             text=True).stdout.split()
+        g = GlibcRand(seed)
+        assert [_bits(util_randn(g)) for _ in range(10000)] == out, seed
+
+
+def test_host_util_randn_matches_reference_fixture(tmp_path):
+    # the C++ host layer's Util::randn (diana-tucker_amd/host/include/util.hpp)
+    # against the same reference draws
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    inc = os.path.join(os.path.dirname(__file__), "..", "diana-tucker_amd", "host", "include")
+    src = tmp_path / "r.cpp"
+    src.write_text('#include <cstdio>\n#include <cstring>\n#include <cstdint>\n#include "util.hpp"\n'
+                   "int main(int c, char **v) { srand((unsigned)strtoul(v[1], 0, 10));\n"
+                   "  for (int i = 0; i < 1000; i++) { double x = Util::randn(); uint64_t b;\n"
+                   '    memcpy(&b, &x, 8); printf("%016llx\\n", (unsigned long long)b); } }\n')
+    exe = tmp_path / "r"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I", inc, str(src), "-o", str(exe)], check=True)
+    with open(REF_RANDN) as f:
+        fx = json.load(f)
+    for seed, want in fx["draws"].items():
+        out = subprocess.run([str(exe), seed], check=True, capture_output=True, text=True).stdout.split()
+        assert out == want[:1000], seed
+
+
 def test_philox_known_answers():
     # Salmon et al. SC'11 / Random123 known-answer vectors for philox4x32-10
     cases = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
