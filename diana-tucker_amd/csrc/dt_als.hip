@@ -804,16 +804,44 @@ int dt_cholqr3_rows(dt_ctx *ctx, double *X, int I, int R, double *ws, int *flag)
     return DT_OK;
 }
 
-// Workspace needs: part (ceil(I/64) R^2) + 2 Rinv (R^2) + 2 panels (I R) doubles.
+// Workspace needs: part (ceil(I/128) R^2) + 2 Rinv (R^2) + 2 panels (I R)
+// doubles; ranks above ALS_MAXR use a Gram, an R^{-1}, a reciprocal diagonal
+// and the two panels.
 size_t dt_als_ws_doubles(size_t I, size_t R) {
-    return ((I + ALS_GR - 1) / ALS_GR) * R * R + 2 * R * R + 2 * I * R + 64;
+    const size_t rows = ((I + ALS_GR - 1) / ALS_GR) * R * R + 2 * R * R + 2 * I * R + 64;
+    const size_t any = 2 * R * R + R + 2 * I * R + 64;
+    return rows > any ? rows : any;
+}
+
+// Ranks above ALS_MAXR: the same iteration with every I x R product on the
+// tiled FP64 GEMM and the R x R Cholesky in global memory (one workgroup):
+//   Q_0 = U;  for k < inner: Y = S Q_k, G = YᵀY, Q_{k+1} = Y chol(G + shift)^{-1};
+//   then U = Q chol(QᵀQ)^{-1}.
+static int als_update_any(dt_ctx *ctx, const double *S, double *U, int I, int R, int inner, double *ws,
+                          int *flag, float *U32, float *M32, uint16_t *Ub16) {
+    const size_t RR = (size_t)R * R, IR = (size_t)I * R;
+    double *G = ws, *Ri = G + RR, *invd = Ri + RR, *Y = invd + R, *Q = Y + IR;
+    const double *cur = U;
+    for (int k = 0; k < inner; k++) {
+        DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, I, R, I, 1.0, S, I, cur, I, 0.0, Y, I));
+        DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, R, R, I, 1.0, Y, I, Y, I, 0.0, G, R));
+        DT_TRY(dt_chol_rinv_ws(ctx, G, R, true, (double)I, Ri, invd, flag));
+        DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, I, R, R, 1.0, Y, I, Ri, R, 0.0, Q, I));
+        cur = Q;
+    }
+    DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, R, R, I, 1.0, cur, I, cur, I, 0.0, G, R));
+    DT_TRY(dt_chol_rinv_ws(ctx, G, R, false, (double)I, Ri, invd, flag));
+    DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, I, R, R, 1.0, cur, I, Ri, R, 0.0, Y, I));
+    DT_TRY(dt_copy_f64(ctx, U, Y, IR));
+    return dt_f64_to_views(ctx, U, (size_t)I, (size_t)R, U32, M32, Ub16, nullptr);
 }
 
 // U (I x R, orthonormal, FP64) <- orthonormal basis of span(S^inner U).
 // U32 (I x R), M32 (R x I), Ub16 (I x R) receive the kernel views (may be null).
 int dt_als_update(dt_ctx *ctx, const double *S, double *U, int I, int R, int inner, double *ws,
                   int *flag, float *U32, float *M32, uint16_t *Ub16) {
-    if (R < 1 || R > ALS_MAXR) return dt_fail(ctx, DT_EUNSUPPORTED, "ALS update: rank %d > %d", R, ALS_MAXR);
+    if (R < 1 || R > I) return dt_fail(ctx, DT_EINVAL, "ALS update: rank %d not in [1, %d]", R, I);
+    if (R > ALS_MAXR) return als_update_any(ctx, S, U, I, R, inner, ws, flag, U32, M32, Ub16);
     if (R <= 16 && I * 16 <= AF_MAXIR) {
         DT_LAUNCH("als_fused", als_fused_kernel<16>, dim3(1), dim3(AF_THREADS), 0, S, U, I, R, inner, flag, U32, M32,
                   Ub16);

@@ -127,7 +127,8 @@ static int als_update(dt_hooi *h, const void *Y, int ydt, const int64_t *sh, int
     for (int i = 0; i < n; i++) F *= sh[i];
     for (int i = n + 1; i < h->N; i++) B *= sh[i];
     const size_t In = (size_t)sh[n], Rn = (size_t)h->R[n];
-    const bool small = (double)In * Rn * Rn <= (double)(1 << 20);
+    // the fused R x R kernels hold their matrices in LDS (R <= 64)
+    const bool small = (double)In * Rn * Rn <= (double)(1 << 20) && Rn <= 64;
     DT_TRY(dt_gram_internal(ctx, Y, ydt, F, (int64_t)In, B, h->S));
     if (!h->args_reference_als) {
         // span-equivalent normalised iteration, row blocks over many workgroups;
@@ -207,7 +208,7 @@ static int ahat_sq_dev(dt_hooi *h, const void *A, int dt) {
     int64_t sh[DT_MAX_NDIM];
     for (int i = 0; i < N; i++) sh[i] = h->R[i];
     const double *cur = h->G64;
-    auto small = [&](int n) { return h->R[n] <= 64; };  // every supported rank
+    auto small = [&](int n) { return h->R[n] <= 64; };  // the batched LDS kernel's orders
     dt_gram_inverse_batch gb{};
     for (int n = 0; n < N; n++) {
         if (!small(n)) continue;

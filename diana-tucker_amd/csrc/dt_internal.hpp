@@ -213,7 +213,7 @@ int dt_als_prep(dt_ctx *ctx, const double *L, int I, int R, double *P, double *w
 int dt_als_finish(dt_ctx *ctx, const double *P, const double *W, int I, int R, double *L,
                   double *work, int *flag);
 int dt_cholqr3_dev(dt_ctx *ctx, double *X, int m, int n, double *work, int *flag);
-// up to DT_MAX_NDIM independent (XᵀX)^{-1} in one launch (one workgroup each)
+// up to DT_MAX_NDIM independent (XᵀX)^{-1} in one launch (one workgroup each), n <= 64
 struct dt_gram_inverse_batch {
     const void *X[DT_MAX_NDIM];
     int dt[DT_MAX_NDIM], m[DT_MAX_NDIM], n[DT_MAX_NDIM];
@@ -222,11 +222,16 @@ struct dt_gram_inverse_batch {
     int ch;  // rows per staged chunk (set by dt_gram_inverse_batch_dev)
 };
 int dt_gram_inverse_batch_dev(dt_ctx *ctx, const dt_gram_inverse_batch &b, int *flag);
+// C = G^{-1}; above order 64 work must hold n ints and C must not alias G
 int dt_inverse_flag(dt_ctx *ctx, const double *G, int n, double *C, double *work, int *flag, int bit);
 int dt_orth_dev(dt_ctx *ctx, double *X, size_t m, size_t n, double *work, int *flag);
 // Rinv (n x n upper, column-major) = inverse Cholesky factor of G (+ the
 // shifted-CholeskyQR shift for m_rows rows when shifted); failure ORs 4 into *flag
 int dt_chol_rinv_dev(dt_ctx *ctx, const double *G, int n, bool shifted, double m_rows, double *Rinv, int *flag);
+// The same for any order: above 64 the factorisation runs in place in G
+// (global, overwritten) with invd (n doubles) as scratch
+int dt_chol_rinv_ws(dt_ctx *ctx, double *G, int n, bool shifted, double m_rows, double *Rinv, double *invd,
+                    int *flag);
 // out[e] = sum over p < nparts of part[p * ne + e] in a fixed order
 int dt_part_reduce(dt_ctx *ctx, const double *part, int nparts, int ne, double *out);
 size_t dt_als_ws_doubles(size_t I, size_t R);

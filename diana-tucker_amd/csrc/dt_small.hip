@@ -96,7 +96,8 @@ __device__ void wg_xm(const double *X, const double *__restrict__ M, int I, int 
     }
 }
 
-// In-place Gauss-Jordan inverse with partial pivoting of the n x n LDS matrix A.
+// In-place Gauss-Jordan inverse with partial pivoting of the n x n matrix A
+// (LDS, or global memory private to this workgroup); perm holds n ints.
 // Returns nonzero on a zero pivot (uniform across the workgroup).
 __device__ int wg_inverse_inplace(double *__restrict__ A, int n, int *__restrict__ perm) {
     __shared__ int piv_s, bad_s;
@@ -106,10 +107,17 @@ __device__ int wg_inverse_inplace(double *__restrict__ A, int n, int *__restrict
     __syncthreads();
     for (int k = 0; k < n; k++) {
         if (tid < 64) {
-            // argmax |A[i][k]| over i >= k in one wave; ties keep the lower row
-            const int i = k + tid;
-            double v = (i < n) ? fabs(A[i + n * k]) : -1.0;
-            int p = (i < n) ? i : n;
+            // argmax |A[i][k]| over i >= k in one wave (rows strided by 64);
+            // ties keep the lower row
+            double v = -1.0;
+            int p = n;
+            for (int i = k + tid; i < n; i += 64) {
+                const double a = fabs(A[i + n * k]);
+                if (a > v) {
+                    v = a;
+                    p = i;
+                }
+            }
             for (int off = 32; off > 0; off >>= 1) {
                 const double ov = __shfl_xor(v, off, 64);
                 const int op = __shfl_xor(p, off, 64);
@@ -167,9 +175,11 @@ __device__ int wg_inverse_inplace(double *__restrict__ A, int n, int *__restrict
     return 0;
 }
 
-// G (n x n SPD, LDS) + shift I -> upper Cholesky factor R (in place in G);
-// Xo (n x n, LDS) <- R^{-1} (upper).  Returns nonzero on a non-positive pivot.
-__device__ int wg_chol_rinv(double *__restrict__ G, int n, double shift, double *__restrict__ Xo) {
+// G (n x n SPD) + shift I -> upper Cholesky factor R (in place in G);
+// Xo (n x n) <- R^{-1} (upper); invd: n doubles.  Each may be LDS or global
+// memory private to this workgroup.  Returns nonzero on a non-positive pivot.
+__device__ int wg_chol_rinv(double *__restrict__ G, int n, double shift, double *__restrict__ Xo,
+                            double *__restrict__ invd) {
     __shared__ int bad_s;
     __shared__ double rjj_s;
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -202,12 +212,10 @@ __device__ int wg_chol_rinv(double *__restrict__ G, int n, double shift, double 
     // Xo = R^{-1}: column j solved by thread j (back substitution), written
     // straight into Xo, which no thread reads during the solve; one division
     // per diagonal entry, shared by all columns
-    __shared__ double invd[kMaxR];
     for (int idx = tid; idx < n * n; idx += nt) Xo[idx] = 0.0;
     for (int i = tid; i < n; i += nt) invd[i] = 1.0 / G[i + n * i];
     __syncthreads();
-    const int j = tid;
-    if (j < n) {
+    for (int j = tid; j < n; j += nt) {
         for (int i = j; i >= 0; i--) {
             double s = (i == j) ? 1.0 : 0.0;
             for (int k = i + 1; k <= j; k++) s -= G[i + n * k] * Xo[k + n * j];
@@ -259,6 +267,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void als_finish_kernel(const double 
 __global__ __launch_bounds__(SMALL_THREADS) void cholqr3_kernel(double *X, int m, int n,
                                                                 int *__restrict__ flag) {
     __shared__ double shift_s;
+    __shared__ double invd[kMaxR];
     double *G = lds_base(), *Ri = G + n * n, *stage = Ri + n * n;
     for (int pass = 0; pass < 3; pass++) {
         wg_xty(X, X, m, n, G, stage, stage_rows(n));
@@ -272,7 +281,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void cholqr3_kernel(double *X, int m
             shift_s = shift;
         }
         __syncthreads();
-        if (wg_chol_rinv(G, n, shift_s, Ri)) {
+        if (wg_chol_rinv(G, n, shift_s, Ri, invd)) {
             if (threadIdx.x == 0) atomicOr(flag, 4);
             return;
         }
@@ -472,6 +481,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void chol_rinv_kernel(const double *
                                                                   double *__restrict__ Rinv,
                                                                   int *__restrict__ flag) {
     __shared__ double shift_s;
+    __shared__ double invd[kMaxR];
     double *M = lds_base(), *Xo = M + n * n;
     for (int i = threadIdx.x; i < n * n; i += blockDim.x) M[i] = G[i];
     __syncthreads();
@@ -485,11 +495,45 @@ __global__ __launch_bounds__(SMALL_THREADS) void chol_rinv_kernel(const double *
         shift_s = shift;
     }
     __syncthreads();
-    if (wg_chol_rinv(M, n, shift_s, Xo)) {
+    if (wg_chol_rinv(M, n, shift_s, Xo, invd)) {
         if (threadIdx.x == 0) atomicOr(flag, 4);
         return;
     }
     for (int i = threadIdx.x; i < n * n; i += blockDim.x) Rinv[i] = Xo[i];
+}
+
+// Orders above kMaxR (no LDS copy): G (n x n SPD, global, overwritten by its
+// upper Cholesky factor) + the optional shifted-CholeskyQR shift -> Rinv
+// (global); invd: n doubles of global scratch.  One workgroup; the matrices
+// are private to it, so the barriers order every access.
+__global__ __launch_bounds__(SMALL_THREADS) void chol_rinv_global_kernel(double *__restrict__ G, int n,
+                                                                         int shifted, double m_rows,
+                                                                         double *__restrict__ Rinv,
+                                                                         double *__restrict__ invd,
+                                                                         int *__restrict__ flag) {
+    __shared__ double shift_s;
+    if (threadIdx.x == 0) {
+        double shift = 0.0;
+        if (shifted) {
+            double tr = 0.0;
+            for (int i = 0; i < n; i++) tr += G[i + (size_t)n * i];
+            shift = 11.0 * (m_rows * n + (double)n * (n + 1)) * 1.1102230246251565e-16 * tr;
+        }
+        shift_s = shift;
+    }
+    __syncthreads();
+    if (wg_chol_rinv(G, n, shift_s, Rinv, invd) && threadIdx.x == 0) atomicOr(flag, 4);
+}
+
+// C = G^{-1} (Gauss-Jordan in place in C, global) for orders above kMaxR;
+// perm: n ints of global scratch.
+__global__ __launch_bounds__(SMALL_THREADS) void inverse_flag_global_kernel(const double *__restrict__ G, int n,
+                                                                            double *__restrict__ C,
+                                                                            int *__restrict__ perm,
+                                                                            int *__restrict__ flag, int bit) {
+    for (size_t i = threadIdx.x; i < (size_t)n * n; i += blockDim.x) C[i] = G[i];
+    __syncthreads();
+    if (wg_inverse_inplace(C, n, perm) && threadIdx.x == 0) atomicOr(flag, bit);
 }
 
 // Out (I x R, global) = S (I x I, symmetric, global) X (I x R, global).
@@ -561,6 +605,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void als_fused_kernel(const double *
                                                                   int *__restrict__ flag) {
     __shared__ int perm[kMaxR];
     __shared__ double shift_s;
+    __shared__ double invd[kMaxR];
     const int CH = stage_rows(R);
     double *M = lds_base(), *Ri = M + R * R, *stage = Ri + R * R;
     for (int e = threadIdx.x; e < I * R; e += blockDim.x) L[e] = U[e];
@@ -596,7 +641,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void als_fused_kernel(const double *
             shift_s = shift;
         }
         __syncthreads();
-        if (wg_chol_rinv(M, R, shift_s, Ri)) {
+        if (wg_chol_rinv(M, R, shift_s, Ri, invd)) {
             if (threadIdx.x == 0) atomicOr(flag, 4);
             return;
         }
@@ -655,7 +700,13 @@ int dt_cholqr3_dev(dt_ctx *ctx, double *X, int m, int n, double *work, int *flag
     return DT_OK;
 }
 int dt_inverse_flag(dt_ctx *ctx, const double *G, int n, double *C, double *work, int *flag, int bit) {
-    (void)work;
+    if (n > kMaxR) {
+        // work: n ints for the pivot rows
+        DT_CHECK_ARG(work && C != G, "inverse: order %d needs scratch and C != G", n);
+        DT_LAUNCH("als_small", inverse_flag_global_kernel, dim3(1), dim3(SMALL_THREADS), 0, G, n, C,
+                  (int *)work, flag, bit);
+        return DT_OK;
+    }
     DT_TRY(check_r(ctx, n));
     DT_LAUNCH("als_small", inverse_flag_kernel, dim3(1), dim3(SMALL_THREADS), (size_t)n * n * 8, G, n, C,
               flag, bit);
@@ -680,6 +731,14 @@ int dt_gram_inverse_batch_dev(dt_ctx *ctx, const dt_gram_inverse_batch &b, int *
     return DT_OK;
 }
 
+int dt_chol_rinv_ws(dt_ctx *ctx, double *G, int n, bool shifted, double m_rows, double *Rinv, double *invd,
+                    int *flag) {
+    if (n <= kMaxR) return dt_chol_rinv_dev(ctx, G, n, shifted, m_rows, Rinv, flag);
+    DT_LAUNCH("cholqr_small", chol_rinv_global_kernel, dim3(1), dim3(SMALL_THREADS), 0, G, n, shifted ? 1 : 0,
+              m_rows, Rinv, invd, flag);
+    return DT_OK;
+}
+
 int dt_chol_rinv_dev(dt_ctx *ctx, const double *G, int n, bool shifted, double m_rows, double *Rinv, int *flag) {
     DT_TRY(check_r(ctx, n));
     DT_LAUNCH("cholqr_small", chol_rinv_kernel, dim3(1), dim3(SMALL_THREADS), 2 * n * n * 8, G, n, shifted ? 1 : 0,
@@ -694,12 +753,12 @@ int dt_chol_rinv_dev(dt_ctx *ctx, const double *G, int n, bool shifted, double m
 int dt_orth_dev(dt_ctx *ctx, double *X, size_t m, size_t n, double *work, int *flag) {
     if ((double)m * n * n <= (double)(1 << 21) && n <= 32)
         return dt_cholqr3_dev(ctx, X, (int)m, (int)n, work, flag);
-    DT_TRY(check_r(ctx, (int)n));
     double *G = work, *Rinv = work + n * n, *T = work + 2 * n * n;
     for (int pass = 0; pass < 3; pass++) {
         DT_TRY(dt_gemm_f64_internal(ctx, 1, 0, n, n, m, 1.0, X, m, X, m, 0.0, G, n));
-        DT_LAUNCH("cholqr_small", chol_rinv_kernel, dim3(1), dim3(SMALL_THREADS), 2 * n * n * 8, G, (int)n,
-                  pass == 0 ? 1 : 0, (double)m, Rinv, flag);
+        // T (m n >= n doubles) is free until the product below: the reciprocal
+        // diagonal of the global-memory factorisation goes there
+        DT_TRY(dt_chol_rinv_ws(ctx, G, (int)n, pass == 0, (double)m, Rinv, T, flag));
         DT_TRY(dt_gemm_f64_internal(ctx, 0, 0, m, n, n, 1.0, X, m, Rinv, n, 0.0, T, m));
         DT_HIP(hipMemcpyAsync(X, T, m * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     }

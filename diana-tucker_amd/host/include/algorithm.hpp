@@ -19,7 +19,8 @@ struct HooiOptions {
     int init = DT_INIT_QR;            // reference: randn + reduced_QR
     int schedule = DT_SCHED_TREE;     // DT_SCHED_REFERENCE = the reference's N+1 chains
     int inner_iter = 5;               // ALS_ default (hooi_als.tpp:9)
-    bool fp64 = false;                // Tensor<double>: compute in FP64 instead of FP32
+    bool fp64 = true;                 // Tensor<double>: compute in FP64 as the reference does;
+                                      // false narrows A to FP32 and runs the FP32 kernels
     bool verbose = true;
     std::vector<double> residuals;    // filled by HOOI_ALS
 };

@@ -436,6 +436,42 @@ def test_hooi_ranks_above_32_mode_sizes_multiple_of_128(ctx):
         assert max(angles) < (1e-7 if dtype == dt.DT_F64 else 1e-3)
 
 
+@pytest.mark.parametrize("ranks,init,flags", [
+    ((100, 100, 100), dt.INIT_HOSVD, dt.FLAG_FP64_FIT),
+    ((100, 80, 70), dt.INIT_QR, dt.FLAG_FP64_FIT),
+    ((100, 100, 100), dt.INIT_QR, dt.FLAG_FP64_FIT | dt.FLAG_REFERENCE_ALS),
+    ((96, 100, 65), dt.INIT_HOSVD, 0),
+])
+def test_hooi_ranks_above_64(ctx, ranks, init, flags):
+    """The reference driver's rank (main.cpp:18,24: R = 100) and other ranks
+    above 64 on 200 x 160 x 120: the factor update runs its products on the
+    tiled FP64 GEMM with the R x R Cholesky (or, with FLAG_REFERENCE_ALS, the
+    literal ALS_ inverses) in global memory, the fit correction inverts
+    Grams above 64 in global memory, the mode products take the generic
+    kernels.  Fit and factors against the oracle's ALS_ from the same start."""
+    shape = (200, 160, 120)
+    g = ctx.gen_tucker(shape, (100, 100, 100), 1e-2, seed=41)
+    A = g["A"].download().astype(np.float64)
+    if init == dt.INIT_HOSVD:
+        U_raw, U0 = None, D.hosvd_factors(A, ranks)
+    else:
+        rng = np.random.default_rng(5)
+        U_raw = [rng.standard_normal((I, R)) for I, R in zip(shape, ranks)]
+        U0 = [D.reduced_qr(u)[0] for u in U_raw]
+    _, Ur, res_ref = D.hooi_als(A, ranks, 3, U0)
+    for dtype in (dt.DT_F64, dt.DT_F32):
+        out = run_gpu_hooi(ctx, A, ranks, 3, init, U_raw=U_raw, dtype=dtype, flags=flags)
+        f_gpu, f_ref = 1 - out["residuals"][-1], 1 - res_ref[-1]
+        angles = [D.sin_max_angle(out["U"][n], Ur[n]) for n in range(3)]
+        print("ranks>64", ranks, "init", init, "flags", flags, "dtype", dtype, "fit gpu", f_gpu, "ref", f_ref,
+              "angles", angles, flush=True)
+        assert abs(f_gpu - f_ref) / f_ref < (1e-10 if dtype == dt.DT_F64 else 1e-5)
+        assert max(angles) < (1e-7 if dtype == dt.DT_F64 else 1e-3)
+        for n in range(3):
+            U = out["U"][n]
+            assert np.abs(U.T @ U - np.eye(ranks[n])).max() < 1e-12
+
+
 def test_hooi_4way_tree_matches_reference_schedule(ctx):
     shape, ranks = (32, 24, 20, 16), (4, 5, 3, 2)
     g = ctx.gen_tucker(shape, ranks, 1e-2, seed=5)
