@@ -86,6 +86,12 @@ int dt_create(dt_ctx **out, int device) {
     if (hipHostMalloc((void **)&ctx->pinned, 4096, hipHostMallocDefault) != hipSuccess) {
         ctx->pinned = nullptr;
     }
+    // the allocation-free entry points' scratch address on this device (its
+    // failure only disables them, with the reason kept for their error)
+    if (dt_noalloc_setup(ctx) != DT_OK) {
+        ctx->noalloc_err = ctx->err;
+        ctx->err.clear();
+    }
     *out = ctx;
     return DT_OK;
 }

@@ -14,6 +14,8 @@
 // splits are summed in FP64 by gram_reduce, which also mirrors the upper half.
 #include "dt_internal.hpp"
 
+#include <mutex>
+
 template <typename T>
 __device__ __forceinline__ float ldg_f(const T *p);
 template <>
@@ -371,31 +373,31 @@ extern "C" int dt_sumsq(dt_ctx *ctx, const void *A, int dt, size_t n, double *ou
     return dt_sumsq_dev(ctx, A, dt, n, out_host);
 }
 
-// S (I x I, FP64, device) = A_(n) A_(n)^T for A viewed as [F, I, B].
-int dt_gram_internal(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t I, int64_t B,
-                     double *S) {
-    if (I == 0) return DT_OK;
-    const int64_t Ctot = F * B;
-    if (dtA == DT_F64) {
-        DT_LAUNCH("gram_generic_f64", cross_gram_generic<double>, gs_grid(I * I), dim3(256), 0,
-                  (const double *)A, (const double *)A, S, F, I, I, B);
-        return DT_OK;
-    }
-    DT_CHECK_ARG(dtA == DT_F32 || dtA == DT_BF16, "gram: unsupported type %d", dtA);
+// Split-K plan of the MFMA Gram of order I over Ctot columns: ~512
+// workgroups in total, at least 256 columns per split so the FP32 partial
+// tiles stay small next to the data read, at most max_split splits.
+static void gram_plan(int64_t I, int64_t Ctot, int64_t max_split, int64_t &nsplit, int64_t &cps) {
     const int nt = (int)ceil_div(I, 64);
     const int ntiles = nt * (nt + 1) / 2;
-    // ~512 workgroups in total, at least 256 columns per split so the FP32
-    // partial tiles stay small next to the data read.
-    int64_t nsplit = 512 / ntiles;
+    nsplit = 512 / ntiles;
     if (nsplit < 1) nsplit = 1;
     int64_t maxsplit = ceil_div(Ctot, 256);
     if (maxsplit < 1) maxsplit = 1;
     if (nsplit > maxsplit) nsplit = maxsplit;
-    if (nsplit > 256) nsplit = 256;
-    int64_t cps = ceil_div(ceil_div(Ctot, nsplit), 32) * 32;
+    if (nsplit > max_split) nsplit = max_split;
+    cps = ceil_div(ceil_div(Ctot, nsplit), 32) * 32;
     nsplit = ceil_div(Ctot, cps);
-    float *Sp = (float *)dt_ws(ctx, WS_GRAM_PART, (size_t)(nsplit * I * I) * sizeof(float));
-    if (!Sp) return DT_ENOMEM;
+}
+
+// S (I x I, FP64) from an FP32 / bf16 tensor viewed as [F, I, B] with the
+// planned split; Sp holds nsplit I x I FP32 partials.  Every partial entry the
+// reduction reads is written by this launch (whole lower-triangle tiles), so
+// Sp needs no clearing.
+static int gram_mfma_run(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t I, int64_t B, double *S,
+                         float *Sp, int64_t nsplit, int64_t cps) {
+    const int64_t Ctot = F * B;
+    const int nt = (int)ceil_div(I, 64);
+    const int ntiles = nt * (nt + 1) / 2;
     dim3 grid((unsigned)ntiles, (unsigned)nsplit);
     if (dtA == DT_F32) {
         if (F == 1) {
@@ -418,11 +420,34 @@ int dt_gram_internal(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t I, 
     return DT_OK;
 }
 
+// S (I x I, FP64, device) = A_(n) A_(n)^T for A viewed as [F, I, B].
+int dt_gram_internal(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t I, int64_t B,
+                     double *S) {
+    if (I == 0) return DT_OK;
+    if (dtA == DT_F64) {
+        DT_LAUNCH("gram_generic_f64", cross_gram_generic<double>, gs_grid(I * I), dim3(256), 0,
+                  (const double *)A, (const double *)A, S, F, I, I, B);
+        return DT_OK;
+    }
+    DT_CHECK_ARG(dtA == DT_F32 || dtA == DT_BF16, "gram: unsupported type %d", dtA);
+    int64_t nsplit, cps;
+    gram_plan(I, F * B, 256, nsplit, cps);
+    float *Sp = (float *)dt_ws(ctx, WS_GRAM_PART, (size_t)(nsplit * I * I) * sizeof(float));
+    if (!Sp) return DT_ENOMEM;
+    return gram_mfma_run(ctx, A, dtA, F, I, B, S, Sp, nsplit, cps);
+}
+
+static int gram_args(dt_ctx *ctx, const void *A, const void *S, const size_t *shape, int ndim, int n, int dtS,
+                     const char *who) {
+    DT_CHECK_ARG(ctx && A && S && shape, "%s: null argument", who);
+    DT_CHECK_ARG(ndim >= 1 && ndim <= DT_MAX_NDIM && n >= 0 && n < ndim, "%s: bad mode", who);
+    DT_CHECK_ARG(dtS == DT_F64 || dtS == DT_F32, "%s: output must be F64 or F32", who);
+    return DT_OK;
+}
+
 extern "C" int dt_gram(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n,
                        void *S, int dtS) {
-    DT_CHECK_ARG(ctx && A && S && shape, "dt_gram: null argument");
-    DT_CHECK_ARG(ndim >= 1 && ndim <= DT_MAX_NDIM && n >= 0 && n < ndim, "dt_gram: bad mode");
-    DT_CHECK_ARG(dtS == DT_F64 || dtS == DT_F32, "dt_gram: output must be F64 or F32");
+    DT_TRY(gram_args(ctx, A, S, shape, ndim, n, dtS, "dt_gram"));
     int64_t F = 1, B = 1;
     for (int i = 0; i < n; i++) F *= (int64_t)shape[i];
     for (int i = n + 1; i < ndim; i++) B *= (int64_t)shape[i];
@@ -432,6 +457,78 @@ extern "C" int dt_gram(dt_ctx *ctx, const void *A, int dtA, int ndim, const size
     if (!tmp) return DT_ENOMEM;
     DT_TRY(dt_gram_internal(ctx, A, dtA, F, I, B, tmp));
     return dt_convert(ctx, S, DT_F32, tmp, DT_F64, (size_t)(I * I));
+}
+
+// ---- the Gram without allocations (dt_gram_noalloc) --------------------------
+// Its device scratch is this file-scope array: each GPU holds its own copy
+// from module load to process exit.  Host code never passes `scratch` itself
+// to a kernel (on the host that name is a stand-in, not a device address):
+// dt_noalloc_setup, run by dt_create with the context's device current, looks
+// up the device address once per device and every later use goes through it.
+static __device__ __align__(256) unsigned char scratch[DT_GRAM_NOALLOC_SCRATCH];
+
+constexpr int kNoAllocMaxDevices = 64;
+static void *g_scratch_dev[kNoAllocMaxDevices];
+static std::mutex g_scratch_mu;
+
+int dt_noalloc_setup(dt_ctx *ctx) {
+    ctx->noalloc_scratch = nullptr;
+    if (ctx->device < 0 || ctx->device >= kNoAllocMaxDevices)
+        return dt_fail(ctx, DT_EUNSUPPORTED, "allocation-free scratch: device %d >= %d", ctx->device,
+                       kNoAllocMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[ctx->device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return dt_fail(ctx, DT_EHIP, "allocation-free scratch: hipGetSymbolAddress: %s", hipGetErrorString(e));
+        }
+        g_scratch_dev[ctx->device] = p;
+    }
+    ctx->noalloc_scratch = g_scratch_dev[ctx->device];
+    return DT_OK;
+}
+
+extern "C" int dt_gram_noalloc(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n,
+                               void *S, int dtS) {
+    DT_TRY(gram_args(ctx, A, S, shape, ndim, n, dtS, "dt_gram_noalloc"));
+    DT_CHECK_ARG(dtA == DT_F64 || dtA == DT_F32 || dtA == DT_BF16, "dt_gram_noalloc: type %d", dtA);
+    if (!ctx->noalloc_scratch)
+        return dt_fail(ctx, DT_EHIP, "dt_gram_noalloc: no scratch address for device %d (%s)", ctx->device,
+                       ctx->noalloc_err.c_str());
+    int64_t F = 1, B = 1;
+    for (int i = 0; i < n; i++) F *= (int64_t)shape[i];
+    for (int i = n + 1; i < ndim; i++) B *= (int64_t)shape[i];
+    const int64_t I = (int64_t)shape[n];
+    if (I == 0) return DT_OK;
+    const size_t cap = DT_GRAM_NOALLOC_SCRATCH;
+    // an F32 output goes through an FP64 Gram at the start of the scratch
+    const size_t tmp_bytes = dtS == DT_F32 ? (((size_t)(I * I) * sizeof(double) + 255) & ~(size_t)255) : 0;
+    const size_t part_bytes = dtA == DT_F64 ? 0 : (size_t)(I * I) * sizeof(float);
+    if (tmp_bytes + part_bytes > cap)
+        return dt_fail(ctx, DT_EINVAL,
+                       "dt_gram_noalloc: a Gram of order %lld needs %zu bytes of scratch, above the %zu-byte maximum",
+                       (long long)I, tmp_bytes + part_bytes, cap);
+    // no event creation inside the entry point: its launches run untimed
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    unsigned char *base = (unsigned char *)ctx->noalloc_scratch;
+    double *Sd = dtS == DT_F64 ? (double *)S : (double *)base;
+    int st = DT_OK;
+    if (dtA == DT_F64) {
+        hipLaunchKernelGGL(cross_gram_generic<double>, gs_grid(I * I), dim3(256), 0, ctx->stream,
+                           (const double *)A, (const double *)A, Sd, F, I, I, B);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) st = dt_fail(ctx, DT_EHIP, "dt_gram_noalloc: %s", hipGetErrorString(e));
+    } else {
+        int64_t nsplit, cps;
+        gram_plan(I, F * B, (int64_t)((cap - tmp_bytes) / part_bytes), nsplit, cps);
+        st = gram_mfma_run(ctx, A, dtA, F, I, B, Sd, (float *)(base + tmp_bytes), nsplit, cps);
+    }
+    if (st == DT_OK && dtS == DT_F32) st = dt_convert(ctx, S, DT_F32, Sd, DT_F64, (size_t)(I * I));
+    ctx->timing = timing;
+    return st;
 }
 
 // C (IA x JB, FP64) = A_(n) B_(n)ᵀ for FP32 / bf16 tensors (split-K MFMA).

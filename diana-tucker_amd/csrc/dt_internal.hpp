@@ -45,7 +45,13 @@ struct dt_ctx {
     double *pinned = nullptr;
     // resident workgroups on this context's device per kernel (occupancy x CUs)
     std::map<const void *, int64_t> resident_wgs;
+    // device address of dt_gram_noalloc's static scratch on this device (set by
+    // dt_create through dt_noalloc_setup; null with the reason in noalloc_err)
+    void *noalloc_scratch = nullptr;
+    std::string noalloc_err;
 };
+// looks up this context's device address of the allocation-free scratch
+int dt_noalloc_setup(dt_ctx *ctx);
 
 // Workspace slot ids (one owner each, so nested helpers never alias).
 enum dt_ws_slot {

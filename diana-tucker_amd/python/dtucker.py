@@ -84,6 +84,7 @@ def lib():
             "dt_ttmc": [vp, vp, i32, i32, C.POINTER(sz), C.POINTER(vp), i32, C.POINTER(sz), C.POINTER(i32),
                         i32, vp, i32],
             "dt_gram": [vp, vp, i32, i32, C.POINTER(sz), i32, vp, i32],
+            "dt_gram_noalloc": [vp, vp, i32, i32, C.POINTER(sz), i32, vp, i32],
             "dt_ttt_except": [vp, vp, vp, i32, i32, C.POINTER(sz), sz, i32, vp],
             "dt_sumsq": [vp, vp, i32, sz, C.POINTER(dbl)],
             "dt_gemm_f64": [vp, i32, i32, sz, sz, sz, dbl, vp, sz, vp, sz, dbl, vp, sz],
@@ -136,7 +137,7 @@ def exported_symbols():
     return ["dt_create", "dt_destroy", "dt_last_error", "dt_malloc", "dt_free", "dt_h2d",
             "dt_d2h", "dt_d2d", "dt_zero", "dt_sync", "dt_convert", "dt_timing_enable",
             "dt_timing_reset", "dt_timing_filter", "dt_timing_get", "dt_timing_list", "dt_timer_start",
-            "dt_timer_stop", "dt_ttm", "dt_ttmc", "dt_gram", "dt_ttt_except", "dt_sumsq", "dt_gemm_f64",
+            "dt_timer_stop", "dt_ttm", "dt_ttmc", "dt_gram", "dt_gram_noalloc", "dt_ttt_except", "dt_sumsq", "dt_gemm_f64",
             "dt_qr_f64", "dt_inverse_f64", "dt_syev_top_f64", "dt_orth_rows_f64",
             "dt_gen_normal", "dt_gen_uniform", "dt_gen_tucker", "dt_hooi_dense",
             "dt_hooi_create", "dt_hooi_sweep", "dt_hooi_fit", "dt_hooi_get", "dt_hooi_destroy",

@@ -114,6 +114,21 @@ int dt_ttmc(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape,
 int dt_gram(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n,
             void *S, int dtS);
 
+/* dt_gram for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  No
+ * counterpart in the reference.  Same arguments and results as dt_gram; the
+ * split-K partials (and, for an F32 S, the FP64 Gram before its conversion)
+ * live in a static device array of DT_GRAM_NOALLOC_SCRATCH bytes, one per GPU,
+ * held from library load to process exit, instead of the context workspace.
+ * Calls no hipMalloc/hipFree and creates no events (its launches are not
+ * timed).  A Gram whose scratch exceeds the maximum returns DT_EINVAL: for
+ * FP32/BF16 input an order above 2048 (F64 S) or 1182 (F32 S); for FP64 input
+ * an order above 1448 with an F32 S (an F64 S needs no scratch).  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (two streams, two contexts, two threads) race. */
+#define DT_GRAM_NOALLOC_SCRATCH ((size_t)16 << 20)
+int dt_gram_noalloc(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n,
+                    void *S, int dtS);
+
 /* C = A_(n) B_(n)^T where A and B agree on every mode except n (C is
  * shapeA[n] x shapeB[n], FP64).  Replaces Function::ttt_except
  * (template/function/tensor.tpp:132-259).  A and B are F64 or F32. */
