@@ -572,6 +572,11 @@ int dt_hooi_fit(dt_hooi *h, int fp64, double *residual) {
     const void *fitA = h->args.A_f32 ? (const void *)h->args.A_f32 : h->args.A;
     const int fitdt = h->args.A_f32 ? DT_F32 : h->args.dtype;
     DT_TRY(dt_zero(ctx, h->dflag, sizeof(int)));
+    if (fp64 && !h->fbuf[0]) {
+        // a session created without DT_FLAG_FP64_FIT has no FP64 chain buffers yet
+        DT_TRY(hooi_alloc(h, &h->fbuf[0], h->buf_elems * 8));
+        DT_TRY(hooi_alloc(h, &h->fbuf[1], h->buf_elems * 8));
+    }
     const int saved = h->args.flags;
     if (fp64) {
         h->args.flags |= DT_FLAG_FP64_FIT;

@@ -562,6 +562,28 @@ def test_hooi_session_matches_one_shot(ctx):
     assert rel(out["G"], one["G"]) < 1e-12
 
 
+def test_hooi_session_fp64_fit_without_the_flag(ctx):
+    """dt_hooi_fit(fp64 = 1) on a session created without DT_FLAG_FP64_FIT
+    (its FP64 chain buffers are made on first use) equals the fit of a session
+    created with the flag, and the FP64 oracle's fit of the same factors."""
+    shape, ranks = (64, 48, 40), (6, 5, 4)
+    g = ctx.gen_tucker(shape, ranks, 1e-3, seed=15, bf16=True)
+    fits = []
+    for flags in (0, dt.FLAG_FP64_FIT):
+        s = ctx.hooi_session(g["A_bf16"], ranks, init=dt.INIT_RANDOM, seed=4, A_f32=g["A"], flags=flags)
+        s.sweep()
+        fits.append(s.fit(fp64=True))
+        U = s.result()["U"]
+        s.close()
+    assert fits[0] == fits[1]
+    A = g["A"].download().astype(np.float64)
+    G = A
+    for n in range(3):
+        G = D.ttm(G, U[n].T, n)
+    res = np.sqrt(max(0.0, 1 - np.sum(G * G) / np.sum(A * A)))
+    assert abs(fits[0] - res) < 1e-9
+
+
 def test_hooi_rank_above_size_is_einval(ctx):
     dA = ctx.to_device(np.ones((4, 5, 6), np.float32))
     with pytest.raises(dt.DTError) as e:
