@@ -463,6 +463,17 @@ int dt_hooi_create(dt_ctx *ctx, const dt_hooi_args *args, dt_hooi **out) {
         maxI = std::max(maxI, h->I[n]);
         maxR = std::max(maxR, h->R[n]);
     }
+    for (int n = 0; n < N; n++) {
+        // Y_(n) has prod_{m != n} R_m columns: a larger R_n leaves its Gram singular
+        int64_t k = 1;
+        for (int m = 0; m < N; m++)
+            if (m != n) k *= h->R[m];
+        if (h->R[n] > k) {
+            delete h;
+            return dt_fail(ctx, DT_EINVAL, "hooi: rank %zu of mode %d above the product %lld of the other ranks",
+                           args->ranks[n], n, (long long)k);
+        }
+    }
     const size_t wsz = h->work_dt == DT_F64 ? 8 : 4;
     int st = DT_OK;
     // largest intermediate: A with one mode contracted

@@ -261,3 +261,14 @@ int dt_ttm2_trail(dt_ctx *ctx, const float *A, int dtA, int64_t F, int64_t K2, i
 int dt_ttm_f64acc(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t K, int64_t B,
                   const double *M64, int64_t J, double *Y);
 int dt_dot_f64_dev(dt_ctx *ctx, const double *a, const double *b, size_t n, double *out_dev);
+// ---- sort and scan (dt_sort.hip) ----------------------------------------------
+// out[i] = sum of in[0..i) (int32; in and out may alias); tmp: dt_scan_tmp_bytes(n)
+int dt_exclusive_scan_i32(dt_ctx *ctx, const int32_t *in, int32_t *out, int64_t n, int32_t *tmp);
+size_t dt_scan_tmp_bytes(int64_t n);
+// stable LSD radix sort of (key, payload) by the low nbits of the key, 8 bits a
+// pass; kin != kout and vin != vout; tmp: dt_radix_sort_tmp_bytes(n, sizeof key)
+size_t dt_radix_sort_tmp_bytes(int64_t n, int key_bytes);
+int dt_radix_sort_u32(dt_ctx *ctx, const uint32_t *kin, uint32_t *kout, const int32_t *vin, int32_t *vout, int64_t n,
+                      int nbits, void *tmp);
+int dt_radix_sort_u64(dt_ctx *ctx, const uint64_t *kin, uint64_t *kout, const int32_t *vin, int32_t *vout, int64_t n,
+                      int nbits, void *tmp);

@@ -15,8 +15,8 @@
 // which it applies to its slice mode): the permutation, the sorted mode-n
 // index of every nonzero, the other modes' indices and the values in that
 // order, row pointers (I_n + 1), and a compact numbering ("slot") of the
-// non-empty rows.  Sorting uses rocPRIM's LSD radix sort (hipcub), which is
-// stable, so the permutation is exactly the reference's.
+// non-empty rows.  Sorting uses the stable LSD radix sort of dt_sort.hip, so
+// the permutation is exactly the reference's.
 //
 // TTMc (sp_ttmc_dev): Y_c[slot(i), :] = sum over the nonzeros of row i of
 // val * (U_a[i_a, :] ⊗ U_b[i_b, :]) for the two other modes a < b (entry
@@ -29,10 +29,9 @@
 // sp_fixup_chunks add in a fixed order (deterministic).  L = Y_cᵀ U_n is a second, dense pass over Y_c
 // (sp_ytu) with per-workgroup partials summed in a fixed order in FP64; the
 // factor update (sp_tl_gram, sp_rmul) never waits for the host.
-#include "dt_internal.hpp"
+#include "dt_sparse.hpp"
 #include "dt_philox.hpp"
 
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -176,7 +175,7 @@ __global__ void gather_by(const T *__restrict__ src, const int32_t *__restrict__
 
 // ---- TTMc --------------------------------------------------------------------
 
-typedef int intx4 __attribute__((ext_vector_type(4)));
+typedef dt_intx4 intx4;
 typedef float floatx2 __attribute__((ext_vector_type(2)));
 typedef int intx2 __attribute__((ext_vector_type(2)));
 typedef double double2x __attribute__((ext_vector_type(2)));
@@ -843,24 +842,6 @@ __global__ void sumsq_f32_sp(const float *__restrict__ v, int64_t n, double *__r
 
 // ---- the sparse tensor ---------------------------------------------------------
 
-struct dt_sp_mode {
-    int32_t *perm = nullptr, *row = nullptr, *rowptr = nullptr, *slot = nullptr, *rows = nullptr;
-    intx4 *rec = nullptr;  // 3-way tensors: TTMc records in this mode's order
-    int64_t nrows = 0;
-};
-
-struct dt_sp {
-    int ndim = 0;
-    int64_t dims[DT_MAX_NDIM] = {};
-    int64_t nnz = 0;
-    int slice_mode = 0;
-    int32_t *idx[DT_MAX_NDIM] = {};
-    float *val = nullptr;
-    dt_sp_mode mode[DT_MAX_NDIM];
-    double normA2 = 0.0;
-    std::vector<void *> owned;
-};
-
 static void *sp_alloc(dt_sp *sp, size_t bytes) {
     void *p = nullptr;
     if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) return nullptr;
@@ -868,8 +849,8 @@ static void *sp_alloc(dt_sp *sp, size_t bytes) {
     return p;
 }
 
-// temp storage for hipcub calls (grow-only workspace slot)
-static void *cub_tmp(dt_ctx *ctx, size_t bytes) { return dt_ws(ctx, WS_SPARSE3, std::max<size_t>(bytes, 256)); }
+// temp storage of the radix sort and the scans (grow-only workspace slot)
+static void *sort_tmp(dt_ctx *ctx, size_t bytes) { return dt_ws(ctx, WS_SPARSE3, std::max<size_t>(bytes, 256)); }
 
 static int sort_mode_internal(dt_ctx *ctx, int64_t nnz, const int32_t *keys, int64_t dim, int32_t *perm,
                               int32_t *sorted_keys, int32_t *rowptr) {
@@ -877,13 +858,9 @@ static int sort_mode_internal(dt_ctx *ctx, int64_t nnz, const int32_t *keys, int
     if (!iota) return DT_ENOMEM;
     DT_LAUNCH("sp_sort", iota32, sp_grid(nnz), dim3(256), 0, iota, nnz);
     const int nb = bits_for(dim);
-    size_t tb = 0;
-    DT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (const uint32_t *)keys, (uint32_t *)sorted_keys, iota,
-                                              perm, (int)nnz, 0, nb, ctx->stream));
-    void *tmp = cub_tmp(ctx, tb);
+    void *tmp = sort_tmp(ctx, dt_radix_sort_tmp_bytes(nnz, 4));
     if (!tmp) return DT_ENOMEM;
-    DT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, (const uint32_t *)keys, (uint32_t *)sorted_keys, iota, perm,
-                                              (int)nnz, 0, nb, ctx->stream));
+    DT_TRY(dt_radix_sort_u32(ctx, (const uint32_t *)keys, (uint32_t *)sorted_keys, iota, perm, nnz, nb, tmp));
     // row pointers from the sorted keys (int32: nnz < 2^31)
     DT_LAUNCH("sp_sort", row_bounds, sp_grid(nnz + 1), dim3(256), 0, (const int32_t *)sorted_keys, nnz, dim, rowptr);
     return DT_OK;
@@ -901,15 +878,9 @@ static int sort_rows_by_a(dt_ctx *ctx, int64_t nnz, int ba, int bs, intx4 *rec) 
     intx4 *tmp = (intx4 *)(((uintptr_t)(p2 + nnz) + 255) & ~(uintptr_t)255);
     int st = DT_OK;
     hipLaunchKernelGGL(rec_keys, sp_grid(nnz), dim3(256), 0, ctx->stream, rec, nnz, ba, k1, p1);
-    size_t tb = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k1, k2, p1, p2, (int)nnz, 0, ba + bs, ctx->stream) !=
-        hipSuccess)
-        st = DT_EHIP;
-    void *ctmp = st == DT_OK ? cub_tmp(ctx, tb) : nullptr;
-    if (st == DT_OK && !ctmp) st = DT_ENOMEM;
-    if (st == DT_OK &&
-        hipcub::DeviceRadixSort::SortPairs(ctmp, tb, k1, k2, p1, p2, (int)nnz, 0, ba + bs, ctx->stream) != hipSuccess)
-        st = DT_EHIP;
+    void *ctmp = sort_tmp(ctx, dt_radix_sort_tmp_bytes(nnz, 8));
+    if (!ctmp) st = DT_ENOMEM;
+    if (st == DT_OK) st = dt_radix_sort_u64(ctx, k1, k2, p1, p2, nnz, ba + bs, ctmp);
     if (st == DT_OK) {
         hipLaunchKernelGGL(rec_gather, sp_grid(nnz), dim3(256), 0, ctx->stream, (const intx4 *)rec, p2, nnz, tmp);
         if (hipMemcpyAsync(rec, tmp, (size_t)nnz * 16, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
@@ -987,17 +958,11 @@ extern "C" int dt_sp_gen_zipf(dt_ctx *ctx, int ndim, const size_t *dims, int64_t
     int32_t *heads = uidx + n;
     DT_LAUNCH("sp_gen", zipf_draw, sp_grid(n), dim3(256), 0, n, ndim, dcdf, d_coff, d_dims, d_shift, seed, key, pos,
               vdraw);
-    size_t tb = 0;
-    DT_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, key2, pos, pos2, (int)n, 0, tot, ctx->stream));
-    void *tmp = cub_tmp(ctx, tb);
+    void *tmp = sort_tmp(ctx, std::max(dt_radix_sort_tmp_bytes(n, 8), dt_scan_tmp_bytes(n)));
     if (!tmp) return DT_ENOMEM;
-    DT_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, key, key2, pos, pos2, (int)n, 0, tot, ctx->stream));
+    DT_TRY(dt_radix_sort_u64(ctx, key, key2, pos, pos2, n, tot, tmp));
     DT_LAUNCH("sp_gen", run_heads, sp_grid(n), dim3(256), 0, key2, n, flag);
-    size_t tb2 = 0;
-    DT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, flag, uidx, (int)n, ctx->stream));
-    tmp = cub_tmp(ctx, tb2);
-    if (!tmp) return DT_ENOMEM;
-    DT_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb2, flag, uidx, (int)n, ctx->stream));
+    DT_TRY(dt_exclusive_scan_i32(ctx, flag, uidx, n, (int32_t *)tmp));
     int32_t last[2];
     DT_HIP(hipMemcpyAsync(&last[0], uidx + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
     DT_HIP(hipMemcpyAsync(&last[1], flag + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -1023,9 +988,9 @@ extern "C" int dt_sp_destroy(dt_sp *sp) {
     return DT_OK;
 }
 
-extern "C" int dt_sp_create(dt_ctx *ctx, int ndim, const size_t *dims, int64_t nnz, const int32_t *const *idx,
-                            const float *val, dt_sp **out) {
-    DT_CHECK_ARG(ctx && dims && idx && val && out, "dt_sp_create: null argument");
+static int sp_create_impl(dt_ctx *ctx, int ndim, const size_t *dims, int64_t nnz, const int32_t *const *idx,
+                          const float *val, const double *val64, dt_sp **out) {
+    DT_CHECK_ARG(ctx && dims && idx && (val || val64) && out, "dt_sp_create: null argument");
     DT_CHECK_ARG(ndim >= 2 && ndim <= DT_MAX_NDIM, "dt_sp_create: bad ndim %d", ndim);
     DT_CHECK_ARG(nnz >= 1 && nnz < ((int64_t)1 << 31), "dt_sp_create: nnz out of range");
     dt_sp *sp = new dt_sp();
@@ -1069,8 +1034,17 @@ extern "C" int dt_sp_create(dt_ctx *ctx, int ndim, const size_t *dims, int64_t n
     }
     sp->val = (float *)sp_alloc(sp, (size_t)nnz * 4);
     if (!sp->val) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
-    if (hipMemcpyAsync(sp->val, val, (size_t)nnz * 4, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+    if (val64) {
+        // FP64 values kept for the FP64 path; the FP32 copy is their rounding
+        sp->val64 = (double *)sp_alloc(sp, (size_t)nnz * 8);
+        if (!sp->val64) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
+        if (hipMemcpyAsync(sp->val64, val64, (size_t)nnz * 8, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+            return fail(DT_EHIP, "dt_sp_create: copy failed");
+        if (dt_convert(ctx, sp->val, DT_F32, sp->val64, DT_F64, (size_t)nnz) != DT_OK)
+            return fail(DT_EHIP, "dt_sp_create: conversion failed");
+    } else if (hipMemcpyAsync(sp->val, val, (size_t)nnz * 4, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess) {
         return fail(DT_EHIP, "dt_sp_create: copy failed");
+    }
     for (int n = 0; n < ndim; n++) {
         dt_sp_mode &m = sp->mode[n];
         const int64_t I = sp->dims[n];
@@ -1089,11 +1063,8 @@ extern "C" int dt_sp_create(dt_ctx *ctx, int ndim, const size_t *dims, int64_t n
         int32_t *f = (int32_t *)dt_ws(ctx, WS_SPARSE1, (size_t)(I + 1) * 4 + 256);
         if (!f) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
         hipLaunchKernelGGL(nonempty_flags, sp_grid(I), dim3(256), 0, ctx->stream, m.rowptr, I, f);
-        size_t tb = 0;
-        if (hipcub::DeviceScan::ExclusiveSum(nullptr, tb, f, m.slot, (int)I, ctx->stream) != hipSuccess)
-            return fail(DT_EHIP, "dt_sp_create: scan failed");
-        void *tmp = cub_tmp(ctx, tb);
-        if (!tmp || hipcub::DeviceScan::ExclusiveSum(tmp, tb, f, m.slot, (int)I, ctx->stream) != hipSuccess)
+        int32_t *stmp = (int32_t *)sort_tmp(ctx, dt_scan_tmp_bytes(I));
+        if (!stmp || dt_exclusive_scan_i32(ctx, f, m.slot, I, stmp) != DT_OK)
             return fail(DT_EHIP, "dt_sp_create: scan failed");
         int32_t lastv[2];
         if (hipMemcpyAsync(&lastv[0], m.slot + I - 1, 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
@@ -1126,7 +1097,13 @@ extern "C" int dt_sp_create(dt_ctx *ctx, int ndim, const size_t *dims, int64_t n
         }
         if (hipGetLastError() != hipSuccess) return fail(DT_EHIP, "dt_sp_create: launch failed");
     }
-    // ||A||^2 in FP64
+    // ||A||^2 in FP64 (of the FP64 values when the tensor has them)
+    if (sp->val64) {
+        if (dt_sumsq_dev(ctx, sp->val64, DT_F64, (size_t)nnz, &sp->normA2) != DT_OK)
+            return fail(DT_EHIP, "dt_sp_create: norm failed");
+        *out = sp;
+        return DT_OK;
+    }
     double *part = (double *)dt_ws(ctx, WS_SUMSQ, 1024 * 8);
     if (!part) return fail(DT_ENOMEM, "dt_sp_create: out of memory");
     hipLaunchKernelGGL(sumsq_f32_sp, dim3(1024), dim3(256), 0, ctx->stream, sp->val, nnz, part);
@@ -1137,6 +1114,18 @@ extern "C" int dt_sp_create(dt_ctx *ctx, int ndim, const size_t *dims, int64_t n
     for (double v : hp) sp->normA2 += v;
     *out = sp;
     return DT_OK;
+}
+
+extern "C" int dt_sp_create(dt_ctx *ctx, int ndim, const size_t *dims, int64_t nnz, const int32_t *const *idx,
+                            const float *val, dt_sp **out) {
+    DT_CHECK_ARG(val, "dt_sp_create: null values");
+    return sp_create_impl(ctx, ndim, dims, nnz, idx, val, nullptr, out);
+}
+
+extern "C" int dt_sp_create_f64(dt_ctx *ctx, int ndim, const size_t *dims, int64_t nnz, const int32_t *const *idx,
+                                const double *val, dt_sp **out) {
+    DT_CHECK_ARG(val, "dt_sp_create_f64: null values");
+    return sp_create_impl(ctx, ndim, dims, nnz, idx, nullptr, val, out);
 }
 
 extern "C" int dt_sp_info(const dt_sp *sp, int *slice_mode, int64_t *nnz, double *norm) {
@@ -1232,167 +1221,293 @@ extern "C" int dt_sp_ttmc(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur,
 }
 
 // ---- sparse HOOI -----------------------------------------------------------------
+// A session holds the factors and the buffers of one decomposition; every
+// sweep runs Sp_TTMC_ALS_ (sp_hooi_als.tpp:48-65) for each mode in order and
+// forms the core from the last mode.  Two paths:
+//   fast    : 3-way, every rank <= 16, FP32 values -- the MFMA TTMc pipeline
+//             above, factors as FP32 rows padded to 16;
+//   generic : any other shape (dt_sparse_nway.hip), in FP64 for a tensor with
+//             FP64 values, else FP32; factor update W = Y_(n) L, then
+//             shifted Cholesky-QR3 (dt_orth_dev).
+// No step waits for the host inside a sweep; the sweep reads back the core's
+// norm, each mode's ||L|| and the breakdown flag once.
+struct dt_sp_hooi {
+    dt_ctx *ctx = nullptr;
+    dt_sp *sp = nullptr;
+    int N = 0;
+    int R[DT_MAX_NDIM] = {};
+    bool fast = false;
+    int T = DT_F32;               // generic path: element type
+    std::vector<void *> mine;
+    double *U64[DT_MAX_NDIM] = {};
+    void *Urow[DT_MAX_NDIM] = {};  // fast: FP32 I x 16 rows; generic: T rows I x R
+    void *Yc = nullptr;
+    double *L = nullptr, *G64 = nullptr;
+    double *part = nullptr, *Gs = nullptr, *Rinv = nullptr, *work = nullptr;
+    int *dflag = nullptr;
+    double *scal = nullptr;       // [0] = ||G||, [1 + n] = ||L|| of mode n
+    int64_t Klast = 0;            // core: Klast x R[N-1]
+    int sweeps = 0;
+    double total_ms = 0.0, res = 0.0;
+};
+
+constexpr int kSpMaxWg = 2048;  // fast factor update: workgroups (Gram partials)
+
+static void *sh_alloc(dt_sp_hooi *h, size_t bytes) {
+    void *p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    h->mine.push_back(p);
+    return p;
+}
+
+// fast path: U_n <- orth(Y_(n) L): sp_tl_gram, then three Cholesky-QR passes
+// (the first shifted) whose Grams come from the previous pass's partials; the
+// last pass also writes the FP32 view
+static int sh_fast_update(dt_sp_hooi *h, int n) {
+    dt_ctx *ctx = h->ctx;
+    dt_sp *sp = h->sp;
+    const int64_t I = sp->dims[n];
+    const int Rn = h->R[n];
+    const int64_t ntiles = (I + 15) / 16;
+    const int64_t tpw = std::max<int64_t>(1, (ntiles + 4 * kSpMaxWg - 1) / (4 * kSpMaxWg));
+    const int ng = (int)(((ntiles + tpw - 1) / tpw + 3) / 4);
+    const dt_sp_mode &md = sp->mode[n];
+    DT_LAUNCH("sp_tl_gram", sp_tl_gram, dim3((unsigned)ng), dim3(256), 0, (const float *)h->Yc, md.rowptr, md.slot, I,
+              Rn, (const double *)h->L, tpw, h->U64[n], h->part);
+    auto rmul_gram = sp_rmul<true, false>;
+    auto rmul_view = sp_rmul<false, true>;
+    for (int pass = 0; pass < 3; pass++) {
+        DT_TRY(dt_part_reduce(ctx, h->part, ng, Rn * Rn, h->Gs));
+        DT_TRY(dt_chol_rinv_dev(ctx, h->Gs, Rn, pass == 0, (double)I, h->Rinv, h->dflag));
+        if (pass < 2) {
+            DT_LAUNCH("sp_rmul", rmul_gram, dim3((unsigned)ng), dim3(256), 0, h->U64[n], I, Rn, (const double *)h->Rinv,
+                      tpw, h->part, (float *)nullptr);
+        } else {
+            DT_LAUNCH("sp_rmul", rmul_view, dim3((unsigned)ng), dim3(256), 0, h->U64[n], I, Rn, (const double *)h->Rinv,
+                      tpw, (double *)nullptr, (float *)h->Urow[n]);
+        }
+    }
+    return DT_OK;
+}
+
+static int sh_views(dt_sp_hooi *h, int m) {
+    dt_ctx *ctx = h->ctx;
+    if (h->fast) {
+        DT_LAUNCH("sp_views", rows16_view, sp_grid(h->sp->dims[m]), dim3(256), 0, h->U64[m], h->sp->dims[m], h->R[m],
+                  (float *)h->Urow[m]);
+        return DT_OK;
+    }
+    return dt_spg_rows_view(ctx, h->U64[m], h->sp->dims[m], h->R[m], h->Urow[m], h->T);
+}
+
+extern "C" int dt_sp_hooi_destroy(dt_sp_hooi *h) {
+    if (!h) return DT_OK;
+    (void)hipStreamSynchronize(h->ctx->stream);
+    for (void *p : h->mine) (void)hipFree(p);
+    delete h;
+    return DT_OK;
+}
+
+extern "C" int dt_sp_hooi_create(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, uint64_t seed,
+                                 const double *const *U_init, dt_sp_hooi **out) {
+    DT_CHECK_ARG(ctx && sp && ranks && out, "dt_sp_hooi_create: null argument");
+    *out = nullptr;
+    const int N = sp->ndim;
+    bool small = N == 3 && sp->val64 == nullptr;
+    int64_t K[DT_MAX_NDIM];
+    for (int m = 0; m < N; m++) {
+        DT_CHECK_ARG(ranks[m] >= 1 && (int64_t)ranks[m] <= sp->dims[m] && ranks[m] <= (size_t)DT_SPG_RMAX,
+                     "dt_sp_hooi_create: rank %zu of mode %d not in [1, min(%d, I = %lld)]", ranks[m], m, DT_SPG_RMAX,
+                     (long long)sp->dims[m]);
+        small = small && ranks[m] <= 16;
+    }
+    int64_t maxK = 1;
+    for (int n = 0; n < N; n++) {
+        K[n] = 1;
+        for (int m = 0; m < N; m++)
+            if (m != n) K[n] *= (int64_t)ranks[m];
+        maxK = std::max(maxK, K[n]);
+    }
+    for (int n = 0; n < N; n++)
+        DT_CHECK_ARG((int64_t)ranks[n] <= K[n],
+                     "dt_sp_hooi_create: rank %zu of mode %d above the product %lld of the other ranks (Y_(n) has "
+                     "that many columns, so its leading subspace has at most that dimension)",
+                     ranks[n], n, (long long)K[n]);
+    DT_CHECK_ARG(small || maxK <= DT_SPG_KMAX,
+                 "dt_sp_hooi_create: product of the other modes' ranks %lld above %d", (long long)maxK, DT_SPG_KMAX);
+    dt_sp_hooi *h = new dt_sp_hooi();
+    h->ctx = ctx;
+    h->sp = sp;
+    h->N = N;
+    h->fast = small;
+    h->T = sp->val64 ? DT_F64 : DT_F32;
+    for (int m = 0; m < N; m++) h->R[m] = (int)ranks[m];
+    h->Klast = h->fast ? SP_E : K[N - 1];
+    const size_t tsz = h->T == DT_F64 ? 8 : 4;
+    int64_t maxrows = 0, maxI = 0, maxR = 0;
+    for (int m = 0; m < N; m++) {
+        maxrows = std::max(maxrows, sp->mode[m].nrows);
+        maxI = std::max(maxI, sp->dims[m]);
+        maxR = std::max<int64_t>(maxR, h->R[m]);
+    }
+    bool ok = true;
+    for (int m = 0; m < N && ok; m++) {
+        h->U64[m] = (double *)sh_alloc(h, (size_t)sp->dims[m] * h->R[m] * 8);
+        h->Urow[m] = sh_alloc(h, h->fast ? (size_t)sp->dims[m] * 16 * 4 : (size_t)sp->dims[m] * h->R[m] * tsz);
+        ok = h->U64[m] && h->Urow[m];
+    }
+    const int64_t ycw = h->fast ? SP_E : maxK;
+    if (ok) ok = (h->Yc = sh_alloc(h, (size_t)std::max<int64_t>(maxrows, 1) * ycw * (h->fast ? 4 : tsz))) != nullptr;
+    const size_t lsz = h->fast ? (size_t)SP_E * 16 : (size_t)maxK * maxR;
+    if (ok) ok = (h->L = (double *)sh_alloc(h, lsz * 8)) != nullptr;
+    if (ok) ok = (h->G64 = (double *)sh_alloc(h, lsz * 8)) != nullptr;
+    if (ok && h->fast) ok = (h->part = (double *)sh_alloc(h, (size_t)kSpMaxWg * 256 * 8)) != nullptr;
+    if (ok) ok = (h->Gs = (double *)sh_alloc(h, 256 * 8)) != nullptr;
+    if (ok) ok = (h->Rinv = (double *)sh_alloc(h, 256 * 8)) != nullptr;
+    if (ok && !h->fast) ok = (h->work = (double *)sh_alloc(h, (size_t)(2 * maxR * maxR + maxI * maxR + 64) * 8)) != nullptr;
+    if (ok) ok = (h->dflag = (int *)sh_alloc(h, 256)) != nullptr;
+    if (ok) ok = (h->scal = (double *)sh_alloc(h, (DT_MAX_NDIM + 1) * 8)) != nullptr;
+    if (!ok) {
+        dt_sp_hooi_destroy(h);
+        return dt_fail(ctx, DT_ENOMEM, "dt_sp_hooi_create: out of memory");
+    }
+    int st = dt_zero(ctx, h->dflag, sizeof(int));
+    if (st == DT_OK) st = dt_zero(ctx, h->scal, (DT_MAX_NDIM + 1) * 8);
+    if (h->fast && st == DT_OK)
+        for (int m = 0; m < N && st == DT_OK; m++) st = dt_zero(ctx, h->Urow[m], (size_t)sp->dims[m] * 16 * 4);
+    // start: the caller's factors, or random orthonormal (Philox stream 200 + m);
+    // QR with positive R diagonal, the basis orth2's Gram-Schmidt gives
+    for (int m = 0; m < N && st == DT_OK; m++) {
+        const size_t sz = (size_t)sp->dims[m] * h->R[m];
+        if (U_init && U_init[m]) {
+            if (hipMemcpyAsync(h->U64[m], U_init[m], sz * 8, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+                st = dt_fail(ctx, DT_EHIP, "dt_sp_hooi_create: copy failed");
+        } else {
+            st = dt_fill_normal_f64(ctx, h->U64[m], sz, seed, 200 + (uint64_t)m);
+        }
+        if (st == DT_OK) st = dt_qr_f64(ctx, h->U64[m], nullptr, h->U64[m], (size_t)sp->dims[m], (size_t)h->R[m]);
+        if (st == DT_OK) st = sh_views(h, m);
+    }
+    if (st != DT_OK) {
+        dt_sp_hooi_destroy(h);
+        return st;
+    }
+    *out = h;
+    return DT_OK;
+}
+
+extern "C" int dt_sp_hooi_sweep(dt_sp_hooi *h, double *L_norms, double *residual) {
+    if (!h) return DT_EINVAL;
+    dt_ctx *ctx = h->ctx;
+    dt_sp *sp = h->sp;
+    const int N = h->N;
+    hipEvent_t e0 = dt_get_event(ctx), e1 = dt_get_event(ctx);
+    (void)hipEventRecord(e0, ctx->stream);
+    int st = DT_OK;
+    for (int n = 0; n < N && st == DT_OK; n++) {
+        if (h->fast) {
+            const float *Ur[3] = {(const float *)h->Urow[0], (const float *)h->Urow[1], (const float *)h->Urow[2]};
+            st = sp_ttmc_dev(ctx, sp, n, Ur, (float *)h->Yc, h->L);  // Y_c and L = Y_cᵀ U_n
+            if (st == DT_OK) st = dt_spg_norm(ctx, h->L, (int64_t)SP_E * 16, h->scal + 1 + n);
+            if (st == DT_OK) st = sh_fast_update(h, n);              // U_n <- orth(Y_(n) L)
+        } else {
+            int64_t K = 1;
+            for (int m = 0; m < N; m++)
+                if (m != n) K *= h->R[m];
+            st = dt_spg_ttmc(ctx, sp, n, h->Urow, h->T, h->R, h->Yc);
+            if (st == DT_OK) st = dt_spg_ytu(ctx, sp, n, h->Urow[n], h->T, h->R, h->Yc, h->L);
+            if (st == DT_OK) st = dt_spg_norm(ctx, h->L, K * h->R[n], h->scal + 1 + n);
+            if (st == DT_OK) st = dt_spg_yl(ctx, sp, n, h->T, h->R, h->Yc, h->L, h->U64[n]);
+            if (st == DT_OK) st = dt_orth_dev(ctx, h->U64[n], (size_t)sp->dims[n], (size_t)h->R[n], h->work, h->dflag);
+            if (st == DT_OK) st = sh_views(h, n);
+        }
+    }
+    // core = Y_(N-1)ᵀ U_{N-1} with the updated U_{N-1}: Y_(N-1) does not depend
+    // on U_{N-1}, so the last mode's Y_c is reused
+    if (st == DT_OK) {
+        if (h->fast)
+            st = sp_ytu_dev(ctx, sp, N - 1, (const float *)h->Urow[N - 1], (const float *)h->Yc, h->G64);
+        else
+            st = dt_spg_ytu(ctx, sp, N - 1, h->Urow[N - 1], h->T, h->R, h->Yc, h->G64);
+    }
+    if (st == DT_OK) st = dt_spg_norm(ctx, h->G64, h->Klast * (h->fast ? 16 : h->R[N - 1]), h->scal);
+    (void)hipEventRecord(e1, ctx->stream);
+    double sc[DT_MAX_NDIM + 1];
+    if (st == DT_OK &&
+        (hipMemcpyAsync(sc, h->scal, sizeof(sc), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+         hipMemcpyAsync(ctx->pinned, h->dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+         hipStreamSynchronize(ctx->stream) != hipSuccess))
+        st = dt_fail(ctx, DT_EHIP, "dt_sp_hooi_sweep: readback failed");
+    float ms = 0.f;
+    if (st == DT_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+    ctx->event_pool.push_back(e0);
+    ctx->event_pool.push_back(e1);
+    if (st != DT_OK) return st;
+    if (*(const int *)ctx->pinned != 0)
+        return dt_fail(ctx, DT_ENUMERIC,
+                       "dt_sp_hooi_sweep: Cholesky breakdown in the factor update of sweep %d (Y_(n) L rank-deficient)",
+                       h->sweeps);
+    h->total_ms += ms;
+    h->sweeps++;
+    const double g2 = sc[0] * sc[0];
+    h->res = std::sqrt(std::max(0.0, 1.0 - g2 / sp->normA2));
+    if (L_norms)
+        for (int n = 0; n < N; n++) L_norms[n] = sc[1 + n];
+    if (residual) *residual = h->res;
+    return DT_OK;
+}
+
+extern "C" int dt_sp_hooi_get(dt_sp_hooi *h, double *const *U_out, double *G_out, double *sweeps_ms) {
+    if (!h) return DT_EINVAL;
+    dt_ctx *ctx = h->ctx;
+    dt_sp *sp = h->sp;
+    const int N = h->N;
+    for (int m = 0; m < N; m++)
+        if (U_out && U_out[m])
+            DT_HIP(hipMemcpyAsync(U_out[m], h->U64[m], (size_t)sp->dims[m] * h->R[m] * 8, hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+    if (G_out) {
+        DT_CHECK_ARG(h->sweeps > 0, "dt_sp_hooi_get: the core needs a sweep");
+        if (h->fast) {
+            // G (R0 x R1 x R2, column-major): G[r0 + R0 r1 + R0 R1 r2] = L[(r0 + 16 r1) * 16 + r2]
+            std::vector<double> g(SP_E * 16), gc((size_t)h->R[0] * h->R[1] * h->R[2]);
+            DT_HIP(hipMemcpyAsync(g.data(), h->G64, g.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+            DT_HIP(hipStreamSynchronize(ctx->stream));
+            for (int r2 = 0; r2 < h->R[2]; r2++)
+                for (int r1 = 0; r1 < h->R[1]; r1++)
+                    for (int r0 = 0; r0 < h->R[0]; r0++)
+                        gc[(size_t)(r0 + h->R[0] * (r1 + h->R[1] * r2))] = g[(size_t)((r0 + 16 * r1) * 16 + r2)];
+            DT_HIP(hipMemcpyAsync(G_out, gc.data(), gc.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        } else {
+            // L of the last mode (K x R_{N-1}, column-major, K in the other modes'
+            // order with the first fastest) is the column-major core
+            DT_HIP(hipMemcpyAsync(G_out, h->G64, (size_t)h->Klast * h->R[N - 1] * 8, hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+        }
+    }
+    if (sweeps_ms) *sweeps_ms = h->total_ms;
+    DT_HIP(hipStreamSynchronize(ctx->stream));
+    return DT_OK;
+}
 
 extern "C" int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int max_iter, uint64_t seed,
                               const double *const *U_init, double *const *U_out, double *G_out,
                               double *residual_per_sweep, double *residual, double *sweeps_ms) {
     DT_CHECK_ARG(ctx && sp && ranks && residual, "dt_hooi_sparse: null argument");
-    DT_CHECK_ARG(sp->ndim == 3, "dt_hooi_sparse: 3-way tensors only");
-    int Rk[3];
-    for (int m = 0; m < 3; m++) {
-        DT_CHECK_ARG(ranks[m] >= 1 && ranks[m] <= 16 && (int64_t)ranks[m] <= sp->dims[m],
-                     "dt_hooi_sparse: rank %zu of mode %d not in [1, min(16, I)]", ranks[m], m);
-        Rk[m] = (int)ranks[m];
-    }
     DT_CHECK_ARG(max_iter >= 1, "dt_hooi_sparse: max_iter >= 1");
-    const int R = 16;  // padded width of the device views
-    // factors: FP64 I x R column-major (orthonormalised), FP32 row-major views
-    std::vector<void *> mine;
-    auto alloc = [&](size_t bytes) -> void * {
-        void *p = nullptr;
-        if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) return nullptr;
-        mine.push_back(p);
-        return p;
-    };
-    auto cleanup = [&] {
-        (void)hipStreamSynchronize(ctx->stream);
-        for (void *p : mine) (void)hipFree(p);
-    };
-    double *U64[3];
-    float *U32r[3];
-    int64_t maxrows = 0;
-    for (int m = 0; m < 3; m++) {
-        maxrows = std::max(maxrows, sp->mode[m].nrows);
-    }
-    for (int m = 0; m < 3; m++) {
-        U64[m] = (double *)alloc((size_t)sp->dims[m] * Rk[m] * 8);
-        U32r[m] = (float *)alloc((size_t)sp->dims[m] * R * 4);
-        if (!U64[m] || !U32r[m]) {
-            cleanup();
-            return dt_fail(ctx, DT_ENOMEM, "dt_hooi_sparse: out of memory");
-        }
-    }
-    float *Yc = (float *)alloc((size_t)std::max<int64_t>(maxrows, 1) * SP_E * 4);
-    double *L = (double *)alloc(SP_E * 16 * 8);
-    double *G64 = (double *)alloc(SP_E * 16 * 8);
-    constexpr int kMaxWg = 2048;  // factor-update workgroups (Gram partials)
-    double *part = (double *)alloc((size_t)kMaxWg * 256 * 8);
-    double *Gs = (double *)alloc(256 * 8);
-    double *Rinv = (double *)alloc(256 * 8);
-    int *dflag = (int *)alloc(256);
-    if (!Yc || !L || !G64 || !part || !Gs || !Rinv || !dflag) {
-        cleanup();
-        return dt_fail(ctx, DT_ENOMEM, "dt_hooi_sparse: out of memory");
-    }
-    int st = hipMemsetAsync(dflag, 0, sizeof(int), ctx->stream) == hipSuccess ? DT_OK : DT_EHIP;
-    // U_n <- orth(Y_(n) L): sp_tl_gram, then three Cholesky-QR passes (the first
-    // shifted) whose Grams come from the previous pass's partials; the last
-    // pass also writes the FP32 view
-    auto update = [&](int n) -> int {
-        const int64_t I = sp->dims[n];
-        const int Rn = Rk[n];
-        const int64_t ntiles = (I + 15) / 16;
-        const int64_t tpw = std::max<int64_t>(1, (ntiles + 4 * kMaxWg - 1) / (4 * kMaxWg));
-        const int ng = (int)(((ntiles + tpw - 1) / tpw + 3) / 4);
-        const dt_sp_mode &md = sp->mode[n];
-        DT_LAUNCH("sp_tl_gram", sp_tl_gram, dim3((unsigned)ng), dim3(256), 0, (const float *)Yc, md.rowptr, md.slot,
-                  I, Rn, (const double *)L, tpw, U64[n], part);
-        auto rmul_gram = sp_rmul<true, false>;
-        auto rmul_view = sp_rmul<false, true>;
-        for (int pass = 0; pass < 3; pass++) {
-            DT_TRY(dt_part_reduce(ctx, part, ng, Rn * Rn, Gs));
-            DT_TRY(dt_chol_rinv_dev(ctx, Gs, Rn, pass == 0, (double)I, Rinv, dflag));
-            if (pass < 2) {
-                DT_LAUNCH("sp_rmul", rmul_gram, dim3((unsigned)ng), dim3(256), 0, U64[n], I, Rn,
-                          (const double *)Rinv, tpw, part, (float *)nullptr);
-            } else {
-                DT_LAUNCH("sp_rmul", rmul_view, dim3((unsigned)ng), dim3(256), 0, U64[n], I, Rn,
-                          (const double *)Rinv, tpw, (double *)nullptr, U32r[n]);
-            }
-        }
-        return DT_OK;
-    };
-    auto views = [&](int m) {
-        dt_ktimer kt(ctx, "sp_views");
-        hipLaunchKernelGGL(rows16_view, sp_grid(sp->dims[m]), dim3(256), 0, ctx->stream, U64[m], sp->dims[m],
-                           Rk[m], U32r[m]);
-        return hipGetLastError() == hipSuccess ? DT_OK : DT_EHIP;
-    };
-    // start: the caller's factors, or random orthonormal (Philox stream 200 + m);
-    // QR with positive R diagonal, the basis orth2's Gram-Schmidt gives
-    for (int m = 0; m < 3 && st == DT_OK; m++) {
-        const size_t sz = (size_t)sp->dims[m] * Rk[m];
-        if (U_init && U_init[m]) {
-            if (hipMemcpyAsync(U64[m], U_init[m], sz * 8, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
-                st = DT_EHIP;
-        } else {
-            st = dt_fill_normal_f64(ctx, U64[m], sz, seed, 200 + (uint64_t)m);
-        }
-        if (st == DT_OK) st = dt_qr_f64(ctx, U64[m], nullptr, U64[m], (size_t)sp->dims[m], (size_t)Rk[m]);
-        if (st == DT_OK) st = views(m);
-    }
-    hipEvent_t e0 = dt_get_event(ctx), e1 = dt_get_event(ctx);
-    double total_ms = 0.0, res = 0.0;
+    dt_sp_hooi *h = nullptr;
+    DT_TRY(dt_sp_hooi_create(ctx, sp, ranks, seed, U_init, &h));
+    int st = DT_OK;
     for (int it = 0; it < max_iter && st == DT_OK; it++) {
-        (void)hipEventRecord(e0, ctx->stream);
-        for (int n = 0; n < 3 && st == DT_OK; n++) {
-            const float *Ur[3] = {U32r[0], U32r[1], U32r[2]};
-            st = sp_ttmc_dev(ctx, sp, n, Ur, Yc, L);  // Y_c and L = Y_cᵀ U_n
-            if (st == DT_OK) st = update(n);           // U_n <- orth(Y_(n) L)
-        }
-        // core = Y_(2)ᵀ U_2 with the updated U_2: Y_(2) does not depend on U_2, so
-        // the last mode's Y_c is reused
-        if (st == DT_OK) st = sp_ytu_dev(ctx, sp, 2, U32r[2], Yc, G64);
-        (void)hipEventRecord(e1, ctx->stream);
-        if (st != DT_OK) break;
-        std::vector<double> g(SP_E * 16);
-        if (hipMemcpyAsync(g.data(), G64, g.size() * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-            hipMemcpyAsync(ctx->pinned, dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-            hipStreamSynchronize(ctx->stream) != hipSuccess) {
-            st = DT_EHIP;
-            break;
-        }
-        if (*(const int *)ctx->pinned != 0) {
-            cleanup();
-            return dt_fail(ctx, DT_ENUMERIC,
-                           "dt_hooi_sparse: Cholesky breakdown in the factor update of sweep %d (Y_(n) L rank-deficient)",
-                           it);
-        }
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        total_ms += ms;
-        double g2 = 0.0;
-        for (double v : g) g2 += v * v;
-        res = std::sqrt(std::max(0.0, 1.0 - g2 / sp->normA2));
-        if (residual_per_sweep) residual_per_sweep[it] = res;
+        double r = 0.0;
+        st = dt_sp_hooi_sweep(h, nullptr, &r);
+        if (st == DT_OK && residual_per_sweep) residual_per_sweep[it] = r;
+        if (st == DT_OK) *residual = r;
     }
-    ctx->event_pool.push_back(e0);
-    ctx->event_pool.push_back(e1);
-    if (st == DT_OK) {
-        *residual = res;
-        if (sweeps_ms) *sweeps_ms = total_ms;
-        for (int m = 0; m < 3; m++)
-            if (U_out && U_out[m])
-                if (hipMemcpyAsync(U_out[m], U64[m], (size_t)sp->dims[m] * Rk[m] * 8, hipMemcpyDeviceToDevice,
-                                   ctx->stream) != hipSuccess)
-                    st = DT_EHIP;
-        // G (R0 x R1 x R2, column-major): G[r0 + R0 r1 + R0 R1 r2] = L[(r0 + 16 r1) * 16 + r2]
-        if (G_out && st == DT_OK) {
-            std::vector<double> g(SP_E * 16), gc((size_t)Rk[0] * Rk[1] * Rk[2]);
-            if (hipMemcpyAsync(g.data(), G64, g.size() * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-                hipStreamSynchronize(ctx->stream) != hipSuccess)
-                st = DT_EHIP;
-            for (int r2 = 0; r2 < Rk[2]; r2++)
-                for (int r1 = 0; r1 < Rk[1]; r1++)
-                    for (int r0 = 0; r0 < Rk[0]; r0++)
-                        gc[(size_t)(r0 + Rk[0] * (r1 + Rk[1] * r2))] = g[(size_t)((r0 + 16 * r1) * 16 + r2)];
-            if (st == DT_OK &&
-                hipMemcpyAsync(G_out, gc.data(), gc.size() * 8, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
-                st = DT_EHIP;
-            if (st == DT_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) st = DT_EHIP;
-        }
-    }
-    cleanup();
-    if (st != DT_OK && ctx->err.empty()) return dt_fail(ctx, st, "dt_hooi_sparse: failed");
+    if (st == DT_OK) st = dt_sp_hooi_get(h, U_out, G_out, sweeps_ms);
+    dt_sp_hooi_destroy(h);
     return st;
 }

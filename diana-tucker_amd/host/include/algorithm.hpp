@@ -91,10 +91,10 @@ inline SweepWork hooi_sweep_work(const shape_t &I, const shape_t &R, int schedul
     return w;
 }
 
-// One sparse HOOI sweep (3-way, mode-sorted copies): per mode the TTMc reads
-// every nonzero (three indices and a value), writes Y_(n) and reads the two
-// other factors once, nnz (2 prod_{m != n} R_m + R_n) flop (the Kronecker
-// rows and the Y_(n)ᵀU_n product).
+// One sparse HOOI sweep (mode-sorted copies): per mode the TTMc reads every
+// nonzero (N indices and a value), writes Y_(n) and reads the other factors
+// once, nnz (2 prod_{m != n} R_m + R_n) flop (the Kronecker rows and the
+// Y_(n)ᵀU_n product).
 inline SweepWork sp_sweep_work(const shape_t &I, const shape_t &R, long long nnz) {
     SweepWork w;
     for (size_t n = 0; n < I.size(); n++) {
@@ -217,8 +217,12 @@ namespace SpTucker {
 // GPU: the reference's start (per mode, Tensor {R_n, I_n}.randn() after the
 // caller's srand, orthonormalised; the transposed factor U_n = Ut_nᵀ), then
 // per sweep and mode U_n <- orth(Y_(n) Y_(n)ᵀ U_n) and the core from the
-// last mode, with the reference's output lines.  Returns (G, Ut) with Ut[n]
-// stored as {R_n, I_n} like the reference.  Ranks up to 16, 3-way tensors.
+// last mode, with the reference's output lines: "||A||_F", per sweep
+// "Calculating iteration k ..." and each mode's ||L||_F (printed as ||G||_F
+// by Sp_TTMC_ALS_, :51-53), then the core's norm and the residual.  Returns
+// (G, Ut) with Ut[n] stored as {R_n, I_n} like the reference.  Any number of
+// modes; ranks <= 64 with prod_{m != n} R_m <= 4096 (3-way FP32 tensors with
+// ranks <= 16 take the MFMA pipeline).
 template <typename Ty>
 std::tuple<Tensor<Ty>, std::vector<Tensor<Ty>>> Sp_HOOI_ALS(const SpTensor<Ty> &A, const shape_t &R,
                                                             size_t max_iter) {
@@ -227,21 +231,14 @@ std::tuple<Tensor<Ty>, std::vector<Tensor<Ty>>> Sp_HOOI_ALS(const SpTensor<Ty> &
     SummaryScope region("SpTucker::Sp_HOOI_ALS()");
     const shape_t &I = A.shape();
     dt_ctx *ctx = diana::Device::ctx();
-    double a2 = 0.0;
-    int64_t nnz = 0;
-    {
-        int sm = 0;
-        double nrm = 0.0;
-        dt_sp_info(A.device(), &sm, &nnz, &nrm);
-        a2 = nrm * nrm;
-    }
-    output("||A||_F = " + std::to_string(std::sqrt(a2)));
-    auto dglobal = new DistributionGlobal();
+    const double a_norm = Function::fnorm(A);
+    output("||A||_F = " + std::to_string(a_norm));
+    static DistributionGlobal global;
     std::vector<diana::DevBuf *> dU0, dU;
     std::vector<const double *> u0p;
     std::vector<double *> up;
     for (size_t n = 0; n < N; n++) {
-        Tensor<double> Ut(dglobal, {R[n], I[n]}, false);
+        Tensor<double> Ut(&global, {R[n], I[n]}, false);
         Ut.randn();
         std::vector<double> U(I[n] * R[n]);
         for (size_t i = 0; i < I[n]; i++)
@@ -254,39 +251,54 @@ std::tuple<Tensor<Ty>, std::vector<Tensor<Ty>>> Sp_HOOI_ALS(const SpTensor<Ty> &
     size_t csz = 1;
     for (auto r : R) csz *= r;
     diana::DevBuf dG(csz * 8);
-    std::vector<double> per(max_iter ? max_iter : 1);
-    double res = 0.0, ms = 0.0;
-    for (size_t it = 0; it < max_iter; it++) output("Calculating iteration " + std::to_string(it + 1) + " ...");
+    dt_sp_hooi *h = nullptr;
+    diana::check(dt_sp_hooi_create(ctx, A.device(), R.data(), 0, u0p.data(), &h), "Sp_HOOI_ALS");
+    int64_t nnz = 0;
     {
-        const diana::SweepWork sw = diana::sp_sweep_work(I, R, (long long)nnz);
-        SummaryScope sc("SpTucker::Sp_HOOI_ALS::sweeps()", sw.flop * (long long)max_iter,
-                        sw.bytes * (long long)max_iter);
-        diana::check(dt_hooi_sparse(ctx, A.device(), R.data(), (int)max_iter, 0, u0p.data(), up.data(),
-                                    (double *)dG.p, per.data(), &res, &ms),
-                     "Sp_HOOI_ALS");
+        int sm = 0;
+        double nrm = 0.0;
+        dt_sp_info(A.device(), &sm, &nnz, &nrm);
     }
+    const diana::SweepWork sw = diana::sp_sweep_work(I, R, (long long)nnz);
+    std::vector<double> ln(N);
+    int st = DT_OK;
+    for (size_t it = 0; it < max_iter && st == DT_OK; it++) {
+        output("Calculating iteration " + std::to_string(it + 1) + " ...");
+        double res = 0.0;
+        {
+            SummaryScope sc("SpTucker::Sp_HOOI_ALS::sweep()", sw.flop, sw.bytes);
+            st = dt_sp_hooi_sweep(h, ln.data(), &res);
+        }
+        if (st == DT_OK)
+            for (size_t n = 0; n < N; n++) output("||G||_F = " + std::to_string(ln[n]));
+    }
+    if (st == DT_OK) st = dt_sp_hooi_get(h, up.data(), max_iter > 0 ? (double *)dG.p : nullptr, nullptr);
+    const std::string err = st == DT_OK ? "" : dt_last_error(ctx);
+    dt_sp_hooi_destroy(h);
+    if (st != DT_OK) error("Sp_HOOI_ALS: " + err + " (status " + std::to_string(st) + ")");
     std::vector<Tensor<Ty>> Ut;
     std::vector<double> tmp;
     for (size_t n = 0; n < N; n++) {
         tmp.resize(I[n] * R[n]);
         dU[n]->to_host(tmp.data(), tmp.size() * 8);
-        Tensor<Ty> ut(dglobal, {R[n], I[n]}, false);
+        Tensor<Ty> ut(&global, {R[n], I[n]}, false);
         for (size_t i = 0; i < I[n]; i++)
             for (size_t r = 0; r < R[n]; r++) ut[r + R[n] * i] = (Ty)tmp[i + I[n] * r];
         Ut.push_back(ut);
         delete dU0[n];
         delete dU[n];
     }
-    Tensor<Ty> G(dglobal, R, false);
-    tmp.resize(csz);
-    dG.to_host(tmp.data(), csz * 8);
+    Tensor<Ty> G(&global, R, false);
+    tmp.assign(csz, 0.0);
+    if (max_iter > 0) dG.to_host(tmp.data(), csz * 8);
     double g2 = 0.0;
     for (size_t i = 0; i < csz; i++) {
         G[i] = (Ty)tmp[i];
         g2 += tmp[i] * tmp[i];
     }
     output("||G||_F = " + std::to_string(std::sqrt(g2)));
-    output("Residual: sqrt(1 - ||G||_F^2 / ||A||_F^2) = " + std::to_string(std::sqrt(std::max(0.0, 1 - g2 / a2))));
+    output("Residual: sqrt(1 - ||G||_F^2 / ||A||_F^2) = " +
+           std::to_string(std::sqrt(std::max(0.0, 1 - g2 / (a_norm * a_norm)))));
     return std::make_tuple(G, Ut);
 }
 }  // namespace SpTucker

@@ -5,6 +5,7 @@
 #define DIANA_AMD_FUNCTION_HPP
 #include <tuple>
 #include <vector>
+#include "sp_tensor.hpp"
 #include "summary.hpp"
 #include "tensor.hpp"
 
@@ -152,6 +153,87 @@ template <typename Ty>
 double fnorm(const Tensor<Ty> &A) { return A.fnorm(); }
 template <typename Ty>
 Ty sum(const Tensor<Ty> &A) { return A.sum(); }
+
+// ---- sparse (include/function.hpp:70-90, template/function/sp_tensor.tpp) ----
+namespace detail {
+// device copies of the transposed factors Ut[m] (R_m x I_m column-major FP64),
+// mode n's taken from `Rn` (the reference passes it separately)
+template <typename Ty>
+struct SpFactors {
+    std::vector<diana::DevBuf *> buf;
+    std::vector<const double *> ptr;
+    std::vector<size_t> ranks;
+    SpFactors(const SpTensor<Ty> &A, const std::vector<Tensor<Ty>> &M, size_t n, const Tensor<Ty> &Rn) {
+        if (M.size() != A.ndim()) error("sparse TTMc: one factor per mode");
+        for (size_t m = 0; m < A.ndim(); m++) {
+            const Tensor<Ty> &U = m == n ? Rn : M[m];
+            if (!U.is_matrix() || U.shape()[1] != A.shape()[m]) error("sparse TTMc: factor " + std::to_string(m) + " must be R x I");
+            std::vector<double> h(U.data(), U.data() + U.size());
+            buf.push_back(new diana::DevBuf(h.data(), h.size() * 8));
+            ptr.push_back((const double *)buf.back()->p);
+            ranks.push_back(U.shape()[0]);
+        }
+    }
+    ~SpFactors() {
+        for (auto *b : buf) delete b;
+    }
+    size_t K(size_t n) const {
+        size_t k = 1;
+        for (size_t m = 0; m < ranks.size(); m++)
+            if (m != n) k *= ranks[m];
+        return k;
+    }
+};
+}  // namespace detail
+
+// sp_tensor.tpp:236-286 as intended: L = Y_(n)ᵀ U_n (prod_{m != n} R_m x R_n),
+// Y_(n) the slices' Kronecker rows of the other modes' factors (first other
+// mode fastest), U_n = Rᵀ.  M[m]: Ut[m], R_m x I_m.
+template <typename Ty>
+Tensor<Ty> ttmc_mTR(const SpTensor<Ty> &A, const std::vector<Tensor<Ty>> &M, size_t n, const Tensor<Ty> &R) {
+    detail::SpFactors<Ty> f(A, M, n, R);
+    const size_t K = f.K(n), Rn = f.ranks[n];
+    SummaryScope sc("Function::ttmc_mTR()", (long long)A.nnz() * (long long)(2 * K + Rn),
+                    (long long)A.nnz() * (long long)(4 * A.ndim() + (A.fp64() ? 8 : 4)));
+    diana::DevBuf dL(K * Rn * 8);
+    diana::check(dt_sp_ttmc_mtr(diana::Device::ctx(), A.device(), (int)n, f.ptr.data(), f.ranks.data(),
+                                (double *)dL.p), "ttmc_mTR");
+    std::vector<double> h(K * Rn);
+    dL.to_host(h.data(), h.size() * 8);
+    static DistributionGlobal global;
+    Tensor<Ty> ret(&global, {K, Rn}, false);
+    for (size_t i = 0; i < h.size(); i++) ret[i] = (Ty)h[i];
+    return ret;
+}
+
+// sp_tensor.tpp:288-365 as intended: ret (R_n x I_n) = (Y_(n) L)ᵀ; slices
+// without nonzeros give zero columns.
+template <typename Ty>
+void ttmc_mTL(const SpTensor<Ty> &A, const std::vector<Tensor<Ty>> &M, size_t n, const Tensor<Ty> &L, Tensor<Ty> &ret) {
+    detail::SpFactors<Ty> f(A, M, n, M[n]);
+    const size_t K = f.K(n), Rn = L.shape()[1];
+    if (!L.is_matrix() || L.shape()[0] != K || Rn != f.ranks[n]) error("ttmc_mTL: L must be prod R_{-n} x R_n");
+    if (ret.size() != Rn * A.shape()[n]) error("ttmc_mTL: ret must be R_n x I_n");
+    SummaryScope sc("Function::ttmc_mTL()", (long long)A.nnz() * (long long)(2 * K) + 2LL * (long long)(A.shape()[n] * K * Rn),
+                    (long long)A.nnz() * (long long)(4 * A.ndim() + (A.fp64() ? 8 : 4)));
+    std::vector<double> hl(L.data(), L.data() + L.size());
+    diana::DevBuf dL(hl.data(), hl.size() * 8), dR(ret.size() * 8);
+    diana::check(dt_sp_ttmc_mtl(diana::Device::ctx(), A.device(), (int)n, f.ptr.data(), f.ranks.data(),
+                                (const double *)dL.p, (double *)dR.p), "ttmc_mTL");
+    std::vector<double> h(ret.size());
+    dR.to_host(h.data(), h.size() * 8);
+    for (size_t i = 0; i < h.size(); i++) ret[i] = (Ty)h[i];
+}
+
+// sp_tensor.tpp:6-14: sqrt of the FP64 sum of squares of the values
+template <typename Ty>
+double fnorm(const SpTensor<Ty> &A) {
+    int sm = 0;
+    int64_t nnz = 0;
+    double nrm = 0.0;
+    diana::check(dt_sp_info(A.device(), &sm, &nnz, &nrm), "fnorm");
+    return nrm;
+}
 
 // tensor.tpp:350-465 at one rank: the block already is the global tensor.
 template <typename Ty>

@@ -123,6 +123,14 @@ def lib():
             "dt_sp_ttmc": [vp, vp, i32, C.POINTER(vp), C.POINTER(sz), vp],
             "dt_hooi_sparse": [vp, vp, C.POINTER(sz), i32, u64, C.POINTER(vp), C.POINTER(vp), vp,
                                C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)],
+            "dt_sp_create_f64": [vp, i32, C.POINTER(sz), C.c_int64, C.POINTER(vp), vp, C.POINTER(vp)],
+            "dt_sp_merge_duplicates": [vp, i32, C.POINTER(sz), C.c_int64, C.POINTER(vp), vp, C.POINTER(C.c_int64)],
+            "dt_sp_ttmc_mtr": [vp, vp, i32, C.POINTER(vp), C.POINTER(sz), vp],
+            "dt_sp_ttmc_mtl": [vp, vp, i32, C.POINTER(vp), C.POINTER(sz), vp, vp],
+            "dt_sp_hooi_create": [vp, vp, C.POINTER(sz), u64, C.POINTER(vp), C.POINTER(vp)],
+            "dt_sp_hooi_sweep": [vp, C.POINTER(dbl), C.POINTER(dbl)],
+            "dt_sp_hooi_get": [vp, C.POINTER(vp), vp, C.POINTER(dbl)],
+            "dt_sp_hooi_destroy": [vp],
         }
         for name, argtypes in sig.items():
             f = getattr(L, name)
@@ -145,7 +153,8 @@ def exported_symbols():
             "dt_mlp_forward", "dt_mlp_loss_grad", "dt_mlp_adam_step", "dt_sketch_features",
             "dt_sketch_features_kron", "dt_learned_start", "dt_sp_gen_zipf", "dt_sp_sort_mode",
             "dt_sp_create", "dt_sp_destroy", "dt_sp_info", "dt_sp_mode_arrays", "dt_sp_ttmc",
-            "dt_hooi_sparse"]
+            "dt_hooi_sparse", "dt_sp_create_f64", "dt_sp_merge_duplicates", "dt_sp_ttmc_mtr", "dt_sp_ttmc_mtl",
+            "dt_sp_hooi_create", "dt_sp_hooi_sweep", "dt_sp_hooi_get", "dt_sp_hooi_destroy"]
 
 
 class DeviceArray:
@@ -593,7 +602,8 @@ def download_rows(D: DeviceArray) -> np.ndarray:
 
 
 class SpTensor:
-    """dt_sp: a 3-way COO tensor on the device with its per-mode sorted copies."""
+    """dt_sp: an N-way COO tensor on the device with its per-mode sorted copies
+    (FP32 values, or FP64 values when val is an F64 array)."""
 
     def __init__(self, ctx: Context, dims, idx, val, nnz: int):
         self.ctx = ctx
@@ -603,17 +613,47 @@ class SpTensor:
         nd = len(self.dims)
         dd = (C.c_size_t * nd)(*self.dims)
         ip = (C.c_void_p * nd)(*[a.ptr for a in idx])
-        ctx.check(ctx.L.dt_sp_create(ctx.handle, nd, dd, self.nnz, ip, val.ptr, C.byref(h)))
+        create = ctx.L.dt_sp_create_f64 if val.dtype == DT_F64 else ctx.L.dt_sp_create
+        ctx.check(create(ctx.handle, nd, dd, self.nnz, ip, val.ptr, C.byref(h)))
         self.handle = h
         sm, nz, nrm = C.c_int(), C.c_int64(), C.c_double()
         ctx.L.dt_sp_info(h, C.byref(sm), C.byref(nz), C.byref(nrm))
         self.slice_mode, self.norm = sm.value, nrm.value
 
     @staticmethod
-    def from_host(ctx: Context, dims, idx_lists, vals):
+    def from_host(ctx: Context, dims, idx_lists, vals, f64=False):
         idx = [ctx.to_device(np.ascontiguousarray(a, dtype=np.int32)) for a in idx_lists]
-        val = ctx.to_device(np.ascontiguousarray(vals, dtype=np.float32))
+        val = ctx.to_device(np.ascontiguousarray(vals, dtype=np.float64 if f64 else np.float32))
         return SpTensor(ctx, dims, idx, val, len(vals))
+
+    def _ut(self, Us):
+        """device Ut[m] (R_m x I_m column-major) from host factors U[m] (I_m x R_m)."""
+        return [self.ctx.to_device(np.ascontiguousarray(np.asarray(u, dtype=np.float64).T)) for u in Us]
+
+    def ttmc_mtr(self, n: int, Us):
+        """Function::ttmc_mTR: L = Y_(n)^T U_n (K x R_n) for host factors Us[m] (I_m x R_m)."""
+        ranks = [np.shape(u)[1] for u in Us]
+        ut = self._ut(Us)
+        K = int(np.prod([r for m, r in enumerate(ranks) if m != n]))
+        L = self.ctx.array((K, ranks[n]), DT_F64)
+        up = (C.c_void_p * len(Us))(*[u.ptr for u in ut])
+        rk = (C.c_size_t * len(Us))(*ranks)
+        self.ctx.check(self.ctx.L.dt_sp_ttmc_mtr(self.ctx.handle, self.handle, n, up, rk, L.ptr))
+        return L.download()
+
+    def ttmc_mtl(self, n: int, Us, L):
+        """Function::ttmc_mTL: (Y_(n) L)^T (R_n x I_n) for host factors and host L (K x R_n)."""
+        ranks = [np.shape(u)[1] for u in Us]
+        ut = self._ut(Us)
+        dL = self.ctx.to_device(np.asarray(L, dtype=np.float64))
+        ret = self.ctx.array((ranks[n], self.dims[n]), DT_F64)
+        up = (C.c_void_p * len(Us))(*[u.ptr for u in ut])
+        rk = (C.c_size_t * len(Us))(*ranks)
+        self.ctx.check(self.ctx.L.dt_sp_ttmc_mtl(self.ctx.handle, self.handle, n, up, rk, dL.ptr, ret.ptr))
+        return ret.download()
+
+    def hooi_session(self, ranks, seed=0, U_init=None):
+        return SpHooiSession(self, ranks, seed, U_init)
 
     def mode_arrays(self, n: int):
         """(perm, rowptr, nonempty_rows) of mode n, downloaded."""
@@ -633,15 +673,18 @@ class SpTensor:
         self.ctx.check(self.ctx.L.dt_sp_ttmc(self.ctx.handle, self.handle, n, ptrs, rk, Y.ptr))
         return Y
 
-    def hooi(self, max_iter=10, seed=0, want_factors=True, ranks=(16, 16, 16), U_init=None):
+    def hooi(self, max_iter=10, seed=0, want_factors=True, ranks=None, U_init=None):
+        nd = len(self.dims)
+        if ranks is None:
+            ranks = (16,) * nd
         U = [self.ctx.array((d, r), DT_F64) for d, r in zip(self.dims, ranks)] if want_factors else []
         G = self.ctx.array(tuple(ranks), DT_F64) if want_factors else None
-        up = (C.c_void_p * 3)(*[u.ptr for u in U]) if U else None
+        up = (C.c_void_p * nd)(*[u.ptr for u in U]) if U else None
         dinit = [self.ctx.to_device(np.asarray(u, dtype=np.float64)) for u in U_init] if U_init is not None else None
-        ip = (C.c_void_p * 3)(*[u.ptr for u in dinit]) if dinit else None
+        ip = (C.c_void_p * nd)(*[u.ptr for u in dinit]) if dinit else None
         per = (C.c_double * max_iter)()
         res, ms = C.c_double(), C.c_double()
-        rk = (C.c_size_t * 3)(*ranks)
+        rk = (C.c_size_t * nd)(*ranks)
         self.ctx.check(self.ctx.L.dt_hooi_sparse(self.ctx.handle, self.handle, rk, max_iter, seed, ip, up,
                                                   G.ptr if G else None, per, C.byref(res), C.byref(ms)))
         return {"U": [u.download() for u in U], "G": G.download() if G else None,
@@ -651,6 +694,53 @@ class SpTensor:
         if self.handle:
             self.ctx.L.dt_sp_destroy(self.handle)
             self.handle = None
+
+
+class SpHooiSession:
+    """dt_sp_hooi_create / dt_sp_hooi_sweep / dt_sp_hooi_get: one sweep per call."""
+
+    def __init__(self, sp: SpTensor, ranks, seed=0, U_init=None):
+        self.sp, self.ctx, self.ranks = sp, sp.ctx, tuple(int(r) for r in ranks)
+        nd = len(sp.dims)
+        self._init = [self.ctx.to_device(np.asarray(u, dtype=np.float64)) for u in U_init] if U_init else None
+        ip = (C.c_void_p * nd)(*[u.ptr for u in self._init]) if self._init else None
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.L.dt_sp_hooi_create(self.ctx.handle, sp.handle, (C.c_size_t * nd)(*self.ranks), seed,
+                                                    ip, C.byref(h)))
+        self.h = h
+
+    def sweep(self):
+        """(residual, [||L||_F per mode])"""
+        nd = len(self.sp.dims)
+        ln = (C.c_double * nd)()
+        r = C.c_double()
+        self.ctx.check(self.ctx.L.dt_sp_hooi_sweep(self.h, ln, C.byref(r)))
+        return r.value, list(ln)
+
+    def result(self):
+        U = [self.ctx.array((d, r), DT_F64) for d, r in zip(self.sp.dims, self.ranks)]
+        G = self.ctx.array(self.ranks, DT_F64)
+        ms = C.c_double()
+        self.ctx.check(self.ctx.L.dt_sp_hooi_get(self.h, (C.c_void_p * len(U))(*[u.ptr for u in U]), G.ptr,
+                                                 C.byref(ms)))
+        return {"U": [u.download() for u in U], "G": G.download(), "sweeps_ms": ms.value}
+
+    def close(self):
+        if self.h:
+            self.ctx.L.dt_sp_hooi_destroy(self.h)
+            self.h = None
+
+
+def sp_merge_duplicates(ctx: Context, dims, idx_lists, vals):
+    """dt_sp_merge_duplicates on host lists: (idx lists, FP64 values) merged."""
+    nd = len(dims)
+    idx = [ctx.to_device(np.ascontiguousarray(a, dtype=np.int32)) for a in idx_lists]
+    val = ctx.to_device(np.ascontiguousarray(vals, dtype=np.float64))
+    nu = C.c_int64()
+    ctx.check(ctx.L.dt_sp_merge_duplicates(ctx.handle, nd, (C.c_size_t * nd)(*dims), len(vals),
+                                           (C.c_void_p * nd)(*[a.ptr for a in idx]), val.ptr, C.byref(nu)))
+    k = nu.value
+    return [a.download()[:k] for a in idx], val.download()[:k]
 
 
 def sp_gen_zipf(ctx: Context, dims, ndraws: int, exponent: float, seed: int):

@@ -185,11 +185,19 @@ int dt_gen_tucker(dt_ctx *ctx, const dt_tucker_spec *spec, float *A_f32, uint16_
                   double *const *U_true, double *normX_host, double *normA_host);
 
 /* ---- sparse ---------------------------------------------------------------
- * 3-way sparse HOOI of Algorithm::SpTucker::Sp_HOOI_ALS
- * (template/algorithm/sp_tucker/sp_hooi_als.tpp:67-118): random orthonormal
- * start, then per sweep and mode n one step U_n <- orth(Y_(n) Y_(n)ᵀ U_n)
- * (Sp_TTMC_ALS_, :48-65 = ttmc_mTR + ttmc_mTL + orth2), core = Y_(N-1)ᵀ U_{N-1}
- * (:110).  Indices int32, values FP32, all device pointers. */
+ * Sparse HOOI of Algorithm::SpTucker::Sp_HOOI_ALS
+ * (template/algorithm/sp_tucker/sp_hooi_als.tpp:67-118) for any number of
+ * modes: random orthonormal start, then per sweep and mode n one step
+ * U_n <- orth(Y_(n) Y_(n)ᵀ U_n) (Sp_TTMC_ALS_, :48-65 = ttmc_mTR + ttmc_mTL +
+ * orth2), core = Y_(N-1)ᵀ U_{N-1} (:110).  Indices int32, values FP32 (or FP64
+ * with dt_sp_create_f64), all device pointers.  Y_(n) columns follow the
+ * reference's Kronecker order (add_outer_product_all, sp_tensor.tpp:367-417):
+ * the other modes ascending, the first fastest.  3-way tensors with FP32
+ * values and every rank <= 16 take the MFMA TTMc pipeline; other shapes the
+ * general kernels (every rank <= 64, product of the other modes' ranks <= 4096),
+ * in FP64 when the tensor holds FP64 values.  Every rank must be at most its
+ * mode's size and at most the product of the other ranks (Y_(n) has that many
+ * columns); else DT_EINVAL. */
 typedef struct dt_sp dt_sp;
 /* COO from independent per-mode truncated Zipf draws P(k) ~ (k+1)^-s (inverse
  * CDF on an FP64 table; Philox uniform streams 7000 + n, values stream 7100,
@@ -209,6 +217,18 @@ int dt_sp_sort_mode(dt_ctx *ctx, int64_t nnz, const int32_t *keys, size_t dim, i
  * stably sorted copy per mode; slice mode = the largest mode, first on ties. */
 int dt_sp_create(dt_ctx *ctx, int ndim, const size_t *dims, int64_t nnz, const int32_t *const *idx,
                  const float *val, dt_sp **sp);
+/* SpTensor<double>(data_buffer*): the same with FP64 values, kept for the FP64
+ * path (TTMc, L and the factor update accumulate in FP64); an FP32 rounding
+ * of them serves nothing else. */
+int dt_sp_create_f64(dt_ctx *ctx, int ndim, const size_t *dims, int64_t nnz, const int32_t *const *idx,
+                     const double *val, dt_sp **sp);
+/* Ingest step after read_data (src/read_data.cpp:80-127, which keeps every
+ * line): nonzeros with equal coordinates are merged in place, values summed
+ * in FP64 in input order at the first occurrence; survivors keep their input
+ * order (a list without duplicates is unchanged).  *nnz_out receives the new
+ * count; indices outside [0, dims) give DT_EINVAL. */
+int dt_sp_merge_duplicates(dt_ctx *ctx, int ndim, const size_t *dims, int64_t nnz, int32_t *const *idx,
+                           double *val, int64_t *nnz_out);
 int dt_sp_destroy(dt_sp *sp);
 int dt_sp_info(const dt_sp *sp, int *slice_mode, int64_t *nnz, double *norm);
 int dt_sp_mode_arrays(const dt_sp *sp, int n, const int32_t **perm, const int32_t **rowptr,
@@ -219,10 +239,33 @@ int dt_sp_mode_arrays(const dt_sp *sp, int n, const int32_t **perm, const int32_
  * Replaces the Kronecker accumulation of ttmc_mTR / ttmc_mTL
  * (sp_tensor.tpp:236-365, add_outer_product_all :367-417); ranks 16 only. */
 int dt_sp_ttmc(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, const size_t *ranks, float *Y);
-/* Sparse HOOI: ranks R_n <= 16, max_iter sweeps.  Start: U_init[n] (device
+/* Function::ttmc_mTR (include/function.hpp:70, template/function/
+ * sp_tensor.tpp:236-286) as intended: L = Y_(n)ᵀ U_n, K x R_n column-major FP64
+ * with K = prod_{m != n} R_m in the Kronecker order above.  Ut[m]: R_m x I_m
+ * column-major FP64 (the reference's transposed factors), every mode's. */
+int dt_sp_ttmc_mtr(dt_ctx *ctx, dt_sp *sp, int n, const double *const *Ut, const size_t *ranks, double *L);
+/* Function::ttmc_mTL (include/function.hpp:74, sp_tensor.tpp:288-365) as
+ * intended: ret (R_n x I_n column-major FP64) = (Y_(n) L)ᵀ for L (K x R_n
+ * column-major); slices without nonzeros give zero columns. */
+int dt_sp_ttmc_mtl(dt_ctx *ctx, dt_sp *sp, int n, const double *const *Ut, const size_t *ranks, const double *L,
+                   double *ret);
+/* The Sp_HOOI_ALS loop one sweep at a time (sp_hooi_als.tpp:98-109), for the
+ * host layer's per-sweep lines: create takes the start as dt_hooi_sparse does;
+ * a sweep returns each mode's ||L||_F (what Sp_TTMC_ALS_ prints as ||G||_F,
+ * :51-53; ndim entries, or NULL) and the residual sqrt(1 - ||G||^2/||A||^2) of
+ * the core after the sweep; get copies the factors (I_n x R_n column-major
+ * FP64), the core (R_0 x ... x R_{N-1} column-major FP64) and the sweeps'
+ * device time. */
+typedef struct dt_sp_hooi dt_sp_hooi;
+int dt_sp_hooi_create(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, uint64_t seed, const double *const *U_init,
+                      dt_sp_hooi **h);
+int dt_sp_hooi_sweep(dt_sp_hooi *h, double *L_norms, double *residual);
+int dt_sp_hooi_get(dt_sp_hooi *h, double *const *U_out, double *G_out, double *sweeps_ms);
+int dt_sp_hooi_destroy(dt_sp_hooi *h);
+/* Sparse HOOI: ranks as above, max_iter sweeps.  Start: U_init[n] (device
  * FP64 shape[n] x R_n column-major, orthonormalised here), or, when U_init is
  * NULL, Philox normals (streams 200 + n).  U_out[n] (shape[n] x R_n FP64
- * column-major) and G_out (R_0 x R_1 x R_2 FP64 column-major) may be NULL;
+ * column-major) and G_out (R_0 x ... x R_{N-1} FP64 column-major) may be NULL;
  * residual_per_sweep holds max_iter entries (or NULL); sweeps_ms = device
  * time of the sweeps. */
 int dt_hooi_sparse(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, int max_iter, uint64_t seed,

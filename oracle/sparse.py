@@ -13,6 +13,10 @@
   val * U_a[i_a, r_a] * U_b[i_b, r_b] (ttmc_mTR / ttmc_mTL with
   add_outer_product_all, template/function/sp_tensor.tpp:236-417, as intended:
   the Kronecker row of a slice reset per slice and paired with row i of U_n).
+* ttmc_mTR / ttmc_mTL (sp_tensor.tpp:236-365, as intended): Y_(n)ᵀ U_n and
+  (Y_(n) L)ᵀ.
+* Duplicate coordinates merged at their first occurrence (FP64 sums in input
+  order), the rest in input order.
 * Sparse HOOI (sp_hooi_als.tpp:48-118): per sweep and mode n,
   U_n <- orth(Y_(n) Y_(n)ᵀ U_n) with orth = QR with positive R diagonal (the
   span and basis modified Gram-Schmidt (orth2) gives); core = Y_(N-1)ᵀ U_{N-1}.
@@ -93,8 +97,39 @@ def ttmc(dims, idx, vals, Us, n):
     return Y
 
 
-def hooi(dims, idx, vals, U0, sweeps):
-    """Sparse HOOI from factors U0 (orthonormal columns); residual per sweep."""
+def ttmc_mtr(dims, idx, vals, Us, n):
+    """Function::ttmc_mTR (sp_tensor.tpp:236-286) as intended: L = Y_(n)ᵀ U_n,
+    K x R_n with K in the Kronecker order of ttmc."""
+    return ttmc(dims, idx, vals, Us, n).T @ np.asarray(Us[n], dtype=np.float64)
+
+
+def ttmc_mtl(dims, idx, vals, Us, n, L):
+    """Function::ttmc_mTL (sp_tensor.tpp:288-365) as intended: (Y_(n) L)ᵀ,
+    R_n x I_n."""
+    return (ttmc(dims, idx, vals, Us, n) @ np.asarray(L, dtype=np.float64)).T
+
+
+def merge_first_occurrence(idx, vals):
+    """Equal coordinates merged: one entry per coordinate, at its first
+    occurrence, value = FP64 sum of its entries in input order; the survivors
+    keep their input order (ingest step after read_data, read_data.cpp:80-127)."""
+    coords = np.stack([np.asarray(i, dtype=np.int64) for i in idx], axis=1)
+    _, first, inv = np.unique(coords, axis=0, return_index=True, return_inverse=True)
+    inv = np.asarray(inv).reshape(-1)
+    sums = np.zeros(len(first))
+    v = np.asarray(vals, dtype=np.float64)
+    for k in range(len(v)):  # sequential: input order
+        sums[inv[k]] += v[k]
+    order = np.argsort(first, kind="stable")
+    keep = first[order]
+    return [np.asarray(i)[keep] for i in idx], sums[order]
+
+
+def hooi(dims, idx, vals, U0, sweeps, lnorms=None):
+    """Sparse HOOI from factors U0 (orthonormal columns); residual per sweep.
+    When lnorms is a list, ||L||_F of every mode and sweep is appended to it
+    (L = Y_(n)ᵀ U_n before the update: what Sp_TTMC_ALS_ prints as ||G||_F,
+    sp_hooi_als.tpp:51-53)."""
     Us = [np.asarray(u, dtype=np.float64) for u in U0]
     normA2 = float(np.sum(np.asarray(vals, dtype=np.float64) ** 2))
     res = []
@@ -102,7 +137,10 @@ def hooi(dims, idx, vals, U0, sweeps):
     for _ in range(sweeps):
         for n in range(N):
             Y = ttmc(dims, idx, vals, Us, n)
-            Us[n] = D.orth_posdiag(Y @ (Y.T @ Us[n]))
+            L = Y.T @ Us[n]
+            if lnorms is not None:
+                lnorms.append(float(np.linalg.norm(L)))
+            Us[n] = D.orth_posdiag(Y @ L)
         Y = ttmc(dims, idx, vals, Us, N - 1)
         G = Y.T @ Us[N - 1]
         res.append(np.sqrt(max(0.0, 1.0 - float(np.sum(G * G)) / normA2)))

@@ -589,6 +589,10 @@ def test_hooi_rank_above_size_is_einval(ctx):
     with pytest.raises(dt.DTError) as e:
         ctx.hooi_dense(dA, (5, 2, 2), max_iter=1, init=dt.INIT_RANDOM)
     assert e.value.code == dt.DT_EINVAL
+    # rank 5 of mode 2 above R_0 R_1 = 4: Y_(2) has four columns
+    with pytest.raises(dt.DTError) as e:
+        ctx.hooi_dense(dA, (2, 2, 5), max_iter=1, init=dt.INIT_RANDOM)
+    assert e.value.code == dt.DT_EINVAL
 
 
 def test_hooi_zero_tensor_is_enumeric(ctx):

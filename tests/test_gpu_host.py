@@ -80,35 +80,74 @@ def check_summary(err, region, calls):
     assert any(l.startswith("GPU kernels") for l in lines)
 
 
-def test_spmain_driver_matches_oracle_from_the_reference_start(tmp_path):
-    """spmain.cpp's workload through the host layer: read a .tns file, build the
-    SpTensor, Sp_HOOI_ALS with truncation 5 from the reference's start (srand(1),
-    Tensor {R, I}.randn() per mode); the printed residual equals the sparse
-    oracle's from the same start (glibc rand() restated)."""
+def run_spmain_vs_oracle(tmp_path, idx, vals, R, sweeps, text_vals=None):
+    """Write a .tns file (1-based indices), run spmain on it and compare with the
+    sparse oracle from the reference's start (srand(1), Tensor {R, I}.randn()
+    per mode, orthonormalised): each sweep's "Calculating iteration" line
+    followed by one "||G||_F" line per mode (||L|| of Sp_TTMC_ALS_,
+    sp_hooi_als.tpp:51-53), then the core's norm and the residual, all printed
+    with six decimals.  Duplicate coordinates are merged before the oracle runs,
+    as the SpTensor ingest does; values are FP64 end to end."""
     import sys
     import numpy as np
     sys.path.insert(0, ROOT)
     from oracle import dense as D
     from oracle import sparse as S
     from oracle.glibc_rand import GlibcRand, randn_fill
-    dims, R, sweeps = (60, 50, 40), 5, 2
-    idx, vals = S.gen_zipf(dims, 6000, 1.1, 4)
-    vals = np.round(vals.astype(np.float64), 6)  # what the text file holds
-    dims = tuple(int(i.max()) + 1 for i in idx)  # read_data: dims = largest index + 1
-    path = tmp_path / "small.tns"
+    N = len(idx)
+    path = tmp_path / "in.tns"
     with open(path, "w") as f:
         for k in range(len(vals)):
-            f.write(" ".join(str(int(idx[n][k]) + 1) for n in range(3)) + f" {vals[k]:.6f}\n")
+            v = text_vals[k] if text_vals is not None else f"{vals[k]:.6f}"
+            f.write(" ".join(str(int(idx[n][k]) + 1) for n in range(N)) + f" {v}\n")
     p = run("spmain", path, 1, R, sweeps)
     assert p.returncode == 0, p.stderr
-    res = [float(l.split("=")[-1]) for l in p.stderr.splitlines() if l.startswith("Residual")]
-    assert len(res) == 1
+    lines = p.stderr.splitlines()
+    dims = tuple(int(i.max()) + 1 for i in idx)  # read_data: dims = largest index + 1
+    midx, mvals = S.merge_first_occurrence(idx, vals)
     g = GlibcRand(1)
     U0 = []
     for I in dims:
         ut = np.array(randn_fill(g, R * I)).reshape((R, I), order="F")  # Tensor {R, I}
         U0.append(D.orth_posdiag(ut.T))
-    _, _, rr = S.hooi(dims, idx, vals.astype(np.float32), U0, sweeps)
+    ln = []
+    _, Gr, rr = S.hooi(dims, midx, mvals, U0, sweeps, lnorms=ln)
+    it = [i for i, l in enumerate(lines) if l.startswith("Calculating iteration")]
+    assert len(it) == sweeps
+    got = []
+    for i in it:
+        got += [float(l.split("=")[-1]) for l in lines[i + 1:i + 1 + N]]
+        assert all(l.startswith("||G||_F") for l in lines[i + 1:i + 1 + N])
+    print("spmain ||L||", got, "oracle", ln, flush=True)
+    assert np.allclose(got, ln, rtol=1e-6, atol=2e-6)
+    res = [float(l.split("=")[-1]) for l in lines if l.startswith("Residual")]
+    assert len(res) == 1
     print("spmain residual", res[0], "oracle", rr[-1], flush=True)
-    assert abs(res[0] - rr[-1]) < 5e-6   # printed with 6 decimals; FP32 TTMc
-    check_summary(p.stderr, "SpTucker::Sp_HOOI_ALS::sweeps()", 1)
+    assert abs(res[0] - rr[-1]) < 2e-6   # printed with 6 decimals; FP64 path
+    check_summary(p.stderr, "SpTucker::Sp_HOOI_ALS::sweep()", sweeps)
+
+
+def test_spmain_driver_matches_oracle_from_the_reference_start(tmp_path):
+    """spmain.cpp's workload through the host layer on a 3-way Zipf file."""
+    import sys
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from oracle import sparse as S
+    idx, vals = S.gen_zipf((60, 50, 40), 6000, 1.1, 4)
+    vals = np.round(vals.astype(np.float64), 6)  # what the text file holds
+    run_spmain_vs_oracle(tmp_path, idx, vals, 5, 2)
+
+
+def test_spmain_4way_file_signed_values_and_duplicates(tmp_path):
+    """A 4-way file whose values carry signs and exponents and whose
+    coordinates repeat (merged at ingest), truncation 4, three sweeps."""
+    import numpy as np
+    rng = np.random.default_rng(8)
+    dims, nnz = (30, 25, 20, 12), 5000
+    idx = [np.minimum((rng.pareto(1.0, nnz) * d / 6).astype(np.int64), d - 1) for d in dims]
+    mant = rng.integers(1, 10 ** 6, nnz)
+    expo = rng.integers(-4, 2, nnz)
+    sign = np.where(rng.random(nnz) < 0.3, "-", "")
+    text = [f"{s}{m}e{e}" for s, m, e in zip(sign, mant, expo)]
+    vals = np.array([float(t) for t in text])
+    run_spmain_vs_oracle(tmp_path, idx, vals, 4, 3, text_vals=text)

@@ -32,11 +32,19 @@ def test_zipf_generator_bit_exact(ctx, dims, ndraws, seed):
     assert np.array_equal(val.download()[:nnz], rval)
 
 
-@pytest.mark.parametrize("n,dim,seed", [(1000, 7, 1), (100000, 1000, 2), (3000000, 1 << 20, 3)])
+@pytest.mark.parametrize("n,dim,seed", [(1000, 7, 1), (100000, 1000, 2), (3000000, 1 << 20, 3),
+                                         (1, 5, 4), (4095, 3, 5), (4096, 300, 6), (4097, 1 << 17, 7),
+                                         (65537, 1 << 24, 8), (20000, 1, 9), (12289, 1 << 25, 10)])
 def test_sort_mode_bit_exact(ctx, n, dim, seed):
-    """perm == argsort(kind=stable) (the constructor's counting sort), row pointers == histogram scan."""
+    """perm == argsort(kind=stable) (the constructor's counting sort), row pointers
+    == histogram scan; sizes around the sort's 4096-item tiles, one to four
+    8-bit passes, a single key value."""
+    print("sort", n, dim, flush=True)
     rng = np.random.default_rng(seed)
-    keys = np.minimum((rng.pareto(1.1, n) * 3).astype(np.int64), dim - 1).astype(np.int32)
+    if dim >= 1 << 24:
+        keys = rng.integers(0, dim, n).astype(np.int32)
+    else:
+        keys = np.minimum((rng.pareto(1.1, n) * 3).astype(np.int64), dim - 1).astype(np.int32)
     perm, rowptr = dt.sp_sort_mode(ctx, ctx.to_device(keys), n, dim)
     rp, rr = S.sort_mode(keys, dim)
     assert np.array_equal(perm.download(), rp)
@@ -190,11 +198,12 @@ def test_sparse_hooi_ranks_below_16_from_given_start(ctx):
 
 
 def test_sparse_bad_ranks_einval(ctx):
+    """A rank above its mode's size is refused (ranks above 16 take the general path)."""
     idx, vals = make_coo(np.random.default_rng(1), (40, 30, 20), 500)
     sp = dt.SpTensor.from_host(ctx, (40, 30, 20), idx, vals)
     try:
         with pytest.raises(dt.DTError) as e:
-            ctx.check(ctx.L.dt_hooi_sparse(ctx.handle, sp.handle, (dt.C.c_size_t * 3)(17, 8, 8), 1, 0, None, None,
+            ctx.check(ctx.L.dt_hooi_sparse(ctx.handle, sp.handle, (dt.C.c_size_t * 3)(41, 8, 8), 1, 0, None, None,
                                            None, None, dt.C.byref(dt.C.c_double()), None))
         assert e.value.code == dt.DT_EINVAL
     finally:

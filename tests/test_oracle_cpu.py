@@ -346,3 +346,76 @@ def test_zipf_cdf_and_merge_small():
     keys = set(zip(*[u.tolist() for u in uidx]))
     assert len(keys) == len(uval) < 5000
     assert np.isclose(float(np.sum(uval, dtype=np.float64)), float(np.sum(vals, dtype=np.float64)), rtol=1e-6)
+
+
+def test_sparse_ttmc_oracle_equals_dense_chain_nway():
+    """4-way and 2-way: the Kronecker rows (first other mode fastest) equal the
+    dense TTM chain's unfolding; ttmc_mTR / ttmc_mTL are its products with U_n
+    and L (closed forms Y_(n)ᵀ U_n and (Y_(n) L)ᵀ of the dense chain)."""
+    from oracle import sparse as S
+    rng = np.random.default_rng(13)
+    for dims, ranks in (((6, 5, 4, 3), (2, 3, 2, 2)), ((9, 7), (3, 2))):
+        nnz = 60
+        idx = [rng.integers(0, d, nnz) for d in dims]
+        vals = rng.standard_normal(nnz)
+        Us = [rng.standard_normal((d, r)) for d, r in zip(dims, ranks)]
+        A = S.densify(dims, idx, vals)
+        for n in range(len(dims)):
+            T = A
+            for m in range(len(dims)):
+                if m != n:
+                    T = D.ttm(T, Us[m].T, m)
+            Yd = D.tenmat(T, n)
+            assert np.allclose(S.ttmc(dims, idx, vals, Us, n), Yd, rtol=1e-12, atol=1e-12)
+            L = S.ttmc_mtr(dims, idx, vals, Us, n)
+            assert np.allclose(L, Yd.T @ Us[n], rtol=1e-12, atol=1e-12)
+            Lr = rng.standard_normal(L.shape)
+            assert np.allclose(S.ttmc_mtl(dims, idx, vals, Us, n, Lr), (Yd @ Lr).T, rtol=1e-12, atol=1e-12)
+
+
+def test_sparse_merge_first_occurrence_brute_force():
+    """Against a dict walk in input order: one entry per coordinate at its first
+    occurrence, FP64 sums in input order, survivors in input order; a list
+    without duplicates is unchanged."""
+    from oracle import sparse as S
+    rng = np.random.default_rng(17)
+    for dims, nnz in (((5, 4, 3), 200), ((3, 3, 2, 2), 90), ((1000, 900), 50)):
+        idx = [rng.integers(0, d, nnz) for d in dims]
+        vals = rng.standard_normal(nnz)
+        seen = {}
+        order = []
+        for k in range(nnz):
+            c = tuple(int(i[k]) for i in idx)
+            if c not in seen:
+                seen[c] = 0.0
+                order.append(c)
+            seen[c] += float(vals[k])
+        uidx, uval = S.merge_first_occurrence(idx, vals)
+        assert [tuple(int(u[j]) for u in uidx) for j in range(len(uval))] == order
+        assert np.array_equal(uval, np.array([seen[c] for c in order]))
+    idx = [np.arange(10), np.arange(10)[::-1]]
+    uidx, uval = S.merge_first_occurrence(idx, np.arange(10.0))
+    assert all(np.array_equal(a, b) for a, b in zip(uidx, idx)) and np.array_equal(uval, np.arange(10.0))
+
+
+def test_sparse_hooi_oracle_recovers_low_rank_4way():
+    """An exactly rank-(2,2,2,2) 4-way tensor stored as COO: fit 1 and the true
+    subspaces; the ||L|| the oracle records for the last mode of the last sweep
+    equals ||G|| (the core is that mode's L with the updated factor)."""
+    from oracle import sparse as S
+    rng = np.random.default_rng(19)
+    dims, r = (6, 5, 5, 4), 2
+    G = rng.standard_normal((r,) * 4) * 5
+    Ut = [np.linalg.qr(rng.standard_normal((d, r)))[0] for d in dims]
+    A = G
+    for n in range(4):
+        A = D.ttm(A, Ut[n], n)
+    idx = [a.ravel(order="F") for a in np.meshgrid(*[np.arange(d) for d in dims], indexing="ij")]
+    vals = A.ravel(order="F")
+    U0 = [np.linalg.qr(rng.standard_normal((d, r)))[0] for d in dims]
+    ln = []
+    Us, Gc, res = S.hooi(dims, idx, vals, U0, 20, lnorms=ln)
+    assert res[-1] < 1e-6 and len(ln) == 80
+    for n in range(4):
+        assert D.sin_max_angle(Us[n], Ut[n]) < 1e-6
+    assert np.isclose(np.linalg.norm(Gc), np.linalg.norm(A), rtol=1e-10)
