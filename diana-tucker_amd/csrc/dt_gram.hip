@@ -117,6 +117,148 @@ __global__ __launch_bounds__(256) void gram_mfma(const TA *__restrict__ A, float
             }
 }
 
+// gram_big: S = A_(n) A_(n)ᵀ for I >= 256 over many columns.  A workgroup of
+// eight waves owns a 256 x 256 block (a, b), a >= b, of 256-row panels of
+// A_(n) and one column range (split-K); it stages chunks of GB_KC columns of
+// the panel(s) in LDS (two buffers: the next chunk's global loads are in
+// flight while the current chunk's MFMAs run, one barrier per chunk) and
+// accumulates 16 x 16 tiles on v_mfma_f32_16x16x4_f32.  MFMA step s of lane
+// (c, g) pairs k = 4 g + s of the chunk in both operands (the Gram sums over
+// k, so any pairing that is the same on both sides gives the same sum), so a
+// lane's four values of a 16-row block come from one 16-byte LDS read.
+//   diagonal block (a == b): wave w owns the lower tiles of row blocks w and
+//     15 - w (17 tiles); off-diagonal: row blocks 2w, 2w + 1 of panel a against
+//     the 16 row blocks of panel b (32 tiles).
+// Row layout in LDS (pitch GB_KC + 4 floats) when A_(n)'s columns are
+// contiguous in memory (F % 32 == 0: a chunk is 32 consecutive f of one b, a
+// row is 128 bytes); column layout (pitch rows + 4) for mode 0 (F == 1, a
+// column is contiguous).  Rows past I read as zero.
+constexpr int GB_T = 512, GB_P = 256, GB_KC = 16, GB_KS = GB_KC + 4, GB_RS = 2 * GB_P + 4;
+constexpr int GB_BUF = 2 * GB_P * GB_KS > GB_KC * GB_RS ? 2 * GB_P * GB_KS : GB_KC * GB_RS;
+
+__device__ __forceinline__ floatx4 gb_ld4(const float *p) { return *(const floatx4 *)p; }
+__device__ __forceinline__ floatx4 gb_ld4(const uint16_t *p) {
+    const uint2 u = *(const uint2 *)p;
+    return floatx4{__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u), __uint_as_float(u.y << 16),
+                   __uint_as_float(u.y & 0xffff0000u)};
+}
+
+template <typename TA, bool KMAJOR>
+__global__ __launch_bounds__(GB_T) void gram_big(const TA *__restrict__ A, float *__restrict__ Sp, int64_t F,
+                                                 int64_t I, int64_t Ctot, int64_t cps) {
+    __shared__ float X[2][GB_BUF];
+    int pa, pb;
+    tri_decode((int)blockIdx.x, pa, pb);
+    const bool diag = pa == pb;
+    const int nrows = diag ? GB_P : 2 * GB_P;
+    const int64_t c_begin = (int64_t)blockIdx.y * cps;
+    const int64_t c_end = c_begin + cps < Ctot ? c_begin + cps : Ctot;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int c = lane & 15, g = lane >> 4;
+    // global row of staged row r
+    auto grow = [&](int r) -> int64_t { return r < GB_P ? (int64_t)pa * GB_P + r : (int64_t)pb * GB_P + (r - GB_P); };
+    // chunk loads: nrows x GB_KC values as float4s, at most 4 per thread
+    constexpr int NL = 2 * GB_P * GB_KC / 4 / GB_T;  // 4
+    floatx4 ld[NL];
+    auto fetch = [&](int64_t c0) {
+#pragma unroll
+        for (int q = 0; q < NL; q++) {
+            const int e = tid + GB_T * q;
+            floatx4 v = {0.f, 0.f, 0.f, 0.f};
+            if (KMAJOR) {
+                // F == 1: element (row, column c0 + k) at A[row + I (c0 + k)]
+                const int nr4 = nrows / 4;
+                const int k = e / nr4, r4 = e % nr4;
+                const int64_t gr = grow(4 * r4);
+                if (k < GB_KC && gr < I && c0 + k < c_end) v = gb_ld4(A + gr + I * (c0 + k));
+            } else {
+                // F % 32 == 0: a chunk is f0 .. f0 + GB_KC - 1 of one b
+                const int r = e / (GB_KC / 4), part = e % (GB_KC / 4);
+                const int64_t gr = grow(r);
+                const int64_t b = c0 / F, f0 = c0 - b * F;
+                if (r < nrows && gr < I) v = gb_ld4(A + f0 + 4 * part + F * gr + F * I * b);
+            }
+            ld[q] = v;
+        }
+    };
+    auto stage = [&](float *buf) {
+#pragma unroll
+        for (int q = 0; q < NL; q++) {
+            const int e = tid + GB_T * q;
+            if (KMAJOR) {
+                const int nr4 = nrows / 4;
+                const int k = e / nr4, r4 = e % nr4;
+                if (k < GB_KC) *(floatx4 *)&buf[k * GB_RS + 4 * r4] = ld[q];
+            } else {
+                const int r = e / (GB_KC / 4), part = e % (GB_KC / 4);
+                if (r < nrows) *(floatx4 *)&buf[r * GB_KS + 4 * part] = ld[q];
+            }
+        }
+    };
+    // four values of staged row block q for lane (c, g): k = 4 g .. 4 g + 3
+    auto frag = [&](const float *buf, int q) -> floatx4 {
+        const int r = 16 * q + c;
+        if (KMAJOR)
+            return floatx4{buf[(4 * g) * GB_RS + r], buf[(4 * g + 1) * GB_RS + r], buf[(4 * g + 2) * GB_RS + r],
+                           buf[(4 * g + 3) * GB_RS + r]};
+        return *(const floatx4 *)&buf[r * GB_KS + 4 * g];
+    };
+    floatx4 acc[2][16];
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) acc[u][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // row blocks of this wave's A operands and the first B row block
+    const int ra0 = diag ? w : 2 * w, ra1 = diag ? 15 - w : 2 * w + 1, rb0 = diag ? 0 : 16;
+    const int lim0 = diag ? w : 15, lim1 = diag ? 15 - w : 15;
+    int cur = 0;
+    if (c_begin < c_end) {
+        fetch(c_begin);
+        stage(X[0]);
+    }
+    __syncthreads();
+    for (int64_t c0 = c_begin; c0 < c_end; c0 += GB_KC) {
+        const bool more = c0 + GB_KC < c_end;
+        if (more) fetch(c0 + GB_KC);
+        const float *buf = X[cur];
+        const floatx4 a0 = frag(buf, ra0), a1 = frag(buf, ra1);
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const floatx4 bj = frag(buf, rb0 + j);
+            if (j <= lim0) {
+#pragma unroll
+                for (int st = 0; st < 4; st++)
+                    acc[0][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[st], bj[st], acc[0][j], 0, 0, 0);
+            }
+            if (j <= lim1) {
+#pragma unroll
+                for (int st = 0; st < 4; st++)
+                    acc[1][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[st], bj[st], acc[1][j], 0, 0, 0);
+            }
+        }
+        if (more) stage(X[cur ^ 1]);
+        __syncthreads();
+        cur ^= 1;
+    }
+    float *S = Sp + (int64_t)blockIdx.y * I * I;
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int ra = u == 0 ? ra0 : ra1;
+        const int lim = u == 0 ? lim0 : lim1;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            if (j > lim) continue;
+            const int64_t gj = grow(16 * (rb0 + j) + c);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int64_t gi = grow(16 * ra + 4 * g + v);
+                if (gi < I && gj < I) S[gi + I * gj] = acc[u][j][v];
+            }
+        }
+    }
+}
+
 // Cross product C = A_(n) B_(n)ᵀ (IA x JB) of two tensors that differ only in
 // mode n, on the same MFMA tiling as gram_mfma: workgroup = one 64x64 tile
 // (ti over IA, tj over JB) and one column range (split-K).  FP32 partial
@@ -209,14 +351,15 @@ __global__ void cross_reduce(const float *__restrict__ Cp, double *__restrict__ 
     }
 }
 
-// S[i, j] = sum over splits of the partial stored in the lower tile triangle.
+// S[i, j] = sum over splits of the partial stored in the lower triangle of
+// tiles of `tile` rows (64 for gram_mfma, 16 for gram_big).
 __global__ void gram_reduce(const float *__restrict__ Sp, double *__restrict__ S, int64_t I,
-                            int nsplit) {
+                            int nsplit, int tile) {
     const int64_t total = I * I;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
         const int64_t i = idx % I, j = idx / I;
-        const int64_t src = (i / 64 >= j / 64) ? idx : (j + I * i);
+        const int64_t src = (i / tile >= j / tile) ? idx : (j + I * i);
         double s = 0.0;
         for (int q = 0; q < nsplit; q++) s += (double)Sp[(int64_t)q * total + src];
         S[idx] = s;
@@ -373,15 +516,24 @@ extern "C" int dt_sumsq(dt_ctx *ctx, const void *A, int dt, size_t n, double *ou
     return dt_sumsq_dev(ctx, A, dt, n, out_host);
 }
 
-// Split-K plan of the MFMA Gram of order I over Ctot columns: ~512
-// workgroups in total, at least 256 columns per split so the FP32 partial
-// tiles stay small next to the data read, at most max_split splits.
-static void gram_plan(int64_t I, int64_t Ctot, int64_t max_split, int64_t &nsplit, int64_t &cps) {
-    const int nt = (int)ceil_div(I, 64);
+// gram_big applies to orders of at least 256 with many columns, when A_(n)'s
+// chunks load as aligned 16-byte (FP32) or 8-byte (bf16) vectors: mode 0 with
+// I % 4 == 0, or F % 32 == 0.
+static bool gram_big_fits(int64_t F, int64_t I, int64_t Ctot, const void *A) {
+    if (I < GB_P || Ctot < (int64_t)1 << 16 || ((uintptr_t)A & 15) != 0) return false;
+    return (F == 1 && I % 4 == 0) || (F % 32 == 0);
+}
+
+// Split-K plan of the MFMA Gram of order I over Ctot columns, at most
+// max_split splits.  gram_mfma: ~512 workgroups of 64 x 64 tiles, at least 256
+// columns per split.  gram_big: ~1024 workgroups of 256 x 256 blocks (four per
+// CU over the run), at least 1024 columns per split.
+static void gram_plan(int64_t I, int64_t Ctot, int64_t max_split, bool big, int64_t &nsplit, int64_t &cps) {
+    const int nt = (int)ceil_div(I, big ? GB_P : 64);
     const int ntiles = nt * (nt + 1) / 2;
-    nsplit = 512 / ntiles;
+    nsplit = (big ? 1024 : 512) / ntiles;
     if (nsplit < 1) nsplit = 1;
-    int64_t maxsplit = ceil_div(Ctot, 256);
+    int64_t maxsplit = ceil_div(Ctot, big ? 1024 : 256);
     if (maxsplit < 1) maxsplit = 1;
     if (nsplit > maxsplit) nsplit = maxsplit;
     if (nsplit > max_split) nsplit = max_split;
@@ -394,8 +546,30 @@ static void gram_plan(int64_t I, int64_t Ctot, int64_t max_split, int64_t &nspli
 // reduction reads is written by this launch (whole lower-triangle tiles), so
 // Sp needs no clearing.
 static int gram_mfma_run(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t I, int64_t B, double *S,
-                         float *Sp, int64_t nsplit, int64_t cps) {
+                         float *Sp, int64_t nsplit, int64_t cps, bool big) {
     const int64_t Ctot = F * B;
+    if (big) {
+        const int np = (int)ceil_div(I, GB_P);
+        dim3 grid((unsigned)(np * (np + 1) / 2), (unsigned)nsplit);
+#define GBIG(T, KM) \
+    DT_LAUNCH("gram_big", (gram_big<T, KM>), grid, dim3(GB_T), 0, (const T *)A, Sp, F, I, Ctot, cps)
+        if (dtA == DT_F32) {
+            if (F == 1) {
+                GBIG(float, true);
+            } else {
+                GBIG(float, false);
+            }
+        } else {
+            if (F == 1) {
+                GBIG(uint16_t, true);
+            } else {
+                GBIG(uint16_t, false);
+            }
+        }
+#undef GBIG
+        DT_LAUNCH("gram_reduce", gram_reduce, gs_grid(I * I), dim3(256), 0, Sp, S, I, (int)nsplit, 16);
+        return DT_OK;
+    }
     const int nt = (int)ceil_div(I, 64);
     const int ntiles = nt * (nt + 1) / 2;
     dim3 grid((unsigned)ntiles, (unsigned)nsplit);
@@ -416,7 +590,7 @@ static int gram_mfma_run(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t
                       (const uint16_t *)A, Sp, F, I, Ctot, cps);
         }
     }
-    DT_LAUNCH("gram_reduce", gram_reduce, gs_grid(I * I), dim3(256), 0, Sp, S, I, (int)nsplit);
+    DT_LAUNCH("gram_reduce", gram_reduce, gs_grid(I * I), dim3(256), 0, Sp, S, I, (int)nsplit, 64);
     return DT_OK;
 }
 
@@ -431,10 +605,11 @@ int dt_gram_internal(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t I, 
     }
     DT_CHECK_ARG(dtA == DT_F32 || dtA == DT_BF16, "gram: unsupported type %d", dtA);
     int64_t nsplit, cps;
-    gram_plan(I, F * B, 256, nsplit, cps);
+    const bool big = gram_big_fits(F, I, F * B, A);
+    gram_plan(I, F * B, 256, big, nsplit, cps);
     float *Sp = (float *)dt_ws(ctx, WS_GRAM_PART, (size_t)(nsplit * I * I) * sizeof(float));
     if (!Sp) return DT_ENOMEM;
-    return gram_mfma_run(ctx, A, dtA, F, I, B, S, Sp, nsplit, cps);
+    return gram_mfma_run(ctx, A, dtA, F, I, B, S, Sp, nsplit, cps, big);
 }
 
 static int gram_args(dt_ctx *ctx, const void *A, const void *S, const size_t *shape, int ndim, int n, int dtS,
@@ -523,8 +698,9 @@ extern "C" int dt_gram_noalloc(dt_ctx *ctx, const void *A, int dtA, int ndim, co
         if (e != hipSuccess) st = dt_fail(ctx, DT_EHIP, "dt_gram_noalloc: %s", hipGetErrorString(e));
     } else {
         int64_t nsplit, cps;
-        gram_plan(I, F * B, (int64_t)((cap - tmp_bytes) / part_bytes), nsplit, cps);
-        st = gram_mfma_run(ctx, A, dtA, F, I, B, Sd, (float *)(base + tmp_bytes), nsplit, cps);
+        const bool big = gram_big_fits(F, I, F * B, A);
+        gram_plan(I, F * B, (int64_t)((cap - tmp_bytes) / part_bytes), big, nsplit, cps);
+        st = gram_mfma_run(ctx, A, dtA, F, I, B, Sd, (float *)(base + tmp_bytes), nsplit, cps, big);
     }
     if (st == DT_OK && dtS == DT_F32) st = dt_convert(ctx, S, DT_F32, Sd, DT_F64, (size_t)(I * I));
     ctx->timing = timing;

@@ -202,6 +202,30 @@ def test_gram_vs_oracle(ctx, shape, n):
     assert rel(S64, ref) < 1e-13
 
 
+@pytest.mark.parametrize("shape,n", [((256, 300, 256), 0), ((256, 300, 256), 1), ((256, 300, 256), 2),
+                                     ((520, 160, 420), 0), ((64, 512, 1100), 1), ((128, 512, 1024), 2)])
+@pytest.mark.parametrize("bf16", [False, True])
+def test_gram_panels_vs_oracle(ctx, shape, n, bf16):
+    """Orders >= 256 over >= 65536 columns take the 256-row panel kernel:
+    mode 0 (columns contiguous: column layout in LDS), other modes with
+    F % 32 == 0 (row layout), orders padded to whole panels (300, 520), one
+    and several panels (diagonal and off-diagonal blocks).  FP32 MFMA sums of
+    at most a few thousand terms per split, summed in FP64: 1e-5 relative."""
+    print("gram panels", shape, n, "bf16", bf16, flush=True)
+    rng = np.random.default_rng(17 + n)
+    A = rng.standard_normal(shape).astype(np.float32)
+    if bf16:
+        Ab = dt.bf16_from_f32(A)
+        dA = ctx.to_device(Ab)
+        A = dt.f32_from_bf16(Ab)
+    else:
+        dA = ctx.to_device(A)
+    S = ctx.gram(dA, n).download()
+    ref = D.gram(A.astype(np.float64), n)
+    assert rel(S, ref) < 1e-5
+    assert np.array_equal(S, S.T)
+
+
 def test_ttt_except_vs_oracle(ctx):
     rng = np.random.default_rng(8)
     A = rng.standard_normal((6, 7, 5))
