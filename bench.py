@@ -162,24 +162,49 @@ def mixed_1024x3(ctx, dt, shape=(1024,) * 3, ranks=(64,) * 3, noise=1e-3, decay=
         return ms
 
     fit_mixed_reported = []
+    accepted = []
 
     def run_mixed():
+        """learned start, its BF16 sweeps, the acceptance test of
+        DT_FLAG_LEARNED_ACCEPT (one more sweep must move the fit by less than
+        1e-4: HOOI from the learned start has converged, else the HOSVD start
+        is taken) and the fit on the FP32 copy -- all inside the timed region"""
         ctx.sync()
         t0 = time.perf_counter()
         s = ctx.hooi_session(A16, ranks, init=dt.INIT_LEARNED, model=model.handle, A_f32=A32, flags=kron,
                              seed=seed)
-        for _ in range(k_mixed):
-            s.sweep()
+        r = [s.sweep() for _ in range(k_mixed)]
+        r.append(s.sweep())
+        ok = abs(r[-1] - r[-2]) < dt.LEARNED_ACCEPT_FIT
+        accepted.append(ok)
+        if not ok:
+            s.close()
+            s = ctx.hooi_session(A16, ranks, init=dt.INIT_HOSVD, A_f32=A32)
+            for _ in range(k_mixed):
+                s.sweep()
         fit_mixed_reported.append(1.0 - s.fit(fp64=False))
         ctx.sync()
         ms = (time.perf_counter() - t0) * 1e3
         s.close()
         return ms
 
+    def run_mixed_hosvd_checked():
+        """the same with the HOSVD comparison of DT_FLAG_LEARNED_CHECK: the
+        learned start is kept only when its first sweep fits at least as well
+        as the first sweep from the HOSVD start (that start is computed too)"""
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.hooi_dense(A16, ranks, max_iter=k_mixed, init=dt.INIT_LEARNED, model=model.handle, A_f32=A32,
+                       flags=kron | dt.FLAG_LEARNED_CHECK, seed=seed, want_factors=False)
+        ctx.sync()
+        return (time.perf_counter() - t0) * 1e3
+
     run_plain()
     run_mixed()  # first launches of every kernel
     plain_ms = _median_ms(run_plain)
     mixed_ms = _median_ms(run_mixed)
+    run_mixed_hosvd_checked()
+    checked_ms = _median_ms(run_mixed_hosvd_checked)
     # breakdown of one mixed run
     ctx.timing(True)
     ctx.timing_reset()
@@ -201,10 +226,14 @@ def mixed_1024x3(ctx, dt, shape=(1024,) * 3, ranks=(64,) * 3, noise=1e-3, decay=
                           "max_sin_angle_vs_true": max(angle(a, b) for a, b in zip(U_plain, Ut))},
            "mixed_learned": {"sweeps_bf16": k_mixed, "ms": mixed_ms, "fit": 1.0 - res_mixed[-1],
                              "fit_reported_fp32": fit_mixed_reported[-1],
+                             "fit_error_vs_fp64": fit_mixed_reported[-1] - (1.0 - res_mixed[-1]),
+                             "learned_start_accepted": all(accepted),
+                             "ms_with_hosvd_check": checked_ms,
                              "max_sin_angle_vs_reference": max(angle(a, b) for a, b in zip(U_mixed, ref["U"])),
                              "max_sin_angle_vs_true": max(angle(a, b) for a, b in zip(U_mixed, Ut)),
                              "kernel_ms": kt},
-           "speedup": plain_ms / mixed_ms}
+           "speedup": plain_ms / mixed_ms,
+           "speedup_with_hosvd_check": plain_ms / checked_ms}
     for a in (A32, A16):
         a.free()
     log(f"[bench] time-to-fit plain FP32 {plain_ms:.1f} ms ({k_plain} sweeps), mixed+learned "

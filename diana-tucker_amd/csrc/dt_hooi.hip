@@ -33,7 +33,8 @@ struct dt_hooi {
     int N = 0;
     int64_t I[DT_MAX_NDIM] = {}, R[DT_MAX_NDIM] = {};
     int64_t total = 0;
-    double normA2 = 0.0;
+    double normA2 = 0.0;       // ||A||^2 of A_f32 when given, else of A
+    double normA2_sweep = 0.0; // ||A||^2 of the tensor the sweeps read (A itself)
     // factors and their kernel views
     double *U64[DT_MAX_NDIM] = {};
     float *U32[DT_MAX_NDIM] = {};
@@ -242,7 +243,9 @@ static int ahat_sq_dev(dt_hooi *h, const void *A, int dt) {
     return dt_dot_f64_dev(ctx, cur, h->G64, (size_t)csz, h->dscal);
 }
 
-static int core_residual(dt_hooi *h, const void *A, int dt, double *res) {
+// sqrt(1 - ||G||^2 / normA2): normA2 must be the squared norm of the tensor the
+// core was formed from
+static int core_residual(dt_hooi *h, const void *A, int dt, double normA2, double *res) {
     dt_ctx *ctx = h->ctx;
     DT_TRY(ahat_sq_dev(h, A, dt));
     // one read-back per sweep: ||Ahat||^2 and the breakdown flags
@@ -259,7 +262,7 @@ static int core_residual(dt_hooi *h, const void *A, int dt, double *res) {
                        (flag & 8) ? "singular factor Gram; " : "");
     }
     h->G_norm = std::sqrt(g2);
-    const double ratio = h->normA2 > 0.0 ? g2 / h->normA2 : 1.0;
+    const double ratio = normA2 > 0.0 ? g2 / normA2 : 1.0;
     *res = std::sqrt(std::max(0.0, 1.0 - ratio));
     return DT_OK;
 }
@@ -526,6 +529,9 @@ int dt_hooi_create(dt_ctx *ctx, const dt_hooi_args *args, dt_hooi **out) {
     st = dt_sumsq_dev(ctx, nsrc, ndt, (size_t)h->total, &h->normA2);
     if (st == DT_OK && !(h->normA2 > 0.0 && std::isfinite(h->normA2)))
         st = dt_fail(ctx, DT_ENUMERIC, "hooi: ||A|| is zero or not finite");
+    h->normA2_sweep = h->normA2;
+    if (st == DT_OK && args->A_f32 && args->dtype != DT_F32)
+        st = dt_sumsq_dev(ctx, args->A, args->dtype, (size_t)h->total, &h->normA2_sweep);
     if (st == DT_OK) {
         hipEventRecord(ctx->t0, ctx->stream);
         st = init_factors(h);
@@ -561,7 +567,11 @@ int dt_hooi_sweep(dt_hooi *h, int fp32, double *residual) {
     double res = 0.0;
     const void *fitA = h->args.A_f32 ? (const void *)h->args.A_f32 : A;
     const int fitdt = h->args.A_f32 ? DT_F32 : dt;
-    if (st == DT_OK) st = core_residual(h, fitA, fitdt, &res);
+    // with DT_FLAG_FP64_FIT the core is formed again from fitA; otherwise it is
+    // the sweep's own, from the tensor the sweep read
+    const bool refit = (h->args.flags & DT_FLAG_FP64_FIT) != 0;
+    const double n2 = refit || A != h->args.A ? h->normA2 : h->normA2_sweep;
+    if (st == DT_OK) st = core_residual(h, fitA, fitdt, n2, &res);
     hipEventRecord(e1, ctx->stream);
     hipEventSynchronize(e1);
     float ms = 0.f;
@@ -595,7 +605,7 @@ int dt_hooi_fit(dt_hooi *h, int fp64, double *residual) {
         h->args.flags &= ~DT_FLAG_FP64_FIT;
         DT_TRY(core_from(h, fitA, fitdt));
     }
-    const int st = core_residual(h, fitA, fitdt, residual);
+    const int st = core_residual(h, fitA, fitdt, h->normA2, residual);
     h->args.flags = saved;
     if (st == DT_OK) h->last_res = *residual;
     return st;
@@ -608,7 +618,8 @@ int dt_hooi_get(dt_hooi *h, dt_hooi_result *res) {
         DT_TRY(core_only(h));
         double r = 0.0;
         const void *fitA = h->args.A_f32 ? (const void *)h->args.A_f32 : h->args.A;
-        DT_TRY(core_residual(h, fitA, h->args.A_f32 ? DT_F32 : h->args.dtype, &r));
+        const double n2 = (h->args.flags & DT_FLAG_FP64_FIT) ? h->normA2 : h->normA2_sweep;
+        DT_TRY(core_residual(h, fitA, h->args.A_f32 ? DT_F32 : h->args.dtype, n2, &r));
         h->last_res = r;
     }
     int64_t csz = 1;
@@ -640,8 +651,13 @@ int dt_hooi_dense(dt_ctx *ctx, const dt_hooi_args *args, dt_hooi_result *res) {
         // that sweep's fit is not a number in (0, 1], and, with
         // DT_FLAG_LEARNED_CHECK, when it is worse than the first sweep from the
         // HOSVD start.
-        double rl = 0.0;
+        double rl = 0.0, rl2 = -1.0;
         st = dt_hooi_sweep(h, polish && args->max_iter == 1, &rl);
+        // acceptance: one more sweep moves the fit by less than the fit
+        // tolerance, i.e. HOOI from the learned start has converged to that
+        // tolerance after its first sweep; otherwise the HOSVD start is taken.
+        // With max_iter >= 2 that sweep is the run's own second sweep.
+        if (st == DT_OK && (args->flags & DT_FLAG_LEARNED_ACCEPT)) st = dt_hooi_sweep(h, 0, &rl2);
         if (st != DT_OK) {
             dt_hooi_destroy(h);
             return st;
@@ -649,6 +665,7 @@ int dt_hooi_dense(dt_ctx *ctx, const dt_hooi_args *args, dt_hooi_result *res) {
         res->learned_first_res = rl;
         res->hosvd_first_res = -1.0;
         bool fallback = !(rl >= 0.0 && rl < 1.0);
+        if ((args->flags & DT_FLAG_LEARNED_ACCEPT) && !(std::fabs(rl2 - rl) < DT_LEARNED_ACCEPT_FIT)) fallback = true;
         dt_hooi *hh = nullptr;
         double rh = 0.0;
         if (fallback || (args->flags & DT_FLAG_LEARNED_CHECK)) {
@@ -675,6 +692,12 @@ int dt_hooi_dense(dt_ctx *ctx, const dt_hooi_args *args, dt_hooi_result *res) {
         if (res->residual_per_sweep) res->residual_per_sweep[0] = r;
         prev = r;
         first = 1;
+        if (!fallback && rl2 >= 0.0 && args->max_iter >= 2) {
+            // the acceptance sweep is the run's second sweep
+            if (res->residual_per_sweep) res->residual_per_sweep[1] = rl2;
+            prev = rl2;
+            first = 2;
+        }
     }
     for (int it = first; it < args->max_iter; it++) {
         const bool last = it == args->max_iter - 1;

@@ -20,7 +20,8 @@ DT_MAX_NDIM = 8
 INIT_GIVEN, INIT_QR, INIT_HOSVD, INIT_RANDOM, INIT_LEARNED = range(5)
 SCHED_REFERENCE, SCHED_TREE = 0, 1
 FLAG_DIRECT_RESIDUAL, FLAG_FP32_POLISH, FLAG_FP64_FIT, FLAG_REFERENCE_ALS = 1, 2, 4, 8
-FLAG_LEARNED_CHECK, FLAG_SKETCH_KRON = 16, 32
+FLAG_LEARNED_CHECK, FLAG_SKETCH_KRON, FLAG_LEARNED_ACCEPT = 16, 32, 64
+LEARNED_ACCEPT_FIT = 1e-4
 SKETCH_GAUSSIAN, SKETCH_KRONECKER = 0, 1
 
 _NP_OF = {DT_F64: np.float64, DT_F32: np.float32, DT_BF16: np.uint16, DT_I32: np.int32}

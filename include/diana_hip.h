@@ -340,8 +340,14 @@ enum {
                                     core dominates 1 - ||G||^2/||A||^2 */
     DT_FLAG_LEARNED_CHECK = 16,  /* DT_INIT_LEARNED: also run the first sweep from the HOSVD
                                     start and keep whichever fits better (the fallback) */
-    DT_FLAG_SKETCH_KRON = 32     /* DT_INIT_LEARNED: Kronecker sketch (model input q^(N-1)) */
+    DT_FLAG_SKETCH_KRON = 32,    /* DT_INIT_LEARNED: Kronecker sketch (model input q^(N-1)) */
+    DT_FLAG_LEARNED_ACCEPT = 64  /* DT_INIT_LEARNED: accept the learned start only when the sweep
+                                    after its first one moves the fit by less than
+                                    DT_LEARNED_ACCEPT_FIT (HOOI from it has converged to the fit
+                                    tolerance), else fall back to the HOSVD start; costs one sweep
+                                    (none when max_iter >= 2), not a HOSVD */
 };
+#define DT_LEARNED_ACCEPT_FIT 1e-4
 
 typedef struct dt_hooi_args {
     int32_t ndim;

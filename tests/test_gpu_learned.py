@@ -187,6 +187,35 @@ def test_learned_init_falls_back_to_hosvd_for_a_bad_model(ctx):
         m.close()
 
 
+def test_learned_start_acceptance_rule(ctx):
+    """DT_FLAG_LEARNED_ACCEPT (no HOSVD computed unless the test fails): from
+    the sketch-range start of an untrained network HOOI has converged after
+    one sweep (the second sweep moves the fit by less than 1e-4) and the start
+    is kept; from the start of a network with huge random weights and one
+    inner ALS step per mode it has not, and the HOSVD start is taken instead --
+    whose run then matches the HOSVD-started HOOI."""
+    shape, ranks = (64, 64, 64), (8, 8, 8)
+    g = ctx.gen_tucker(shape, ranks, 1e-3, seed=45, decay=0.8)
+    m = dt.MLP(ctx, 16, 64, 8, seed=0)
+    try:
+        good = ctx.hooi_dense(g["A"], ranks, max_iter=2, init=dt.INIT_LEARNED, model=m.handle,
+                              flags=dt.FLAG_LEARNED_ACCEPT | dt.FLAG_FP64_FIT, seed=3)
+        assert good["init_used"] == dt.INIT_LEARNED
+        rng = np.random.default_rng(0)
+        m.set_params(rng.standard_normal(m.nparam).astype(np.float32) * 5.0)
+        # noise 0.1, slow decay: from that start the second sweep still moves the fit by ~4e-4
+        g2 = ctx.gen_tucker(shape, ranks, 1e-1, seed=45, decay=0.95)
+        bad = ctx.hooi_dense(g2["A"], ranks, max_iter=2, init=dt.INIT_LEARNED, model=m.handle,
+                             flags=dt.FLAG_LEARNED_ACCEPT | dt.FLAG_FP64_FIT, seed=3, inner_iter=1)
+        ref = ctx.hooi_dense(g2["A"], ranks, max_iter=2, init=dt.INIT_HOSVD, flags=dt.FLAG_FP64_FIT, inner_iter=1)
+        print("accept: good", good["init_used"], "bad", bad["init_used"], bad["residuals"], ref["residuals"],
+              flush=True)
+        assert bad["init_used"] == dt.INIT_HOSVD
+        assert abs(bad["residual"] - ref["residual"]) < 1e-9
+    finally:
+        m.close()
+
+
 def test_learned_init_without_model_is_einval(ctx):
     dA = ctx.to_device(np.ones((8, 8, 8), np.float32))
     with pytest.raises(dt.DTError) as e:
