@@ -328,25 +328,35 @@ def measured_traffic(name):
     return None
 
 
-def cpu_baseline_dense(sample=(96, 96, 96, 96), ranks=(16,) * 4, full=(256,) * 4):
-    """The oracle (numpy FP64 restatement of the reference HOOI_ALS with its
-    N+1 full TTM chains and 5-step ALS_) timed for one sweep on a bounded
-    sample of the workload; scaled to full-size sweeps/s by element count.
-    The sweep's cost does not depend on the values, so the sample is N(0,1)."""
-    import numpy as np
-    from oracle import dense as D
-    A = np.random.default_rng(20000905).standard_normal(sample)
-    U0 = [np.linalg.qr(np.random.default_rng(n).standard_normal((I, R)))[0]
-          for n, (I, R) in enumerate(zip(sample, ranks))]
-    t0 = time.perf_counter()
-    D.hooi_als(A, ranks, 1, U0)
-    dt_s = time.perf_counter() - t0
-    scale = math.prod(full) / math.prod(sample)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-    return {"value": 1.0 / (dt_s * scale), "unit": "iters/s", "cores": threads, "kind": "port",
-            "sample": f"1 sweep of the FP64 numpy oracle (reference schedule) on {sample} "
-                      f"ranks {ranks} = {dt_s:.2f} s, scaled x{scale:.0f} by element count "
-                      f"to the {full} workload"}
+CPU_HOOI = os.path.join(ROOT, "oracle", "bin", "cpu_hooi")
+
+
+def cpu_baseline_dense(shape=(256,) * 4, ranks=(16,) * 4):
+    """The C++/OpenMP FP64 restatement of the reference's HOOI_ALS schedule
+    (oracle/cpu_hooi.cpp: A.copy(), unfold copy -> blocked DGEMM -> fold per
+    product, 5-step ALS_) on the host cores at the workload's full size: its
+    two kinds of chain are timed as the sweep runs them and the sweep is
+    composed as chain 0 + N x chain 1 (a whole FP64 sweep at 256^4 would take
+    minutes).  The tensor is N(0,1): the cost does not depend on the values.
+    Also reports the host, the thread count and the restatement's own DGEMM
+    GFLOP/s at 4096^3."""
+    import subprocess
+    if not os.path.exists(CPU_HOOI):
+        log(f"[bench] {CPU_HOOI} missing (make -C oracle): no CPU baseline")
+        return None
+    env = dict(os.environ)
+    env.setdefault("OMP_PROC_BIND", "close")
+    cmd = [CPU_HOOI, "pieces", "--shape", ",".join(map(str, shape)), "--ranks", ",".join(map(str, ranks)),
+           "--dgemm", "4096"]
+    out = subprocess.run(cmd, check=True, capture_output=True, text=True, env=env).stdout
+    r = json.loads(out.strip().splitlines()[-1])
+    sweep_s = r["sweep_s_composed"]
+    return {"value": 1.0 / sweep_s, "unit": "iters/s", "cores": r["threads"], "kind": "port",
+            "host": r["host"], "nproc": r["nproc"], "dgemm_gflops_4096": r["dgemm_gflops"],
+            "sample": f"C++/OpenMP FP64 restatement of the reference schedule at {shape} ranks {ranks}: "
+                      f"chains 0 and 1 of a sweep timed at full size ({r['chain_s'][0]:.2f} s, {r['chain_s'][1]:.2f} s), "
+                      f"sweep = chain 0 + {len(shape)} x chain 1 = {sweep_s:.2f} s on {r['threads']} threads",
+            "chain1_parts_s": r["parts_chain1_s"], "command": " ".join(["oracle/bin/cpu_hooi"] + cmd[1:])}
 
 
 def main():
@@ -385,7 +395,7 @@ def main():
     achieved = dom.get("GB_per_s")
     cpu = None
     if not args.no_cpu:
-        log("[bench] CPU baseline (oracle) ...")
+        log("[bench] CPU baseline (C++/OpenMP restatement of the reference schedule) ...")
         cpu = cpu_baseline_dense()
         log(f"[bench] CPU baseline {cpu['value']:.4g} iters/s")
     line = {
