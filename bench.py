@@ -397,7 +397,8 @@ def main():
     if not args.no_cpu:
         log("[bench] CPU baseline (C++/OpenMP restatement of the reference schedule) ...")
         cpu = cpu_baseline_dense()
-        log(f"[bench] CPU baseline {cpu['value']:.4g} iters/s")
+        if cpu:
+            log(f"[bench] CPU baseline {cpu['value']:.4g} iters/s")
     line = {
         "metric": "HOOI iters/s (dense 256^4 FP32, ranks 16^4)",
         "value": value,

@@ -419,3 +419,25 @@ def test_sparse_hooi_oracle_recovers_low_rank_4way():
     for n in range(4):
         assert D.sin_max_angle(Us[n], Ut[n]) < 1e-6
     assert np.isclose(np.linalg.norm(Gc), np.linalg.norm(A), rtol=1e-10)
+
+
+def test_cpu_restatement_matches_oracle(tmp_path):
+    """oracle/cpu_hooi.cpp (the C++/OpenMP restatement timed as the CPU
+    baseline) gives the numpy oracle's residual per sweep from the same tensor
+    and start: 3- and 4-way, ranks below and at a third of the mode sizes."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "..", "oracle", "bin", "cpu_hooi")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-s", "-C", os.path.join(os.path.dirname(__file__), "..", "oracle")], check=True)
+    rng = np.random.default_rng(3)
+    for shape, ranks in (((24, 20, 16), (4, 3, 5)), ((12, 10, 9, 8), (3, 4, 2, 3)), ((40, 35, 30), (6, 6, 6))):
+        A = rng.standard_normal(shape)
+        U0 = [np.linalg.qr(rng.standard_normal((I, R)))[0] for I, R in zip(shape, ranks)]
+        fa, fu = tmp_path / "A.bin", tmp_path / "U.bin"
+        A.ravel(order="F").tofile(fa)
+        np.concatenate([u.ravel(order="F") for u in U0]).tofile(fu)
+        out = subprocess.run([exe, "check", "--shape", ",".join(map(str, shape)), "--ranks", ",".join(map(str, ranks)),
+                              "--sweeps", "3", "--tensor", str(fa), "--start", str(fu)],
+                             capture_output=True, text=True, check=True).stdout.split()
+        _, _, rr = D.hooi_als(A, ranks, 3, U0)
+        assert np.allclose([float(x) for x in out], rr, rtol=1e-12), (shape, out, rr)
