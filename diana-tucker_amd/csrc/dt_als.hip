@@ -780,13 +780,36 @@ static int rows(dt_ctx *ctx, int mode, const double *S, const double *Y, int I, 
     return launch_rows<64>(ctx, mode, S, Y, I, R, Rinv, out, part, U32, M32, Ub16);
 }
 
+// Rank check of the Q-only Cholesky-QR3: diag(R) of X = Q R is the product of
+// the three passes' diagonals 1 / Rinv_k[j][j].  d[j] accumulates it; after the
+// last pass a column whose diagonal is at most 10 R u ||X||_F (||X||_F^2 = the
+// trace of the first Gram, in d[R]) means X is rank-deficient to working
+// precision: ORs 4 into *flag.  One thread.
+__global__ void cholqr_rank_track(const double *__restrict__ Rinv, const double *__restrict__ G0, int R, int pass,
+                                  double *__restrict__ d, int *__restrict__ flag) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (pass == 0) {
+        double tr = 0.0;
+        for (int j = 0; j < R; j++) tr += G0[j + (size_t)R * j];
+        d[R] = tr;
+    }
+    bool bad = false;
+    for (int j = 0; j < R; j++) {
+        const double rj = 1.0 / Rinv[j + (size_t)R * j];
+        d[j] = pass == 0 ? rj : d[j] * rj;
+        if (pass == 2 && !(d[j] > 10.0 * R * 1.1102230246251565e-16 * sqrt(d[R]))) bad = true;
+    }
+    if (bad) atomicOr(flag, 4);
+}
+
 // Q <- the orthonormal factor of X (I x R FP64, R <= 64) with positive R
 // diagonal: shifted Cholesky-QR, then two plain passes (Fukaya et al. 2020),
 // each pass one als_chol (partial Grams summed, R^{-1}) and one als_rows
 // (X <- X R^{-1} in place) that also leaves the next pass's partial Grams.
-// Nothing waits for the host; a Cholesky breakdown ORs 4 into *flag.
-// ws: (ceil(I / 128) + 1) R^2 doubles.
-int dt_cholqr3_rows(dt_ctx *ctx, double *X, int I, int R, double *ws, int *flag) {
+// Nothing waits for the host; a Cholesky breakdown ORs 4 into *flag, and with
+// rank_check so does a rank-deficient X (cholqr_rank_track).
+// ws: (ceil(I / 128) + 2) R^2 + R + 1 doubles.
+int dt_cholqr3_rows(dt_ctx *ctx, double *X, int I, int R, double *ws, int *flag, bool rank_check) {
     if (R < 1 || R > ALS_MAXR) return dt_fail(ctx, DT_EUNSUPPORTED, "Cholesky-QR: %d columns > %d", R, ALS_MAXR);
     const int nb = (I + ALS_GR - 1) / ALS_GR;
     double *part = ws, *ri = ws + (size_t)nb * R * R;
@@ -797,8 +820,13 @@ int dt_cholqr3_rows(dt_ctx *ctx, double *X, int I, int R, double *ws, int *flag)
     } else {
         DT_LAUNCH("als_gram", als_gram_kernel<64>, dim3(nb), dim3(ALS_THREADS), 0, X, I, R, part);
     }
+    double *G0 = ri + (size_t)R * R, *dtrack = G0 + (size_t)R * R;
+    if (rank_check) DT_TRY(dt_part_reduce(ctx, part, nb, R * R, G0));
     for (int pass = 0; pass < 3; pass++) {
         DT_TRY(launch_chol(ctx, part, nb, R, I, pass == 0 ? 1 : 0, ri, flag));
+        if (rank_check)
+            DT_LAUNCH("cholqr_small", cholqr_rank_track, dim3(1), dim3(64), 0, (const double *)ri, (const double *)G0, R,
+                      pass, dtrack, flag);
         DT_TRY(rows(ctx, 1, nullptr, X, I, R, ri, X, pass < 2 ? part : nullptr));
     }
     return DT_OK;

@@ -490,12 +490,13 @@ extern "C" int dt_qr_f64(dt_ctx *ctx, double *Q, double *R, const double *A, siz
     // Cholesky and runs a tiled GEMM with few tiles at these shapes)
     if (!R && n > 16 && n <= 64 && m >= 256 && m < ((size_t)1 << 31) && !(n <= (size_t)TS_NMAX && m >= 4096)) {
         const size_t nb = (m + 127) / 128;
-        double *ws = (double *)dt_ws(ctx, WS_CHOLQR, ((nb + 1) * nn + 8) * sizeof(double));
+        const size_t wsd = (nb + 2) * nn + n + 8;
+        double *ws = (double *)dt_ws(ctx, WS_CHOLQR, (wsd + 8) * sizeof(double));
         if (!ws) return DT_ENOMEM;
-        int *dflag = (int *)(ws + (nb + 1) * nn);
+        int *dflag = (int *)(ws + wsd);
         if (Q != A) DT_HIP(hipMemcpyAsync(Q, A, m * n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         DT_TRY(dt_zero(ctx, dflag, sizeof(int)));
-        DT_TRY(dt_cholqr3_rows(ctx, Q, (int)m, (int)n, ws, dflag));
+        DT_TRY(dt_cholqr3_rows(ctx, Q, (int)m, (int)n, ws, dflag, true));
         int *pin = (int *)ctx->pinned;
         DT_HIP(hipMemcpyAsync(pin, dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         DT_HIP(hipStreamSynchronize(ctx->stream));

@@ -242,9 +242,10 @@ int dt_chol_rinv_ws(dt_ctx *ctx, double *G, int n, bool shifted, double m_rows, 
 int dt_part_reduce(dt_ctx *ctx, const double *part, int nparts, int ne, double *out);
 size_t dt_als_ws_doubles(size_t I, size_t R);
 // X (I x R, R <= 64) <- its orthonormal QR factor (positive R diagonal) by
-// shifted Cholesky-QR3 over many workgroups; no host wait, breakdown ORs 4
-// into *flag; ws: (ceil(I / 128) + 1) R^2 doubles
-int dt_cholqr3_rows(dt_ctx *ctx, double *X, int I, int R, double *ws, int *flag);
+// shifted Cholesky-QR3 over many workgroups; no host wait, breakdown (and, with
+// rank_check, a rank-deficient X) ORs 4 into *flag;
+// ws: (ceil(I / 128) + 2) R^2 + R + 1 doubles
+int dt_cholqr3_rows(dt_ctx *ctx, double *X, int I, int R, double *ws, int *flag, bool rank_check = false);
 int dt_als_update(dt_ctx *ctx, const double *S, double *U, int I, int R, int inner, double *ws,
                   int *flag, float *U32, float *M32, uint16_t *Ub16);
 int dt_als_fused(dt_ctx *ctx, const double *S, double *U, int I, int R, int inner, double *L,

@@ -297,6 +297,30 @@ def test_qr_f64_q_only_non_finite_is_enumeric(ctx):
     Q, _ = ctx.qr(ctx.to_device(rng.standard_normal((600, 40))), want_r=False)  # the context still works
 
 
+@pytest.mark.parametrize("n,dup", [(40, (7, 8)), (64, (0, 63)), (20, (3, 3))])
+def test_qr_f64_q_only_rank_deficient_is_enumeric(ctx, n, dup):
+    """A duplicated column (or, for (3, 3), a zero column) makes the input
+    rank-deficient: the Q-only multi-workgroup Cholesky-QR3 reports DT_ENUMERIC
+    (the product of its passes' R diagonals falls below 10 n u ||A||_F) instead
+    of a basis; an input of condition 1e9 still passes (the shifted-pass case)."""
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((700, n))
+    i, j = dup
+    if i == j:
+        A[:, i] = 0.0
+    else:
+        A[:, j] = A[:, i]
+    with pytest.raises(dt.DTError) as e:
+        ctx.qr(ctx.to_device(A), want_r=False)
+    assert e.value.code == dt.DT_ENUMERIC
+    U, _ = np.linalg.qr(rng.standard_normal((700, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    B = U @ np.diag(np.logspace(0, -9, n)) @ V.T
+    Q, _ = ctx.qr(ctx.to_device(B), want_r=False)
+    Qh = Q.download()
+    assert np.abs(Qh.T @ Qh - np.eye(n)).max() < 1e-12
+
+
 @pytest.mark.parametrize("m", [500, 6000])
 def test_qr_ill_conditioned_uses_shift(ctx, m):
     """Condition 1e9 breaks the plain Cholesky-QR; the shifted pass recovers
