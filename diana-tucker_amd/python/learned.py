@@ -15,7 +15,7 @@ T = U* U*ᵀ X[:, :R].  This fixes a basis of span(U*) in the input's own
 frame, so a row-wise network can be trained with a squared loss.
 
     python learned.py train [--config tucker128_r16|tucker256_r64] [--n-train N] [--epochs E] [--model PATH]
-    python learned.py eval  [--config tucker128_r16|tucker256_r64] [--n-eval N] [--model PATH] [--report PATH]
+    python learned.py eval  [--config tucker128_r16|tucker256_r64] [--n-eval N] [--model PATH] [--report PATH] [--rows PATH]
 """
 import argparse
 import json
@@ -89,7 +89,14 @@ def make_examples(ctx, ids, cfg, sweeps=12, log=print):
 
 
 def train(ctx, X, T, cfg, epochs=8, batch=16384, lr=2e-4, seed=0, Xv=None, Tv=None, log=print):
-    """Adam on the device; cosine learning-rate decay; returns (model, history)."""
+    """Adam on the device; cosine learning-rate decay; returns (model, history).
+
+    With validation rows (Xv, Tv) the parameters of the epoch whose starts
+    are closest to the targets' subspaces on the validation tensors are kept
+    (subspace_error; epoch 0, the untrained network whose start is the sketch
+    range itself, included), so the returned model is never worse there than
+    the method without it.  The squared loss alone is no such measure: a
+    per-column rescaling lowers it and leaves every start subspace as it was."""
     R = cfg["ranks"][0]
     m = dt.MLP(ctx, cfg["S"], cfg["hidden"], R, seed=seed)
     rows = X.shape[0]
@@ -99,9 +106,12 @@ def train(ctx, X, T, cfg, epochs=8, batch=16384, lr=2e-4, seed=0, Xv=None, Tv=No
     hist = []
     dXv = dt.upload_rows(ctx, Xv) if Xv is not None else None
     dTv = dt.upload_rows(ctx, Tv) if Tv is not None else None
+    best = None
     if dXv is not None:
-        hist.append({"epoch": 0, "val": m.loss_grad(dXv, dTv, Xv.shape[0])})
-        log(f"[learned] epoch 0 val {hist[-1]['val']:.5f}")
+        hist.append({"epoch": 0, "val": m.loss_grad(dXv, dTv, Xv.shape[0]),
+                     "val_subspace": subspace_error(m, dXv, Tv, cfg)})
+        best = (hist[-1]["val_subspace"], 0, m.get_params())
+        log(f"[learned] epoch 0 val {hist[-1]['val']:.7g} subspace {hist[-1]['val_subspace']:.7g}")
     step = 0
     for ep in range(epochs):
         perm = rng.permutation(rows)
@@ -119,9 +129,41 @@ def train(ctx, X, T, cfg, epochs=8, batch=16384, lr=2e-4, seed=0, Xv=None, Tv=No
         h = {"epoch": ep + 1, "train": tl / nb}
         if dXv is not None:
             h["val"] = m.loss_grad(dXv, dTv, Xv.shape[0])
+            h["val_subspace"] = subspace_error(m, dXv, Tv, cfg)
+            if h["val_subspace"] < best[0]:
+                best = (h["val_subspace"], ep + 1, m.get_params())
         hist.append(h)
-        log(f"[learned] epoch {ep + 1}: train {h['train']:.5f}" + (f" val {h['val']:.5f}" if "val" in h else ""))
+        log(f"[learned] epoch {ep + 1}: train {h['train']:.7g}" +
+            (f" val {h['val']:.7g} subspace {h['val_subspace']:.7g}" if "val" in h else ""))
+    if best is not None:
+        m.set_params(best[2])
+        hist.append({"kept_epoch": best[1], "val_subspace": best[0]})
+        log(f"[learned] kept epoch {best[1]} (subspace {best[0]:.7g})")
+    if dXv is not None:
+        dXv.free()
+        dTv.free()
     return m, hist
+
+
+def subspace_error(m, dX, T, cfg):
+    """Mean over the (tensor, mode) blocks of the rows (I_n rows each, in the
+    order make_examples writes them) of sum_k sin^2(theta_k), the principal
+    angles between the span of the network's output block and the span of
+    its target block (= span(U*))."""
+    rows, R = T.shape
+    Z = m.forward(dX, rows)
+    Zh = dt.download_rows(Z).astype(np.float64)
+    Z.free()
+    sizes = list(cfg["shape"])
+    out, r0, b = [], 0, 0
+    while r0 < rows:
+        I = sizes[b % len(sizes)]
+        qz = np.linalg.qr(Zh[r0:r0 + I])[0]
+        qt = np.linalg.qr(T[r0:r0 + I].astype(np.float64))[0]
+        out.append(R - float(np.sum((qz.T @ qt) ** 2)))
+        r0 += I
+        b += 1
+    return float(np.mean(out))
 
 
 class _RowsView:
@@ -145,10 +187,12 @@ def load_model(ctx, path):
 
 
 def sweeps_to_fit(ctx, A, cfg, init, model=None, max_sweeps=10, target=None, seed=0):
-    """Per-sweep residuals (FP64-accumulated core) from `init`; the first sweep
-    whose fit is within 1e-4 of the target fit (or None)."""
+    """Residual of the start itself (core of the start factors, FP64
+    accumulated), per-sweep residuals from `init`, and the first sweep whose
+    fit is within 1e-4 of the target fit (or None)."""
     s = ctx.hooi_session(A, cfg["ranks"], init=init, model=model.handle if model else None, seed=seed,
                          flags=dt.FLAG_FP64_FIT | learned_flags(cfg))
+    start = s.fit(fp64=True)
     res = [s.sweep() for _ in range(max_sweeps)]
     s.close()
     hit = None
@@ -157,7 +201,7 @@ def sweeps_to_fit(ctx, A, cfg, init, model=None, max_sweeps=10, target=None, see
             if (1.0 - r) >= target - 1e-4:
                 hit = k + 1
                 break
-    return res, hit
+    return start, res, hit
 
 
 def timed_run(ctx, A, cfg, init, sweeps, model=None, seed=0):
@@ -190,9 +234,10 @@ def evaluate(ctx, model, ids, cfg, max_sweeps=10, ref_sweeps=25, log=print):
         row = {"t": int(t), "target_fit": target}
         for name, init, mdl in (("hosvd", dt.INIT_HOSVD, None), ("sketch", dt.INIT_LEARNED, plain),
                                 ("learned", dt.INIT_LEARNED, model)):
-            res, hit = sweeps_to_fit(ctx, A, cfg, init, mdl, max_sweeps, target, seed=cfg["base_seed"] + t)
+            start, res, hit = sweeps_to_fit(ctx, A, cfg, init, mdl, max_sweeps, target, seed=cfg["base_seed"] + t)
             ms = timed_run(ctx, A, cfg, init, hit if hit else max_sweeps, mdl, seed=cfg["base_seed"] + t)
-            row[name] = {"first_res": res[0], "gap1": (1.0 - res[0]) - target, "sweeps": hit, "ms": ms}
+            row[name] = {"start_res": start, "gap0": (1.0 - start) - target, "first_res": res[0],
+                         "gap1": (1.0 - res[0]) - target, "sweeps": hit, "ms": ms}
         rows.append(row)
         A.free()
         if (k + 1) % 32 == 0:
@@ -210,6 +255,7 @@ def summarize(rows):
                      "mean_sweeps": float(np.mean(ok)) if ok else None,
                      "mean_ms": float(np.mean([r[name]["ms"] for r in rows])),
                      "median_ms": float(np.median([r[name]["ms"] for r in rows])),
+                     "mean_start_gap": float(np.mean([r[name]["gap0"] for r in rows])),
                      "mean_first_gap": float(np.mean([r[name]["gap1"] for r in rows]))}
     return out
 
@@ -223,7 +269,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--lr", type=float, default=2e-4)
     ap.add_argument("--model", default=None)
-    ap.add_argument("--report", default=None)
+    ap.add_argument("--report", default=None, help="report path (default: models/learned_<config>_report.json)")
+    ap.add_argument("--rows", default=None, help="also write every evaluated tensor's row here")
     args = ap.parse_args()
     cfg = dict(CONFIGS[args.config])
     if args.n_train is None:
@@ -235,8 +282,11 @@ def main():
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     with dt.Context(0) as ctx:
         if args.what == "train":
-            tr = list(range(args.n_train))
-            va = list(range(cfg["n_train"], cfg["n_train"] + min(64, cfg["n_total"] - cfg["n_train"])))
+            # validation tensors are the last of the training ids, so the
+            # held-out ids (n_train and up) stay unseen by the early stop
+            n_val = min(64, args.n_train // 6)
+            tr = list(range(args.n_train - n_val))
+            va = list(range(args.n_train - n_val, args.n_train))
             t0 = time.time()
             X, T = make_examples(ctx, tr, cfg, log=log)
             Xv, Tv = make_examples(ctx, va, cfg, log=log)
@@ -244,7 +294,7 @@ def main():
             m, hist = train(ctx, X, T, cfg, epochs=args.epochs, lr=args.lr, Xv=Xv, Tv=Tv, log=log)
             t2 = time.time()
             meta = {"cfg": {k: list(v) if isinstance(v, tuple) else v for k, v in cfg.items()},
-                    "n_train": args.n_train, "epochs": args.epochs, "lr": args.lr, "history": hist,
+                    "n_train": args.n_train, "n_val": n_val, "epochs": args.epochs, "lr": args.lr, "history": hist,
                     "data_s": t1 - t0, "train_s": t2 - t1}
             save_model(m, args.model, meta)
             print(json.dumps(meta))
@@ -252,12 +302,23 @@ def main():
         else:
             m = load_model(ctx, args.model)
             ids = list(range(cfg["n_train"], cfg["n_train"] + args.n_eval))
+            evaluate(ctx, m, ids[:1], cfg, max_sweeps=2, ref_sweeps=2, log=lambda *a: None)  # first launches
             rows = evaluate(ctx, m, ids, cfg, log=log)
             summ = summarize(rows)
             print(json.dumps({"summary": summ, "n_eval": len(ids)}))
-            if args.report:
-                with open(args.report, "w") as f:
-                    json.dump({"summary": summ, "rows": rows}, f)
+            # the model's report: its training record, the held-out summary
+            # and the commands that made both
+            with np.load(args.model, allow_pickle=False) as f:
+                meta = json.loads(str(f["meta"]))
+            meta.update({"eval_summary": summ, "n_eval": len(ids),
+                         "command_train": f"python diana-tucker_amd/python/learned.py train --config {cfg['name']}",
+                         "command_eval": f"python diana-tucker_amd/python/learned.py eval --config {cfg['name']}"})
+            report = args.report or os.path.join(MODEL_DIR, f"learned_{cfg['name']}_report.json")
+            with open(report, "w") as f:
+                json.dump(meta, f, indent=1)
+            if args.rows:
+                with open(args.rows, "w") as f:
+                    json.dump(rows, f)
             m.close()
 
 

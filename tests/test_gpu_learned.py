@@ -224,22 +224,32 @@ def test_learned_init_without_model_is_einval(ctx):
 
 
 def test_trained_model_128cube_against_hosvd_and_sketch_only(ctx):
-    """The committed network (trained on tensors 0..3583 by the library's own
-    kernels) on 8 held-out tensors of the learned-init workload (128^3, ranks
-    16, decay U(0.7, 0.95), noise log-uniform [1e-3, 1e-1]): every start
-    reaches the reference fit within 1e-4; the learned start needs no more
-    sweeps than HOSVD and reaches it in less wall time (the sketch replaces
-    the three Gram + eigen-solves); against the sketch alone (the same
-    network with its last layer zeroed) the trained network's first-sweep fit
-    is not worse beyond 1e-6."""
+    """What the committed network gains over the method without it, on 8
+    held-out tensors of the learned-init workload (128^3, ranks 16, decay
+    U(0.7, 0.95), noise log-uniform [1e-3, 1e-1]).  Training (tensors
+    0..3519, the library's own kernels) lowers the squared loss but never the
+    validation tensors' (3520..3583) subspace error, so the early stop keeps
+    the untrained network: the learned start IS the sketch range, and the
+    network gains nothing in start fit, first-sweep fit or sweeps.  Every
+    start reaches the reference fit within 1e-4 in one sweep; the sketch start
+    does so in less wall time than HOSVD (the sketch replaces the three Gram +
+    eigen-solves)."""
     import os
     import sys
     sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                                     "diana-tucker_amd", "python"))
     import learned as LN
     cfg = LN.FAMILY_128
-    m = LN.load_model(ctx, os.path.join(LN.MODEL_DIR, "learned_tucker128_r16.npz"))
+    path = os.path.join(LN.MODEL_DIR, "learned_tucker128_r16.npz")
+    m = LN.load_model(ctx, path)
     try:
+        with np.load(path, allow_pickle=False) as f:
+            import json
+            hist = json.loads(str(f["meta"]))["history"]
+        kept = hist[-1]
+        vals = [h["val_subspace"] for h in hist if "epoch" in h]
+        assert kept["val_subspace"] == min(vals)
+        print("kept epoch", kept["kept_epoch"], "validation subspace errors", vals, flush=True)
         ids = list(range(cfg["n_train"], cfg["n_train"] + 8))
         # warm-up (first launches of every kernel)
         LN.evaluate(ctx, m, ids[:1], cfg, max_sweeps=3, ref_sweeps=5, log=lambda *a: None)
@@ -250,7 +260,35 @@ def test_trained_model_128cube_against_hosvd_and_sketch_only(ctx):
             assert s[name]["reached"] == len(ids)
         assert s["learned"]["mean_sweeps"] <= s["hosvd"]["mean_sweeps"]
         assert s["learned"]["mean_ms"] < 0.8 * s["hosvd"]["mean_ms"]
-        for r in rows:
-            assert r["learned"]["gap1"] >= r["sketch"]["gap1"] - 1e-6
+        gain0 = s["learned"]["mean_start_gap"] - s["sketch"]["mean_start_gap"]
+        gain1 = s["learned"]["mean_first_gap"] - s["sketch"]["mean_first_gap"]
+        print("network gain over the sketch alone: start fit", gain0, "first-sweep fit", gain1, flush=True)
+        # the kept network is never worse than the sketch range on the
+        # validation tensors; on held-out tensors neither fit may fall behind
+        assert gain0 >= -1e-9 and gain1 >= -1e-9
+    finally:
+        m.close()
+
+
+def test_training_keeps_the_epoch_with_the_best_validation_subspace(ctx):
+    """train() with validation rows returns the parameters of the epoch whose
+    starts are closest to the targets' subspaces, epoch 0 (the untrained
+    network) included, and its subspace_error is that epoch's."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "diana-tucker_amd", "python"))
+    import learned as LN
+    cfg = dict(LN.FAMILY_128, shape=(32, 32, 32), ranks=(4, 4, 4), S=8, hidden=32, n_train=6, n_total=8)
+    X, T = LN.make_examples(ctx, range(5), cfg, sweeps=4, log=lambda *a: None)
+    Xv, Tv = LN.make_examples(ctx, range(5, 7), cfg, sweeps=4, log=lambda *a: None)
+    m, hist = LN.train(ctx, X, T, cfg, epochs=3, batch=64, lr=3e-3, Xv=Xv, Tv=Tv, log=lambda *a: None)
+    try:
+        errs = [h["val_subspace"] for h in hist if "epoch" in h]
+        assert len(errs) == 4
+        assert hist[-1]["kept_epoch"] == int(np.argmin(errs))
+        dXv = dt.upload_rows(ctx, Xv)
+        assert abs(LN.subspace_error(m, dXv, Tv, cfg) - min(errs)) < 1e-6
+        dXv.free()
     finally:
         m.close()
