@@ -381,12 +381,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-sparse", action="store_true")
+    ap.add_argument("--section", choices=["all", "dense", "mixed", "sparse"], default="all",
+                    help="profiler runs: 'mixed' or 'sparse' runs that section alone and prints its own JSON; "
+                         "'dense' runs the dense section alone (no CPU baseline)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build keeps one rank's block on one "
               "MI355X (no collectives); --gpus > 1 is not supported.", file=sys.stderr)
         sys.exit(2)
     import dtucker as dt
+    if args.section in ("mixed", "sparse"):
+        with dt.Context(0) as ctx:
+            out = mixed_1024x3(ctx, dt) if args.section == "mixed" else sparse_zipf(ctx, dt)
+        print(json.dumps({"section": args.section, **out}), flush=True)
+        return
+    if args.section == "dense":
+        args.no_cpu = args.no_mixed = args.no_sparse = True
     with dt.Context(0) as ctx:
         r = dense_256x4(ctx, dt, args.steps, args.warmup)
         del r["A"]

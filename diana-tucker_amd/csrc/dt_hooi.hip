@@ -488,7 +488,9 @@ int dt_hooi_create(dt_ctx *ctx, const dt_hooi_args *args, dt_hooi **out) {
     for (int n = 0; n < N; n++) csz *= h->R[n];
     csz = std::max<int64_t>(csz, 64);
     if (st == DT_OK) st = hooi_alloc(h, &h->Gbuf, (size_t)csz * wsz);
-    if (st == DT_OK) st = hooi_alloc(h, (void **)&h->S, (size_t)(maxI * maxI) * 8);
+    if (st == DT_OK && hooi_alloc(h, (void **)&h->S, (size_t)(maxI * maxI) * 8) != DT_OK)
+        st = dt_fail(ctx, DT_ENOMEM, "hooi: the FP64 mode Gram S of the longest mode (%lld) needs %.3g GB",
+                     (long long)maxI, (double)maxI * (double)maxI * 8e-9);
     const size_t ir = (size_t)(maxI * maxR) * 8, rr = (size_t)(maxR * maxR) * 8;
     if (st == DT_OK) st = hooi_alloc(h, (void **)&h->L, ir);
     if (st == DT_OK) st = hooi_alloc(h, (void **)&h->P, ir);

@@ -385,7 +385,15 @@ typedef struct dt_hooi_result {
 
 /* Algorithm::Tucker::HOOI_ALS (include/algorithm.hpp:12-14,
  * template/algorithm/tucker/hooi_als.tpp:64-123) on one GPU: create, max_iter
- * sweeps (early stop on tol), copy out, destroy. */
+ * sweeps (early stop on tol), copy out, destroy.
+ *
+ * Device memory beside A: four intermediates of max_n (prod I / I_n) R_n
+ * elements, the core, O(I_max R_max) factor buffers, and the mode Gram
+ * S = Y_(n) Y_(n)ᵀ in FP64: 8 I_max^2 bytes (8 GiB at I_max = 32768; no
+ * longer mode than about 190000 fits 288 GB).  A mode too long for S fails
+ * in create with DT_ENOMEM naming S.  The reference's ALS_ never forms S
+ * (W = ttt_except(Y, Y x_n Pᵀ), hooi_als.tpp:26-27); here every inner step
+ * W = S P then costs an I x I x R product instead of two passes over Y. */
 int dt_hooi_dense(dt_ctx *ctx, const dt_hooi_args *args, dt_hooi_result *res);
 
 /* The same loop one sweep at a time (the reference's `for iter < max_iter`

@@ -643,6 +643,22 @@ def test_hooi_rank_above_size_is_einval(ctx):
     assert e.value.code == dt.DT_EINVAL
 
 
+def test_hooi_mode_too_long_for_the_gram_is_enomem(ctx):
+    """A 200000 x 2 x 2 tensor: its mode-0 FP64 Gram would need 320 GB, more
+    than the device holds, so create fails with DT_ENOMEM naming S (and
+    frees what it had allocated: a normal run follows)."""
+    dA = ctx.to_device(np.ones((200000, 2, 2), np.float32))
+    with pytest.raises(dt.DTError) as e:
+        ctx.hooi_dense(dA, (2, 2, 2), max_iter=1, init=dt.INIT_RANDOM)
+    print("long mode:", e.value, flush=True)
+    assert e.value.code == dt.DT_ENOMEM
+    assert "mode Gram S" in str(e.value)
+    dA.free()
+    g = ctx.gen_tucker((40, 30, 20), (4, 4, 4), 1e-2, seed=3)
+    out = ctx.hooi_dense(g["A"], (4, 4, 4), max_iter=2, init=dt.INIT_HOSVD)
+    assert out["residual"] < 2e-2
+
+
 def test_hooi_zero_tensor_is_enumeric(ctx):
     dA = ctx.to_device(np.zeros((8, 8, 8), np.float32))
     with pytest.raises(dt.DTError) as e:

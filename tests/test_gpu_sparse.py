@@ -98,13 +98,26 @@ def test_sp_ttmc_vs_oracle(ctx, dims, nnz, long_row):
         sp.close()
 
 
-@pytest.mark.parametrize("case", ["one", "row63", "row64", "row65", "rows130", "two_rows_129"])
+@pytest.mark.parametrize("case", ["one", "row63", "row64", "row65", "rows130", "two_rows_129", "fibres"])
 def test_sp_ttmc_tiny_edges(ctx, case):
     """Batch-boundary edges of the TTMc (batches of 64 records): a single
-    nonzero, one row of 63 / 64 / 65 nonzeros, 130 rows of one nonzero each
-    and two rows whose boundary falls inside the second batch."""
+    nonzero, one row of 63 / 64 / 65 nonzeros, 130 rows of one nonzero each,
+    two rows whose boundary falls inside the second batch, and two rows whose
+    nonzeros share one index of a gathered mode in runs of 1, 8, 9, 17 and 64."""
     rng = np.random.default_rng(5)
-    if case == "one":
+    if case == "fibres":
+        # row 1 of mode 0 holds runs of 9, 17 and 64 nonzeros sharing their
+        # index in mode 2, row 3 runs of 1 and 8
+        dims = (4, 64, 3)
+        i0, i1, i2 = [], [], []
+        for r, lens in ((1, (9, 17, 64)), (3, (1, 8, 0))):
+            for k2, ln in enumerate(lens):
+                sel = rng.permutation(64)[:ln]
+                i0 += [r] * ln
+                i1 += list(sel)
+                i2 += [k2] * ln
+        idx = [np.array(i0), np.array(i1), np.array(i2)]
+    elif case == "one":
         dims = (5, 4, 3)
         idx = [np.array([3]), np.array([1]), np.array([2])]
     elif case.startswith("row"):
