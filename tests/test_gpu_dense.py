@@ -241,6 +241,12 @@ def test_sumsq(ctx):
         assert abs(ctx.sumsq(ctx.to_device(x)) - np.sum(x * x)) < 1e-9 * n
         x32 = x.astype(np.float32)
         assert abs(ctx.sumsq(ctx.to_device(x32)) - np.sum(x32.astype(np.float64) ** 2)) < 1e-9 * n
+    # BF16: 8 values per load when n % 8 == 0 (1024, 123456), one per load otherwise
+    for n in (7, 1024, 123456, 123457):
+        bits = dt.bf16_from_f32(rng.standard_normal(n).astype(np.float32))
+        d = ctx.array((n,), dt.DT_BF16).upload(bits)
+        xb = dt.f32_from_bf16(bits).astype(np.float64)
+        assert abs(ctx.sumsq(d) - np.sum(xb ** 2)) < 1e-9 * n, n
 
 
 @pytest.mark.parametrize("tA,tB,m,n,k", [(0, 0, 33, 17, 70), (1, 0, 16, 16, 5000),
