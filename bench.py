@@ -260,7 +260,10 @@ def sparse_zipf(ctx, dt, dims=(1 << 20, 1 << 18, 1 << 16), ndraws=100_000_000, s
     draws per mode (duplicates merged), ranks (16,16,16), 10 sweeps.  The
     TTMc kernel of each mode is timed with HIP events inside the sweeps; its
     algorithmic bytes are nnz (4N + 4) + I_n prod_{m!=n} R_m 4 + sum_{m!=n} I_m R_m 4
-    (read the mode-sorted nonzeros, write Y_(n), read the factors once)."""
+    (read the mode-sorted nonzeros, write Y_(n), read the factors once).  Beside
+    that rate, each mode carries the HBM traffic the hardware counters measured
+    for the kernel (profiles/kernel_counters_sparse_zipf.txt: the factor rows
+    that miss L2 are fetched again) over this run's time."""
     log(f"[bench] sparse: generating {dims}, {ndraws:.0e} Zipf(1.1) draws")
     ctx.sync()
     t0 = time.perf_counter()
@@ -289,6 +292,7 @@ def sparse_zipf(ctx, dt, dims=(1 << 20, 1 << 18, 1 << 16), ndraws=100_000_000, s
     ctx.timing(False)
     sp.close()
     modes = {}
+    traffic = measured_sparse_traffic()
     for n in range(3):
         cnt, ms = kt.get(f"sp_ttmc_m{n}", (0, 0.0))
         if not cnt:
@@ -299,6 +303,10 @@ def sparse_zipf(ctx, dt, dims=(1 << 20, 1 << 18, 1 << 16), ndraws=100_000_000, s
         modes[f"mode{n}"] = {"launches": cnt, "avg_ms": avg, "bytes": nbytes,
                              "GB_per_s": nbytes / (avg * 1e-3) / 1e9,
                              "frac_of_8TBs": nbytes / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        if traffic:
+            # the measured HBM traffic (counters) over this run's time
+            modes[f"mode{n}"]["hbm_traffic_bytes"] = traffic[n]
+            modes[f"mode{n}"]["traffic_frac_of_8TBs"] = traffic[n] / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS
     return {"workload": "sparse HOOI 2^20 x 2^18 x 2^16, 1e8 Zipf(1.1) draws per mode merged, ranks (16,16,16)",
             "nnz": nnz, "sweeps": sweeps, "ms_per_sweep": out["sweeps_ms"] / sweeps,
             "residual": out["residual"], "residuals": out["residuals"], "ttmc": modes,
@@ -340,6 +348,25 @@ def measured_traffic(name):
     except OSError:
         return None
     return None
+
+
+def measured_sparse_traffic():
+    """HBM bytes (read + write) per launch of the sparse TTMc of each mode from
+    the committed counter summary profiles/kernel_counters_sparse_zipf.txt
+    (its `HBM read bytes (GB)` and `HBM write bytes (GB)` rows), or None."""
+    path = os.path.join(ROOT, "profiles", "kernel_counters_sparse_zipf.txt")
+    rows = {}
+    try:
+        with open(path) as f:
+            for line in f:
+                for key in ("HBM read bytes (GB)", "HBM write bytes (GB)"):
+                    if line.startswith(key):
+                        rows[key] = [float(x) * 1e9 for x in line[len(key):].split()]
+    except (OSError, ValueError):
+        return None
+    if len(rows) != 2 or any(len(v) != 3 for v in rows.values()):
+        return None
+    return [r + w for r, w in zip(rows["HBM read bytes (GB)"], rows["HBM write bytes (GB)"])]
 
 
 CPU_HOOI = os.path.join(ROOT, "oracle", "bin", "cpu_hooi")
