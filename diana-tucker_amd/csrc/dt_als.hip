@@ -676,6 +676,157 @@ __global__ __launch_bounds__(AF_THREADS) void als_fused_kernel(const double *__r
     }
 }
 
+// ---- the update over row tiles: one launch per step --------------------------
+// For R <= 16 and 160 <= I <= 512 the iteration runs on ceil(I / 16)
+// workgroups, one per 16-row tile, and each step is one launch (the launch
+// boundary is the only grid-wide synchronisation).  The steps are the fused
+// kernel's, with the product grouped as Y_{k+1} = (S Y_k) R_k^{-1}:
+//   AT_FIRST: Y_1 = S U
+//   AT_STEP : R_k = chol(Y_kᵀ Y_k + shift), Y_{k+1} = (S Y_k) R_k^{-1}
+//   AT_Q    : R = chol(Y_innerᵀ Y_inner + shift), Q = Y_inner R^{-1}
+//   AT_LAST : R = chol(QᵀQ), U = Q R^{-1} and its kernel views
+// A workgroup writes its 16 rows of the new panel (I x 16 row-major FP64,
+// columns R..15 zero) and their 16 x 16 partial Gram; the next launch sums
+// the partials in tile order and every workgroup factors the same Gram
+// (af_chol on wave 0) while its S rows and the panel are already loaded.
+// S Y_k: wave w takes the 64-wide k chunks w and w + 4, lane (c, g) holding
+// S[row c][k0 + 4s + g] (a column of the symmetric S) and Y_k[k0 + 4s + g][c]
+// for the 16 MFMA steps s; the four waves' 16 x 16 partials are added in wave
+// order.  Every launch reads S's rows of its tile (32 I bytes), the panel and
+// the partials: a few hundred KB per launch, latency-bound.
+enum { AT_FIRST = 0, AT_STEP = 1, AT_Q = 2, AT_LAST = 3 };
+constexpr int AT_MINI = 160, AT_MAXI = 512;
+
+template <int MODE>
+__global__ __launch_bounds__(256) void als_tile(const double *__restrict__ S, const double *__restrict__ Yin,
+                                                const double *__restrict__ Pin, double *__restrict__ Yout,
+                                                double *__restrict__ Pout, int I, int R, int *__restrict__ flag,
+                                                double *__restrict__ U, float *__restrict__ U32,
+                                                float *__restrict__ M32, uint16_t *__restrict__ Ub16) {
+    constexpr int CM = 16;
+    constexpr bool HAS_S = MODE == AT_FIRST || MODE == AT_STEP;
+    __shared__ double G[CM][CM + 1], Ri[CM][CM + 1], X[CM][CM + 1];
+    __shared__ double Zp[4][256];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int T = (I + 15) / 16, r0 = blockIdx.x * 16;
+    const int row = r0 + c, rowc = row < I ? row : I - 1;
+    // the operands of S Y_k, loaded before anything waits (indices clamped
+    // into range, values outside it zeroed)
+    double a[2][16], b[2][16];
+    if (HAS_S) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int k0 = (wv + 4 * h) * 64;
+            if (k0 >= I) continue;
+#pragma unroll
+            for (int s = 0; s < 16; s++) {
+                const int kk = k0 + 4 * s + g;
+                const int kc = kk < I ? kk : I - 1;
+                const double sv = S[(size_t)kc + (size_t)I * rowc];
+                const double yv = MODE == AT_FIRST ? U[(size_t)kc + (size_t)I * (c < R ? c : 0)] : Yin[(size_t)kc * CM + c];
+                a[h][s] = kk < I && row < I ? sv : 0.0;
+                b[h][s] = kk < I && (MODE != AT_FIRST || c < R) ? yv : 0.0;
+            }
+        }
+    } else {
+        // AT_Q / AT_LAST: the tile's rows of the panel, [row][column]
+        const int rr = r0 + (tid >> 4);
+        Zp[0][tid] = rr < I ? Yin[(size_t)rr * CM + (tid & 15)] : 0.0;
+    }
+    if (MODE != AT_FIRST) {
+        double sum = 0.0;
+        for (int t = 0; t < T; t++) sum += Pin[(size_t)t * 256 + tid];
+        const int ga = tid >> 4, gb = tid & 15;
+        if (ga < R && gb < R) G[ga][gb] = sum;
+        __syncthreads();
+        af_chol<CM>(G, Ri, X, R, I, MODE == AT_LAST ? 0 : 1, flag);
+    } else {
+        Ri[tid >> 4][tid & 15] = (tid >> 4) == (tid & 15) ? 1.0 : 0.0;
+    }
+    if (HAS_S) {
+        doublex4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            if ((wv + 4 * h) * 64 >= I) break;
+#pragma unroll
+            for (int s = 0; s < 16; s++) d = __builtin_amdgcn_mfma_f64_16x16x4f64(a[h][s], b[h][s], d, 0, 0, 0);
+        }
+        // register v of lane (c, g) holds row 4v + g, column c
+#pragma unroll
+        for (int v = 0; v < 4; v++) Zp[wv][(4 * v + g) * 16 + c] = d[v];
+    }
+    __syncthreads();
+    if (HAS_S) {
+        const double z = ((Zp[0][tid] + Zp[1][tid]) + Zp[2][tid]) + Zp[3][tid];
+        __syncthreads();
+        Zp[0][tid] = z;
+        __syncthreads();
+    }
+    if (wv != 0) return;
+    // the tile's rows of the new panel: Z R^{-1} (K = 16 on the MFMA)
+    doublex4 y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+        y = __builtin_amdgcn_mfma_f64_16x16x4f64(Zp[0][c * 16 + 4 * s + g], Ri[4 * s + g][c], y, 0, 0, 0);
+    if (MODE == AT_LAST) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = r0 + 4 * v + g;
+            if (i < I && c < R) {
+                const double val = y[v];
+                U[(size_t)i + (size_t)I * c] = val;
+                if (U32) U32[(size_t)i + (size_t)I * c] = (float)val;
+                if (Ub16) Ub16[(size_t)i + (size_t)I * c] = f32_to_bf16((float)val);
+                if (M32) M32[c + (size_t)R * i] = (float)val;
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int i = r0 + 4 * v + g;
+        Zp[1][(4 * v + g) * 16 + c] = i < I ? y[v] : 0.0;
+        if (i < I) Yout[(size_t)i * CM + c] = y[v];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // partial Gram of the 16 rows: A[i][k] = Y[k][i] and B[k][j] = Y[k][j]
+    // are the same value in lane (c, g) at step s (k = 4s + g)
+    doublex4 p = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const double yy = Zp[1][(4 * s + g) * 16 + c];
+        p = __builtin_amdgcn_mfma_f64_16x16x4f64(yy, yy, p, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) Pout[(size_t)blockIdx.x * 256 + (4 * v + g) * 16 + c] = p[v];
+}
+
+// ws: two panels (I x 16) and two sets of ceil(I / 16) partial Grams (16 x 16).
+static int als_update_tiles(dt_ctx *ctx, const double *S, double *U, int I, int R, int inner, double *ws,
+                            int *flag, float *U32, float *M32, uint16_t *Ub16) {
+    const int T = (I + 15) / 16;
+    double *Y[2] = {ws, ws + (size_t)I * 16};
+    double *P[2] = {ws + (size_t)2 * I * 16, ws + (size_t)2 * I * 16 + (size_t)T * 256};
+    DT_LAUNCH("als_tile", als_tile<AT_FIRST>, dim3(T), dim3(256), 0, S, (const double *)nullptr,
+              (const double *)nullptr, Y[0], P[0], I, R, flag, U, (float *)nullptr, (float *)nullptr,
+              (uint16_t *)nullptr);
+    int cur = 0;
+    for (int k = 1; k < inner; k++, cur ^= 1)
+        DT_LAUNCH("als_tile", als_tile<AT_STEP>, dim3(T), dim3(256), 0, S, (const double *)Y[cur],
+                  (const double *)P[cur], Y[cur ^ 1], P[cur ^ 1], I, R, flag, (double *)nullptr, (float *)nullptr,
+                  (float *)nullptr, (uint16_t *)nullptr);
+    DT_LAUNCH("als_tile", als_tile<AT_Q>, dim3(T), dim3(256), 0, S, (const double *)Y[cur], (const double *)P[cur],
+              Y[cur ^ 1], P[cur ^ 1], I, R, flag, (double *)nullptr, (float *)nullptr, (float *)nullptr,
+              (uint16_t *)nullptr);
+    cur ^= 1;
+    DT_LAUNCH("als_tile", als_tile<AT_LAST>, dim3(T), dim3(256), 0, S, (const double *)Y[cur],
+              (const double *)P[cur], (double *)nullptr, (double *)nullptr, I, R, flag, U, U32, M32, Ub16);
+    return DT_OK;
+}
+
 // Mode 0 on FP64 MFMA for I % 128 == 0: out = (S Y) Rinv, a workgroup owning
 // 16 rows.  Wave w sums k over [w I/4, (w + 1) I/4) in chunks of 16: lane
 // (c, g) loads S[row0 + c][k0 + 4g .. +3] (S symmetric, so that is 32
@@ -834,11 +985,14 @@ int dt_cholqr3_rows(dt_ctx *ctx, double *X, int I, int R, double *ws, int *flag,
 
 // Workspace needs: part (ceil(I/128) R^2) + 2 Rinv (R^2) + 2 panels (I R)
 // doubles; ranks above ALS_MAXR use a Gram, an R^{-1}, a reciprocal diagonal
-// and the two panels.
+// and the two panels; the row-tile update two I x 16 panels and two sets of
+// ceil(I / 16) 16 x 16 partial Grams.
 size_t dt_als_ws_doubles(size_t I, size_t R) {
     const size_t rows = ((I + ALS_GR - 1) / ALS_GR) * R * R + 2 * R * R + 2 * I * R + 64;
     const size_t any = 2 * R * R + R + 2 * I * R + 64;
-    return rows > any ? rows : any;
+    const size_t tiles = 2 * I * 16 + 2 * ((I + 15) / 16) * 256 + 64;
+    const size_t m = rows > any ? rows : any;
+    return m > tiles ? m : tiles;
 }
 
 // Ranks above ALS_MAXR: the same iteration with every I x R product on the
@@ -870,6 +1024,8 @@ int dt_als_update(dt_ctx *ctx, const double *S, double *U, int I, int R, int inn
                   int *flag, float *U32, float *M32, uint16_t *Ub16) {
     if (R < 1 || R > I) return dt_fail(ctx, DT_EINVAL, "ALS update: rank %d not in [1, %d]", R, I);
     if (R > ALS_MAXR) return als_update_any(ctx, S, U, I, R, inner, ws, flag, U32, M32, Ub16);
+    if (R <= 16 && I >= AT_MINI && I <= AT_MAXI && inner >= 1)
+        return als_update_tiles(ctx, S, U, I, R, inner, ws, flag, U32, M32, Ub16);
     if (R <= 16 && I * 16 <= AF_MAXIR) {
         DT_LAUNCH("als_fused", als_fused_kernel<16>, dim3(1), dim3(AF_THREADS), 0, S, U, I, R, inner, flag, U32, M32,
                   Ub16);

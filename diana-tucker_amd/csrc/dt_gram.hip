@@ -59,37 +59,46 @@ __global__ __launch_bounds__(256) void gram_mfma(const TA *__restrict__ A, float
 #pragma unroll
         for (int b = 0; b < 2; b++) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    for (int64_t c0 = c_begin; c0 < c_end; c0 += 32) {
+    // the chunk starting at c0 into registers: slot q of this thread is row
+    // r, column kc of the chunk as in the LDS layout (F1: r = tid % 64,
+    // kc = tid / 64 + 4q; else kc = tid % 32, r = tid / 32 + 8q); columns past
+    // c_end and rows past I read as zero.  Without F1 the thread's column is
+    // the same for every slot, so its (f, b) split is one division per chunk
+    // and the slots are a fixed stride of 8 F elements apart.
+    const int tid = threadIdx.x;
+    const int64_t rowa = (int64_t)ti * 64, rowb = (int64_t)tj * 64;
+    auto load = [&](int64_t c0, float (&va)[8], float (&vb)[8]) {
+        if (F1) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int64_t kk = c0 + (tid >> 6) + 4 * q;
+                const int64_t ia = rowa + (tid & 63), ib = rowb + (tid & 63);
+                va[q] = kk < c_end && ia < I ? ldg_f<TA>(A + I * kk + ia) : 0.f;
+                vb[q] = !diag && kk < c_end && ib < I ? ldg_f<TA>(A + I * kk + ib) : 0.f;
+            }
+        } else {
+            const int64_t kk = c0 + (tid & 31);
+            const bool ok = kk < c_end;
+            const int64_t b = !ok ? 0 : (F <= 0x7fffffff && kk <= 0x7fffffff) ? (int64_t)((uint32_t)kk / (uint32_t)F) : kk / F;
+            const int64_t base = (kk - b * F) + F * I * b + F * (int64_t)(tid >> 5);
+            const TA *pa = A + base + F * rowa, *pb = A + base + F * rowb;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int64_t r = (tid >> 5) + 8 * q;
+                va[q] = ok && rowa + r < I ? ldg_f<TA>(pa + 8 * F * q) : 0.f;
+                vb[q] = !diag && ok && rowb + r < I ? ldg_f<TA>(pb + 8 * F * q) : 0.f;
+            }
+        }
+    };
+    auto stage = [&](const float (&va)[8], const float (&vb)[8]) {
 #pragma unroll
         for (int q = 0; q < 8; q++) {
-            const int e = threadIdx.x + 256 * q;
-            int r, kc;
-            if (F1) {
-                r = e & 63;
-                kc = e >> 6;
-            } else {
-                kc = e & 31;
-                r = e >> 5;
-            }
-            const int64_t kk = c0 + kc;
-            float va = 0.f, vb = 0.f;
-            if (kk < c_end) {
-                int64_t base;
-                if (F1) {
-                    base = I * kk;
-                } else {
-                    const int64_t b = kk / F;
-                    base = (kk - b * F) + F * I * b;
-                }
-                const int64_t ia = (int64_t)ti * 64 + r;
-                const int64_t ib = (int64_t)tj * 64 + r;
-                if (ia < I) va = ldg_f<TA>(A + base + F * ia);
-                if (!diag && ib < I) vb = ldg_f<TA>(A + base + F * ib);
-            }
-            Xa[r * GS + kc] = va;
-            if (!diag) Xb[r * GS + kc] = vb;
+            const int r = F1 ? (tid & 63) : (tid >> 5) + 8 * q, kc = F1 ? (tid >> 6) + 4 * q : (tid & 31);
+            Xa[r * GS + kc] = va[q];
+            if (!diag) Xb[r * GS + kc] = vb[q];
         }
-        __syncthreads();
+    };
+    auto mma = [&]() {
         const float *XB = diag ? Xa : Xb;
 #pragma unroll
         for (int ks = 0; ks < 32; ks += 4) {
@@ -102,6 +111,23 @@ __global__ __launch_bounds__(256) void gram_mfma(const TA *__restrict__ A, float
             acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
             acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
         }
+    };
+    // two register sets alternate (the loop body is unrolled by two, so no
+    // register holding a load in flight is copied): the loads of chunk k + 1
+    // are in flight while chunk k runs from LDS
+    float xa[8], xb[8], ya[8], yb[8];
+    load(c_begin, xa, xb);
+    for (int64_t c0 = c_begin; c0 < c_end; c0 += 64) {
+        stage(xa, xb);
+        __syncthreads();
+        if (c0 + 32 < c_end) load(c0 + 32, ya, yb);
+        mma();
+        __syncthreads();
+        if (c0 + 32 >= c_end) break;
+        stage(ya, yb);
+        __syncthreads();
+        if (c0 + 64 < c_end) load(c0 + 64, xa, xb);
+        mma();
         __syncthreads();
     }
     float *S = Sp + (int64_t)blockIdx.y * I * I;

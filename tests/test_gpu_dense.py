@@ -472,6 +472,33 @@ def test_hooi_ranks_above_32(ctx):
         assert max(angles) < (1e-7 if dtype == dt.DT_F64 else 1e-3)
 
 
+def test_hooi_row_tile_als_matches_oracle(ctx):
+    """Modes of 160 to 512 rows at ranks <= 16 take the factor update over
+    16-row tiles, one launch per step (200 rows: a ragged last tile; rank 12:
+    padded columns); fit and factors against the oracle's ALS_ from the same
+    HOSVD start, in FP64 and FP32."""
+    shape, ranks = (256, 200, 160), (16, 12, 16)
+    g = ctx.gen_tucker(shape, ranks, 1e-3, seed=41)
+    A = g["A"].download().astype(np.float64)
+    Uh = D.hosvd_factors(A, ranks)
+    Gr, Ur, res_ref = D.hooi_als(A, ranks, 4, Uh)
+    for dtype in (dt.DT_F64, dt.DT_F32):
+        ctx.timing(True)
+        ctx.timing_reset()
+        out = run_gpu_hooi(ctx, A, ranks, 4, dt.INIT_HOSVD, dtype=dtype, flags=dt.FLAG_FP64_FIT)
+        names = ctx.timing_list()
+        ctx.timing(False)
+        assert "als_tile" in names and "als_fused" not in names
+        f_gpu, f_ref = 1 - out["residuals"][-1], 1 - res_ref[-1]
+        angles = [D.sin_max_angle(out["U"][n], Ur[n]) for n in range(3)]
+        print("row tiles dtype", dtype, "fit gpu", f_gpu, "ref", f_ref, "angles", angles, flush=True)
+        assert abs(f_gpu - f_ref) / f_ref < (1e-10 if dtype == dt.DT_F64 else 1e-5)
+        assert max(angles) < (1e-7 if dtype == dt.DT_F64 else 1e-3)
+        for n in range(3):
+            Un = out["U"][n]
+            assert np.abs(Un.T @ Un - np.eye(ranks[n])).max() < 1e-12
+
+
 def test_hooi_ranks_above_32_mode_sizes_multiple_of_128(ctx):
     """Ranks (40, 48, 64) on 256 x 128 x 128: every mode size is a multiple of
     128, so the update's (S Y) R^{-1} steps run on FP64 MFMA (K split over four
