@@ -368,13 +368,27 @@ __global__ __launch_bounds__(256) void cross_mfma(const TA *__restrict__ A, cons
             }
 }
 
+// The FP64 sum of split q = 0, 1, ... of p[q * total], in that order; the
+// partials are loaded eight at a time so that their loads are in flight
+// together.
+__device__ __forceinline__ double sum_splits(const float *__restrict__ p, int64_t total, int nsplit) {
+    double s = 0.0;
+    int q = 0;
+    for (; q + 8 <= nsplit; q += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = p[(int64_t)(q + u) * total];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += (double)v[u];
+    }
+    for (; q < nsplit; q++) s += (double)p[(int64_t)q * total];
+    return s;
+}
+
 __global__ void cross_reduce(const float *__restrict__ Cp, double *__restrict__ C, int64_t total, int nsplit) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
-        double s = 0.0;
-        for (int q = 0; q < nsplit; q++) s += (double)Cp[(int64_t)q * total + idx];
-        C[idx] = s;
-    }
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride)
+        C[idx] = sum_splits(Cp + idx, total, nsplit);
 }
 
 // S[i, j] = sum over splits of the partial stored in the lower triangle of
@@ -383,12 +397,22 @@ __global__ void gram_reduce(const float *__restrict__ Sp, double *__restrict__ S
                             int nsplit, int tile) {
     const int64_t total = I * I;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const bool small = total <= 0x7fffffff;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
-        const int64_t i = idx % I, j = idx / I;
-        const int64_t src = (i / tile >= j / tile) ? idx : (j + I * i);
-        double s = 0.0;
-        for (int q = 0; q < nsplit; q++) s += (double)Sp[(int64_t)q * total + src];
-        S[idx] = s;
+        int64_t i, j;
+        bool lower;
+        if (small) {
+            const uint32_t u = (uint32_t)idx, n = (uint32_t)I, t = (uint32_t)tile;
+            i = u % n;
+            j = u / n;
+            lower = (uint32_t)i / t >= (uint32_t)j / t;
+        } else {
+            i = idx % I;
+            j = idx / I;
+            lower = i / tile >= j / tile;
+        }
+        const int64_t src = lower ? idx : (j + I * i);
+        S[idx] = sum_splits(Sp + src, total, nsplit);
     }
 }
 

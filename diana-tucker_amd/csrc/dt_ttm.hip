@@ -695,10 +695,13 @@ int dt_ttm_views(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t K, int6
 #undef M0F
         }
         if (J <= 64 && F > 1 && (F % 4) == 0 && fv.M32) {
-            // f tile per wave: 64*FB; keep JB*FB <= 4.
+            // f tile per wave: 64*FB; keep JB*FB <= 4, and take a wider tile
+            // only while it still leaves >= 2048 waves (8 per CU) to keep
+            // enough loads in flight (a 67 MB tensor at FB = 4 is 256 waves)
+            auto waves_at = [&](int fbv) { return ceil_div(F, 64 * fbv) * B; };
             int fb = 1;
-            if (jbn == 1) fb = F >= 256 ? 4 : (F >= 128 ? 2 : 1);
-            else if (jbn == 2) fb = F >= 128 ? 2 : 1;
+            if (jbn == 1) fb = F >= 256 && waves_at(4) >= 2048 ? 4 : (F >= 128 && waves_at(2) >= 2048 ? 2 : 1);
+            else if (jbn == 2) fb = F >= 128 && waves_at(2) >= 2048 ? 2 : 1;
             const int64_t tiles = ceil_div(F, 64 * fb);
             const int64_t waves = tiles * B;
             dim3 grid((unsigned)ceil_div(waves, 4));
