@@ -816,6 +816,14 @@ __global__ __launch_bounds__(256) void als_tile(const double *__restrict__ S, co
     const int c = lane & 15, g = lane >> 4;
     const int T = (I + 15) / 16, r0 = blockIdx.x * 16;
     const int row = r0 + c, rowc = row < I ? row : I - 1;
+    // the previous launch's partial Grams first (they gate the Cholesky):
+    // all T <= 32 loads in flight together, summed below in tile order
+    constexpr int TMAX = AT_MAXI / 16;
+    double pv[TMAX];
+    if (MODE != AT_FIRST) {
+#pragma unroll
+        for (int t = 0; t < TMAX; t++) pv[t] = t < T ? Pin[(size_t)t * 256 + tid] : 0.0;
+    }
     // the operands of S Y_k, loaded before anything waits (indices clamped
     // into range, values outside it zeroed)
     double a[2][16], b[2][16];
@@ -841,7 +849,9 @@ __global__ __launch_bounds__(256) void als_tile(const double *__restrict__ S, co
     }
     if (MODE != AT_FIRST) {
         double sum = 0.0;
-        for (int t = 0; t < T; t++) sum += Pin[(size_t)t * 256 + tid];
+#pragma unroll
+        for (int t = 0; t < TMAX; t++)
+            if (t < T) sum += pv[t];
         const int ga = tid >> 4, gb = tid & 15;
         if (ga < R && gb < R) G[ga][gb] = sum;
         __syncthreads();
