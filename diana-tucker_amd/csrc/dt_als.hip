@@ -793,17 +793,19 @@ __global__ __launch_bounds__(AF_THREADS) void als_fused_kernel(const double *__r
 // A workgroup writes its 16 rows of the new panel (I x 16 row-major FP64,
 // columns R..15 zero) and their 16 x 16 partial Gram; the next launch sums
 // the partials in tile order and every workgroup factors the same Gram
-// (af_chol on wave 0) while its S rows and the panel are already loaded.
-// S Y_k: wave w takes the 64-wide k chunks w and w + 4, lane (c, g) holding
-// S[row c][k0 + 4s + g] (a column of the symmetric S) and Y_k[k0 + 4s + g][c]
-// for the 16 MFMA steps s; the four waves' 16 x 16 partials are added in wave
-// order.  Every launch reads S's rows of its tile (32 I bytes), the panel and
+// (af_chol on wave 0) while its S rows and the panel are in flight.
+// S Y_k runs on waves 1..4, beside the Cholesky: wave 1 + w takes the 64-wide
+// k chunks w and w + 4, lane (c, g) holding S[row c][k0 + 4s + g] (a column of
+// the symmetric S) and Y_k[k0 + 4s + g][c] for the 16 MFMA steps s; the four
+// waves' 16 x 16 partials are added in wave order.  Every launch reads S's rows of its tile (32 I bytes), the panel and
 // the partials: a few hundred KB per launch, latency-bound.
 enum { AT_FIRST = 0, AT_STEP = 1, AT_Q = 2, AT_LAST = 3 };
 constexpr int AT_MINI = 160, AT_MAXI = 512;
 
+constexpr int AT_THREADS = 320;  // wave 0: Cholesky; waves 1..4: S Y_k
+
 template <int MODE>
-__global__ __launch_bounds__(256) void als_tile(const double *__restrict__ S, const double *__restrict__ Yin,
+__global__ __launch_bounds__(AT_THREADS) void als_tile(const double *__restrict__ S, const double *__restrict__ Yin,
                                                 const double *__restrict__ Pin, double *__restrict__ Yout,
                                                 double *__restrict__ Pout, int I, int R, int *__restrict__ flag,
                                                 double *__restrict__ U, float *__restrict__ U32,
@@ -812,25 +814,34 @@ __global__ __launch_bounds__(256) void als_tile(const double *__restrict__ S, co
     constexpr bool HAS_S = MODE == AT_FIRST || MODE == AT_STEP;
     __shared__ double G[CM][CM + 1], Ri[CM][CM + 1], X[CM][CM + 1];
     __shared__ double Zp[4][256];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, mw = wv - 1;
     const int c = lane & 15, g = lane >> 4;
     const int T = (I + 15) / 16, r0 = blockIdx.x * 16;
+    const bool ent = tid < 256;  // one entry of a 16 x 16 tile per thread
     const int row = r0 + c, rowc = row < I ? row : I - 1;
     // the previous launch's partial Grams first (they gate the Cholesky):
     // all T <= 32 loads in flight together, summed below in tile order
     constexpr int TMAX = AT_MAXI / 16;
     double pv[TMAX];
-    if (MODE != AT_FIRST) {
+    if (MODE != AT_FIRST && ent) {
 #pragma unroll
         for (int t = 0; t < TMAX; t++) pv[t] = t < T ? Pin[(size_t)t * 256 + tid] : 0.0;
     }
-    // the operands of S Y_k, loaded before anything waits (indices clamped
-    // into range, values outside it zeroed)
+    if (!HAS_S && ent) {
+        // AT_Q / AT_LAST: the tile's rows of the panel, [row][column]
+        const int rr = r0 + (tid >> 4);
+        Zp[0][tid] = rr < I ? Yin[(size_t)rr * CM + (tid & 15)] : 0.0;
+    }
+    // the operands of S Y_k (indices clamped into range, values outside it
+    // zeroed).  Loads complete in issue order, so after a Cholesky they are
+    // issued only once the partial Grams are in: then they overlap the
+    // factorization instead of delaying its start.
     double a[2][16], b[2][16];
-    if (HAS_S) {
+    auto load_sy = [&]() {
+        if (!HAS_S || wv == 0) return;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-            const int k0 = (wv + 4 * h) * 64;
+            const int k0 = (mw + 4 * h) * 64;
             if (k0 >= I) continue;
 #pragma unroll
             for (int s = 0; s < 16; s++) {
@@ -842,40 +853,40 @@ __global__ __launch_bounds__(256) void als_tile(const double *__restrict__ S, co
                 b[h][s] = kk < I && (MODE != AT_FIRST || c < R) ? yv : 0.0;
             }
         }
-    } else {
-        // AT_Q / AT_LAST: the tile's rows of the panel, [row][column]
-        const int rr = r0 + (tid >> 4);
-        Zp[0][tid] = rr < I ? Yin[(size_t)rr * CM + (tid & 15)] : 0.0;
-    }
+    };
+    if (MODE == AT_FIRST) load_sy();
     if (MODE != AT_FIRST) {
-        double sum = 0.0;
+        if (ent) {
+            double sum = 0.0;
 #pragma unroll
-        for (int t = 0; t < TMAX; t++)
-            if (t < T) sum += pv[t];
-        const int ga = tid >> 4, gb = tid & 15;
-        if (ga < R && gb < R) G[ga][gb] = sum;
+            for (int t = 0; t < TMAX; t++)
+                if (t < T) sum += pv[t];
+            const int ga = tid >> 4, gb = tid & 15;
+            if (ga < R && gb < R) G[ga][gb] = sum;
+        }
         __syncthreads();
+        load_sy();
         af_chol<CM>(G, Ri, X, R, I, MODE == AT_LAST ? 0 : 1, flag);
-    } else {
+    } else if (ent) {
         Ri[tid >> 4][tid & 15] = (tid >> 4) == (tid & 15) ? 1.0 : 0.0;
     }
-    if (HAS_S) {
+    if (HAS_S && wv > 0) {
         doublex4 d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-            if ((wv + 4 * h) * 64 >= I) break;
+            if ((mw + 4 * h) * 64 >= I) break;
 #pragma unroll
             for (int s = 0; s < 16; s++) d = __builtin_amdgcn_mfma_f64_16x16x4f64(a[h][s], b[h][s], d, 0, 0, 0);
         }
         // register v of lane (c, g) holds row 4v + g, column c
 #pragma unroll
-        for (int v = 0; v < 4; v++) Zp[wv][(4 * v + g) * 16 + c] = d[v];
+        for (int v = 0; v < 4; v++) Zp[mw][(4 * v + g) * 16 + c] = d[v];
     }
     __syncthreads();
     if (HAS_S) {
-        const double z = ((Zp[0][tid] + Zp[1][tid]) + Zp[2][tid]) + Zp[3][tid];
+        const double z = ent ? ((Zp[0][tid] + Zp[1][tid]) + Zp[2][tid]) + Zp[3][tid] : 0.0;
         __syncthreads();
-        Zp[0][tid] = z;
+        if (ent) Zp[0][tid] = z;
         __syncthreads();
     }
     if (wv != 0) return;
@@ -925,19 +936,19 @@ static int als_update_tiles(dt_ctx *ctx, const double *S, double *U, int I, int 
     const int T = (I + 15) / 16;
     double *Y[2] = {ws, ws + (size_t)I * 16};
     double *P[2] = {ws + (size_t)2 * I * 16, ws + (size_t)2 * I * 16 + (size_t)T * 256};
-    DT_LAUNCH("als_tile", als_tile<AT_FIRST>, dim3(T), dim3(256), 0, S, (const double *)nullptr,
+    DT_LAUNCH("als_tile", als_tile<AT_FIRST>, dim3(T), dim3(AT_THREADS), 0, S, (const double *)nullptr,
               (const double *)nullptr, Y[0], P[0], I, R, flag, U, (float *)nullptr, (float *)nullptr,
               (uint16_t *)nullptr);
     int cur = 0;
     for (int k = 1; k < inner; k++, cur ^= 1)
-        DT_LAUNCH("als_tile", als_tile<AT_STEP>, dim3(T), dim3(256), 0, S, (const double *)Y[cur],
+        DT_LAUNCH("als_tile", als_tile<AT_STEP>, dim3(T), dim3(AT_THREADS), 0, S, (const double *)Y[cur],
                   (const double *)P[cur], Y[cur ^ 1], P[cur ^ 1], I, R, flag, (double *)nullptr, (float *)nullptr,
                   (float *)nullptr, (uint16_t *)nullptr);
-    DT_LAUNCH("als_tile", als_tile<AT_Q>, dim3(T), dim3(256), 0, S, (const double *)Y[cur], (const double *)P[cur],
+    DT_LAUNCH("als_tile", als_tile<AT_Q>, dim3(T), dim3(AT_THREADS), 0, S, (const double *)Y[cur], (const double *)P[cur],
               Y[cur ^ 1], P[cur ^ 1], I, R, flag, (double *)nullptr, (float *)nullptr, (float *)nullptr,
               (uint16_t *)nullptr);
     cur ^= 1;
-    DT_LAUNCH("als_tile", als_tile<AT_LAST>, dim3(T), dim3(256), 0, S, (const double *)Y[cur],
+    DT_LAUNCH("als_tile", als_tile<AT_LAST>, dim3(T), dim3(AT_THREADS), 0, S, (const double *)Y[cur],
               (const double *)P[cur], (double *)nullptr, (double *)nullptr, I, R, flag, U, U32, M32, Ub16);
     return DT_OK;
 }
