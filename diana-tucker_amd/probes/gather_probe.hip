@@ -4,7 +4,12 @@
 // wave-wide load per 16 rows (four lanes per row, 16 bytes each: the TTMc's
 // layout), per 4 rows (16 lanes per row, 4 bytes each), per 8 rows (8 lanes,
 // 8 bytes), or one row piece per lane, next to a coalesced stream of the same
-// table.  Eight workgroups of four waves per CU, 200 x 8 loads per wave; the
+// table.  Partially used 16-row loads show what a load's unused lanes cost:
+// 4 of its 16 rows read with the other lanes masked off (a divergent branch),
+// with the other lanes out of a buffer resource's range (they return zero and
+// touch no memory), or with the 16 row slots holding 4 distinct rows; a load
+// with every lane out of range gives the floor of one load instruction.
+// Eight workgroups of four waves per CU, 200 x 8 loads per wave; the
 // row numbers come from a hash of (wave, iteration, load, lane group) masked
 // to the table, so every access stays inside it.  Prints, per case, the time
 // per wave-wide load instruction per CU (2.4 GHz clock) and the row bytes
@@ -43,6 +48,35 @@ __global__ __launch_bounds__(256) void rows_x4(const float *__restrict__ T, unsi
         for (int s = 0; s < kLoads; s++) {
             const unsigned r = mix(w * 977 + it * 131 + s * 17 + (lane >> 2)) & mask;
             acc += *(const f4 *)(T + r * 16 + 4 * (lane & 3));
+        }
+    out[blockIdx.x * 256 + threadIdx.x] = acc.x + acc.y + acc.z + acc.w;
+}
+
+// 16-row loads of which only 4 rows are used.  MODE 0: the other lanes
+// masked off; MODE 1: the other lanes out of a buffer resource's range;
+// MODE 2: every lane out of range; MODE 3: four distinct rows, each read by
+// four lane groups.
+template <int MODE>
+__global__ __launch_bounds__(256) void rows_x4_part(const float *__restrict__ T, unsigned mask, float *out) {
+    const int lane = threadIdx.x & 63, p4 = lane >> 2, c4 = lane & 3;
+    const unsigned w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // byte range of the whole table; offsets at or past it read zero
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)T, (short)0, (int)(kMaxRows * 64), 0x00020000);
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int it = 0; it < kIters; it++)
+#pragma unroll
+        for (int s = 0; s < kLoads; s++) {
+            const unsigned r = mix(w * 977 + it * 131 + s * 17 + (MODE == 3 ? (p4 & 3) : p4)) & mask;
+            if (MODE == 0) {
+                if (p4 < 4) acc += *(const f4 *)(T + r * 16 + 4 * c4);
+            } else if (MODE == 1 || MODE == 2) {
+                // out-of-range offsets still depend on r, so that no two loads are the same
+                const unsigned off = (MODE == 1 && p4 < 4) ? r * 64 + 16 * c4 : 0x80000000u + r * 64 + 16 * c4;
+                acc += __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+            } else {
+                acc += *(const f4 *)(T + r * 16 + 4 * c4);
+            }
         }
     out[blockIdx.x * 256 + threadIdx.x] = acc.x + acc.y + acc.z + acc.w;
 }
@@ -131,7 +165,9 @@ int main() {
         double row_bytes_per_load;
     } ks[] = {{"4 lanes/row x 16 B", rows_x4, 1024}, {"8 lanes/row x 8 B", rows_x2, 512},
               {"16 lanes/row x 4 B", rows_x1, 256}, {"1 lane/row x 16 B", rows_lane, 1024},
-              {"coalesced 1 KB", stream_x4, 1024}};
+              {"coalesced 1 KB", stream_x4, 1024},
+              {"4 of 16 rows, masked", rows_x4_part<0>, 256}, {"4 of 16 rows, 12 OOB", rows_x4_part<1>, 256},
+              {"all lanes OOB", rows_x4_part<2>, 0}, {"4 rows x 4 slots", rows_x4_part<3>, 256}};
     struct Table {
         const char *name;
         unsigned rows;

@@ -98,14 +98,38 @@ def test_sp_ttmc_vs_oracle(ctx, dims, nnz, long_row):
         sp.close()
 
 
-@pytest.mark.parametrize("case", ["one", "row63", "row64", "row65", "rows130", "two_rows_129", "fibres"])
+@pytest.mark.parametrize("case", ["one", "row63", "row64", "row65", "rows130", "two_rows_129", "fibres",
+                                  "row65_wide", "fibres_wide"])
 def test_sp_ttmc_tiny_edges(ctx, case):
     """Batch-boundary edges of the TTMc (batches of 64 records): a single
     nonzero, one row of 63 / 64 / 65 nonzeros, 130 rows of one nonzero each,
     two rows whose boundary falls inside the second batch, and two rows whose
-    nonzeros share one index of a gathered mode in runs of 1, 8, 9, 17 and 64."""
+    nonzeros share one index of a gathered mode in runs of 1, 8, 9, 17 and 64.
+    The "_wide" cases put the same rows among gathered modes of thousands of
+    rows (the runs' indices between 600 heavily used others)."""
     rng = np.random.default_rng(5)
-    if case == "fibres":
+    if case == "fibres_wide":
+        # row 0 of mode 0: 600 mode-2 indices used 70 times each; rows 1 and
+        # 3 as in "fibres" on three other mode-2 indices
+        dims = (4, 20000, 3000)
+        i0, i1, i2 = [], [], []
+        for k2 in range(600):
+            sel = rng.permutation(20000)[:70]
+            i0 += [0] * 70
+            i1 += list(sel)
+            i2 += [k2] * 70
+        for r, lens in ((1, (9, 17, 64)), (3, (1, 8, 0))):
+            for j, ln in enumerate(lens):
+                sel = rng.permutation(20000)[:ln]
+                i0 += [r] * ln
+                i1 += list(sel)
+                i2 += [1000 + 997 * j] * ln
+        idx = [np.array(i0), np.array(i1), np.array(i2)]
+    elif case == "row65_wide":
+        dims = (4, 5000, 4000)
+        flat = rng.permutation(5000 * 4000)[:65]
+        idx = [np.full(65, 2), flat % 5000, flat // 5000]
+    elif case == "fibres":
         # row 1 of mode 0 holds runs of 9, 17 and 64 nonzeros sharing their
         # index in mode 2, row 3 runs of 1 and 8
         dims = (4, 64, 3)
