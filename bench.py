@@ -173,6 +173,7 @@ def mixed_1024x3(ctx, dt, shape=(1024,) * 3, ranks=(64,) * 3, noise=1e-3, decay=
         return ms
 
     fit_mixed_reported = []
+    fit_mixed_fp64 = []
     accepted = []
 
     def run_mixed():
@@ -196,6 +197,8 @@ def mixed_1024x3(ctx, dt, shape=(1024,) * 3, ranks=(64,) * 3, noise=1e-3, decay=
         fit_mixed_reported.append(1.0 - s.fit(fp64=False))
         ctx.sync()
         ms = (time.perf_counter() - t0) * 1e3
+        # outside the timed region: the same factors' fit with FP64 accumulation
+        fit_mixed_fp64.append(1.0 - s.fit(fp64=True))
         s.close()
         return ms
 
@@ -235,9 +238,10 @@ def mixed_1024x3(ctx, dt, shape=(1024,) * 3, ranks=(64,) * 3, noise=1e-3, decay=
            "plain_fp32": {"sweeps": k_plain, "ms": plain_ms, "fit": 1.0 - res_plain[-1],
                           "max_sin_angle_vs_reference": max(angle(a, b) for a, b in zip(U_plain, ref["U"])),
                           "max_sin_angle_vs_true": max(angle(a, b) for a, b in zip(U_plain, Ut))},
-           "mixed_learned": {"sweeps_bf16": k_mixed, "ms": mixed_ms, "fit": 1.0 - res_mixed[-1],
+           "mixed_learned": {"sweeps_bf16": k_mixed, "ms": mixed_ms, "fit": fit_mixed_fp64[-1],
+                             "fit_after_first_sweeps": 1.0 - res_mixed[-1],
                              "fit_reported_fp32": fit_mixed_reported[-1],
-                             "fit_error_vs_fp64": fit_mixed_reported[-1] - (1.0 - res_mixed[-1]),
+                             "fit_error_vs_fp64": fit_mixed_reported[-1] - fit_mixed_fp64[-1],
                              "learned_start_accepted": all(accepted),
                              "ms_with_hosvd_check": checked_ms,
                              "max_sin_angle_vs_reference": max(angle(a, b) for a, b in zip(U_mixed, ref["U"])),
