@@ -31,6 +31,7 @@
 // factor update (sp_tl_gram, sp_rmul) never waits for the host.
 #include "dt_sparse.hpp"
 #include "dt_philox.hpp"
+#include "dt_chol16.hpp"
 
 
 #include <algorithm>
@@ -752,21 +753,34 @@ __global__ __launch_bounds__(256) void sp_tl_gram(const float *__restrict__ Yc, 
     sp_gram_out(gacc, R, part);
 }
 
-// U <- U Rinv in place (U: I x R FP64 column-major, Rinv: R x R upper,
-// column-major) on FP64 MFMA: for a 16-row tile lane (c, g) feeds
-// U[r0 + c][4s + g] as A and Rinv[4s + g][c] as B of step s and receives
+// One Cholesky-QR pass: U <- U R^{-1} in place (U: I x R FP64 column-major,
+// R <= 16) with R = chol(G + shift), G the summed Gram of U (R x R), the
+// shift 11 (I R + R(R+1)) u tr(G) when `shifted`.  Every workgroup factors G
+// itself (wave 0, chol16_rinv: a few microseconds, no launch of its own) and
+// all of them get the same R^{-1}; a non-positive pivot sets bit 4 of flag.
+// The product runs on FP64 MFMA: for a 16-row tile lane (c, g) feeds
+// U[r0 + c][4s + g] as A and R^{-1}[4s + g][c] as B of step s and receives
 // Unew[r0 + 4v + g][c] in register v.  GRAM: partial Grams of the new U.
 // VIEW: also the FP32 row-major copy, 16 columns (zero past R).
 template <bool GRAM, bool VIEW>
 __global__ __launch_bounds__(256) void sp_rmul(double *__restrict__ U, int64_t I, int R,
-                                               const double *__restrict__ Rinv, int64_t tiles_per_wave,
-                                               double *__restrict__ part, float *__restrict__ Ur) {
+                                               const double *__restrict__ Gs, int shifted, int *__restrict__ flag,
+                                               int64_t tiles_per_wave, double *__restrict__ part,
+                                               float *__restrict__ Ur) {
     __shared__ double Wt[4][16][18];  // per wave: the new tile by column
+    __shared__ double Gl[16][17], Ril[16][17];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int c = lane & 15, g = lane >> 4;
+    {
+        const int i = threadIdx.x >> 4, k = threadIdx.x & 15;
+        Gl[i][k] = (i < R && k < R) ? Gs[i + R * k] : 0.0;
+    }
+    __syncthreads();
+    chol16_rinv(Gl, Ril, R, (int)I, shifted, flag);  // wave 0; the others return at once
+    __syncthreads();
     double b[4];
 #pragma unroll
-    for (int s = 0; s < 4; s++) b[s] = (4 * s + g < R && c < R) ? Rinv[(4 * s + g) + R * c] : 0.0;
+    for (int s = 0; s < 4; s++) b[s] = (4 * s + g < R && c < R) ? Ril[4 * s + g][c] : 0.0;
     const int64_t ntiles = (I + 15) / 16;
     const int64_t t0 = ((int64_t)blockIdx.x * 4 + wv) * tiles_per_wave;
     const int64_t t1 = t0 + tiles_per_wave < ntiles ? t0 + tiles_per_wave : ntiles;
@@ -1243,7 +1257,7 @@ struct dt_sp_hooi {
     void *Urow[DT_MAX_NDIM] = {};  // fast: FP32 I x 16 rows; generic: T rows I x R
     void *Yc = nullptr;
     double *L = nullptr, *G64 = nullptr;
-    double *part = nullptr, *Gs = nullptr, *Rinv = nullptr, *work = nullptr;
+    double *part = nullptr, *Gs = nullptr, *work = nullptr;
     int *dflag = nullptr;
     double *scal = nullptr;       // [0] = ||G||, [1 + n] = ||L|| of mode n
     int64_t Klast = 0;            // core: Klast x R[N-1]
@@ -1281,13 +1295,13 @@ static int sh_fast_update(dt_sp_hooi *h, int n) {
     auto rmul_view = sp_rmul<false, true>;
     for (int pass = 0; pass < 3; pass++) {
         DT_TRY(dt_part_reduce(ctx, h->part, ng, Rn * Rn, h->Gs));
-        DT_TRY(dt_chol_rinv_dev(ctx, h->Gs, Rn, pass == 0, (double)I, h->Rinv, h->dflag));
+        const int shifted = pass == 0 ? 1 : 0;
         if (pass < 2) {
-            DT_LAUNCH("sp_rmul", rmul_gram, dim3((unsigned)ng), dim3(256), 0, h->U64[n], I, Rn, (const double *)h->Rinv,
-                      tpw, h->part, (float *)nullptr);
+            DT_LAUNCH("sp_rmul", rmul_gram, dim3((unsigned)ng), dim3(256), 0, h->U64[n], I, Rn, (const double *)h->Gs,
+                      shifted, h->dflag, tpw, h->part, (float *)nullptr);
         } else {
-            DT_LAUNCH("sp_rmul", rmul_view, dim3((unsigned)ng), dim3(256), 0, h->U64[n], I, Rn, (const double *)h->Rinv,
-                      tpw, (double *)nullptr, (float *)h->Urow[n]);
+            DT_LAUNCH("sp_rmul", rmul_view, dim3((unsigned)ng), dim3(256), 0, h->U64[n], I, Rn, (const double *)h->Gs,
+                      shifted, h->dflag, tpw, (double *)nullptr, (float *)h->Urow[n]);
         }
     }
     return DT_OK;
@@ -1366,7 +1380,6 @@ extern "C" int dt_sp_hooi_create(dt_ctx *ctx, dt_sp *sp, const size_t *ranks, ui
     if (ok) ok = (h->G64 = (double *)sh_alloc(h, lsz * 8)) != nullptr;
     if (ok && h->fast) ok = (h->part = (double *)sh_alloc(h, (size_t)kSpMaxWg * 256 * 8)) != nullptr;
     if (ok) ok = (h->Gs = (double *)sh_alloc(h, 256 * 8)) != nullptr;
-    if (ok) ok = (h->Rinv = (double *)sh_alloc(h, 256 * 8)) != nullptr;
     if (ok && !h->fast) ok = (h->work = (double *)sh_alloc(h, (size_t)(2 * maxR * maxR + maxI * maxR + 64) * 8)) != nullptr;
     if (ok) ok = (h->dflag = (int *)sh_alloc(h, 256)) != nullptr;
     if (ok) ok = (h->scal = (double *)sh_alloc(h, (DT_MAX_NDIM + 1) * 8)) != nullptr;
