@@ -464,67 +464,61 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 // the first kernel.  Every sum runs in a fixed order for a given range size.
 // Range w's chunk of the chain of row r that starts at range s0 = w (owner
 // when w holds the row's first nonzero, else a chunk start inside the chain):
-// the sum of slots [w, min(w + SP_FC, we + 1)), four 64-thread groups taking
-// every fourth slot, added in group order.
-__device__ __forceinline__ floatx4 sp_chunk_sum(const float *__restrict__ carry, int64_t w, bool owner, int64_t we,
-                                                 floatx4 (*part)[64]) {
-    const int grp = threadIdx.x >> 6, t = threadIdx.x & 63;
+// the sum of slots [w, min(w + SP_FC, we + 1)) by one wave (lane t: entries
+// 4t .. 4t + 3), eight slots in flight, added in a fixed order.
+__device__ __forceinline__ floatx4 sp_chunk_sum(const float *__restrict__ carry, int64_t w, bool owner, int64_t we) {
+    const int t = threadIdx.x;
     const int64_t qe = w + SP_FC - 1 < we ? w + SP_FC - 1 : we;
-    floatx4 s0 = {0.f, 0.f, 0.f, 0.f}, s1 = s0, s2 = s0, s3 = s0;
-    int64_t q = w + grp;
     auto slot = [&](int64_t x) {
         return *(const floatx4 *)(carry + (x * 2 + (owner && x == w ? 1 : 0)) * SP_E + 4 * t);
     };
-    for (; q + 12 <= qe; q += 16) {
-        s0 += slot(q);
-        s1 += slot(q + 4);
-        s2 += slot(q + 8);
-        s3 += slot(q + 12);
-    }
-    for (; q <= qe; q += 4) s0 += slot(q);
-    part[grp][t] = (s0 + s1) + (s2 + s3);
-    __syncthreads();
-    return ((part[0][t] + part[1][t]) + part[2][t]) + part[3][t];
+    floatx4 acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+    int64_t q = w;
+    for (; q + 7 <= qe; q += 8)
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc[u] += slot(q + u);
+#pragma unroll
+    for (int u = 0; u < 7; u++)
+        if (q + u <= qe) acc[u] += slot(q + u);
+    return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 }
 
-__global__ __launch_bounds__(256) void sp_fixup(SpModeArgs a, const float *__restrict__ carry, int64_t per_wave,
-                                                float *__restrict__ csum, float *__restrict__ Yc,
-                                                const int *__restrict__ fix_list, const int *__restrict__ fix_count) {
-    __shared__ floatx4 part[4][64];
+// One wave per entry of the fix-up list (most chains are two or three slots
+// long, so the entries are many and short).
+__global__ __launch_bounds__(64) void sp_fixup(SpModeArgs a, const float *__restrict__ carry, int64_t per_wave,
+                                               float *__restrict__ csum, float *__restrict__ Yc,
+                                               const int *__restrict__ fix_list, const int *__restrict__ fix_count) {
     const int nlist = *fix_count;
+    const int t = threadIdx.x;
     for (int li = blockIdx.x; li < nlist; li += gridDim.x) {
-    __syncthreads();  // part is reused across entries
-    const int64_t w = fix_list[li];
-    const int64_t P0 = w * per_wave;
-    const int64_t P1 = P0 + per_wave < a.nnz ? P0 + per_wave : a.nnz;
-    const int t = threadIdx.x & 63;
-    // role 1: owner of the chain of its last row
-    {
-        const int r = a.row[P1 - 1];
-        const int64_t rs = a.rowptr[r], re = a.rowptr[r + 1];
-        if (re > P1 && rs >= P0) {  // uniform
-            const int64_t we = (re - 1) / per_wave;
-            const floatx4 v = sp_chunk_sum(carry, w, true, we, part);
-            if (threadIdx.x < 64) {
+        const int64_t w = fix_list[li];
+        const int64_t P0 = w * per_wave;
+        const int64_t P1 = P0 + per_wave < a.nnz ? P0 + per_wave : a.nnz;
+        // role 1: owner of the chain of its last row
+        {
+            const int r = a.row[P1 - 1];
+            const int64_t rs = a.rowptr[r], re = a.rowptr[r + 1];
+            if (re > P1 && rs >= P0) {  // uniform
+                const int64_t we = (re - 1) / per_wave;
+                const floatx4 v = sp_chunk_sum(carry, w, true, we);
                 float *dst = we - w < SP_FC ? Yc + (int64_t)a.slot[r] * SP_E : csum + (w * 2 + 1) * SP_E;
                 *(floatx4 *)(dst + 4 * t) = v;
             }
-            __syncthreads();  // part is reused below
         }
-    }
-    // role 2: start of a later chunk of the chain of its first row
-    {
-        const int r = a.row[P0];
-        const int64_t rs = a.rowptr[r], re = a.rowptr[r + 1];
-        if (rs < P0) {  // uniform
-            const int64_t o = rs / per_wave;
-            if ((w - o) % SP_FC == 0) {
-                const int64_t we = (re - 1) / per_wave;
-                const floatx4 v = sp_chunk_sum(carry, w, false, we, part);
-                if (threadIdx.x < 64) *(floatx4 *)(csum + (w * 2) * SP_E + 4 * t) = v;
+        // role 2: start of a later chunk of the chain of its first row
+        {
+            const int r = a.row[P0];
+            const int64_t rs = a.rowptr[r], re = a.rowptr[r + 1];
+            if (rs < P0) {  // uniform
+                const int64_t o = rs / per_wave;
+                if ((w - o) % SP_FC == 0) {
+                    const int64_t we = (re - 1) / per_wave;
+                    *(floatx4 *)(csum + (w * 2) * SP_E + 4 * t) = sp_chunk_sum(carry, w, false, we);
+                }
             }
         }
-    }
     }
 }
 
@@ -1210,8 +1204,8 @@ static int sp_ttmc_dev(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, fl
         DT_LAUNCH(names[n], sp_ttmc16<false>, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, args, Ur[a], Ur[b],
                   Yc, carry, per_wave, nwaves, fix_list, fix_count);
     }
-    const unsigned nfix = (unsigned)std::min<int64_t>(nwaves, 1024);  // workgroups walking the list
-    DT_LAUNCH("sp_fixup", sp_fixup, dim3(nfix), dim3(256), 0, args, carry, per_wave, csum, Yc, (const int *)fix_list,
+    const unsigned nfix = (unsigned)std::min<int64_t>(nwaves, 8192);  // one-wave workgroups walking the list
+    DT_LAUNCH("sp_fixup", sp_fixup, dim3(nfix), dim3(64), 0, args, carry, per_wave, csum, Yc, (const int *)fix_list,
               (const int *)fix_count);
     DT_LAUNCH("sp_fixup", sp_fixup_chunks, dim3(nfix), dim3(64), 0, args, (const float *)csum, per_wave, Yc,
               (const int *)fix_list, (const int *)fix_count);
