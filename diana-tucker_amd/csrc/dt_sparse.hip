@@ -465,24 +465,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 // Range w's chunk of the chain of row r that starts at range s0 = w (owner
 // when w holds the row's first nonzero, else a chunk start inside the chain):
 // the sum of slots [w, min(w + SP_FC, we + 1)) by one wave (lane t: entries
-// 4t .. 4t + 3), eight slots in flight, added in a fixed order.
+// 4t .. 4t + 3), sixteen slots in flight (a hot row's chunk takes four
+// rounds of loads), added in a fixed order.
+__device__ __forceinline__ floatx4 sp_sum16(const floatx4 (&acc)[16]) {
+    floatx4 h[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) h[u] = acc[2 * u] + acc[2 * u + 1];
+    return ((h[0] + h[1]) + (h[2] + h[3])) + ((h[4] + h[5]) + (h[6] + h[7]));
+}
+
 __device__ __forceinline__ floatx4 sp_chunk_sum(const float *__restrict__ carry, int64_t w, bool owner, int64_t we) {
     const int t = threadIdx.x;
     const int64_t qe = w + SP_FC - 1 < we ? w + SP_FC - 1 : we;
     auto slot = [&](int64_t x) {
         return *(const floatx4 *)(carry + (x * 2 + (owner && x == w ? 1 : 0)) * SP_E + 4 * t);
     };
-    floatx4 acc[8];
+    floatx4 acc[16];
 #pragma unroll
-    for (int u = 0; u < 8; u++) acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int u = 0; u < 16; u++) acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
     int64_t q = w;
-    for (; q + 7 <= qe; q += 8)
+    for (; q + 15 <= qe; q += 16)
 #pragma unroll
-        for (int u = 0; u < 8; u++) acc[u] += slot(q + u);
+        for (int u = 0; u < 16; u++) acc[u] += slot(q + u);
 #pragma unroll
-    for (int u = 0; u < 7; u++)
+    for (int u = 0; u < 15; u++)
         if (q + u <= qe) acc[u] += slot(q + u);
-    return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    return sp_sum16(acc);
 }
 
 // One wave per entry of the fix-up list (most chains are two or three slots
@@ -539,17 +547,18 @@ __global__ __launch_bounds__(64) void sp_fixup_chunks(SpModeArgs a, const float 
         if (we - w < SP_FC) continue;  // finished by sp_fixup
         const int t = threadIdx.x;
         auto chunk = [&](int64_t s) { return *(const floatx4 *)(csum + (s * 2) * SP_E + 4 * t); };
-        // eight chunk sums in flight, added in a fixed order
-        floatx4 acc[8];
+        // sixteen chunk sums in flight, added in a fixed order
+        floatx4 acc[16];
 #pragma unroll
-        for (int u = 0; u < 8; u++) acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int u = 0; u < 16; u++) acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
         int64_t s = w + SP_FC;
-        for (; s + 7 * SP_FC <= we; s += 8 * SP_FC)
+        for (; s + 15 * SP_FC <= we; s += 16 * SP_FC)
 #pragma unroll
-            for (int u = 0; u < 8; u++) acc[u] += chunk(s + u * SP_FC);
-        for (; s <= we; s += SP_FC) acc[0] += chunk(s);
-        const floatx4 tot = *(const floatx4 *)(csum + (w * 2 + 1) * SP_E + 4 * t) +
-                            (((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7])));
+            for (int u = 0; u < 16; u++) acc[u] += chunk(s + u * SP_FC);
+#pragma unroll
+        for (int u = 0; u < 15; u++)
+            if (s + u * SP_FC <= we) acc[u] += chunk(s + u * SP_FC);
+        const floatx4 tot = *(const floatx4 *)(csum + (w * 2 + 1) * SP_E + 4 * t) + sp_sum16(acc);
         *(floatx4 *)(Yc + (int64_t)a.slot[r] * SP_E + 4 * t) = tot;
     }
 }
@@ -1204,7 +1213,9 @@ static int sp_ttmc_dev(dt_ctx *ctx, dt_sp *sp, int n, const float *const *Ur, fl
         DT_LAUNCH(names[n], sp_ttmc16<false>, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, args, Ur[a], Ur[b],
                   Yc, carry, per_wave, nwaves, fix_list, fix_count);
     }
-    const unsigned nfix = (unsigned)std::min<int64_t>(nwaves, 8192);  // one-wave workgroups walking the list
+    // one-wave workgroups, as many as ranges (the list has at most one entry
+    // per range, so each wave takes at most one)
+    const unsigned nfix = (unsigned)nwaves;
     DT_LAUNCH("sp_fixup", sp_fixup, dim3(nfix), dim3(64), 0, args, carry, per_wave, csum, Yc, (const int *)fix_list,
               (const int *)fix_count);
     DT_LAUNCH("sp_fixup", sp_fixup_chunks, dim3(nfix), dim3(64), 0, args, (const float *)csum, per_wave, Yc,
