@@ -176,7 +176,7 @@ def mixed_1024x3(ctx, dt, shape=(1024,) * 3, ranks=(64,) * 3, noise=1e-3, decay=
     fit_mixed_fp64 = []
     accepted = []
 
-    def run_mixed():
+    def run_mixed(check=True):
         """learned start, its BF16 sweeps, the acceptance test of
         DT_FLAG_LEARNED_ACCEPT (one more sweep must move the fit by less than
         1e-4: HOOI from the learned start has converged, else the HOSVD start
@@ -198,7 +198,8 @@ def mixed_1024x3(ctx, dt, shape=(1024,) * 3, ranks=(64,) * 3, noise=1e-3, decay=
         ctx.sync()
         ms = (time.perf_counter() - t0) * 1e3
         # outside the timed region: the same factors' fit with FP64 accumulation
-        fit_mixed_fp64.append(1.0 - s.fit(fp64=True))
+        if check:
+            fit_mixed_fp64.append(1.0 - s.fit(fp64=True))
         s.close()
         return ms
 
@@ -222,7 +223,7 @@ def mixed_1024x3(ctx, dt, shape=(1024,) * 3, ranks=(64,) * 3, noise=1e-3, decay=
     # breakdown of one mixed run
     ctx.timing(True)
     ctx.timing_reset()
-    run_mixed()
+    run_mixed(check=False)
     kt = {k: round(v[1], 3) for k, v in sorted(ctx.timing_list().items(), key=lambda kv: -kv[1][1])[:12]}
     ctx.timing(False)
     model.close()

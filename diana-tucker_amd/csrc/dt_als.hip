@@ -206,62 +206,9 @@ __global__ __launch_bounds__(ALS_THREADS) void als_gram_kernel(const double *__r
 }
 
 
-// One wave: A (CM x CM SPD, LDS, element (i, k) at A[i * lda + k]) = RᵀR.
-// Writes R (upper part; the lower part holds leftovers) to Rm and R^{-1}
-// (upper, zeros below) to Ri, both with the same element order.  Lane k < CM
-// holds column k in registers and row j of each trailing update comes from
-// v_readlane, as in als_chol_kernel.  Returns 1 + the first bad pivot, or 0.
-template <int CM>
-__device__ int chol_wave(const double *__restrict__ A, int lda, double *__restrict__ Rm, int ldr,
-                         double *__restrict__ Ri, int ldi) {
-    const int lane = threadIdx.x & 63;
-    const int k = lane < CM ? lane : CM - 1;
-    double col[CM];
-#pragma unroll
-    for (int i = 0; i < CM; i++) col[i] = A[i * lda + k];
-    int bad = 0;
-    double invr[CM];
-#pragma unroll
-    for (int j = 0; j < CM; j++) {
-        double d = rl_f64(col[j], j);
-        if (!(d > 0.0) || !isfinite(d)) {
-            if (!bad) bad = j + 1;
-            d = 1.0;
-        }
-        const double rjj = sqrt(d), ir = 1.0 / rjj;
-        invr[j] = ir;
-        const double gjk = col[j];
-        const double f = gjk * (ir * ir);
-#pragma unroll
-        for (int i = j + 1; i < CM; i++) col[i] -= rl_f64(col[j], i) * f;
-        col[j] = (k == j) ? rjj : gjk * ir;
-    }
-    if (lane < CM)
-#pragma unroll
-        for (int i = 0; i < CM; i++) Rm[i * ldr + k] = col[i];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    double x[CM];
-#pragma unroll
-    for (int i = CM - 1; i >= 0; i--) {
-        double s0 = (i == k) ? 1.0 : 0.0, s1 = 0.0;
-#pragma unroll
-        for (int m = i + 1; m < CM; m += 2) {
-            s0 -= Rm[i * ldr + m] * x[m];
-            if (m + 1 < CM) s1 -= Rm[i * ldr + m + 1] * x[m + 1];
-        }
-        x[i] = (s0 + s1) * invr[i];
-    }
-    if (lane < CM)
-#pragma unroll
-        for (int i = 0; i < CM; i++) Ri[i * ldi + k] = x[i];
-    return bad;
-}
-
 // Order 64 as a 4 x 4 grid of 16 x 16 blocks (G: LDS, row stride 65, upper
 // part used), right-looking: for block column k, one wave factors the
-// diagonal block (chol_wave<16>: R_kk and R_kk⁻¹), the workgroup forms the
+// diagonal block (chol16_rinv on its symmetric copy: R_kk⁻¹), the workgroup forms the
 // block row R_kj = R_kk⁻ᵀ G_kj and updates the trailing blocks
 // G_ij -= R_kiᵀ R_kj (i <= j).  Then R⁻¹ by block diagonals d = 1..3:
 // X_ij = -R_ii⁻¹ sum_{k = i+1..j} R_ik X_kj, every X_kj of a smaller d being
@@ -272,16 +219,21 @@ __device__ void chol64_blocked(double (*G)[65], double (*Rb)[65], int R, double 
     __shared__ double Id[4][16][17];  // inverses of the diagonal blocks of R
     __shared__ double Xi[64][65];     // R^{-1}
     __shared__ double Tt[3][16][17];
+    __shared__ double Gd[16][17];     // the diagonal block, both triangles
     __shared__ int bad_s;
     const int tid = threadIdx.x, nt = blockDim.x;
     if (tid == 0) bad_s = 0;
     for (int kb = 0; kb < 4; kb++) {
         const int k0 = 16 * kb;
         __syncthreads();
-        if (tid < 64) {
-            const int b = chol_wave<16>(&G[k0][k0], 65, &Rb[k0][k0], 65, &Id[kb][0][0], 17);
-            if (tid == 0 && b && !bad_s) bad_s = k0 + b;
+        for (int e = tid; e < 256; e += nt) {
+            const int i = e >> 4, k = e & 15;
+            Gd[i][k] = i <= k ? G[k0 + i][k0 + k] : G[k0 + k][k0 + i];
         }
+        __syncthreads();
+        // wave 0 (the others return at once); a bad pivot ORs 4 into *flag
+        // and is recorded for the early exit below
+        chol16_rinv(Gd, Id[kb], 16, 0, 0, &bad_s);
         __syncthreads();
         const int nc = 48 - k0;  // columns right of the diagonal block
         for (int e = tid; e < 16 * nc; e += nt) {
