@@ -228,18 +228,21 @@ constexpr int LDS_NMAX = 96;
 __global__ __launch_bounds__(512) void chol_lds_kernel(const double *__restrict__ G, int n, double shift,
                                                       double *__restrict__ R, double *__restrict__ Rinv,
                                                       int *__restrict__ info) {
-    __shared__ double W[LDS_NMAX * LDS_NMAX];
-    __shared__ double X[LDS_NMAX * LDS_NMAX];
+    // W and X with an odd leading dimension ld: threads that walk a row across
+    // columns (stride ld doubles) fall on different LDS banks
+    __shared__ double W[LDS_NMAX * (LDS_NMAX + 1)];
+    __shared__ double X[LDS_NMAX * (LDS_NMAX + 1)];
     __shared__ int bad;
     const int tid = threadIdx.x, nt = blockDim.x;
+    const int ld = n | 1;
     for (int idx = tid; idx < n * n; idx += nt) {
         const int i = idx % n, j = idx / n;
-        W[idx] = (i <= j) ? G[idx] + (i == j ? shift : 0.0) : 0.0;
+        W[i + ld * j] = (i <= j) ? G[idx] + (i == j ? shift : 0.0) : 0.0;
     }
     if (tid == 0) bad = 0;
     __syncthreads();
     for (int j = 0; j < n; j++) {
-        const double d = W[j + n * j];
+        const double d = W[j + ld * j];
         const bool ok = d > 0.0 && isfinite(d);
         if (!ok) {
             if (tid == 0) bad = j + 1;
@@ -247,13 +250,13 @@ __global__ __launch_bounds__(512) void chol_lds_kernel(const double *__restrict_
         }
         const double rjj = sqrt(d);
         // scale row j right of the diagonal (threads own distinct k)
-        for (int k = j + 1 + tid; k < n; k += nt) W[j + n * k] /= rjj;
+        for (int k = j + 1 + tid; k < n; k += nt) W[j + ld * k] /= rjj;
         __syncthreads();
-        if (tid == 0) W[j + n * j] = rjj;
+        if (tid == 0) W[j + ld * j] = rjj;
         const int m = n - j - 1;
         for (int idx = tid; idx < m * m; idx += nt) {
             const int i = j + 1 + idx % m, k = j + 1 + idx / m;
-            if (i <= k) W[i + n * k] -= W[j + n * i] * W[j + n * k];
+            if (i <= k) W[i + ld * k] -= W[j + ld * i] * W[j + ld * k];
         }
         __syncthreads();
     }
@@ -262,17 +265,18 @@ __global__ __launch_bounds__(512) void chol_lds_kernel(const double *__restrict_
     if (bad) return;
     // Rinv by columns (thread c: upper-triangular back substitution in LDS)
     for (int c = tid; c < n; c += nt) {
-        for (int i = c + 1; i < n; i++) X[i + n * c] = 0.0;
+        for (int i = c + 1; i < n; i++) X[i + ld * c] = 0.0;
         for (int i = c; i >= 0; i--) {
             double sacc = (i == c) ? 1.0 : 0.0;
-            for (int k = i + 1; k <= c; k++) sacc -= W[i + n * k] * X[k + n * c];
-            X[i + n * c] = sacc / W[i + n * i];
+            for (int k = i + 1; k <= c; k++) sacc -= W[i + ld * k] * X[k + ld * c];
+            X[i + ld * c] = sacc / W[i + ld * i];
         }
     }
     __syncthreads();
     for (int idx = tid; idx < n * n; idx += nt) {
-        R[idx] = W[idx];
-        Rinv[idx] = X[idx];
+        const int i = idx % n, j = idx / n;
+        R[idx] = W[i + ld * j];
+        Rinv[idx] = X[i + ld * j];
     }
 }
 
@@ -771,7 +775,9 @@ __global__ __launch_bounds__(1024) void jacobi_lds_kernel(const double *__restri
     Hin += (size_t)blockIdx.x * n * n;
     Vout += (size_t)blockIdx.x * n * n;
     w += (size_t)blockIdx.x * n;
-    __shared__ double H[LDS_NMAX * LDS_NMAX];
+    // H with an odd leading dimension: lanes that read one row across columns
+    // (stride ld doubles) then fall on different LDS banks
+    __shared__ double H[LDS_NMAX * (LDS_NMAX + 1)];
     __shared__ double V[LDS_NMAX * LDS_NMAX];
     __shared__ double rc[LDS_NMAX / 2], rsn[LDS_NMAX / 2];
     __shared__ int rp[LDS_NMAX / 2], rq[LDS_NMAX / 2];
@@ -781,9 +787,10 @@ __global__ __launch_bounds__(1024) void jacobi_lds_kernel(const double *__restri
     const int tid = threadIdx.x, nt = blockDim.x;
     const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
     const int np = (n + 1) & ~1, npair = np / 2;  // npair <= 48 < 64
+    const int ld = n | 1;
     for (int j = wv; j < n; j += nw)
         for (int i = lane; i < n; i += 64) {
-            H[i + n * j] = Hin[i + n * j];
+            H[i + ld * j] = Hin[i + n * j];
             V[i + n * j] = i == j ? 1.0 : 0.0;
         }
     __syncthreads();
@@ -791,7 +798,7 @@ __global__ __launch_bounds__(1024) void jacobi_lds_kernel(const double *__restri
         double off = 0.0, tot = 0.0;
         for (int j = wv; j < n; j += nw)
             for (int i = lane; i < n; i += 64) {
-                const double h = H[i + n * j];
+                const double h = H[i + ld * j];
                 tot += h * h;
                 if (i != j) off += h * h;
             }
@@ -827,8 +834,8 @@ __global__ __launch_bounds__(1024) void jacobi_lds_kernel(const double *__restri
                 const int p = a < b ? a : b, qq = a < b ? b : a;
                 double c = 1.0, sn = 0.0;  // identity for the pair holding the padding index n
                 if (qq < n) {
-                    const double apq = H[p + n * qq];
-                    const double app = H[p + n * p], aqq = H[qq + n * qq];
+                    const double apq = H[p + ld * qq];
+                    const double app = H[p + ld * p], aqq = H[qq + ld * qq];
                     // a rotation is skipped when apq is at the rounding level of the
                     // diagonal pair (a few ulps: rotations leave that much behind)
                     if (fabs(apq) > 1e-300 && fabs(apq) > 1e-15 * sqrt(fabs(app * aqq))) {
@@ -865,10 +872,10 @@ __global__ __launch_bounds__(1024) void jacobi_lds_kernel(const double *__restri
                 sP[i] = rsn[P];
                 const int r1 = p1[i] < n ? p1[i] : p0[i];
                 const int c1 = k1 ? q1 : q0;
-                h[i][0] = H[p0[i] + n * q0];
-                h[i][1] = H[p0[i] + n * c1];
-                h[i][2] = H[r1 + n * q0];
-                h[i][3] = H[r1 + n * c1];
+                h[i][0] = H[p0[i] + ld * q0];
+                h[i][1] = H[p0[i] + ld * c1];
+                h[i][2] = H[r1 + ld * q0];
+                h[i][3] = H[r1 + ld * c1];
                 const int ra = lane < n ? lane : n - 1, rb = lane + 64 < n ? lane + 64 : n - 1;
                 const int v1 = p1[i] < n ? p1[i] : p0[i];
                 v[i][0] = V[ra + n * p0[i]];
@@ -887,10 +894,10 @@ __global__ __launch_bounds__(1024) void jacobi_lds_kernel(const double *__restri
                     const double x00 = cQ * h00 - sQ * h01, x01 = sQ * h00 + cQ * h01;
                     const double x10 = cQ * h10 - sQ * h11, x11 = sQ * h10 + cQ * h11;
                     // rows: (y0, y1) = (cP x0 - sP x1, sP x0 + cP x1) per column
-                    H[p0[i] + n * q0] = cP[i] * x00 - sP[i] * x10;
-                    if (k1) H[p0[i] + n * q1] = cP[i] * x01 - sP[i] * x11;
-                    if (r1) H[p1[i] + n * q0] = sP[i] * x00 + cP[i] * x10;
-                    if (r1 && k1) H[p1[i] + n * q1] = sP[i] * x01 + cP[i] * x11;
+                    H[p0[i] + ld * q0] = cP[i] * x00 - sP[i] * x10;
+                    if (k1) H[p0[i] + ld * q1] = cP[i] * x01 - sP[i] * x11;
+                    if (r1) H[p1[i] + ld * q0] = sP[i] * x00 + cP[i] * x10;
+                    if (r1 && k1) H[p1[i] + ld * q1] = sP[i] * x01 + cP[i] * x11;
                 }
                 // V <- V J on pair P's columns (the padding pair has none to rotate)
                 if (r1) {
@@ -911,16 +918,16 @@ __global__ __launch_bounds__(1024) void jacobi_lds_kernel(const double *__restri
     }
     // descending order by rank (ties broken by index)
     for (int i = tid; i < n; i += nt) {
-        const double di = H[i + n * i];
+        const double di = H[i + ld * i];
         int rank = 0;
         for (int j = 0; j < n; j++) {
-            const double dj = H[j + n * j];
+            const double dj = H[j + ld * j];
             rank += (dj > di) || (dj == di && j < i);
         }
         order[rank] = i;
     }
     __syncthreads();
-    for (int i = tid; i < n; i += nt) w[i] = H[order[i] + n * order[i]];
+    for (int i = tid; i < n; i += nt) w[i] = H[order[i] + ld * order[i]];
     for (int idx = tid; idx < n * n; idx += nt) {
         const int r = idx % n, j = idx / n;
         Vout[idx] = V[r + n * order[j]];
