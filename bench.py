@@ -220,11 +220,14 @@ def mixed_1024x3(ctx, dt, shape=(1024,) * 3, ranks=(64,) * 3, noise=1e-3, decay=
     mixed_ms = _median_ms(run_mixed)
     run_mixed_hosvd_checked()
     checked_ms = _median_ms(run_mixed_hosvd_checked)
-    # breakdown of one mixed run
+    # breakdown of one mixed run and of one plain run
     ctx.timing(True)
     ctx.timing_reset()
     run_mixed(check=False)
     kt = {k: round(v[1], 3) for k, v in sorted(ctx.timing_list().items(), key=lambda kv: -kv[1][1])[:12]}
+    ctx.timing_reset()
+    run_plain()
+    kt_plain = {k: round(v[1], 3) for k, v in sorted(ctx.timing_list().items(), key=lambda kv: -kv[1][1])[:12]}
     ctx.timing(False)
     model.close()
 
@@ -236,7 +239,7 @@ def mixed_1024x3(ctx, dt, shape=(1024,) * 3, ranks=(64,) * 3, noise=1e-3, decay=
     out = {"workload": "dense HOOI 1024^3, core 64^3 decaying 0.93^k, noise 1e-3, ranks (64,64,64)",
            "reference_fit": target,
            "closed_form_fit": 1.0 - noise * math.sqrt(1 - np.prod(ranks) / np.prod(shape)) / math.sqrt(1 + noise ** 2),
-           "plain_fp32": {"sweeps": k_plain, "ms": plain_ms, "fit": 1.0 - res_plain[-1],
+           "plain_fp32": {"sweeps": k_plain, "ms": plain_ms, "fit": 1.0 - res_plain[-1], "kernel_ms": kt_plain,
                           "max_sin_angle_vs_reference": max(angle(a, b) for a, b in zip(U_plain, ref["U"])),
                           "max_sin_angle_vs_true": max(angle(a, b) for a, b in zip(U_plain, Ut))},
            "mixed_learned": {"sweeps_bf16": k_mixed, "ms": mixed_ms, "fit": fit_mixed_fp64[-1],
