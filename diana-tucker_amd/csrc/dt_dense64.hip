@@ -27,10 +27,15 @@ static inline dim3 gs_grid(int64_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// GEMM: C = alpha op(A) op(B) + beta C (column-major).  32x32 output tile per
-// workgroup, each thread 2x2 outputs, k staged in chunks of 16 through LDS.
-// With part != nullptr the workgroup (blockIdx.z) writes its unscaled partial
-// sum over its k range into part[z][i + m*j].
+// GEMM: C = alpha op(A) op(B) + beta C (column-major) on FP64 MFMA.  A
+// workgroup computes a 64 x 64 tile of C over its k range (blockIdx.z); wave w
+// owns the 32 x 32 quarter (w & 1, w >> 1) as 2 x 2 v_mfma_f64_16x16x4f64
+// tiles.  k is staged 16 at a time through LDS, op(A) as [row][k] and op(B) as
+// [k][column], so a transpose only changes which threads load which entries
+// (consecutive threads always read consecutive addresses).  MFMA step s, lane
+// (c, g): op(A)[row c][k 4s + g] and op(B)[k 4s + g][column c]; register v of
+// the result is row 4v + g, column c.  With part != nullptr the workgroup
+// writes its unscaled partial sum over its k range into part[z][i + m*j].
 __global__ __launch_bounds__(256) void gemm_f64_kernel(int tA, int tB, int64_t m, int64_t n,
                                                        int64_t k, double alpha,
                                                        const double *__restrict__ A, int64_t lda,
@@ -38,43 +43,34 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int tA, int tB, int64_t m
                                                        double beta, double *__restrict__ C,
                                                        int64_t ldc, int64_t kchunk,
                                                        double *__restrict__ part) {
-    __shared__ double As[32][17];
-    __shared__ double Bs[16][33];
-    const int64_t i0 = (int64_t)blockIdx.x * 32;
-    const int64_t j0 = (int64_t)blockIdx.y * 32;
+    __shared__ double As[64][17];
+    __shared__ double Bs[16][65];
+    const int64_t i0 = (int64_t)blockIdx.x * 64;
+    const int64_t j0 = (int64_t)blockIdx.y * 64;
     const int64_t ks = (int64_t)blockIdx.z * kchunk;
-    int64_t ke = ks + kchunk;
-    if (ke > k) ke = k;
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    const int64_t ke = ks + kchunk < k ? ks + kchunk : k;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int wm = (wv & 1) * 32, wn = (wv >> 1) * 32;
+    doublex4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = doublex4{0.0, 0.0, 0.0, 0.0};
     for (int64_t k0 = ks; k0 < ke; k0 += 16) {
 #pragma unroll
-        for (int q = 0; q < 2; q++) {
-            const int e = threadIdx.x + 256 * q;
-            int i, kk;
-            if (tA) {
-                kk = e & 15;
-                i = e >> 4;
-            } else {
-                i = e & 31;
-                kk = e >> 5;
-            }
+        for (int q = 0; q < 4; q++) {
+            const int e = tid + 256 * q;
+            const int i = tA ? e >> 4 : e & 63, kk = tA ? e & 15 : e >> 6;
             const int64_t gi = i0 + i, gk = k0 + kk;
             double v = 0.0;
             if (gi < m && gk < ke) v = tA ? A[gk + lda * gi] : A[gi + lda * gk];
             As[i][kk] = v;
         }
 #pragma unroll
-        for (int q = 0; q < 2; q++) {
-            const int e = threadIdx.x + 256 * q;
-            int j, kk;
-            if (tB) {
-                j = e & 31;
-                kk = e >> 5;
-            } else {
-                kk = e & 15;
-                j = e >> 4;
-            }
+        for (int q = 0; q < 4; q++) {
+            const int e = tid + 256 * q;
+            const int j = tB ? e & 63 : e >> 4, kk = tB ? e >> 6 : e & 15;
             const int64_t gj = j0 + j, gk = k0 + kk;
             double v = 0.0;
             if (gj < n && gk < ke) v = tB ? B[gj + ldb * gk] : B[gk + ldb * gj];
@@ -82,29 +78,31 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(int tA, int tB, int64_t m
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < 16; kk++) {
-            const double a0 = As[tx][kk], a1 = As[tx + 16][kk];
-            const double b0 = Bs[kk][ty], b1 = Bs[kk][ty + 16];
-            acc[0][0] = fma(a0, b0, acc[0][0]);
-            acc[0][1] = fma(a0, b1, acc[0][1]);
-            acc[1][0] = fma(a1, b0, acc[1][0]);
-            acc[1][1] = fma(a1, b1, acc[1][1]);
+        for (int s = 0; s < 4; s++) {
+            const double a0 = As[wm + c][4 * s + g], a1 = As[wm + 16 + c][4 * s + g];
+            const double b0 = Bs[4 * s + g][wn + c], b1 = Bs[4 * s + g][wn + 16 + c];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
         }
         __syncthreads();
     }
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) {
-            const int64_t i = i0 + tx + 16 * a, j = j0 + ty + 16 * b;
-            if (i >= m || j >= n) continue;
-            if (part) {
-                part[(int64_t)blockIdx.z * m * n + i + m * j] = acc[a][b];
-            } else {
-                const double old = beta == 0.0 ? 0.0 : beta * C[i + ldc * j];
-                C[i + ldc * j] = alpha * acc[a][b] + old;
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int64_t i = i0 + wm + 16 * a + 4 * v + g, j = j0 + wn + 16 * b + c;
+                if (i >= m || j >= n) continue;
+                if (part) {
+                    part[(int64_t)blockIdx.z * m * n + i + m * j] = acc[a][b][v];
+                } else {
+                    const double old = beta == 0.0 ? 0.0 : beta * C[i + ldc * j];
+                    C[i + ldc * j] = alpha * acc[a][b][v] + old;
+                }
             }
-        }
 }
 
 __global__ void gemm_reduce(const double *__restrict__ part, int nsplit, int64_t m, int64_t n,
@@ -133,15 +131,15 @@ int dt_gemm_f64_internal(dt_ctx *ctx, int tA, int tB, size_t m, size_t n, size_t
                          const double *A, size_t lda, const double *B, size_t ldb, double beta,
                          double *C, size_t ldc) {
     if (m == 0 || n == 0) return DT_OK;
-    const int64_t tiles = ceil_div((int64_t)m, 32) * ceil_div((int64_t)n, 32);
+    // split k so that about 512 workgroups run, each over at least 64 of k
+    const int64_t tiles = ceil_div((int64_t)m, 64) * ceil_div((int64_t)n, 64);
     int64_t nsplit = 1;
-    if (k > 2048 && tiles < 512) {
-        nsplit = std::min<int64_t>(ceil_div((int64_t)k, 1024), std::max<int64_t>(1, 2048 / tiles));
-    }
+    if (tiles < 512 && k >= 128)
+        nsplit = std::min<int64_t>(ceil_div((int64_t)k, 64), ceil_div((int64_t)512, tiles));
     int64_t kchunk = ceil_div(ceil_div((int64_t)k, nsplit), 16) * 16;
     if (kchunk == 0) kchunk = 16;
     nsplit = std::max<int64_t>(1, ceil_div((int64_t)k, kchunk));
-    dim3 grid((unsigned)ceil_div((int64_t)m, 32), (unsigned)ceil_div((int64_t)n, 32),
+    dim3 grid((unsigned)ceil_div((int64_t)m, 64), (unsigned)ceil_div((int64_t)n, 64),
               (unsigned)nsplit);
     if (nsplit == 1) {
         DT_LAUNCH("gemm_f64", gemm_f64_kernel, grid, dim3(256), 0, tA, tB, (int64_t)m, (int64_t)n,
@@ -228,17 +226,25 @@ constexpr int LDS_NMAX = 96;
 __global__ __launch_bounds__(512) void chol_lds_kernel(const double *__restrict__ G, int n, double shift,
                                                       double *__restrict__ R, double *__restrict__ Rinv,
                                                       int *__restrict__ info) {
-    // W and X with an odd leading dimension ld: threads that walk a row across
-    // columns (stride ld doubles) fall on different LDS banks
+    // W (upper part: the working matrix, ending as R) and Z (lower part: the
+    // identity under the same row operations, ending as R^{-T}) with an odd
+    // leading dimension ld, so threads that walk a row across columns (stride
+    // ld doubles) fall on different LDS banks.  Pivot j: every row i > j of
+    // the trailing block loses (W[j][i] / W[j][j]) times row j, on W and on Z,
+    // then row j is scaled by W[j][j]^{-1/2}; so E G = R for the product E of
+    // these operations, E = R^{-T} is what Z becomes, and no triangular solve
+    // is needed.  Threads form a 32 x 16 grid (no index divisions).
     __shared__ double W[LDS_NMAX * (LDS_NMAX + 1)];
-    __shared__ double X[LDS_NMAX * (LDS_NMAX + 1)];
+    __shared__ double Z[LDS_NMAX * (LDS_NMAX + 1)];
     __shared__ int bad;
     const int tid = threadIdx.x, nt = blockDim.x;
+    const int tx = tid & 31, ty = tid >> 5;  // 32 x 16
     const int ld = n | 1;
-    for (int idx = tid; idx < n * n; idx += nt) {
-        const int i = idx % n, j = idx / n;
-        W[i + ld * j] = (i <= j) ? G[idx] + (i == j ? shift : 0.0) : 0.0;
-    }
+    for (int j = ty; j < n; j += 16)
+        for (int i = tx; i < n; i += 32) {
+            W[i + ld * j] = (i <= j) ? G[i + (size_t)n * j] + (i == j ? shift : 0.0) : 0.0;
+            Z[i + ld * j] = i == j ? 1.0 : 0.0;
+        }
     if (tid == 0) bad = 0;
     __syncthreads();
     for (int j = 0; j < n; j++) {
@@ -248,36 +254,33 @@ __global__ __launch_bounds__(512) void chol_lds_kernel(const double *__restrict_
             if (tid == 0) bad = j + 1;
             break;  // uniform: every thread read the same d
         }
-        const double rjj = sqrt(d);
-        // scale row j right of the diagonal (threads own distinct k)
-        for (int k = j + 1 + tid; k < n; k += nt) W[j + ld * k] /= rjj;
-        __syncthreads();
-        if (tid == 0) W[j + ld * j] = rjj;
-        const int m = n - j - 1;
-        for (int idx = tid; idx < m * m; idx += nt) {
-            const int i = j + 1 + idx % m, k = j + 1 + idx / m;
-            if (i <= k) W[i + ld * k] -= W[j + ld * i] * W[j + ld * k];
+        const double inv_d = 1.0 / d;
+        // trailing block of W (upper part, j < i <= k) and rows i > j of Z
+        // (columns 0..j, Z[i][c] at Z[i + ld c])
+        for (int k = j + 1 + ty; k < n; k += 16) {
+            const double fk = W[j + ld * k] * inv_d;
+            for (int i = j + 1 + tx; i <= k; i += 32) W[i + ld * k] -= W[j + ld * i] * fk;
         }
+        for (int c = ty; c <= j; c += 16) {
+            const double zj = Z[j + ld * c];
+            for (int i = j + 1 + tx; i < n; i += 32) Z[i + ld * c] -= W[j + ld * i] * inv_d * zj;
+        }
+        __syncthreads();
+        const double rs = 1.0 / sqrt(d);
+        for (int k = j + 1 + tid; k < n; k += nt) W[j + ld * k] *= rs;
+        for (int c = tid; c <= j; c += nt) Z[j + ld * c] *= rs;
+        if (tid == 0) W[j + ld * j] = d * rs;
         __syncthreads();
     }
     __syncthreads();
     if (tid == 0) info[0] = bad;
     if (bad) return;
-    // Rinv by columns (thread c: upper-triangular back substitution in LDS)
-    for (int c = tid; c < n; c += nt) {
-        for (int i = c + 1; i < n; i++) X[i + ld * c] = 0.0;
-        for (int i = c; i >= 0; i--) {
-            double sacc = (i == c) ? 1.0 : 0.0;
-            for (int k = i + 1; k <= c; k++) sacc -= W[i + ld * k] * X[k + ld * c];
-            X[i + ld * c] = sacc / W[i + ld * i];
+    // R = upper part of W; R^{-1}[i][c] = Z[c][i] (upper)
+    for (int j = ty; j < n; j += 16)
+        for (int i = tx; i < n; i += 32) {
+            R[i + (size_t)n * j] = i <= j ? W[i + ld * j] : 0.0;
+            Rinv[i + (size_t)n * j] = i <= j ? Z[j + ld * i] : 0.0;
         }
-    }
-    __syncthreads();
-    for (int idx = tid; idx < n * n; idx += nt) {
-        const int i = idx % n, j = idx / n;
-        R[idx] = W[i + ld * j];
-        Rinv[idx] = X[i + ld * j];
-    }
 }
 
 static int launch_chol(dt_ctx *ctx, const double *G, size_t n, double shift, double *R, double *Rinv,

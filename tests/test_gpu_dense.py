@@ -250,7 +250,9 @@ def test_sumsq(ctx):
 
 
 @pytest.mark.parametrize("tA,tB,m,n,k", [(0, 0, 33, 17, 70), (1, 0, 16, 16, 5000),
-                                         (0, 1, 64, 40, 8), (1, 1, 9, 70, 31)])
+                                         (0, 1, 64, 40, 8), (1, 1, 9, 70, 31),
+                                         (0, 0, 1024, 80, 1024), (1, 0, 80, 80, 1024), (0, 0, 1024, 80, 80),
+                                         (0, 1, 130, 65, 200), (1, 1, 129, 67, 3000), (0, 0, 5000, 100, 100)])
 def test_gemm_f64(ctx, tA, tB, m, n, k):
     rng = np.random.default_rng(10)
     A = rng.standard_normal((k, m) if tA else (m, k))
