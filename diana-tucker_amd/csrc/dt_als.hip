@@ -157,52 +157,40 @@ __global__ __launch_bounds__(ALS_THREADS) void als_rows_kernel(
     }
 }
 
-// part[w] = X[rows_w, :]ᵀ X[rows_w, :] for ALS_GR-row blocks w of X (I x R),
-// staged through LDS 64 rows at a time.
+// part[w] = X[rows_w, :]ᵀ X[rows_w, :] for ALS_GR-row blocks w of X (I x R).
 template <int RMAX>
 __global__ __launch_bounds__(ALS_THREADS) void als_gram_kernel(const double *__restrict__ X, int I, int R,
                                                                double *__restrict__ part) {
-    constexpr int CR = 64;
-    constexpr int NB = (RMAX + 15) / 16;  // entries (tx + 16 a, ty + 16 b), a, b < NB
-    __shared__ double Xs[CR][RMAX + 1];
-    const int tid = threadIdx.x;
-    const int tx = tid & 15, ty = tid >> 4;
+    // partial Gram XᵀX of rows [rbeg, rend): the block is staged in LDS once
+    // (coalesced loads; rows past rend and columns past R zero), then 16 x 16
+    // tiles of the R x R result are dealt to the four waves in turn on FP64
+    // MFMA: tile (ti, tj), step s, lane (c, g): Xs[4s + g][16 ti + c] as A and
+    // Xs[4s + g][16 tj + c] as B; register v is entry (16 ti + 4v + g, 16 tj + c)
+    constexpr int NT = RMAX / 16;
+    __shared__ double Xs[ALS_GR][RMAX + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int c = lane & 15, g = lane >> 4;
     const int rbeg = blockIdx.x * ALS_GR;
     const int rend = (rbeg + ALS_GR) < I ? (rbeg + ALS_GR) : I;
-    double acc[NB][NB];
-#pragma unroll
-    for (int a = 0; a < NB; a++)
-#pragma unroll
-        for (int b = 0; b < NB; b++) acc[a][b] = 0.0;
-    for (int r0 = rbeg; r0 < rend; r0 += CR) {
-        for (int e = tid; e < CR * R; e += ALS_THREADS) {
-            const int r = e % CR, j = e / CR;
-            Xs[r][j] = r0 + r < rend ? X[(size_t)(r0 + r) + (size_t)I * j] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int r = 0; r < CR; r++) {
-            double xi[NB], xj[NB];
-#pragma unroll
-            for (int a = 0; a < NB; a++) {
-                xi[a] = Xs[r][(tx + 16 * a) < R ? tx + 16 * a : 0];
-                xj[a] = Xs[r][(ty + 16 * a) < R ? ty + 16 * a : 0];
-            }
-#pragma unroll
-            for (int a = 0; a < NB; a++)
-#pragma unroll
-                for (int b = 0; b < NB; b++) acc[a][b] = fma(xi[a], xj[b], acc[a][b]);
-        }
-        __syncthreads();
+    for (int e = tid; e < ALS_GR * RMAX; e += ALS_THREADS) {
+        const int r = e % ALS_GR, j = e / ALS_GR;
+        Xs[r][j] = (rbeg + r < rend && j < R) ? X[(size_t)(rbeg + r) + (size_t)I * j] : 0.0;
     }
+    __syncthreads();
     double *p = part + (size_t)blockIdx.x * R * R;
+    for (int t = wv; t < NT * NT; t += 4) {
+        const int ti = t % NT, tj = t / NT;
+        doublex4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+        for (int s = 0; s < ALS_GR / 4; s++)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Xs[4 * s + g][16 * ti + c], Xs[4 * s + g][16 * tj + c], acc,
+                                                       0, 0, 0);
 #pragma unroll
-    for (int a = 0; a < NB; a++)
-#pragma unroll
-        for (int b = 0; b < NB; b++) {
-            const int i = tx + 16 * a, j = ty + 16 * b;
-            if (i < R && j < R) p[i + (size_t)R * j] = acc[a][b];
+        for (int v = 0; v < 4; v++) {
+            const int i = 16 * ti + 4 * v + g, j = 16 * tj + c;
+            if (i < R && j < R) p[i + (size_t)R * j] = acc[v];
         }
+    }
 }
 
 
