@@ -1,7 +1,7 @@
 // Row-wise MLP of the learned start: forward, backward and Adam on the GPU.
 //
-// The reference has no learned component; this is the north-star's learned
-// initialisation.  One network maps each row x (d_in features of a mode's
+// The reference has no learned component; this is the learned start that
+// replaces the HOSVD initialisation.  One network maps each row x (d_in features of a mode's
 // sketch) to d_out values:
 //     h1 = relu(x W1 + b1),  h2 = relu(h1 W2 + b2),  z = x[:d_out] + h2 W3 + b3
 // W1: d_in x H, W2: H x H, W3: H x d_out, all row-major; the residual term
