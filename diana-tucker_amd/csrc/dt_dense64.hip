@@ -766,66 +766,102 @@ __global__ __launch_bounds__(256) void jacobi_kernel(double *__restrict__ H, int
 }
 
 // Cyclic Jacobi with H and V resident in LDS (n <= LDS_NMAX), same
-// round-robin ordering; the off-diagonal norm is a block reduction.  A round's
-// rotations act on disjoint index pairs, so H <- Jᵀ H J splits into 2 x 2
-// blocks (pair P's rows, pair Q's columns) that one thread each updates in
-// place, column rotation first and row rotation second (the operations, and
-// so the results, of rotating all columns and then all rows); V <- V J in the
-// same phase.  Two barriers per round.
-// One matrix per workgroup: workgroup b solves Hin + b n^2 into Vout + b n^2, w + b n.
-__global__ __launch_bounds__(1024) void jacobi_lds_kernel(const double *__restrict__ Hin, int n,
-                                                        double *__restrict__ Vout, double *__restrict__ w) {
+// round-robin ordering as jacobi_kernel.  H is kept as its upper triangle
+// (the input symmetrised on load; element (i, j) at min + ld * max).  A
+// round's rotations act on disjoint index pairs, so H <- Jᵀ H J splits into
+// 2 x 2 blocks (the rows of pair P, the columns of pair Q) and only the blocks
+// with P <= Q are distinct: one thread updates each in place, column rotation
+// first and row rotation second, and every thread also rotates a fixed set of
+// (row, pair) entries of V <- V J.  The block and V assignments are computed
+// once, so no lane idles and the LDS traffic of a round is the upper triangle
+// and V once each.  Two barriers per round (rotations, then the updates).
+// One matrix per workgroup of JAC_THREADS: workgroup b solves Hin + b n^2
+// into Vout + b n^2, w + b n.  NMAX (<= LDS_NMAX) bounds n: up to 88 every
+// thread owns one block and at most four V entries, whose reads are all
+// issued together; above, two blocks and five V entries, the V entries one
+// at a time (registers).
+constexpr int JAC_THREADS = 1024;
+constexpr int JAC_NSMALL = 88;
+template <int NMAX>
+__global__ __launch_bounds__(JAC_THREADS) void jacobi_lds_kernel(const double *__restrict__ Hin, int n,
+                                                               double *__restrict__ Vout, double *__restrict__ w) {
     Hin += (size_t)blockIdx.x * n * n;
     Vout += (size_t)blockIdx.x * n * n;
     w += (size_t)blockIdx.x * n;
-    // H with an odd leading dimension: lanes that read one row across columns
-    // (stride ld doubles) then fall on different LDS banks
+    // odd leading dimension: lanes that walk a row across columns (stride ld
+    // doubles) fall on different LDS banks
     __shared__ double H[LDS_NMAX * (LDS_NMAX + 1)];
     __shared__ double V[LDS_NMAX * LDS_NMAX];
-    __shared__ double rc[LDS_NMAX / 2], rsn[LDS_NMAX / 2];
-    __shared__ int rp[LDS_NMAX / 2], rq[LDS_NMAX / 2];
-    __shared__ double red_off[16], red_tot[16];
+    __shared__ double2 rcs[LDS_NMAX / 2];  // (c, s) of pair P this round
+    __shared__ int2 rpq[LDS_NMAX / 2];     // (p, q), p < q; q == n is the padding index
+    __shared__ double red_off[16], red_dg[16];
     __shared__ int order[LDS_NMAX];
     __shared__ int rotated;  // a rotation of this sweep changed something
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+    constexpr int NT = JAC_THREADS, NW = NT / 64;
+    static_assert(NMAX <= LDS_NMAX, "order above the LDS arrays");
+    constexpr int MAXPAIR = (NMAX + 1) / 2;
+    constexpr int KH = (MAXPAIR * (MAXPAIR + 1) / 2 + NT - 1) / NT;  // blocks per thread
+    constexpr int KV = (NMAX * MAXPAIR + NT - 1) / NT;              // V entries per thread
+    constexpr bool VHOIST = KH == 1;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
     const int np = (n + 1) & ~1, npair = np / 2;  // npair <= 48 < 64
     const int ld = n | 1;
-    for (int j = wv; j < n; j += nw)
-        for (int i = lane; i < n; i += 64) {
-            H[i + ld * j] = Hin[i + n * j];
-            V[i + n * j] = i == j ? 1.0 : 0.0;
-        }
+    const int nblk = npair * (npair + 1) / 2, nvit = n * npair;
+    // block b = Q (Q + 1) / 2 + P, P <= Q; V entry e = row + n P
+    int bP[KH], bQ[KH], vr[KV], vP[KV];
+#pragma unroll
+    for (int k = 0; k < KH; k++) {
+        const int b = tid + NT * k < nblk ? tid + NT * k : nblk - 1;
+        int Q = (int)((sqrtf(8.0f * (float)b + 1.0f) - 1.0f) * 0.5f);
+        while (Q > 0 && Q * (Q + 1) / 2 > b) Q--;
+        while ((Q + 1) * (Q + 2) / 2 <= b) Q++;
+        bQ[k] = Q;
+        bP[k] = b - Q * (Q + 1) / 2;
+    }
+#pragma unroll
+    for (int k = 0; k < KV; k++) {
+        const int e = tid + NT * k < nvit ? tid + NT * k : nvit - 1;
+        vP[k] = e / n;
+        vr[k] = e - vP[k] * n;
+    }
+    for (int j = wv; j < n; j += NW)
+        for (int i = lane; i <= j; i += 64) H[i + ld * j] = 0.5 * (Hin[i + n * j] + Hin[j + n * i]);
+    for (int e = tid; e < n * n; e += NT) V[e] = (e % n) == (e / n) ? 1.0 : 0.0;
     __syncthreads();
     for (int sweep = 0; sweep < 40; sweep++) {
-        double off = 0.0, tot = 0.0;
-        for (int j = wv; j < n; j += nw)
-            for (int i = lane; i < n; i += 64) {
+        double off = 0.0, dg = 0.0;
+        for (int j = wv; j < n; j += NW)
+            for (int i = lane; i <= j; i += 64) {
                 const double h = H[i + ld * j];
-                tot += h * h;
-                if (i != j) off += h * h;
+                if (i == j)
+                    dg += h * h;
+                else
+                    off += 2.0 * h * h;
             }
         for (int o = 32; o > 0; o >>= 1) {
             off += __shfl_xor(off, o);
-            tot += __shfl_xor(tot, o);
+            dg += __shfl_xor(dg, o);
         }
         if (lane == 0) {
             red_off[wv] = off;
-            red_tot[wv] = tot;
+            red_dg[wv] = dg;
         }
         __syncthreads();
         off = 0.0;
-        tot = 0.0;
-        for (int i = 0; i < nw; i++) {
+        dg = 0.0;
+        for (int i = 0; i < NW; i++) {
             off += red_off[i];
-            tot += red_tot[i];
+            dg += red_dg[i];
         }
         __syncthreads();  // red_* reused next sweep
+        const double tot = dg + off;
         if (!(off > 1e-30 * tot) || tot == 0.0) break;  // uniform
         if (tid == 0) rotated = 0;
         __syncthreads();
         for (int round = 0; round < np - 1; round++) {
-            for (int q = tid; q < npair; q += nt) {
+            if (tid < npair) {
+                const int q = tid;
                 int a, b;
                 if (q == 0) {
                     a = 0;
@@ -849,69 +885,92 @@ __global__ __launch_bounds__(1024) void jacobi_lds_kernel(const double *__restri
                         sn = t * c;
                     }
                 }
-                rp[q] = p;
-                rq[q] = qq;
-                rc[q] = c;
-                rsn[q] = sn;
+                rpq[q] = make_int2(p, qq);
+                rcs[q] = make_double2(c, sn);
             }
             __syncthreads();
-            // H blocks (pair P of rows, pair Q = lane of columns) and V columns
-            // (pair P, rows lane and lane + 64); the waves take P = wv, wv + 16,
-            // wv + 32 (npair <= 48), and every LDS read of the phase is issued
-            // before the arithmetic
-            constexpr int JP = (LDS_NMAX / 2 + 15) / 16;
-            const int Q = lane < npair ? lane : npair - 1;
-            const int q0 = rp[Q], q1 = rq[Q];
-            const double cQ = rc[Q], sQ = rsn[Q];
-            const bool k1 = q1 < n;
-            int p0[JP], p1[JP];
-            double cP[JP], sP[JP], h[JP][4], v[JP][4];
+            // H: every LDS read of the blocks is issued before the arithmetic;
+            // blocks past the thread's share read valid addresses and store nothing
+            int2 pa[KH], pb[KH];
+            double2 ca[KH], cb[KH];
 #pragma unroll
-            for (int i = 0; i < JP; i++) {
-                const int P = wv + 16 * i < npair ? wv + 16 * i : npair - 1;
-                p0[i] = rp[P];
-                p1[i] = rq[P];
-                cP[i] = rc[P];
-                sP[i] = rsn[P];
-                const int r1 = p1[i] < n ? p1[i] : p0[i];
-                const int c1 = k1 ? q1 : q0;
-                h[i][0] = H[p0[i] + ld * q0];
-                h[i][1] = H[p0[i] + ld * c1];
-                h[i][2] = H[r1 + ld * q0];
-                h[i][3] = H[r1 + ld * c1];
-                const int ra = lane < n ? lane : n - 1, rb = lane + 64 < n ? lane + 64 : n - 1;
-                const int v1 = p1[i] < n ? p1[i] : p0[i];
-                v[i][0] = V[ra + n * p0[i]];
-                v[i][1] = V[ra + n * v1];
-                v[i][2] = V[rb + n * p0[i]];
-                v[i][3] = V[rb + n * v1];
+            for (int k = 0; k < KH; k++) {
+                pa[k] = rpq[bP[k]];
+                pb[k] = rpq[bQ[k]];
+                ca[k] = rcs[bP[k]];
+                cb[k] = rcs[bQ[k]];
+            }
+            int2 vpq[VHOIST ? KV : 1];
+            double2 vcs[VHOIST ? KV : 1];
+            if constexpr (VHOIST) {
+#pragma unroll
+                for (int k = 0; k < KV; k++) {
+                    vpq[k] = rpq[vP[k]];
+                    vcs[k] = rcs[vP[k]];
+                }
+            }
+            int adr[KH][4];
+            double h[KH][4];
+#pragma unroll
+            for (int k = 0; k < KH; k++) {
+                const int p0 = pa[k].x, p1 = pa[k].y, q0 = pb[k].x, q1 = pb[k].y;
+                const bool r1 = p1 < n, k1 = q1 < n;
+                auto at = [ld](int i, int j) { return i <= j ? i + ld * j : j + ld * i; };
+                adr[k][0] = at(p0, q0);
+                adr[k][1] = k1 ? at(p0, q1) : adr[k][0];
+                adr[k][2] = r1 ? at(p1, q0) : adr[k][0];
+                adr[k][3] = (r1 && k1) ? at(p1, q1) : adr[k][0];
+#pragma unroll
+                for (int u = 0; u < 4; u++) h[k][u] = H[adr[k][u]];
+            }
+            double v[VHOIST ? KV : 1][2];
+            if constexpr (VHOIST) {
+#pragma unroll
+                for (int k = 0; k < KV; k++) {
+                    const int p1 = vpq[k].y < n ? vpq[k].y : vpq[k].x;
+                    v[k][0] = V[vr[k] + n * vpq[k].x];
+                    v[k][1] = V[vr[k] + n * p1];
+                }
             }
 #pragma unroll
-            for (int i = 0; i < JP; i++) {
-                if (wv + 16 * i >= npair) break;  // uniform
-                const bool r1 = p1[i] < n;
-                if (lane < npair) {
-                    const double h00 = h[i][0], h01 = k1 ? h[i][1] : 0.0;
-                    const double h10 = r1 ? h[i][2] : 0.0, h11 = (r1 && k1) ? h[i][3] : 0.0;
-                    // columns: (x0, x1) = (cQ h0 - sQ h1, sQ h0 + cQ h1) per row
-                    const double x00 = cQ * h00 - sQ * h01, x01 = sQ * h00 + cQ * h01;
-                    const double x10 = cQ * h10 - sQ * h11, x11 = sQ * h10 + cQ * h11;
-                    // rows: (y0, y1) = (cP x0 - sP x1, sP x0 + cP x1) per column
-                    H[p0[i] + ld * q0] = cP[i] * x00 - sP[i] * x10;
-                    if (k1) H[p0[i] + ld * q1] = cP[i] * x01 - sP[i] * x11;
-                    if (r1) H[p1[i] + ld * q0] = sP[i] * x00 + cP[i] * x10;
-                    if (r1 && k1) H[p1[i] + ld * q1] = sP[i] * x01 + cP[i] * x11;
-                }
-                // V <- V J on pair P's columns (the padding pair has none to rotate)
-                if (r1) {
-                    const double c = cP[i], sn = sP[i];
-                    if (lane < n) {
-                        V[lane + n * p0[i]] = c * v[i][0] - sn * v[i][1];
-                        V[lane + n * p1[i]] = sn * v[i][0] + c * v[i][1];
+            for (int k = 0; k < KH; k++) {
+                if (tid + NT * k >= nblk) break;
+                const bool r1 = pa[k].y < n, k1 = pb[k].y < n, diag = bP[k] == bQ[k];
+                const double cP = ca[k].x, sP = ca[k].y, cQ = cb[k].x, sQ = cb[k].y;
+                const double h00 = h[k][0], h01 = k1 ? h[k][1] : 0.0;
+                const double h10 = r1 ? h[k][2] : 0.0, h11 = (r1 && k1) ? h[k][3] : 0.0;
+                // columns: (x0, x1) = (cQ h0 - sQ h1, sQ h0 + cQ h1) per row
+                const double x00 = cQ * h00 - sQ * h01, x01 = sQ * h00 + cQ * h01;
+                const double x10 = cQ * h10 - sQ * h11, x11 = sQ * h10 + cQ * h11;
+                // rows: (y0, y1) = (cP x0 - sP x1, sP x0 + cP x1) per column
+                H[adr[k][0]] = cP * x00 - sP * x10;
+                if (k1) H[adr[k][1]] = cP * x01 - sP * x11;
+                // a diagonal block's (p1, p0) is its (p0, p1), stored once
+                if (r1 && !diag) H[adr[k][2]] = sP * x00 + cP * x10;
+                if (r1 && k1) H[adr[k][3]] = sP * x01 + cP * x11;
+            }
+            // V <- V J on pair P's columns (the padding pair has none to rotate)
+            if constexpr (VHOIST) {
+#pragma unroll
+                for (int k = 0; k < KV; k++) {
+                    if (tid + NT * k >= nvit) break;
+                    if (vpq[k].y < n) {
+                        const double c = vcs[k].x, sn = vcs[k].y;
+                        V[vr[k] + n * vpq[k].x] = c * v[k][0] - sn * v[k][1];
+                        V[vr[k] + n * vpq[k].y] = sn * v[k][0] + c * v[k][1];
                     }
-                    if (lane + 64 < n) {
-                        V[lane + 64 + n * p0[i]] = c * v[i][2] - sn * v[i][3];
-                        V[lane + 64 + n * p1[i]] = sn * v[i][2] + c * v[i][3];
+                }
+            } else {
+                // one entry's loads and stores at a time keeps the registers in bounds
+#pragma unroll
+                for (int k = 0; k < KV; k++) {
+                    if (tid + NT * k >= nvit) break;
+                    const int2 pq = rpq[vP[k]];
+                    if (pq.y < n) {
+                        const double2 cs = rcs[vP[k]];
+                        const double v0 = V[vr[k] + n * pq.x], v1 = V[vr[k] + n * pq.y];
+                        V[vr[k] + n * pq.x] = cs.x * v0 - cs.y * v1;
+                        V[vr[k] + n * pq.y] = cs.y * v0 + cs.x * v1;
                     }
                 }
             }
@@ -920,7 +979,7 @@ __global__ __launch_bounds__(1024) void jacobi_lds_kernel(const double *__restri
         if (!rotated) break;  // uniform: read after the round's barrier
     }
     // descending order by rank (ties broken by index)
-    for (int i = tid; i < n; i += nt) {
+    for (int i = tid; i < n; i += NT) {
         const double di = H[i + ld * i];
         int rank = 0;
         for (int j = 0; j < n; j++) {
@@ -930,8 +989,8 @@ __global__ __launch_bounds__(1024) void jacobi_lds_kernel(const double *__restri
         order[rank] = i;
     }
     __syncthreads();
-    for (int i = tid; i < n; i += nt) w[i] = H[order[i] + ld * order[i]];
-    for (int idx = tid; idx < n * n; idx += nt) {
+    for (int i = tid; i < n; i += NT) w[i] = H[order[i] + ld * order[i]];
+    for (int idx = tid; idx < n * n; idx += NT) {
         const int r = idx % n, j = idx / n;
         Vout[idx] = V[r + n * order[j]];
     }
@@ -939,10 +998,20 @@ __global__ __launch_bounds__(1024) void jacobi_lds_kernel(const double *__restri
 
 // `count` independent n x n problems (n <= LDS_NMAX) stored back to back, one
 // workgroup each.
+static int launch_jacobi_lds(dt_ctx *ctx, double *H, size_t n, int count, double *V, double *w) {
+    if (n <= (size_t)JAC_NSMALL) {
+        DT_LAUNCH("jacobi", jacobi_lds_kernel<JAC_NSMALL>, dim3((unsigned)count), dim3(JAC_THREADS), 0, H, (int)n,
+                  V, w);
+    } else {
+        DT_LAUNCH("jacobi", jacobi_lds_kernel<LDS_NMAX>, dim3((unsigned)count), dim3(JAC_THREADS), 0, H, (int)n, V,
+                  w);
+    }
+    return DT_OK;
+}
+
 int dt_jacobi_eig_batched(dt_ctx *ctx, double *H, size_t n, int count, double *V, double *w) {
     if (n > (size_t)LDS_NMAX || count < 1) return dt_fail(ctx, DT_EUNSUPPORTED, "batched Jacobi: n %zu", n);
-    DT_LAUNCH("jacobi", jacobi_lds_kernel, dim3((unsigned)count), dim3(1024), 0, H, (int)n, V, w);
-    return DT_OK;
+    return launch_jacobi_lds(ctx, H, n, count, V, w);
 }
 
 int dt_jacobi_eig(dt_ctx *ctx, double *H, size_t n, double *V, double *w) {
@@ -951,8 +1020,7 @@ int dt_jacobi_eig(dt_ctx *ctx, double *H, size_t n, double *V, double *w) {
     if (!rot) return DT_ENOMEM;
     int *order = (int *)(rot + 2 * np + 8);
     if (n <= (size_t)LDS_NMAX) {
-        DT_LAUNCH("jacobi", jacobi_lds_kernel, dim3(1), dim3(1024), 0, H, (int)n, V, w);
-        return DT_OK;
+        return launch_jacobi_lds(ctx, H, n, 1, V, w);
     }
     DT_LAUNCH("jacobi", jacobi_kernel, dim3(1), dim3(256), 0, H, (int)n, V, w, rot, order);
     return DT_OK;

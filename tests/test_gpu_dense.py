@@ -358,7 +358,8 @@ def test_inverse_singular_reports_enumeric(ctx):
     assert e.value.code == dt.DT_ENUMERIC
 
 
-@pytest.mark.parametrize("n,r", [(8, 3), (33, 5), (64, 8), (81, 9), (150, 20), (256, 16), (1024, 64)])
+@pytest.mark.parametrize("n,r", [(1, 1), (2, 1), (3, 2), (8, 3), (33, 5), (64, 8), (81, 9), (95, 7), (96, 12),
+                                 (150, 20), (256, 16), (1024, 64)])
 def test_syev_top(ctx, n, r):
     print("syev", n, r, flush=True)
     rng = np.random.default_rng(14)
