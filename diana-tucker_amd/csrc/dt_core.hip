@@ -83,6 +83,14 @@ int dt_create(dt_ctx **out, int device) {
     }
     hipEventCreate(&ctx->t0);
     hipEventCreate(&ctx->t1);
+    // side stream for overlapped work: default priority, like the main one;
+    // without it side work runs on the main stream
+    if (hipStreamCreateWithFlags(&ctx->side_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->side_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->side_done, hipEventDisableTiming) != hipSuccess) {
+        if (ctx->side_stream) hipStreamDestroy(ctx->side_stream);
+        ctx->side_stream = nullptr;
+    }
     if (hipHostMalloc((void **)&ctx->pinned, 4096, hipHostMallocDefault) != hipSuccess) {
         ctx->pinned = nullptr;
     }
@@ -107,6 +115,12 @@ int dt_destroy(dt_ctx *ctx) {
     if (ctx->t0) hipEventDestroy(ctx->t0);
     if (ctx->t1) hipEventDestroy(ctx->t1);
     if (ctx->pinned) hipHostFree(ctx->pinned);
+    if (ctx->side_stream) {
+        hipStreamSynchronize(ctx->side_stream);
+        hipStreamDestroy(ctx->side_stream);
+    }
+    if (ctx->side_fork) hipEventDestroy(ctx->side_fork);
+    if (ctx->side_done) hipEventDestroy(ctx->side_done);
     hipStreamDestroy(ctx->stream);
     delete ctx;
     return DT_OK;

@@ -525,10 +525,12 @@ static inline dim3 gs_grid(int64_t n) {
     return dim3((unsigned)g);
 }
 
-// Sum of squares, result left on the device in out_dev[0].
-static int sumsq_to_dev(dt_ctx *ctx, const void *A, int dt, size_t n, double *out_dev) {
-    const int nb = 2048;
-    double *part = (double *)dt_ws(ctx, WS_SUMSQ, nb * sizeof(double));
+// Sum of squares, result left on the device in out_dev[0]; block partials in
+// workspace slot `slot`.
+constexpr int SUMSQ_BLOCKS = 2048;
+static int sumsq_to_dev(dt_ctx *ctx, const void *A, int dt, size_t n, double *out_dev, int slot = WS_SUMSQ) {
+    const int nb = SUMSQ_BLOCKS;
+    double *part = (double *)dt_ws(ctx, slot, nb * sizeof(double));
     if (!part) return DT_ENOMEM;
     if (dt == DT_F32 && (n % 4) == 0 && (((uintptr_t)A) & 15) == 0) {
         DT_LAUNCH("sumsq", sumsq_f32x4, dim3(nb), dim3(256), 0, (const float4 *)A, (int64_t)(n / 4),
@@ -582,6 +584,68 @@ int dt_sumsq_dev(dt_ctx *ctx, const void *A, int dt, size_t n, double *out_host)
     DT_HIP(hipMemcpyAsync(ctx->pinned, dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     DT_HIP(hipStreamSynchronize(ctx->stream));
     *out_host = ctx->pinned[0];
+    return DT_OK;
+}
+
+int dt_side_sumsq_setup(dt_ctx *ctx, int count, const void *const *src, const int *dt, const size_t *len) {
+    DT_TRY(dt_side_join(ctx, nullptr));  // a job left queued is finished first
+    DT_CHECK_ARG(count >= 1 && count <= 2, "side work: %d sums", count);
+    // buffers now, so that no allocation (and its implicit synchronisation) happens later
+    auto &j = ctx->side;
+    j.out_dev = (double *)dt_ws(ctx, WS_SIDE_OUT, 64);
+    for (int i = 0; i < count; i++)
+        if (!dt_ws(ctx, WS_SIDE0 + i, SUMSQ_BLOCKS * sizeof(double))) return DT_ENOMEM;
+    if (!j.out_dev) return DT_ENOMEM;
+    for (int i = 0; i < count; i++) {
+        j.src[i] = src[i];
+        j.dt[i] = dt[i];
+        j.len[i] = len[i];
+    }
+    j.count = count;
+    j.kicked = false;
+    return DT_OK;
+}
+
+int dt_side_kick(dt_ctx *ctx) {
+    auto &j = ctx->side;
+    if (j.count == 0 || j.kicked) return DT_OK;
+    j.kicked = true;
+    hipStream_t main_stream = ctx->stream;
+    const bool timing = ctx->timing;
+    if (ctx->side_stream) {
+        // the side stream starts after all the main stream holds so far
+        DT_HIP(hipEventRecord(ctx->side_fork, main_stream));
+        DT_HIP(hipStreamWaitEvent(ctx->side_stream, ctx->side_fork, 0));
+        ctx->stream = ctx->side_stream;  // the launches below go to the side stream, untimed
+        ctx->timing = false;
+    }
+    int st = DT_OK;
+    for (int i = 0; i < j.count && st == DT_OK; i++)
+        st = j.len[i] == 0 ? DT_OK : sumsq_to_dev(ctx, j.src[i], j.dt[i], j.len[i], j.out_dev + i, WS_SIDE0 + i);
+    ctx->stream = main_stream;
+    ctx->timing = timing;
+    if (st == DT_OK && ctx->side_stream) DT_HIP(hipEventRecord(ctx->side_done, ctx->side_stream));
+    return st;
+}
+
+int dt_side_join(dt_ctx *ctx, double *out_host) {
+    auto &j = ctx->side;
+    if (j.count == 0) return DT_OK;
+    int st = dt_side_kick(ctx);
+    if (ctx->side_stream && j.kicked) {
+        // waits even after a failed kick, so no side launch outlives the job
+        if (hipStreamWaitEvent(ctx->stream, ctx->side_done, 0) != hipSuccess) hipStreamSynchronize(ctx->side_stream);
+    }
+    const int count = j.count;
+    j.count = 0;
+    j.kicked = false;
+    DT_TRY(st);
+    if (!out_host) return DT_OK;
+    for (int i = 0; i < count; i++)
+        if (j.len[i] == 0) DT_HIP(hipMemsetAsync(j.out_dev + i, 0, sizeof(double), ctx->stream));
+    DT_HIP(hipMemcpyAsync(ctx->pinned, j.out_dev, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    DT_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < count; i++) out_host[i] = ctx->pinned[i];
     return DT_OK;
 }
 

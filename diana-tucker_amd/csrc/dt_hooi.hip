@@ -416,6 +416,7 @@ static int init_factors(dt_hooi *h) {
             for (int i = 0; i < n; i++) F *= h->I[i];
             for (int i = n + 1; i < N; i++) B *= h->I[i];
             DT_TRY(dt_gram_internal(ctx, src, sdt, F, h->I[n], B, h->S));
+            DT_TRY(dt_side_kick(ctx));  // the eigen step is latency-bound: the norms overlap it
             DT_TRY(dt_syev_top_f64(ctx, h->S, (size_t)h->I[n], (size_t)h->R[n], h->U64[n], nullptr));
         }
     } else if (a.init == DT_INIT_LEARNED) {
@@ -525,15 +526,15 @@ int dt_hooi_create(dt_ctx *ctx, const dt_hooi_args *args, dt_hooi **out) {
         dt_hooi_destroy(h);
         return st;
     }
-    // ||A||: of the FP32 copy when one is given (mixed precision), else of A.
-    const void *nsrc = args->A_f32 ? (const void *)args->A_f32 : args->A;
-    const int ndt = args->A_f32 ? DT_F32 : args->dtype;
-    st = dt_sumsq_dev(ctx, nsrc, ndt, (size_t)h->total, &h->normA2);
-    if (st == DT_OK && !(h->normA2 > 0.0 && std::isfinite(h->normA2)))
-        st = dt_fail(ctx, DT_ENUMERIC, "hooi: ||A|| is zero or not finite");
-    h->normA2_sweep = h->normA2;
-    if (st == DT_OK && args->A_f32 && args->dtype != DT_F32)
-        st = dt_sumsq_dev(ctx, args->A, args->dtype, (size_t)h->total, &h->normA2_sweep);
+    // ||A|| of the FP32 copy when one is given (mixed precision), else of A,
+    // and ||A||^2 of the BF16 tensor the sweeps read: side work that the start
+    // launches where its eigen step leaves HBM idle (dt_side_kick), joined
+    // before the start returns
+    const void *nsrc[2] = {args->A_f32 ? (const void *)args->A_f32 : args->A, args->A};
+    const int ndt[2] = {args->A_f32 ? DT_F32 : args->dtype, args->dtype};
+    const size_t nlen[2] = {(size_t)h->total, (size_t)h->total};
+    const int nsum = (args->A_f32 && args->dtype != DT_F32) ? 2 : 1;
+    st = dt_side_sumsq_setup(ctx, nsum, nsrc, ndt, nlen);
     if (st == DT_OK) {
         hipEventRecord(ctx->t0, ctx->stream);
         st = init_factors(h);
@@ -542,6 +543,15 @@ int dt_hooi_create(dt_ctx *ctx, const dt_hooi_args *args, dt_hooi **out) {
         float ms = 0.f;
         hipEventElapsedTime(&ms, ctx->t0, ctx->t1);
         h->init_ms = ms;
+    }
+    double norms[2] = {0.0, 0.0};
+    const int jst = dt_side_join(ctx, st == DT_OK ? norms : nullptr);
+    if (st == DT_OK) st = jst;
+    if (st == DT_OK) {
+        h->normA2 = norms[0];
+        h->normA2_sweep = nsum == 2 ? norms[1] : norms[0];
+        if (!(h->normA2 > 0.0 && std::isfinite(h->normA2)))
+            st = dt_fail(ctx, DT_ENUMERIC, "hooi: ||A|| is zero or not finite");
     }
     if (st != DT_OK) {
         dt_hooi_destroy(h);

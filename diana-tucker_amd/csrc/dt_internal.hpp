@@ -49,6 +49,20 @@ struct dt_ctx {
     // dt_create through dt_noalloc_setup; null with the reason in noalloc_err)
     void *noalloc_scratch = nullptr;
     std::string noalloc_err;
+    // Side work: sums of squares that read only the input tensor, queued by
+    // dt_side_sumsq_setup, started on side_stream (default priority) by
+    // dt_side_kick where the main stream enters a phase that leaves HBM idle,
+    // and waited for by dt_side_join before their results are read.
+    hipStream_t side_stream = nullptr;
+    hipEvent_t side_fork = nullptr, side_done = nullptr;
+    struct {
+        int count = 0;
+        bool kicked = false;
+        const void *src[2] = {};
+        int dt[2] = {};
+        size_t len[2] = {};
+        double *out_dev = nullptr;  // count doubles
+    } side;
 };
 // looks up this context's device address of the allocation-free scratch
 int dt_noalloc_setup(dt_ctx *ctx);
@@ -84,6 +98,9 @@ enum dt_ws_slot {
     WS_MLP2,
     WS_SKETCH,         // learned start: every mode's sketch, Gram and eigenvectors
     WS_CHOLQR,         // dt_qr_f64's multi-workgroup Cholesky-QR: partial Grams, R^{-1}, flag
+    WS_SIDE0,          // side work: block partials of the first and second sum, results
+    WS_SIDE1,
+    WS_SIDE_OUT,
     WS_COUNT_
 };
 static_assert(WS_COUNT_ <= dt_ctx::kSlots, "too many workspace slots");
@@ -200,6 +217,12 @@ int dt_ttm_views(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t K, int6
 int dt_gram_internal(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t I, int64_t B,
                      double *S);
 int dt_sumsq_dev(dt_ctx *ctx, const void *A, int dt, size_t n, double *out_host);
+// Side work (dt_ctx::side): queue the sums of squares of up to two tensors;
+// start them (no-op when none is queued or they already run); wait for them
+// and copy the sums to out_host (nullptr discards them).
+int dt_side_sumsq_setup(dt_ctx *ctx, int count, const void *const *src, const int *dt, const size_t *len);
+int dt_side_kick(dt_ctx *ctx);
+int dt_side_join(dt_ctx *ctx, double *out_host);
 int dt_transpose_f64(dt_ctx *ctx, double *B, const double *A, size_t m, size_t n);
 int dt_f64_to_views(dt_ctx *ctx, const double *U64, size_t K, size_t J, float *U32, float *M32,
                     uint16_t *Ub16, double *M64);

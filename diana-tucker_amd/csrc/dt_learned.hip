@@ -264,7 +264,9 @@ extern "C" int dt_learned_start(dt_ctx *ctx, dt_mlp *mlp, const void *A, int dtA
         DT_TRY(sketch_gram(ctx, sw[n], I, S, G + (size_t)n * S * S));
         off += I;
     }
-    // the eigen-decompositions of all modes in one launch
+    // the eigen-decompositions of all modes in one launch (a few workgroups:
+    // queued side work, such as the tensor norms, runs beside it)
+    DT_TRY(dt_side_kick(ctx));
     DT_TRY(dt_jacobi_eig_batched(ctx, G, (size_t)S, ndim, V, w));
     off = 0;
     for (int n = 0; n < ndim; n++) {
