@@ -38,7 +38,7 @@ struct dt_ctx {
     std::map<std::string, dt_kstat> stats;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     // Workspace slots: each slot is a device buffer that only grows.
-    static constexpr int kSlots = 32;
+    static constexpr int kSlots = 40;
     void *ws[kSlots] = {};
     size_t ws_size[kSlots] = {};
     // Host-pinned scalar staging area for small device->host results.
@@ -98,6 +98,7 @@ enum dt_ws_slot {
     WS_MLP2,
     WS_SKETCH,         // learned start: every mode's sketch, Gram and eigenvectors
     WS_CHOLQR,         // dt_qr_f64's multi-workgroup Cholesky-QR: partial Grams, R^{-1}, flag
+    WS_TTM_SWZ,        // mode-0 BF16 product: the factor in MFMA operand order
     WS_SIDE0,          // side work: block partials of the first and second sum, results
     WS_SIDE1,
     WS_SIDE_OUT,

@@ -442,6 +442,141 @@ __global__ __launch_bounds__(256) void ttm_m0_bf16(const uint16_t *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// F == 1, BF16 input, the factor held in LDS (K (16 JB) bf16 <= 128 KB).  The
+// operand layout of v_mfma_f32_16x16x32_bf16 puts 16 columns x 64 bytes in
+// one load (half of each 128-byte line, like a random row gather); here each
+// load instruction takes 8 columns x 128 bytes (lane L: column L / 8, bytes
+// 16 (L % 8)), the wave stores its 32 columns of the step to its own LDS tile
+// (column stride 144 bytes, so a 16-lane group of the operand reads below
+// covers all 64 banks) and reads them back in the operand layout.  The
+// factor, pre-arranged in operand order (m0_factor_swizzle), is loaded into
+// LDS once per workgroup.  Persistent: wave w of workgroup b takes column
+// groups (32 columns) 4 b + w, + 4 G, ... (G workgroups) as one stream of
+// 64-k steps, PD of them in flight, so the vector-memory queue holds only the
+// tensor's loads and never drains at a group boundary.
+constexpr int M0L_CST = 144;         // LDS bytes per column of a wave's tile
+constexpr int M0L_UMAX = 8192;       // u32x4 of the factor in LDS (128 KB): K JB <= 4096
+
+// Um[((s * JB + jb) * 2 + h) * 64 + lane] = U rows k = 64 s + 32 h + 8 g .. + 7
+// of column j = 16 jb + c as 8 bf16 (zero for j >= J), lane = 16 g + c.
+__global__ void m0_factor_swizzle(const uint16_t *__restrict__ Ub, int K, int J, int JB, u32x4 *__restrict__ Um) {
+    const int64_t total = (int64_t)(K / 64) * JB * 2 * 64;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int lane = (int)(e & 63);
+        const int64_t q = e >> 6;
+        const int h = (int)(q & 1);
+        const int jb = (int)((q >> 1) % JB);
+        const int s = (int)((q >> 1) / JB);
+        const int c = lane & 15, g = lane >> 4;
+        const int j = 16 * jb + c;
+        u32x4 v = {0u, 0u, 0u, 0u};
+        if (j < J) v = *(const u32x4 *)(Ub + (int64_t)K * j + 64 * s + 32 * h + 8 * g);
+        Um[e] = v;
+    }
+}
+
+// B % 32 == 0 (ngroups = B / 32), K % (64 PD) == 0, K JB <= 4096.
+template <int JB, int PD, bool NT>
+__global__ __launch_bounds__(256) void ttm_m0_bf16_lds(const uint16_t *__restrict__ A,
+                                                       const u32x4 *__restrict__ Um,
+                                                       float *__restrict__ Y, int K, int64_t ngroups, int J) {
+    constexpr int CB = 2;            // 16-column blocks per group
+    constexpr int NL = 2 * CB;       // load instructions per step (8 columns each)
+    constexpr int USL = JB * 2 * 64; // u32x4 per step of the factor
+    __shared__ u32x4 Us[M0L_UMAX];
+    __shared__ __attribute__((aligned(16))) unsigned char tile[4][16 * CB * M0L_CST];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int c = lane & 15;
+    const int g = lane >> 4;
+    const int nsteps = K / 64;
+    for (int e = tid; e < nsteps * USL; e += 256) Us[e] = Um[e];
+    __syncthreads();
+    const int64_t gstride = (int64_t)gridDim.x * 4;
+    const int64_t g0 = (int64_t)blockIdx.x * 4 + wave;
+    if (g0 >= ngroups) return;  // after the only workgroup barrier
+    unsigned char *T = tile[wave];
+    const int64_t mygroups = (ngroups - g0 + gstride - 1) / gstride;
+
+    // load cursor (group lg, step ls), PD steps ahead of the compute cursor;
+    // past the wave's last group it reloads its first group (not used)
+    const int lc = lane >> 3, lq = lane & 7;
+    const uint32_t off0 = (uint32_t)(K * lc + 8 * lq);
+    int64_t lg = g0;
+    int ls = 0;
+    u32x4 r[PD][NL];
+    auto load = [&](int p) {
+        const int64_t gg = lg < ngroups ? lg : g0;
+        const uint16_t *Aw = A + (int64_t)K * (32 * gg);
+#pragma unroll
+        for (int i = 0; i < NL; i++)
+            r[p][i] = ld_a((const u32x4 *)(Aw + (off0 + (uint32_t)(8 * i * K + 64 * ls))), NT);
+        const bool wrap = ls + 1 == nsteps;  // selects, no branch: the wait counts stay exact
+        lg = wrap ? lg + gstride : lg;
+        ls = wrap ? 0 : ls + 1;
+    };
+#pragma unroll
+    for (int p = 0; p < PD; p++) load(p);
+    floatx4 acc[JB][CB];
+#pragma unroll
+    for (int jb = 0; jb < JB; jb++)
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++) acc[jb][cb] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int64_t t = 0; t < mygroups; t++) {
+        const int64_t cg = g0 + t * gstride;
+        for (int s0 = 0; s0 < nsteps; s0 += PD) {
+#pragma unroll
+            for (int p = 0; p < PD; p++) {
+                const int s = s0 + p;
+#pragma unroll
+                for (int i = 0; i < NL; i++) *(u32x4 *)(T + (8 * i + lc) * M0L_CST + 16 * lq) = r[p][i];
+                load(p);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                u32x4 a[2][CB], u[2][JB];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+#pragma unroll
+                    for (int jb = 0; jb < JB; jb++) u[h][jb] = Us[(s * JB + jb) * 128 + h * 64 + lane];
+#pragma unroll
+                    for (int cb = 0; cb < CB; cb++)
+                        a[h][cb] = *(const u32x4 *)(T + (16 * cb + c) * M0L_CST + 16 * (4 * h + g));
+                }
+#pragma unroll
+                for (int h = 0; h < 2; h++)
+#pragma unroll
+                    for (int jb = 0; jb < JB; jb++)
+#pragma unroll
+                        for (int cb = 0; cb < CB; cb++) acc[jb][cb] = mfma_bf16_32(u[h][jb], a[h][cb], acc[jb][cb]);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
+        // group done: its 32 columns of Y, then a fresh accumulator
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++) {
+            const int64_t b = 32 * cg + 16 * cb + c;
+#pragma unroll
+            for (int jb = 0; jb < JB; jb++) {
+                const int j = 16 * jb + 4 * g;
+                float *y = Y + j + (int64_t)J * b;
+                if ((J & 3) == 0 && j + 3 < J) {
+                    *(float4 *)y = make_float4(acc[jb][cb][0], acc[jb][cb][1], acc[jb][cb][2], acc[jb][cb][3]);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        if (j + v < J) y[v] = acc[jb][cb][v];
+                }
+                acc[jb][cb] = floatx4{0.f, 0.f, 0.f, 0.f};
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // F > 1, BF16 input.  Wave = 128 consecutive f (8 per lane column) for one b.
 // Each lane loads 8 k-rows of 8 f-contiguous bf16 and transposes the 8x8 block
 // in registers so that MFMA t receives the 8 k values of f = f0 + 8c + t.
@@ -728,6 +863,42 @@ int dt_ttm_views(dt_ctx *ctx, const void *A, int dtA, int64_t F, int64_t K, int6
         return DT_OK;
     }
     if (dtA == DT_BF16) {
+        // whole-line loads through LDS with the factor resident there: B % 32 == 0,
+        // K % 128 == 0 and the factor's operand copy within 128 KB
+        const int jbv = jbn == 1 ? 1 : (jbn == 2 ? 2 : 4);
+        if (J <= 64 && F == 1 && (K % 128) == 0 && B % 32 == 0 && (int64_t)K * jbv <= 4096 && fv.Ub16) {
+            u32x4 *Um = (u32x4 *)dt_ws(ctx, WS_TTM_SWZ, (size_t)K * 16 * jbv * sizeof(uint16_t));
+            if (!Um) return DT_ENOMEM;
+            const int64_t nsw = (K / 64) * jbv * 2 * 64;
+            DT_LAUNCH("ttm_m0_swizzle", m0_factor_swizzle, dim3((unsigned)std::min<int64_t>(ceil_div(nsw, 256), 1024)),
+                      dim3(256), 0, fv.Ub16, (int)K, (int)J, jbv, Um);
+            const int64_t ngroups = B / 32;
+            // one workgroup per CU (the factor fills the LDS); PD: steps in flight per wave
+            int cus = 0;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus < 1)
+                cus = 256;
+            dim3 grid((unsigned)std::min<int64_t>(cus, ceil_div(ngroups, 4)));
+            const int nst = (int)(K / 64);
+            const int pdv = nst % 8 == 0 ? 8 : (nst % 4 == 0 ? 4 : 2);
+#define M0L(JBV, PDV)                                                                          \
+    if (stream) {                                                                              \
+        DT_LAUNCH("ttm_m0_bf16_stream", (ttm_m0_bf16_lds<JBV, PDV, true>), grid, dim3(256), 0, \
+                  (const uint16_t *)A, Um, (float *)Y, (int)K, ngroups, (int)J);               \
+    } else {                                                                                   \
+        DT_LAUNCH("ttm_m0_bf16", (ttm_m0_bf16_lds<JBV, PDV, false>), grid, dim3(256), 0,       \
+                  (const uint16_t *)A, Um, (float *)Y, (int)K, ngroups, (int)J);               \
+    }                                                                                          \
+    return DT_OK;
+#define M0LP(JBV)                   \
+    if (pdv == 8) { M0L(JBV, 8) }   \
+    if (pdv == 4) { M0L(JBV, 4) }   \
+    M0L(JBV, 2)
+            if (jbv == 1) { M0LP(1) }
+            if (jbv == 2) { M0LP(2) }
+            M0LP(4)
+#undef M0LP
+#undef M0L
+        }
         if (J <= 64 && F == 1 && (K % 8) == 0 && fv.Ub16) {
             dim3 grid((unsigned)ceil_div(B, 256));
 #define M0B(JBV)                                                                          \

@@ -53,6 +53,12 @@ TTM_CASES = [
     ((256, 32, 24), 0, 64), ((32, 256, 24), 1, 64), ((32, 24, 256), 2, 64),
     ((96, 80, 16), 0, 8), ((96, 80, 16), 2, 8), ((40, 36, 20), 1, 33), ((40, 36, 20), 0, 70),
     ((16, 256, 64), 1, 16), ((5, 4, 3), 1, 4), ((7, 9, 6, 5), 2, 3), ((128, 64, 8, 8), 3, 48),
+    # first mode with K % 128 == 0: for BF16 the persistent kernel with the factor
+    # in LDS when B % 32 == 0 (groups of 32 columns: 5 leave waves idle, 1100 give
+    # waves two groups), else the direct loads (280, 65); J of one, two and four
+    # 16-blocks
+    ((128, 40, 7), 0, 33), ((512, 64, 16), 0, 64), ((384, 32, 16), 0, 20), ((256, 64, 8), 0, 7),
+    ((256, 13, 5), 0, 7), ((128, 32, 5), 0, 16), ((128, 64, 550), 0, 64),
 ]
 
 
