@@ -771,15 +771,15 @@ __global__ __launch_bounds__(256) void jacobi_kernel(double *__restrict__ H, int
 // round's rotations act on disjoint index pairs, so H <- Jᵀ H J splits into
 // 2 x 2 blocks (the rows of pair P, the columns of pair Q) and only the blocks
 // with P <= Q are distinct: one thread updates each in place, column rotation
-// first and row rotation second, and every thread also rotates a fixed set of
-// (row, pair) entries of V <- V J.  The block and V assignments are computed
-// once, so no lane idles and the LDS traffic of a round is the upper triangle
-// and V once each.  Two barriers per round (rotations, then the updates).
+// first and row rotation second.  V <- V J of a round is applied during the
+// next round's rotation phase, by waves 1..15 while wave 0 computes the
+// rotations (the parameters of consecutive rounds alternate between two
+// buffers), and after the last round.  The block and V assignments are
+// computed once, so the LDS traffic of a round is the upper triangle and V
+// once each.  Two barriers per round (rotations and V, then H).
 // One matrix per workgroup of JAC_THREADS: workgroup b solves Hin + b n^2
 // into Vout + b n^2, w + b n.  NMAX (<= LDS_NMAX) bounds n: up to 88 every
-// thread owns one block and at most four V entries, whose reads are all
-// issued together; above, two blocks and five V entries, the V entries one
-// at a time (registers).
+// thread owns one block of H, above two.
 constexpr int JAC_THREADS = 1024;
 constexpr int JAC_NSMALL = 88;
 template <int NMAX>
@@ -792,8 +792,8 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_lds_kernel(const double *_
     // doubles) fall on different LDS banks
     __shared__ double H[LDS_NMAX * (LDS_NMAX + 1)];
     __shared__ double V[LDS_NMAX * LDS_NMAX];
-    __shared__ double2 rcs[LDS_NMAX / 2];  // (c, s) of pair P this round
-    __shared__ int2 rpq[LDS_NMAX / 2];     // (p, q), p < q; q == n is the padding index
+    __shared__ double2 rcs[2][LDS_NMAX / 2];  // (c, s) of pair P, rounds alternating
+    __shared__ int2 rpq[2][LDS_NMAX / 2];     // (p, q), p < q; q == n is the padding index
     __shared__ double red_off[16], red_dg[16];
     __shared__ int order[LDS_NMAX];
     __shared__ int rotated;  // a rotation of this sweep changed something
@@ -801,14 +801,14 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_lds_kernel(const double *_
     static_assert(NMAX <= LDS_NMAX, "order above the LDS arrays");
     constexpr int MAXPAIR = (NMAX + 1) / 2;
     constexpr int KH = (MAXPAIR * (MAXPAIR + 1) / 2 + NT - 1) / NT;  // blocks per thread
-    constexpr int KV = (NMAX * MAXPAIR + NT - 1) / NT;              // V entries per thread
-    constexpr bool VHOIST = KH == 1;
+    constexpr int NVT = NT - 64;                                      // threads of the V update
+    constexpr int KV = (NMAX * MAXPAIR + NVT - 1) / NVT;             // V entries per thread
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
     const int np = (n + 1) & ~1, npair = np / 2;  // npair <= 48 < 64
     const int ld = n | 1;
     const int nblk = npair * (npair + 1) / 2, nvit = n * npair;
-    // block b = Q (Q + 1) / 2 + P, P <= Q; V entry e = row + n P
+    // block b = Q (Q + 1) / 2 + P, P <= Q; V entry e = row + n P, e = tid - 64 + NVT k
     int bP[KH], bQ[KH], vr[KV], vP[KV];
 #pragma unroll
     for (int k = 0; k < KH; k++) {
@@ -819,16 +819,46 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_lds_kernel(const double *_
         bQ[k] = Q;
         bP[k] = b - Q * (Q + 1) / 2;
     }
+    const int vt = tid - 64;
 #pragma unroll
     for (int k = 0; k < KV; k++) {
-        const int e = tid + NT * k < nvit ? tid + NT * k : nvit - 1;
+        const int e0 = vt + NVT * k;
+        const int e = e0 >= 0 && e0 < nvit ? e0 : nvit - 1;
         vP[k] = e / n;
         vr[k] = e - vP[k] * n;
     }
+    // V <- V J with the rotations in buffer `buf` (waves 1..15; reads issued together)
+    auto apply_v = [&](int buf) {
+        int2 vpq[KV];
+        double2 vcs[KV];
+        double v[KV][2];
+#pragma unroll
+        for (int k = 0; k < KV; k++) {
+            vpq[k] = rpq[buf][vP[k]];
+            vcs[k] = rcs[buf][vP[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < KV; k++) {
+            const int p1 = vpq[k].y < n ? vpq[k].y : vpq[k].x;
+            v[k][0] = V[vr[k] + n * vpq[k].x];
+            v[k][1] = V[vr[k] + n * p1];
+        }
+#pragma unroll
+        for (int k = 0; k < KV; k++) {
+            if (vt + NVT * k >= nvit) break;
+            if (vpq[k].y < n) {  // the padding pair has no column to rotate
+                const double c = vcs[k].x, sn = vcs[k].y;
+                V[vr[k] + n * vpq[k].x] = c * v[k][0] - sn * v[k][1];
+                V[vr[k] + n * vpq[k].y] = sn * v[k][0] + c * v[k][1];
+            }
+        }
+    };
     for (int j = wv; j < n; j += NW)
         for (int i = lane; i <= j; i += 64) H[i + ld * j] = 0.5 * (Hin[i + n * j] + Hin[j + n * i]);
     for (int e = tid; e < n * n; e += NT) V[e] = (e % n) == (e / n) ? 1.0 : 0.0;
     __syncthreads();
+    int gr = 0;            // rounds run so far, over all sweeps
+    bool pending = false;  // V of round gr - 1 not yet applied
     for (int sweep = 0; sweep < 40; sweep++) {
         double off = 0.0, dg = 0.0;
         for (int j = wv; j < n; j += NW)
@@ -859,34 +889,39 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_lds_kernel(const double *_
         if (!(off > 1e-30 * tot) || tot == 0.0) break;  // uniform
         if (tid == 0) rotated = 0;
         __syncthreads();
-        for (int round = 0; round < np - 1; round++) {
-            if (tid < npair) {
-                const int q = tid;
-                int a, b;
-                if (q == 0) {
-                    a = 0;
-                    b = 1 + (round % (np - 1));
-                } else {
-                    a = 1 + ((round + q) % (np - 1));
-                    b = 1 + ((round + np - 1 - q) % (np - 1));
-                }
-                const int p = a < b ? a : b, qq = a < b ? b : a;
-                double c = 1.0, sn = 0.0;  // identity for the pair holding the padding index n
-                if (qq < n) {
-                    const double apq = H[p + ld * qq];
-                    const double app = H[p + ld * p], aqq = H[qq + ld * qq];
-                    // a rotation is skipped when apq is at the rounding level of the
-                    // diagonal pair (a few ulps: rotations leave that much behind)
-                    if (fabs(apq) > 1e-300 && fabs(apq) > 1e-15 * sqrt(fabs(app * aqq))) {
-                        rotated = 1;
-                        const double tau = (aqq - app) / (2.0 * apq);
-                        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                        c = 1.0 / sqrt(1.0 + t * t);
-                        sn = t * c;
+        for (int round = 0; round < np - 1; round++, gr++) {
+            const int buf = gr & 1;
+            if (wv == 0) {
+                if (tid < npair) {
+                    const int q = tid;
+                    int a, b;
+                    if (q == 0) {
+                        a = 0;
+                        b = 1 + (round % (np - 1));
+                    } else {
+                        a = 1 + ((round + q) % (np - 1));
+                        b = 1 + ((round + np - 1 - q) % (np - 1));
                     }
+                    const int p = a < b ? a : b, qq = a < b ? b : a;
+                    double c = 1.0, sn = 0.0;  // identity for the pair holding the padding index n
+                    if (qq < n) {
+                        const double apq = H[p + ld * qq];
+                        const double app = H[p + ld * p], aqq = H[qq + ld * qq];
+                        // a rotation is skipped when apq is at the rounding level of the
+                        // diagonal pair (a few ulps: rotations leave that much behind)
+                        if (fabs(apq) > 1e-300 && fabs(apq) > 1e-15 * sqrt(fabs(app * aqq))) {
+                            rotated = 1;
+                            const double tau = (aqq - app) / (2.0 * apq);
+                            const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                            c = 1.0 / sqrt(1.0 + t * t);
+                            sn = t * c;
+                        }
+                    }
+                    rpq[buf][q] = make_int2(p, qq);
+                    rcs[buf][q] = make_double2(c, sn);
                 }
-                rpq[q] = make_int2(p, qq);
-                rcs[q] = make_double2(c, sn);
+            } else if (pending) {
+                apply_v(buf ^ 1);
             }
             __syncthreads();
             // H: every LDS read of the blocks is issued before the arithmetic;
@@ -895,19 +930,10 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_lds_kernel(const double *_
             double2 ca[KH], cb[KH];
 #pragma unroll
             for (int k = 0; k < KH; k++) {
-                pa[k] = rpq[bP[k]];
-                pb[k] = rpq[bQ[k]];
-                ca[k] = rcs[bP[k]];
-                cb[k] = rcs[bQ[k]];
-            }
-            int2 vpq[VHOIST ? KV : 1];
-            double2 vcs[VHOIST ? KV : 1];
-            if constexpr (VHOIST) {
-#pragma unroll
-                for (int k = 0; k < KV; k++) {
-                    vpq[k] = rpq[vP[k]];
-                    vcs[k] = rcs[vP[k]];
-                }
+                pa[k] = rpq[buf][bP[k]];
+                pb[k] = rpq[buf][bQ[k]];
+                ca[k] = rcs[buf][bP[k]];
+                cb[k] = rcs[buf][bQ[k]];
             }
             int adr[KH][4];
             double h[KH][4];
@@ -922,15 +948,6 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_lds_kernel(const double *_
                 adr[k][3] = (r1 && k1) ? at(p1, q1) : adr[k][0];
 #pragma unroll
                 for (int u = 0; u < 4; u++) h[k][u] = H[adr[k][u]];
-            }
-            double v[VHOIST ? KV : 1][2];
-            if constexpr (VHOIST) {
-#pragma unroll
-                for (int k = 0; k < KV; k++) {
-                    const int p1 = vpq[k].y < n ? vpq[k].y : vpq[k].x;
-                    v[k][0] = V[vr[k] + n * vpq[k].x];
-                    v[k][1] = V[vr[k] + n * p1];
-                }
             }
 #pragma unroll
             for (int k = 0; k < KH; k++) {
@@ -949,34 +966,14 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_lds_kernel(const double *_
                 if (r1 && !diag) H[adr[k][2]] = sP * x00 + cP * x10;
                 if (r1 && k1) H[adr[k][3]] = sP * x01 + cP * x11;
             }
-            // V <- V J on pair P's columns (the padding pair has none to rotate)
-            if constexpr (VHOIST) {
-#pragma unroll
-                for (int k = 0; k < KV; k++) {
-                    if (tid + NT * k >= nvit) break;
-                    if (vpq[k].y < n) {
-                        const double c = vcs[k].x, sn = vcs[k].y;
-                        V[vr[k] + n * vpq[k].x] = c * v[k][0] - sn * v[k][1];
-                        V[vr[k] + n * vpq[k].y] = sn * v[k][0] + c * v[k][1];
-                    }
-                }
-            } else {
-                // one entry's loads and stores at a time keeps the registers in bounds
-#pragma unroll
-                for (int k = 0; k < KV; k++) {
-                    if (tid + NT * k >= nvit) break;
-                    const int2 pq = rpq[vP[k]];
-                    if (pq.y < n) {
-                        const double2 cs = rcs[vP[k]];
-                        const double v0 = V[vr[k] + n * pq.x], v1 = V[vr[k] + n * pq.y];
-                        V[vr[k] + n * pq.x] = cs.x * v0 - cs.y * v1;
-                        V[vr[k] + n * pq.y] = cs.y * v0 + cs.x * v1;
-                    }
-                }
-            }
+            pending = true;
             __syncthreads();
         }
         if (!rotated) break;  // uniform: read after the round's barrier
+    }
+    if (pending) {
+        if (wv > 0) apply_v((gr - 1) & 1);
+        __syncthreads();
     }
     // descending order by rank (ties broken by index)
     for (int i = tid; i < n; i += NT) {
