@@ -889,19 +889,20 @@ __global__ __launch_bounds__(JAC_THREADS) void jacobi_lds_kernel(const double *_
         if (!(off > 1e-30 * tot) || tot == 0.0) break;  // uniform
         if (tid == 0) rotated = 0;
         __syncthreads();
+        // thread q < npair of wave 0 pairs a = 1 + (round + q) mod (np - 1) with
+        // b = 1 + (round - q) mod (np - 1) (thread 0: 0 with 1 + round), both
+        // stepped by one per round without a division
+        const int m1 = np - 1;
+        int ra = tid % m1, rb = tid == 0 ? 0 : m1 - tid % m1;
         for (int round = 0; round < np - 1; round++, gr++) {
             const int buf = gr & 1;
             if (wv == 0) {
                 if (tid < npair) {
                     const int q = tid;
-                    int a, b;
-                    if (q == 0) {
-                        a = 0;
-                        b = 1 + (round % (np - 1));
-                    } else {
-                        a = 1 + ((round + q) % (np - 1));
-                        b = 1 + ((round + np - 1 - q) % (np - 1));
-                    }
+                    const int a = q == 0 ? 0 : 1 + ra;
+                    const int b = 1 + rb;
+                    ra = ra + 1 == m1 ? 0 : ra + 1;
+                    rb = rb + 1 == m1 ? 0 : rb + 1;
                     const int p = a < b ? a : b, qq = a < b ? b : a;
                     double c = 1.0, sn = 0.0;  // identity for the pair holding the padding index n
                     if (qq < n) {
