@@ -16,9 +16,12 @@
  *
  * Device memory: pointers marked "device" are allocations made with dt_malloc()
  * (or any hipMalloc'd memory on the context's device).  All work is enqueued on
- * the context's single HIP stream; entry points that return host values
- * synchronise that stream.  A context must be used from one host thread at a
- * time (the reference is driven by one thread, spmain.cpp:10-15).
+ * the context's HIP stream; entry points that return host values synchronise
+ * that stream.  dt_hooi_create also runs the tensor norms on a second stream of
+ * the context (default priority) beside the start's eigen step and waits for
+ * them before it returns, so no work of a call is left on that stream.  A
+ * context must be used from one host thread at a time (the reference is driven
+ * by one thread, spmain.cpp:10-15).
  */
 #ifndef DIANA_HIP_H
 #define DIANA_HIP_H
@@ -48,7 +51,8 @@ enum { DT_F64 = 0, DT_F32 = 1, DT_BF16 = 2, DT_I32 = 3 };
 typedef struct dt_ctx dt_ctx;
 
 /* ---- context, memory, stream ------------------------------------------ */
-/* Creates a context on HIP device `device` with one stream.  Replaces the
+/* Creates a context on HIP device `device` with its stream (and the side
+ * stream dt_hooi_create uses internally).  Replaces the
  * per-tensor `new Operator<Ty>()` (template/tensor_mpi.tpp:26) and the
  * communicator set-up (src/communicator.cpp:3-15) of a single rank. */
 int dt_create(dt_ctx **ctx, int device);
