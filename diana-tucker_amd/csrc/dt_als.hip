@@ -194,14 +194,25 @@ __global__ __launch_bounds__(ALS_THREADS) void als_gram_kernel(const double *__r
 }
 
 
+// C += A B for 16 x 16 blocks on one wave's FP64 MFMA: a(k) gives A[c][k] and
+// b(k) gives B[k][c] for the lane's column c = lane % 16 and k = 4 s + lane / 16;
+// acc[v] ends as C[4 v + lane / 16][c].
+template <typename FA, typename FB>
+__device__ __forceinline__ doublex4 mm16(FA a, FB b, doublex4 acc) {
+    const int g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int s = 0; s < 4; s++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a(4 * s + g), b(4 * s + g), acc, 0, 0, 0);
+    return acc;
+}
+
 // Order 64 as a 4 x 4 grid of 16 x 16 blocks (G: LDS, row stride 65, upper
 // part used), right-looking: for block column k, one wave factors the
-// diagonal block (chol16_rinv on its symmetric copy: R_kk⁻¹), the workgroup forms the
-// block row R_kj = R_kk⁻ᵀ G_kj and updates the trailing blocks
-// G_ij -= R_kiᵀ R_kj (i <= j).  Then R⁻¹ by block diagonals d = 1..3:
-// X_ij = -R_ii⁻¹ sum_{k = i+1..j} R_ik X_kj, every X_kj of a smaller d being
-// known.  Writes the R x R corner of R⁻¹ (column-major) unless a pivot fails,
-// which ORs 4 into *flag.
+// diagonal block (chol16_rinv on its symmetric copy: R_kk⁻¹), the waves form
+// the block row R_kj = R_kk⁻ᵀ G_kj and update the trailing blocks
+// G_ij -= R_kiᵀ R_kj (i <= j), one 16 x 16 tile per wave on FP64 MFMA.  Then
+// R⁻¹ by block diagonals d = 1..3: X_ij = -R_ii⁻¹ sum_{k = i+1..j} R_ik X_kj,
+// every X_kj of a smaller d being known (again one tile per wave).  Writes the
+// R x R corner of R⁻¹ (column-major) unless a pivot fails, which ORs 4 into *flag.
 __device__ void chol64_blocked(double (*G)[65], double (*Rb)[65], int R, double *__restrict__ Rinv_out,
                                int *__restrict__ flag) {
     __shared__ double Id[4][16][17];  // inverses of the diagonal blocks of R
@@ -210,6 +221,9 @@ __device__ void chol64_blocked(double (*G)[65], double (*Rb)[65], int R, double 
     __shared__ double Gd[16][17];     // the diagonal block, both triangles
     __shared__ int bad_s;
     const int tid = threadIdx.x, nt = blockDim.x;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), nwv = nt >> 6;
+    const int lane = tid & 63, c = lane & 15, g = lane >> 4;
+    const doublex4 zero4 = {0.0, 0.0, 0.0, 0.0};
     if (tid == 0) bad_s = 0;
     for (int kb = 0; kb < 4; kb++) {
         const int k0 = 16 * kb;
@@ -223,21 +237,22 @@ __device__ void chol64_blocked(double (*G)[65], double (*Rb)[65], int R, double 
         // and is recorded for the early exit below
         chol16_rinv(Gd, Id[kb], 16, 0, 0, &bad_s);
         __syncthreads();
-        const int nc = 48 - k0;  // columns right of the diagonal block
-        for (int e = tid; e < 16 * nc; e += nt) {
-            const int i = e & 15, j = k0 + 16 + (e >> 4);
-            double s = 0.0;
-            for (int m = 0; m <= i; m++) s += Id[kb][m][i] * G[k0 + m][j];  // (R_kk⁻ᵀ)_im, upper R_kk⁻¹
-            Rb[k0 + i][j] = s;
+        const int ntl = 3 - kb;  // 16-column tiles right of the diagonal block
+        for (int jt = wv; jt < ntl; jt += nwv) {
+            const int j0 = k0 + 16 * (jt + 1);
+            const doublex4 acc = mm16([&](int k) { return k <= c ? Id[kb][k][c] : 0.0; },  // (R_kk⁻ᵀ)[c][k]
+                                      [&](int k) { return G[k0 + k][j0 + c]; }, zero4);
+#pragma unroll
+            for (int v = 0; v < 4; v++) Rb[k0 + 4 * v + g][j0 + c] = acc[v];
         }
         __syncthreads();
-        for (int e = tid; e < nc * nc; e += nt) {
-            const int i = k0 + 16 + e % nc, j = k0 + 16 + e / nc;
-            if (i <= j) {
-                double s = G[i][j];
-                for (int m = 0; m < 16; m++) s -= Rb[k0 + m][i] * Rb[k0 + m][j];
-                G[i][j] = s;
-            }
+        for (int t = wv; t < ntl * (ntl + 1) / 2; t += nwv) {
+            const int tj = t >= 3 ? 2 : (t >= 1 ? 1 : 0), ti = t - tj * (tj + 1) / 2;  // ti <= tj
+            const int i0 = k0 + 16 * (ti + 1), j0 = k0 + 16 * (tj + 1);
+            const doublex4 acc = mm16([&](int k) { return Rb[k0 + k][i0 + c]; },
+                                      [&](int k) { return Rb[k0 + k][j0 + c]; }, zero4);
+#pragma unroll
+            for (int v = 0; v < 4; v++) G[i0 + 4 * v + g][j0 + c] -= acc[v];
         }
     }
     __syncthreads();
@@ -247,23 +262,25 @@ __device__ void chol64_blocked(double (*G)[65], double (*Rb)[65], int R, double 
     }
     for (int e = tid; e < 64 * 64; e += nt) {
         const int i = e & 63, j = e >> 6;
-        Xi[i][j] = (i >> 4) == (j >> 4) ? Id[i >> 4][i & 15][j & 15] : 0.0;
+        Xi[i][j] = ((i >> 4) == (j >> 4) && (i & 15) <= (j & 15)) ? Id[i >> 4][i & 15][j & 15] : 0.0;
     }
     for (int d = 1; d < 4; d++) {
         __syncthreads();
-        for (int e = tid; e < (4 - d) * 256; e += nt) {
-            const int ib = e >> 8, i = e & 15, j = (e >> 4) & 15, jb = ib + d;
-            double s = 0.0;
+        for (int ib = wv; ib < 4 - d; ib += nwv) {
+            const int jb = ib + d;
+            doublex4 acc = zero4;
             for (int kb = ib + 1; kb <= jb; kb++)
-                for (int m = 0; m < 16; m++) s += Rb[16 * ib + i][16 * kb + m] * Xi[16 * kb + m][16 * jb + j];
-            Tt[ib][i][j] = s;
+                acc = mm16([&](int k) { return Rb[16 * ib + c][16 * kb + k]; },
+                           [&](int k) { return Xi[16 * kb + k][16 * jb + c]; }, acc);
+#pragma unroll
+            for (int v = 0; v < 4; v++) Tt[ib][4 * v + g][c] = acc[v];
         }
         __syncthreads();
-        for (int e = tid; e < (4 - d) * 256; e += nt) {
-            const int ib = e >> 8, i = e & 15, j = (e >> 4) & 15, jb = ib + d;
-            double s = 0.0;
-            for (int m = i; m < 16; m++) s += Id[ib][i][m] * Tt[ib][m][j];
-            Xi[16 * ib + i][16 * jb + j] = -s;
+        for (int ib = wv; ib < 4 - d; ib += nwv) {
+            const doublex4 acc = mm16([&](int k) { return k >= c ? Id[ib][c][k] : 0.0; },
+                                      [&](int k) { return Tt[ib][k][c]; }, zero4);
+#pragma unroll
+            for (int v = 0; v < 4; v++) Xi[16 * ib + 4 * v + g][16 * (ib + d) + c] = -acc[v];
         }
     }
     __syncthreads();
