@@ -288,9 +288,11 @@ __device__ void chol64_blocked(double (*G)[65], double (*Rb)[65], int R, double 
 }
 
 // One workgroup: G = sum_w part[w] (fixed order); optional shift; Rinv = chol(G)^{-1}.
-// CM (16, 32 or 64 >= R) is the padded order of the register Cholesky.
-template <int CM>
-__global__ __launch_bounds__(ALS_THREADS) void als_chol_kernel(const double *__restrict__ part,
+// CM (16, 32 or 64 >= R) is the padded order of the register Cholesky; NTH
+// threads (512 at order 64: eight partials' loads in flight per pass and eight
+// waves for the MFMA tiles of the blocked factorization).
+template <int CM, int NTH>
+__global__ __launch_bounds__(NTH) void als_chol_kernel(const double *__restrict__ part,
                                                                int nparts, int R, int I,
                                                                int shifted,
                                                                double *__restrict__ Rinv_out,
@@ -302,32 +304,33 @@ __global__ __launch_bounds__(ALS_THREADS) void als_chol_kernel(const double *__r
     {
         // G = sum of the partials in a fixed order; each pass over w issues
         // this thread's loads for all its entries together
-        constexpr int NE = (CM * CM + ALS_THREADS - 1) / ALS_THREADS;
+        constexpr int NE = (CM * CM + NTH - 1) / NTH;
+        constexpr int PW = NE <= 8 ? 8 : 4;  // partials per pass
         const size_t RR = (size_t)R * R;
         double acc[NE];
 #pragma unroll
         for (int q = 0; q < NE; q++) acc[q] = 0.0;
-        // four partials' loads in flight at a time, added in partial order
+        // PW partials' loads in flight at a time, added in partial order
         // (a partial past nparts reads the last one and adds zero)
-        for (int w0 = 0; w0 < nparts; w0 += 4) {
-            double v[4][NE];
+        for (int w0 = 0; w0 < nparts; w0 += PW) {
+            double v[PW][NE];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < PW; u++) {
                 const int w = w0 + u < nparts ? w0 + u : nparts - 1;
 #pragma unroll
                 for (int q = 0; q < NE; q++) {
-                    const int e = tid + ALS_THREADS * q;
+                    const int e = tid + NTH * q;
                     v[u][q] = part[(size_t)w * RR + (e < R * R ? e : 0)];
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < PW; u++)
 #pragma unroll
                 for (int q = 0; q < NE; q++) acc[q] += w0 + u < nparts ? v[u][q] : 0.0;
         }
 #pragma unroll
         for (int q = 0; q < NE; q++) {
-            const int e = tid + ALS_THREADS * q;
+            const int e = tid + NTH * q;
             if (e < R * R) G[e % R][e / R] = acc[q];
         }
     }
@@ -344,7 +347,7 @@ __global__ __launch_bounds__(ALS_THREADS) void als_chol_kernel(const double *__r
     __syncthreads();
     if (tid < R) G[tid][tid] += shift_s;
     // pad to CM with the identity: chol(diag(G, I)) = diag(chol(G), I)
-    for (int e = tid; e < CM * CM; e += ALS_THREADS) {
+    for (int e = tid; e < CM * CM; e += NTH) {
         const int i = e % CM, j = e / CM;
         if (i >= R || j >= R) G[i][j] = (i == j) ? 1.0 : 0.0;
     }
@@ -409,13 +412,13 @@ __global__ __launch_bounds__(ALS_THREADS) void als_chol_kernel(const double *__r
 static int launch_chol(dt_ctx *ctx, const double *part, int nparts, int R, int I, int shifted, double *ri,
                        int *flag) {
     if (R <= 16) {
-        DT_LAUNCH("als_chol", als_chol_kernel<16>, dim3(1), dim3(ALS_THREADS), 0, part, nparts, R, I, shifted, ri,
+        DT_LAUNCH("als_chol", (als_chol_kernel<16, ALS_THREADS>), dim3(1), dim3(ALS_THREADS), 0, part, nparts, R, I, shifted, ri,
                   flag);
     } else if (R <= 32) {
-        DT_LAUNCH("als_chol", als_chol_kernel<32>, dim3(1), dim3(ALS_THREADS), 0, part, nparts, R, I, shifted, ri,
+        DT_LAUNCH("als_chol", (als_chol_kernel<32, ALS_THREADS>), dim3(1), dim3(ALS_THREADS), 0, part, nparts, R, I, shifted, ri,
                   flag);
     } else {
-        DT_LAUNCH("als_chol", als_chol_kernel<64>, dim3(1), dim3(ALS_THREADS), 0, part, nparts, R, I, shifted, ri,
+        DT_LAUNCH("als_chol", (als_chol_kernel<64, 512>), dim3(1), dim3(512), 0, part, nparts, R, I, shifted, ri,
                   flag);
     }
     return DT_OK;
