@@ -99,6 +99,7 @@ enum dt_ws_slot {
     WS_SKETCH,         // learned start: every mode's sketch, Gram and eigenvectors
     WS_CHOLQR,         // dt_qr_f64's multi-workgroup Cholesky-QR: partial Grams, R^{-1}, flag
     WS_TTM_SWZ,        // mode-0 BF16 product: the factor in MFMA operand order
+    WS_SKPRE,          // learned start: the Kronecker sketch prefix modes 0 and 1 share
     WS_SIDE0,          // side work: block partials of the first and second sum, results
     WS_SIDE1,
     WS_SIDE_OUT,

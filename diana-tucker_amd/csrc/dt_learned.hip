@@ -165,8 +165,13 @@ __global__ void unfold_to_f64(const float *__restrict__ T, int ndim, int n, int6
 
 // Kronecker sketch of mode n: with Ydst the unfolded sketch (I x S FP64) is
 // written there and nothing else happens; without, the features go to X.
+// pre (optional, modes 0 and 1 of a tensor of 3 or more modes): A already
+// contracted over modes ndim - 1 .. 2 with their Ω in that order (FP32,
+// I_0 x I_1 x q ...), the prefix both modes' chains share; the chain then
+// runs only its last product, with the same kernels and operands, so the
+// sketch is the same.
 static int sketch_kron(dt_ctx *ctx, const void *A, int dtA, int ndim, const size_t *shape, int n, int q,
-                       uint64_t seed, float *X, double *Ydst) {
+                       uint64_t seed, float *X, double *Ydst, const float *pre = nullptr) {
     DT_CHECK_ARG(ctx && A && shape && (X || Ydst), "dt_sketch_features_kron: null argument");
     DT_CHECK_ARG(ndim >= 2 && ndim <= DT_MAX_NDIM && n >= 0 && n < ndim, "dt_sketch_features_kron: bad mode");
     DT_CHECK_ARG(dtA == DT_F32 || dtA == DT_BF16, "dt_sketch_features_kron: tensor must be F32 or BF16");
@@ -204,7 +209,14 @@ static int sketch_kron(dt_ctx *ctx, const void *A, int dtA, int ndim, const size
         Ms[k] = Om + off[idx[k]];
         Js[k] = (size_t)q;
     }
-    DT_TRY(dt_ttmc(ctx, A, dtA, ndim, shape, Ms, DT_F32, Js, idx, nm, Yt, DT_F32));
+    if (pre && n < 2 && ndim >= 3) {
+        size_t psh[DT_MAX_NDIM];
+        for (int m = 0; m < ndim; m++) psh[m] = m < 2 ? shape[m] : (size_t)q;
+        DT_TRY(dt_ttmc(ctx, pre, DT_F32, ndim, psh, Ms + (nm - 1), DT_F32, Js + (nm - 1), idx + (nm - 1), 1, Yt,
+                       DT_F32));
+    } else {
+        DT_TRY(dt_ttmc(ctx, A, dtA, ndim, shape, Ms, DT_F32, Js, idx, nm, Yt, DT_F32));
+    }
     DT_LAUNCH("sketch_unfold", unfold_to_f64, lg_grid(I * S), dim3(256), 0, Yt, ndim, n, I, q, I * S,
               Ydst ? Ydst : ws);
     if (Ydst) return DT_OK;
@@ -253,12 +265,36 @@ extern "C" int dt_learned_start(dt_ctx *ctx, dt_mlp *mlp, const void *A, int dtA
     double *V = G + (size_t)ndim * S * S;
     double *w = V + (size_t)ndim * S * S;
     double *sg = w + (size_t)ndim * S;
+    // Kronecker sketches of modes 0 and 1 share A x_{ndim-1} Ω ... x_2 Ω (a full
+    // read of A): computed once here
+    float *pre = nullptr;
+    if (sketch == DT_SKETCH_KRONECKER && ndim >= 3) {
+        size_t pe = shape[0] * shape[1], off_m[DT_MAX_NDIM], tot = 0;
+        for (int m = 2; m < ndim; m++) {
+            pe *= (size_t)q;
+            off_m[m] = tot;
+            tot += (size_t)q * shape[m];
+        }
+        float *Om = (float *)dt_ws(ctx, WS_MLP0, tot * sizeof(float) + 256);
+        pre = (float *)dt_ws(ctx, WS_SKPRE, pe * sizeof(float));
+        if (!Om || !pre) return DT_ENOMEM;
+        const void *Ms[DT_MAX_NDIM];
+        size_t Js[DT_MAX_NDIM];
+        int idx[DT_MAX_NDIM], nm = 0;
+        for (int m = ndim - 1; m >= 2; m--) {
+            DT_TRY(dt_gen_normal(ctx, Om + off_m[m], DT_F32, (size_t)q * shape[m], seed, 6000 + (uint64_t)m));
+            Ms[nm] = Om + off_m[m];
+            Js[nm] = (size_t)q;
+            idx[nm++] = m;
+        }
+        DT_TRY(dt_ttmc(ctx, A, dtA, ndim, shape, Ms, DT_F32, Js, idx, nm, pre, DT_F32));
+    }
     int64_t off = 0;
     for (int n = 0; n < ndim; n++) {
         const int64_t I = (int64_t)shape[n];
         sw[n] = SketchWs{sk + 2 * off * S, sk + 2 * off * S + I * S, sg + (size_t)n * S};
         if (sketch == DT_SKETCH_KRONECKER)
-            DT_TRY(sketch_kron(ctx, A, dtA, ndim, shape, n, q, seed, nullptr, sw[n].Y));
+            DT_TRY(sketch_kron(ctx, A, dtA, ndim, shape, n, q, seed, nullptr, sw[n].Y, pre));
         else
             DT_TRY(sketch_gauss(ctx, A, dtA, ndim, shape, n, S, seed, nullptr, sw[n].Y));
         DT_TRY(sketch_gram(ctx, sw[n], I, S, G + (size_t)n * S * S));
