@@ -146,6 +146,33 @@ def test_learned_start_with_untrained_model_is_the_sketch_range(ctx):
         m.close()
 
 
+@pytest.mark.parametrize("shape,q,r,dtype", [((48, 40, 32), 4, 5, dt.DT_BF16), ((16, 12, 10, 8), 3, 3, dt.DT_F32)])
+def test_learned_start_kron_shares_the_sketch_prefix_exactly(ctx, shape, q, r, dtype):
+    """The batched learned start contracts A over the modes above 1 once for the
+    Kronecker sketches of modes 0 and 1; with a fresh model its start is QR of
+    the first R feature columns.  The per-mode features (their own full chain,
+    then the same Gram, Jacobi and feature kernels) must give that subspace to
+    the rounding of the FP64 QR: a sketch formed differently would differ at
+    FP32 rounding, about 1e-7."""
+    print("kron prefix", shape, q, r, dtype, flush=True)
+    rng = np.random.default_rng(sum(shape))
+    A = rng.standard_normal(shape).astype(np.float32)
+    if dtype == dt.DT_BF16:
+        dA = ctx.array(shape, dt.DT_BF16).upload(dt.bf16_from_f32(A))
+    else:
+        dA = ctx.to_device(A)
+    S = q ** (len(shape) - 1)
+    m = dt.MLP(ctx, S, 64, r, seed=0)
+    try:
+        Us = [u.download() for u in ctx.learned_start(m, dA, (r,) * len(shape), seed=5,
+                                                      sketch=dt.SKETCH_KRONECKER)]
+        for n in range(len(shape)):
+            X = ctx.sketch_features_kron(dA, n, q, seed=5)
+            assert D.sin_max_angle(Us[n], np.linalg.qr(X[:, :r].astype(np.float64))[0]) < 1e-10, n
+    finally:
+        m.close()
+
+
 def test_learned_init_runs_hooi_and_reports_first_sweeps(ctx):
     """DT_INIT_LEARNED with DT_FLAG_LEARNED_CHECK: both first sweeps are
     reported and the better start is kept; the result matches the
