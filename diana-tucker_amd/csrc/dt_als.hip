@@ -939,7 +939,11 @@ __global__ void cholqr_rank_track(const double *__restrict__ Rinv, const double 
 // (X <- X R^{-1} in place) that also leaves the next pass's partial Grams.
 // Nothing waits for the host; a Cholesky breakdown ORs 4 into *flag, and with
 // rank_check so does a rank-deficient X (cholqr_rank_track).
-// ws: (ceil(I / 128) + 2) R^2 + R + 1 doubles.
+// ws: dt_cholqr3_ws_doubles(I, R) doubles.
+size_t dt_cholqr3_ws_doubles(size_t I, size_t R) {
+    return ((I + ALS_GR - 1) / ALS_GR + 2) * R * R + R + 1;
+}
+
 int dt_cholqr3_rows(dt_ctx *ctx, double *X, int I, int R, double *ws, int *flag, bool rank_check) {
     if (R < 1 || R > ALS_MAXR) return dt_fail(ctx, DT_EUNSUPPORTED, "Cholesky-QR: %d columns > %d", R, ALS_MAXR);
     const int nb = (I + ALS_GR - 1) / ALS_GR;

@@ -496,8 +496,7 @@ extern "C" int dt_qr_f64(dt_ctx *ctx, double *Q, double *R, const double *A, siz
     // host wait for its breakdown flag (the GEMM path below waits after every
     // Cholesky and runs a tiled GEMM with few tiles at these shapes)
     if (!R && n > 16 && n <= 64 && m >= 256 && m < ((size_t)1 << 31) && !(n <= (size_t)TS_NMAX && m >= 4096)) {
-        const size_t nb = (m + 127) / 128;
-        const size_t wsd = (nb + 2) * nn + n + 8;
+        const size_t wsd = dt_cholqr3_ws_doubles(m, n) + 8;
         double *ws = (double *)dt_ws(ctx, WS_CHOLQR, (wsd + 8) * sizeof(double));
         if (!ws) return DT_ENOMEM;
         int *dflag = (int *)(ws + wsd);

@@ -271,6 +271,8 @@ size_t dt_als_ws_doubles(size_t I, size_t R);
 // rank_check, a rank-deficient X) ORs 4 into *flag;
 // ws: (ceil(I / 128) + 2) R^2 + R + 1 doubles
 int dt_cholqr3_rows(dt_ctx *ctx, double *X, int I, int R, double *ws, int *flag, bool rank_check = false);
+// doubles of the workspace dt_cholqr3_rows takes for an I x R panel
+size_t dt_cholqr3_ws_doubles(size_t I, size_t R);
 int dt_als_update(dt_ctx *ctx, const double *S, double *U, int I, int R, int inner, double *ws,
                   int *flag, float *U32, float *M32, uint16_t *Ub16);
 int dt_als_fused(dt_ctx *ctx, const double *S, double *U, int I, int R, int inner, double *L,
