@@ -31,7 +31,7 @@
 constexpr int ALS_THREADS = 256;
 constexpr int ALS_CH = 128;     // k-chunk staged in LDS
 constexpr int ALS_MAXR = 64;
-constexpr int ALS_GR = 128;     // rows per workgroup of the partial Gram
+constexpr int ALS_GR = 64;      // rows per workgroup of the partial Gram
 
 // rows per workgroup of als_rows_kernel: one output element per thread
 template <int RMAX>
