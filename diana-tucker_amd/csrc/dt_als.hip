@@ -289,8 +289,8 @@ __device__ void chol64_blocked(double (*G)[65], double (*Rb)[65], int R, double 
 
 // One workgroup: G = sum_w part[w] (fixed order); optional shift; Rinv = chol(G)^{-1}.
 // CM (16, 32 or 64 >= R) is the padded order of the register Cholesky; NTH
-// threads (512 at order 64: eight partials' loads in flight per pass and eight
-// waves for the MFMA tiles of the blocked factorization).
+// threads (1024 at order 64: sixteen partials' loads in flight per pass and
+// sixteen waves for the MFMA tiles of the blocked factorization).
 template <int CM, int NTH>
 __global__ __launch_bounds__(NTH) void als_chol_kernel(const double *__restrict__ part,
                                                                int nparts, int R, int I,
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(NTH) void als_chol_kernel(const double *__restrict_
         // G = sum of the partials in a fixed order; each pass over w issues
         // this thread's loads for all its entries together
         constexpr int NE = (CM * CM + NTH - 1) / NTH;
-        constexpr int PW = NE <= 8 ? 8 : 4;  // partials per pass
+        constexpr int PW = NE <= 4 ? 16 : (NE <= 8 ? 8 : 4);  // partials per pass
         const size_t RR = (size_t)R * R;
         double acc[NE];
 #pragma unroll
@@ -418,7 +418,7 @@ static int launch_chol(dt_ctx *ctx, const double *part, int nparts, int R, int I
         DT_LAUNCH("als_chol", (als_chol_kernel<32, ALS_THREADS>), dim3(1), dim3(ALS_THREADS), 0, part, nparts, R, I, shifted, ri,
                   flag);
     } else {
-        DT_LAUNCH("als_chol", (als_chol_kernel<64, 512>), dim3(1), dim3(512), 0, part, nparts, R, I, shifted, ri,
+        DT_LAUNCH("als_chol", (als_chol_kernel<64, 1024>), dim3(1), dim3(1024), 0, part, nparts, R, I, shifted, ri,
                   flag);
     }
     return DT_OK;
