@@ -633,8 +633,10 @@ int dt_side_join(dt_ctx *ctx, double *out_host) {
     if (j.count == 0) return DT_OK;
     int st = dt_side_kick(ctx);
     if (ctx->side_stream && j.kicked) {
-        // waits even after a failed kick, so no side launch outlives the job
-        if (hipStreamWaitEvent(ctx->stream, ctx->side_done, 0) != hipSuccess) hipStreamSynchronize(ctx->side_stream);
+        // after a failed kick side_done may not mark this job's launches: wait
+        // for the side stream itself, so no side launch outlives the job
+        if (st != DT_OK || hipStreamWaitEvent(ctx->stream, ctx->side_done, 0) != hipSuccess)
+            hipStreamSynchronize(ctx->side_stream);
     }
     const int count = j.count;
     j.count = 0;
