@@ -915,22 +915,27 @@ static int rows(dt_ctx *ctx, int mode, const double *S, const double *Y, int I, 
 // the three passes' diagonals 1 / Rinv_k[j][j].  d[j] accumulates it; after the
 // last pass a column whose diagonal is at most 10 R u ||X||_F (||X||_F^2 = the
 // trace of the first Gram, in d[R]) means X is rank-deficient to working
-// precision: ORs 4 into *flag.  One thread.
+// precision: ORs 4 into *flag.  One wave, lane j for column j (R <= 64).
 __global__ void cholqr_rank_track(const double *__restrict__ Rinv, const double *__restrict__ G0, int R, int pass,
                                   double *__restrict__ d, int *__restrict__ flag) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int j = threadIdx.x;
+    if (blockIdx.x != 0 || j >= 64) return;  // one full wave: every lane below takes part
+    double tr;
     if (pass == 0) {
-        double tr = 0.0;
-        for (int j = 0; j < R; j++) tr += G0[j + (size_t)R * j];
-        d[R] = tr;
+        tr = j < R ? G0[j + (size_t)R * j] : 0.0;
+        for (int o = 32; o > 0; o >>= 1) tr += __shfl_xor(tr, o);
+        if (j == 0) d[R] = tr;
+    } else {
+        tr = d[R];
     }
     bool bad = false;
-    for (int j = 0; j < R; j++) {
+    if (j < R) {
         const double rj = 1.0 / Rinv[j + (size_t)R * j];
-        d[j] = pass == 0 ? rj : d[j] * rj;
-        if (pass == 2 && !(d[j] > 10.0 * R * 1.1102230246251565e-16 * sqrt(d[R]))) bad = true;
+        const double dj = pass == 0 ? rj : d[j] * rj;
+        d[j] = dj;
+        bad = pass == 2 && !(dj > 10.0 * R * 1.1102230246251565e-16 * sqrt(tr));
     }
-    if (bad) atomicOr(flag, 4);
+    if (__ballot(bad) != 0 && j == 0) atomicOr(flag, 4);
 }
 
 // Q <- the orthonormal factor of X (I x R FP64, R <= 64) with positive R
