@@ -359,6 +359,9 @@ __device__ void wg_xtx_mfma(const T *__restrict__ X, int m, int n, double *__res
 }
 
 // out = x y (n x n, LDS, column-major)
+// out = x yᵀ (n x n, LDS, column-major): the B operand is read along a row of
+// y (consecutive lanes, consecutive addresses), not down a column, whose
+// 512-byte stride at n = 64 puts a column group's 16 lanes on the same banks.
 __device__ void wg_small_mm_mfma(const double *__restrict__ x, const double *__restrict__ y, int n,
                                  double *__restrict__ out) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -370,7 +373,7 @@ __device__ void wg_small_mm_mfma(const double *__restrict__ x, const double *__r
         for (int k0 = 0; k0 < n; k0 += 4) {
             const int k = k0 + g;
             const double a = (i < n && k < n) ? x[i + n * k] : 0.0;
-            const double b = (k < n && j < n) ? y[k + n * j] : 0.0;
+            const double b = (k < n && j < n) ? y[j + n * k] : 0.0;  // (yᵀ)[k][j]
             acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
         }
 #pragma unroll
@@ -440,9 +443,9 @@ __global__ __launch_bounds__(SMALL_THREADS) void gram_inverse_batch_kernel(dt_gr
         for (int e = tid; e < n * n; e += nt) C[e] = ((e % n) == (e / n) ? 2.0 : 0.0) - G[e];
         __syncthreads();
         for (int it = 0; it < 5; it++) {
-            wg_small_mm_mfma(G, C, n, T);  // T = G C
+            wg_small_mm_mfma(C, G, n, T);  // T = C Gᵀ = C G (G symmetric)
             __syncthreads();
-            wg_small_mm_mfma(C, T, n, T2);  // T2 = C G C
+            wg_small_mm_mfma(C, T, n, T2);  // T2 = C Tᵀ = C G C
             __syncthreads();
             for (int e = tid; e < n * n; e += nt) C[e] = 2.0 * C[e] - T2[e];
             __syncthreads();
